@@ -1,0 +1,197 @@
+/*
+ * board.h -- nibble bitboard primitives shared by the host C ABI and the
+ * gfx950 kernels (compiled as __host__ __device__ under hipcc and as plain C
+ * by the host compiler).
+ *
+ * Encoding (api/cablegen.h:9-12): tile x in [0,16) is the nibble at bits
+ * [60-4x, 63-4x]; row r is the 16-bit group at bits [48-16r, 63-16r]; within
+ * a row value column 0 is the most significant nibble.  0 = empty, 1..13 =
+ * 2^v, 0xE = unmergeable non-formation tile, 0xF = formation cell.
+ *
+ * The implementations are bit-parallel (no per-tile loops on the hot path):
+ * rotations and mirrors are mask/shift networks, the lock test is a nibble
+ * "all ones" mask, and canonicalisation is the unsigned max of the 8 images.
+ */
+#ifndef CABLEGEN_BOARD_H
+#define CABLEGEN_BOARD_H
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#define CG_HD __host__ __device__ __forceinline__
+#else
+#define CG_HD static inline
+#endif
+
+#define CG_NIB_LO 0x1111111111111111ULL
+
+/* mask with 0x1 in every nibble equal to 0xF */
+CG_HD uint64_t cg_fmask(uint64_t b)
+{
+    return b & (b >> 1) & (b >> 2) & (b >> 3) & CG_NIB_LO;
+}
+
+/* mask with 0x1 in every nibble equal to 0 */
+CG_HD uint64_t cg_zmask(uint64_t b)
+{
+    uint64_t t = b | (b >> 1) | (b >> 2) | (b >> 3);
+    return ~t & CG_NIB_LO;
+}
+
+CG_HD int cg_popc64(uint64_t x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __popcll(x);
+#else
+    return __builtin_popcountll(x);
+#endif
+}
+
+CG_HD int cg_empties(uint64_t b) { return cg_popc64(cg_zmask(b)); }
+
+/* rotate_clockwise (board.c:239-248): out(r,c) = in(3-c, r) */
+CG_HD uint64_t cg_rot_cw(uint64_t b)
+{
+    b = ((b & 0xff00ff0000000000ULL) >> 8) | ((b & 0x00ff00ff00000000ULL) >> 32) |
+        ((b & 0x00000000ff00ff00ULL) << 32) | ((b & 0x0000000000ff00ffULL) << 8);
+    b = ((b & 0xf0f00000f0f00000ULL) >> 4) | ((b & 0x0f0f00000f0f0000ULL) >> 16) |
+        ((b & 0x0000f0f00000f0f0ULL) << 16) | ((b & 0x00000f0f00000f0fULL) << 4);
+    return b;
+}
+
+/* row-order reversal (flip, board.c:376-381) */
+CG_HD uint64_t cg_flip_rows(uint64_t b)
+{
+    b = (b >> 32) | (b << 32);
+    return ((b & 0xffff0000ffff0000ULL) >> 16) | ((b & 0x0000ffff0000ffffULL) << 16);
+}
+
+/* nibble reversal inside every 16-bit row (mirror left/right) */
+CG_HD uint64_t cg_mirror_rows(uint64_t b)
+{
+    b = ((b & 0xff00ff00ff00ff00ULL) >> 8) | ((b & 0x00ff00ff00ff00ffULL) << 8);
+    return ((b & 0xf0f0f0f0f0f0f0f0ULL) >> 4) | ((b & 0x0f0f0f0f0f0f0f0fULL) << 4);
+}
+
+CG_HD uint64_t cg_umax(uint64_t a, uint64_t b) { return a > b ? a : b; }
+
+/* Canonical form: unsigned max over the 8 D4 images (board.c:314-337).
+ * With y = cw(b): images are b, y, rot180 b, rot180 y and their row flips;
+ * rot180 = flip o mirror. */
+CG_HD uint64_t cg_canon(uint64_t b)
+{
+    uint64_t y = cg_rot_cw(b);
+    uint64_t mb = cg_mirror_rows(b), my = cg_mirror_rows(y);
+    uint64_t m = cg_umax(b, y);
+    m = cg_umax(m, cg_umax(mb, my));               /* f a^2 b, f a^3 b */
+    m = cg_umax(m, cg_umax(cg_flip_rows(b), cg_flip_rows(y)));
+    m = cg_umax(m, cg_umax(cg_flip_rows(mb), cg_flip_rows(my)));  /* a^2, a^3 */
+    return m;
+}
+
+/* nibble reversal of a 16-bit row (right move = rev o left o rev) */
+CG_HD unsigned cg_row_rev(unsigned r)
+{
+    return ((r & 0xfu) << 12) | ((r & 0xf0u) << 4) | ((r >> 4) & 0xf0u) | (r >> 12);
+}
+
+/* Lock rule (board.c:29-38, 117): a row move is allowed unless a 0xF cell was
+ * vacated, or always with free_formation. */
+CG_HD int cg_row_locked(unsigned pre, unsigned post, int free_formation)
+{
+    uint64_t fp = cg_fmask(pre), fq = cg_fmask(post);
+    return free_formation || (fp & ~fq) == 0;
+}
+
+/* One horizontal board move using a left-move row table `L` (65536 u16).
+ * dir 0 = left, 1 = right.  Returns 1 iff valid (some row changed and every
+ * row is locked, movedir_hori board.c:151-169). */
+CG_HD int cg_move_hori_tab(const uint16_t *L, uint64_t *b, int dir, int ff)
+{
+    uint64_t pre = *b, out = 0;
+    int changed = 0, ok = 1;
+#if defined(__HIP_DEVICE_COMPILE__)
+#pragma unroll
+#endif
+    for (int i = 0; i < 4; i++) {
+        unsigned row = (unsigned)(pre >> (16 * i)) & 0xffffu;
+        unsigned nr = dir ? cg_row_rev(L[cg_row_rev(row)]) : L[row];
+        changed |= nr != row;
+        ok &= cg_row_locked(row, nr, ff);
+        out |= (uint64_t)nr << (16 * i);
+    }
+    if (ok && changed) { *b = out; return 1; }
+    return 0;
+}
+
+/* movedir_unstable (board.c:195-214): up = cw then right, down = cw then
+ * left; the moved board stays rotated.  On failure *b is unspecified. */
+CG_HD int cg_move_tab(const uint16_t *L, uint64_t *b, int dir, int ff)
+{
+    if (dir >= 2) *b = cg_rot_cw(*b);
+    return cg_move_hori_tab(L, b, (dir == 2) ? 1 : (dir == 3 ? 0 : dir), ff);
+}
+
+/* prune_board (generate.c:98-131) as a multiset test: a value v with
+ * smallest_large <= v < 0xE is large; v == smallest_large may occur 5 times,
+ * other large values twice; the small-tile sum counts every v <
+ * smallest_large (an empty cell adds 1); prune if it exceeds stsl + 64 or if
+ * there are more than ltc large tiles. */
+CG_HD int cg_prune(uint64_t b, long stsl, long ltc, long sl)
+{
+    int large = 0, sts = 0, bad = 0;
+    unsigned cnt_lo = 0, cnt_hi = 0;   /* 4-bit counters: values 0-7 / 8-15 */
+#if defined(__HIP_DEVICE_COMPILE__)
+#pragma unroll
+#endif
+    for (int x = 0; x < 16; x++) {
+        int v = (int)(b >> (60 - 4 * x)) & 0xf;
+        if (v >= sl && v < 0xe) {
+            large++;
+            if (v < 8) cnt_lo += 1u << (4 * v);
+            else cnt_hi += 1u << (4 * (v - 8));
+        } else if (v < 0xe) {
+            sts += 1 << v;
+        }
+    }
+#if defined(__HIP_DEVICE_COMPILE__)
+#pragma unroll
+#endif
+    for (int v = 0; v < 14; v++) {
+        unsigned c = v < 8 ? (cnt_lo >> (4 * v)) & 0xf : (cnt_hi >> (4 * (v - 8))) & 0xf;
+        bad |= (v == sl) ? (c > 5) : (c > 2);
+    }
+    return bad || sts > stsl + 64 || large > ltc;
+}
+
+/* cmpbrd (solve.c:226-231) with the mask precomputed: (b & mask) == w */
+CG_HD uint64_t cg_win_mask(uint64_t w)
+{
+    uint64_t m = (w >> 2) | w;
+    m = ((m >> 1) | m) & CG_NIB_LO;
+    return m * 0xf;
+}
+
+/* board holds the nibble v (nonzero v) */
+CG_HD int cg_has_nibble(uint64_t b, unsigned v)
+{
+    return cg_fmask(~(b ^ (CG_NIB_LO * (uint64_t)(v & 0xf)))) != 0;
+}
+
+/* tile sum (board.c:340-347): sum of 2^v over v not in {0, 0xE, 0xF} */
+CG_HD int cg_tile_sum(uint64_t b)
+{
+    int s = 0;
+    for (int x = 0; x < 16; x++) {
+        int v = (int)(b >> (60 - 4 * x)) & 0xf;
+        if (v != 0 && v < 0xe) s += 1 << v;
+    }
+    return s;
+}
+
+CG_HD uint64_t cg_set_tile(uint64_t b, int x, unsigned v)
+{
+    int sh = 60 - 4 * x;
+    return (b & ~(0xfULL << sh)) | ((uint64_t)v << sh);
+}
+
+#endif

@@ -1,0 +1,898 @@
+/*
+ * engine.hip -- host side of the gfx950 engine: device state, workspace,
+ * the per-layer generation step, the layer-by-layer solve and adjoint, the
+ * device-resident tablebase, the streaming interface used by the drop-in
+ * generate()/solve(), device primitives and kernel timers.
+ *
+ * One HIP stream (default priority) carries all work.  Layer arrays live in
+ * a bump arena of large hipMalloc chunks (a C2-sized run keeps ~1e9 boards,
+ * ~24 GB with values and argmax words, resident in HBM); per-step
+ * temporaries live in workspace buffers that only grow.
+ */
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+#include "board.h"
+#include "engine.h"
+#include "kernels.h"
+#include "../../include/cablegen_gpu.h"
+
+using namespace cg;
+
+/* ------------------------------------------------------------------------ */
+/* errors                                                                    */
+/* ------------------------------------------------------------------------ */
+
+static std::string g_err;
+
+static int fail(const char *what, hipError_t e)
+{
+    char buf[512];
+    snprintf(buf, sizeof buf, "%s: %s", what, hipGetErrorString(e));
+    g_err = buf;
+    return -1;
+}
+
+static int fail_msg(const char *what)
+{
+    g_err = what;
+    return -1;
+}
+
+#define HC(x)                                      \
+    do {                                           \
+        hipError_t e_ = (x);                       \
+        if (e_ != hipSuccess) return fail(#x, e_); \
+    } while (0)
+
+/* collects launch and runtime errors after a synchronisation point */
+static int check_sync(hipStream_t st, const char *where)
+{
+    hipError_t e = hipStreamSynchronize(st);
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e == hipSuccess) e = kernels_take_error();
+    if (e != hipSuccess) return fail(where, e);
+    return 0;
+}
+
+extern "C" const char *cge_last_error(void) { return g_err.c_str(); }
+extern "C" const char *cg_last_error(void) { return g_err.c_str(); }
+
+/* ------------------------------------------------------------------------ */
+/* device state                                                              */
+/* ------------------------------------------------------------------------ */
+
+struct Buf {
+    uint64_t *p = nullptr;
+    size_t cap = 0;
+};
+
+struct Engine {
+    bool init = false;
+    hipStream_t st = nullptr;
+    uint16_t *lut[2] = { nullptr, nullptr };     /* left-move table per ignore_f */
+    unsigned long long *d_cnt = nullptr;          /* device counters [8] */
+    unsigned long long *h_cnt = nullptr;          /* pinned mirror */
+    Buf A, B, Cb;                                 /* candidate / sort buffers */
+    void *scratch = nullptr;
+    size_t scratch_cap = 0;
+    uint64_t *d_wmask = nullptr, *d_wval = nullptr;
+    size_t wcap = 0;
+};
+static Engine E;
+
+static int eng_init()
+{
+    if (E.init) return 0;
+    HC(hipStreamCreate(&E.st));
+    HC(hipMalloc((void **)&E.d_cnt, 8 * sizeof(unsigned long long)));
+    HC(hipHostMalloc((void **)&E.h_cnt, 8 * sizeof(unsigned long long), hipHostMallocDefault));
+    E.init = true;
+    return 0;
+}
+
+static int ensure(Buf &b, size_t n)
+{
+    if (n <= b.cap && b.p) return 0;
+    if (b.p) HC(hipFree(b.p));
+    b.p = nullptr;
+    size_t c = std::max<size_t>(n + n / 4, 1024);
+    HC(hipMalloc((void **)&b.p, c * sizeof(uint64_t)));
+    b.cap = c;
+    return 0;
+}
+
+static int ensure_scratch(size_t n)
+{
+    size_t need = sort_scratch_bytes(n);
+    if (need <= E.scratch_cap && E.scratch) return 0;
+    if (E.scratch) HC(hipFree(E.scratch));
+    E.scratch = nullptr;
+    size_t c = need + need / 4 + 4096;
+    HC(hipMalloc(&E.scratch, c));
+    E.scratch_cap = c;
+    return 0;
+}
+
+static const uint16_t *get_lut(int ignore_f)
+{
+    int i = ignore_f ? 1 : 0;
+    if (!E.lut[i]) {
+        std::vector<uint16_t> h(65536);
+        cge_build_row_table(i, h.data());
+        if (hipMalloc((void **)&E.lut[i], 65536 * sizeof(uint16_t)) != hipSuccess) return nullptr;
+        if (hipMemcpy(E.lut[i], h.data(), 65536 * sizeof(uint16_t), hipMemcpyHostToDevice) !=
+            hipSuccess)
+            return nullptr;
+    }
+    return E.lut[i];
+}
+
+/* ------------------------------------------------------------------------ */
+/* kernel timers                                                             */
+/* ------------------------------------------------------------------------ */
+
+enum { PK_MOVE, PK_SPAWN, PK_SORT_HIST, PK_SORT_SCATTER, PK_UNIQUE, PK_SOLVE, PK_ADJOINT, PK_LOOKUP,
+       PK_N };
+static const char *PK_NAMES[PK_N] = { "move", "spawn", "sort_hist", "sort_scatter",
+                                      "unique", "solve", "adjoint", "lookup" };
+
+struct ProfRec {
+    int kind;
+    hipEvent_t a, b;
+    double bytes;
+};
+static bool g_prof = false;
+static std::vector<ProfRec> g_recs;
+static double g_ms[PK_N], g_bytes[PK_N];
+static long g_launches[PK_N];
+
+static int prof_begin(int kind)
+{
+    if (!g_prof) return -1;
+    ProfRec r;
+    r.kind = kind;
+    r.bytes = 0;
+    if (hipEventCreate(&r.a) != hipSuccess || hipEventCreate(&r.b) != hipSuccess) return -1;
+    hipEventRecord(r.a, E.st);
+    g_recs.push_back(r);
+    return (int)g_recs.size() - 1;
+}
+
+static void prof_end(int h, double bytes)
+{
+    if (h < 0) return;
+    hipEventRecord(g_recs[h].b, E.st);
+    g_recs[h].bytes = bytes;
+}
+
+static void prof_set_bytes(int h, double bytes)
+{
+    if (h >= 0) g_recs[h].bytes = bytes;
+}
+
+static void prof_collect()
+{
+    if (g_recs.empty()) return;
+    hipStreamSynchronize(E.st);
+    for (auto &r : g_recs) {
+        float ms = 0;
+        hipEventElapsedTime(&ms, r.a, r.b);
+        g_ms[r.kind] += ms;
+        g_bytes[r.kind] += r.bytes;
+        g_launches[r.kind]++;
+        hipEventDestroy(r.a);
+        hipEventDestroy(r.b);
+    }
+    g_recs.clear();
+}
+
+/* sort hooks: the radix passes run inside kernels.hip; these brackets let
+ * the timers see each pass */
+namespace cg {
+int prof_hook_begin(int kind) { return prof_begin(kind); }
+void prof_hook_end(int h, double bytes) { prof_end(h, bytes); }
+}
+
+extern "C" void cg_prof_enable(int on)
+{
+    eng_init();
+    g_prof = on != 0;
+}
+
+extern "C" void cg_prof_reset(void)
+{
+    prof_collect();
+    for (int i = 0; i < PK_N; i++) g_ms[i] = g_bytes[i] = 0, g_launches[i] = 0;
+}
+
+extern "C" long cg_prof_query(const char *kernel, double *total_ms, double *total_bytes)
+{
+    prof_collect();
+    for (int i = 0; i < PK_N; i++)
+        if (!strcmp(kernel, PK_NAMES[i])) {
+            if (total_ms) *total_ms = g_ms[i];
+            if (total_bytes) *total_bytes = g_bytes[i];
+            return g_launches[i];
+        }
+    return -1;
+}
+
+/* ------------------------------------------------------------------------ */
+/* layer arena                                                               */
+/* ------------------------------------------------------------------------ */
+
+struct Arena {
+    std::vector<void *> chunks;
+    char *cur = nullptr;
+    size_t left = 0;
+    static constexpr size_t CHUNK = (size_t)2 << 30;
+
+    void *alloc(size_t bytes)
+    {
+        bytes = (bytes + 255) & ~(size_t)255;
+        if (bytes == 0) bytes = 256;
+        if (bytes > left) {
+            size_t c = std::max(CHUNK, bytes);
+            void *p = nullptr;
+            if (hipMalloc(&p, c) != hipSuccess) return nullptr;
+            chunks.push_back(p);
+            cur = (char *)p;
+            left = c;
+        }
+        void *r = cur;
+        cur += bytes;
+        left -= bytes;
+        return r;
+    }
+    void release()
+    {
+        for (void *p : chunks) hipFree(p);
+        chunks.clear();
+        cur = nullptr;
+        left = 0;
+    }
+};
+
+/* ------------------------------------------------------------------------ */
+/* the generation step                                                       */
+/* ------------------------------------------------------------------------ */
+
+static MoveRules move_rules(const cg_rules &r)
+{
+    MoveRules m;
+    m.ff = r.free_formation;
+    m.prune = r.prune;
+    m.stsl = r.stsl;
+    m.ltc = r.ltc;
+    m.sl = r.smallest_large;
+    return m;
+}
+
+static int read_counter(size_t *out)
+{
+    HC(hipMemcpyAsync(E.h_cnt, E.d_cnt, sizeof(unsigned long long), hipMemcpyDeviceToHost, E.st));
+    if (check_sync(E.st, "counter")) return -1;
+    *out = (size_t)E.h_cnt[0];
+    return 0;
+}
+
+/* sorts and dedupes buf[0..n) (clobbering it and E.B); returns the count and
+ * leaves a SortState whose finish the caller enqueues */
+static int sort_begin(uint64_t *buf, size_t n, SortState *ss)
+{
+    if (ensure(E.B, n)) return -1;
+    if (ensure_scratch(n)) return -1;
+    *ss = sort_unique_begin(E.st, buf, E.B.p, n, E.scratch);
+    hipError_t e = kernels_take_error();
+    if (e != hipSuccess) return fail("sort_unique", e);
+    return 0;
+}
+
+typedef uint64_t *(*AllocFn)(size_t n, void *ctx);
+
+/* One layer step (generate.c:409-550): P (pre-move layer) and the carried
+ * spawn-4 set C give M_s (written to alloc_m), P_{s+2} and the new carry
+ * (written to nxtP / nxtC). */
+static int gen_step(const cg_rules &r, const uint64_t *P, size_t nP, const uint64_t *C, size_t nC,
+                    AllocFn alloc_m, void *mctx, uint64_t **M, size_t *nM, Buf &nxtP, size_t *nP2,
+                    Buf &nxtC, size_t *nC2)
+{
+    const uint16_t *lut = get_lut(r.ignore_f);
+    if (!lut) return fail_msg("row table upload failed");
+    /* move */
+    if (ensure(E.A, 4 * nP + 1)) return -1;
+    HC(hipMemsetAsync(E.d_cnt, 0, sizeof(unsigned long long), E.st));
+    int h = prof_begin(PK_MOVE);
+    launch_move(E.st, P, nP, E.A.p, E.d_cnt, lut, move_rules(r));
+    prof_end(h, 0);
+    size_t ncand = 0;
+    if (read_counter(&ncand)) return -1;
+    prof_set_bytes(h, 8.0 * nP + 8.0 * ncand);
+    SortState ss;
+    if (sort_begin(E.A.p, ncand, &ss)) return -1;
+    *nM = ss.n_unique;
+    *M = alloc_m(ss.n_unique, mctx);
+    if (!*M) return fail_msg("layer allocation failed");
+    int hu = prof_begin(PK_UNIQUE);
+    sort_unique_finish(E.st, ss, *M);
+    prof_end(hu, 8.0 * ncand + 8.0 * ss.n_unique);
+    /* spawn */
+    HC(hipMemsetAsync(E.d_cnt, 0, sizeof(unsigned long long), E.st));
+    launch_count_empties(E.st, *M, *nM, E.d_cnt);
+    size_t sumE = 0;
+    if (read_counter(&sumE)) return -1;
+    if (ensure(E.A, sumE + nC + 1) || ensure(E.Cb, sumE + 1)) return -1;
+    HC(hipMemsetAsync(E.d_cnt, 0, sizeof(unsigned long long), E.st));
+    int hs = prof_begin(PK_SPAWN);
+    launch_spawn(E.st, *M, *nM, E.A.p, E.Cb.p, E.d_cnt);
+    prof_end(hs, 8.0 * *nM + 16.0 * sumE);
+    if (nC) HC(hipMemcpyAsync(E.A.p + sumE, C, nC * sizeof(uint64_t), hipMemcpyDeviceToDevice, E.st));
+    /* P_{s+2} = uniq(S2 u C) */
+    if (sort_begin(E.A.p, sumE + nC, &ss)) return -1;
+    if (ensure(nxtP, ss.n_unique + 1)) return -1;
+    hu = prof_begin(PK_UNIQUE);
+    sort_unique_finish(E.st, ss, nxtP.p);
+    prof_end(hu, 8.0 * (sumE + nC) + 8.0 * ss.n_unique);
+    *nP2 = ss.n_unique;
+    /* C' = uniq(S4) */
+    if (sort_begin(E.Cb.p, sumE, &ss)) return -1;
+    if (ensure(nxtC, ss.n_unique + 1)) return -1;
+    hu = prof_begin(PK_UNIQUE);
+    sort_unique_finish(E.st, ss, nxtC.p);
+    prof_end(hu, 8.0 * sumE + 8.0 * ss.n_unique);
+    *nC2 = ss.n_unique;
+    return check_sync(E.st, "gen_step");
+}
+
+/* ------------------------------------------------------------------------ */
+/* winstates                                                                 */
+/* ------------------------------------------------------------------------ */
+
+static int upload_win(const uint64_t *images, size_t n, const cg_rules &r, WinRules *w)
+{
+    if (n > E.wcap || !E.d_wmask) {
+        if (E.d_wmask) hipFree(E.d_wmask);
+        if (E.d_wval) hipFree(E.d_wval);
+        size_t c = std::max<size_t>(n, 64);
+        HC(hipMalloc((void **)&E.d_wmask, c * 8));
+        HC(hipMalloc((void **)&E.d_wval, c * 8));
+        E.wcap = c;
+    }
+    std::vector<uint64_t> m(n);
+    for (size_t i = 0; i < n; i++) m[i] = cg_win_mask(images[i]);
+    if (n) {
+        HC(hipMemcpy(E.d_wmask, m.data(), n * 8, hipMemcpyHostToDevice));
+        HC(hipMemcpy(E.d_wval, images, n * 8, hipMemcpyHostToDevice));
+    }
+    w->mask = E.d_wmask;
+    w->val = E.d_wval;
+    w->n = (int)n;
+    w->nox = r.nox;
+    w->score = r.score;
+    return 0;
+}
+
+/* the 8 images of a winstate, get_all_rots (board.c:349-374) */
+static void win_images(const uint64_t *ws, size_t n, std::vector<uint64_t> &out)
+{
+    out.clear();
+    for (size_t i = 0; i < n; i++) {
+        uint64_t b = ws[i], x = b;
+        out.push_back(b);
+        for (int k = 1; k < 4; k++) out.push_back(x = cg_rot_cw(x));
+        x = cg_flip_rows(b);
+        out.push_back(x);
+        for (int k = 1; k < 4; k++) out.push_back(x = cg_rot_cw(x));
+    }
+}
+
+static int split_count(size_t n) { return n < 1024 ? (int)n : 1024; }
+
+/* ------------------------------------------------------------------------ */
+/* device-resident tablebase                                                 */
+/* ------------------------------------------------------------------------ */
+
+struct TbLayer {
+    uint64_t *keys = nullptr;
+    size_t n = 0;
+    double *vals = nullptr;
+    uint64_t *arg = nullptr;
+    uint64_t *split = nullptr;
+    int ns = 0;
+};
+
+struct cg_tablebase {
+    cg_rules r;
+    int start = 0;
+    std::vector<TbLayer> L;
+    Arena arena;
+    double p4 = 0.1;
+    bool have_arg = false;
+    bool solved = false;
+    int solve_lo = 0;          /* lowest solved layer index */
+    std::vector<uint64_t> W;   /* expanded winstates */
+    Buf P, C, P2, C2;
+};
+
+static LayerView view_of(const cg_tablebase *tb, int idx)
+{
+    LayerView v{ nullptr, nullptr, 0, nullptr, 0 };
+    if (idx < 0 || idx >= (int)tb->L.size()) return v;
+    const TbLayer &l = tb->L[idx];
+    if (!l.vals) return v;
+    v.keys = l.keys;
+    v.vals = l.vals;
+    v.n = l.n;
+    v.split = l.split;
+    v.ns = l.ns;
+    return v;
+}
+
+static uint64_t *arena_alloc_u64(size_t n, void *ctx)
+{
+    return (uint64_t *)((Arena *)ctx)->alloc(std::max<size_t>(n, 1) * sizeof(uint64_t));
+}
+
+extern "C" cg_tablebase *cg_tb_create(const cg_rules *rules)
+{
+    if (eng_init()) return nullptr;
+    cg_tablebase *tb = new cg_tablebase();
+    tb->r = *rules;
+    return tb;
+}
+
+static void buf_free(Buf &b)
+{
+    if (b.p) hipFree(b.p);
+    b.p = nullptr;
+    b.cap = 0;
+}
+
+extern "C" void cg_tb_free(cg_tablebase *tb)
+{
+    if (!tb) return;
+    hipStreamSynchronize(E.st);
+    tb->arena.release();
+    buf_free(tb->P);
+    buf_free(tb->C);
+    buf_free(tb->P2);
+    buf_free(tb->C2);
+    delete tb;
+}
+
+extern "C" int cg_tb_generate(cg_tablebase *tb, const uint64_t *initial, size_t n, int start, int end)
+{
+    if (eng_init()) return -1;
+    tb->arena.release();
+    tb->L.clear();
+    tb->start = start;
+    tb->solved = false;
+    tb->have_arg = false;
+    if (ensure(tb->P, n + 1)) return -1;
+    if (n) HC(hipMemcpy(tb->P.p, initial, n * 8, hipMemcpyHostToDevice));
+    size_t nP = n, nC = 0;
+    for (int s = start; s <= end; s += 2) {
+        TbLayer l;
+        size_t nP2 = 0, nC2 = 0;
+        if (gen_step(tb->r, tb->P.p, nP, tb->C.p, nC, arena_alloc_u64, &tb->arena, &l.keys, &l.n,
+                     tb->P2, &nP2, tb->C2, &nC2))
+            return -1;
+        l.ns = split_count(l.n);
+        if (l.ns) {
+            l.split = (uint64_t *)tb->arena.alloc(l.ns * 8);
+            launch_build_split(E.st, l.keys, l.n, l.split, l.ns);
+        }
+        tb->L.push_back(l);
+        std::swap(tb->P, tb->P2);
+        std::swap(tb->C, tb->C2);
+        nP = nP2;
+        nC = nC2;
+    }
+    return check_sync(E.st, "cg_tb_generate");
+}
+
+static int solve_layer_dev(const cg_rules &r, const uint64_t *M, size_t n, double *V, uint64_t *arg,
+                           LayerView t2, LayerView t4, const WinRules &w, double p4)
+{
+    const uint16_t *lut = get_lut(r.ignore_f);
+    if (!lut) return fail_msg("row table upload failed");
+    unsigned long long *stats = nullptr;
+    if (g_prof) {
+        HC(hipMemsetAsync(E.d_cnt + 2, 0, 2 * sizeof(unsigned long long), E.st));
+        stats = E.d_cnt + 2;
+    }
+    int h = prof_begin(PK_SOLVE);
+    launch_solve(E.st, M, n, V, arg, t2, t4, w, p4, lut, r.free_formation, stats);
+    prof_end(h, 0);
+    if (g_prof) {
+        HC(hipMemcpyAsync(E.h_cnt + 2, E.d_cnt + 2, 16, hipMemcpyDeviceToHost, E.st));
+        if (check_sync(E.st, "solve stats")) return -1;
+        double L = (double)E.h_cnt[2], H = (double)E.h_cnt[3];
+        prof_set_bytes(h, 16.0 * n + 24.0 * L + 8.0 * H + (arg ? 8.0 * n : 0.0));
+    }
+    return 0;
+}
+
+extern "C" int cg_tb_solve(cg_tablebase *tb, const uint64_t *winstates, size_t n_win, int end_solve,
+                           double p4, int keep_argmax)
+{
+    if (eng_init()) return -1;
+    win_images(winstates, n_win, tb->W);
+    WinRules w;
+    if (upload_win(tb->W.data(), tb->W.size(), tb->r, &w)) return -1;
+    tb->p4 = p4;
+    int nl = (int)tb->L.size();
+    for (int i = 0; i < nl; i++) tb->L[i].vals = nullptr, tb->L[i].arg = nullptr;
+    tb->solve_lo = nl;
+    /* solve.c:203: s runs down from the top while s >= end_solve and s != 0 */
+    for (int i = nl - 1; i >= 0; i--) {
+        int s = tb->start + 2 * i;
+        if (s < end_solve || s <= 0) break;
+        TbLayer &l = tb->L[i];
+        l.vals = (double *)tb->arena.alloc(std::max<size_t>(l.n, 1) * 8);
+        if (keep_argmax) l.arg = (uint64_t *)tb->arena.alloc(std::max<size_t>(l.n, 1) * 8);
+        if (!l.vals || (keep_argmax && !l.arg)) return fail_msg("value allocation failed");
+        if (solve_layer_dev(tb->r, l.keys, l.n, l.vals, l.arg, view_of(tb, i + 1), view_of(tb, i + 2),
+                            w, p4))
+            return -1;
+        tb->solve_lo = i;
+    }
+    tb->solved = true;
+    tb->have_arg = keep_argmax != 0;
+    return check_sync(E.st, "cg_tb_solve");
+}
+
+extern "C" int cg_tb_adjoint(cg_tablebase *tb, const uint64_t *initial, size_t n, double *P, double *dP,
+                             double *D)
+{
+    if (eng_init()) return -1;
+    if (!tb->solved || !tb->have_arg) return fail_msg("cg_tb_adjoint needs cg_tb_solve(keep_argmax=1)");
+    const uint16_t *lut = get_lut(tb->r.ignore_f);
+    if (!lut) return fail_msg("row table upload failed");
+    int nl = (int)tb->L.size();
+    WinRules w;
+    if (upload_win(tb->W.data(), tb->W.size(), tb->r, &w)) return -1;
+    /* occupation per layer + D per layer + P, in one scratch allocation */
+    size_t tot = 0;
+    std::vector<size_t> off(nl + 1);
+    for (int i = 0; i < nl; i++) off[i] = tot, tot += tb->L[i].n;
+    off[nl] = tot;
+    double *occ = nullptr, *dD = nullptr;
+    uint64_t *dinit = nullptr;
+    HC(hipMalloc((void **)&occ, std::max<size_t>(tot, 1) * 8));
+    HC(hipMalloc((void **)&dD, (nl + 2) * 8));
+    HC(hipMalloc((void **)&dinit, std::max<size_t>(n, 1) * 8));
+    HC(hipMemsetAsync(occ, 0, std::max<size_t>(tot, 1) * 8, E.st));
+    HC(hipMemsetAsync(dD, 0, (nl + 2) * 8, E.st));
+    if (n) HC(hipMemcpyAsync(dinit, initial, n * 8, hipMemcpyHostToDevice, E.st));
+    double *dPp = dD + nl;
+    if (nl > 0 && tb->solve_lo == 0)
+        launch_adjoint_init(E.st, dinit, n, view_of(tb, 0), occ, lut, tb->r.free_formation, dPp);
+    for (int i = 0; i < nl; i++) {
+        if (!tb->L[i].vals) continue;
+        LayerView t2 = view_of(tb, i + 1), t4 = view_of(tb, i + 2);
+        double *o2 = (i + 1 < nl) ? occ + off[i + 1] : nullptr;
+        double *o4 = (i + 2 < nl) ? occ + off[i + 2] : nullptr;
+        int h = prof_begin(PK_ADJOINT);
+        launch_adjoint(E.st, tb->L[i].keys, tb->L[i].n, occ + off[i], tb->L[i].arg, t2, o2, t4, o4, w,
+                       tb->p4, lut, tb->r.free_formation, dD + i);
+        prof_end(h, 24.0 * tb->L[i].n);
+    }
+    std::vector<double> h(nl + 1);
+    HC(hipMemcpyAsync(h.data(), dD, (nl + 1) * 8, hipMemcpyDeviceToHost, E.st));
+    if (check_sync(E.st, "cg_tb_adjoint")) return -1;
+    hipFree(occ);
+    hipFree(dD);
+    hipFree(dinit);
+    double s = 0;
+    for (int i = 0; i < nl; i++) {
+        s += h[i];
+        if (D) D[i] = h[i];
+    }
+    if (P) *P = h[nl];
+    if (dP) *dP = s;
+    return 0;
+}
+
+extern "C" int cg_tb_num_layers(const cg_tablebase *tb) { return (int)tb->L.size(); }
+extern "C" int cg_tb_start(const cg_tablebase *tb) { return tb->start; }
+
+extern "C" size_t cg_tb_layer_size(const cg_tablebase *tb, int idx)
+{
+    return (idx >= 0 && idx < (int)tb->L.size()) ? tb->L[idx].n : 0;
+}
+
+extern "C" size_t cg_tb_total_boards(const cg_tablebase *tb)
+{
+    size_t s = 0;
+    for (auto &l : tb->L) s += l.n;
+    return s;
+}
+
+extern "C" int cg_tb_copy_layer(const cg_tablebase *tb, int idx, uint64_t *keys, double *values)
+{
+    if (idx < 0 || idx >= (int)tb->L.size()) return fail_msg("layer index out of range");
+    const TbLayer &l = tb->L[idx];
+    if (keys && l.n) HC(hipMemcpy(keys, l.keys, l.n * 8, hipMemcpyDeviceToHost));
+    if (values && l.n) {
+        if (!l.vals) return fail_msg("layer not solved");
+        HC(hipMemcpy(values, l.vals, l.n * 8, hipMemcpyDeviceToHost));
+    }
+    return 0;
+}
+
+extern "C" int cg_tb_layer_device(const cg_tablebase *tb, int idx, const uint64_t **keys,
+                                  const double **values)
+{
+    if (idx < 0 || idx >= (int)tb->L.size()) return fail_msg("layer index out of range");
+    if (keys) *keys = tb->L[idx].keys;
+    if (values) *values = tb->L[idx].vals;
+    return 0;
+}
+
+extern "C" int cg_tb_write_files(const cg_tablebase *tb, const char *boards_fmt, const char *tables_fmt)
+{
+    std::vector<uint64_t> k;
+    std::vector<double> v;
+    for (int i = 0; i < (int)tb->L.size(); i++) {
+        const TbLayer &l = tb->L[i];
+        int s = tb->start + 2 * i;
+        k.resize(l.n);
+        if (cg_tb_copy_layer(tb, i, k.data(), nullptr)) return -1;
+        char name[4096];
+        if (boards_fmt) {
+            snprintf(name, sizeof name, boards_fmt, s);
+            FILE *f = fopen(name, "wb");
+            if (!f) return fail_msg("cannot open boards file");
+            fwrite(k.data(), 8, l.n, f);
+            fclose(f);
+        }
+        if (tables_fmt && l.vals) {
+            v.resize(l.n);
+            if (cg_tb_copy_layer(tb, i, nullptr, v.data())) return -1;
+            snprintf(name, sizeof name, tables_fmt, s);
+            FILE *f = fopen(name, "wb");
+            if (!f) return fail_msg("cannot open tables file");
+            fwrite(k.data(), 8, l.n, f);
+            fwrite(v.data(), 8, l.n, f);
+            fclose(f);
+        }
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* device primitives                                                         */
+/* ------------------------------------------------------------------------ */
+
+extern "C" int cg_dev_move(const cg_rules *rules, const uint64_t *d_in, size_t n, uint64_t *d_out,
+                           size_t *n_out)
+{
+    if (eng_init()) return -1;
+    const uint16_t *lut = get_lut(rules->ignore_f);
+    if (!lut) return fail_msg("row table upload failed");
+    HC(hipMemsetAsync(E.d_cnt, 0, 8, E.st));
+    int h = prof_begin(PK_MOVE);
+    launch_move(E.st, d_in, n, d_out, E.d_cnt, lut, move_rules(*rules));
+    prof_end(h, 0);
+    if (read_counter(n_out)) return -1;
+    prof_set_bytes(h, 8.0 * n + 8.0 * *n_out);
+    return 0;
+}
+
+extern "C" int cg_dev_spawn(const uint64_t *d_in, size_t n, uint64_t *d_out2, uint64_t *d_out4,
+                            size_t *n_out)
+{
+    if (eng_init()) return -1;
+    HC(hipMemsetAsync(E.d_cnt, 0, 8, E.st));
+    int h = prof_begin(PK_SPAWN);
+    launch_spawn(E.st, d_in, n, d_out2, d_out4, E.d_cnt);
+    prof_end(h, 0);
+    if (read_counter(n_out)) return -1;
+    prof_set_bytes(h, 8.0 * n + 16.0 * *n_out);
+    return 0;
+}
+
+extern "C" int cg_dev_sort_unique(uint64_t *d_keys, size_t n, uint64_t *d_out, size_t *n_unique)
+{
+    if (eng_init()) return -1;
+    SortState ss;
+    if (sort_begin(d_keys, n, &ss)) return -1;
+    int hu = prof_begin(PK_UNIQUE);
+    sort_unique_finish(E.st, ss, d_out);
+    prof_end(hu, 8.0 * n + 8.0 * ss.n_unique);
+    *n_unique = ss.n_unique;
+    return check_sync(E.st, "cg_dev_sort_unique");
+}
+
+extern "C" int cg_dev_lookup(const uint64_t *d_keys, const double *d_vals, size_t n, const uint64_t *d_q,
+                             size_t nq, double *d_out, int canonicalize)
+{
+    if (eng_init()) return -1;
+    LayerView t{ d_keys, d_vals, n, nullptr, split_count(n) };
+    uint64_t *split = nullptr;
+    if (t.ns) {
+        HC(hipMalloc((void **)&split, t.ns * 8));
+        launch_build_split(E.st, d_keys, n, split, t.ns);
+        t.split = split;
+    }
+    int h = prof_begin(PK_LOOKUP);
+    launch_lookup(E.st, t, d_q, nq, d_out, canonicalize);
+    prof_end(h, 16.0 * nq);
+    int rc = check_sync(E.st, "cg_dev_lookup");
+    if (split) hipFree(split);
+    return rc;
+}
+
+extern "C" int cg_dev_solve_layer(const cg_rules *rules, const uint64_t *d_M, size_t n, double *d_V,
+                                  const uint64_t *d_K2, const double *d_V2, size_t n2,
+                                  const uint64_t *d_K4, const double *d_V4, size_t n4,
+                                  const uint64_t *win, size_t n_images, double p4)
+{
+    if (eng_init()) return -1;
+    WinRules w;
+    if (upload_win(win, n_images, *rules, &w)) return -1;
+    LayerView t2{ d_K2, d_V2, n2, nullptr, split_count(n2) };
+    LayerView t4{ d_K4, d_V4, n4, nullptr, split_count(n4) };
+    uint64_t *sp = nullptr;
+    HC(hipMalloc((void **)&sp, (size_t)(t2.ns + t4.ns + 1) * 8));
+    t2.split = sp;
+    t4.split = sp + t2.ns;
+    launch_build_split(E.st, d_K2, n2, (uint64_t *)t2.split, t2.ns);
+    launch_build_split(E.st, d_K4, n4, (uint64_t *)t4.split, t4.ns);
+    int rc = solve_layer_dev(*rules, d_M, n, d_V, nullptr, t2, t4, w, p4);
+    if (!rc) rc = check_sync(E.st, "cg_dev_solve_layer");
+    hipFree(sp);
+    return rc;
+}
+
+extern "C" int cg_dev_sync(void)
+{
+    if (eng_init()) return -1;
+    return check_sync(E.st, "cg_dev_sync");
+}
+
+/* ------------------------------------------------------------------------ */
+/* streaming interface for the drop-in generate() / solve()                  */
+/* ------------------------------------------------------------------------ */
+
+static cg_rules to_rules(const cge_rules *r)
+{
+    cg_rules x;
+    x.free_formation = r->free_formation;
+    x.ignore_f = r->ignore_f;
+    x.prune = r->prune;
+    x.stsl = r->stsl;
+    x.ltc = r->ltc;
+    x.smallest_large = r->smallest_large;
+    x.nox = r->nox;
+    x.score = r->score;
+    return x;
+}
+
+static struct {
+    cg_rules r;
+    Buf P, C, P2, C2, M;
+    size_t nP = 0, nC = 0, nM = 0;
+} G;
+
+static uint64_t *gbuf_alloc(size_t n, void *)
+{
+    if (ensure(G.M, n + 1)) return nullptr;
+    return G.M.p;
+}
+
+extern "C" int cge_gen_begin(const cge_rules *r, const uint64_t *initial, size_t n)
+{
+    if (eng_init()) return -1;
+    G.r = to_rules(r);
+    if (ensure(G.P, n + 1)) return -1;
+    if (n) HC(hipMemcpy(G.P.p, initial, n * 8, hipMemcpyHostToDevice));
+    G.nP = n;
+    G.nC = 0;
+    G.nM = 0;
+    return 0;
+}
+
+extern "C" int cge_gen_step(size_t *n_layer)
+{
+    uint64_t *M = nullptr;
+    size_t nP2 = 0, nC2 = 0;
+    if (gen_step(G.r, G.P.p, G.nP, G.C.p, G.nC, gbuf_alloc, nullptr, &M, &G.nM, G.P2, &nP2, G.C2,
+                 &nC2))
+        return -1;
+    std::swap(G.P, G.P2);
+    std::swap(G.C, G.C2);
+    G.nP = nP2;
+    G.nC = nC2;
+    *n_layer = G.nM;
+    return 0;
+}
+
+extern "C" int cge_gen_fetch(uint64_t *host, size_t n)
+{
+    if (n > G.nM) return fail_msg("cge_gen_fetch: too many boards");
+    if (n) HC(hipMemcpy(host, G.M.p, n * 8, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+extern "C" void cge_gen_end(void)
+{
+    buf_free(G.P);
+    buf_free(G.C);
+    buf_free(G.P2);
+    buf_free(G.C2);
+    buf_free(G.M);
+}
+
+struct SolveLayerBuf {
+    Buf keys, vals, split;
+    size_t n = 0;
+    int ns = 0;
+};
+
+static struct {
+    cg_rules r;
+    WinRules w;
+    double p4 = 0.1;
+    SolveLayerBuf L[3];   /* rotating: cur, s+2, s+4 */
+    int i2 = -1, i4 = -1;
+} S;
+
+static LayerView sview(int i)
+{
+    LayerView v{ nullptr, nullptr, 0, nullptr, 0 };
+    if (i < 0) return v;
+    v.keys = S.L[i].keys.p;
+    v.vals = (const double *)S.L[i].vals.p;
+    v.n = S.L[i].n;
+    v.split = S.L[i].split.p;
+    v.ns = S.L[i].ns;
+    return v;
+}
+
+extern "C" int cge_solve_begin(const cge_rules *r, const uint64_t *win, size_t n_images, double p4)
+{
+    if (eng_init()) return -1;
+    S.r = to_rules(r);
+    S.p4 = p4;
+    S.i2 = S.i4 = -1;
+    return upload_win(win, n_images, S.r, &S.w);
+}
+
+extern "C" int cge_solve_step(const uint64_t *keys, size_t n, double *values)
+{
+    int cur = 0;
+    while (cur == S.i2 || cur == S.i4) cur++;
+    SolveLayerBuf &b = S.L[cur];
+    if (ensure(b.keys, n + 1) || ensure(b.vals, n + 1)) return -1;
+    b.n = n;
+    b.ns = split_count(n);
+    if (ensure(b.split, (size_t)b.ns + 1)) return -1;
+    if (n) HC(hipMemcpyAsync(b.keys.p, keys, n * 8, hipMemcpyHostToDevice, E.st));
+    launch_build_split(E.st, b.keys.p, n, b.split.p, b.ns);
+    if (solve_layer_dev(S.r, b.keys.p, n, (double *)b.vals.p, nullptr, sview(S.i2), sview(S.i4), S.w,
+                        S.p4))
+        return -1;
+    if (n) HC(hipMemcpyAsync(values, b.vals.p, n * 8, hipMemcpyDeviceToHost, E.st));
+    if (check_sync(E.st, "cge_solve_step")) return -1;
+    S.i4 = S.i2;
+    S.i2 = cur;
+    return 0;
+}
+
+extern "C" void cge_solve_end(void)
+{
+    for (auto &l : S.L) {
+        buf_free(l.keys);
+        buf_free(l.vals);
+        buf_free(l.split);
+        l.n = 0;
+    }
+    S.i2 = S.i4 = -1;
+}

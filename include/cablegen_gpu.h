@@ -1,0 +1,128 @@
+/*
+ * cablegen_gpu.h -- extension entry points of the MI355X engine.
+ *
+ * These have no counterpart in the reference's public header
+ * (api/cablegen.h); the drop-in entry points are in cablegen.h.  They expose
+ * the device-resident pipeline (layers kept in HBM between generate, solve
+ * and the adjoint), the per-stage device primitives used by the kernel
+ * microbenchmarks and parity tests, the batched multi-table run and the
+ * per-kernel timers.
+ *
+ * Conventions: plain C types, host or device pointers as stated, sizes in
+ * elements.  Every function returns 0 on success and a negative value on
+ * failure; cg_last_error() gives the message.  Calls are blocking unless
+ * stated and, like the reference's API, not reentrant.
+ */
+#ifndef CABLEGEN_GPU_H
+#define CABLEGEN_GPU_H
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* The rules that shape layers and values: the settings the reference reads
+ * on this path (inc/settings.h:15-31, src/settings.c:35-59). */
+typedef struct {
+    int free_formation;     /* [Cablegen] free_formation            */
+    int ignore_f;           /* [Cablegen] ignore_f                  */
+    int prune;              /* [Generate] prune                     */
+    long stsl;              /* [Generate] stsl                      */
+    long ltc;               /* [Generate] ltc                       */
+    long smallest_large;    /* [Generate] smallest_large            */
+    int nox;                /* [Cablegen] nox                       */
+    int score;              /* [Solve] score                        */
+} cg_rules;
+
+const char *cg_last_error(void);
+
+/* ---------------- device-resident tablebase ---------------- */
+
+typedef struct cg_tablebase cg_tablebase;
+
+cg_tablebase *cg_tb_create(const cg_rules *rules);
+void cg_tb_free(cg_tablebase *tb);
+
+/* Generates M_s for s = start, start+2, ..., end from the (pre-move)
+ * initial boards (host memory), keeping every layer in HBM.  Same recurrence
+ * as generate() (src/generate.c:551-592). */
+int cg_tb_generate(cg_tablebase *tb, const uint64_t *initial, size_t n, int start, int end);
+
+/* Solves every stored layer with s >= end_solve from the top down, as
+ * solve() does (src/solve.c:172-224).  winstates are the unexpanded boards
+ * (host); their 8 images are added here.  p4 is the 4-spawn probability
+ * (0.1 reproduces the reference).  keep_argmax stores the 2-bit argmax move
+ * per (board, empty cell, spawn) needed by cg_tb_adjoint. */
+int cg_tb_solve(cg_tablebase *tb, const uint64_t *winstates, size_t n_win, int end_solve,
+                double p4, int keep_argmax);
+
+/* Exact derivative of the initial boards' mean win probability with respect
+ * to p4, by an upward occupation-measure sweep over the optimal policy
+ * (first strict maximum over left, right, up, down, as the CLI's best()
+ * picks moves, frontends/cli/main.c:174-189).  Needs cg_tb_solve(...,
+ * keep_argmax=1) with the same p4.  D receives cg_tb_num_layers() per-layer
+ * sensitivities (may be NULL). */
+int cg_tb_adjoint(cg_tablebase *tb, const uint64_t *initial, size_t n, double *P, double *dP,
+                  double *D);
+
+int cg_tb_num_layers(const cg_tablebase *tb);
+int cg_tb_start(const cg_tablebase *tb);
+size_t cg_tb_layer_size(const cg_tablebase *tb, int idx);
+/* total boards over all layers */
+size_t cg_tb_total_boards(const cg_tablebase *tb);
+/* copies layer idx (tile sum start + 2 idx) to host; either pointer may be NULL */
+int cg_tb_copy_layer(const cg_tablebase *tb, int idx, uint64_t *keys, double *values);
+/* device pointers of layer idx (values NULL before solve) */
+int cg_tb_layer_device(const cg_tablebase *tb, int idx, const uint64_t **keys,
+                       const double **values);
+/* writes %d.boards / %d.tables files in the reference's formats
+ * (src/generate.c:74-87, src/solve.c:29-45); either fmt may be NULL */
+int cg_tb_write_files(const cg_tablebase *tb, const char *boards_fmt, const char *tables_fmt);
+
+/* ---------------- device primitives (device pointers) ---------------- */
+
+/* move + lock + prune + canonicalise of n boards into out (capacity >= 4n);
+ * *n_out receives the number of candidates (unsorted). */
+int cg_dev_move(const cg_rules *rules, const uint64_t *d_in, size_t n, uint64_t *d_out,
+                size_t *n_out);
+/* spawn expansion: out2/out4 (capacity >= 16n) receive canon(b[x<-1]) and
+ * canon(b[x<-2]) for every empty cell; *n_out the count. */
+int cg_dev_spawn(const uint64_t *d_in, size_t n, uint64_t *d_out2, uint64_t *d_out4,
+                 size_t *n_out);
+/* sorts d_keys[0..n) and writes the unique keys to d_out (capacity n);
+ * d_keys is clobbered.  *n_unique receives the count. */
+int cg_dev_sort_unique(uint64_t *d_keys, size_t n, uint64_t *d_out, size_t *n_unique);
+/* exact-match lookup of nq queries into a sorted unique layer (keys, values);
+ * misses give 0 (lookup, src/solve.c:108-164). */
+int cg_dev_lookup(const uint64_t *d_keys, const double *d_vals, size_t n, const uint64_t *d_q,
+                  size_t nq, double *d_out, int canonicalize);
+/* solve one layer against two solved successor layers (device pointers);
+ * win_images are the already-expanded winstates (host). */
+int cg_dev_solve_layer(const cg_rules *rules, const uint64_t *d_M, size_t n, double *d_V,
+                       const uint64_t *d_K2, const double *d_V2, size_t n2,
+                       const uint64_t *d_K4, const double *d_V4, size_t n4,
+                       const uint64_t *win_images, size_t n_images, double p4);
+/* synchronises the library's stream */
+int cg_dev_sync(void);
+
+/* ---------------- kernel timers ---------------- */
+
+/* When enabled, the library brackets the launches of its main kernels with
+ * HIP events on the stream they run on and accumulates the time and the
+ * algorithmic bytes of each launch. */
+void cg_prof_enable(int on);
+void cg_prof_reset(void);
+/* kernel: "move", "spawn", "sort_hist", "sort_scatter", "unique", "solve",
+ * "adjoint", "lookup".  Returns the number of launches. */
+long cg_prof_query(const char *kernel, double *total_ms, double *total_bytes);
+
+/* ---------------- settings used by the drop-in API ---------------- */
+
+/* Copies the library's current settings singleton into rules. */
+void cg_rules_from_settings(cg_rules *rules);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
