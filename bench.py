@@ -1,0 +1,178 @@
+"""Benchmark: boards/s of layered 2048 tablebase generate + solve on one MI355X.
+
+Workload (one step): the DPDGap-512 tablebase -- initial board
+0x00008765FFF3FFF2 (tile sum 492), formation cells as walls (ignore_f),
+prune stsl=58 ltc=5 smallest_large=6, generation to tile sum 1100, solve
+from 1100 down to 0 with the winstate 0x0000900000000000 (512 next to the
+formation, all 8 images).  Every layer stays resident in HBM; the input (one
+board) is on the device before the timed region starts.
+
+value = total boards over all layers / (T_generate + T_solve), per step.
+
+Also reported:
+  roofline      the kernel with the largest total time in the timed steps,
+                timed with HIP events on the library's stream; algorithmic
+                bytes per launch as documented in DESIGN.md
+  cpu_baseline  the oracle (a pthreads C port of the reference's CPU
+                algorithm) on a bounded sample: full generation steps and
+                solves of three consecutive mid-table layers, whose outputs
+                are also checked bit-for-bit against the GPU's
+  adjoint       time of the exact dP(win)/dp4 sweep relative to a plain solve
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+METRIC = "boards/sec gen+solve, DPDGap-512, 1 GPU vs host CPU; adjoint/solve time ratio"
+INITIAL = [0x00008765FFF3FFF2]
+WINSTATES = [0x0000900000000000]
+END_GEN = 1100
+RULES = dict(ignore_f=True, prune=True, stsl=58, ltc=5, smallest_large=6)
+HBM_PEAK_GBS = 8000.0
+KERNELS = ["move", "spawn", "sort_hist", "sort_scatter", "unique", "solve"]
+
+
+def cpu_baseline(tb, cg, cores):
+    """Oracle timing on three consecutive full-size layers in the middle of
+    the table (generation step + solve of each), checked against the GPU."""
+    from tests import oracle_lib as O
+    cfg = O.make_cfg(cores=cores, **RULES)
+    sizes = [tb.layer_size(i) for i in range(tb.num_layers)]
+    i0 = int(np.argmax(sizes[4:-4])) + 4 - 1
+    idx = [i0, i0 + 1, i0 + 2]
+    # pre-move layer P_s = uniq(S2(M_{s-2}) u S4(M_{s-4})) and the carry S4(M_{s-2})
+    m2 = tb.keys(idx[0] - 1)
+    m4 = tb.keys(idx[0] - 2)
+    s2_a, s4_a = O.spawn_sets(cfg, m2)
+    _, s4_b = O.spawn_sets(cfg, m4)
+    P = np.union1d(s2_a, s4_b).astype(np.uint64)
+    Cc = s4_a
+    W = O.expand_winstates(WINSTATES)
+    t_gen = t_solve = 0.0
+    boards = 0
+    exact = True
+    for i in idx:
+        t0 = time.perf_counter()
+        M, P, Cc = O.gen_layer(cfg, P, Cc)
+        t_gen += time.perf_counter() - t0
+        gk = tb.keys(i)
+        exact &= bool(np.array_equal(M, gk))
+        K2, V2 = tb.keys(i + 1), tb.values(i + 1)
+        K4, V4 = tb.keys(i + 2), tb.values(i + 2)
+        t0 = time.perf_counter()
+        V = O.solve_layer(cfg, M, K2, V2, K4, V4, W)
+        t_solve += time.perf_counter() - t0
+        exact &= bool(np.array_equal(V.view(np.uint64), tb.values(i).view(np.uint64)))
+        boards += len(M)
+    s0 = tb.start + 2 * idx[0]
+    return {
+        "value": boards / (t_gen + t_solve), "unit": "boards/s", "cores": cores, "kind": "port",
+        "sample": f"generation step + solve of the layers s={s0},{s0 + 2},{s0 + 4} "
+                  f"({boards} boards; gen {t_gen:.2f}s, solve {t_solve:.2f}s); "
+                  f"outputs bit-identical to the GPU's: {exact}",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-adjoint", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU sharding is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import cablegen_amd as cg
+    r = cg.rules(**RULES)
+    tb = cg.Tablebase(r)
+    start = cg.lib().get_sum(INITIAL[0])
+
+    def step():
+        t0 = time.perf_counter()
+        tb.generate(INITIAL, start, END_GEN)
+        t1 = time.perf_counter()
+        tb.solve(WINSTATES)
+        t2 = time.perf_counter()
+        return t1 - t0, t2 - t1
+
+    cg.prof_enable(True)
+    for _ in range(args.warmup):
+        step()
+    cg.prof_reset()
+    cg.lib().cg_dev_sync()
+    gen_t, sol_t = [], []
+    for _ in range(args.steps):
+        g, s = step()
+        gen_t.append(g)
+        sol_t.append(s)
+    cg.lib().cg_dev_sync()
+    total = tb.total_boards()
+    T = (sum(gen_t) + sum(sol_t)) / args.steps
+    value = total / T
+
+    kern = {}
+    for k in KERNELS:
+        n, ms, b = cg.prof_query(k)
+        kern[k] = dict(launches=n, ms=ms, bytes=b)
+    cg.prof_enable(False)
+    dom = max(KERNELS, key=lambda k: kern[k]["ms"])
+    d = kern[dom]
+    avg_ms = d["ms"] / max(d["launches"], 1)
+    avg_bytes = d["bytes"] / max(d["launches"], 1)
+    achieved = avg_bytes / (avg_ms * 1e6) if avg_ms else 0.0
+
+    out = {
+        "metric": METRIC, "value": value, "unit": "boards/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": T * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u64 keys / f64 values", "data": "synthetic (the DPDGap-512 table itself)",
+        "config": {"workload": "DPDGap-512 tablebase: initial 0x00008765FFF3FFF2 (sum 492), "
+                               "ignore_f, prune stsl=58 ltc=5 smallest_large=6, "
+                               "generate 492..1100, solve 1100..0, winstate 0x0000900000000000",
+                   "layers": tb.num_layers, "boards": total,
+                   "gen_ms": 1e3 * sum(gen_t) / args.steps, "solve_ms": 1e3 * sum(sol_t) / args.steps},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "avg_launch_us": avg_ms * 1e3, "avg_algorithmic_bytes": avg_bytes},
+        "kernels": {k: {"launches": v["launches"] // max(args.steps, 1),
+                        "ms_per_step": v["ms"] / args.steps,
+                        "GB_s": (v["bytes"] / v["ms"] / 1e6) if v["ms"] else 0.0}
+                    for k, v in kern.items()},
+    }
+
+    if not args.no_adjoint:
+        tb.solve(WINSTATES)
+        cg.lib().cg_dev_sync()
+        t0 = time.perf_counter()
+        tb.solve(WINSTATES)
+        t1 = time.perf_counter()
+        tb.solve(WINSTATES, keep_argmax=True)
+        t2 = time.perf_counter()
+        P, dP, D = tb.adjoint(INITIAL)
+        t3 = time.perf_counter()
+        out["adjoint"] = {"P_win": P, "dP_dp4": dP, "plain_solve_ms": 1e3 * (t1 - t0),
+                          "solve_with_argmax_ms": 1e3 * (t2 - t1), "adjoint_sweep_ms": 1e3 * (t3 - t2),
+                          "adjoint_over_solve": (t3 - t2) / (t1 - t0),
+                          "argmax_solve_plus_adjoint_over_solve": (t3 - t1) / (t1 - t0)}
+
+    if not args.no_cpu:
+        cores = min(os.cpu_count() or 1, 16)
+        out["cpu_baseline"] = cpu_baseline(tb, cg, cores)
+        out["speedup_vs_cpu"] = value / out["cpu_baseline"]["value"]
+    tb.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -54,7 +54,8 @@ def build_lib():
 def build_oracle():
     _run(["make", "-s", "-B", "-f", "oracle/Makefile"])
     if os.path.isdir("/root/reference/src"):
-        _run(["make", "-s", "-j8", "-f", "oracle/Makefile.ref"])
+        _run(["make", "-s", "-j8", "-f", "oracle/Makefile.ref", "all"])
+        _run(["make", "-s", "-B", "-f", "oracle/Makefile.ref", "cli"])
 
 
 def build_all():

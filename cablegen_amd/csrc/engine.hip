@@ -148,6 +148,19 @@ struct ProfRec {
 };
 static bool g_prof = false;
 static std::vector<ProfRec> g_recs;
+static std::vector<hipEvent_t> g_evpool;
+
+static hipEvent_t ev_get()
+{
+    if (!g_evpool.empty()) {
+        hipEvent_t e = g_evpool.back();
+        g_evpool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
 static double g_ms[PK_N], g_bytes[PK_N];
 static long g_launches[PK_N];
 
@@ -157,7 +170,9 @@ static int prof_begin(int kind)
     ProfRec r;
     r.kind = kind;
     r.bytes = 0;
-    if (hipEventCreate(&r.a) != hipSuccess || hipEventCreate(&r.b) != hipSuccess) return -1;
+    r.a = ev_get();
+    r.b = ev_get();
+    if (!r.a || !r.b) return -1;
     hipEventRecord(r.a, E.st);
     g_recs.push_back(r);
     return (int)g_recs.size() - 1;
@@ -185,8 +200,8 @@ static void prof_collect()
         g_ms[r.kind] += ms;
         g_bytes[r.kind] += r.bytes;
         g_launches[r.kind]++;
-        hipEventDestroy(r.a);
-        hipEventDestroy(r.b);
+        g_evpool.push_back(r.a);
+        g_evpool.push_back(r.b);
     }
     g_recs.clear();
 }
@@ -227,34 +242,45 @@ extern "C" long cg_prof_query(const char *kernel, double *total_ms, double *tota
 /* ------------------------------------------------------------------------ */
 
 struct Arena {
-    std::vector<void *> chunks;
-    char *cur = nullptr;
-    size_t left = 0;
+    struct Chunk {
+        char *p;
+        size_t size;
+    };
+    std::vector<Chunk> chunks;
+    size_t idx = 0;          /* chunk being filled */
+    size_t used = 0;         /* bytes used in chunks[idx] */
     static constexpr size_t CHUNK = (size_t)2 << 30;
 
     void *alloc(size_t bytes)
     {
         bytes = (bytes + 255) & ~(size_t)255;
         if (bytes == 0) bytes = 256;
-        if (bytes > left) {
+        while (idx < chunks.size() && used + bytes > chunks[idx].size) {
+            idx++;
+            used = 0;
+        }
+        if (idx == chunks.size()) {
             size_t c = std::max(CHUNK, bytes);
             void *p = nullptr;
             if (hipMalloc(&p, c) != hipSuccess) return nullptr;
-            chunks.push_back(p);
-            cur = (char *)p;
-            left = c;
+            chunks.push_back({ (char *)p, c });
+            used = 0;
         }
-        void *r = cur;
-        cur += bytes;
-        left -= bytes;
+        void *r = chunks[idx].p + used;
+        used += bytes;
         return r;
+    }
+    /* rewinds without freeing: the chunks are reused by the next run */
+    void reset()
+    {
+        idx = 0;
+        used = 0;
     }
     void release()
     {
-        for (void *p : chunks) hipFree(p);
+        for (auto &c : chunks) hipFree(c.p);
         chunks.clear();
-        cur = nullptr;
-        left = 0;
+        reset();
     }
 };
 
@@ -468,7 +494,8 @@ extern "C" void cg_tb_free(cg_tablebase *tb)
 extern "C" int cg_tb_generate(cg_tablebase *tb, const uint64_t *initial, size_t n, int start, int end)
 {
     if (eng_init()) return -1;
-    tb->arena.release();
+    HC(hipStreamSynchronize(E.st));
+    tb->arena.reset();
     tb->L.clear();
     tb->start = start;
     tb->solved = false;

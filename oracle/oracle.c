@@ -505,6 +505,25 @@ int orc_gen_layer(const orc_cfg *cfg, const uint64_t *P, size_t nP,
     return 0;
 }
 
+int orc_spawn_sets(const orc_cfg *cfg, const uint64_t *M, size_t nM,
+                   uint64_t **S2, size_t *n2, uint64_t **S4, size_t *n4)
+{
+    int cores = cfg->cores > 0 ? cfg->cores : 1;
+    gen_task *tasks = calloc((size_t)cores, sizeof(gen_task));
+    uint64_t **arrs = malloc(sizeof(uint64_t *) * (size_t)cores);
+    size_t *sz = malloc(sizeof(size_t) * (size_t)cores);
+    run_tasks(cfg, M, nM, OP_SPAWN, tasks);
+    for (int i = 0; i < cores; i++) { arrs[i] = tasks[i].out2.p; sz[i] = tasks[i].out2.n; }
+    *S2 = orc_kway_dedupe(arrs, sz, (size_t)cores, n2);
+    for (int i = 0; i < cores; i++) { arrs[i] = tasks[i].out4.p; sz[i] = tasks[i].out4.n; }
+    *S4 = orc_kway_dedupe(arrs, sz, (size_t)cores, n4);
+    for (int i = 0; i < cores; i++) { free(tasks[i].out2.p); free(tasks[i].out4.p); }
+    free(tasks);
+    free(arrs);
+    free(sz);
+    return 0;
+}
+
 int orc_generate(const orc_cfg *cfg, const uint64_t *initial, size_t n_init,
                  int start, int end, uint64_t ***layers, size_t **sizes, int *nlayers)
 {
@@ -763,10 +782,10 @@ int orc_adjoint(const orc_cfg *cfg, uint64_t **layers, const size_t *sizes, int 
 
 uint64_t orc_hash_u64(const uint64_t *a, size_t n)
 {
+    /* FNV-1a over 64-bit words (not bytes): h = (h ^ w) * prime */
     uint64_t h = 1469598103934665603ULL;
-    const unsigned char *p = (const unsigned char *)a;
-    for (size_t i = 0; i < n * 8; i++) {
-        h ^= p[i];
+    for (size_t i = 0; i < n; i++) {
+        h ^= a[i];
         h *= 1099511628211ULL;
     }
     return h;

@@ -61,6 +61,11 @@ int orc_gen_layer(const orc_cfg *cfg, const uint64_t *P, size_t nP,
                   const uint64_t *C, size_t nC,
                   uint64_t **M, size_t *nM, uint64_t **P2, size_t *nP2,
                   uint64_t **C2, size_t *nC2);
+/* The spawn sets of a post-move layer: S2 = uniq{canon(m[x<-1])},
+ * S4 = uniq{canon(m[x<-2])} over empty cells (generate.c:177-199), with the
+ * same threads/sort/merge as a generation step.  Outputs are malloc'd. */
+int orc_spawn_sets(const orc_cfg *cfg, const uint64_t *M, size_t nM,
+                   uint64_t **S2, size_t *n2, uint64_t **S4, size_t *n4);
 /* Full generation s = start..end step 2; stores every M_s.  layers[i] is the
  * layer at tile sum start + 2i; nlayers = (end-start)/2 + 1. */
 int orc_generate(const orc_cfg *cfg, const uint64_t *initial, size_t n_init,
@@ -91,7 +96,7 @@ int orc_adjoint(const orc_cfg *cfg, uint64_t **layers, const size_t *sizes, int 
                 const uint64_t *initial, size_t n_init, double p4,
                 double *P_out, double *dP_out, double *D);
 
-/* FNV-1a 64 over a u64 array's little-endian bytes (layer manifests). */
+/* FNV-1a 64 over the u64 words of an array (layer manifests). */
 uint64_t orc_hash_u64(const uint64_t *a, size_t n);
 void     orc_free(void *p);
 

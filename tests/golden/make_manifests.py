@@ -1,7 +1,7 @@
 """Writes the layer manifests under tests/golden/.
 
-Each manifest lists, per tile-sum layer s: the board count, the FNV-1a-64 hash
-of the sorted u64 keys and of the f64 value bits, exactly as the %d.boards /
+Each manifest lists, per tile-sum layer s: the board count, the word-wise
+FNV-1a-64 hash of the sorted u64 keys and of the f64 value bits, exactly as the %d.boards /
 %d.tables files hold them.
 
   c1_ref.json      config 1 (2x4, initial 0x11000000FFFFFFFF, winstate 128 at
@@ -35,6 +35,23 @@ C2 = dict(initial=[0x00008765FFF3FFF2], win=[0x0000900000000000], end=1100,
           rules=dict(ignore_f=True, prune=True, stsl=58, ltc=5, smallest_large=6))
 
 
+def dump(obj):
+    """JSON with one layer row per line."""
+    txt = json.dumps(obj, indent=1)
+    out, depth_row = [], False
+    for line in txt.splitlines():
+        st = line.strip()
+        if depth_row:
+            out[-1] += st
+            if st.startswith("]"):
+                depth_row = False
+            continue
+        out.append(line)
+        if st == "[" and out and len(out) > 1 and out[-2].strip().endswith(("[", "],")):
+            depth_row = True
+    return "\n".join(out) + "\n"
+
+
 def make_c1():
     out = {"config": "2x4 initial 0x11000000FFFFFFFF winstate 0x7000000000000000 end 260",
            "source": "reference library built from its sources", "variants": {}}
@@ -51,7 +68,7 @@ def make_c1():
                            "%016x" % O.hash_u64(v.view(np.uint64))])
         out["variants"]["ignore_f" if ig else "default"] = layers
     with open(os.path.join(HERE, "c1_ref.json"), "w") as f:
-        json.dump(out, f, indent=0)
+        f.write(dump(out))
 
 
 def make_c2(cores=8):
@@ -74,7 +91,7 @@ def make_c2(cores=8):
            "p_initial_layer_max": float(max((v.max() for v in vals if len(v)), default=0.0)),
            "layers": rows}
     with open(os.path.join(HERE, "c2_oracle.json"), "w") as f:
-        json.dump(out, f, indent=0)
+        f.write(dump(out))
 
 
 if __name__ == "__main__":

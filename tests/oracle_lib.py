@@ -71,6 +71,8 @@ def orc():
                                     C.POINTER(U64P), C.POINTER(C.c_size_t),
                                     C.POINTER(U64P), C.POINTER(C.c_size_t),
                                     C.POINTER(U64P), C.POINTER(C.c_size_t)]
+        L.orc_spawn_sets.argtypes = [C.POINTER(OrcCfg), U64P, C.c_size_t, C.POINTER(U64P),
+                                     C.POINTER(C.c_size_t), C.POINTER(U64P), C.POINTER(C.c_size_t)]
         L.orc_generate.argtypes = [C.POINTER(OrcCfg), U64P, C.c_size_t, C.c_int, C.c_int,
                                    C.POINTER(C.POINTER(U64P)), C.POINTER(C.POINTER(C.c_size_t)),
                                    C.POINTER(C.c_int)]
@@ -140,6 +142,16 @@ def gen_layer(cfg, P, Cset):
     orc().orc_gen_layer(C.byref(cfg), pp, len(P), cp, len(Cset), C.byref(M), C.byref(nM),
                         C.byref(P2), C.byref(nP2), C.byref(C2), C.byref(nC2))
     return _take(M, nM.value), _take(P2, nP2.value), _take(C2, nC2.value)
+
+
+def spawn_sets(cfg, M):
+    """(S2, S4): sorted unique spawn sets of a post-move layer."""
+    M, mp = _u64(M)
+    S2, S4 = U64P(), U64P()
+    n2, n4 = C.c_size_t(), C.c_size_t()
+    orc().orc_spawn_sets(C.byref(cfg), mp, len(M), C.byref(S2), C.byref(n2), C.byref(S4),
+                         C.byref(n4))
+    return _take(S2, n2.value), _take(S4, n4.value)
 
 
 def generate(cfg, initial, start, end):

@@ -1,0 +1,413 @@
+"""Parity of the HIP path against the oracle, through the C ABI (GPU tests).
+
+Integer work (layers) must be bit-identical; FP64 values must be
+bit-identical too (same summation order, no contraction), which is stricter
+than the 1e-9 tolerance the values are specified with.  The adjoint is
+checked against the oracle's FP64 gradient (1e-12 relative) and central
+differences of GPU re-solves (1e-6 relative).
+"""
+import ctypes as C
+import json
+import os
+import random
+
+import numpy as np
+import pytest
+
+from tests import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def cg():
+    import cablegen_amd
+    cablegen_amd.lib()
+    assert torch.cuda.is_available(), "GPU tests need an MI355X"
+    return cablegen_amd
+
+
+def dev(a):
+    """uint64 numpy -> device int64 tensor (plumbing only)."""
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.uint64).view(np.int64)).cuda()
+
+
+def host(t, n=None):
+    a = t.cpu().numpy().view(np.uint64)
+    return a if n is None else a[:n]
+
+
+def rand_boards(rng, fmask, n, p0=0.4, kmax=9):
+    out = np.zeros(n, dtype=np.uint64)
+    for x in range(16):
+        sh = 60 - 4 * x
+        if (fmask >> sh) & 0xf:
+            out |= np.uint64(0xf << sh)
+        else:
+            v = rng.integers(1, kmax + 1, size=n).astype(np.uint64)
+            v[rng.random(n) < p0] = 0
+            out |= v << np.uint64(sh)
+    return out
+
+
+FORMATIONS = [0x00000000FFFFFFFF, 0x000000000000FFFF, 0x00000000FFF0FFF0, 0x00000000FF00FF00, 0]
+
+
+# ---------------------------------------------------------------- primitives
+
+@pytest.mark.parametrize("ff,ig,prune", [(0, 0, 0), (0, 1, 0), (1, 0, 0), (0, 1, 1)])
+def test_move_kernel(cg, ff, ig, prune):
+    rng = np.random.default_rng(1 + ff + 2 * ig + 4 * prune)
+    cfg = O.make_cfg(free_formation=ff, ignore_f=ig, prune=prune, stsl=58, ltc=5, smallest_large=6)
+    r = cg.rules(free_formation=ff, ignore_f=ig, prune=prune, stsl=58, ltc=5, smallest_large=6)
+    for fm in FORMATIONS:
+        b = rand_boards(rng, fm, 20000)
+        exp = []
+        for x in b:
+            x = int(x)
+            if prune and O.orc().orc_prune(x, 58, 5, 6):
+                continue
+            for d in range(4):
+                ok, m = O.move(cfg, x, d)
+                if ok and not (prune and O.orc().orc_prune(m, 58, 5, 6)):
+                    exp.append(O.canon(m))
+        din = dev(b)
+        dout = torch.empty(4 * len(b) + 1, dtype=torch.int64, device="cuda")
+        n = C.c_size_t()
+        assert cg.lib().cg_dev_move(C.byref(r), din.data_ptr(), len(b), dout.data_ptr(),
+                                    C.byref(n)) == 0
+        got = np.sort(host(dout, n.value))
+        assert np.array_equal(got, np.sort(np.array(exp, dtype=np.uint64)))
+
+
+def test_spawn_kernel(cg):
+    rng = np.random.default_rng(7)
+    for fm in FORMATIONS:
+        b = rand_boards(rng, fm, 5000)
+        e2, e4 = [], []
+        for x in b:
+            x = int(x)
+            for c in range(16):
+                if (x >> (60 - 4 * c)) & 0xf == 0:
+                    e2.append(O.canon(x | (1 << (60 - 4 * c))))
+                    e4.append(O.canon(x | (2 << (60 - 4 * c))))
+        d2 = torch.empty(16 * len(b), dtype=torch.int64, device="cuda")
+        d4 = torch.empty(16 * len(b), dtype=torch.int64, device="cuda")
+        n = C.c_size_t()
+        assert cg.lib().cg_dev_spawn(dev(b).data_ptr(), len(b), d2.data_ptr(), d4.data_ptr(),
+                                     C.byref(n)) == 0
+        assert n.value == len(e2)
+        # same order within a board (ascending cell); boards' blocks may interleave
+        assert np.array_equal(np.sort(host(d2, n.value)), np.sort(np.array(e2, np.uint64)))
+        assert np.array_equal(np.sort(host(d4, n.value)), np.sort(np.array(e4, np.uint64)))
+
+
+@pytest.mark.parametrize("case", ["empty", "one", "two_equal", "all_equal", "random_dups",
+                                  "full64", "one_bit", "high_bits", "big"])
+def test_sort_unique(cg, case):
+    rng = np.random.default_rng(hash(case) & 0xffff)
+    if case == "empty":
+        a = np.zeros(0, np.uint64)
+    elif case == "one":
+        a = np.array([12345], np.uint64)
+    elif case == "two_equal":
+        a = np.array([7, 7], np.uint64)
+    elif case == "all_equal":
+        a = np.full(100000, 0xFFF1FFF200000000, np.uint64)
+    elif case == "random_dups":
+        a = rng.integers(0, 50000, size=300001, dtype=np.uint64)
+    elif case == "full64":
+        a = rng.integers(0, 2**64 - 1, size=200000, dtype=np.uint64, endpoint=True)
+        a[::7] = a[0]
+        a[1] = 0
+        a[2] = np.uint64(2**64 - 1)
+    elif case == "one_bit":
+        a = (rng.integers(0, 2, size=70000, dtype=np.uint64) << np.uint64(37)) | np.uint64(5)
+    elif case == "high_bits":
+        a = rng.integers(0, 2**20, size=100000, dtype=np.uint64) << np.uint64(44)
+    else:
+        a = rand_boards(rng, 0x00000000FFF0FFF0, 1 << 24)
+    d = dev(a) if len(a) else torch.empty(1, dtype=torch.int64, device="cuda")
+    out = torch.empty(max(len(a), 1), dtype=torch.int64, device="cuda")
+    n = C.c_size_t()
+    assert cg.lib().cg_dev_sort_unique(d.data_ptr(), len(a), out.data_ptr(), C.byref(n)) == 0
+    assert np.array_equal(host(out, n.value), np.unique(a))
+
+
+@pytest.mark.parametrize("n", [0, 1, 50, 1023, 1024, 1025, 300000])
+def test_lookup_kernel(cg, n):
+    rng = np.random.default_rng(n + 3)
+    keys = np.unique(rand_boards(rng, 0x00000000FFF0FFF0, n) if n else np.zeros(0, np.uint64))
+    keys = np.unique(np.array([O.canon(int(k)) for k in keys], dtype=np.uint64)) if n < 5000 else \
+        np.unique(keys)
+    vals = rng.random(len(keys))
+    q = np.concatenate([keys[rng.integers(0, max(len(keys), 1), size=4000)] if len(keys) else
+                        np.zeros(0, np.uint64), rand_boards(rng, 0x00000000FFF0FFF0, 1000)])
+    exp = np.array([O.orc().orc_lookup(int(x), keys.ctypes.data_as(O.U64P),
+                                       vals.ctypes.data_as(O.F64P), len(keys), 0) for x in q])
+    dk = dev(keys) if len(keys) else torch.empty(1, dtype=torch.int64, device="cuda")
+    dv = torch.from_numpy(vals).cuda() if len(keys) else torch.empty(1, dtype=torch.float64,
+                                                                      device="cuda")
+    dq = dev(q)
+    do = torch.empty(len(q), dtype=torch.float64, device="cuda")
+    assert cg.lib().cg_dev_lookup(dk.data_ptr(), dv.data_ptr(), len(keys), dq.data_ptr(), len(q),
+                                  do.data_ptr(), 0) == 0
+    assert np.array_equal(do.cpu().numpy().view(np.uint64), exp.view(np.uint64))
+
+
+# ---------------------------------------------------------------- tablebases
+
+def oracle_table(cfg, initial, win, start, end, end_solve=0):
+    layers = O.generate(cfg, initial, start, end)
+    return layers, O.solve(cfg, layers, start, end, end_solve, win)
+
+
+def check_tb(tb, layers, vals):
+    assert tb.num_layers == len(layers)
+    for i in range(len(layers)):
+        assert np.array_equal(tb.keys(i), layers[i]), i
+        if vals[i] is not None:
+            assert np.array_equal(tb.values(i).view(np.uint64), vals[i].view(np.uint64)), i
+
+
+TB_CASES = {
+    "c1": (dict(), [0x11000000FFFFFFFF], [0x7000000000000000], 4, 260),
+    "c1_ignore_f": (dict(ignore_f=True), [0x11000000FFFFFFFF], [0x7000000000000000], 4, 260),
+    "free_formation": (dict(free_formation=True), [0x11000000FFFFFFFF], [0x7000000000000000], 4, 40),
+    "nox": (dict(ignore_f=True, nox=7), [0x11000000FFFFFFFF], [0x7000000000000000], 4, 200),
+    "score": (dict(ignore_f=True, score=True), [0x11000000FFFFFFFF], [0x7000000000000000], 4, 120),
+    "dpdgap_noprune": (dict(ignore_f=True), [0x00008765FFF3FFF2], [0x0000900000000000], 492, 560),
+    "dpdgap_prune": (dict(ignore_f=True, prune=True, stsl=58, ltc=5, smallest_large=6),
+                     [0x00008765FFF3FFF2], [0x0000900000000000], 492, 600),
+    "two_initial": (dict(ignore_f=True), [0x11000000FFFFFFFF, 0x10100000FFFFFFFF],
+                    [0x7000000000000000, 0x0000000000007000], 4, 160),
+    "3x4": (dict(ignore_f=True), [0x1200000000000000 | 0xFFFF], [0x6000000000000000], 6, 90),
+}
+
+
+@pytest.mark.parametrize("name", list(TB_CASES))
+def test_tablebase_vs_oracle(cg, name):
+    rules, initial, win, start, end = TB_CASES[name]
+    cfg = O.make_cfg(cores=8, **rules)
+    layers, vals = oracle_table(cfg, initial, win, start, end)
+    tb = cg.Tablebase(cg.rules(**rules))
+    tb.generate(initial, start, end)
+    tb.solve(win)
+    check_tb(tb, layers, vals)
+    tb.close()
+
+
+def test_partial_solve_range(cg):
+    rules, initial, win, start, end = TB_CASES["c1_ignore_f"]
+    cfg = O.make_cfg(cores=8, **rules)
+    layers, vals = oracle_table(cfg, initial, win, start, end, end_solve=150)
+    tb = cg.Tablebase(cg.rules(**rules))
+    tb.generate(initial, start, end)
+    tb.solve(win, end_solve=150)
+    for i in range(len(layers)):
+        if vals[i] is None:
+            continue
+        assert np.array_equal(tb.values(i).view(np.uint64), vals[i].view(np.uint64))
+    tb.close()
+
+
+def test_c1_manifest(cg):
+    man = json.load(open(os.path.join(HERE, "golden", "c1_ref.json")))
+    for name, ig in (("default", False), ("ignore_f", True)):
+        rows = man["variants"][name]
+        tb = cg.Tablebase(cg.rules(ignore_f=ig))
+        tb.generate([0x11000000FFFFFFFF], rows[0][0], rows[-1][0])
+        tb.solve([0x7000000000000000])
+        for i, (s, n, hk, hv) in enumerate(rows):
+            assert tb.layer_size(i) == n
+            assert "%016x" % O.hash_u64(tb.keys(i)) == hk, s
+            assert "%016x" % O.hash_u64(tb.values(i).view(np.uint64)) == hv, s
+        tb.close()
+
+
+# ---------------------------------------------------------------- drop-in ABI
+
+def test_dropin_generate_solve_files(cg, tmp_path):
+    """generate()/solve() through the api/cablegen.h entry points, with the
+    settings read from a config file, against the reference-made manifest."""
+    d = str(tmp_path)
+    os.makedirs(d + "/b")
+    os.makedirs(d + "/t")
+    with open(d + "/c.conf", "w") as f:
+        f.write("[Cablegen]\nignore_f = true\ncores = 4\n[Generate]\nprune = false\n"
+                f"dir = {d}/b/\nend = 260\n[Solve]\ndir = {d}/t/\nend = 0\n")
+    L = cg.lib()
+    L.change_config((d + "/c.conf").encode())
+    init = np.array([0x11000000FFFFFFFF], np.uint64)
+    win = np.array([0x7000000000000000], np.uint64)
+    sa = cg.SArr(True, init.ctypes.data_as(cg.U64P), 1)
+    L.generate(4, 260, f"{d}/b/%d.boards".encode(), C.byref(sa))
+    wa = cg.SArr(True, win.ctypes.data_as(cg.U64P), 1)
+    L.solve(260, 0, f"{d}/b/%d.boards".encode(), f"{d}/t/%d.tables".encode(), C.byref(wa))
+    rows = json.load(open(os.path.join(HERE, "golden", "c1_ref.json")))["variants"]["ignore_f"]
+    for s, n, hk, hv in rows:
+        b = O.read_boards_file(f"{d}/b/{s}.boards")
+        k, v = O.read_table_file(f"{d}/t/{s}.tables")
+        assert len(b) == n and "%016x" % O.hash_u64(b) == hk, s
+        assert np.array_equal(k, b)
+        assert "%016x" % O.hash_u64(v.view(np.uint64)) == hv, s
+    # layers below the start are solved as empty tables (missing files)
+    assert os.path.getsize(f"{d}/t/2.tables") == 0
+
+
+# ---------------------------------------------------------------- adjoint
+
+def test_adjoint_vs_oracle_and_fd(cg):
+    rules, initial, win, start, end = TB_CASES["c1_ignore_f"]
+    cfg = O.make_cfg(cores=8, **rules)
+    layers, vals = oracle_table(cfg, initial, win, start, end)
+    Po, dPo, Do = O.adjoint(cfg, layers, start, vals, win, initial)
+    tb = cg.Tablebase(cg.rules(**rules))
+    tb.generate(initial, start, end)
+    tb.solve(win, keep_argmax=True)
+    P, dP, D = tb.adjoint(initial)
+    assert P == Po
+    assert abs(dP - dPo) <= 1e-12 * abs(dPo)
+    assert np.allclose(D, Do, rtol=1e-10, atol=1e-15)
+
+    def p_at(p4):
+        tb.solve(win, p4=p4, keep_argmax=True)
+        return tb.adjoint(initial)[0]
+
+    h = 1e-6
+    fd = (p_at(0.1 + h) - p_at(0.1 - h)) / (2 * h)
+    assert abs(fd - dP) <= 1e-6 * abs(dP)
+    tb.close()
+
+
+def test_adjoint_autograd(cg):
+    from cablegen_amd.autograd import win_probability
+    rules, initial, win, start, end = TB_CASES["c1_ignore_f"]
+    tb = cg.Tablebase(cg.rules(**rules))
+    tb.generate(initial, start, 150)
+    p4 = torch.tensor(0.1, dtype=torch.float64, requires_grad=True)
+    P = win_probability(p4, tb, initial, win)
+    P.backward()
+    tb.solve(win, keep_argmax=True)
+    _, dP, _ = tb.adjoint(initial)
+    # the occupation push uses FP64 atomics, so two sweeps may differ in the last ulps
+    assert abs(p4.grad.item() - dP) <= 1e-12 * abs(dP)
+    assert torch.autograd.gradcheck(lambda x: win_probability(x, tb, initial, win),
+                                    (torch.tensor(0.1, dtype=torch.float64, requires_grad=True),),
+                                    eps=1e-6, atol=1e-8, rtol=1e-6)
+    tb.close()
+
+
+# ---------------------------------------------------------------- full size C2
+
+C2 = dict(rules=dict(ignore_f=True, prune=True, stsl=58, ltc=5, smallest_large=6),
+          initial=[0x00008765FFF3FFF2], win=[0x0000900000000000], end=1100)
+
+
+@pytest.fixture(scope="module")
+def c2_table(cg):
+    tb = cg.Tablebase(cg.rules(**C2["rules"]))
+    tb.generate(C2["initial"], 492, C2["end"])
+    tb.solve(C2["win"], keep_argmax=True)
+    yield tb
+    tb.close()
+
+
+def test_c2_properties(cg, c2_table):
+    """Size-independent properties at full size: every layer sorted, unique,
+    canonical prefix constant, tile sums match; values in [0, 1]."""
+    tb = c2_table
+    assert tb.num_layers == 305
+    for i in range(0, tb.num_layers, 7):
+        k = tb.keys(i)
+        if len(k) == 0:
+            continue
+        assert np.all(k[1:] > k[:-1])
+        assert np.all((k & np.uint64(0xFFF0FFF000000000)) == np.uint64(0xFFF0FFF000000000))
+        s = 492 + 2 * i
+        for x in k[:: max(1, len(k) // 200)]:
+            assert O.orc().orc_get_sum(int(x)) == s
+            assert O.canon(int(x)) == int(x)
+        v = tb.values(i)
+        assert np.all((v >= 0) & (v <= 1))
+
+
+def test_c2_sampled_layers_vs_oracle(cg, c2_table):
+    """Full-size layers: one oracle generation step and solve of mid-table
+    layers, from the GPU's own neighbouring layers, must match bit for bit."""
+    tb = c2_table
+    cfg = O.make_cfg(cores=16, **C2["rules"])
+    W = O.expand_winstates(C2["win"])
+    for i in (60, 150):
+        s2a, s4a = O.spawn_sets(cfg, tb.keys(i - 1))
+        _, s4b = O.spawn_sets(cfg, tb.keys(i - 2))
+        P = np.union1d(s2a, s4b).astype(np.uint64)
+        M, P2, C2_ = O.gen_layer(cfg, P, s4a)
+        assert np.array_equal(M, tb.keys(i))
+        sel = np.arange(0, len(M), 13)
+        V = O.solve_layer(cfg, M[sel], tb.keys(i + 1), tb.values(i + 1), tb.keys(i + 2),
+                          tb.values(i + 2), W)
+        assert np.array_equal(V.view(np.uint64), tb.values(i)[sel].view(np.uint64))
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(HERE, "golden", "c2_oracle.json")),
+                    reason="C2 manifest not generated")
+def test_c2_manifest(cg, c2_table):
+    """Every layer and every value of the full DPDGap-512 table against the
+    oracle's manifest (keys and value bits)."""
+    tb = c2_table
+    rows = json.load(open(os.path.join(HERE, "golden", "c2_oracle.json")))["layers"]
+    assert len(rows) == tb.num_layers
+    for i, (s, n, hk, hv) in enumerate(rows):
+        assert tb.layer_size(i) == n, s
+        assert "%016x" % O.hash_u64(tb.keys(i)) == hk, s
+        assert "%016x" % O.hash_u64(tb.values(i).view(np.uint64)) == hv, s
+
+
+def test_c2_adjoint_fd(cg, c2_table):
+    """dP/dp4 on the DPDGap-512 table against central differences of FP64
+    re-solves at p4 = 0.1 +- 1e-5."""
+    tb = c2_table
+    P, dP, D = tb.adjoint(C2["initial"])
+    assert np.isclose(D.sum(), dP, rtol=1e-12)
+
+    def p_at(p4):
+        tb.solve(C2["win"], p4=p4, keep_argmax=True)
+        return tb.adjoint(C2["initial"])[0]
+
+    h = 1e-5
+    fd = (p_at(0.1 + h) - p_at(0.1 - h)) / (2 * h)
+    tb.solve(C2["win"], keep_argmax=True)
+    assert abs(fd - dP) <= 1e-6 * abs(dP), (fd, dP)
+
+
+CLI = os.path.join(os.path.dirname(HERE), "oracle", "_ref", "cablegen_cli_mi355x")
+
+
+@pytest.mark.skipif(not os.path.exists(CLI), reason="reference CLI not built against the library")
+def test_reference_cli_on_library(cg, tmp_path):
+    """The reference's own CLI (frontends/cli/main.c, unchanged) linked to
+    libcablegen.so: `generate` and `solve` driven by a config file must write
+    the reference's layers and tables."""
+    import subprocess
+    d = str(tmp_path)
+    os.makedirs(d + "/b")
+    os.makedirs(d + "/t")
+    np.array([0x11000000FFFFFFFF], np.uint64).tofile(d + "/initial")
+    np.array([0x7000000000000000], np.uint64).tofile(d + "/winstates")
+    with open(d + "/c.conf", "w") as f:
+        f.write(f"[Cablegen]\nignore_f = true\ncores = 2\n[Generate]\ndir = {d}/b/\n"
+                f"initial = {d}/initial\nend = 260\n[Solve]\ndir = {d}/t/\n"
+                f"winstates = {d}/winstates\nend = 0\n")
+    for cmd in ("generate", "solve"):
+        r = subprocess.run([CLI, f"-C={d}/c.conf", cmd], cwd=d, capture_output=True, text=True,
+                           timeout=120)
+        assert r.returncode == 0, r.stdout + r.stderr
+    rows = json.load(open(os.path.join(HERE, "golden", "c1_ref.json")))["variants"]["ignore_f"]
+    for s, n, hk, hv in rows:
+        k, v = O.read_table_file(f"{d}/t/{s}.tables")
+        assert len(k) == n and "%016x" % O.hash_u64(k) == hk, s
+        assert "%016x" % O.hash_u64(v.view(np.uint64)) == hv, s

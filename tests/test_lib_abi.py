@@ -1,0 +1,164 @@
+"""CPU tests of the drop-in library: it loads, exports every function that
+include/*.h declares, keeps the reference's struct layouts, and its host
+utilities (row tables, movedir, lookup, settings, files) agree with the
+oracle.  No kernel is launched here."""
+import ctypes as C
+import os
+import random
+import re
+
+import numpy as np
+import pytest
+
+from tests import oracle_lib as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.fixture(scope="module")
+def cg():
+    import cablegen_amd
+    cablegen_amd.lib()
+    return cablegen_amd
+
+
+def declared_functions(path):
+    txt = open(path).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    txt = re.sub(r"#define[^\n]*", "", txt)
+    names = set()
+    for m in re.finditer(r"([A-Za-z_][A-Za-z0-9_]*)\s*\(([^;{}]*)\)\s*;", txt):
+        if m.group(1) not in ("if", "while", "sizeof"):
+            names.add(m.group(1))
+    return names
+
+
+@pytest.mark.parametrize("header", ["cablegen.h", "cablegen_gpu.h"])
+def test_exports_every_declared_symbol(cg, header):
+    names = declared_functions(os.path.join(ROOT, "include", header))
+    assert len(names) > 10
+    L = cg.lib()
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_reference_api_names_present(cg):
+    """Every function of api/cablegen.h:41-79 is declared in include/cablegen.h."""
+    want = {"movedir", "generate_lut", "read_boards", "read_boards2", "change_config", "lookup",
+            "get_settings_min", "init_settings", "generate", "solve", "malloc_errcheck",
+            "init_darr", "init_sarr", "shrink_darr", "concat", "destroy_darr", "destroy_sarr",
+            "push_back", "write_boards", "read_table", "pun_uint64", "test", "get_sum",
+            "output_board", "get_version"}
+    assert want <= declared_functions(os.path.join(ROOT, "include", "cablegen.h"))
+
+
+def test_struct_layouts(cg):
+    assert C.sizeof(O.DArr) == 32 and C.sizeof(O.SArr) == 24 and C.sizeof(O.Table) == 48
+    assert C.sizeof(cg.MinSettings) == 80
+
+
+def test_self_test_and_version(cg):
+    assert cg.lib().test()
+    assert cg.lib().get_version()
+
+
+def test_row_tables_match_oracle(cg):
+    for ig in (False, True):
+        mv, _ = O.lut(False, ig)
+        assert np.array_equal(cg.row_table(ig), mv[0])
+
+
+def test_host_movedir_and_sum(cg, tmp_path):
+    L = cg.lib()
+    rng = random.Random(2)
+    for ig in (False, True):
+        with open(tmp_path / "c.conf", "w") as f:
+            f.write(f"[Cablegen]\nignore_f = {'true' if ig else 'false'}\n")
+        L.change_config(str(tmp_path / "c.conf").encode())
+        assert L.get_settings_min().contents.ignore_f == ig
+        cfg = O.make_cfg(ignore_f=ig)
+        for _ in range(3000):
+            fm = rng.choice([0x00000000FFFFFFFF, 0x00000000FFF0FFF0, 0])
+            b = 0
+            for x in range(16):
+                sh = 60 - 4 * x
+                b |= (0xf if (fm >> sh) & 0xf else (rng.randint(1, 9) if rng.random() > 0.4 else 0)) << sh
+            assert L.get_sum(b) == O.orc().orc_get_sum(b)
+            for d in range(4):
+                x = C.c_uint64(b)
+                ok = L.movedir(C.byref(x), d)
+                ok_o, y = O.move(cfg, b, d)
+                assert ok == ok_o
+                if ok:
+                    # movedir rotates up/down back; the oracle's stays rotated
+                    assert O.canon(x.value) == O.canon(y)
+                    if d < 2:
+                        assert x.value == y
+
+
+def test_host_lookup_known_answer(cg):
+    L = cg.lib()
+    k = np.arange(100, dtype=np.uint64)
+    v = (99 - np.arange(100)).astype(np.float64)
+    t = cg.Table(cg.SArr(True, k.ctypes.data_as(cg.U64P), 100),
+                 cg.SArr(True, v.view(np.uint64).ctypes.data_as(cg.U64P), 100))
+    for i in range(100):
+        assert L.lookup(i, C.byref(t), False) == 99 - i
+    assert L.lookup(100, C.byref(t), False) == 0.0
+    e = cg.Table(cg.SArr(True, None, 0), cg.SArr(True, None, 0))
+    assert L.lookup(5, C.byref(e), True) == 0.0
+
+
+def test_settings_defaults_and_keys(cg, tmp_path):
+    L = cg.lib()
+    with open(tmp_path / "c.conf", "w") as f:
+        f.write("; comment\n[CABLEGEN]\nCores = 7\nnox = 9\nfree_formation = TRUE\n"
+                "[Generate]\nend = 900\ndir = /x/b/\n[Solve]\nend = 12\nwinstates = ./w\n")
+    L.change_config(str(tmp_path / "c.conf").encode())
+    s = L.get_settings_min().contents
+    assert s.cores == 7 and s.nox == 9 and s.free_formation and s.end_gen == 900
+    assert s.bdir == b"/x/b/" and s.end_solve == 12 and s.winstates == b"./w"
+    assert s.tdir == b"./tables/" and s.initial == b"./initial"   # defaults kept
+    r = cg.Rules()
+    L.cg_rules_from_settings(C.byref(r))
+    assert r.free_formation == 1 and r.nox == 9 and r.stsl == 200 and r.smallest_large == 6
+
+
+def test_files_round_trip(cg, tmp_path):
+    L = cg.lib()
+    a = np.array([3, 1, 4, 1, 5, 9, 2, 6], dtype=np.uint64)
+    sa = cg.SArr(True, a.ctypes.data_as(cg.U64P), len(a))
+    L.write_boards.argtypes = [cg.SArr, C.c_char_p, C.c_int]
+    L.write_boards(sa, str(tmp_path / "%d.boards").encode(), 42)
+    L.read_boards.restype = cg.SArr
+    L.read_boards.argtypes = [C.c_char_p]
+    r = L.read_boards(str(tmp_path / "42.boards").encode())
+    assert r.valid and r.size == 8
+    assert np.array_equal(np.ctypeslib.as_array(r.bp, shape=(8,)), a)
+    r = L.read_boards(str(tmp_path / "missing.boards").encode())
+    assert not r.valid and r.size == 0
+    k = np.arange(5, dtype=np.uint64)
+    v = np.linspace(0, 1, 5)
+    with open(tmp_path / "t.tables", "wb") as f:
+        f.write(k.tobytes() + v.tobytes())
+    t = cg.Table()
+    L.read_table.argtypes = [C.POINTER(cg.Table), C.c_char_p]
+    L.read_table(C.byref(t), str(tmp_path / "t.tables").encode())
+    assert t.key.size == 5
+    assert np.array_equal(np.ctypeslib.as_array(t.value.bp, shape=(5,)).view(np.float64), v)
+
+
+def test_arrays(cg):
+    L = cg.lib()
+    L.init_darr.restype = O.DArr
+    L.init_darr.argtypes = [C.c_bool, C.c_size_t]
+    L.push_back.restype = C.c_bool
+    L.push_back.argtypes = [C.POINTER(O.DArr), C.c_uint64]
+    L.shrink_darr.restype = O.SArr
+    L.shrink_darr.argtypes = [C.POINTER(O.DArr)]
+    d = L.init_darr(False, 0)
+    for i in range(1000):
+        L.push_back(C.byref(d), i)
+    s = L.shrink_darr(C.byref(d))
+    assert s.valid and s.size == 1000
+    assert np.array_equal(np.ctypeslib.as_array(s.bp, shape=(1000,)), np.arange(1000))

@@ -453,3 +453,14 @@ def test_adjoint_vs_finite_differences():
     fd = (p_at(0.1 + h) - p_at(0.1 - h)) / (2 * h)
     assert abs(fd - dP) <= 1e-6 * abs(dP), (fd, dP)
     assert p_at(0.1) == P
+
+
+def test_spawn_sets_consistent_with_gen_layer():
+    cfg = O.make_cfg(cores=3, ignore_f=True, prune=True, stsl=58, ltc=5, smallest_large=6)
+    P, Cc = np.array([0x00008765FFF3FFF2], np.uint64), np.zeros(0, np.uint64)
+    for _ in range(12):
+        M, P2, C2 = O.gen_layer(cfg, P, Cc)
+        S2, S4 = O.spawn_sets(cfg, M)
+        assert np.array_equal(P2, np.union1d(S2, Cc))
+        assert np.array_equal(C2, S4)
+        P, Cc = P2, C2
