@@ -36,7 +36,7 @@ WINSTATES = [0x0000900000000000]
 END_GEN = 1100
 RULES = dict(ignore_f=True, prune=True, stsl=58, ltc=5, smallest_large=6)
 HBM_PEAK_GBS = 8000.0
-KERNELS = ["move", "spawn", "sort_hist", "sort_scatter", "unique", "solve"]
+KERNELS = ["move", "spawn", "sort_hist", "sort_pass", "unique", "solve"]
 
 
 def cpu_baseline(tb, cg, cores):

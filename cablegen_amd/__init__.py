@@ -89,6 +89,8 @@ def lib():
                                      C.c_void_p, C.c_void_p, C.c_size_t,
                                      C.c_void_p, C.c_void_p, C.c_size_t,
                                      U64P, C.c_size_t, C.c_double]
+    L.cg_tune_set.argtypes = [C.c_char_p, C.c_int]
+    L.cg_tune_get.argtypes = [C.c_char_p]
     L.cg_prof_enable.argtypes = [C.c_int]
     L.cg_prof_query.restype = C.c_long
     L.cg_prof_query.argtypes = [C.c_char_p, F64P, F64P]
@@ -217,6 +219,14 @@ class Tablebase:
     def write_files(self, boards_fmt=None, tables_fmt=None):
         _check(self.L.cg_tb_write_files(self.h, boards_fmt.encode() if boards_fmt else None,
                                         tables_fmt.encode() if tables_fmt else None), "write")
+
+
+def tune_set(knob, value):
+    _check(lib().cg_tune_set(knob.encode(), int(value)), "cg_tune_set")
+
+
+def tune_get(knob):
+    return lib().cg_tune_get(knob.encode())
 
 
 def prof_enable(on=True):

@@ -77,6 +77,7 @@ struct Engine {
     uint16_t *lut[2] = { nullptr, nullptr };     /* left-move table per ignore_f */
     unsigned long long *d_cnt = nullptr;          /* device counters [8] */
     unsigned long long *h_cnt = nullptr;          /* pinned mirror */
+    unsigned *d_err = nullptr, *h_err = nullptr;  /* sort wait-budget flag */
     Buf A, B, Cb;                                 /* candidate / sort buffers */
     void *scratch = nullptr;
     size_t scratch_cap = 0;
@@ -91,6 +92,10 @@ static int eng_init()
     HC(hipStreamCreate(&E.st));
     HC(hipMalloc((void **)&E.d_cnt, 8 * sizeof(unsigned long long)));
     HC(hipHostMalloc((void **)&E.h_cnt, 8 * sizeof(unsigned long long), hipHostMallocDefault));
+    HC(hipMalloc((void **)&E.d_err, 64));
+    HC(hipHostMalloc((void **)&E.h_err, 64, hipHostMallocDefault));
+    HC(hipMemset(E.d_err, 0, 64));
+    *E.h_err = 0;
     E.init = true;
     return 0;
 }
@@ -136,9 +141,9 @@ static const uint16_t *get_lut(int ignore_f)
 /* kernel timers                                                             */
 /* ------------------------------------------------------------------------ */
 
-enum { PK_MOVE, PK_SPAWN, PK_SORT_HIST, PK_SORT_SCATTER, PK_UNIQUE, PK_SOLVE, PK_ADJOINT, PK_LOOKUP,
+enum { PK_MOVE, PK_SPAWN, PK_SORT_HIST, PK_SORT_PASS, PK_UNIQUE, PK_SOLVE, PK_ADJOINT, PK_LOOKUP,
        PK_N };
-static const char *PK_NAMES[PK_N] = { "move", "spawn", "sort_hist", "sort_scatter",
+static const char *PK_NAMES[PK_N] = { "move", "spawn", "sort_hist", "sort_pass",
                                       "unique", "solve", "adjoint", "lookup" };
 
 struct ProfRec {
@@ -270,6 +275,16 @@ struct Arena {
         used += bytes;
         return r;
     }
+    /* gives back the tail of the most recent allocation */
+    void shrink_last(void *p, size_t old_bytes, size_t new_bytes)
+    {
+        old_bytes = (old_bytes + 255) & ~(size_t)255;
+        new_bytes = (new_bytes + 255) & ~(size_t)255;
+        if (old_bytes == 0) old_bytes = 256;
+        if (new_bytes == 0) new_bytes = 256;
+        if (idx < chunks.size() && (char *)p + old_bytes == chunks[idx].p + used && new_bytes <= old_bytes)
+            used -= old_bytes - new_bytes;
+    }
     /* rewinds without freeing: the chunks are reused by the next run */
     void reset()
     {
@@ -299,80 +314,112 @@ static MoveRules move_rules(const cg_rules &r)
     return m;
 }
 
-static int read_counter(size_t *out)
+/* copies device counters [lo, lo+n) to E.h_cnt, synchronises and checks the
+ * sort error flag */
+static int read_counters(int lo, int n)
 {
-    HC(hipMemcpyAsync(E.h_cnt, E.d_cnt, sizeof(unsigned long long), hipMemcpyDeviceToHost, E.st));
-    if (check_sync(E.st, "counter")) return -1;
-    *out = (size_t)E.h_cnt[0];
+    HC(hipMemcpyAsync(E.h_cnt + lo, E.d_cnt + lo, n * sizeof(unsigned long long),
+                      hipMemcpyDeviceToHost, E.st));
+    HC(hipMemcpyAsync(E.h_err, E.d_err, sizeof(unsigned), hipMemcpyDeviceToHost, E.st));
+    if (check_sync(E.st, "counters")) return -1;
+    if (*E.h_err) {
+        *E.h_err = 0;
+        hipMemsetAsync(E.d_err, 0, sizeof(unsigned), E.st);
+        hipStreamSynchronize(E.st);
+        return fail_msg("sort: a chained-scan wait exceeded its budget");
+    }
     return 0;
 }
 
-/* sorts and dedupes buf[0..n) (clobbering it and E.B); returns the count and
- * leaves a SortState whose finish the caller enqueues */
-static int sort_begin(uint64_t *buf, size_t n, SortState *ss)
+static uint64_t vary_of(size_t n, uint64_t o, uint64_t a) { return n > 1 ? (o & ~a) : 0; }
+
+/* enqueues sort + unique of buf[0..n) into dst; count lands in d_cnt[slot] */
+static int sort_enqueue(uint64_t *buf, size_t n, uint64_t vary, uint64_t *dst, int slot, int *hu)
 {
-    if (ensure(E.B, n)) return -1;
+    if (ensure(E.B, n + 1)) return -1;
     if (ensure_scratch(n)) return -1;
-    *ss = sort_unique_begin(E.st, buf, E.B.p, n, E.scratch);
+    sort_unique(E.st, buf, E.B.p, n, vary, E.scratch, dst, E.d_cnt + slot, E.d_err, hu);
     hipError_t e = kernels_take_error();
     if (e != hipSuccess) return fail("sort_unique", e);
     return 0;
 }
 
-typedef uint64_t *(*AllocFn)(size_t n, void *ctx);
+struct LayerAlloc {
+    /* returns room for n keys; shrink() gives back the unused tail */
+    virtual uint64_t *alloc(size_t n) = 0;
+    virtual void shrink(uint64_t *p, size_t cap, size_t n) { (void)p, (void)cap, (void)n; }
+    virtual ~LayerAlloc() {}
+};
+
+/* OR / AND of the carried spawn-4 set (same set, same statistics as S4) */
+struct Carry {
+    uint64_t o = 0, a = ~0ULL;
+};
+
+static int nonzero_nibbles(uint64_t x)
+{
+    int c = 0;
+    for (int i = 0; i < 16; i++) c += ((x >> (4 * i)) & 0xf) != 0;
+    return c;
+}
 
 /* One layer step (generate.c:409-550): P (pre-move layer) and the carried
- * spawn-4 set C give M_s (written to alloc_m), P_{s+2} and the new carry
- * (written to nxtP / nxtC). */
+ * spawn-4 set C give M_s (from `la`), P_{s+2} and the new carry (written to
+ * nxtP / nxtC).  Four host synchronisations: after the moves, after M_s,
+ * after the spawns, after P_{s+2} and C'. */
 static int gen_step(const cg_rules &r, const uint64_t *P, size_t nP, const uint64_t *C, size_t nC,
-                    AllocFn alloc_m, void *mctx, uint64_t **M, size_t *nM, Buf &nxtP, size_t *nP2,
+                    Carry &carry, LayerAlloc &la, uint64_t **M, size_t *nM, Buf &nxtP, size_t *nP2,
                     Buf &nxtC, size_t *nC2)
 {
     const uint16_t *lut = get_lut(r.ignore_f);
     if (!lut) return fail_msg("row table upload failed");
-    /* move */
+    /* moves */
     if (ensure(E.A, 4 * nP + 1)) return -1;
-    HC(hipMemsetAsync(E.d_cnt, 0, sizeof(unsigned long long), E.st));
+    launch_init_stats(E.st, E.d_cnt);
     int h = prof_begin(PK_MOVE);
     launch_move(E.st, P, nP, E.A.p, E.d_cnt, lut, move_rules(r));
     prof_end(h, 0);
-    size_t ncand = 0;
-    if (read_counter(&ncand)) return -1;
+    if (read_counters(0, 3)) return -1;
+    size_t ncand = E.h_cnt[0];
+    uint64_t mor = E.h_cnt[1], mand = E.h_cnt[2];
     prof_set_bytes(h, 8.0 * nP + 8.0 * ncand);
-    SortState ss;
-    if (sort_begin(E.A.p, ncand, &ss)) return -1;
-    *nM = ss.n_unique;
-    *M = alloc_m(ss.n_unique, mctx);
+    /* M_s = uniq(candidates) */
+    *M = la.alloc(ncand);
     if (!*M) return fail_msg("layer allocation failed");
-    int hu = prof_begin(PK_UNIQUE);
-    sort_unique_finish(E.st, ss, *M);
-    prof_end(hu, 8.0 * ncand + 8.0 * ss.n_unique);
-    /* spawn */
-    HC(hipMemsetAsync(E.d_cnt, 0, sizeof(unsigned long long), E.st));
-    launch_count_empties(E.st, *M, *nM, E.d_cnt);
-    size_t sumE = 0;
-    if (read_counter(&sumE)) return -1;
-    if (ensure(E.A, sumE + nC + 1) || ensure(E.Cb, sumE + 1)) return -1;
-    HC(hipMemsetAsync(E.d_cnt, 0, sizeof(unsigned long long), E.st));
+    int hu;
+    if (sort_enqueue(E.A.p, ncand, vary_of(ncand, mor, mand), *M, 5, &hu)) return -1;
+    if (read_counters(5, 1)) return -1;
+    *nM = E.h_cnt[5];
+    prof_set_bytes(hu, 8.0 * ncand + 8.0 * *nM);
+    la.shrink(*M, ncand, *nM);
+    /* spawns: every board has at most 16 - (cells nonzero in all of M_s) empties */
+    size_t emax = (size_t)(16 - nonzero_nibbles(*nM ? mand : ~0ULL));
+    size_t cap = emax * *nM;
+    if (ensure(E.A, nC + cap + 1) || ensure(E.Cb, cap + 1)) return -1;
+    if (nC) HC(hipMemcpyAsync(E.A.p, C, nC * sizeof(uint64_t), hipMemcpyDeviceToDevice, E.st));
+    launch_init_stats(E.st, E.d_cnt);
     int hs = prof_begin(PK_SPAWN);
-    launch_spawn(E.st, *M, *nM, E.A.p, E.Cb.p, E.d_cnt);
-    prof_end(hs, 8.0 * *nM + 16.0 * sumE);
-    if (nC) HC(hipMemcpyAsync(E.A.p + sumE, C, nC * sizeof(uint64_t), hipMemcpyDeviceToDevice, E.st));
-    /* P_{s+2} = uniq(S2 u C) */
-    if (sort_begin(E.A.p, sumE + nC, &ss)) return -1;
-    if (ensure(nxtP, ss.n_unique + 1)) return -1;
-    hu = prof_begin(PK_UNIQUE);
-    sort_unique_finish(E.st, ss, nxtP.p);
-    prof_end(hu, 8.0 * (sumE + nC) + 8.0 * ss.n_unique);
-    *nP2 = ss.n_unique;
-    /* C' = uniq(S4) */
-    if (sort_begin(E.Cb.p, sumE, &ss)) return -1;
-    if (ensure(nxtC, ss.n_unique + 1)) return -1;
-    hu = prof_begin(PK_UNIQUE);
-    sort_unique_finish(E.st, ss, nxtC.p);
-    prof_end(hu, 8.0 * sumE + 8.0 * ss.n_unique);
-    *nC2 = ss.n_unique;
-    return check_sync(E.st, "gen_step");
+    launch_spawn(E.st, *M, *nM, E.A.p + nC, E.Cb.p, E.d_cnt);
+    prof_end(hs, 0);
+    if (read_counters(0, 5)) return -1;
+    size_t sumE = E.h_cnt[0];
+    prof_set_bytes(hs, 8.0 * *nM + 16.0 * sumE);
+    uint64_t or2 = E.h_cnt[1], and2 = E.h_cnt[2], or4 = E.h_cnt[3], and4 = E.h_cnt[4];
+    /* P_{s+2} = uniq(C u S2), C' = uniq(S4) */
+    size_t n2 = nC + sumE;
+    if (ensure(nxtP, n2 + 1) || ensure(nxtC, sumE + 1)) return -1;
+    int hu2, hu4;
+    if (sort_enqueue(E.A.p, n2, vary_of(n2, or2 | carry.o, and2 & carry.a), nxtP.p, 5, &hu2))
+        return -1;
+    if (sort_enqueue(E.Cb.p, sumE, vary_of(sumE, or4, and4), nxtC.p, 6, &hu4)) return -1;
+    if (read_counters(5, 2)) return -1;
+    *nP2 = E.h_cnt[5];
+    *nC2 = E.h_cnt[6];
+    prof_set_bytes(hu2, 8.0 * n2 + 8.0 * *nP2);
+    prof_set_bytes(hu4, 8.0 * sumE + 8.0 * *nC2);
+    carry.o = or4;
+    carry.a = and4;
+    return 0;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -459,10 +506,18 @@ static LayerView view_of(const cg_tablebase *tb, int idx)
     return v;
 }
 
-static uint64_t *arena_alloc_u64(size_t n, void *ctx)
-{
-    return (uint64_t *)((Arena *)ctx)->alloc(std::max<size_t>(n, 1) * sizeof(uint64_t));
-}
+struct ArenaLayerAlloc : LayerAlloc {
+    Arena *a;
+    explicit ArenaLayerAlloc(Arena *x) : a(x) {}
+    uint64_t *alloc(size_t n) override
+    {
+        return (uint64_t *)a->alloc(std::max<size_t>(n, 1) * sizeof(uint64_t));
+    }
+    void shrink(uint64_t *p, size_t cap, size_t n) override
+    {
+        a->shrink_last(p, std::max<size_t>(cap, 1) * 8, std::max<size_t>(n, 1) * 8);
+    }
+};
 
 extern "C" cg_tablebase *cg_tb_create(const cg_rules *rules)
 {
@@ -503,11 +558,13 @@ extern "C" int cg_tb_generate(cg_tablebase *tb, const uint64_t *initial, size_t 
     if (ensure(tb->P, n + 1)) return -1;
     if (n) HC(hipMemcpy(tb->P.p, initial, n * 8, hipMemcpyHostToDevice));
     size_t nP = n, nC = 0;
+    Carry carry;
+    ArenaLayerAlloc la(&tb->arena);
     for (int s = start; s <= end; s += 2) {
         TbLayer l;
         size_t nP2 = 0, nC2 = 0;
-        if (gen_step(tb->r, tb->P.p, nP, tb->C.p, nC, arena_alloc_u64, &tb->arena, &l.keys, &l.n,
-                     tb->P2, &nP2, tb->C2, &nC2))
+        if (gen_step(tb->r, tb->P.p, nP, tb->C.p, nC, carry, la, &l.keys, &l.n, tb->P2, &nP2,
+                     tb->C2, &nC2))
             return -1;
         l.ns = split_count(l.n);
         if (l.ns) {
@@ -703,11 +760,12 @@ extern "C" int cg_dev_move(const cg_rules *rules, const uint64_t *d_in, size_t n
     if (eng_init()) return -1;
     const uint16_t *lut = get_lut(rules->ignore_f);
     if (!lut) return fail_msg("row table upload failed");
-    HC(hipMemsetAsync(E.d_cnt, 0, 8, E.st));
+    launch_init_stats(E.st, E.d_cnt);
     int h = prof_begin(PK_MOVE);
     launch_move(E.st, d_in, n, d_out, E.d_cnt, lut, move_rules(*rules));
     prof_end(h, 0);
-    if (read_counter(n_out)) return -1;
+    if (read_counters(0, 1)) return -1;
+    *n_out = E.h_cnt[0];
     prof_set_bytes(h, 8.0 * n + 8.0 * *n_out);
     return 0;
 }
@@ -716,11 +774,12 @@ extern "C" int cg_dev_spawn(const uint64_t *d_in, size_t n, uint64_t *d_out2, ui
                             size_t *n_out)
 {
     if (eng_init()) return -1;
-    HC(hipMemsetAsync(E.d_cnt, 0, 8, E.st));
+    launch_init_stats(E.st, E.d_cnt);
     int h = prof_begin(PK_SPAWN);
     launch_spawn(E.st, d_in, n, d_out2, d_out4, E.d_cnt);
     prof_end(h, 0);
-    if (read_counter(n_out)) return -1;
+    if (read_counters(0, 1)) return -1;
+    *n_out = E.h_cnt[0];
     prof_set_bytes(h, 8.0 * n + 16.0 * *n_out);
     return 0;
 }
@@ -728,13 +787,15 @@ extern "C" int cg_dev_spawn(const uint64_t *d_in, size_t n, uint64_t *d_out2, ui
 extern "C" int cg_dev_sort_unique(uint64_t *d_keys, size_t n, uint64_t *d_out, size_t *n_unique)
 {
     if (eng_init()) return -1;
-    SortState ss;
-    if (sort_begin(d_keys, n, &ss)) return -1;
-    int hu = prof_begin(PK_UNIQUE);
-    sort_unique_finish(E.st, ss, d_out);
-    prof_end(hu, 8.0 * n + 8.0 * ss.n_unique);
-    *n_unique = ss.n_unique;
-    return check_sync(E.st, "cg_dev_sort_unique");
+    launch_init_stats(E.st, E.d_cnt);
+    launch_or_and(E.st, d_keys, n, E.d_cnt);
+    if (read_counters(0, 3)) return -1;
+    int hu;
+    if (sort_enqueue(d_keys, n, vary_of(n, E.h_cnt[1], E.h_cnt[2]), d_out, 5, &hu)) return -1;
+    if (read_counters(5, 1)) return -1;
+    *n_unique = E.h_cnt[5];
+    prof_set_bytes(hu, 8.0 * n + 8.0 * *n_unique);
+    return 0;
 }
 
 extern "C" int cg_dev_lookup(const uint64_t *d_keys, const double *d_vals, size_t n, const uint64_t *d_q,
@@ -778,6 +839,26 @@ extern "C" int cg_dev_solve_layer(const cg_rules *rules, const uint64_t *d_M, si
     return rc;
 }
 
+extern "C" int cg_tune_set(const char *knob, int value)
+{
+    if (!strcmp(knob, "sort_kpt")) {
+        set_sort_kpt(value);
+        return 0;
+    }
+    if (!strcmp(knob, "sort_lbw")) {
+        set_sort_lbw(value);
+        return 0;
+    }
+    return fail_msg("unknown tuning knob");
+}
+
+extern "C" int cg_tune_get(const char *knob)
+{
+    if (!strcmp(knob, "sort_kpt")) return get_sort_kpt();
+    if (!strcmp(knob, "sort_lbw")) return get_sort_lbw();
+    return -1;
+}
+
 extern "C" int cg_dev_sync(void)
 {
     if (eng_init()) return -1;
@@ -806,13 +887,16 @@ static struct {
     cg_rules r;
     Buf P, C, P2, C2, M;
     size_t nP = 0, nC = 0, nM = 0;
+    Carry carry;
 } G;
 
-static uint64_t *gbuf_alloc(size_t n, void *)
-{
-    if (ensure(G.M, n + 1)) return nullptr;
-    return G.M.p;
-}
+struct BufLayerAlloc : LayerAlloc {
+    uint64_t *alloc(size_t n) override
+    {
+        if (ensure(G.M, n + 1)) return nullptr;
+        return G.M.p;
+    }
+};
 
 extern "C" int cge_gen_begin(const cge_rules *r, const uint64_t *initial, size_t n)
 {
@@ -823,6 +907,7 @@ extern "C" int cge_gen_begin(const cge_rules *r, const uint64_t *initial, size_t
     G.nP = n;
     G.nC = 0;
     G.nM = 0;
+    G.carry = Carry();
     return 0;
 }
 
@@ -830,8 +915,8 @@ extern "C" int cge_gen_step(size_t *n_layer)
 {
     uint64_t *M = nullptr;
     size_t nP2 = 0, nC2 = 0;
-    if (gen_step(G.r, G.P.p, G.nP, G.C.p, G.nC, gbuf_alloc, nullptr, &M, &G.nM, G.P2, &nP2, G.C2,
-                 &nC2))
+    BufLayerAlloc la;
+    if (gen_step(G.r, G.P.p, G.nP, G.C.p, G.nC, G.carry, la, &M, &G.nM, G.P2, &nP2, G.C2, &nC2))
         return -1;
     std::swap(G.P, G.P2);
     std::swap(G.C, G.C2);

@@ -37,7 +37,7 @@ struct LayerView {
 };
 
 /* kernel-timer brackets provided by the engine (kind: 2 = radix histogram,
- * 3 = radix scatter); a negative handle means timing is off */
+ * 3 = radix pass, 4 = unique); a negative handle means timing is off */
 int prof_hook_begin(int kind);
 void prof_hook_end(int h, double bytes);
 
@@ -48,32 +48,56 @@ hipError_t kernels_take_error();
 size_t sort_scratch_bytes(size_t n);
 
 /* move + lock + prune + canonicalise; appends valid canonical successors to
- * out (capacity 4n) and adds their number to *count (device counter). */
+ * out (capacity 4n), adds their number to stats[0] and ORs / ANDs them into
+ * stats[1] / stats[2]. */
 void launch_move(hipStream_t st, const uint64_t *in, size_t n, uint64_t *out,
-                 unsigned long long *count, const uint16_t *lut, MoveRules r);
+                 unsigned long long *stats, const uint16_t *lut, MoveRules r);
 
 /* sum of empty cells over a layer into *count (device counter) */
 void launch_count_empties(hipStream_t st, const uint64_t *in, size_t n, unsigned long long *count);
 
 /* spawn expansion: for each board and empty cell x (ascending), canon(b[x<-1])
- * into out2 and canon(b[x<-2]) into out4 at the same positions; *count
- * receives the number written. */
+ * into out2 and canon(b[x<-2]) into out4 at the same positions; stats[0]
+ * receives the number written, stats[1..2] / stats[3..4] the OR / AND of the
+ * out2 / out4 keys. */
 void launch_spawn(hipStream_t st, const uint64_t *in, size_t n, uint64_t *out2, uint64_t *out4,
-                  unsigned long long *count);
+                  unsigned long long *stats);
 
-/* Sort + unique in two steps.  sort_unique_begin sorts keys[0..n) ascending
- * (LSD radix over the varying bits only, ping-ponging with tmp), counts the
- * unique keys and synchronises st; sort_unique_finish writes the unique keys
- * to dst (capacity >= n_unique).  `scratch` must hold sort_scratch_bytes(n)
- * and must not be reused before finish has been enqueued. */
-struct SortState {
-    const uint64_t *sorted;
-    size_t n;
-    size_t n_unique;
-    const uint64_t *chunk_off;
+/* Order-preserving packing of the varying key bits: up to 4 runs of bits,
+ * most significant first (more fragmented masks use one contiguous range). */
+struct KeyXform {
+    int nseg;
+    int nbits;
+    int lo[4];
+    int len[4];
+    uint64_t mask[4];
 };
-SortState sort_unique_begin(hipStream_t st, uint64_t *keys, uint64_t *tmp, size_t n, void *scratch);
-void sort_unique_finish(hipStream_t st, const SortState &ss, uint64_t *dst);
+KeyXform make_xform(uint64_t vary);
+
+/* Sorts keys[0..n) ascending and writes the unique keys to dst (capacity n),
+ * entirely on the stream (no host synchronisation): onesweep LSD radix over
+ * the bits set in `vary` (OR & ~AND of the keys; bits outside it must be
+ * constant), then a chained-scan compaction.  The unique count lands in
+ * *d_nuniq.  tmp holds n keys; scratch holds sort_scratch_bytes(n).  A
+ * chained-scan wait that exceeds its budget sets *d_err (never cleared
+ * here); the caller checks it after synchronising.  keys and tmp are
+ * clobbered.  *prof_unique receives the timer handle of the compaction. */
+uint64_t *sort_unique(hipStream_t st, uint64_t *keys, uint64_t *tmp, size_t n, uint64_t vary,
+                      void *scratch, uint64_t *dst, unsigned long long *d_nuniq, unsigned *d_err,
+                      int *prof_unique);
+
+/* tuning: keys per thread of the sort kernels (16 or 32) */
+void set_sort_kpt(int kpt);
+int get_sort_kpt();
+/* tuning: look-back window of the radix passes (16, 32 or 64 tiles) */
+void set_sort_lbw(int w);
+int get_sort_lbw();
+
+void launch_fill_u64(hipStream_t st, uint64_t *p, size_t n, uint64_t v);
+/* stats[0] = 0 (count), stats[1,3] = 0 (OR), stats[2,4] = ~0 (AND) */
+void launch_init_stats(hipStream_t st, unsigned long long *stats);
+/* OR / AND of keys into stats[1] / stats[2] */
+void launch_or_and(hipStream_t st, const uint64_t *k, size_t n, unsigned long long *stats);
 
 /* builds ns splitters for a sorted layer */
 void launch_build_split(hipStream_t st, const uint64_t *keys, size_t n, uint64_t *split, int ns);

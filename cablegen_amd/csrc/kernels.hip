@@ -82,6 +82,28 @@ __device__ __forceinline__ T block_excl_scan(T v, T *total, T *wtmp)
     return r;
 }
 
+/* OR and AND of per-thread values over the block, one atomic pair per block */
+__device__ __forceinline__ void block_or_and(uint64_t o, uint64_t a, unsigned long long *dst_or,
+                                             unsigned long long *dst_and, uint64_t *ro, uint64_t *ra)
+{
+    __syncthreads();   /* ro/ra may still be read by a previous call */
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        o |= __shfl_xor(o, d, 64);
+        a &= __shfl_xor(a, d, 64);
+    }
+    int nw = (blockDim.x + 63) >> 6;
+    if (lane_id() == 0) ro[threadIdx.x >> 6] = o, ra[threadIdx.x >> 6] = a;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < nw; w++) o |= ro[w], a &= ra[w];
+        o |= ro[0];
+        a &= ra[0];
+        if (o) atomicOr(dst_or, (unsigned long long)o);
+        if (~a) atomicAnd(dst_and, (unsigned long long)a);
+    }
+}
+
 /* ------------------------------------------------------------------------ */
 /* move + lock + prune + canonicalise                                        */
 /* ------------------------------------------------------------------------ */
@@ -90,13 +112,15 @@ constexpr int MV_THREADS = 1024;
 
 __global__ __launch_bounds__(MV_THREADS) void k_move(const uint64_t *__restrict__ in, size_t n,
                                                      uint64_t *__restrict__ out,
-                                                     unsigned long long *count,
+                                                     unsigned long long *stats,
                                                      const uint16_t *__restrict__ glut,
                                                      MoveRules r)
 {
     __shared__ uint16_t lut[65536];
     __shared__ uint32_t wtmp[17];
     __shared__ unsigned long long base_sh;
+    __shared__ uint64_t ro[16], ra[16];
+    uint64_t kor = 0, kand = ~0ULL;
     {
         const uint4 *g = reinterpret_cast<const uint4 *>(glut);
         uint4 *l = reinterpret_cast<uint4 *>(lut);
@@ -121,23 +145,25 @@ __global__ __launch_bounds__(MV_THREADS) void k_move(const uint64_t *__restrict_
                 }
             }
         }
+        for (uint32_t j = 0; j < k; j++) kor |= res[j], kand &= res[j];
         uint32_t tot;
         uint32_t off = block_excl_scan<uint32_t>(k, &tot, wtmp);
-        if (threadIdx.x == 0) base_sh = tot ? atomicAdd(count, (unsigned long long)tot) : 0;
+        if (threadIdx.x == 0) base_sh = tot ? atomicAdd(&stats[0], (unsigned long long)tot) : 0;
         __syncthreads();
         unsigned long long base = base_sh;
         for (uint32_t j = 0; j < k; j++) out[base + off + j] = res[j];
         __syncthreads();
     }
+    block_or_and(kor, kand, &stats[1], &stats[2], ro, ra);
 }
 
 void launch_move(hipStream_t st, const uint64_t *in, size_t n, uint64_t *out,
-                 unsigned long long *count, const uint16_t *lut, MoveRules r)
+                 unsigned long long *stats, const uint16_t *lut, MoveRules r)
 {
     if (!n) return;
     size_t blocks = (n + MV_THREADS - 1) / MV_THREADS;
     if (blocks > 512) blocks = 512;
-    hipLaunchKernelGGL(k_move, dim3((unsigned)blocks), dim3(MV_THREADS), 0, st, in, n, out, count,
+    hipLaunchKernelGGL(k_move, dim3((unsigned)blocks), dim3(MV_THREADS), 0, st, in, n, out, stats,
                        lut, r);
 }
 
@@ -180,10 +206,12 @@ __device__ __forceinline__ int next_cell(uint64_t &z)
 __global__ __launch_bounds__(SP_THREADS) void k_spawn(const uint64_t *__restrict__ in, size_t n,
                                                       uint64_t *__restrict__ out2,
                                                       uint64_t *__restrict__ out4,
-                                                      unsigned long long *count)
+                                                      unsigned long long *stats)
 {
     __shared__ uint32_t wtmp[17];
     __shared__ unsigned long long base_sh;
+    __shared__ uint64_t ro[16], ra[16];
+    uint64_t or2 = 0, and2 = ~0ULL, or4 = 0, and4 = ~0ULL;
     const size_t stride = (size_t)gridDim.x * SP_THREADS;
     for (size_t b0 = (size_t)blockIdx.x * SP_THREADS; b0 < n; b0 += stride) {
         size_t i = b0 + threadIdx.x;
@@ -192,358 +220,570 @@ __global__ __launch_bounds__(SP_THREADS) void k_spawn(const uint64_t *__restrict
         uint32_t e = (uint32_t)cg_popc64(z);
         uint32_t tot;
         uint32_t off = block_excl_scan<uint32_t>(e, &tot, wtmp);
-        if (threadIdx.x == 0) base_sh = tot ? atomicAdd(count, (unsigned long long)tot) : 0;
+        if (threadIdx.x == 0) base_sh = tot ? atomicAdd(&stats[0], (unsigned long long)tot) : 0;
         __syncthreads();
         size_t o = base_sh + off;
         while (z) {
             int x = next_cell(z);
-            out2[o] = cg_canon(cg_set_tile(b, x, 1));
-            out4[o] = cg_canon(cg_set_tile(b, x, 2));
+            uint64_t c2 = cg_canon(cg_set_tile(b, x, 1)), c4 = cg_canon(cg_set_tile(b, x, 2));
+            out2[o] = c2;
+            out4[o] = c4;
+            or2 |= c2;
+            and2 &= c2;
+            or4 |= c4;
+            and4 &= c4;
             o++;
         }
         __syncthreads();
     }
+    block_or_and(or2, and2, &stats[1], &stats[2], ro, ra);
+    block_or_and(or4, and4, &stats[3], &stats[4], ro, ra);
 }
 
 void launch_spawn(hipStream_t st, const uint64_t *in, size_t n, uint64_t *out2, uint64_t *out4,
-                  unsigned long long *count)
+                  unsigned long long *stats)
 {
     if (!n) return;
     size_t blocks = (n + SP_THREADS - 1) / SP_THREADS;
-    if (blocks > 8192) blocks = 8192;
+    if (blocks > 1024) blocks = 1024;   /* bounds the OR/AND atomics per launch */
     hipLaunchKernelGGL(k_spawn, dim3((unsigned)blocks), dim3(SP_THREADS), 0, st, in, n, out2, out4,
-                       count);
+                       stats);
 }
 
 /* ------------------------------------------------------------------------ */
-/* device-wide exclusive scan (u32 -> u64), three kernels                    */
+/* small fills (graph-safe replacements for memset)                          */
 /* ------------------------------------------------------------------------ */
 
-constexpr int SC_THREADS = 1024, SC_IPT = 4, SC_TILE = SC_THREADS * SC_IPT;
-
-__global__ __launch_bounds__(SC_THREADS) void k_scan_reduce(const uint32_t *__restrict__ in,
-                                                            size_t m, uint64_t *sums)
+__global__ void k_fill_u64(uint64_t *p, size_t n, uint64_t v)
 {
-    __shared__ uint64_t wtmp[17];
-    size_t base = (size_t)blockIdx.x * SC_TILE;
-    uint64_t s = 0;
-#pragma unroll
-    for (int j = 0; j < SC_IPT; j++) {
-        size_t i = base + (size_t)j * SC_THREADS + threadIdx.x;
-        if (i < m) s += in[i];
-    }
-    uint64_t tot;
-    block_excl_scan<uint64_t>(s, &tot, wtmp);
-    if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        p[i] = v;
 }
 
-__global__ __launch_bounds__(SC_THREADS) void k_scan_top(uint64_t *sums, size_t nb, uint64_t *total)
+void launch_fill_u64(hipStream_t st, uint64_t *p, size_t n, uint64_t v)
 {
-    __shared__ uint64_t wtmp[17];
-    uint64_t carry = 0;
-    for (size_t c = 0; c < nb; c += SC_THREADS) {
-        size_t i = c + threadIdx.x;
-        uint64_t v = i < nb ? sums[i] : 0;
-        uint64_t tot;
-        uint64_t ex = block_excl_scan<uint64_t>(v, &tot, wtmp);
-        if (i < nb) sums[i] = carry + ex;
-        carry += tot;
-    }
-    if (threadIdx.x == 0 && total) *total = carry;
+    if (!n) return;
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 2048) blocks = 2048;
+    hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)blocks), dim3(256), 0, st, p, n, v);
 }
 
-__global__ __launch_bounds__(SC_THREADS) void k_scan_down(const uint32_t *__restrict__ in, size_t m,
-                                                          const uint64_t *sums, uint64_t *out)
+/* producer statistics: [0] count, then (OR, AND) pairs; AND starts all-ones */
+__global__ void k_init_stats(unsigned long long *s)
 {
-    __shared__ uint64_t wtmp[17];
-    size_t base = (size_t)blockIdx.x * SC_TILE + (size_t)threadIdx.x * SC_IPT;
-    uint32_t v[SC_IPT];
-    uint64_t s = 0;
-#pragma unroll
-    for (int j = 0; j < SC_IPT; j++) {
-        v[j] = base + j < m ? in[base + j] : 0;
-        s += v[j];
-    }
-    uint64_t tot;
-    uint64_t ex = block_excl_scan<uint64_t>(s, &tot, wtmp) + sums[blockIdx.x];
-#pragma unroll
-    for (int j = 0; j < SC_IPT; j++) {
-        if (base + j < m) out[base + j] = ex;
-        ex += v[j];
-    }
+    if (threadIdx.x < 8) s[threadIdx.x] = (threadIdx.x == 2 || threadIdx.x == 4) ? ~0ULL : 0ULL;
 }
 
-/* scratch layout for scan of m elements: sums[ceil(m/SC_TILE)] */
-static size_t scan_scratch_elems(size_t m) { return (m + SC_TILE - 1) / SC_TILE + 1; }
-
-static void device_scan(hipStream_t st, const uint32_t *in, size_t m, uint64_t *out, uint64_t *sums,
-                        uint64_t *total)
+void launch_init_stats(hipStream_t st, unsigned long long *stats)
 {
-    size_t nb = (m + SC_TILE - 1) / SC_TILE;
-    if (!nb) return;
-    hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(SC_THREADS), 0, st, in, m, sums);
-    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SC_THREADS), 0, st, sums, nb, total);
-    hipLaunchKernelGGL(k_scan_down, dim3((unsigned)nb), dim3(SC_THREADS), 0, st, in, m, sums, out);
+    hipLaunchKernelGGL(k_init_stats, dim3(1), dim3(64), 0, st, stats);
 }
 
-/* ------------------------------------------------------------------------ */
-/* LSD radix sort (keys only, u64) + unique                                  */
-/* ------------------------------------------------------------------------ */
-
-constexpr int RS_THREADS = 256, RS_WAVES = RS_THREADS / 64, RS_KPT = 16;
-constexpr int RS_TILE = RS_THREADS * RS_KPT;       /* 4096 keys */
-constexpr int RS_WCHUNK = 64 * RS_KPT;              /* 1024 keys per wave */
-constexpr int RS_BITS = 8, RS_R = 1 << RS_BITS;
-
-__global__ __launch_bounds__(RS_THREADS) void k_diff_or(const uint64_t *__restrict__ k, size_t n,
-                                                        unsigned long long *acc)
+/* OR / AND of a key array into stats[1], stats[2] */
+__global__ __launch_bounds__(256) void k_or_and(const uint64_t *__restrict__ k, size_t n,
+                                                unsigned long long *stats)
 {
-    __shared__ uint64_t red[RS_WAVES];
-    uint64_t k0 = k[0], m = 0;
-    for (size_t i = (size_t)blockIdx.x * RS_THREADS + threadIdx.x; i < n;
-         i += (size_t)gridDim.x * RS_THREADS)
-        m |= k[i] ^ k0;
+    __shared__ uint64_t ro[4], ra[4];
+    uint64_t o = 0, a = ~0ULL;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        uint64_t v = k[i];
+        o |= v;
+        a &= v;
+    }
 #pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) m |= __shfl_xor(m, d, 64);
-    if (lane_id() == 0) red[threadIdx.x >> 6] = m;
+    for (int d = 32; d >= 1; d >>= 1) {
+        o |= __shfl_xor(o, d, 64);
+        a &= __shfl_xor(a, d, 64);
+    }
+    if (lane_id() == 0) ro[threadIdx.x >> 6] = o, ra[threadIdx.x >> 6] = a;
     __syncthreads();
     if (threadIdx.x == 0) {
-        uint64_t t = 0;
-        for (int w = 0; w < RS_WAVES; w++) t |= red[w];
-        if (t) atomicOr(acc, (unsigned long long)t);
+        for (int w = 1; w < 4; w++) o |= ro[w], a &= ra[w];
+        o |= ro[0];
+        a &= ra[0];
+        atomicOr(&stats[1], (unsigned long long)o);
+        atomicAnd(&stats[2], (unsigned long long)a);
     }
 }
 
-__global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint64_t *__restrict__ keys, size_t n,
-                                                        int shift, uint32_t *counts, size_t nblocks)
+void launch_or_and(hipStream_t st, const uint64_t *k, size_t n, unsigned long long *stats)
 {
-    __shared__ uint32_t h[RS_R];
-    for (int i = threadIdx.x; i < RS_R; i += RS_THREADS) h[i] = 0;
-    __syncthreads();
-    size_t base = (size_t)blockIdx.x * RS_TILE;
-#pragma unroll 4
-    for (int j = 0; j < RS_KPT; j++) {
-        size_t i = base + (size_t)j * RS_THREADS + threadIdx.x;
-        if (i < n) atomicAdd(&h[(unsigned)(keys[i] >> shift) & (RS_R - 1)], 1u);
-    }
-    __syncthreads();
-    for (int d = threadIdx.x; d < RS_R; d += RS_THREADS)
-        counts[(size_t)d * nblocks + blockIdx.x] = h[d];
+    if (!n) return;
+    size_t blocks = (n + 256 * 16 - 1) / (256 * 16);
+    if (blocks > 2048) blocks = 2048;
+    hipLaunchKernelGGL(k_or_and, dim3((unsigned)blocks), dim3(256), 0, st, k, n, stats);
 }
 
-/* offs[d * nblocks + b] = global output position of the first key of digit d
- * in tile b (exclusive scan over the digit-major count matrix). */
-__global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t *__restrict__ in,
-                                                           uint64_t *__restrict__ out, size_t n,
-                                                           int shift, const uint64_t *__restrict__ offs,
-                                                           size_t nblocks)
+/* ------------------------------------------------------------------------ */
+/* onesweep LSD radix sort over the varying key bits + unique                */
+/* ------------------------------------------------------------------------ */
+/* The bits that differ between keys (OR & ~AND of the key set, produced by
+ * the kernels that wrote the keys) are packed into a dense key on the fly
+ * (an order-preserving bijection on the key set), so a DPDGap layer with 40
+ * varying bits spread over bits 0..51 sorts in 5 passes of 8 bits.  One
+ * histogram kernel computes every pass's digit counts in a single read; each
+ * pass is one kernel whose tiles take their global offsets from a chained
+ * (decoupled look-back) scan over per-tile digit counts.  A look-back wait
+ * is bounded: if it ever exceeds its budget the kernel sets an error flag,
+ * finishes, and the host reports a failure. */
+
+constexpr int OS_THREADS = 256, OS_WAVES = OS_THREADS / 64;
+constexpr int OS_KPT_MIN = 16;   /* keys per thread: 16 or 32 (tiles of 4096 or 8192 keys) */
+constexpr int OS_TILE_MIN = OS_THREADS * OS_KPT_MIN;
+static int g_sort_kpt = 32;
+static int g_sort_lbw = 16;   /* look-back window of the radix passes: 16, 32 or 64 tiles */
+constexpr int OS_BITS = 8, OS_R = 1 << OS_BITS;
+constexpr int OS_MAXPASS = 8;
+constexpr uint64_t OS_AGG = 1ULL << 62, OS_INC = 2ULL << 62, OS_VAL = (1ULL << 62) - 1;
+constexpr int OS_SPIN_LIMIT = 1 << 22;
+
+__device__ __forceinline__ uint64_t xform(uint64_t k, const KeyXform &x)
 {
-    __shared__ uint32_t wcnt[RS_WAVES * RS_R];
-    __shared__ uint32_t wpre[RS_WAVES * RS_R];
-    __shared__ uint32_t tstart[RS_R];
-    __shared__ uint64_t gbase[RS_R];
-    __shared__ uint64_t stage[RS_TILE];
+    /* branch-free: unused segments have mask 0 and shift 0 */
+    uint64_t r = 0;
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+        uint64_t part = (k >> x.lo[s]) & x.mask[s];
+        r = (x.len[s] >= 64 ? 0ULL : (r << x.len[s])) | part;
+    }
+    return r;
+}
+
+__device__ __forceinline__ uint64_t ld_flag(const uint64_t *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void st_flag(uint64_t *p, uint64_t v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+/* waits until tile t's status is published; bounded */
+__device__ __forceinline__ uint64_t wait_flag(const uint64_t *p, unsigned *err)
+{
+    uint64_t v;
+    int spins = 0;
+    while (((v = ld_flag(p)) >> 62) == 0) {
+        if (++spins > OS_SPIN_LIMIT) {
+            atomicOr(err, 1u);
+            return OS_INC;
+        }
+        __builtin_amdgcn_s_sleep(2);
+    }
+    return v;
+}
+
+/* Exclusive prefix for one chained-scan lane: status of tile i at
+ * status[i * stride].  The flags are coherent across XCDs, so every load is
+ * a long round trip and the loads themselves cost bandwidth: the window
+ * grows 1, 2, 4, ... up to LB_W predecessors per step (usually the nearest
+ * one or two already carry an inclusive prefix), sums aggregates
+ * nearest-first, and waits only on the first predecessor that has not
+ * published yet. */
+template <int LB_W>
+__device__ __forceinline__ uint64_t lookback(const uint64_t *status, size_t stride, size_t tile,
+                                             unsigned *err)
+{
+    uint64_t excl = 0;
+    long t = (long)tile - 1;
+    int win = 1;
+    while (t >= 0) {
+        uint64_t v[LB_W];
+#pragma unroll
+        for (int j = 0; j < LB_W; j++)
+            v[j] = (j < win && t - j >= 0) ? ld_flag(status + (size_t)(t - j) * stride) : OS_INC;
+        int j = 0;
+        for (; j < win; j++) {
+            uint64_t f = v[j] >> 62;
+            if (f == 0) break;
+            excl += v[j] & OS_VAL;
+            if (f == 2) return excl;
+        }
+        if (j == win) {
+            t -= win;
+            win = win * 2 < LB_W ? win * 2 : LB_W;
+            continue;
+        }
+        uint64_t w = wait_flag(status + (size_t)(t - j) * stride, err);
+        excl += w & OS_VAL;
+        if ((w >> 62) == 2) return excl;
+        t = t - j - 1;
+    }
+    return excl;
+}
+
+__global__ __launch_bounds__(OS_THREADS) void k_os_hist(const uint64_t *__restrict__ keys, size_t n,
+                                                        KeyXform x, int npass, uint32_t *ghist)
+{
+    __shared__ uint32_t h[OS_WAVES][OS_MAXPASS][OS_R];
+    for (int i = threadIdx.x; i < OS_WAVES * OS_MAXPASS * OS_R; i += OS_THREADS)
+        (&h[0][0][0])[i] = 0;
+    __syncthreads();
+    const int w = threadIdx.x >> 6;
+    constexpr int HB = 8;   /* keys loaded per thread before counting */
+    for (size_t b0 = (size_t)blockIdx.x * OS_THREADS * HB; b0 < n;
+         b0 += (size_t)gridDim.x * OS_THREADS * HB) {
+        uint64_t kk[HB];
+#pragma unroll
+        for (int j = 0; j < HB; j++) {
+            size_t i = b0 + (size_t)j * OS_THREADS + threadIdx.x;
+            kk[j] = i < n ? keys[i] : 0;
+        }
+#pragma unroll
+        for (int j = 0; j < HB; j++) {
+            size_t i = b0 + (size_t)j * OS_THREADS + threadIdx.x;
+            if (i >= n) continue;
+            uint64_t c = xform(kk[j], x);
+            for (int p = 0; p < npass; p++)
+                atomicAdd(&h[w][p][(c >> (OS_BITS * p)) & (OS_R - 1)], 1u);
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < npass * OS_R; i += OS_THREADS) {
+        int p = i / OS_R, d = i % OS_R;
+        uint32_t t = h[0][p][d] + h[1][p][d] + h[2][p][d] + h[3][p][d];
+        if (t) atomicAdd(&ghist[i], t);
+    }
+}
+
+/* exclusive scan of every pass's 256 digit counts */
+__global__ __launch_bounds__(OS_R) void k_os_scan(const uint32_t *ghist, int npass, uint64_t *gbase)
+{
+    __shared__ uint64_t wtmp[17];
+    for (int p = 0; p < npass; p++) {
+        uint64_t tot;
+        uint64_t ex = block_excl_scan<uint64_t>(ghist[p * OS_R + threadIdx.x], &tot, wtmp);
+        gbase[p * OS_R + threadIdx.x] = ex;
+    }
+}
+
+template <int OS_KPT, int LB_W>
+__global__ __launch_bounds__(OS_THREADS) void k_os_pass(const uint64_t *__restrict__ in,
+                                                        uint64_t *__restrict__ out, size_t n,
+                                                        KeyXform x, int shift,
+                                                        const uint64_t *__restrict__ gbase,
+                                                        uint64_t *status, unsigned *tile_ctr,
+                                                        unsigned *err)
+{
+    constexpr int OS_TILE = OS_THREADS * OS_KPT, OS_WCHUNK = 64 * OS_KPT;
+    __shared__ uint32_t wcnt[OS_WAVES * OS_R];
+    __shared__ uint32_t wpre[OS_WAVES * OS_R];
+    __shared__ uint32_t tstart[OS_R];
+    __shared__ uint64_t obase[OS_R];
+    __shared__ uint64_t stage[OS_TILE];
     __shared__ uint32_t wtmp[17];
+    __shared__ unsigned tile_sh;
 
     const int w = threadIdx.x >> 6, lane = lane_id();
-    for (int i = threadIdx.x; i < RS_WAVES * RS_R; i += RS_THREADS) wcnt[i] = 0;
+    if (threadIdx.x == 0) tile_sh = atomicAdd(tile_ctr, 1u);
+    for (int i = threadIdx.x; i < OS_WAVES * OS_R; i += OS_THREADS) wcnt[i] = 0;
     __syncthreads();
-
-    const size_t base = (size_t)blockIdx.x * RS_TILE + (size_t)w * RS_WCHUNK;
-    uint64_t key[RS_KPT];
-    uint32_t rank[RS_KPT];
+    const size_t tile = tile_sh;
+    const size_t base = tile * OS_TILE + (size_t)w * OS_WCHUNK;
+    uint64_t key[OS_KPT];
+    uint32_t rank[OS_KPT];
     const uint64_t lt = lanemask_lt();
 #pragma unroll
-    for (int j = 0; j < RS_KPT; j++) {
+    for (int j = 0; j < OS_KPT; j++) {
+        size_t i = base + (size_t)j * 64 + lane;
+        key[j] = i < n ? in[i] : 0;
+    }
+#pragma unroll
+    for (int j = 0; j < OS_KPT; j++) {
         size_t i = base + (size_t)j * 64 + lane;
         bool valid = i < n;
-        uint64_t kk = valid ? in[i] : 0;
-        key[j] = kk;
-        unsigned d = (unsigned)(kk >> shift) & (RS_R - 1);
+        uint64_t kk = key[j];
+        unsigned d = (unsigned)(xform(kk, x) >> shift) & (OS_R - 1);
         uint64_t peers = __ballot(valid);
 #pragma unroll
-        for (int bit = 0; bit < RS_BITS; bit++) {
+        for (int bit = 0; bit < OS_BITS; bit++) {
             bool s = (d >> bit) & 1u;
             uint64_t bb = __ballot(s);
             peers &= s ? bb : ~bb;
         }
         uint32_t before = (uint32_t)__popcll(peers & lt);
-        uint32_t cnt = wcnt[w * RS_R + d];
+        uint32_t cnt = wcnt[w * OS_R + d];
         __builtin_amdgcn_wave_barrier();
-        if (valid && before == 0) wcnt[w * RS_R + d] = cnt + (uint32_t)__popcll(peers);
+        if (valid && before == 0) wcnt[w * OS_R + d] = cnt + (uint32_t)__popcll(peers);
         __builtin_amdgcn_wave_barrier();
         rank[j] = cnt + before;
     }
     __syncthreads();
-    /* per digit: exclusive prefix over waves, tile total, tile start */
-    uint32_t tot_d = 0;
-    for (int d = threadIdx.x; d < RS_R; d += RS_THREADS) {
-        uint32_t s = 0;
+    /* thread d: wave prefixes and the tile's count of digit d */
+    const int d = threadIdx.x;
+    uint32_t tot = 0;
 #pragma unroll
-        for (int ww = 0; ww < RS_WAVES; ww++) {
-            wpre[ww * RS_R + d] = s;
-            s += wcnt[ww * RS_R + d];
-        }
-        tot_d = s;
-        gbase[d] = offs[(size_t)d * nblocks + blockIdx.x];
+    for (int ww = 0; ww < OS_WAVES; ww++) {
+        wpre[ww * OS_R + d] = tot;
+        tot += wcnt[ww * OS_R + d];
     }
+    /* chained scan over tiles for digit d */
+    uint64_t *my = status + tile * OS_R + d;
+    uint64_t excl = 0;
+    if (tile == 0) {
+        st_flag(my, OS_INC | tot);
+    } else {
+        st_flag(my, OS_AGG | tot);
+        excl = lookback<LB_W>(status + d, OS_R, tile, err);
+        st_flag(my, OS_INC | (excl + tot));
+    }
+    obase[d] = gbase[d] + excl;
     uint32_t all;
-    uint32_t st = block_excl_scan<uint32_t>(tot_d, &all, wtmp);   /* RS_THREADS == RS_R */
-    tstart[threadIdx.x] = st;
+    tstart[d] = block_excl_scan<uint32_t>(tot, &all, wtmp);   /* OS_THREADS == OS_R */
     __syncthreads();
 #pragma unroll
-    for (int j = 0; j < RS_KPT; j++) {
+    for (int j = 0; j < OS_KPT; j++) {
         size_t i = base + (size_t)j * 64 + lane;
         if (i < n) {
-            unsigned d = (unsigned)(key[j] >> shift) & (RS_R - 1);
-            stage[tstart[d] + wpre[w * RS_R + d] + rank[j]] = key[j];
+            unsigned dd = (unsigned)(xform(key[j], x) >> shift) & (OS_R - 1);
+            stage[tstart[dd] + wpre[w * OS_R + dd] + rank[j]] = key[j];
         }
     }
     __syncthreads();
-    size_t tile0 = (size_t)blockIdx.x * RS_TILE;
-    uint32_t tn = (uint32_t)((n - tile0) < (size_t)RS_TILE ? (n - tile0) : (size_t)RS_TILE);
-    for (uint32_t p = threadIdx.x; p < tn; p += RS_THREADS) {
+    size_t tile0 = tile * OS_TILE;
+    uint32_t tn = (uint32_t)((n - tile0) < (size_t)OS_TILE ? (n - tile0) : (size_t)OS_TILE);
+    for (uint32_t p = threadIdx.x; p < tn; p += OS_THREADS) {
         uint64_t kk = stage[p];
-        unsigned d = (unsigned)(kk >> shift) & (RS_R - 1);
-        out[gbase[d] + (p - tstart[d])] = kk;
+        unsigned dd = (unsigned)(xform(kk, x) >> shift) & (OS_R - 1);
+        out[obase[dd] + (p - tstart[dd])] = kk;
     }
 }
 
-/* unique compaction over sorted keys: per 1024-key wave chunk */
-__global__ __launch_bounds__(RS_THREADS) void k_uniq_count(const uint64_t *__restrict__ k, size_t n,
-                                                           uint32_t *cc)
+/* unique compaction of a sorted array with a chained scan over tiles;
+ * the last tile writes the total to *total */
+template <int OS_KPT>
+__global__ __launch_bounds__(OS_THREADS) void k_os_unique(const uint64_t *__restrict__ k, size_t n,
+                                                          uint64_t *__restrict__ dst,
+                                                          uint64_t *status, unsigned *tile_ctr,
+                                                          unsigned long long *total, unsigned *err)
 {
+    constexpr int OS_TILE = OS_THREADS * OS_KPT, OS_WCHUNK = 64 * OS_KPT;
+    __shared__ unsigned tile_sh;
+    __shared__ uint32_t wc[OS_WAVES];
+    __shared__ uint64_t excl_sh;
     const int w = threadIdx.x >> 6, lane = lane_id();
-    size_t chunk = (size_t)blockIdx.x * RS_WAVES + w;
-    size_t base = chunk * RS_WCHUNK;
-    if (base >= n) return;
-    uint32_t c = 0;
-#pragma unroll 4
-    for (int j = 0; j < RS_KPT; j++) {
-        size_t i = base + (size_t)j * 64 + lane;
-        bool f = i < n && (i == 0 || k[i] != k[i - 1]);
-        c += (uint32_t)__popcll(__ballot(f));
-    }
-    if (lane == 0) cc[chunk] = c;
-}
-
-__global__ __launch_bounds__(RS_THREADS) void k_uniq_write(const uint64_t *__restrict__ k, size_t n,
-                                                           const uint64_t *__restrict__ co,
-                                                           uint64_t *__restrict__ dst)
-{
-    const int w = threadIdx.x >> 6, lane = lane_id();
-    size_t chunk = (size_t)blockIdx.x * RS_WAVES + w;
-    size_t base = chunk * RS_WCHUNK;
-    if (base >= n) return;
-    uint64_t o = co[chunk];
+    if (threadIdx.x == 0) tile_sh = atomicAdd(tile_ctr, 1u);
+    __syncthreads();
+    const size_t tile = tile_sh;
+    const size_t base = tile * OS_TILE + (size_t)w * OS_WCHUNK;
     const uint64_t lt = lanemask_lt();
-#pragma unroll 4
-    for (int j = 0; j < RS_KPT; j++) {
+    uint32_t c = 0;
+    uint64_t m[OS_KPT], kv[OS_KPT];
+    /* one load per key; the predecessor comes from the neighbouring lane,
+     * lane 0 of each row reads the key before the row */
+#pragma unroll
+    for (int j = 0; j < OS_KPT; j++) {
         size_t i = base + (size_t)j * 64 + lane;
-        uint64_t kk = i < n ? k[i] : 0;
-        bool f = i < n && (i == 0 || kk != k[i - 1]);
-        uint64_t m = __ballot(f);
-        if (f) dst[o + __popcll(m & lt)] = kk;
-        o += __popcll(m);
+        kv[j] = i < n ? k[i] : 0;
     }
+    uint64_t prev0[OS_KPT];
+#pragma unroll
+    for (int j = 0; j < OS_KPT; j++) {
+        size_t i0 = base + (size_t)j * 64;
+        prev0[j] = (lane == 0 && i0 > 0 && i0 <= n) ? k[i0 - 1] : 0;
+    }
+#pragma unroll
+    for (int j = 0; j < OS_KPT; j++) {
+        size_t i = base + (size_t)j * 64 + lane;
+        uint64_t pv = __shfl_up(kv[j], 1, 64);
+        if (lane == 0) pv = prev0[j];
+        bool f = i < n && (i == 0 || kv[j] != pv);
+        m[j] = __ballot(f);
+        c += (uint32_t)__popcll(m[j]);
+    }
+    if (lane == 0) wc[w] = c;
+    __syncthreads();
+    if (w == 0) {
+        /* wave 0 publishes and looks back 64 tiles per step (lane i reads tile t - i) */
+        uint32_t tot = wc[0] + wc[1] + wc[2] + wc[3];
+        uint64_t *my = status + tile;
+        uint64_t excl = 0;
+        if (tile == 0) {
+            if (lane == 0) st_flag(my, OS_INC | tot);
+        } else {
+            if (lane == 0) st_flag(my, OS_AGG | tot);
+            long t = (long)tile - 1;
+            while (t >= 0) {
+                long ti = t - lane;
+                uint64_t v = ti >= 0 ? ld_flag(status + ti) : OS_INC;
+                uint64_t f = v >> 62;
+                uint64_t notready = __ballot(f == 0), inc = __ballot(f == 2);
+                /* nearest lane that is not ready, and nearest inclusive one */
+                int nr = notready ? __ffsll((long long)notready) - 1 : 64;
+                int ni = inc ? __ffsll((long long)inc) - 1 : 64;
+                int stop = nr < ni ? nr : ni;          /* lanes before `stop` are aggregates */
+                uint64_t part = (lane < stop || (lane == ni && ni < nr)) ? (v & OS_VAL) : 0;
+#pragma unroll
+                for (int dd = 32; dd >= 1; dd >>= 1) part += __shfl_xor(part, dd, 64);
+                excl += part;
+                if (ni < nr) break;                    /* reached an inclusive prefix */
+                if (nr < 64) {
+                    /* wait for the nearest unpublished tile, then continue past it */
+                    uint64_t wv = 0;
+                    if (lane == 0) wv = wait_flag(status + (t - nr), err);
+                    wv = __shfl(wv, 0, 64);
+                    excl += wv & OS_VAL;
+                    if ((wv >> 62) == 2) break;
+                    t = t - nr - 1;
+                } else {
+                    t -= 64;
+                }
+            }
+            if (lane == 0) st_flag(my, OS_INC | (excl + tot));
+        }
+        if (lane == 0) {
+            excl_sh = excl;
+            if ((tile + 1) * OS_TILE >= n) *total = excl + tot;
+        }
+    }
+    __syncthreads();
+    uint64_t o = excl_sh;
+    for (int ww = 0; ww < w; ww++) o += wc[ww];
+#pragma unroll
+    for (int j = 0; j < OS_KPT; j++) {
+        size_t i = base + (size_t)j * 64 + lane;
+        if ((m[j] >> lane) & 1ULL) dst[o + __popcll(m[j] & lt)] = kv[j];
+        (void)i;
+        o += __popcll(m[j]);
+    }
+}
+
+KeyXform make_xform(uint64_t vary)
+{
+    KeyXform x;
+    for (int i = 0; i < 4; i++) x.lo[i] = 0, x.len[i] = 0, x.mask[i] = 0;
+    x.nseg = 0;
+    x.nbits = 0;
+    if (!vary) return x;
+    int lo_s[32], len_s[32], ns = 0;
+    int b = 63;
+    while (b >= 0) {   /* runs of set bits, most significant first */
+        if (!((vary >> b) & 1)) {
+            b--;
+            continue;
+        }
+        int hi = b;
+        while (b >= 0 && ((vary >> b) & 1)) b--;
+        lo_s[ns] = b + 1;
+        len_s[ns] = hi - b;
+        ns++;
+    }
+    if (ns > 4) {
+        /* too fragmented: one contiguous range from the lowest to the highest varying bit */
+        int top = 63 - __builtin_clzll(vary), bot = __builtin_ctzll(vary);
+        ns = 1;
+        lo_s[0] = bot;
+        len_s[0] = top - bot + 1;
+    }
+    for (int i = 0; i < ns; i++) {
+        x.lo[i] = lo_s[i];
+        x.len[i] = len_s[i];
+        x.mask[i] = len_s[i] >= 64 ? ~0ULL : ((1ULL << len_s[i]) - 1);
+        x.nbits += len_s[i];
+    }
+    x.nseg = ns;
+    return x;
 }
 
 size_t sort_scratch_bytes(size_t n)
 {
-    size_t nb = (n + RS_TILE - 1) / RS_TILE;
-    size_t m = nb * RS_R;                          /* count matrix */
-    size_t chunks = (n + RS_WCHUNK - 1) / RS_WCHUNK;
-    size_t mm = m > chunks ? m : chunks;
-    return 64 + mm * 4 + mm * 8 + scan_scratch_elems(mm) * 8 + 64;
+    size_t tiles = (n + OS_TILE_MIN - 1) / OS_TILE_MIN + 1;
+    return 4096 + OS_MAXPASS * OS_R * (4 + 8) + tiles * OS_R * 8 + tiles * 8 + 256;
 }
 
-struct ScratchLayout {
-    unsigned long long *diff;
-    uint64_t *total;
-    uint32_t *counts;
-    uint64_t *offs;
-    uint64_t *sums;
+struct OsScratch {
+    unsigned *ctr;        /* OS_MAXPASS + 1 tile counters */
+    uint32_t *ghist;
+    uint64_t *gbase;
+    uint64_t *status;
+    uint64_t *ustatus;
 };
 
-static ScratchLayout carve(void *scratch, size_t n)
+static OsScratch carve(void *scratch, size_t n)
 {
-    size_t nb = (n + RS_TILE - 1) / RS_TILE;
-    size_t m = nb * RS_R;
-    size_t chunks = (n + RS_WCHUNK - 1) / RS_WCHUNK;
-    size_t mm = m > chunks ? m : chunks;
+    size_t tiles = (n + OS_TILE_MIN - 1) / OS_TILE_MIN + 1;
     char *p = (char *)scratch;
-    ScratchLayout s;
-    s.diff = (unsigned long long *)p;
-    s.total = (uint64_t *)(p + 8);
-    p += 64;
-    s.counts = (uint32_t *)p;
-    p += (mm * 4 + 7) / 8 * 8;
-    s.offs = (uint64_t *)p;
-    p += mm * 8;
-    s.sums = (uint64_t *)p;
+    OsScratch s;
+    s.ctr = (unsigned *)p;
+    p += 256;
+    s.ghist = (uint32_t *)p;
+    p += OS_MAXPASS * OS_R * 4;
+    s.gbase = (uint64_t *)p;
+    p += OS_MAXPASS * OS_R * 8;
+    s.ustatus = (uint64_t *)p;
+    p += (tiles * 8 + 255) / 256 * 256;
+    s.status = (uint64_t *)p;
     return s;
 }
 
-SortState sort_unique_begin(hipStream_t st, uint64_t *keys, uint64_t *tmp, size_t n, void *scratch)
+static void launch_pass(hipStream_t st, int kpt, int lbw, unsigned tiles, const uint64_t *src,
+                        uint64_t *dst, size_t n, KeyXform x, int shift, const uint64_t *gbase,
+                        uint64_t *status, unsigned *ctr, unsigned *err)
 {
-    SortState ss;
-    ss.sorted = keys;
-    ss.n = n;
-    ss.n_unique = 0;
-    ss.chunk_off = nullptr;
-    if (n == 0) return ss;
-    ScratchLayout s = carve(scratch, n);
-    unsigned long long h_diff = 0;
-    if (n > 1) {
-        CGK(hipMemsetAsync(s.diff, 0, 8, st));
-        size_t blocks = (n + RS_THREADS * 16 - 1) / (RS_THREADS * 16);
-        if (blocks > 4096) blocks = 4096;
-        hipLaunchKernelGGL(k_diff_or, dim3((unsigned)blocks), dim3(RS_THREADS), 0, st, keys, n,
-                           s.diff);
-        CGK(hipMemcpyAsync(&h_diff, s.diff, 8, hipMemcpyDeviceToHost, st));
-        CGK(hipStreamSynchronize(st));
+#define CG_PASS(K, W)                                                                          \
+    hipLaunchKernelGGL((k_os_pass<K, W>), dim3(tiles), dim3(OS_THREADS), 0, st, src, dst, n, x, \
+                       shift, gbase, status, ctr, err)
+    if (kpt == 32) {
+        if (lbw >= 64) CG_PASS(32, 64);
+        else if (lbw >= 32) CG_PASS(32, 32);
+        else CG_PASS(32, 16);
+    } else {
+        if (lbw >= 64) CG_PASS(16, 64);
+        else if (lbw >= 32) CG_PASS(16, 32);
+        else CG_PASS(16, 16);
     }
-    uint64_t *src = keys, *dst = tmp;
-    if (h_diff) {
-        int lo = __builtin_ctzll(h_diff), hi = 64 - __builtin_clzll(h_diff);
-        size_t nb = (n + RS_TILE - 1) / RS_TILE;
-        for (int shift = lo; shift < hi; shift += RS_BITS) {
-            int h = prof_hook_begin(2);
-            hipLaunchKernelGGL(k_rs_hist, dim3((unsigned)nb), dim3(RS_THREADS), 0, st, src, n,
-                               shift, s.counts, nb);
-            prof_hook_end(h, 8.0 * n);
-            device_scan(st, s.counts, nb * RS_R, s.offs, s.sums, nullptr);
-            h = prof_hook_begin(3);
-            hipLaunchKernelGGL(k_rs_scatter, dim3((unsigned)nb), dim3(RS_THREADS), 0, st, src, dst,
-                               n, shift, s.offs, nb);
-            prof_hook_end(h, 16.0 * n);
-            uint64_t *t = src;
-            src = dst;
-            dst = t;
-        }
-    }
-    /* unique count */
-    size_t chunks = (n + RS_WCHUNK - 1) / RS_WCHUNK;
-    size_t ublocks = (chunks + RS_WAVES - 1) / RS_WAVES;
-    hipLaunchKernelGGL(k_uniq_count, dim3((unsigned)ublocks), dim3(RS_THREADS), 0, st, src, n,
-                       s.counts);
-    device_scan(st, s.counts, chunks, s.offs, s.sums, s.total);
-    uint64_t h_total = 0;
-    CGK(hipMemcpyAsync(&h_total, s.total, 8, hipMemcpyDeviceToHost, st));
-    CGK(hipStreamSynchronize(st));
-    ss.sorted = src;
-    ss.n_unique = (size_t)h_total;
-    ss.chunk_off = s.offs;
-    return ss;
+#undef CG_PASS
 }
 
-void sort_unique_finish(hipStream_t st, const SortState &ss, uint64_t *dst)
+uint64_t *sort_unique(hipStream_t st, uint64_t *keys, uint64_t *tmp, size_t n, uint64_t vary,
+                      void *scratch, uint64_t *dst, unsigned long long *d_nuniq, unsigned *d_err,
+                      int *prof_unique)
 {
-    if (ss.n == 0) return;
-    size_t chunks = (ss.n + RS_WCHUNK - 1) / RS_WCHUNK;
-    size_t ublocks = (chunks + RS_WAVES - 1) / RS_WAVES;
-    hipLaunchKernelGGL(k_uniq_write, dim3((unsigned)ublocks), dim3(RS_THREADS), 0, st, ss.sorted,
-                       ss.n, ss.chunk_off, dst);
+    *prof_unique = -1;
+    if (n == 0) {
+        launch_fill_u64(st, (uint64_t *)d_nuniq, 1, 0);
+        return keys;
+    }
+    OsScratch s = carve(scratch, n);
+    const int kpt = g_sort_kpt == 32 ? 32 : 16;
+    const size_t tile_keys = (size_t)OS_THREADS * kpt;
+    const size_t tiles = (n + tile_keys - 1) / tile_keys;
+    KeyXform x = make_xform(n > 1 ? vary : 0);
+    int npass = (x.nbits + OS_BITS - 1) / OS_BITS;
+    /* counters, error flag and histograms (256 + 8 KiB) cleared together */
+    launch_fill_u64(st, (uint64_t *)scratch, (256 + OS_MAXPASS * OS_R * 4) / 8, 0);
+    uint64_t *src = keys, *dst2 = tmp;
+    if (npass > 0) {
+        size_t hb = (n + OS_TILE_MIN - 1) / OS_TILE_MIN;
+        if (hb > 2048) hb = 2048;
+        int h = prof_hook_begin(2);
+        hipLaunchKernelGGL(k_os_hist, dim3((unsigned)hb), dim3(OS_THREADS), 0, st, keys, n, x, npass,
+                           s.ghist);
+        prof_hook_end(h, 8.0 * n);
+        hipLaunchKernelGGL(k_os_scan, dim3(1), dim3(OS_R), 0, st, s.ghist, npass, s.gbase);
+        for (int p = 0; p < npass; p++) {
+            launch_fill_u64(st, s.status, tiles * OS_R, 0);
+            h = prof_hook_begin(3);
+            launch_pass(st, kpt, g_sort_lbw, (unsigned)tiles, src, dst2, n, x, OS_BITS * p,
+                        s.gbase + OS_R * p, s.status, s.ctr + p, d_err);
+            prof_hook_end(h, 16.0 * n);
+            uint64_t *t = src;
+            src = dst2;
+            dst2 = t;
+        }
+    }
+    launch_fill_u64(st, s.ustatus, tiles, 0);
+    *prof_unique = prof_hook_begin(4);
+    if (kpt == 32)
+        hipLaunchKernelGGL(k_os_unique<32>, dim3((unsigned)tiles), dim3(OS_THREADS), 0, st, src, n,
+                           dst, s.ustatus, s.ctr + OS_MAXPASS, d_nuniq, d_err);
+    else
+        hipLaunchKernelGGL(k_os_unique<16>, dim3((unsigned)tiles), dim3(OS_THREADS), 0, st, src, n,
+                           dst, s.ustatus, s.ctr + OS_MAXPASS, d_nuniq, d_err);
+    prof_hook_end(*prof_unique, 8.0 * n);
+    return src;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -867,3 +1107,10 @@ void launch_adjoint_init(hipStream_t st, const uint64_t *init, size_t n, LayerVi
 }
 
 } // namespace cg
+
+namespace cg {
+void set_sort_kpt(int kpt) { g_sort_kpt = kpt == 32 ? 32 : 16; }
+void set_sort_lbw(int w) { g_sort_lbw = w >= 64 ? 64 : (w >= 32 ? 32 : 16); }
+int get_sort_lbw() { return g_sort_lbw; }
+int get_sort_kpt() { return g_sort_kpt; }
+}

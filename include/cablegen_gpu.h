@@ -106,6 +106,14 @@ int cg_dev_solve_layer(const cg_rules *rules, const uint64_t *d_M, size_t n, dou
 /* synchronises the library's stream */
 int cg_dev_sync(void);
 
+/* ---------------- tuning knobs ---------------- */
+
+/* "sort_kpt": keys per thread of the radix-sort and unique kernels (16 or
+ * 32; tiles of 4096 or 8192 keys).  "sort_lbw": look-back window of the
+ * radix passes (16, 32 or 64 tiles).  Returns 0, or -1 for an unknown knob. */
+int cg_tune_set(const char *knob, int value);
+int cg_tune_get(const char *knob);
+
 /* ---------------- kernel timers ---------------- */
 
 /* When enabled, the library brackets the launches of its main kernels with
