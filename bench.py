@@ -80,6 +80,106 @@ def cpu_baseline(tb, cg, cores):
     }
 
 
+def micro(args):
+    """Kernel roofline microbench (seed 1): 2^30 random boards under the
+    DPDGap mask -> 4 moves + canonical form -> sort + dedupe; then 2^28
+    queries (80% hits) into a sorted 2^30-entry layer.  HBM GB/s per kernel
+    from the library's HIP-event timers, algorithmic bytes as documented."""
+    import ctypes as C
+    import torch
+    import cablegen_amd as cg
+    L = cg.lib()
+    log2n = args.micro_log2
+    n = 1 << log2n
+    nq = n >> 2
+    fmask = 0x00000000FFF0FFF0
+    r = cg.rules(ignore_f=True)
+    dev = torch.device("cuda")
+    out = {"metric": "kernel roofline microbench (HBM GB/s per kernel)", "n_boards": n,
+           "queries": nq, "seed": 1, "formation_mask": "0x00000000FFF0FFF0",
+           "rules": "ignore_f, no prune", "kernels": {}}
+
+    def timed(name, fn):
+        cg.prof_enable(True)
+        cg.prof_reset()
+        fn()
+        res = {}
+        for k in ("move", "sort_hist", "sort_pass", "unique", "lookup"):
+            cnt, ms, b = cg.prof_query(k)
+            if cnt:
+                res[k] = {"launches": cnt, "ms": ms, "GB_s": b / ms / 1e6 if ms else 0.0,
+                          "frac_of_8TBs": (b / ms / 1e6) / HBM_PEAK_GBS if ms else 0.0}
+        cg.prof_enable(False)
+        out["kernels"][name] = res
+
+    boards = torch.empty(n, dtype=torch.int64, device=dev)
+    assert L.cg_dev_random_boards(boards.data_ptr(), n, 1, fmask, 0.4, 9, 0) == 0
+    cand = torch.empty(4 * n, dtype=torch.int64, device=dev)
+    ncand = C.c_size_t()
+    for _ in range(2):   # warm-up + measured
+        timed("move", lambda: L.cg_dev_move(C.byref(r), boards.data_ptr(), n, cand.data_ptr(),
+                                            C.byref(ncand)))
+    uniq = torch.empty(ncand.value, dtype=torch.int64, device=dev)
+    nu = C.c_size_t()
+    t0 = time.perf_counter()
+    timed("sort_unique", lambda: L.cg_dev_sort_unique(cand.data_ptr(), ncand.value, uniq.data_ptr(),
+                                                      C.byref(nu)))
+    out["candidates"] = ncand.value
+    out["unique"] = nu.value
+    out["sort_unique_wall_ms"] = (time.perf_counter() - t0) * 1e3
+    sk = out["kernels"]["sort_unique"]
+    tot_ms = sum(v["ms"] for v in sk.values())
+    out["sort_unique_stage_GB_s"] = (8.0 * ncand.value + 8.0 * nu.value) / tot_ms / 1e6
+    del cand, uniq
+    torch.cuda.empty_cache()
+    # Lookup layer: the skewed distribution above has far fewer than 2^30
+    # distinct boards (2^30 draws give ~4e7), so the layer is drawn with free
+    # cells uniform in 0..13 (about 3.6e10 canonical boards), canonicalised,
+    # sorted, deduplicated and cut to its first 2^log2n entries.
+    extra = n + n // 8
+    lay = torch.empty(extra, dtype=torch.int64, device=dev)
+    assert L.cg_dev_random_boards(lay.data_ptr(), extra, 2, fmask, 0.0, 13, 1) == 0
+    assert L.cg_dev_canon(lay.data_ptr(), extra) == 0
+    keys = torch.empty(extra, dtype=torch.int64, device=dev)
+    nk = C.c_size_t()
+    assert L.cg_dev_sort_unique(lay.data_ptr(), extra, keys.data_ptr(), C.byref(nk)) == 0
+    del lay
+    nl = min(nk.value, n)
+    keys = keys[:nl].contiguous()
+    vals = torch.arange(nl, dtype=torch.float64, device=dev) / nl + 1.0 / (4 * nl)
+    g = torch.Generator(device=dev)
+    g.manual_seed(1)
+    nh = (nq * 4) // 5
+    q = torch.empty(nq, dtype=torch.int64, device=dev)
+    q[:nh] = keys[torch.randint(0, nl, (nh,), device=dev, generator=g)]
+    # 20% verified-absent canonical boards: draw, look up, keep the misses
+    need = nq - nh
+    cand_q = torch.empty(2 * need, dtype=torch.int64, device=dev)
+    assert L.cg_dev_random_boards(cand_q.data_ptr(), 2 * need, 3, fmask, 0.0, 13, 1) == 0
+    assert L.cg_dev_canon(cand_q.data_ptr(), 2 * need) == 0
+    probe = torch.empty(2 * need, dtype=torch.float64, device=dev)
+    assert L.cg_dev_lookup(keys.data_ptr(), vals.data_ptr(), nl, cand_q.data_ptr(), 2 * need,
+                           probe.data_ptr(), 0) == 0
+    absent = cand_q[probe == 0]
+    assert absent.numel() >= need
+    q[nh:] = absent[:need]
+    del cand_q, probe, absent
+    q = q[torch.randperm(nq, device=dev, generator=g)].contiguous()
+    res = torch.empty(nq, dtype=torch.float64, device=dev)
+    for _ in range(2):
+        timed("lookup", lambda: L.cg_dev_lookup(keys.data_ptr(), vals.data_ptr(), nl, q.data_ptr(),
+                                                nq, res.data_ptr(), 0))
+    hits = int((res > 0).sum().item())
+    out["layer_entries"] = nl
+    out["hit_fraction"] = hits / nq
+    # algorithmic bytes of the lookup: 24 B per query (query in, one key probe,
+    # result out) + 8 B per hit (value)
+    lk = out["kernels"]["lookup"]["lookup"]
+    lk["algorithmic_GB_s"] = (24.0 * nq + 8.0 * hits) / lk["ms"] / 1e6
+    lk["frac_of_8TBs"] = lk["algorithmic_GB_s"] / HBM_PEAK_GBS
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -87,11 +187,17 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--micro", action="store_true", help="kernel roofline microbench instead")
+    ap.add_argument("--micro-log2", type=int, default=30, help="log2 of the microbench board count")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU sharding is described in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+
+    if args.micro:
+        micro(args)
+        return
 
     import cablegen_amd as cg
     r = cg.rules(**RULES)

@@ -89,6 +89,9 @@ def lib():
                                      C.c_void_p, C.c_void_p, C.c_size_t,
                                      C.c_void_p, C.c_void_p, C.c_size_t,
                                      U64P, C.c_size_t, C.c_double]
+    L.cg_dev_random_boards.argtypes = [C.c_void_p, C.c_size_t, C.c_uint64, C.c_uint64, C.c_double,
+                                       C.c_int, C.c_int]
+    L.cg_dev_canon.argtypes = [C.c_void_p, C.c_size_t]
     L.cg_tune_set.argtypes = [C.c_char_p, C.c_int]
     L.cg_tune_get.argtypes = [C.c_char_p]
     L.cg_prof_enable.argtypes = [C.c_int]

@@ -111,6 +111,17 @@ static int ensure(Buf &b, size_t n)
     return 0;
 }
 
+static int ensure_scratch_bytes(size_t need)
+{
+    if (need <= E.scratch_cap && E.scratch) return 0;
+    if (E.scratch) HC(hipFree(E.scratch));
+    E.scratch = nullptr;
+    size_t c = need + need / 4 + 4096;
+    HC(hipMalloc(&E.scratch, c));
+    E.scratch_cap = c;
+    return 0;
+}
+
 static int ensure_scratch(size_t n)
 {
     size_t need = sort_scratch_bytes(n);
@@ -150,6 +161,8 @@ struct ProfRec {
     int kind;
     hipEvent_t a, b;
     double bytes;
+    int stats_slot;      /* solve launches: lookup counters read at collection */
+    double base_bytes;
 };
 static bool g_prof = false;
 static std::vector<ProfRec> g_recs;
@@ -175,6 +188,8 @@ static int prof_begin(int kind)
     ProfRec r;
     r.kind = kind;
     r.bytes = 0;
+    r.stats_slot = -1;
+    r.base_bytes = 0;
     r.a = ev_get();
     r.b = ev_get();
     if (!r.a || !r.b) return -1;
@@ -195,11 +210,24 @@ static void prof_set_bytes(int h, double bytes)
     if (h >= 0) g_recs[h].bytes = bytes;
 }
 
+/* per-launch lookup counters of the solve kernel (pairs of u64) */
+static unsigned long long *g_stats = nullptr;
+static int g_stats_cap = 0, g_stats_used = 0;
+
 static void prof_collect()
 {
     if (g_recs.empty()) return;
     hipStreamSynchronize(E.st);
+    std::vector<unsigned long long> st;
+    if (g_stats_used) {
+        st.resize(2 * g_stats_used);
+        hipMemcpy(st.data(), g_stats, st.size() * 8, hipMemcpyDeviceToHost);
+    }
     for (auto &r : g_recs) {
+        if (r.stats_slot >= 0) {
+            double L = (double)st[2 * r.stats_slot], H = (double)st[2 * r.stats_slot + 1];
+            r.bytes = r.base_bytes + 24.0 * L + 8.0 * H;
+        }
         float ms = 0;
         hipEventElapsedTime(&ms, r.a, r.b);
         g_ms[r.kind] += ms;
@@ -209,6 +237,7 @@ static void prof_collect()
         g_evpool.push_back(r.b);
     }
     g_recs.clear();
+    g_stats_used = 0;
 }
 
 /* sort hooks: the radix passes run inside kernels.hip; these brackets let
@@ -369,7 +398,7 @@ static int nonzero_nibbles(uint64_t x)
  * after the spawns, after P_{s+2} and C'. */
 static int gen_step(const cg_rules &r, const uint64_t *P, size_t nP, const uint64_t *C, size_t nC,
                     Carry &carry, LayerAlloc &la, uint64_t **M, size_t *nM, Buf &nxtP, size_t *nP2,
-                    Buf &nxtC, size_t *nC2)
+                    Buf &nxtC, size_t *nC2, uint64_t *m_or = nullptr, uint64_t *m_and = nullptr)
 {
     const uint16_t *lut = get_lut(r.ignore_f);
     if (!lut) return fail_msg("row table upload failed");
@@ -382,6 +411,8 @@ static int gen_step(const cg_rules &r, const uint64_t *P, size_t nP, const uint6
     if (read_counters(0, 3)) return -1;
     size_t ncand = E.h_cnt[0];
     uint64_t mor = E.h_cnt[1], mand = E.h_cnt[2];
+    if (m_or) *m_or = mor;
+    if (m_and) *m_and = mand;
     prof_set_bytes(h, 8.0 * nP + 8.0 * ncand);
     /* M_s = uniq(candidates) */
     *M = la.alloc(ncand);
@@ -464,7 +495,6 @@ static void win_images(const uint64_t *ws, size_t n, std::vector<uint64_t> &out)
     }
 }
 
-static int split_count(size_t n) { return n < 1024 ? (int)n : 1024; }
 
 /* ------------------------------------------------------------------------ */
 /* device-resident tablebase                                                 */
@@ -475,9 +505,52 @@ struct TbLayer {
     size_t n = 0;
     double *vals = nullptr;
     uint64_t *arg = nullptr;
-    uint64_t *split = nullptr;
-    int ns = 0;
+    uint32_t *dir = nullptr;
+    LayerIndexPlan plan;
 };
+
+static LayerView make_view(const uint64_t *keys, const double *vals, size_t n, const uint32_t *dir,
+                           const LayerIndexPlan &p)
+{
+    LayerView v;
+    v.keys = keys;
+    v.vals = vals;
+    v.n = n;
+    v.dir = dir;
+    v.bits = p.bits;
+    v.shift = p.shift;
+    v.cmask = p.cmask;
+    v.cval = p.cval;
+    v.x = p.x;
+    return v;
+}
+
+static LayerView empty_view()
+{
+    LayerView v;
+    memset(&v, 0, sizeof v);
+    return v;
+}
+
+/* builds the prefix-bucket directory of a sorted unique layer into dir
+ * (plan.dir_entries u32) */
+static int build_index(const uint64_t *keys, size_t n, const LayerIndexPlan &plan, uint32_t *dir)
+{
+    if (ensure_scratch_bytes(dir_scratch_bytes(plan.dir_entries))) return -1;
+    launch_build_dir(E.st, keys, n, plan, dir, E.scratch);
+    return 0;
+}
+
+/* OR / AND of a device key array (one synchronisation) */
+static int key_or_and(const uint64_t *keys, size_t n, uint64_t *o, uint64_t *a)
+{
+    launch_init_stats(E.st, E.d_cnt);
+    launch_or_and(E.st, keys, n, E.d_cnt);
+    if (read_counters(1, 2)) return -1;
+    *o = E.h_cnt[1];
+    *a = E.h_cnt[2];
+    return 0;
+}
 
 struct cg_tablebase {
     cg_rules r;
@@ -494,16 +567,10 @@ struct cg_tablebase {
 
 static LayerView view_of(const cg_tablebase *tb, int idx)
 {
-    LayerView v{ nullptr, nullptr, 0, nullptr, 0 };
-    if (idx < 0 || idx >= (int)tb->L.size()) return v;
+    if (idx < 0 || idx >= (int)tb->L.size()) return empty_view();
     const TbLayer &l = tb->L[idx];
-    if (!l.vals) return v;
-    v.keys = l.keys;
-    v.vals = l.vals;
-    v.n = l.n;
-    v.split = l.split;
-    v.ns = l.ns;
-    return v;
+    if (!l.vals) return empty_view();
+    return make_view(l.keys, l.vals, l.n, l.dir, l.plan);
 }
 
 struct ArenaLayerAlloc : LayerAlloc {
@@ -563,14 +630,14 @@ extern "C" int cg_tb_generate(cg_tablebase *tb, const uint64_t *initial, size_t 
     for (int s = start; s <= end; s += 2) {
         TbLayer l;
         size_t nP2 = 0, nC2 = 0;
+        uint64_t mor = 0, mand = ~0ULL;
         if (gen_step(tb->r, tb->P.p, nP, tb->C.p, nC, carry, la, &l.keys, &l.n, tb->P2, &nP2,
-                     tb->C2, &nC2))
+                     tb->C2, &nC2, &mor, &mand))
             return -1;
-        l.ns = split_count(l.n);
-        if (l.ns) {
-            l.split = (uint64_t *)tb->arena.alloc(l.ns * 8);
-            launch_build_split(E.st, l.keys, l.n, l.split, l.ns);
-        }
+        l.plan = plan_index(l.n, mor, mand);
+        l.dir = (uint32_t *)tb->arena.alloc(l.plan.dir_entries * 4);
+        if (!l.dir) return fail_msg("index allocation failed");
+        if (build_index(l.keys, l.n, l.plan, l.dir)) return -1;
         tb->L.push_back(l);
         std::swap(tb->P, tb->P2);
         std::swap(tb->C, tb->C2);
@@ -586,18 +653,25 @@ static int solve_layer_dev(const cg_rules &r, const uint64_t *M, size_t n, doubl
     const uint16_t *lut = get_lut(r.ignore_f);
     if (!lut) return fail_msg("row table upload failed");
     unsigned long long *stats = nullptr;
+    int slot = -1;
     if (g_prof) {
-        HC(hipMemsetAsync(E.d_cnt + 2, 0, 2 * sizeof(unsigned long long), E.st));
-        stats = E.d_cnt + 2;
+        if (g_stats_used == g_stats_cap) {
+            /* grow: collect pending records first so the old array can go */
+            prof_collect();
+            if (g_stats) hipFree(g_stats);
+            g_stats_cap = g_stats_cap ? 2 * g_stats_cap : 4096;
+            HC(hipMalloc((void **)&g_stats, (size_t)g_stats_cap * 16));
+        }
+        slot = g_stats_used++;
+        stats = g_stats + 2 * slot;
+        launch_fill_u64(E.st, (uint64_t *)stats, 2, 0);
     }
     int h = prof_begin(PK_SOLVE);
     launch_solve(E.st, M, n, V, arg, t2, t4, w, p4, lut, r.free_formation, stats);
     prof_end(h, 0);
-    if (g_prof) {
-        HC(hipMemcpyAsync(E.h_cnt + 2, E.d_cnt + 2, 16, hipMemcpyDeviceToHost, E.st));
-        if (check_sync(E.st, "solve stats")) return -1;
-        double L = (double)E.h_cnt[2], H = (double)E.h_cnt[3];
-        prof_set_bytes(h, 16.0 * n + 24.0 * L + 8.0 * H + (arg ? 8.0 * n : 0.0));
+    if (h >= 0) {
+        g_recs[h].stats_slot = slot;
+        g_recs[h].base_bytes = 16.0 * n + (arg ? 8.0 * n : 0.0);
     }
     return 0;
 }
@@ -798,23 +872,35 @@ extern "C" int cg_dev_sort_unique(uint64_t *d_keys, size_t n, uint64_t *d_out, s
     return 0;
 }
 
+/* a temporary index for caller-owned layers */
+struct TmpIndex {
+    uint32_t *dir = nullptr;
+    LayerIndexPlan plan;
+    ~TmpIndex()
+    {
+        if (dir) hipFree(dir);
+    }
+    int build(const uint64_t *keys, size_t n)
+    {
+        uint64_t o = 0, a = ~0ULL;
+        if (n && key_or_and(keys, n, &o, &a)) return -1;
+        plan = plan_index(n, o, a);
+        HC(hipMalloc((void **)&dir, plan.dir_entries * 4));
+        return build_index(keys, n, plan, dir);
+    }
+};
+
 extern "C" int cg_dev_lookup(const uint64_t *d_keys, const double *d_vals, size_t n, const uint64_t *d_q,
                              size_t nq, double *d_out, int canonicalize)
 {
     if (eng_init()) return -1;
-    LayerView t{ d_keys, d_vals, n, nullptr, split_count(n) };
-    uint64_t *split = nullptr;
-    if (t.ns) {
-        HC(hipMalloc((void **)&split, t.ns * 8));
-        launch_build_split(E.st, d_keys, n, split, t.ns);
-        t.split = split;
-    }
+    TmpIndex ix;
+    if (ix.build(d_keys, n)) return -1;
+    LayerView t = make_view(d_keys, d_vals, n, ix.dir, ix.plan);
     int h = prof_begin(PK_LOOKUP);
     launch_lookup(E.st, t, d_q, nq, d_out, canonicalize);
     prof_end(h, 16.0 * nq);
-    int rc = check_sync(E.st, "cg_dev_lookup");
-    if (split) hipFree(split);
-    return rc;
+    return check_sync(E.st, "cg_dev_lookup");
 }
 
 extern "C" int cg_dev_solve_layer(const cg_rules *rules, const uint64_t *d_M, size_t n, double *d_V,
@@ -825,18 +911,28 @@ extern "C" int cg_dev_solve_layer(const cg_rules *rules, const uint64_t *d_M, si
     if (eng_init()) return -1;
     WinRules w;
     if (upload_win(win, n_images, *rules, &w)) return -1;
-    LayerView t2{ d_K2, d_V2, n2, nullptr, split_count(n2) };
-    LayerView t4{ d_K4, d_V4, n4, nullptr, split_count(n4) };
-    uint64_t *sp = nullptr;
-    HC(hipMalloc((void **)&sp, (size_t)(t2.ns + t4.ns + 1) * 8));
-    t2.split = sp;
-    t4.split = sp + t2.ns;
-    launch_build_split(E.st, d_K2, n2, (uint64_t *)t2.split, t2.ns);
-    launch_build_split(E.st, d_K4, n4, (uint64_t *)t4.split, t4.ns);
+    TmpIndex i2, i4;
+    if (i2.build(d_K2, n2) || i4.build(d_K4, n4)) return -1;
+    LayerView t2 = make_view(d_K2, d_V2, n2, i2.dir, i2.plan);
+    LayerView t4 = make_view(d_K4, d_V4, n4, i4.dir, i4.plan);
     int rc = solve_layer_dev(*rules, d_M, n, d_V, nullptr, t2, t4, w, p4);
     if (!rc) rc = check_sync(E.st, "cg_dev_solve_layer");
-    hipFree(sp);
     return rc;
+}
+
+extern "C" int cg_dev_random_boards(uint64_t *d_out, size_t n, uint64_t seed, uint64_t fmask,
+                                    double p_empty, int kmax, int uniform)
+{
+    if (eng_init()) return -1;
+    launch_random_boards(E.st, d_out, n, seed, fmask, p_empty, kmax, uniform);
+    return check_sync(E.st, "cg_dev_random_boards");
+}
+
+extern "C" int cg_dev_canon(uint64_t *d_keys, size_t n)
+{
+    if (eng_init()) return -1;
+    launch_canon(E.st, d_keys, n);
+    return check_sync(E.st, "cg_dev_canon");
 }
 
 extern "C" int cg_tune_set(const char *knob, int value)
@@ -943,9 +1039,9 @@ extern "C" void cge_gen_end(void)
 }
 
 struct SolveLayerBuf {
-    Buf keys, vals, split;
+    Buf keys, vals, dir;
     size_t n = 0;
-    int ns = 0;
+    LayerIndexPlan plan;
 };
 
 static struct {
@@ -958,14 +1054,9 @@ static struct {
 
 static LayerView sview(int i)
 {
-    LayerView v{ nullptr, nullptr, 0, nullptr, 0 };
-    if (i < 0) return v;
-    v.keys = S.L[i].keys.p;
-    v.vals = (const double *)S.L[i].vals.p;
-    v.n = S.L[i].n;
-    v.split = S.L[i].split.p;
-    v.ns = S.L[i].ns;
-    return v;
+    if (i < 0) return empty_view();
+    const SolveLayerBuf &b = S.L[i];
+    return make_view(b.keys.p, (const double *)b.vals.p, b.n, (const uint32_t *)b.dir.p, b.plan);
 }
 
 extern "C" int cge_solve_begin(const cge_rules *r, const uint64_t *win, size_t n_images, double p4)
@@ -984,10 +1075,12 @@ extern "C" int cge_solve_step(const uint64_t *keys, size_t n, double *values)
     SolveLayerBuf &b = S.L[cur];
     if (ensure(b.keys, n + 1) || ensure(b.vals, n + 1)) return -1;
     b.n = n;
-    b.ns = split_count(n);
-    if (ensure(b.split, (size_t)b.ns + 1)) return -1;
     if (n) HC(hipMemcpyAsync(b.keys.p, keys, n * 8, hipMemcpyHostToDevice, E.st));
-    launch_build_split(E.st, b.keys.p, n, b.split.p, b.ns);
+    uint64_t o = 0, a = ~0ULL;
+    if (n && key_or_and(b.keys.p, n, &o, &a)) return -1;
+    b.plan = plan_index(n, o, a);
+    if (ensure(b.dir, b.plan.dir_entries / 2 + 1)) return -1;
+    if (build_index(b.keys.p, n, b.plan, (uint32_t *)b.dir.p)) return -1;
     if (solve_layer_dev(S.r, b.keys.p, n, (double *)b.vals.p, nullptr, sview(S.i2), sview(S.i4), S.w,
                         S.p4))
         return -1;
@@ -1003,7 +1096,7 @@ extern "C" void cge_solve_end(void)
     for (auto &l : S.L) {
         buf_free(l.keys);
         buf_free(l.vals);
-        buf_free(l.split);
+        buf_free(l.dir);
         l.n = 0;
     }
     S.i2 = S.i4 = -1;

@@ -27,14 +27,47 @@ struct WinRules {
     int score;
 };
 
-/* A solved (or generated) layer with its lookup index. */
+/* Order-preserving packing of the varying key bits: up to 4 runs of bits,
+ * most significant first (more fragmented masks use one contiguous range). */
+struct KeyXform {
+    int nseg;
+    int nbits;
+    int lo[4];
+    int len[4];
+    uint64_t mask[4];
+};
+KeyXform make_xform(uint64_t vary);
+
+/* A sorted unique layer with its prefix-bucket index.  Keys of the layer
+ * agree on the bits in cmask (value cval); the remaining bits are packed by
+ * x and the top `bits` packed bits select a bucket: dir[b] is the index of
+ * the first key whose bucket is >= b (dir has 2^bits + 1 entries).  With
+ * bits ~ log2(n) a bucket holds one or two keys on average, so a lookup is
+ * one directory read and a short search. */
 struct LayerView {
     const uint64_t *keys;
     const double *vals;
     size_t n;
-    const uint64_t *split;  /* ns splitters: split[i] = keys[i*n/ns] */
-    int ns;
+    const uint32_t *dir;
+    int bits;
+    int shift;        /* bucket = xform(q) >> shift */
+    uint64_t cmask;
+    uint64_t cval;
+    KeyXform x;
 };
+
+/* index of a layer: directory sizing from n and the OR / AND of its keys */
+struct LayerIndexPlan {
+    KeyXform x;
+    int bits, shift;
+    uint64_t cmask, cval;
+    size_t dir_entries;   /* 2^bits + 1 */
+};
+LayerIndexPlan plan_index(size_t n, uint64_t key_or, uint64_t key_and);
+/* builds dir for a sorted unique layer; scratch holds dir_scratch_bytes(entries) */
+size_t dir_scratch_bytes(size_t entries);
+void launch_build_dir(hipStream_t st, const uint64_t *keys, size_t n, const LayerIndexPlan &p,
+                      uint32_t *dir, void *scratch);
 
 /* kernel-timer brackets provided by the engine (kind: 2 = radix histogram,
  * 3 = radix pass, 4 = unique); a negative handle means timing is off */
@@ -63,16 +96,6 @@ void launch_count_empties(hipStream_t st, const uint64_t *in, size_t n, unsigned
 void launch_spawn(hipStream_t st, const uint64_t *in, size_t n, uint64_t *out2, uint64_t *out4,
                   unsigned long long *stats);
 
-/* Order-preserving packing of the varying key bits: up to 4 runs of bits,
- * most significant first (more fragmented masks use one contiguous range). */
-struct KeyXform {
-    int nseg;
-    int nbits;
-    int lo[4];
-    int len[4];
-    uint64_t mask[4];
-};
-KeyXform make_xform(uint64_t vary);
 
 /* Sorts keys[0..n) ascending and writes the unique keys to dst (capacity n),
  * entirely on the stream (no host synchronisation): onesweep LSD radix over
@@ -99,8 +122,6 @@ void launch_init_stats(hipStream_t st, unsigned long long *stats);
 /* OR / AND of keys into stats[1] / stats[2] */
 void launch_or_and(hipStream_t st, const uint64_t *k, size_t n, unsigned long long *stats);
 
-/* builds ns splitters for a sorted layer */
-void launch_build_split(hipStream_t st, const uint64_t *keys, size_t n, uint64_t *split, int ns);
 
 /* solve one layer: V (and optionally the 2-bit argmax word per board) */
 void launch_solve(hipStream_t st, const uint64_t *M, size_t n, double *V, uint64_t *argmax,
@@ -116,6 +137,12 @@ void launch_adjoint(hipStream_t st, const uint64_t *M, size_t n, const double *o
  * successor and the mean win probability to *P */
 void launch_adjoint_init(hipStream_t st, const uint64_t *init, size_t n, LayerView t0, double *occ0,
                          const uint16_t *lut, int ff, double *P);
+
+/* random boards under a formation mask (see k_random_boards) */
+void launch_random_boards(hipStream_t st, uint64_t *out, size_t n, uint64_t seed, uint64_t fmask,
+                          double p_empty, int kmax, int uniform);
+/* a[i] = canonical form of a[i] */
+void launch_canon(hipStream_t st, uint64_t *a, size_t n);
 
 /* batched exact-match lookups (microbench / API): out[i] = value or 0 */
 void launch_lookup(hipStream_t st, LayerView t, const uint64_t *q, size_t nq, double *out,

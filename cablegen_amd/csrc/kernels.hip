@@ -13,8 +13,9 @@
  *     wave-ballot (match-any) stable ranking inside each 4096-key tile and a
  *     staged, coalesced scatter;
  *   - dedupe is a ballot/popcount stream compaction over the sorted keys;
- *   - the solve keeps a splitter index of each successor layer in LDS and
- *     finishes every lookup with a short binary search in HBM; the FP64 spawn
+ *   - every layer carries a prefix-bucket directory over its packed varying
+ *     bits (about one bucket per key), so a lookup is one directory read and
+ *     a one- or two-step search instead of a full binary search; the FP64 spawn
  *     sums are accumulated per board in ascending cell order (as the reference
  *     does), so the values are bit-identical to a sequential evaluation.
  */
@@ -787,62 +788,154 @@ uint64_t *sort_unique(hipStream_t st, uint64_t *keys, uint64_t *tmp, size_t n, u
 }
 
 /* ------------------------------------------------------------------------ */
-/* splitter index + lookup                                                    */
+/* prefix-bucket directory + lookup                                          */
 /* ------------------------------------------------------------------------ */
 
-__global__ void k_build_split(const uint64_t *__restrict__ keys, size_t n, uint64_t *split, int ns)
+LayerIndexPlan plan_index(size_t n, uint64_t key_or, uint64_t key_and)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < ns) split[i] = keys[(size_t)i * n / (size_t)ns];
+    LayerIndexPlan p;
+    uint64_t vary = n > 1 ? (key_or & ~key_and) : 0;
+    p.x = make_xform(vary);
+    p.cmask = ~vary;
+    p.cval = key_and & p.cmask;
+    int lg = 0;
+    while (((size_t)1 << (lg + 1)) <= n) lg++;
+    int bits = lg < p.x.nbits ? lg : p.x.nbits;
+    if (bits > 30) bits = 30;
+    p.bits = bits;
+    p.shift = p.x.nbits - bits;
+    p.dir_entries = ((size_t)1 << bits) + 1;
+    return p;
 }
 
-void launch_build_split(hipStream_t st, const uint64_t *keys, size_t n, uint64_t *split, int ns)
+/* bucket counts: sorted keys make equal buckets contiguous, so each wave adds
+ * one count per run of equal buckets */
+__global__ __launch_bounds__(256) void k_dir_count(const uint64_t *__restrict__ keys, size_t n,
+                                                   KeyXform x, int shift, uint32_t *cnt)
 {
-    if (!n || ns <= 0) return;
-    hipLaunchKernelGGL(k_build_split, dim3((ns + 255) / 256), dim3(256), 0, st, keys, n, split, ns);
+    size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    bool valid = i < n;
+    uint64_t b = valid ? (xform(keys[i], x) >> shift) : ~0ULL;
+    uint64_t bp = __shfl_up(b, 1, 64);
+    int lane = lane_id();
+    bool start = valid && (lane == 0 || bp != b);
+    uint64_t starts = __ballot(start);
+    uint64_t vmask = __ballot(valid);
+    if (start) {
+        uint64_t later = starts & ~((2ULL << lane) - 1ULL);   /* run starts after this lane */
+        uint64_t endmask = later | ~vmask;
+        endmask &= ~((2ULL << lane) - 1ULL);
+        int end = endmask ? __ffsll((long long)endmask) - 1 : 64;
+        atomicAdd(&cnt[b], (uint32_t)(end - lane));
+    }
 }
 
-/* index of q in the layer, or -1: the last splitter <= q (LDS) bounds the
- * range, then a lower-bound binary search over that range in HBM. */
-__device__ __forceinline__ long layer_find(uint64_t q, const LayerView &t, const uint64_t *spl)
+constexpr int DS_THREADS = 1024, DS_IPT = 4, DS_TILE = DS_THREADS * DS_IPT;
+
+__global__ __launch_bounds__(DS_THREADS) void k_dscan_reduce(const uint32_t *__restrict__ in, size_t m,
+                                                             uint32_t *sums)
 {
-    if (t.n == 0 || q < spl[0]) return -1;
-    int lo = 0, hi = t.ns;
-    while (hi - lo > 1) {
-        int mid = (lo + hi) >> 1;
-        if (spl[mid] <= q) lo = mid;
+    __shared__ uint32_t wtmp[17];
+    size_t base = (size_t)blockIdx.x * DS_TILE;
+    uint32_t s = 0;
+#pragma unroll
+    for (int j = 0; j < DS_IPT; j++) {
+        size_t i = base + (size_t)j * DS_THREADS + threadIdx.x;
+        if (i < m) s += in[i];
+    }
+    uint32_t tot;
+    block_excl_scan<uint32_t>(s, &tot, wtmp);
+    if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(DS_THREADS) void k_dscan_top(uint32_t *sums, size_t nb)
+{
+    __shared__ uint32_t wtmp[17];
+    uint32_t carry = 0;
+    for (size_t c = 0; c < nb; c += DS_THREADS) {
+        size_t i = c + threadIdx.x;
+        uint32_t v = i < nb ? sums[i] : 0;
+        uint32_t tot;
+        uint32_t ex = block_excl_scan<uint32_t>(v, &tot, wtmp);
+        if (i < nb) sums[i] = carry + ex;
+        carry += tot;
+    }
+}
+
+/* exclusive scan of cnt[0..m) into dir[0..m) plus dir[m] = total */
+__global__ __launch_bounds__(DS_THREADS) void k_dscan_down(const uint32_t *__restrict__ in, size_t m,
+                                                           const uint32_t *sums, uint32_t *out,
+                                                           uint32_t total)
+{
+    __shared__ uint32_t wtmp[17];
+    size_t base = (size_t)blockIdx.x * DS_TILE + (size_t)threadIdx.x * DS_IPT;
+    uint32_t v[DS_IPT];
+    uint32_t s = 0;
+#pragma unroll
+    for (int j = 0; j < DS_IPT; j++) {
+        v[j] = base + j < m ? in[base + j] : 0;
+        s += v[j];
+    }
+    uint32_t tot;
+    uint32_t ex = block_excl_scan<uint32_t>(s, &tot, wtmp) + sums[blockIdx.x];
+#pragma unroll
+    for (int j = 0; j < DS_IPT; j++) {
+        if (base + j < m) out[base + j] = ex;
+        ex += v[j];
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) out[m] = total;
+}
+
+size_t dir_scratch_bytes(size_t entries)
+{
+    return entries * 4 + ((entries + DS_TILE - 1) / DS_TILE + 1) * 4 + 1024;
+}
+
+void launch_build_dir(hipStream_t st, const uint64_t *keys, size_t n, const LayerIndexPlan &p,
+                      uint32_t *dir, void *scratch)
+{
+    size_t m = p.dir_entries - 1;   /* buckets */
+    uint32_t *cnt = (uint32_t *)scratch;
+    uint32_t *sums = cnt + ((m + 2 + 63) / 64) * 64;
+    launch_fill_u64(st, (uint64_t *)cnt, (m + 1) / 2 + 1, 0);
+    if (n) {
+        size_t blocks = (n + 255) / 256;
+        hipLaunchKernelGGL(k_dir_count, dim3((unsigned)blocks), dim3(256), 0, st, keys, n, p.x,
+                           p.shift, cnt);
+    }
+    size_t nb = (m + DS_TILE - 1) / DS_TILE;
+    hipLaunchKernelGGL(k_dscan_reduce, dim3((unsigned)nb), dim3(DS_THREADS), 0, st, cnt, m, sums);
+    hipLaunchKernelGGL(k_dscan_top, dim3(1), dim3(DS_THREADS), 0, st, sums, nb);
+    hipLaunchKernelGGL(k_dscan_down, dim3((unsigned)nb), dim3(DS_THREADS), 0, st, cnt, m, sums, dir,
+                       (uint32_t)n);
+}
+
+/* index of q in the layer, or -1: constant bits must match; the bucket's
+ * range comes from the directory, then a lower-bound search inside it */
+__device__ __forceinline__ long layer_find(uint64_t q, const LayerView &t)
+{
+    if (t.n == 0 || (q & t.cmask) != t.cval) return -1;
+    uint64_t b = xform(q, t.x) >> t.shift;
+    size_t lo = t.dir[b], hi = t.dir[b + 1];
+    size_t e = hi;
+    while (lo < hi) {
+        size_t mid = (lo + hi) >> 1;
+        if (t.keys[mid] < q) lo = mid + 1;
         else hi = mid;
     }
-    size_t a = (size_t)lo * t.n / (size_t)t.ns;
-    size_t e = (size_t)(lo + 1) * t.n / (size_t)t.ns;
-    size_t b = e;
-    while (a < b) {
-        size_t mid = (a + b) >> 1;
-        if (t.keys[mid] < q) a = mid + 1;
-        else b = mid;
-    }
-    return (a < e && t.keys[a] == q) ? (long)a : -1;
+    return (lo < e && t.keys[lo] == q) ? (long)lo : -1;
 }
 
-constexpr int NS_MAX = 1024;
 constexpr int LK_THREADS = 256;
-
-__device__ __forceinline__ void load_split(uint64_t *dst, const LayerView &t)
-{
-    for (int i = threadIdx.x; i < t.ns; i += blockDim.x) dst[i] = t.split[i];
-}
 
 __global__ __launch_bounds__(LK_THREADS) void k_lookup(LayerView t, const uint64_t *__restrict__ q,
                                                        size_t nq, double *__restrict__ out, int canon)
 {
-    __shared__ uint64_t spl[NS_MAX];
-    load_split(spl, t);
-    __syncthreads();
     for (size_t i = (size_t)blockIdx.x * LK_THREADS + threadIdx.x; i < nq;
          i += (size_t)gridDim.x * LK_THREADS) {
         uint64_t k = q[i];
         if (canon) k = cg_canon(k);
-        long j = layer_find(k, t, spl);
+        long j = layer_find(k, t);
         out[i] = j >= 0 ? t.vals[j] : 0.0;
     }
 }
@@ -855,6 +948,80 @@ void launch_lookup(hipStream_t st, LayerView t, const uint64_t *q, size_t nq, do
     if (blocks > 16384) blocks = 16384;
     hipLaunchKernelGGL(k_lookup, dim3((unsigned)blocks), dim3(LK_THREADS), 0, st, t, q, nq, out,
                        canonicalize);
+}
+
+/* ------------------------------------------------------------------------ */
+/* workload utilities: random boards, canonical form of an array             */
+/* ------------------------------------------------------------------------ */
+
+__device__ __forceinline__ uint64_t splitmix64(uint64_t x)
+{
+    x += 0x9e3779b97f4a7c15ULL;
+    x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    x = (x ^ (x >> 27)) * 0x94d049bb133111ebULL;
+    return x ^ (x >> 31);
+}
+
+/* Board i: formation cells (nibbles set in fmask) are 0xF.  Skewed mode:
+ * every free nibble is 0 with probability p0, otherwise k in 1..kmax with
+ * P(k) proportional to 2^-k.  Uniform mode: every free nibble is uniform in
+ * 0..kmax.  Random words are splitmix64(seed * 2^40 + 64 * i + j) for
+ * j = 0, 1, ...: the empty test uses the low 32 bits of one word,
+ * k = 1 + (number of trailing zeros of the high 32 bits), and a k above kmax
+ * draws the next word (rejection); uniform values are (word >> 32) mod
+ * (kmax + 1). */
+__global__ void k_random_boards(uint64_t *out, size_t n, uint64_t seed, uint64_t fmask,
+                                uint32_t p0_threshold, int kmax, int uniform)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        uint64_t ctr = (seed << 40) + 64 * (uint64_t)i;
+        uint64_t b = 0;
+        for (int x = 0; x < 16; x++) {
+            int sh = 60 - 4 * x;
+            if ((fmask >> sh) & 0xf) {
+                b |= 0xfULL << sh;
+                continue;
+            }
+            uint64_t r = splitmix64(ctr++);
+            if (uniform) {
+                b |= (uint64_t)((uint32_t)(r >> 32) % (uint32_t)(kmax + 1)) << sh;
+                continue;
+            }
+            if ((uint32_t)r < p0_threshold) continue;
+            int k;
+            for (;;) {
+                uint32_t hi = (uint32_t)(r >> 32);
+                k = 1 + (hi ? __builtin_ctz(hi) : 32);
+                if (k <= kmax) break;
+                r = splitmix64(ctr++);
+            }
+            b |= (uint64_t)k << sh;
+        }
+        out[i] = b;
+    }
+}
+
+void launch_random_boards(hipStream_t st, uint64_t *out, size_t n, uint64_t seed, uint64_t fmask,
+                          double p_empty, int kmax, int uniform)
+{
+    if (!n) return;
+    uint32_t thr = (uint32_t)(p_empty * 4294967296.0);
+    hipLaunchKernelGGL(k_random_boards, dim3(4096), dim3(256), 0, st, out, n, seed, fmask, thr, kmax,
+                       uniform);
+}
+
+__global__ void k_canon_inplace(uint64_t *a, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        a[i] = cg_canon(a[i]);
+}
+
+void launch_canon(hipStream_t st, uint64_t *a, size_t n)
+{
+    if (!n) return;
+    hipLaunchKernelGGL(k_canon_inplace, dim3(4096), dim3(256), 0, st, a, n);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -874,7 +1041,7 @@ __device__ __forceinline__ bool win_satisfied(uint64_t b, const WinRules &w)
 
 /* max over the 4 moves of the successor value; returns the first strict
  * maximum's direction in *arg (0 when every value is 0). */
-__device__ __forceinline__ double best_successor(uint64_t c, const LayerView &t, const uint64_t *spl,
+__device__ __forceinline__ double best_successor(uint64_t c, const LayerView &t,
                                                  const uint16_t *lut, int ff, int *arg,
                                                  uint32_t &nlook, uint32_t &nhit)
 {
@@ -885,7 +1052,7 @@ __device__ __forceinline__ double best_successor(uint64_t c, const LayerView &t,
     for (int d = 0; d < 4; d++) {
         uint64_t m = d < 2 ? c : y;
         if (cg_move_hori_tab(lut, &m, d == 1 || d == 2, ff)) {
-            long j = layer_find(cg_canon(m), t, spl);
+            long j = layer_find(cg_canon(m), t);
             nlook++;
             nhit += j >= 0;
             double v = j >= 0 ? t.vals[j] : 0.0;
@@ -906,14 +1073,11 @@ __global__ __launch_bounds__(SV_THREADS) void k_solve(const uint64_t *__restrict
                                                       const uint16_t *__restrict__ lut, int ff,
                                                       unsigned long long *stats)
 {
-    __shared__ uint64_t spl2[NS_MAX], spl4[NS_MAX];
     __shared__ uint64_t sb[SV_THREADS];
     __shared__ uint8_t pb[16 * SV_THREADS], pc[16 * SV_THREADS];
     __shared__ double m2s[SV_THREADS], m4s[SV_THREADS];
     __shared__ uint8_t ags[SV_THREADS];
     __shared__ uint32_t wtmp[17];
-    load_split(spl2, t2);
-    load_split(spl4, t4);
     const double w2 = 1.0 - p4;
     uint32_t nlook = 0, nhit = 0;
     for (size_t tile = blockIdx.x; tile * SV_THREADS < n; tile += gridDim.x) {
@@ -945,10 +1109,10 @@ __global__ __launch_bounds__(SV_THREADS) void k_solve(const uint64_t *__restrict
                 uint64_t pbv = sb[pb[p]];
                 int x = pc[p];
                 int g2, g4;
-                m2s[threadIdx.x] = best_successor(cg_set_tile(pbv, x, 1), t2, spl2, lut, ff, &g2,
-                                                  nlook, nhit);
-                m4s[threadIdx.x] = best_successor(cg_set_tile(pbv, x, 2), t4, spl4, lut, ff, &g4,
-                                                  nlook, nhit);
+                m2s[threadIdx.x] = best_successor(cg_set_tile(pbv, x, 1), t2, lut, ff, &g2, nlook,
+                                                  nhit);
+                m4s[threadIdx.x] = best_successor(cg_set_tile(pbv, x, 2), t4, lut, ff, &g4, nlook,
+                                                  nhit);
                 ags[threadIdx.x] = (uint8_t)(g2 | (g4 << 2));
             }
             __syncthreads();
@@ -1014,11 +1178,7 @@ __global__ __launch_bounds__(AD_THREADS) void k_adjoint(const uint64_t *__restri
                                                         const uint16_t *__restrict__ lut, int ff,
                                                         double *D)
 {
-    __shared__ uint64_t spl2[NS_MAX], spl4[NS_MAX];
     __shared__ double red[AD_THREADS / 64];
-    load_split(spl2, t2);
-    load_split(spl4, t4);
-    __syncthreads();
     const double w2 = 1.0 - p4;
     double contrib = 0.0;
     for (size_t i = (size_t)blockIdx.x * AD_THREADS + threadIdx.x; i < n;
@@ -1037,12 +1197,11 @@ __global__ __launch_bounds__(AD_THREADS) void k_adjoint(const uint64_t *__restri
 #pragma unroll
             for (int k = 0; k < 2; k++) {
                 const LayerView &t = k ? t4 : t2;
-                const uint64_t *spl = k ? spl4 : spl2;
                 int d = (int)(am >> (4 * x + 2 * k)) & 3;
                 uint64_t m = cg_set_tile(b, x, (unsigned)(k + 1));
                 if (d >= 2) m = cg_rot_cw(m);
                 if (!cg_move_hori_tab(lut, &m, d == 1 || d == 2, ff)) continue;
-                long j = layer_find(cg_canon(m), t, spl);
+                long j = layer_find(cg_canon(m), t);
                 if (j < 0) continue;
                 double v = t.vals[j];
                 if (k) a4 += v;
@@ -1077,9 +1236,6 @@ void launch_adjoint(hipStream_t st, const uint64_t *M, size_t n, const double *o
 __global__ void k_adjoint_init(const uint64_t *init, size_t n, LayerView t0, double *occ0,
                                const uint16_t *lut, int ff, double *P)
 {
-    __shared__ uint64_t spl[NS_MAX];
-    load_split(spl, t0);
-    __syncthreads();
     for (size_t i = threadIdx.x; i < n; i += blockDim.x) {
         uint64_t b = init[i];
         double best = 0.0;
@@ -1087,7 +1243,7 @@ __global__ void k_adjoint_init(const uint64_t *init, size_t n, LayerView t0, dou
         for (int d = 0; d < 4; d++) {
             uint64_t m = b;
             if (!cg_move_tab(lut, &m, d, ff)) continue;
-            long j = layer_find(cg_canon(m), t0, spl);
+            long j = layer_find(cg_canon(m), t0);
             double v = j >= 0 ? t0.vals[j] : 0.0;
             if (v > best) {
                 best = v;

@@ -103,6 +103,14 @@ int cg_dev_solve_layer(const cg_rules *rules, const uint64_t *d_M, size_t n, dou
                        const uint64_t *d_K2, const double *d_V2, size_t n2,
                        const uint64_t *d_K4, const double *d_V4, size_t n4,
                        const uint64_t *win_images, size_t n_images, double p4);
+/* n random boards: cells in fmask are 0xF; every other cell is empty with
+ * probability p_empty, else exponent k in 1..kmax with P(k) ~ 2^-k
+ * (uniform = 0), or uniform in 0..kmax (uniform = 1); drawn from a
+ * counter-based splitmix64 stream keyed by (seed, board index). */
+int cg_dev_random_boards(uint64_t *d_out, size_t n, uint64_t seed, uint64_t fmask, double p_empty,
+                         int kmax, int uniform);
+/* in place: d_keys[i] = canonical form (max of the 8 symmetry images) */
+int cg_dev_canon(uint64_t *d_keys, size_t n);
 /* synchronises the library's stream */
 int cg_dev_sync(void);
 
