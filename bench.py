@@ -180,6 +180,61 @@ def micro(args):
     print(json.dumps(out))
 
 
+def batch_bench(args):
+    """Batching: 64 small tables (cablegen_amd.batch_configs, seeds 0..63)
+    generated and solved together vs one at a time on the same GPU."""
+    import cablegen_amd as cg
+    from cablegen_amd.batch_configs import batch
+    cfgs = batch(args.batch_tables)
+    r = cg.rules(ignore_f=True)
+
+    tb = cg.Tablebase(r)   # one device-resident table reused: its arena is kept
+
+    def one_at_a_time():
+        tot = 0
+        t0 = time.perf_counter()
+        for c in cfgs:
+            tb.generate(c["initial"], c["start"], c["end"])
+            tb.solve(c["winstates"])
+            tot += tb.total_boards()
+        return tot, time.perf_counter() - t0
+
+    B = cg.Batch(r)
+    for c in cfgs:
+        B.add(c["initial"], c["start"], c["end"], c["winstates"])
+
+    def batched():
+        t0 = time.perf_counter()
+        B.generate()
+        t1 = time.perf_counter()
+        B.solve(use_graph=True)
+        t2 = time.perf_counter()
+        return B.total_boards(), t1 - t0, t2 - t1
+
+    one_at_a_time()
+    batched()
+    n1, t_one = one_at_a_time()
+    cg.prof_enable(True)
+    cg.prof_reset()
+    nb, tg, ts = batched()
+    kern = {k: cg.prof_query(k) for k in ("move", "spawn", "sort_hist", "sort_pass", "unique")}
+    cg.prof_enable(False)
+    assert n1 == nb
+    sizes = sorted((B.total_boards_table(t) for t in range(len(cfgs))), reverse=True)
+    out = {"metric": "batched tablebases: boards/s, batched vs one at a time",
+           "tables": len(cfgs), "boards": nb, "one_at_a_time_s": t_one,
+           "batched_generate_s": tg, "batched_solve_s": ts,
+           "one_at_a_time_boards_per_s": n1 / t_one, "batched_boards_per_s": nb / (tg + ts),
+           "speedup": t_one / (tg + ts),
+           "largest_tables": sizes[:5], "median_table": sizes[len(sizes) // 2],
+           "batched_kernels_ms": {k: v[1] for k, v in kern.items()},
+           "workload": "seeds 0..63: formation in {2x4, 3x4, DPDGap}, 2-4 tiles of 2^1..2^7, "
+                       "goal 2^(max+2) top-left, end = start + 2*U{10..40}, ignore_f, no prune"}
+    B.close()
+    tb.close()
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -189,6 +244,8 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--micro", action="store_true", help="kernel roofline microbench instead")
     ap.add_argument("--micro-log2", type=int, default=30, help="log2 of the microbench board count")
+    ap.add_argument("--batch", action="store_true", help="batched-tables benchmark instead")
+    ap.add_argument("--batch-tables", type=int, default=64)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -197,6 +254,9 @@ def main():
 
     if args.micro:
         micro(args)
+        return
+    if args.batch:
+        batch_bench(args)
         return
 
     import cablegen_amd as cg

@@ -92,6 +92,19 @@ def lib():
     L.cg_dev_random_boards.argtypes = [C.c_void_p, C.c_size_t, C.c_uint64, C.c_uint64, C.c_double,
                                        C.c_int, C.c_int]
     L.cg_dev_canon.argtypes = [C.c_void_p, C.c_size_t]
+    L.cg_batch_create.restype = C.c_void_p
+    L.cg_batch_create.argtypes = [C.POINTER(Rules)]
+    L.cg_batch_free.argtypes = [C.c_void_p]
+    L.cg_batch_add.argtypes = [C.c_void_p, U64P, C.c_size_t, C.c_int, C.c_int, U64P, C.c_size_t]
+    L.cg_batch_generate.argtypes = [C.c_void_p]
+    L.cg_batch_solve.argtypes = [C.c_void_p, C.c_double, C.c_int]
+    L.cg_batch_num_tables.argtypes = [C.c_void_p]
+    L.cg_batch_num_layers.argtypes = [C.c_void_p, C.c_int]
+    L.cg_batch_layer_size.restype = C.c_size_t
+    L.cg_batch_layer_size.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    L.cg_batch_total_boards.restype = C.c_size_t
+    L.cg_batch_total_boards.argtypes = [C.c_void_p]
+    L.cg_batch_copy_layer.argtypes = [C.c_void_p, C.c_int, C.c_int, U64P, F64P]
     L.cg_tune_set.argtypes = [C.c_char_p, C.c_int]
     L.cg_tune_get.argtypes = [C.c_char_p]
     L.cg_prof_enable.argtypes = [C.c_int]
@@ -230,6 +243,69 @@ def tune_set(knob, value):
 
 def tune_get(knob):
     return lib().cg_tune_get(knob.encode())
+
+
+class Batch:
+    """Many small tables generated and solved together (cg_batch_*)."""
+
+    def __init__(self, r):
+        self.L = lib()
+        self.h = self.L.cg_batch_create(C.byref(r))
+        if not self.h:
+            raise CablegenError(self.L.cg_last_error().decode())
+
+    def close(self):
+        if self.h:
+            self.L.cg_batch_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add(self, initial, start, end, winstates):
+        a, p = _u64(initial)
+        w, wp = _u64(winstates)
+        t = self.L.cg_batch_add(self.h, p, len(a), int(start), int(end), wp, len(w))
+        if t < 0:
+            raise CablegenError(self.L.cg_last_error().decode())
+        return t
+
+    def generate(self):
+        _check(self.L.cg_batch_generate(self.h), "cg_batch_generate")
+
+    def solve(self, p4=0.1, use_graph=True):
+        _check(self.L.cg_batch_solve(self.h, float(p4), int(use_graph)), "cg_batch_solve")
+
+    @property
+    def num_tables(self):
+        return self.L.cg_batch_num_tables(self.h)
+
+    def num_layers(self, t):
+        return self.L.cg_batch_num_layers(self.h, int(t))
+
+    def layer_size(self, t, i):
+        return self.L.cg_batch_layer_size(self.h, int(t), int(i))
+
+    def total_boards(self):
+        return self.L.cg_batch_total_boards(self.h)
+
+    def total_boards_table(self, t):
+        return sum(self.layer_size(t, i) for i in range(self.num_layers(t)))
+
+    def keys(self, t, i):
+        k = np.zeros(self.layer_size(t, i), dtype=np.uint64)
+        _check(self.L.cg_batch_copy_layer(self.h, int(t), int(i), k.ctypes.data_as(U64P), None),
+               "copy")
+        return k
+
+    def values(self, t, i):
+        v = np.zeros(self.layer_size(t, i), dtype=np.float64)
+        _check(self.L.cg_batch_copy_layer(self.h, int(t), int(i), None, v.ctypes.data_as(F64P)),
+               "copy")
+        return v
 
 
 def prof_enable(on=True):

@@ -352,9 +352,11 @@ static int read_counters(int lo, int n)
     HC(hipMemcpyAsync(E.h_err, E.d_err, sizeof(unsigned), hipMemcpyDeviceToHost, E.st));
     if (check_sync(E.st, "counters")) return -1;
     if (*E.h_err) {
+        unsigned e = *E.h_err;
         *E.h_err = 0;
         hipMemsetAsync(E.d_err, 0, sizeof(unsigned), E.st);
         hipStreamSynchronize(E.st);
+        if (e & 2u) return fail_msg("batch: a board's formation cells left the table's canonical place");
         return fail_msg("sort: a chained-scan wait exceeded its budget");
     }
     return 0;
@@ -1100,4 +1102,380 @@ extern "C" void cge_solve_end(void)
         l.n = 0;
     }
     S.i2 = S.i4 = -1;
+}
+
+/* ------------------------------------------------------------------------ */
+/* batched tables                                                            */
+/* ------------------------------------------------------------------------ */
+/* Many small tables share every kernel launch: per layer step one move, one
+ * sort/dedupe, one spawn, two sorts and one directory build serve all the
+ * tables (composite keys carry the table id), with four host
+ * synchronisations per step whatever the number of tables.  The solve of a
+ * step is one kernel over a per-table descriptor array; the whole downward
+ * sweep is captured once in a hipGraph and replayed. */
+
+struct cg_batch {
+    cg_rules r;
+    struct Tab {
+        std::vector<uint64_t> init;
+        int start = 0, end = 0, nsteps = 0;
+        std::vector<uint64_t> wimg;
+        BatchTable dev;
+    };
+    std::vector<Tab> tabs;
+    Arena arena;
+    BatchTable *d_tabs = nullptr;
+    struct Step {
+        uint64_t *keys = nullptr;
+        double *vals = nullptr;
+        uint32_t *dir = nullptr;
+        size_t n = 0;
+        std::vector<size_t> off;            /* ntab + 1 */
+        std::vector<LayerIndexPlan> plan;   /* per table */
+        std::vector<size_t> dir_off;        /* per table */
+    };
+    std::vector<Step> steps;
+    int K = 0;
+    Buf P, C, P2, C2, M;
+    unsigned long long *d_tstat = nullptr;
+    std::vector<unsigned long long> h_tstat;
+    /* solve */
+    BSolveDesc *d_desc = nullptr;
+    std::vector<size_t> tiles;
+    uint64_t *d_wmask = nullptr, *d_wval = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    bool solved = false;
+    double graph_p4 = -1;
+};
+
+extern "C" cg_batch *cg_batch_create(const cg_rules *rules)
+{
+    if (eng_init()) return nullptr;
+    if (rules->free_formation) {
+        fail_msg("batched tables need free_formation = false (formation cells must stay put)");
+        return nullptr;
+    }
+    cg_batch *b = new cg_batch();
+    b->r = *rules;
+    return b;
+}
+
+static void batch_drop_graph(cg_batch *b)
+{
+    if (b->gexec) hipGraphExecDestroy(b->gexec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    b->gexec = nullptr;
+    b->graph = nullptr;
+}
+
+extern "C" void cg_batch_free(cg_batch *b)
+{
+    if (!b) return;
+    hipStreamSynchronize(E.st);
+    batch_drop_graph(b);
+    b->arena.release();
+    buf_free(b->P);
+    buf_free(b->C);
+    buf_free(b->P2);
+    buf_free(b->C2);
+    buf_free(b->M);
+    if (b->d_tabs) hipFree(b->d_tabs);
+    if (b->d_tstat) hipFree(b->d_tstat);
+    delete b;
+}
+
+extern "C" int cg_batch_add(cg_batch *b, const uint64_t *initial, size_t n, int start, int end,
+                            const uint64_t *winstates, size_t n_win)
+{
+    if (!n) return fail_msg("cg_batch_add: no initial boards");
+    if (b->tabs.size() >= (1u << (64 - BATCH_TID_SHIFT))) return fail_msg("cg_batch_add: too many tables");
+    if (end < start) return fail_msg("cg_batch_add: end < start");
+    cg_batch::Tab t;
+    uint64_t c0 = cg_canon(initial[0]);
+    uint64_t fnib = cg_fmask(c0) * 0xf;
+    int nfree = 0;
+    for (int x = 0; x < 16; x++)
+        if (!((fnib >> (60 - 4 * x)) & 0xf)) t.dev.pos[nfree++] = (uint8_t)x;
+    if (nfree * 4 > BATCH_TID_SHIFT)
+        return fail_msg("cg_batch_add: more than 12 free cells do not fit a composite key");
+    t.dev.fnib = fnib;
+    t.dev.nfree = nfree;
+    t.start = start;
+    t.end = end;
+    t.nsteps = (end - start) / 2 + 1;
+    t.dev.nsteps = t.nsteps;
+    for (size_t i = 0; i < n; i++) {
+        uint64_t c = cg_canon(initial[i]);
+        if ((c & fnib) != fnib) return fail_msg("cg_batch_add: initial boards differ in formation");
+        t.init.push_back(c);
+    }
+    win_images(winstates, n_win, t.wimg);
+    b->tabs.push_back(t);
+    b->solved = false;
+    return (int)b->tabs.size() - 1;
+}
+
+struct BatchMAlloc : LayerAlloc {
+    Buf *buf;
+    explicit BatchMAlloc(Buf *x) : buf(x) {}
+    uint64_t *alloc(size_t n) override
+    {
+        if (ensure(*buf, n + 1)) return nullptr;
+        return buf->p;
+    }
+};
+
+static uint64_t host_benc(uint64_t c, const BatchTable &T, uint64_t tid)
+{
+    uint64_t k = 0;
+    for (int j = 0; j < T.nfree; j++) k = (k << 4) | ((c >> (60 - 4 * T.pos[j])) & 0xf);
+    return (tid << BATCH_TID_SHIFT) | k;
+}
+
+extern "C" int cg_batch_generate(cg_batch *b)
+{
+    if (eng_init()) return -1;
+    const int ntab = (int)b->tabs.size();
+    if (!ntab) return fail_msg("cg_batch_generate: no tables");
+    HC(hipStreamSynchronize(E.st));
+    batch_drop_graph(b);
+    b->arena.reset();
+    b->steps.clear();
+    b->solved = false;
+    b->d_desc = nullptr;
+    const uint16_t *lut = get_lut(b->r.ignore_f);
+    if (!lut) return fail_msg("row table upload failed");
+    /* device table descriptors and per-table statistics */
+    if (b->d_tabs) hipFree(b->d_tabs);
+    if (b->d_tstat) hipFree(b->d_tstat);
+    std::vector<BatchTable> ht(ntab);
+    b->K = 0;
+    for (int t = 0; t < ntab; t++) {
+        ht[t] = b->tabs[t].dev;
+        b->K = std::max(b->K, b->tabs[t].nsteps);
+    }
+    HC(hipMalloc((void **)&b->d_tabs, ntab * sizeof(BatchTable)));
+    HC(hipMemcpy(b->d_tabs, ht.data(), ntab * sizeof(BatchTable), hipMemcpyHostToDevice));
+    HC(hipMalloc((void **)&b->d_tstat, 3 * ntab * sizeof(unsigned long long)));
+    b->h_tstat.assign(3 * ntab, 0);
+    /* pre-move layer 0: canonical initial boards as composite keys */
+    std::vector<uint64_t> p0;
+    for (int t = 0; t < ntab; t++)
+        for (uint64_t c : b->tabs[t].init) p0.push_back(host_benc(c, b->tabs[t].dev, (uint64_t)t));
+    if (ensure(b->P, p0.size() + 1)) return -1;
+    HC(hipMemcpy(b->P.p, p0.data(), p0.size() * 8, hipMemcpyHostToDevice));
+    size_t nP = p0.size(), nC = 0;
+    Carry carry;
+    std::vector<uint64_t> init_tstat(3 * ntab);
+    for (int t = 0; t < ntab; t++) init_tstat[3 * t + 2] = ~0ULL;
+    for (int k = 0; k < b->K; k++) {
+        cg_batch::Step st;
+        /* moves of every table */
+        if (ensure(E.A, 4 * nP + 1)) return -1;
+        launch_init_stats(E.st, E.d_cnt);
+        int h = prof_begin(PK_MOVE);
+        launch_bmove(E.st, b->P.p, nP, E.A.p, E.d_cnt, lut, move_rules(b->r), b->d_tabs, k, E.d_err);
+        prof_end(h, 0);
+        if (read_counters(0, 3)) return -1;
+        size_t ncand = E.h_cnt[0];
+        uint64_t mor = E.h_cnt[1], mand = E.h_cnt[2];
+        prof_set_bytes(h, 8.0 * nP + 8.0 * ncand);
+        if (ensure(b->M, ncand + 1)) return -1;
+        int hu;
+        if (sort_enqueue(E.A.p, ncand, vary_of(ncand, mor, mand), b->M.p, 5, &hu)) return -1;
+        if (read_counters(5, 1)) return -1;
+        size_t nM = E.h_cnt[5];
+        prof_set_bytes(hu, 8.0 * ncand + 8.0 * nM);
+        /* boards of the step, per-table segments and statistics */
+        st.n = nM;
+        st.keys = (uint64_t *)b->arena.alloc(std::max<size_t>(nM, 1) * 8);
+        if (!st.keys) return fail_msg("batch layer allocation failed");
+        HC(hipMemcpyAsync(b->d_tstat, init_tstat.data(), 3 * ntab * 8, hipMemcpyHostToDevice, E.st));
+        launch_bdecode(E.st, b->M.p, nM, st.keys, b->d_tabs, b->d_tstat);
+        /* spawns of every table */
+        size_t cap = 16 * nM;
+        if (ensure(E.A, nC + cap + 1) || ensure(E.Cb, cap + 1)) return -1;
+        if (nC) HC(hipMemcpyAsync(E.A.p, b->C.p, nC * 8, hipMemcpyDeviceToDevice, E.st));
+        launch_init_stats(E.st, E.d_cnt);
+        int hs = prof_begin(PK_SPAWN);
+        launch_bspawn(E.st, b->M.p, nM, E.A.p + nC, E.Cb.p, E.d_cnt, b->d_tabs, k, E.d_err);
+        prof_end(hs, 0);
+        HC(hipMemcpyAsync(b->h_tstat.data(), b->d_tstat, 3 * ntab * 8, hipMemcpyDeviceToHost, E.st));
+        if (read_counters(0, 5)) return -1;
+        size_t sumE = E.h_cnt[0];
+        prof_set_bytes(hs, 8.0 * nM + 16.0 * sumE);
+        uint64_t or2 = E.h_cnt[1], and2 = E.h_cnt[2], or4 = E.h_cnt[3], and4 = E.h_cnt[4];
+        size_t n2 = nC + sumE;
+        if (ensure(b->P2, n2 + 1) || ensure(b->C2, sumE + 1)) return -1;
+        int hu2, hu4;
+        if (sort_enqueue(E.A.p, n2, vary_of(n2, or2 | carry.o, and2 & carry.a), b->P2.p, 5, &hu2))
+            return -1;
+        if (sort_enqueue(E.Cb.p, sumE, vary_of(sumE, or4, and4), b->C2.p, 6, &hu4)) return -1;
+        /* directories of every table (enqueued before the synchronisation) */
+        st.off.assign(ntab + 1, 0);
+        st.plan.resize(ntab);
+        st.dir_off.resize(ntab);
+        std::vector<BatchIndexPlan> bp(ntab);
+        size_t entries = 0;
+        for (int t = 0; t < ntab; t++) {
+            size_t nt = b->h_tstat[3 * t];
+            st.off[t + 1] = st.off[t] + nt;
+            st.plan[t] = plan_index(nt, b->h_tstat[3 * t + 1], b->h_tstat[3 * t + 2]);
+            st.dir_off[t] = entries;
+            bp[t].x = st.plan[t].x;
+            bp[t].shift = st.plan[t].shift;
+            bp[t].dir_base = entries;
+            entries += st.plan[t].dir_entries;
+        }
+        if (st.off[ntab] != nM) return fail_msg("batch: per-table counts disagree with the layer");
+        st.dir = (uint32_t *)b->arena.alloc((entries + 1) * 4);
+        size_t *d_seg = (size_t *)b->arena.alloc((ntab + 1) * sizeof(size_t));
+        BatchIndexPlan *d_plan = (BatchIndexPlan *)b->arena.alloc(ntab * sizeof(BatchIndexPlan));
+        if (!st.dir || !d_seg || !d_plan) return fail_msg("batch index allocation failed");
+        HC(hipMemcpy(d_seg, st.off.data(), (ntab + 1) * sizeof(size_t), hipMemcpyHostToDevice));
+        HC(hipMemcpy(d_plan, bp.data(), ntab * sizeof(BatchIndexPlan), hipMemcpyHostToDevice));
+        if (read_counters(5, 2)) return -1;
+        size_t nP2 = E.h_cnt[5], nC2 = E.h_cnt[6];
+        prof_set_bytes(hu2, 8.0 * n2 + 8.0 * nP2);
+        prof_set_bytes(hu4, 8.0 * sumE + 8.0 * nC2);
+        if (ensure_scratch_bytes(dir_scratch_bytes(entries + 1))) return -1;
+        launch_bdir(E.st, st.keys, nM, d_seg, ntab, d_plan, entries, st.dir, E.scratch);
+        b->steps.push_back(std::move(st));
+        std::swap(b->P, b->P2);
+        std::swap(b->C, b->C2);
+        nP = nP2;
+        nC = nC2;
+        carry.o = or4;
+        carry.a = and4;
+    }
+    return check_sync(E.st, "cg_batch_generate");
+}
+
+static LayerView batch_view(const cg_batch *b, int t, int layer)
+{
+    if (layer < 0 || layer >= b->tabs[t].nsteps) return empty_view();
+    const cg_batch::Step &st = b->steps[layer];
+    if (!st.vals) return empty_view();
+    size_t nt = st.off[t + 1] - st.off[t];
+    return make_view(st.keys, st.vals, nt, st.dir + st.dir_off[t], st.plan[t]);
+}
+
+extern "C" int cg_batch_solve(cg_batch *b, double p4, int use_graph)
+{
+    if (eng_init()) return -1;
+    const int ntab = (int)b->tabs.size();
+    if ((int)b->steps.size() != b->K || !ntab) return fail_msg("cg_batch_solve needs cg_batch_generate");
+    const uint16_t *lut = get_lut(b->r.ignore_f);
+    if (!lut) return fail_msg("row table upload failed");
+    if (!b->d_desc) {
+        /* values, winstate images and the per-step descriptors, once per generation */
+        for (auto &st : b->steps) {
+            st.vals = (double *)b->arena.alloc(std::max<size_t>(st.n, 1) * 8);
+            if (!st.vals) return fail_msg("batch value allocation failed");
+        }
+        std::vector<uint64_t> wm, wv;
+        std::vector<size_t> woff(ntab + 1, 0);
+        for (int t = 0; t < ntab; t++) {
+            for (uint64_t w : b->tabs[t].wimg) {
+                wm.push_back(cg_win_mask(w));
+                wv.push_back(w);
+            }
+            woff[t + 1] = wm.size();
+        }
+        b->d_wmask = (uint64_t *)b->arena.alloc(std::max<size_t>(wm.size(), 1) * 8);
+        b->d_wval = (uint64_t *)b->arena.alloc(std::max<size_t>(wv.size(), 1) * 8);
+        if (!wm.empty()) {
+            HC(hipMemcpy(b->d_wmask, wm.data(), wm.size() * 8, hipMemcpyHostToDevice));
+            HC(hipMemcpy(b->d_wval, wv.data(), wv.size() * 8, hipMemcpyHostToDevice));
+        }
+        std::vector<BSolveDesc> all((size_t)b->K * ntab);
+        b->tiles.assign(b->K, 0);
+        for (int k = 0; k < b->K; k++) {
+            size_t tile0 = 0;
+            for (int t = 0; t < ntab; t++) {
+                BSolveDesc &d = all[(size_t)k * ntab + t];
+                memset(&d, 0, sizeof d);
+                int layer = b->tabs[t].nsteps - 1 - k;   /* each table from its own top */
+                d.tile0 = tile0;
+                d.win.mask = b->d_wmask + woff[t];
+                d.win.val = b->d_wval + woff[t];
+                d.win.n = (int)(woff[t + 1] - woff[t]);
+                d.win.nox = b->r.nox;
+                d.win.score = b->r.score;
+                if (layer < 0) continue;
+                const cg_batch::Step &st = b->steps[layer];
+                d.M = st.keys + st.off[t];
+                d.n = st.off[t + 1] - st.off[t];
+                d.V = st.vals + st.off[t];
+                d.t2 = batch_view(b, t, layer + 1);
+                d.t4 = batch_view(b, t, layer + 2);
+                tile0 += (d.n + 255) / 256;
+            }
+            b->tiles[k] = tile0;
+        }
+        b->d_desc = (BSolveDesc *)b->arena.alloc(all.size() * sizeof(BSolveDesc));
+        if (!b->d_desc) return fail_msg("batch descriptor allocation failed");
+        HC(hipMemcpy(b->d_desc, all.data(), all.size() * sizeof(BSolveDesc), hipMemcpyHostToDevice));
+    }
+    if (!use_graph) {
+        for (int k = 0; k < b->K; k++) {
+            int h = prof_begin(PK_SOLVE);
+            launch_bsolve(E.st, b->d_desc + (size_t)k * ntab, ntab, b->tiles[k], p4, lut,
+                          b->r.free_formation);
+            prof_end(h, 0);
+        }
+    } else {
+        if (!b->gexec || b->graph_p4 != p4) {
+            batch_drop_graph(b);
+            HC(hipStreamSynchronize(E.st));
+            HC(hipStreamBeginCapture(E.st, hipStreamCaptureModeThreadLocal));
+            for (int k = 0; k < b->K; k++)
+                launch_bsolve(E.st, b->d_desc + (size_t)k * ntab, ntab, b->tiles[k], p4, lut,
+                              b->r.free_formation);
+            hipError_t e = hipStreamEndCapture(E.st, &b->graph);
+            if (e != hipSuccess) return fail("hipStreamEndCapture", e);
+            HC(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
+            b->graph_p4 = p4;
+        }
+        HC(hipGraphLaunch(b->gexec, E.st));
+    }
+    b->solved = true;
+    return check_sync(E.st, "cg_batch_solve");
+}
+
+extern "C" int cg_batch_num_tables(const cg_batch *b) { return (int)b->tabs.size(); }
+extern "C" int cg_batch_num_layers(const cg_batch *b, int t)
+{
+    return (t >= 0 && t < (int)b->tabs.size()) ? b->tabs[t].nsteps : -1;
+}
+
+extern "C" size_t cg_batch_layer_size(const cg_batch *b, int t, int layer)
+{
+    if (t < 0 || t >= (int)b->tabs.size() || layer < 0 || layer >= b->tabs[t].nsteps ||
+        layer >= (int)b->steps.size())
+        return 0;
+    return b->steps[layer].off[t + 1] - b->steps[layer].off[t];
+}
+
+extern "C" size_t cg_batch_total_boards(const cg_batch *b)
+{
+    size_t s = 0;
+    for (auto &st : b->steps) s += st.n;
+    return s;
+}
+
+extern "C" int cg_batch_copy_layer(const cg_batch *b, int t, int layer, uint64_t *keys, double *values)
+{
+    if (t < 0 || t >= (int)b->tabs.size() || layer < 0 || layer >= (int)b->steps.size() ||
+        layer >= b->tabs[t].nsteps)
+        return fail_msg("cg_batch_copy_layer: index out of range");
+    const cg_batch::Step &st = b->steps[layer];
+    size_t o = st.off[t], n = st.off[t + 1] - o;
+    if (keys && n) HC(hipMemcpy(keys, st.keys + o, n * 8, hipMemcpyDeviceToHost));
+    if (values && n) {
+        if (!b->solved) return fail_msg("cg_batch_copy_layer: not solved");
+        HC(hipMemcpy(values, st.vals + o, n * 8, hipMemcpyDeviceToHost));
+    }
+    return 0;
 }

@@ -148,5 +148,51 @@ void launch_canon(hipStream_t st, uint64_t *a, size_t n);
 void launch_lookup(hipStream_t st, LayerView t, const uint64_t *q, size_t nq, double *out,
                    int canonicalize);
 
+/* ---- batched tables ---- */
+constexpr int BATCH_TID_SHIFT = 48;
+
+/* per table: formation nibbles (0xF) of its canonical boards, free tiles in
+ * ascending order, number of layer steps */
+struct BatchTable {
+    uint64_t fnib;
+    uint8_t pos[16];
+    int nfree;
+    int nsteps;
+};
+
+struct BatchIndexPlan {
+    KeyXform x;
+    int shift;
+    size_t dir_base;   /* first bucket of this table in the concatenated directory */
+};
+
+struct BSolveDesc {
+    const uint64_t *M;
+    size_t n;
+    double *V;
+    uint64_t *arg;
+    LayerView t2, t4;
+    WinRules win;
+    size_t tile0;
+};
+
+/* composite-key move / spawn; err |= 2 if a board's formation is not where
+ * the table's canonical form puts it */
+void launch_bmove(hipStream_t st, const uint64_t *in, size_t n, uint64_t *out,
+                  unsigned long long *stats, const uint16_t *lut, MoveRules r, const BatchTable *tabs,
+                  int step, unsigned *err);
+void launch_bspawn(hipStream_t st, const uint64_t *in, size_t n, uint64_t *out2, uint64_t *out4,
+                   unsigned long long *stats, const BatchTable *tabs, int step, unsigned *err);
+/* composite -> boards, with per-table count / OR / AND (tstat[3 t + 0..2],
+ * AND entries preset to all ones) */
+void launch_bdecode(hipStream_t st, const uint64_t *in, size_t n, uint64_t *out,
+                    const BatchTable *tabs, unsigned long long *tstat);
+/* directories of every table of a step in one concatenated array; entries
+ * are indices into the step's concatenated keys */
+void launch_bdir(hipStream_t st, const uint64_t *keys, size_t n, const size_t *seg, int ntab,
+                 const BatchIndexPlan *plan, size_t total_entries, uint32_t *dir, void *scratch);
+void launch_bsolve(hipStream_t st, const BSolveDesc *desc, int ntab, size_t total_tiles, double p4,
+                   const uint16_t *lut, int ff);
+
 } // namespace cg
 #endif

@@ -1066,6 +1066,97 @@ __device__ __forceinline__ double best_successor(uint64_t c, const LayerView &t,
     return best;
 }
 
+struct SolveSmem {
+    uint64_t sb[SV_THREADS];
+    uint8_t pb[16 * SV_THREADS], pc[16 * SV_THREADS];
+    double m2s[SV_THREADS], m4s[SV_THREADS];
+    uint8_t ags[SV_THREADS];
+    uint32_t wtmp[17];
+};
+
+/* One 256-board tile of a layer: (board, empty cell) pairs are compacted per
+ * tile so every lane works on a pair; the FP64 sums of each board run over
+ * its cells in ascending order across the pair chunks. */
+__device__ __forceinline__ void solve_tile(const uint64_t *__restrict__ M, size_t n, size_t tile,
+                                           double *__restrict__ V, uint64_t *__restrict__ argmax,
+                                           const LayerView &t2, const LayerView &t4,
+                                           const WinRules &win, double p4,
+                                           const uint16_t *__restrict__ lut, int ff,
+                                           uint32_t &nlook, uint32_t &nhit, SolveSmem &sm)
+{
+    const double w2 = 1.0 - p4;
+    size_t bi = tile * SV_THREADS + threadIdx.x;
+    bool act = bi < n;
+    uint64_t b = act ? M[bi] : 0;
+    bool sat = act && win_satisfied(b, win);
+    uint64_t z = (act && !sat) ? cg_zmask(b) : 0;
+    uint32_t E = (uint32_t)cg_popc64(z);
+    sm.sb[threadIdx.x] = b;
+    uint32_t T;
+    uint32_t off = block_excl_scan<uint32_t>(E, &T, sm.wtmp);   /* also a barrier */
+    {
+        uint64_t zz = z;
+        uint32_t o = off;
+        while (zz) {
+            int x = next_cell(zz);
+            sm.pb[o] = (uint8_t)threadIdx.x;
+            sm.pc[o] = (uint8_t)x;
+            o++;
+        }
+    }
+    __syncthreads();
+    double a2 = 0.0, a4 = 0.0;
+    uint64_t am = 0;
+    for (uint32_t c0 = 0; c0 < T; c0 += SV_THREADS) {
+        uint32_t p = c0 + threadIdx.x;
+        if (p < T) {
+            uint64_t pbv = sm.sb[sm.pb[p]];
+            int x = sm.pc[p];
+            int g2, g4;
+            sm.m2s[threadIdx.x] = best_successor(cg_set_tile(pbv, x, 1), t2, lut, ff, &g2, nlook, nhit);
+            sm.m4s[threadIdx.x] = best_successor(cg_set_tile(pbv, x, 2), t4, lut, ff, &g4, nlook, nhit);
+            sm.ags[threadIdx.x] = (uint8_t)(g2 | (g4 << 2));
+        }
+        __syncthreads();
+        uint32_t q0 = off > c0 ? off : c0;
+        uint32_t q1 = off + E < c0 + SV_THREADS ? off + E : c0 + SV_THREADS;
+        for (uint32_t q = q0; q < q1; q++) {
+            a2 += sm.m2s[q - c0];
+            a4 += sm.m4s[q - c0];
+            am |= (uint64_t)sm.ags[q - c0] << (4 * sm.pc[q]);
+        }
+        __syncthreads();
+    }
+    if (act) {
+        double v;
+        if (sat) {
+            v = 1.0;
+        } else {
+            v = 0.0;
+            if (E) v = (w2 * a2 / (double)E) + (p4 * a4 / (double)E);
+            if (win.score && v == 0.0) v = (double)cg_tile_sum(b);
+        }
+        V[bi] = v;
+        if (argmax) argmax[bi] = am;
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ void add_stats(unsigned long long *stats, uint32_t nlook, uint32_t nhit)
+{
+    if (!stats) return;
+    unsigned long long l = nlook, h = nhit;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        l += __shfl_xor(l, d, 64);
+        h += __shfl_xor(h, d, 64);
+    }
+    if (lane_id() == 0 && l) {
+        atomicAdd(&stats[0], l);
+        atomicAdd(&stats[1], h);
+    }
+}
+
 __global__ __launch_bounds__(SV_THREADS) void k_solve(const uint64_t *__restrict__ M, size_t n,
                                                       double *__restrict__ V,
                                                       uint64_t *__restrict__ argmax, LayerView t2,
@@ -1073,84 +1164,11 @@ __global__ __launch_bounds__(SV_THREADS) void k_solve(const uint64_t *__restrict
                                                       const uint16_t *__restrict__ lut, int ff,
                                                       unsigned long long *stats)
 {
-    __shared__ uint64_t sb[SV_THREADS];
-    __shared__ uint8_t pb[16 * SV_THREADS], pc[16 * SV_THREADS];
-    __shared__ double m2s[SV_THREADS], m4s[SV_THREADS];
-    __shared__ uint8_t ags[SV_THREADS];
-    __shared__ uint32_t wtmp[17];
-    const double w2 = 1.0 - p4;
+    __shared__ SolveSmem sm;
     uint32_t nlook = 0, nhit = 0;
-    for (size_t tile = blockIdx.x; tile * SV_THREADS < n; tile += gridDim.x) {
-        size_t bi = tile * SV_THREADS + threadIdx.x;
-        bool act = bi < n;
-        uint64_t b = act ? M[bi] : 0;
-        bool sat = act && win_satisfied(b, win);
-        uint64_t z = (act && !sat) ? cg_zmask(b) : 0;
-        uint32_t E = (uint32_t)cg_popc64(z);
-        sb[threadIdx.x] = b;
-        uint32_t T;
-        uint32_t off = block_excl_scan<uint32_t>(E, &T, wtmp);   /* also a barrier */
-        {
-            uint64_t zz = z;
-            uint32_t o = off;
-            while (zz) {
-                int x = next_cell(zz);
-                pb[o] = (uint8_t)threadIdx.x;
-                pc[o] = (uint8_t)x;
-                o++;
-            }
-        }
-        __syncthreads();
-        double a2 = 0.0, a4 = 0.0;
-        uint64_t am = 0;
-        for (uint32_t c0 = 0; c0 < T; c0 += SV_THREADS) {
-            uint32_t p = c0 + threadIdx.x;
-            if (p < T) {
-                uint64_t pbv = sb[pb[p]];
-                int x = pc[p];
-                int g2, g4;
-                m2s[threadIdx.x] = best_successor(cg_set_tile(pbv, x, 1), t2, lut, ff, &g2, nlook,
-                                                  nhit);
-                m4s[threadIdx.x] = best_successor(cg_set_tile(pbv, x, 2), t4, lut, ff, &g4, nlook,
-                                                  nhit);
-                ags[threadIdx.x] = (uint8_t)(g2 | (g4 << 2));
-            }
-            __syncthreads();
-            uint32_t q0 = off > c0 ? off : c0;
-            uint32_t q1 = off + E < c0 + SV_THREADS ? off + E : c0 + SV_THREADS;
-            for (uint32_t q = q0; q < q1; q++) {
-                a2 += m2s[q - c0];
-                a4 += m4s[q - c0];
-                am |= (uint64_t)ags[q - c0] << (4 * pc[q]);
-            }
-            __syncthreads();
-        }
-        if (act) {
-            double v;
-            if (sat) {
-                v = 1.0;
-            } else {
-                v = 0.0;
-                if (E) v = (w2 * a2 / (double)E) + (p4 * a4 / (double)E);
-                if (win.score && v == 0.0) v = (double)cg_tile_sum(b);
-            }
-            V[bi] = v;
-            if (argmax) argmax[bi] = am;
-        }
-        __syncthreads();
-    }
-    if (stats) {
-        unsigned long long l = nlook, h = nhit;
-#pragma unroll
-        for (int d = 32; d >= 1; d >>= 1) {
-            l += __shfl_xor(l, d, 64);
-            h += __shfl_xor(h, d, 64);
-        }
-        if (lane_id() == 0 && l) {
-            atomicAdd(&stats[0], l);
-            atomicAdd(&stats[1], h);
-        }
-    }
+    for (size_t tile = blockIdx.x; tile * SV_THREADS < n; tile += gridDim.x)
+        solve_tile(M, n, tile, V, argmax, t2, t4, win, p4, lut, ff, nlook, nhit, sm);
+    add_stats(stats, nlook, nhit);
 }
 
 void launch_solve(hipStream_t st, const uint64_t *M, size_t n, double *V, uint64_t *argmax,
@@ -1260,6 +1278,285 @@ void launch_adjoint_init(hipStream_t st, const uint64_t *init, size_t n, LayerVi
 {
     if (!n) return;
     hipLaunchKernelGGL(k_adjoint_init, dim3(1), dim3(256), 0, st, init, n, t0, occ0, lut, ff, P);
+}
+
+/* ------------------------------------------------------------------------ */
+/* batched tables: composite keys (table id << 48 | packed free cells)       */
+/* ------------------------------------------------------------------------ */
+/* A table's canonical boards keep their formation cells (all 0xF) at fixed
+ * nibbles, so the free cells alone identify a board; packing them
+ * (lowest tile index most significant) keeps the key order within a table,
+ * and the table id above them keeps tables apart: one sort / dedupe serves
+ * every table of a batch at once. */
+
+__device__ __forceinline__ uint64_t benc(uint64_t b, const BatchTable &T, uint64_t tid, unsigned *err)
+{
+    if ((b & T.fnib) != T.fnib) atomicOr(err, 2u);   /* formation left its canonical place */
+    uint64_t c = 0;
+    for (int j = 0; j < T.nfree; j++) c = (c << 4) | ((b >> (60 - 4 * T.pos[j])) & 0xf);
+    return (tid << BATCH_TID_SHIFT) | c;
+}
+
+__device__ __forceinline__ uint64_t bdec(uint64_t ck, const BatchTable &T)
+{
+    uint64_t c = ck & ((1ULL << BATCH_TID_SHIFT) - 1);
+    uint64_t b = T.fnib;
+    for (int j = T.nfree - 1; j >= 0; j--) {
+        b |= (c & 0xf) << (60 - 4 * T.pos[j]);
+        c >>= 4;
+    }
+    return b;
+}
+
+__global__ __launch_bounds__(MV_THREADS) void k_bmove(const uint64_t *__restrict__ in, size_t n,
+                                                      uint64_t *__restrict__ out,
+                                                      unsigned long long *stats,
+                                                      const uint16_t *__restrict__ glut, MoveRules r,
+                                                      const BatchTable *__restrict__ tabs, int step,
+                                                      unsigned *err)
+{
+    __shared__ uint16_t lut[65536];
+    __shared__ uint32_t wtmp[17];
+    __shared__ unsigned long long base_sh;
+    __shared__ uint64_t ro[16], ra[16];
+    uint64_t kor = 0, kand = ~0ULL;
+    {
+        const uint4 *g = reinterpret_cast<const uint4 *>(glut);
+        uint4 *l = reinterpret_cast<uint4 *>(lut);
+        for (int i = threadIdx.x; i < 65536 / 8; i += MV_THREADS) l[i] = g[i];
+    }
+    __syncthreads();
+    const size_t stride = (size_t)gridDim.x * MV_THREADS;
+    for (size_t b0 = (size_t)blockIdx.x * MV_THREADS; b0 < n; b0 += stride) {
+        size_t i = b0 + threadIdx.x;
+        uint64_t res[4];
+        uint32_t k = 0;
+        if (i < n) {
+            uint64_t ck = in[i];
+            uint64_t tid = ck >> BATCH_TID_SHIFT;
+            const BatchTable &T = tabs[tid];
+            if (step < T.nsteps) {
+                uint64_t b = bdec(ck, T);
+                if (!(r.prune && cg_prune(b, r.stsl, r.ltc, r.sl))) {
+                    uint64_t y = cg_rot_cw(b);
+#pragma unroll
+                    for (int d = 0; d < 4; d++) {
+                        uint64_t t = d < 2 ? b : y;
+                        if (cg_move_hori_tab(lut, &t, d == 1 || d == 2, r.ff) &&
+                            !(r.prune && cg_prune(t, r.stsl, r.ltc, r.sl)))
+                            res[k++] = benc(cg_canon(t), T, tid, err);
+                    }
+                }
+            }
+        }
+        for (uint32_t j = 0; j < k; j++) kor |= res[j], kand &= res[j];
+        uint32_t tot;
+        uint32_t off = block_excl_scan<uint32_t>(k, &tot, wtmp);
+        if (threadIdx.x == 0) base_sh = tot ? atomicAdd(&stats[0], (unsigned long long)tot) : 0;
+        __syncthreads();
+        unsigned long long base = base_sh;
+        for (uint32_t j = 0; j < k; j++) out[base + off + j] = res[j];
+        __syncthreads();
+    }
+    block_or_and(kor, kand, &stats[1], &stats[2], ro, ra);
+}
+
+__global__ __launch_bounds__(SP_THREADS) void k_bspawn(const uint64_t *__restrict__ in, size_t n,
+                                                       uint64_t *__restrict__ out2,
+                                                       uint64_t *__restrict__ out4,
+                                                       unsigned long long *stats,
+                                                       const BatchTable *__restrict__ tabs, int step,
+                                                       unsigned *err)
+{
+    __shared__ uint32_t wtmp[17];
+    __shared__ unsigned long long base_sh;
+    __shared__ uint64_t ro[16], ra[16];
+    uint64_t or2 = 0, and2 = ~0ULL, or4 = 0, and4 = ~0ULL;
+    const size_t stride = (size_t)gridDim.x * SP_THREADS;
+    for (size_t b0 = (size_t)blockIdx.x * SP_THREADS; b0 < n; b0 += stride) {
+        size_t i = b0 + threadIdx.x;
+        uint64_t b = 0, z = 0, tid = 0;
+        if (i < n) {
+            uint64_t ck = in[i];
+            tid = ck >> BATCH_TID_SHIFT;
+            /* the spawns feed step + 1 */
+            if (step + 1 < tabs[tid].nsteps) {
+                b = bdec(ck, tabs[tid]);
+                z = cg_zmask(b);
+            }
+        }
+        uint32_t e = (uint32_t)cg_popc64(z);
+        uint32_t tot;
+        uint32_t off = block_excl_scan<uint32_t>(e, &tot, wtmp);
+        if (threadIdx.x == 0) base_sh = tot ? atomicAdd(&stats[0], (unsigned long long)tot) : 0;
+        __syncthreads();
+        size_t o = base_sh + off;
+        while (z) {
+            int x = next_cell(z);
+            const BatchTable &T = tabs[tid];
+            uint64_t c2 = benc(cg_canon(cg_set_tile(b, x, 1)), T, tid, err);
+            uint64_t c4 = benc(cg_canon(cg_set_tile(b, x, 2)), T, tid, err);
+            out2[o] = c2;
+            out4[o] = c4;
+            or2 |= c2;
+            and2 &= c2;
+            or4 |= c4;
+            and4 &= c4;
+            o++;
+        }
+        __syncthreads();
+    }
+    block_or_and(or2, and2, &stats[1], &stats[2], ro, ra);
+    block_or_and(or4, and4, &stats[3], &stats[4], ro, ra);
+}
+
+/* composite -> boards; per-table count, OR and AND of the boards
+ * (tstat[3 t + 0..2]); keys are sorted, so a wave whose lanes share one
+ * table reduces first and adds once */
+__global__ __launch_bounds__(256) void k_bdecode(const uint64_t *__restrict__ in, size_t n,
+                                                 uint64_t *__restrict__ out,
+                                                 const BatchTable *__restrict__ tabs,
+                                                 unsigned long long *tstat)
+{
+    size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    bool valid = i < n;
+    uint64_t tid = ~0ULL, b = 0;
+    if (valid) {
+        uint64_t ck = in[i];
+        tid = ck >> BATCH_TID_SHIFT;
+        b = bdec(ck, tabs[tid]);
+        out[i] = b;
+    }
+    uint64_t t0 = __shfl(tid, 0, 64);
+    bool same = __ballot(valid && tid == t0) == __ballot(valid) && __ballot(valid) == ~0ULL;
+    if (same) {
+        uint64_t o = b, a = b;
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) {
+            o |= __shfl_xor(o, d, 64);
+            a &= __shfl_xor(a, d, 64);
+        }
+        if (lane_id() == 0) {
+            atomicAdd(&tstat[3 * t0], 64ULL);
+            atomicOr(&tstat[3 * t0 + 1], (unsigned long long)o);
+            atomicAnd(&tstat[3 * t0 + 2], (unsigned long long)a);
+        }
+    } else if (valid) {
+        atomicAdd(&tstat[3 * tid], 1ULL);
+        atomicOr(&tstat[3 * tid + 1], (unsigned long long)b);
+        atomicAnd(&tstat[3 * tid + 2], (unsigned long long)b);
+    }
+}
+
+/* per-table bucket counts into one concatenated array (table t's buckets
+ * start at plan[t].dir_base); keys are a concatenation of sorted segments */
+__global__ __launch_bounds__(256) void k_bdir_count(const uint64_t *__restrict__ keys, size_t n,
+                                                    const size_t *__restrict__ seg, int ntab,
+                                                    const BatchIndexPlan *__restrict__ plan,
+                                                    uint32_t *cnt)
+{
+    size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    bool valid = i < n;
+    uint64_t slot = ~0ULL;
+    if (valid) {
+        int lo = 0, hi = ntab;   /* last t with seg[t] <= i */
+        while (hi - lo > 1) {
+            int mid = (lo + hi) >> 1;
+            if (seg[mid] <= i) lo = mid;
+            else hi = mid;
+        }
+        const BatchIndexPlan &p = plan[lo];
+        slot = p.dir_base + (xform(keys[i], p.x) >> p.shift);
+    }
+    uint64_t sp = __shfl_up(slot, 1, 64);
+    int lane = lane_id();
+    bool start = valid && (lane == 0 || sp != slot);
+    uint64_t starts = __ballot(start);
+    uint64_t vmask = __ballot(valid);
+    if (start) {
+        uint64_t endmask = (starts | ~vmask) & ~((2ULL << lane) - 1ULL);
+        int end = endmask ? __ffsll((long long)endmask) - 1 : 64;
+        atomicAdd(&cnt[slot], (uint32_t)(end - lane));
+    }
+}
+
+void launch_bmove(hipStream_t st, const uint64_t *in, size_t n, uint64_t *out,
+                  unsigned long long *stats, const uint16_t *lut, MoveRules r, const BatchTable *tabs,
+                  int step, unsigned *err)
+{
+    if (!n) return;
+    size_t blocks = (n + MV_THREADS - 1) / MV_THREADS;
+    if (blocks > 512) blocks = 512;
+    hipLaunchKernelGGL(k_bmove, dim3((unsigned)blocks), dim3(MV_THREADS), 0, st, in, n, out, stats,
+                       lut, r, tabs, step, err);
+}
+
+void launch_bspawn(hipStream_t st, const uint64_t *in, size_t n, uint64_t *out2, uint64_t *out4,
+                   unsigned long long *stats, const BatchTable *tabs, int step, unsigned *err)
+{
+    if (!n) return;
+    size_t blocks = (n + SP_THREADS - 1) / SP_THREADS;
+    if (blocks > 1024) blocks = 1024;
+    hipLaunchKernelGGL(k_bspawn, dim3((unsigned)blocks), dim3(SP_THREADS), 0, st, in, n, out2, out4,
+                       stats, tabs, step, err);
+}
+
+void launch_bdecode(hipStream_t st, const uint64_t *in, size_t n, uint64_t *out,
+                    const BatchTable *tabs, unsigned long long *tstat)
+{
+    if (!n) return;
+    hipLaunchKernelGGL(k_bdecode, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, in, n, out,
+                       tabs, tstat);
+}
+
+void launch_bdir(hipStream_t st, const uint64_t *keys, size_t n, const size_t *seg, int ntab,
+                 const BatchIndexPlan *plan, size_t total_entries, uint32_t *dir, void *scratch)
+{
+    size_t m = total_entries;
+    uint32_t *cnt = (uint32_t *)scratch;
+    uint32_t *sums = cnt + ((m + 2 + 63) / 64) * 64;
+    launch_fill_u64(st, (uint64_t *)cnt, (m + 1) / 2 + 1, 0);
+    if (n)
+        hipLaunchKernelGGL(k_bdir_count, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, keys, n,
+                           seg, ntab, plan, cnt);
+    size_t nb = (m + DS_TILE - 1) / DS_TILE;
+    hipLaunchKernelGGL(k_dscan_reduce, dim3((unsigned)nb), dim3(DS_THREADS), 0, st, cnt, m, sums);
+    hipLaunchKernelGGL(k_dscan_top, dim3(1), dim3(DS_THREADS), 0, st, sums, nb);
+    hipLaunchKernelGGL(k_dscan_down, dim3((unsigned)nb), dim3(DS_THREADS), 0, st, cnt, m, sums, dir,
+                       (uint32_t)n);
+}
+
+/* batched solve: block b works on table t with desc[t].tile0 <= b <
+ * desc[t+1].tile0 */
+__global__ __launch_bounds__(SV_THREADS) void k_bsolve(const BSolveDesc *__restrict__ desc, int ntab,
+                                                       double p4, const uint16_t *__restrict__ lut,
+                                                       int ff)
+{
+    __shared__ SolveSmem sm;
+    __shared__ int t_sh;
+    if (threadIdx.x == 0) {
+        int lo = 0, hi = ntab;
+        while (hi - lo > 1) {
+            int mid = (lo + hi) >> 1;
+            if (desc[mid].tile0 <= blockIdx.x) lo = mid;
+            else hi = mid;
+        }
+        t_sh = lo;
+    }
+    __syncthreads();
+    const BSolveDesc &d = desc[t_sh];
+    uint32_t nlook = 0, nhit = 0;
+    size_t tile = blockIdx.x - d.tile0;
+    if (tile * SV_THREADS < d.n)
+        solve_tile(d.M, d.n, tile, d.V, d.arg, d.t2, d.t4, d.win, p4, lut, ff, nlook, nhit, sm);
+}
+
+void launch_bsolve(hipStream_t st, const BSolveDesc *desc, int ntab, size_t total_tiles, double p4,
+                   const uint16_t *lut, int ff)
+{
+    if (!total_tiles) return;
+    hipLaunchKernelGGL(k_bsolve, dim3((unsigned)total_tiles), dim3(SV_THREADS), 0, st, desc, ntab, p4,
+                       lut, ff);
 }
 
 } // namespace cg

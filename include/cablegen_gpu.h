@@ -80,6 +80,30 @@ int cg_tb_layer_device(const cg_tablebase *tb, int idx, const uint64_t **keys,
  * (src/generate.c:74-87, src/solve.c:29-45); either fmt may be NULL */
 int cg_tb_write_files(const cg_tablebase *tb, const char *boards_fmt, const char *tables_fmt);
 
+/* ---------------- batched tables ---------------- */
+
+/* Many small tables generated and solved together: every layer step runs one
+ * set of kernels for all tables (keys carry the table id), and the solve's
+ * downward sweep is one kernel per step, captured in a hipGraph.  Tables
+ * share the rules (free_formation must be false) and may differ in
+ * formation (at most 12 free cells), initial boards, tile-sum range and
+ * winstates; each is identical to the same table run alone. */
+typedef struct cg_batch cg_batch;
+cg_batch *cg_batch_create(const cg_rules *rules);
+void cg_batch_free(cg_batch *b);
+/* adds a table (initial boards and unexpanded winstates in host memory);
+ * returns its index or -1 */
+int cg_batch_add(cg_batch *b, const uint64_t *initial, size_t n, int start, int end,
+                 const uint64_t *winstates, size_t n_win);
+int cg_batch_generate(cg_batch *b);
+/* use_graph: replay the captured sweep (captured on first use per p4) */
+int cg_batch_solve(cg_batch *b, double p4, int use_graph);
+int cg_batch_num_tables(const cg_batch *b);
+int cg_batch_num_layers(const cg_batch *b, int table);
+size_t cg_batch_layer_size(const cg_batch *b, int table, int layer);
+size_t cg_batch_total_boards(const cg_batch *b);
+int cg_batch_copy_layer(const cg_batch *b, int table, int layer, uint64_t *keys, double *values);
+
 /* ---------------- device primitives (device pointers) ---------------- */
 
 /* move + lock + prune + canonicalise of n boards into out (capacity >= 4n);

@@ -411,3 +411,47 @@ def test_reference_cli_on_library(cg, tmp_path):
         k, v = O.read_table_file(f"{d}/t/{s}.tables")
         assert len(k) == n and "%016x" % O.hash_u64(k) == hk, s
         assert "%016x" % O.hash_u64(v.view(np.uint64)) == hv, s
+
+
+# ---------------------------------------------------------------- batched tables
+
+def test_batch_matches_single_tables(cg):
+    from cablegen_amd.batch_configs import batch
+    cfgs = batch(12, lo=3, hi=12)
+    r = cg.rules(ignore_f=True)
+    B = cg.Batch(r)
+    for c in cfgs:
+        B.add(c["initial"], c["start"], c["end"], c["winstates"])
+    B.generate()
+    B.solve(use_graph=False)
+    eager = [[B.values(t, i) for i in range(B.num_layers(t))] for t in range(len(cfgs))]
+    for t, c in enumerate(cfgs):
+        tb = cg.Tablebase(r)
+        tb.generate(c["initial"], c["start"], c["end"])
+        tb.solve(c["winstates"])
+        assert B.num_layers(t) == tb.num_layers
+        for i in range(tb.num_layers):
+            assert np.array_equal(B.keys(t, i), tb.keys(i)), (t, i)
+            assert np.array_equal(eager[t][i].view(np.uint64), tb.values(i).view(np.uint64)), (t, i)
+        tb.close()
+        if t < 3:   # and against the oracle
+            cfg = O.make_cfg(cores=8, ignore_f=True)
+            layers, vals = oracle_table(cfg, c["initial"], c["winstates"], c["start"], c["end"])
+            for i in range(len(layers)):
+                assert np.array_equal(B.keys(t, i), layers[i])
+                assert np.array_equal(eager[t][i].view(np.uint64), vals[i].view(np.uint64))
+    # graph replays, with eager work and synchronisation between them
+    for rep in range(3):
+        B.solve(use_graph=True)
+        for t in range(len(cfgs)):
+            for i in range(B.num_layers(t)):
+                assert np.array_equal(B.values(t, i).view(np.uint64), eager[t][i].view(np.uint64))
+        B.solve(use_graph=False)
+    B.close()
+
+
+def test_batch_rejects_bad_tables(cg):
+    B = cg.Batch(cg.rules(ignore_f=True))
+    with pytest.raises(cg.CablegenError):
+        B.add([0x1200000000000000], 4, 20, [0x5000000000000000])   # 16 free cells
+    B.close()
