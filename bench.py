@@ -98,6 +98,7 @@ def micro(args):
     out = {"metric": "kernel roofline microbench (HBM GB/s per kernel)", "n_boards": n,
            "queries": nq, "seed": 1, "formation_mask": "0x00000000FFF0FFF0",
            "rules": "ignore_f, no prune", "kernels": {}}
+    out["tuning"] = run_autotune(cg, args.autotune)
 
     def timed(name, fn):
         cg.prof_enable(True)
@@ -180,13 +181,28 @@ def micro(args):
     print(json.dumps(out))
 
 
+def run_autotune(cg, effort):
+    """Autotuner picks (untimed, before the warmup); effort < 0 keeps the defaults."""
+    if effort < 0:
+        cg.tune_set("reset", 0)
+        return {"autotune": "off (defaults: 8-bit digits, 8192-key tiles, window 16, dir delta 0)"}
+    t0 = time.perf_counter()
+    rep = cg.autotune(effort)
+    sel = {}
+    for cls, name in enumerate(("n<2^20", "n<2^23", "n>=2^23")):
+        sel[name] = {nb: rep["sort"][cls]["pick"][nb] for nb in ("24", "32", "36", "40", "44", "48", "54")}
+    return {"autotune_s": time.perf_counter() - t0, "effort": effort,
+            "dir_delta": rep["dir"]["pick"], "sort_picks_bits_x_kpt/window": sel}
+
+
 def batch_bench(args):
     """Batching: 64 small tables (cablegen_amd.batch_configs, seeds 0..63)
     generated and solved together vs one at a time on the same GPU."""
     import cablegen_amd as cg
     from cablegen_amd.batch_configs import batch
-    cfgs = batch(args.batch_tables)
+    cfgs = batch(args.batch_tables, args.batch_lo, args.batch_hi)
     r = cg.rules(ignore_f=True)
+    tuning = run_autotune(cg, args.autotune)
 
     tb = cg.Tablebase(r)   # one device-resident table reused: its arena is kept
 
@@ -228,8 +244,10 @@ def batch_bench(args):
            "speedup": t_one / (tg + ts),
            "largest_tables": sizes[:5], "median_table": sizes[len(sizes) // 2],
            "batched_kernels_ms": {k: v[1] for k, v in kern.items()},
-           "workload": "seeds 0..63: formation in {2x4, 3x4, DPDGap}, 2-4 tiles of 2^1..2^7, "
-                       "goal 2^(max+2) top-left, end = start + 2*U{10..40}, ignore_f, no prune"}
+           "tuning": tuning,
+           "workload": "seeds 0..%d: formation in {2x4, 3x4, DPDGap}, 2-4 tiles of 2^1..2^7, "
+                       "goal 2^(max+2) top-left, end = start + 2*U{%d..%d}, ignore_f, no prune"
+                       % (len(cfgs) - 1, args.batch_lo, args.batch_hi)}
     B.close()
     tb.close()
     print(json.dumps(out))
@@ -246,6 +264,10 @@ def main():
     ap.add_argument("--micro-log2", type=int, default=30, help="log2 of the microbench board count")
     ap.add_argument("--batch", action="store_true", help="batched-tables benchmark instead")
     ap.add_argument("--batch-tables", type=int, default=64)
+    ap.add_argument("--batch-lo", type=int, default=10, help="end = start + 2*U{lo..hi}")
+    ap.add_argument("--batch-hi", type=int, default=40)
+    ap.add_argument("--autotune", type=int, default=0,
+                    help="autotuner effort before timing (0 quick, 1 full, -1 off)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -263,6 +285,7 @@ def main():
     r = cg.rules(**RULES)
     tb = cg.Tablebase(r)
     start = cg.lib().get_sum(INITIAL[0])
+    tuning = run_autotune(cg, args.autotune)
 
     def step():
         t0 = time.perf_counter()
@@ -311,6 +334,7 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "avg_launch_us": avg_ms * 1e3, "avg_algorithmic_bytes": avg_bytes},
+        "tuning": tuning,
         "kernels": {k: {"launches": v["launches"] // max(args.steps, 1),
                         "ms_per_step": v["ms"] / args.steps,
                         "GB_s": (v["bytes"] / v["ms"] / 1e6) if v["ms"] else 0.0}

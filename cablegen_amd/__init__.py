@@ -107,6 +107,10 @@ def lib():
     L.cg_batch_copy_layer.argtypes = [C.c_void_p, C.c_int, C.c_int, U64P, F64P]
     L.cg_tune_set.argtypes = [C.c_char_p, C.c_int]
     L.cg_tune_get.argtypes = [C.c_char_p]
+    L.cg_tune_sort_entry.argtypes = [C.c_int] * 5
+    L.cg_tune_sort_query.argtypes = [C.c_size_t, C.c_int, C.POINTER(C.c_int),
+                                     C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.cg_autotune.argtypes = [C.c_int, C.c_char_p, C.c_size_t]
     L.cg_prof_enable.argtypes = [C.c_int]
     L.cg_prof_query.restype = C.c_long
     L.cg_prof_query.argtypes = [C.c_char_p, F64P, F64P]
@@ -243,6 +247,24 @@ def tune_set(knob, value):
 
 def tune_get(knob):
     return lib().cg_tune_get(knob.encode())
+
+
+def tune_sort_query(n, nbits):
+    """(digit bits, keys per thread, look-back window) a sort of n keys with
+    nbits varying bits uses."""
+    b, k, w = C.c_int(), C.c_int(), C.c_int()
+    _check(lib().cg_tune_sort_query(int(n), int(nbits), C.byref(b), C.byref(k), C.byref(w)),
+           "cg_tune_sort_query")
+    return b.value, k.value, w.value
+
+
+def autotune(effort=0):
+    """Runs the autotuner (sort configuration per size class and key width,
+    directory depth) on the GPU, installs its picks and returns its report."""
+    import json
+    buf = C.create_string_buffer(1 << 16)
+    _check(lib().cg_autotune(int(effort), buf, len(buf)), "cg_autotune")
+    return json.loads(buf.value.decode())
 
 
 class Batch:

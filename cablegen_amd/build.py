@@ -37,7 +37,7 @@ def _run(cmd, cwd=ROOT):
 def build_lib():
     os.makedirs(BUILD, exist_ok=True)
     objs = []
-    for src in ("kernels.hip", "engine.hip"):
+    for src in ("kernels.hip", "sort.hip", "engine.hip"):
         obj = os.path.join(BUILD, src.replace(".hip", ".o"))
         _run([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)

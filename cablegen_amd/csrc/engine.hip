@@ -11,6 +11,7 @@
  */
 #include <hip/hip_runtime.h>
 #include <algorithm>
+#include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -19,6 +20,7 @@
 #include "board.h"
 #include "engine.h"
 #include "kernels.h"
+#include "devutil.h"
 #include "../../include/cablegen_gpu.h"
 
 using namespace cg;
@@ -76,8 +78,10 @@ struct Engine {
     hipStream_t st = nullptr;
     uint16_t *lut[2] = { nullptr, nullptr };     /* left-move table per ignore_f */
     unsigned long long *d_cnt = nullptr;          /* device counters [8] */
-    unsigned long long *h_cnt = nullptr;          /* pinned mirror */
-    unsigned *d_err = nullptr, *h_err = nullptr;  /* sort wait-budget flag */
+    unsigned long long *h_cnt = nullptr;          /* host copy of the counters */
+    unsigned *d_err = nullptr;                    /* sort wait-budget / batch formation flags */
+    unsigned long long *h_pub = nullptr;          /* pinned, mapped: counters [0..8), flags [8] */
+    unsigned long long *hd_pub = nullptr;         /* its device address */
     Buf A, B, Cb;                                 /* candidate / sort buffers */
     void *scratch = nullptr;
     size_t scratch_cap = 0;
@@ -91,11 +95,14 @@ static int eng_init()
     if (E.init) return 0;
     HC(hipStreamCreate(&E.st));
     HC(hipMalloc((void **)&E.d_cnt, 8 * sizeof(unsigned long long)));
-    HC(hipHostMalloc((void **)&E.h_cnt, 8 * sizeof(unsigned long long), hipHostMallocDefault));
+    E.h_cnt = (unsigned long long *)calloc(8, sizeof(unsigned long long));
+    if (!E.h_cnt) return fail_msg("host allocation failed");
     HC(hipMalloc((void **)&E.d_err, 64));
-    HC(hipHostMalloc((void **)&E.h_err, 64, hipHostMallocDefault));
     HC(hipMemset(E.d_err, 0, 64));
-    *E.h_err = 0;
+    HC(hipHostMalloc((void **)&E.h_pub, 16 * sizeof(unsigned long long),
+                     hipHostMallocMapped | hipHostMallocCoherent));
+    memset(E.h_pub, 0, 16 * sizeof(unsigned long long));
+    HC(hipHostGetDevicePointer((void **)&E.hd_pub, E.h_pub, 0));
     E.init = true;
     return 0;
 }
@@ -103,6 +110,8 @@ static int eng_init()
 static int ensure(Buf &b, size_t n)
 {
     if (n <= b.cap && b.p) return 0;
+    /* work enqueued on the stream may still use the buffer */
+    if (b.p && E.st) HC(hipStreamSynchronize(E.st));
     if (b.p) HC(hipFree(b.p));
     b.p = nullptr;
     size_t c = std::max<size_t>(n + n / 4, 1024);
@@ -114,6 +123,8 @@ static int ensure(Buf &b, size_t n)
 static int ensure_scratch_bytes(size_t need)
 {
     if (need <= E.scratch_cap && E.scratch) return 0;
+    /* sorts or index builds enqueued on the stream may still use it */
+    if (E.scratch) HC(hipStreamSynchronize(E.st));
     if (E.scratch) HC(hipFree(E.scratch));
     E.scratch = nullptr;
     size_t c = need + need / 4 + 4096;
@@ -126,6 +137,7 @@ static int ensure_scratch(size_t n)
 {
     size_t need = sort_scratch_bytes(n);
     if (need <= E.scratch_cap && E.scratch) return 0;
+    if (E.scratch) HC(hipStreamSynchronize(E.st));
     if (E.scratch) HC(hipFree(E.scratch));
     E.scratch = nullptr;
     size_t c = need + need / 4 + 4096;
@@ -347,13 +359,13 @@ static MoveRules move_rules(const cg_rules &r)
  * sort error flag */
 static int read_counters(int lo, int n)
 {
-    HC(hipMemcpyAsync(E.h_cnt + lo, E.d_cnt + lo, n * sizeof(unsigned long long),
-                      hipMemcpyDeviceToHost, E.st));
-    HC(hipMemcpyAsync(E.h_err, E.d_err, sizeof(unsigned), hipMemcpyDeviceToHost, E.st));
+    /* one small kernel stores the counters and the flags into mapped host
+     * memory (instead of two copy dispatches) */
+    launch_publish(E.st, E.d_cnt, lo, n, E.d_err, E.hd_pub);
     if (check_sync(E.st, "counters")) return -1;
-    if (*E.h_err) {
-        unsigned e = *E.h_err;
-        *E.h_err = 0;
+    for (int i = lo; i < lo + n; i++) E.h_cnt[i] = ((volatile unsigned long long *)E.h_pub)[i];
+    unsigned e = (unsigned)((volatile unsigned long long *)E.h_pub)[8];
+    if (e) {
         hipMemsetAsync(E.d_err, 0, sizeof(unsigned), E.st);
         hipStreamSynchronize(E.st);
         if (e & 2u) return fail_msg("batch: a board's formation cells left the table's canonical place");
@@ -394,13 +406,56 @@ static int nonzero_nibbles(uint64_t x)
     return c;
 }
 
+/* Counts of P_{s+2} and C' that are still on the device (d_cnt[5], d_cnt[6])
+ * when a step returns: the next step's move kernel reads the P count there,
+ * and its first synchronisation brings both to the host. */
+struct StepCounts {
+    bool pending = false;
+    size_t n2 = 0, sumE = 0;   /* input sizes of the two sorts (upper bounds of the counts) */
+    int hu2 = -1, hu4 = -1;    /* timer handles of the two compactions, bytes set on arrival */
+    size_t last_nM = 0;        /* previous layer size: sizes the spawn buffers */
+};
+
+/* spawn buffers are sized before |M_s| is known: for about twice the previous
+ * layer (never more than the candidate count); a step whose spawns do not fit
+ * is repeated with exact buffers */
+static int g_spawn_est_mode = 0;   /* 1: undersized on purpose (exercises the repeat) */
+
+static size_t spawn_rows_estimate(const StepCounts &pc, size_t ncand)
+{
+    if (g_spawn_est_mode == 1) return std::min(ncand, pc.last_nM / 2 + 1);
+    return std::min(ncand, 2 * pc.last_nM + 65536);
+}
+
+static void counts_arrived(StepCounts &pc, size_t nP, size_t nC)
+{
+    prof_set_bytes(pc.hu2, 8.0 * pc.n2 + 8.0 * nP);
+    prof_set_bytes(pc.hu4, 8.0 * pc.sumE + 8.0 * nC);
+    pc.pending = false;
+}
+
+/* brings pending counts to the host (end of a generation) */
+static int counts_finish(StepCounts &pc, size_t *nP, size_t *nC)
+{
+    if (!pc.pending) return 0;
+    if (read_counters(5, 2)) return -1;
+    if (nP) *nP = E.h_cnt[5];
+    if (nC) *nC = E.h_cnt[6];
+    counts_arrived(pc, E.h_cnt[5], E.h_cnt[6]);
+    return 0;
+}
+
 /* One layer step (generate.c:409-550): P (pre-move layer) and the carried
  * spawn-4 set C give M_s (from `la`), P_{s+2} and the new carry (written to
- * nxtP / nxtC).  Four host synchronisations: after the moves, after M_s,
- * after the spawns, after P_{s+2} and C'. */
+ * nxtP / nxtC).  Two host synchronisations: after the moves (which also
+ * brings the previous step's pending counts), and after the spawns (which
+ * brings |M_s|).  When pc.pending, nP and nC are upper bounds and the exact
+ * counts are on the device; on return *nP2 / *nC2 are upper bounds and
+ * pc.pending is set. */
 static int gen_step(const cg_rules &r, const uint64_t *P, size_t nP, const uint64_t *C, size_t nC,
-                    Carry &carry, LayerAlloc &la, uint64_t **M, size_t *nM, Buf &nxtP, size_t *nP2,
-                    Buf &nxtC, size_t *nC2, uint64_t *m_or = nullptr, uint64_t *m_and = nullptr)
+                    StepCounts &pc, Carry &carry, LayerAlloc &la, uint64_t **M, size_t *nM,
+                    Buf &nxtP, size_t *nP2, Buf &nxtC, size_t *nC2, uint64_t *m_or = nullptr,
+                    uint64_t *m_and = nullptr)
 {
     const uint16_t *lut = get_lut(r.ignore_f);
     if (!lut) return fail_msg("row table upload failed");
@@ -408,48 +463,63 @@ static int gen_step(const cg_rules &r, const uint64_t *P, size_t nP, const uint6
     if (ensure(E.A, 4 * nP + 1)) return -1;
     launch_init_stats(E.st, E.d_cnt);
     int h = prof_begin(PK_MOVE);
-    launch_move(E.st, P, nP, E.A.p, E.d_cnt, lut, move_rules(r));
+    launch_move(E.st, P, nP, pc.pending ? E.d_cnt + 5 : nullptr, E.A.p, E.d_cnt, lut, move_rules(r));
     prof_end(h, 0);
-    if (read_counters(0, 3)) return -1;
+    if (read_counters(0, pc.pending ? 7 : 3)) return -1;
+    if (pc.pending) {
+        nP = E.h_cnt[5];
+        nC = E.h_cnt[6];
+        counts_arrived(pc, nP, nC);
+    }
     size_t ncand = E.h_cnt[0];
     uint64_t mor = E.h_cnt[1], mand = E.h_cnt[2];
     if (m_or) *m_or = mor;
     if (m_and) *m_and = mand;
     prof_set_bytes(h, 8.0 * nP + 8.0 * ncand);
-    /* M_s = uniq(candidates) */
+    /* M_s = uniq(candidates); its count goes to d_cnt[7] */
     *M = la.alloc(ncand);
     if (!*M) return fail_msg("layer allocation failed");
     int hu;
-    if (sort_enqueue(E.A.p, ncand, vary_of(ncand, mor, mand), *M, 5, &hu)) return -1;
-    if (read_counters(5, 1)) return -1;
-    *nM = E.h_cnt[5];
+    if (sort_enqueue(E.A.p, ncand, vary_of(ncand, mor, mand), *M, 7, &hu)) return -1;
+    /* spawns: every board has at most 16 - (cells nonzero in all of M_s)
+     * empties; the kernel reads |M_s| from the device, the grid and the
+     * buffers cover ncand */
+    size_t emax = (size_t)(16 - nonzero_nibbles(ncand ? mand : ~0ULL));
+    size_t cap = emax * spawn_rows_estimate(pc, ncand);
+    int hs = -1;
+    size_t sumE = 0;
+    for (int attempt = 0; attempt < 2; attempt++) {
+        if (ensure(E.A, nC + cap + 1) || ensure(E.Cb, cap + 1)) return -1;
+        if (nC) HC(hipMemcpyAsync(E.A.p, C, nC * sizeof(uint64_t), hipMemcpyDeviceToDevice, E.st));
+        launch_init_stats(E.st, E.d_cnt);
+        hs = prof_begin(PK_SPAWN);
+        launch_spawn(E.st, *M, ncand, E.d_cnt + 7, E.A.p + nC, E.Cb.p, cap, E.d_cnt);
+        prof_end(hs, 0);
+        if (read_counters(0, 8)) return -1;
+        *nM = E.h_cnt[7];
+        sumE = E.h_cnt[0];
+        if (sumE <= cap) break;
+        cap = emax * *nM;   /* exact bound, always enough */
+    }
+    pc.last_nM = *nM;
     prof_set_bytes(hu, 8.0 * ncand + 8.0 * *nM);
-    la.shrink(*M, ncand, *nM);
-    /* spawns: every board has at most 16 - (cells nonzero in all of M_s) empties */
-    size_t emax = (size_t)(16 - nonzero_nibbles(*nM ? mand : ~0ULL));
-    size_t cap = emax * *nM;
-    if (ensure(E.A, nC + cap + 1) || ensure(E.Cb, cap + 1)) return -1;
-    if (nC) HC(hipMemcpyAsync(E.A.p, C, nC * sizeof(uint64_t), hipMemcpyDeviceToDevice, E.st));
-    launch_init_stats(E.st, E.d_cnt);
-    int hs = prof_begin(PK_SPAWN);
-    launch_spawn(E.st, *M, *nM, E.A.p + nC, E.Cb.p, E.d_cnt);
-    prof_end(hs, 0);
-    if (read_counters(0, 5)) return -1;
-    size_t sumE = E.h_cnt[0];
     prof_set_bytes(hs, 8.0 * *nM + 16.0 * sumE);
+    la.shrink(*M, ncand, *nM);
     uint64_t or2 = E.h_cnt[1], and2 = E.h_cnt[2], or4 = E.h_cnt[3], and4 = E.h_cnt[4];
-    /* P_{s+2} = uniq(C u S2), C' = uniq(S4) */
+    /* P_{s+2} = uniq(C u S2), C' = uniq(S4); counts stay on the device */
     size_t n2 = nC + sumE;
     if (ensure(nxtP, n2 + 1) || ensure(nxtC, sumE + 1)) return -1;
     int hu2, hu4;
     if (sort_enqueue(E.A.p, n2, vary_of(n2, or2 | carry.o, and2 & carry.a), nxtP.p, 5, &hu2))
         return -1;
     if (sort_enqueue(E.Cb.p, sumE, vary_of(sumE, or4, and4), nxtC.p, 6, &hu4)) return -1;
-    if (read_counters(5, 2)) return -1;
-    *nP2 = E.h_cnt[5];
-    *nC2 = E.h_cnt[6];
-    prof_set_bytes(hu2, 8.0 * n2 + 8.0 * *nP2);
-    prof_set_bytes(hu4, 8.0 * sumE + 8.0 * *nC2);
+    pc.pending = true;
+    pc.n2 = n2;
+    pc.sumE = sumE;
+    pc.hu2 = hu2;
+    pc.hu4 = hu4;
+    *nP2 = n2;
+    *nC2 = sumE;
     carry.o = or4;
     carry.a = and4;
     return 0;
@@ -628,12 +698,13 @@ extern "C" int cg_tb_generate(cg_tablebase *tb, const uint64_t *initial, size_t 
     if (n) HC(hipMemcpy(tb->P.p, initial, n * 8, hipMemcpyHostToDevice));
     size_t nP = n, nC = 0;
     Carry carry;
+    StepCounts pc;
     ArenaLayerAlloc la(&tb->arena);
     for (int s = start; s <= end; s += 2) {
         TbLayer l;
         size_t nP2 = 0, nC2 = 0;
         uint64_t mor = 0, mand = ~0ULL;
-        if (gen_step(tb->r, tb->P.p, nP, tb->C.p, nC, carry, la, &l.keys, &l.n, tb->P2, &nP2,
+        if (gen_step(tb->r, tb->P.p, nP, tb->C.p, nC, pc, carry, la, &l.keys, &l.n, tb->P2, &nP2,
                      tb->C2, &nC2, &mor, &mand))
             return -1;
         l.plan = plan_index(l.n, mor, mand);
@@ -646,6 +717,7 @@ extern "C" int cg_tb_generate(cg_tablebase *tb, const uint64_t *initial, size_t 
         nP = nP2;
         nC = nC2;
     }
+    if (counts_finish(pc, nullptr, nullptr)) return -1;
     return check_sync(E.st, "cg_tb_generate");
 }
 
@@ -838,7 +910,7 @@ extern "C" int cg_dev_move(const cg_rules *rules, const uint64_t *d_in, size_t n
     if (!lut) return fail_msg("row table upload failed");
     launch_init_stats(E.st, E.d_cnt);
     int h = prof_begin(PK_MOVE);
-    launch_move(E.st, d_in, n, d_out, E.d_cnt, lut, move_rules(*rules));
+    launch_move(E.st, d_in, n, nullptr, d_out, E.d_cnt, lut, move_rules(*rules));
     prof_end(h, 0);
     if (read_counters(0, 1)) return -1;
     *n_out = E.h_cnt[0];
@@ -852,7 +924,7 @@ extern "C" int cg_dev_spawn(const uint64_t *d_in, size_t n, uint64_t *d_out2, ui
     if (eng_init()) return -1;
     launch_init_stats(E.st, E.d_cnt);
     int h = prof_begin(PK_SPAWN);
-    launch_spawn(E.st, d_in, n, d_out2, d_out4, E.d_cnt);
+    launch_spawn(E.st, d_in, n, nullptr, d_out2, d_out4, 16 * n, E.d_cnt);
     prof_end(h, 0);
     if (read_counters(0, 1)) return -1;
     *n_out = E.h_cnt[0];
@@ -937,14 +1009,40 @@ extern "C" int cg_dev_canon(uint64_t *d_keys, size_t n)
     return check_sync(E.st, "cg_dev_canon");
 }
 
+/* ------------------------------------------------------------------------ */
+/* tuning knobs and the autotuner                                            */
+/* ------------------------------------------------------------------------ */
+
 extern "C" int cg_tune_set(const char *knob, int value)
 {
+    SortCfg f = sort_forced();
+    if (!strcmp(knob, "sort_bits")) {
+        if (value && (value < 8 || value > 10)) return fail_msg("sort_bits: 0 or 8..10");
+        return sort_force(value, f.kpt, f.lbw);
+    }
     if (!strcmp(knob, "sort_kpt")) {
-        set_sort_kpt(value);
-        return 0;
+        if (value && value != 8 && value != 16 && value != 32) return fail_msg("sort_kpt: 0, 8, 16 or 32");
+        return sort_force(f.bits, value, f.lbw);
     }
     if (!strcmp(knob, "sort_lbw")) {
-        set_sort_lbw(value);
+        if (value && value != 16 && value != 32) return fail_msg("sort_lbw: 0, 16 or 32");
+        return sort_force(f.bits, f.kpt, value);
+    }
+    if (!strcmp(knob, "dir_delta")) {
+        if (value < -4 || value > 3) return fail_msg("dir_delta: -4..3");
+        set_dir_delta(value);
+        return 0;
+    }
+    if (!strcmp(knob, "spawn_estimate")) {
+        if (value != 0 && value != 1) return fail_msg("spawn_estimate: 0 or 1");
+        g_spawn_est_mode = value;
+        return 0;
+    }
+    if (!strcmp(knob, "reset")) {
+        sort_table_reset();
+        sort_force(0, 0, 0);
+        set_dir_delta(0);
+        g_spawn_est_mode = 0;
         return 0;
     }
     return fail_msg("unknown tuning knob");
@@ -952,9 +1050,221 @@ extern "C" int cg_tune_set(const char *knob, int value)
 
 extern "C" int cg_tune_get(const char *knob)
 {
-    if (!strcmp(knob, "sort_kpt")) return get_sort_kpt();
-    if (!strcmp(knob, "sort_lbw")) return get_sort_lbw();
-    return -1;
+    SortCfg f = sort_forced();
+    if (!strcmp(knob, "sort_bits")) return f.bits;
+    if (!strcmp(knob, "sort_kpt")) return f.kpt;
+    if (!strcmp(knob, "sort_lbw")) return f.lbw;
+    if (!strcmp(knob, "dir_delta")) return get_dir_delta();
+    if (!strcmp(knob, "spawn_estimate")) return g_spawn_est_mode;
+    return fail_msg("unknown tuning knob");
+}
+
+extern "C" int cg_tune_sort_entry(int size_class, int nbits, int bits, int kpt, int lbw)
+{
+    SortCfg c = {bits, kpt, lbw};
+    if (sort_table_set(size_class, nbits, c)) return fail_msg("cg_tune_sort_entry: no such entry or configuration");
+    return 0;
+}
+
+extern "C" int cg_tune_sort_query(size_t n, int nbits, int *bits, int *kpt, int *lbw)
+{
+    SortCfg c = sort_choose(n, nbits);
+    if (bits) *bits = c.bits;
+    if (kpt) *kpt = c.kpt;
+    if (lbw) *lbw = c.lbw;
+    return 0;
+}
+
+namespace {
+
+struct TuneBufs {
+    uint64_t *src = nullptr, *work = nullptr, *tmp = nullptr, *dst = nullptr;
+    ~TuneBufs()
+    {
+        for (uint64_t *p : {src, work, tmp, dst})
+            if (p) hipFree(p);
+    }
+    int alloc(size_t n)
+    {
+        for (uint64_t **p : {&src, &work, &tmp, &dst}) HC(hipMalloc((void **)p, (n + 1) * 8));
+        return 0;
+    }
+};
+
+/* histogram + pass time (ms) of sorting src & mask, best of reps */
+int tune_sort_once(TuneBufs &b, size_t n, uint64_t mask, int reps, double *best)
+{
+    *best = 1e30;
+    for (int r = 0; r < reps; r++) {
+        launch_copy_and_u64(E.st, b.work, b.src, n, mask);
+        cg_prof_reset();
+        int hu;
+        sort_unique(E.st, b.work, b.tmp, n, mask, E.scratch, b.dst, E.d_cnt + 7, E.d_err, &hu);
+        hipError_t e = kernels_take_error();
+        if (e != hipSuccess) return fail("autotune sort", e);
+        double mh = 0, mp = 0;
+        cg_prof_query("sort_hist", &mh, nullptr);
+        cg_prof_query("sort_pass", &mp, nullptr);
+        *best = std::min(*best, mh + mp);
+    }
+    return read_counters(7, 1);
+}
+
+void jcat(std::string &s, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    s += buf;
+}
+
+} // namespace
+
+/* Measures every sort configuration at every pass count on board-like keys
+ * (random boards over 16 free cells, empty with p = 0.4, else exponent k with
+ * P(k) ~ 2^-k, masked to the pass count's bits) for three sizes, and fills
+ * the (size class, varying bits) table with the fastest; then times the
+ * directory depths on a lookup workload (a 2^22-board DPDGap layer, 2^25
+ * queries) and sets the fastest.  A configuration replaces the default only
+ * when it is at least 3% faster.  effort scales the sizes (0: a quick pass
+ * with sizes / 8, 1: full).  report (may be NULL) receives a JSON summary. */
+extern "C" int cg_autotune(int effort, char *report, size_t cap)
+{
+    if (eng_init()) return -1;
+    const bool was_prof = g_prof;
+    const SortCfg forced = sort_forced();
+    sort_force(0, 0, 0);
+    sort_table_reset();
+    g_prof = true;
+    cg_prof_reset();
+    const size_t sizes[SORT_SIZE_CLASSES] = {(size_t)1 << 18, (size_t)1 << 21,
+                                             (size_t)1 << (effort > 0 ? 25 : 24)};
+    const int reps = effort > 0 ? 3 : 2;
+    const int ncfg = sort_num_configs();
+    std::string js = "{\"sort\":[";
+    int rc = 0;
+    for (int cls = 0; cls < SORT_SIZE_CLASSES && !rc; cls++) {
+        size_t n = sizes[cls];
+        TuneBufs b;
+        if ((rc = b.alloc(n)) || (rc = ensure_scratch(n))) break;
+        launch_random_boards(E.st, b.src, n, 0x5eed0000ULL + cls, 0, 0.4, 11, 0);
+        /* T[i][k]: configuration i with k passes */
+        std::vector<std::vector<double>> T(ncfg, std::vector<double>(9, 1e30));
+        jcat(js, "%s{\"n\":%zu,\"ms\":{", cls ? "," : "", n);
+        for (int i = 0; i < ncfg && !rc; i++) {
+            SortCfg c = sort_config(i);
+            sort_force(c.bits, c.kpt, c.lbw);
+            int kmax = (64 + c.bits - 1) / c.bits;
+            jcat(js, "%s\"%dx%d/%d\":[", i ? "," : "", c.bits, c.kpt, c.lbw);
+            for (int k = 1; k <= kmax && !rc; k++) {
+                int nb = std::min(64, k * c.bits);
+                uint64_t mask = nb >= 64 ? ~0ULL : ((1ULL << nb) - 1);
+                rc = tune_sort_once(b, n, mask, reps, &T[i][k]);
+                jcat(js, "%s%.4f", k > 1 ? "," : "", T[i][k]);
+            }
+            jcat(js, "]");
+        }
+        sort_force(0, 0, 0);
+        if (rc) break;
+        jcat(js, "},\"pick\":{");
+        for (int nbits = 1; nbits <= 64; nbits++) {
+            SortCfg d = sort_config(0);
+            int best = 0;
+            double tb = T[0][(nbits + d.bits - 1) / d.bits];
+            for (int i = 1; i < ncfg; i++) {
+                SortCfg c = sort_config(i);
+                double t = T[i][(nbits + c.bits - 1) / c.bits];
+                if (t < tb * (best == 0 ? 0.97 : 1.0)) tb = t, best = i;
+            }
+            SortCfg c = sort_config(best);
+            sort_table_set(cls, nbits, c);
+            jcat(js, "%s\"%d\":\"%dx%d/%d\"", nbits > 1 ? "," : "", nbits, c.bits, c.kpt, c.lbw);
+        }
+        jcat(js, "}}");
+    }
+    /* directory depth */
+    if (!rc) {
+        const size_t nl = (size_t)1 << (effort > 0 ? 22 : 20), nq = nl << 3;
+        TuneBufs b;
+        uint64_t *q = nullptr;
+        double *out = nullptr;
+        uint32_t *dir = nullptr;
+        const size_t dir_cap = ((size_t)4 << (63 - __builtin_clzll(nl))) + 1;   /* delta <= +2 */
+        rc = b.alloc(nl);
+        if (!rc && (hipMalloc((void **)&q, nq * 8) != hipSuccess ||
+                    hipMalloc((void **)&out, nq * 8) != hipSuccess ||
+                    hipMalloc((void **)&dir, dir_cap * 4) != hipSuccess))
+            rc = fail_msg("autotune: allocation failed");
+        const uint64_t dpd = 0x00000000FFF0FFF0ULL;
+        if (!rc) {
+            launch_random_boards(E.st, b.src, nl, 0x5eed1000ULL, dpd, 0.4, 11, 0);
+            launch_canon(E.st, b.src, nl);
+            int hu;
+            sort_unique(E.st, b.src, b.tmp, nl, ~dpd, E.scratch, b.dst, E.d_cnt + 7, E.d_err, &hu);
+            rc = read_counters(7, 1);
+        }
+        size_t nu = rc ? 0 : E.h_cnt[7];
+        uint64_t o = 0, a = ~0ULL;
+        if (!rc) rc = key_or_and(b.dst, nu, &o, &a);
+        if (!rc) {
+            launch_random_boards(E.st, q, nq, 0x5eed2000ULL, dpd, 0.4, 11, 0);
+            launch_fill_u64(E.st, (uint64_t *)b.work, nu, 0);   /* values */
+        }
+        double best_t = 1e30;
+        int best_d = 0;
+        double t0 = 0;
+        jcat(js, "],\"dir\":{\"layer\":%zu,\"queries\":%zu,\"ms\":{", nu, nq);
+        for (int d = -3; d <= 2 && !rc; d++) {
+            set_dir_delta(d);
+            LayerIndexPlan plan = plan_index(nu, o, a);
+            if (plan.dir_entries > dir_cap) continue;
+            if (ensure_scratch_bytes(dir_scratch_bytes(plan.dir_entries))) {
+                rc = -1;
+                break;
+            }
+            double tbest = 1e30;
+            for (int r = 0; r < reps && !rc; r++) {
+                hipEvent_t e0 = ev_get(), e1 = ev_get();
+                hipEventRecord(e0, E.st);
+                launch_build_dir(E.st, b.dst, nu, plan, dir, E.scratch);
+                LayerView t = make_view(b.dst, (const double *)b.work, nu, dir, plan);
+                launch_lookup(E.st, t, q, nq, out, 1);
+                hipEventRecord(e1, E.st);
+                hipError_t e = hipEventSynchronize(e1);
+                if (e != hipSuccess) { rc = fail("autotune lookup", e); break; }
+                float ms = 0;
+                hipEventElapsedTime(&ms, e0, e1);
+                g_evpool.push_back(e0);
+                g_evpool.push_back(e1);
+                tbest = std::min(tbest, (double)ms);
+            }
+            if (d == 0) t0 = tbest;
+            if (tbest < best_t) best_t = tbest, best_d = d;
+            jcat(js, "%s\"%d\":%.4f", d > -3 ? "," : "", d, tbest);
+        }
+        if (!rc && t0 > 0 && best_t > 0.97 * t0) best_d = 0;
+        set_dir_delta(best_d);
+        jcat(js, "},\"pick\":%d}}", best_d);
+        if (q) hipFree(q);
+        if (out) hipFree(out);
+        if (dir) hipFree(dir);
+    }
+    cg_prof_reset();
+    g_prof = was_prof;
+    if (forced.bits || forced.kpt || forced.lbw) sort_force(forced.bits, forced.kpt, forced.lbw);
+    if (rc) {
+        sort_table_reset();
+        set_dir_delta(0);
+        return -1;
+    }
+    if (report && cap) {
+        strncpy(report, js.c_str(), cap - 1);
+        report[cap - 1] = 0;
+        if (js.size() >= cap) return fail_msg("cg_autotune: report buffer too small");
+    }
+    return 0;
 }
 
 extern "C" int cg_dev_sync(void)
@@ -986,6 +1296,7 @@ static struct {
     Buf P, C, P2, C2, M;
     size_t nP = 0, nC = 0, nM = 0;
     Carry carry;
+    StepCounts pc;
 } G;
 
 struct BufLayerAlloc : LayerAlloc {
@@ -1006,6 +1317,7 @@ extern "C" int cge_gen_begin(const cge_rules *r, const uint64_t *initial, size_t
     G.nC = 0;
     G.nM = 0;
     G.carry = Carry();
+    G.pc = StepCounts();
     return 0;
 }
 
@@ -1014,7 +1326,8 @@ extern "C" int cge_gen_step(size_t *n_layer)
     uint64_t *M = nullptr;
     size_t nP2 = 0, nC2 = 0;
     BufLayerAlloc la;
-    if (gen_step(G.r, G.P.p, G.nP, G.C.p, G.nC, G.carry, la, &M, &G.nM, G.P2, &nP2, G.C2, &nC2))
+    if (gen_step(G.r, G.P.p, G.nP, G.C.p, G.nC, G.pc, G.carry, la, &M, &G.nM, G.P2, &nP2, G.C2,
+                 &nC2))
         return -1;
     std::swap(G.P, G.P2);
     std::swap(G.C, G.C2);
@@ -1033,6 +1346,7 @@ extern "C" int cge_gen_fetch(uint64_t *host, size_t n)
 
 extern "C" void cge_gen_end(void)
 {
+    counts_finish(G.pc, nullptr, nullptr);
     buf_free(G.P);
     buf_free(G.C);
     buf_free(G.P2);
@@ -1133,6 +1447,7 @@ struct cg_batch {
         std::vector<size_t> off;            /* ntab + 1 */
         std::vector<LayerIndexPlan> plan;   /* per table */
         std::vector<size_t> dir_off;        /* per table */
+        std::vector<BatchIndexPlan> bplan;  /* per table (source of an asynchronous copy) */
     };
     std::vector<Step> steps;
     int K = 0;
@@ -1201,6 +1516,15 @@ extern "C" int cg_batch_add(cg_batch *b, const uint64_t *initial, size_t n, int 
         return fail_msg("cg_batch_add: more than 12 free cells do not fit a composite key");
     t.dev.fnib = fnib;
     t.dev.nfree = nfree;
+    {
+        /* runs of free-cell bits: at most 4 lets xform pack them directly */
+        uint64_t free_bits = ~fnib;
+        int runs = 0;
+        for (int bit = 63; bit >= 0; bit--)
+            if (((free_bits >> bit) & 1) && (bit == 63 || !((free_bits >> (bit + 1)) & 1))) runs++;
+        t.dev.runs = free_bits && runs <= 4;
+        t.dev.fx = make_xform(t.dev.runs ? free_bits : 0);
+    }
     t.start = start;
     t.end = end;
     t.nsteps = (end - start) / 2 + 1;
@@ -1208,6 +1532,8 @@ extern "C" int cg_batch_add(cg_batch *b, const uint64_t *initial, size_t n, int 
     for (size_t i = 0; i < n; i++) {
         uint64_t c = cg_canon(initial[i]);
         if ((c & fnib) != fnib) return fail_msg("cg_batch_add: initial boards differ in formation");
+        if (t.dev.runs && (t.dev.fnib | xform_inv(xform(c, t.dev.fx), t.dev.fx)) != c)
+            return fail_msg("cg_batch_add: free-cell packing does not round-trip");
         t.init.push_back(c);
     }
     win_images(winstates, n_win, t.wimg);
@@ -1267,44 +1593,62 @@ extern "C" int cg_batch_generate(cg_batch *b)
     HC(hipMemcpy(b->P.p, p0.data(), p0.size() * 8, hipMemcpyHostToDevice));
     size_t nP = p0.size(), nC = 0;
     Carry carry;
+    StepCounts pc;
     std::vector<uint64_t> init_tstat(3 * ntab);
     for (int t = 0; t < ntab; t++) init_tstat[3 * t + 2] = ~0ULL;
+    int maxfree = 0;
+    for (int t = 0; t < ntab; t++) maxfree = std::max(maxfree, b->tabs[t].dev.nfree);
+    /* per step: two host synchronisations, as in gen_step */
     for (int k = 0; k < b->K; k++) {
         cg_batch::Step st;
-        /* moves of every table */
+        /* moves of every table (the previous step's counts are still on the device) */
         if (ensure(E.A, 4 * nP + 1)) return -1;
         launch_init_stats(E.st, E.d_cnt);
         int h = prof_begin(PK_MOVE);
-        launch_bmove(E.st, b->P.p, nP, E.A.p, E.d_cnt, lut, move_rules(b->r), b->d_tabs, k, E.d_err);
+        launch_bmove(E.st, b->P.p, nP, pc.pending ? E.d_cnt + 5 : nullptr, E.A.p, E.d_cnt, lut,
+                     move_rules(b->r), b->d_tabs, k, E.d_err);
         prof_end(h, 0);
-        if (read_counters(0, 3)) return -1;
+        if (read_counters(0, pc.pending ? 7 : 3)) return -1;
+        if (pc.pending) {
+            nP = E.h_cnt[5];
+            nC = E.h_cnt[6];
+            counts_arrived(pc, nP, nC);
+        }
         size_t ncand = E.h_cnt[0];
         uint64_t mor = E.h_cnt[1], mand = E.h_cnt[2];
         prof_set_bytes(h, 8.0 * nP + 8.0 * ncand);
         if (ensure(b->M, ncand + 1)) return -1;
         int hu;
-        if (sort_enqueue(E.A.p, ncand, vary_of(ncand, mor, mand), b->M.p, 5, &hu)) return -1;
-        if (read_counters(5, 1)) return -1;
-        size_t nM = E.h_cnt[5];
-        prof_set_bytes(hu, 8.0 * ncand + 8.0 * nM);
-        /* boards of the step, per-table segments and statistics */
-        st.n = nM;
-        st.keys = (uint64_t *)b->arena.alloc(std::max<size_t>(nM, 1) * 8);
+        if (sort_enqueue(E.A.p, ncand, vary_of(ncand, mor, mand), b->M.p, 7, &hu)) return -1;
+        /* boards of the step (count on the device until the next synchronisation) */
+        st.keys = (uint64_t *)b->arena.alloc(std::max<size_t>(ncand, 1) * 8);
         if (!st.keys) return fail_msg("batch layer allocation failed");
         HC(hipMemcpyAsync(b->d_tstat, init_tstat.data(), 3 * ntab * 8, hipMemcpyHostToDevice, E.st));
-        launch_bdecode(E.st, b->M.p, nM, st.keys, b->d_tabs, b->d_tstat);
-        /* spawns of every table */
-        size_t cap = 16 * nM;
-        if (ensure(E.A, nC + cap + 1) || ensure(E.Cb, cap + 1)) return -1;
-        if (nC) HC(hipMemcpyAsync(E.A.p, b->C.p, nC * 8, hipMemcpyDeviceToDevice, E.st));
-        launch_init_stats(E.st, E.d_cnt);
-        int hs = prof_begin(PK_SPAWN);
-        launch_bspawn(E.st, b->M.p, nM, E.A.p + nC, E.Cb.p, E.d_cnt, b->d_tabs, k, E.d_err);
-        prof_end(hs, 0);
-        HC(hipMemcpyAsync(b->h_tstat.data(), b->d_tstat, 3 * ntab * 8, hipMemcpyDeviceToHost, E.st));
-        if (read_counters(0, 5)) return -1;
-        size_t sumE = E.h_cnt[0];
+        launch_bdecode(E.st, b->M.p, ncand, E.d_cnt + 7, st.keys, b->d_tabs, b->d_tstat);
+        /* spawns of every table: at most maxfree empty cells per board */
+        size_t cap = (size_t)maxfree * spawn_rows_estimate(pc, ncand);
+        int hs = -1;
+        size_t nM = 0, sumE = 0;
+        for (int attempt = 0; attempt < 2; attempt++) {
+            if (ensure(E.A, nC + cap + 1) || ensure(E.Cb, cap + 1)) return -1;
+            if (nC) HC(hipMemcpyAsync(E.A.p, b->C.p, nC * 8, hipMemcpyDeviceToDevice, E.st));
+            launch_init_stats(E.st, E.d_cnt);
+            hs = prof_begin(PK_SPAWN);
+            launch_bspawn(E.st, b->M.p, ncand, E.d_cnt + 7, E.A.p + nC, E.Cb.p, cap, E.d_cnt,
+                          b->d_tabs, k, E.d_err);
+            prof_end(hs, 0);
+            HC(hipMemcpyAsync(b->h_tstat.data(), b->d_tstat, 3 * ntab * 8, hipMemcpyDeviceToHost,
+                              E.st));
+            if (read_counters(0, 8)) return -1;
+            nM = E.h_cnt[7];
+            sumE = E.h_cnt[0];
+            if (sumE <= cap) break;
+            cap = (size_t)maxfree * nM;   /* exact bound, always enough */
+        }
+        pc.last_nM = nM;
+        prof_set_bytes(hu, 8.0 * ncand + 8.0 * nM);
         prof_set_bytes(hs, 8.0 * nM + 16.0 * sumE);
+        b->arena.shrink_last(st.keys, std::max<size_t>(ncand, 1) * 8, std::max<size_t>(nM, 1) * 8);
         uint64_t or2 = E.h_cnt[1], and2 = E.h_cnt[2], or4 = E.h_cnt[3], and4 = E.h_cnt[4];
         size_t n2 = nC + sumE;
         if (ensure(b->P2, n2 + 1) || ensure(b->C2, sumE + 1)) return -1;
@@ -1312,20 +1656,21 @@ extern "C" int cg_batch_generate(cg_batch *b)
         if (sort_enqueue(E.A.p, n2, vary_of(n2, or2 | carry.o, and2 & carry.a), b->P2.p, 5, &hu2))
             return -1;
         if (sort_enqueue(E.Cb.p, sumE, vary_of(sumE, or4, and4), b->C2.p, 6, &hu4)) return -1;
-        /* directories of every table (enqueued before the synchronisation) */
+        /* directories of every table of this step */
+        st.n = nM;
         st.off.assign(ntab + 1, 0);
         st.plan.resize(ntab);
         st.dir_off.resize(ntab);
-        std::vector<BatchIndexPlan> bp(ntab);
+        st.bplan.resize(ntab);
         size_t entries = 0;
         for (int t = 0; t < ntab; t++) {
             size_t nt = b->h_tstat[3 * t];
             st.off[t + 1] = st.off[t] + nt;
             st.plan[t] = plan_index(nt, b->h_tstat[3 * t + 1], b->h_tstat[3 * t + 2]);
             st.dir_off[t] = entries;
-            bp[t].x = st.plan[t].x;
-            bp[t].shift = st.plan[t].shift;
-            bp[t].dir_base = entries;
+            st.bplan[t].x = st.plan[t].x;
+            st.bplan[t].shift = st.plan[t].shift;
+            st.bplan[t].dir_base = entries;
             entries += st.plan[t].dir_entries;
         }
         if (st.off[ntab] != nM) return fail_msg("batch: per-table counts disagree with the layer");
@@ -1333,22 +1678,27 @@ extern "C" int cg_batch_generate(cg_batch *b)
         size_t *d_seg = (size_t *)b->arena.alloc((ntab + 1) * sizeof(size_t));
         BatchIndexPlan *d_plan = (BatchIndexPlan *)b->arena.alloc(ntab * sizeof(BatchIndexPlan));
         if (!st.dir || !d_seg || !d_plan) return fail_msg("batch index allocation failed");
-        HC(hipMemcpy(d_seg, st.off.data(), (ntab + 1) * sizeof(size_t), hipMemcpyHostToDevice));
-        HC(hipMemcpy(d_plan, bp.data(), ntab * sizeof(BatchIndexPlan), hipMemcpyHostToDevice));
-        if (read_counters(5, 2)) return -1;
-        size_t nP2 = E.h_cnt[5], nC2 = E.h_cnt[6];
-        prof_set_bytes(hu2, 8.0 * n2 + 8.0 * nP2);
-        prof_set_bytes(hu4, 8.0 * sumE + 8.0 * nC2);
+        /* the host arrays live in the step, so the asynchronous copies may read them later */
+        HC(hipMemcpyAsync(d_seg, st.off.data(), (ntab + 1) * sizeof(size_t), hipMemcpyHostToDevice,
+                          E.st));
+        HC(hipMemcpyAsync(d_plan, st.bplan.data(), ntab * sizeof(BatchIndexPlan),
+                          hipMemcpyHostToDevice, E.st));
         if (ensure_scratch_bytes(dir_scratch_bytes(entries + 1))) return -1;
         launch_bdir(E.st, st.keys, nM, d_seg, ntab, d_plan, entries, st.dir, E.scratch);
         b->steps.push_back(std::move(st));
+        pc.pending = true;
+        pc.n2 = n2;
+        pc.sumE = sumE;
+        pc.hu2 = hu2;
+        pc.hu4 = hu4;
         std::swap(b->P, b->P2);
         std::swap(b->C, b->C2);
-        nP = nP2;
-        nC = nC2;
+        nP = n2;
+        nC = sumE;
         carry.o = or4;
         carry.a = and4;
     }
+    if (counts_finish(pc, nullptr, nullptr)) return -1;
     return check_sync(E.st, "cg_batch_generate");
 }
 

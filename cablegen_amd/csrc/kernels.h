@@ -82,19 +82,21 @@ size_t sort_scratch_bytes(size_t n);
 
 /* move + lock + prune + canonicalise; appends valid canonical successors to
  * out (capacity 4n), adds their number to stats[0] and ORs / ANDs them into
- * stats[1] / stats[2]. */
-void launch_move(hipStream_t st, const uint64_t *in, size_t n, uint64_t *out,
-                 unsigned long long *stats, const uint16_t *lut, MoveRules r);
+ * stats[1] / stats[2].  With dn, the number of boards is min(n, *dn) read on
+ * the device (n bounds the grid); likewise for launch_spawn. */
+void launch_move(hipStream_t st, const uint64_t *in, size_t n, const unsigned long long *dn,
+                 uint64_t *out, unsigned long long *stats, const uint16_t *lut, MoveRules r);
 
 /* sum of empty cells over a layer into *count (device counter) */
 void launch_count_empties(hipStream_t st, const uint64_t *in, size_t n, unsigned long long *count);
 
 /* spawn expansion: for each board and empty cell x (ascending), canon(b[x<-1])
  * into out2 and canon(b[x<-2]) into out4 at the same positions; stats[0]
- * receives the number written, stats[1..2] / stats[3..4] the OR / AND of the
- * out2 / out4 keys. */
-void launch_spawn(hipStream_t st, const uint64_t *in, size_t n, uint64_t *out2, uint64_t *out4,
-                  unsigned long long *stats);
+ * receives the number of spawns, stats[1..2] / stats[3..4] the OR / AND of
+ * the out2 / out4 keys.  Only positions below cap are written: a count above
+ * cap means the buffers were too small and the launch must be repeated. */
+void launch_spawn(hipStream_t st, const uint64_t *in, size_t n, const unsigned long long *dn,
+                  uint64_t *out2, uint64_t *out4, size_t cap, unsigned long long *stats);
 
 
 /* Sorts keys[0..n) ascending and writes the unique keys to dst (capacity n),
@@ -109,14 +111,42 @@ uint64_t *sort_unique(hipStream_t st, uint64_t *keys, uint64_t *tmp, size_t n, u
                       void *scratch, uint64_t *dst, unsigned long long *d_nuniq, unsigned *d_err,
                       int *prof_unique);
 
-/* tuning: keys per thread of the sort kernels (16 or 32) */
-void set_sort_kpt(int kpt);
-int get_sort_kpt();
-/* tuning: look-back window of the radix passes (16, 32 or 64 tiles) */
-void set_sort_lbw(int w);
-int get_sort_lbw();
+/* Sort configurations: digit width (8, 9 or 10 bits; the pass block has one
+ * thread per digit), keys per thread (tile = 2^bits * kpt keys) and look-back
+ * window (16 or 32 tiles).  sort_config(i), i < sort_num_configs(), lists
+ * the combinations that have kernels. */
+struct SortCfg {
+    int bits, kpt, lbw;
+};
+constexpr int SORT_SIZE_CLASSES = 3;   /* n < 2^20, n < 2^23, larger */
+int sort_num_configs();
+SortCfg sort_config(int i);
+int sort_size_class(size_t n);
+/* per (size class, varying key bits) table used by sort_unique */
+int sort_table_set(int cls, int nbits, SortCfg c);
+SortCfg sort_table_get(int cls, int nbits);
+void sort_table_reset();
+/* fields that are nonzero override the table for every sort */
+int sort_force(int bits, int kpt, int lbw);
+SortCfg sort_forced();
+SortCfg sort_choose(size_t n, int nbits);
+/* frees the look-back status array */
+void sort_release();
+
+/* prefix-bucket directory depth: bits = floor(log2 n) + delta, clamped */
+void set_dir_delta(int d);
+int get_dir_delta();
+
+/* records a launch error for kernels_take_error() */
+void kernels_note_error(hipError_t e);
 
 void launch_fill_u64(hipStream_t st, uint64_t *p, size_t n, uint64_t v);
+/* hpub[lo..lo+n) = cnt[lo..lo+n), hpub[8] = *err (hpub: mapped host memory,
+ * n <= 8) */
+void launch_publish(hipStream_t st, const unsigned long long *cnt, int lo, int n, const unsigned *err,
+                    unsigned long long *hpub);
+/* dst[i] = src[i] & mask */
+void launch_copy_and_u64(hipStream_t st, uint64_t *dst, const uint64_t *src, size_t n, uint64_t mask);
 /* stats[0] = 0 (count), stats[1,3] = 0 (OR), stats[2,4] = ~0 (AND) */
 void launch_init_stats(hipStream_t st, unsigned long long *stats);
 /* OR / AND of keys into stats[1] / stats[2] */
@@ -158,6 +188,8 @@ struct BatchTable {
     uint8_t pos[16];
     int nfree;
     int nsteps;
+    int runs;        /* free-cell bits form at most 4 runs: fx packs them (pext-like) */
+    KeyXform fx;
 };
 
 struct BatchIndexPlan {
@@ -178,15 +210,16 @@ struct BSolveDesc {
 
 /* composite-key move / spawn; err |= 2 if a board's formation is not where
  * the table's canonical form puts it */
-void launch_bmove(hipStream_t st, const uint64_t *in, size_t n, uint64_t *out,
-                  unsigned long long *stats, const uint16_t *lut, MoveRules r, const BatchTable *tabs,
-                  int step, unsigned *err);
-void launch_bspawn(hipStream_t st, const uint64_t *in, size_t n, uint64_t *out2, uint64_t *out4,
-                   unsigned long long *stats, const BatchTable *tabs, int step, unsigned *err);
+void launch_bmove(hipStream_t st, const uint64_t *in, size_t n, const unsigned long long *dn,
+                  uint64_t *out, unsigned long long *stats, const uint16_t *lut, MoveRules r,
+                  const BatchTable *tabs, int step, unsigned *err);
+void launch_bspawn(hipStream_t st, const uint64_t *in, size_t n, const unsigned long long *dn,
+                   uint64_t *out2, uint64_t *out4, size_t cap, unsigned long long *stats,
+                   const BatchTable *tabs, int step, unsigned *err);
 /* composite -> boards, with per-table count / OR / AND (tstat[3 t + 0..2],
  * AND entries preset to all ones) */
-void launch_bdecode(hipStream_t st, const uint64_t *in, size_t n, uint64_t *out,
-                    const BatchTable *tabs, unsigned long long *tstat);
+void launch_bdecode(hipStream_t st, const uint64_t *in, size_t n, const unsigned long long *dn,
+                    uint64_t *out, const BatchTable *tabs, unsigned long long *tstat);
 /* directories of every table of a step in one concatenated array; entries
  * are indices into the step's concatenated keys */
 void launch_bdir(hipStream_t st, const uint64_t *keys, size_t n, const size_t *seg, int ntab,

@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 #include "board.h"
 #include "kernels.h"
+#include "devutil.h"
 
 namespace cg {
 
@@ -32,77 +33,13 @@ static void note_err(hipError_t e)
 }
 #define CGK(x) note_err(x)
 
+void kernels_note_error(hipError_t e) { note_err(e); }
+
 hipError_t kernels_take_error()
 {
     hipError_t e = g_kerr;
     g_kerr = hipSuccess;
     return e;
-}
-
-/* ------------------------------------------------------------------------ */
-/* block / wave helpers                                                      */
-/* ------------------------------------------------------------------------ */
-
-__device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
-
-__device__ __forceinline__ uint64_t lanemask_lt()
-{
-    int l = lane_id();
-    return l ? (~0ULL >> (64 - l)) : 0ULL;
-}
-
-template <typename T>
-__device__ __forceinline__ T wave_incl_scan(T v)
-{
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        T u = __shfl_up(v, d, 64);
-        if (lane_id() >= d) v += u;
-    }
-    return v;
-}
-
-/* Exclusive scan over the block (blockDim.x <= 1024).  `wtmp` holds 16 T. */
-template <typename T>
-__device__ __forceinline__ T block_excl_scan(T v, T *total, T *wtmp)
-{
-    int w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
-    T inc = wave_incl_scan(v);
-    if (lane_id() == 63) wtmp[w] = inc;
-    __syncthreads();
-    if (threadIdx.x < 64) {
-        T x = threadIdx.x < nw ? wtmp[threadIdx.x] : (T)0;
-        T xi = wave_incl_scan(x);
-        if (threadIdx.x < nw) wtmp[threadIdx.x] = xi - x;
-        if (threadIdx.x == nw - 1) wtmp[16] = xi;
-    }
-    __syncthreads();
-    T r = wtmp[w] + inc - v;
-    *total = wtmp[16];
-    __syncthreads();
-    return r;
-}
-
-/* OR and AND of per-thread values over the block, one atomic pair per block */
-__device__ __forceinline__ void block_or_and(uint64_t o, uint64_t a, unsigned long long *dst_or,
-                                             unsigned long long *dst_and, uint64_t *ro, uint64_t *ra)
-{
-    __syncthreads();   /* ro/ra may still be read by a previous call */
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) {
-        o |= __shfl_xor(o, d, 64);
-        a &= __shfl_xor(a, d, 64);
-    }
-    int nw = (blockDim.x + 63) >> 6;
-    if (lane_id() == 0) ro[threadIdx.x >> 6] = o, ra[threadIdx.x >> 6] = a;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int w = 1; w < nw; w++) o |= ro[w], a &= ra[w];
-        o |= ro[0];
-        a &= ra[0];
-        if (o) atomicOr(dst_or, (unsigned long long)o);
-        if (~a) atomicAnd(dst_and, (unsigned long long)a);
-    }
 }
 
 /* ------------------------------------------------------------------------ */
@@ -112,6 +49,7 @@ __device__ __forceinline__ void block_or_and(uint64_t o, uint64_t a, unsigned lo
 constexpr int MV_THREADS = 1024;
 
 __global__ __launch_bounds__(MV_THREADS) void k_move(const uint64_t *__restrict__ in, size_t n,
+                                                     const unsigned long long *dn,
                                                      uint64_t *__restrict__ out,
                                                      unsigned long long *stats,
                                                      const uint16_t *__restrict__ glut,
@@ -127,6 +65,7 @@ __global__ __launch_bounds__(MV_THREADS) void k_move(const uint64_t *__restrict_
         uint4 *l = reinterpret_cast<uint4 *>(lut);
         for (int i = threadIdx.x; i < 65536 / 8; i += MV_THREADS) l[i] = g[i];
     }
+    if (dn && *dn < n) n = (size_t)*dn;   /* count produced on the device */
     __syncthreads();
     const size_t stride = (size_t)gridDim.x * MV_THREADS;
     for (size_t b0 = (size_t)blockIdx.x * MV_THREADS; b0 < n; b0 += stride) {
@@ -158,14 +97,14 @@ __global__ __launch_bounds__(MV_THREADS) void k_move(const uint64_t *__restrict_
     block_or_and(kor, kand, &stats[1], &stats[2], ro, ra);
 }
 
-void launch_move(hipStream_t st, const uint64_t *in, size_t n, uint64_t *out,
-                 unsigned long long *stats, const uint16_t *lut, MoveRules r)
+void launch_move(hipStream_t st, const uint64_t *in, size_t n, const unsigned long long *dn,
+                 uint64_t *out, unsigned long long *stats, const uint16_t *lut, MoveRules r)
 {
     if (!n) return;
     size_t blocks = (n + MV_THREADS - 1) / MV_THREADS;
     if (blocks > 512) blocks = 512;
-    hipLaunchKernelGGL(k_move, dim3((unsigned)blocks), dim3(MV_THREADS), 0, st, in, n, out, stats,
-                       lut, r);
+    hipLaunchKernelGGL(k_move, dim3((unsigned)blocks), dim3(MV_THREADS), 0, st, in, n, dn, out,
+                       stats, lut, r);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -205,14 +144,16 @@ __device__ __forceinline__ int next_cell(uint64_t &z)
 }
 
 __global__ __launch_bounds__(SP_THREADS) void k_spawn(const uint64_t *__restrict__ in, size_t n,
+                                                      const unsigned long long *dn,
                                                       uint64_t *__restrict__ out2,
-                                                      uint64_t *__restrict__ out4,
+                                                      uint64_t *__restrict__ out4, size_t cap,
                                                       unsigned long long *stats)
 {
     __shared__ uint32_t wtmp[17];
     __shared__ unsigned long long base_sh;
     __shared__ uint64_t ro[16], ra[16];
     uint64_t or2 = 0, and2 = ~0ULL, or4 = 0, and4 = ~0ULL;
+    if (dn && *dn < n) n = (size_t)*dn;   /* count produced on the device */
     const size_t stride = (size_t)gridDim.x * SP_THREADS;
     for (size_t b0 = (size_t)blockIdx.x * SP_THREADS; b0 < n; b0 += stride) {
         size_t i = b0 + threadIdx.x;
@@ -227,8 +168,10 @@ __global__ __launch_bounds__(SP_THREADS) void k_spawn(const uint64_t *__restrict
         while (z) {
             int x = next_cell(z);
             uint64_t c2 = cg_canon(cg_set_tile(b, x, 1)), c4 = cg_canon(cg_set_tile(b, x, 2));
-            out2[o] = c2;
-            out4[o] = c4;
+            if (o < cap) {   /* past the capacity: only counted (the host retries) */
+                out2[o] = c2;
+                out4[o] = c4;
+            }
             or2 |= c2;
             and2 &= c2;
             or4 |= c4;
@@ -241,14 +184,14 @@ __global__ __launch_bounds__(SP_THREADS) void k_spawn(const uint64_t *__restrict
     block_or_and(or4, and4, &stats[3], &stats[4], ro, ra);
 }
 
-void launch_spawn(hipStream_t st, const uint64_t *in, size_t n, uint64_t *out2, uint64_t *out4,
-                  unsigned long long *stats)
+void launch_spawn(hipStream_t st, const uint64_t *in, size_t n, const unsigned long long *dn,
+                  uint64_t *out2, uint64_t *out4, size_t cap, unsigned long long *stats)
 {
     if (!n) return;
     size_t blocks = (n + SP_THREADS - 1) / SP_THREADS;
     if (blocks > 1024) blocks = 1024;   /* bounds the OR/AND atomics per launch */
-    hipLaunchKernelGGL(k_spawn, dim3((unsigned)blocks), dim3(SP_THREADS), 0, st, in, n, out2, out4,
-                       stats);
+    hipLaunchKernelGGL(k_spawn, dim3((unsigned)blocks), dim3(SP_THREADS), 0, st, in, n, dn, out2,
+                       out4, cap, stats);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -270,10 +213,41 @@ void launch_fill_u64(hipStream_t st, uint64_t *p, size_t n, uint64_t v)
     hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)blocks), dim3(256), 0, st, p, n, v);
 }
 
+__global__ void k_copy_and_u64(uint64_t *__restrict__ dst, const uint64_t *__restrict__ src,
+                               size_t n, uint64_t mask)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i] & mask;
+}
+
+void launch_copy_and_u64(hipStream_t st, uint64_t *dst, const uint64_t *src, size_t n, uint64_t mask)
+{
+    if (!n) return;
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(k_copy_and_u64, dim3((unsigned)blocks), dim3(256), 0, st, dst, src, n, mask);
+}
+
+__global__ void k_publish(const unsigned long long *cnt, int lo, int n, const unsigned *err,
+                          unsigned long long *hpub)
+{
+    int i = threadIdx.x;
+    if (i < n) hpub[lo + i] = cnt[lo + i];
+    if (i == 63) hpub[8] = *err;
+}
+
+void launch_publish(hipStream_t st, const unsigned long long *cnt, int lo, int n, const unsigned *err,
+                    unsigned long long *hpub)
+{
+    hipLaunchKernelGGL(k_publish, dim3(1), dim3(64), 0, st, cnt, lo, n, err, hpub);
+}
+
 /* producer statistics: [0] count, then (OR, AND) pairs; AND starts all-ones */
 __global__ void k_init_stats(unsigned long long *s)
 {
-    if (threadIdx.x < 8) s[threadIdx.x] = (threadIdx.x == 2 || threadIdx.x == 4) ? ~0ULL : 0ULL;
+    /* slots 5..7 hold sort counts that later launches read on the device */
+    if (threadIdx.x < 5) s[threadIdx.x] = (threadIdx.x == 2 || threadIdx.x == 4) ? ~0ULL : 0ULL;
 }
 
 void launch_init_stats(hipStream_t st, unsigned long long *stats)
@@ -317,479 +291,12 @@ void launch_or_and(hipStream_t st, const uint64_t *k, size_t n, unsigned long lo
 }
 
 /* ------------------------------------------------------------------------ */
-/* onesweep LSD radix sort over the varying key bits + unique                */
-/* ------------------------------------------------------------------------ */
-/* The bits that differ between keys (OR & ~AND of the key set, produced by
- * the kernels that wrote the keys) are packed into a dense key on the fly
- * (an order-preserving bijection on the key set), so a DPDGap layer with 40
- * varying bits spread over bits 0..51 sorts in 5 passes of 8 bits.  One
- * histogram kernel computes every pass's digit counts in a single read; each
- * pass is one kernel whose tiles take their global offsets from a chained
- * (decoupled look-back) scan over per-tile digit counts.  A look-back wait
- * is bounded: if it ever exceeds its budget the kernel sets an error flag,
- * finishes, and the host reports a failure. */
-
-constexpr int OS_THREADS = 256, OS_WAVES = OS_THREADS / 64;
-constexpr int OS_KPT_MIN = 16;   /* keys per thread: 16 or 32 (tiles of 4096 or 8192 keys) */
-constexpr int OS_TILE_MIN = OS_THREADS * OS_KPT_MIN;
-static int g_sort_kpt = 32;
-static int g_sort_lbw = 16;   /* look-back window of the radix passes: 16, 32 or 64 tiles */
-constexpr int OS_BITS = 8, OS_R = 1 << OS_BITS;
-constexpr int OS_MAXPASS = 8;
-constexpr uint64_t OS_AGG = 1ULL << 62, OS_INC = 2ULL << 62, OS_VAL = (1ULL << 62) - 1;
-constexpr int OS_SPIN_LIMIT = 1 << 22;
-
-__device__ __forceinline__ uint64_t xform(uint64_t k, const KeyXform &x)
-{
-    /* branch-free: unused segments have mask 0 and shift 0 */
-    uint64_t r = 0;
-#pragma unroll
-    for (int s = 0; s < 4; s++) {
-        uint64_t part = (k >> x.lo[s]) & x.mask[s];
-        r = (x.len[s] >= 64 ? 0ULL : (r << x.len[s])) | part;
-    }
-    return r;
-}
-
-__device__ __forceinline__ uint64_t ld_flag(const uint64_t *p)
-{
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__device__ __forceinline__ void st_flag(uint64_t *p, uint64_t v)
-{
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-/* waits until tile t's status is published; bounded */
-__device__ __forceinline__ uint64_t wait_flag(const uint64_t *p, unsigned *err)
-{
-    uint64_t v;
-    int spins = 0;
-    while (((v = ld_flag(p)) >> 62) == 0) {
-        if (++spins > OS_SPIN_LIMIT) {
-            atomicOr(err, 1u);
-            return OS_INC;
-        }
-        __builtin_amdgcn_s_sleep(2);
-    }
-    return v;
-}
-
-/* Exclusive prefix for one chained-scan lane: status of tile i at
- * status[i * stride].  The flags are coherent across XCDs, so every load is
- * a long round trip and the loads themselves cost bandwidth: the window
- * grows 1, 2, 4, ... up to LB_W predecessors per step (usually the nearest
- * one or two already carry an inclusive prefix), sums aggregates
- * nearest-first, and waits only on the first predecessor that has not
- * published yet. */
-template <int LB_W>
-__device__ __forceinline__ uint64_t lookback(const uint64_t *status, size_t stride, size_t tile,
-                                             unsigned *err)
-{
-    uint64_t excl = 0;
-    long t = (long)tile - 1;
-    int win = 1;
-    while (t >= 0) {
-        uint64_t v[LB_W];
-#pragma unroll
-        for (int j = 0; j < LB_W; j++)
-            v[j] = (j < win && t - j >= 0) ? ld_flag(status + (size_t)(t - j) * stride) : OS_INC;
-        int j = 0;
-        for (; j < win; j++) {
-            uint64_t f = v[j] >> 62;
-            if (f == 0) break;
-            excl += v[j] & OS_VAL;
-            if (f == 2) return excl;
-        }
-        if (j == win) {
-            t -= win;
-            win = win * 2 < LB_W ? win * 2 : LB_W;
-            continue;
-        }
-        uint64_t w = wait_flag(status + (size_t)(t - j) * stride, err);
-        excl += w & OS_VAL;
-        if ((w >> 62) == 2) return excl;
-        t = t - j - 1;
-    }
-    return excl;
-}
-
-__global__ __launch_bounds__(OS_THREADS) void k_os_hist(const uint64_t *__restrict__ keys, size_t n,
-                                                        KeyXform x, int npass, uint32_t *ghist)
-{
-    __shared__ uint32_t h[OS_WAVES][OS_MAXPASS][OS_R];
-    for (int i = threadIdx.x; i < OS_WAVES * OS_MAXPASS * OS_R; i += OS_THREADS)
-        (&h[0][0][0])[i] = 0;
-    __syncthreads();
-    const int w = threadIdx.x >> 6;
-    constexpr int HB = 8;   /* keys loaded per thread before counting */
-    for (size_t b0 = (size_t)blockIdx.x * OS_THREADS * HB; b0 < n;
-         b0 += (size_t)gridDim.x * OS_THREADS * HB) {
-        uint64_t kk[HB];
-#pragma unroll
-        for (int j = 0; j < HB; j++) {
-            size_t i = b0 + (size_t)j * OS_THREADS + threadIdx.x;
-            kk[j] = i < n ? keys[i] : 0;
-        }
-#pragma unroll
-        for (int j = 0; j < HB; j++) {
-            size_t i = b0 + (size_t)j * OS_THREADS + threadIdx.x;
-            if (i >= n) continue;
-            uint64_t c = xform(kk[j], x);
-            for (int p = 0; p < npass; p++)
-                atomicAdd(&h[w][p][(c >> (OS_BITS * p)) & (OS_R - 1)], 1u);
-        }
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < npass * OS_R; i += OS_THREADS) {
-        int p = i / OS_R, d = i % OS_R;
-        uint32_t t = h[0][p][d] + h[1][p][d] + h[2][p][d] + h[3][p][d];
-        if (t) atomicAdd(&ghist[i], t);
-    }
-}
-
-/* exclusive scan of every pass's 256 digit counts */
-__global__ __launch_bounds__(OS_R) void k_os_scan(const uint32_t *ghist, int npass, uint64_t *gbase)
-{
-    __shared__ uint64_t wtmp[17];
-    for (int p = 0; p < npass; p++) {
-        uint64_t tot;
-        uint64_t ex = block_excl_scan<uint64_t>(ghist[p * OS_R + threadIdx.x], &tot, wtmp);
-        gbase[p * OS_R + threadIdx.x] = ex;
-    }
-}
-
-template <int OS_KPT, int LB_W>
-__global__ __launch_bounds__(OS_THREADS) void k_os_pass(const uint64_t *__restrict__ in,
-                                                        uint64_t *__restrict__ out, size_t n,
-                                                        KeyXform x, int shift,
-                                                        const uint64_t *__restrict__ gbase,
-                                                        uint64_t *status, unsigned *tile_ctr,
-                                                        unsigned *err)
-{
-    constexpr int OS_TILE = OS_THREADS * OS_KPT, OS_WCHUNK = 64 * OS_KPT;
-    __shared__ uint32_t wcnt[OS_WAVES * OS_R];
-    __shared__ uint32_t wpre[OS_WAVES * OS_R];
-    __shared__ uint32_t tstart[OS_R];
-    __shared__ uint64_t obase[OS_R];
-    __shared__ uint64_t stage[OS_TILE];
-    __shared__ uint32_t wtmp[17];
-    __shared__ unsigned tile_sh;
-
-    const int w = threadIdx.x >> 6, lane = lane_id();
-    if (threadIdx.x == 0) tile_sh = atomicAdd(tile_ctr, 1u);
-    for (int i = threadIdx.x; i < OS_WAVES * OS_R; i += OS_THREADS) wcnt[i] = 0;
-    __syncthreads();
-    const size_t tile = tile_sh;
-    const size_t base = tile * OS_TILE + (size_t)w * OS_WCHUNK;
-    uint64_t key[OS_KPT];
-    uint32_t rank[OS_KPT];
-    const uint64_t lt = lanemask_lt();
-#pragma unroll
-    for (int j = 0; j < OS_KPT; j++) {
-        size_t i = base + (size_t)j * 64 + lane;
-        key[j] = i < n ? in[i] : 0;
-    }
-#pragma unroll
-    for (int j = 0; j < OS_KPT; j++) {
-        size_t i = base + (size_t)j * 64 + lane;
-        bool valid = i < n;
-        uint64_t kk = key[j];
-        unsigned d = (unsigned)(xform(kk, x) >> shift) & (OS_R - 1);
-        uint64_t peers = __ballot(valid);
-#pragma unroll
-        for (int bit = 0; bit < OS_BITS; bit++) {
-            bool s = (d >> bit) & 1u;
-            uint64_t bb = __ballot(s);
-            peers &= s ? bb : ~bb;
-        }
-        uint32_t before = (uint32_t)__popcll(peers & lt);
-        uint32_t cnt = wcnt[w * OS_R + d];
-        __builtin_amdgcn_wave_barrier();
-        if (valid && before == 0) wcnt[w * OS_R + d] = cnt + (uint32_t)__popcll(peers);
-        __builtin_amdgcn_wave_barrier();
-        rank[j] = cnt + before;
-    }
-    __syncthreads();
-    /* thread d: wave prefixes and the tile's count of digit d */
-    const int d = threadIdx.x;
-    uint32_t tot = 0;
-#pragma unroll
-    for (int ww = 0; ww < OS_WAVES; ww++) {
-        wpre[ww * OS_R + d] = tot;
-        tot += wcnt[ww * OS_R + d];
-    }
-    /* chained scan over tiles for digit d */
-    uint64_t *my = status + tile * OS_R + d;
-    uint64_t excl = 0;
-    if (tile == 0) {
-        st_flag(my, OS_INC | tot);
-    } else {
-        st_flag(my, OS_AGG | tot);
-        excl = lookback<LB_W>(status + d, OS_R, tile, err);
-        st_flag(my, OS_INC | (excl + tot));
-    }
-    obase[d] = gbase[d] + excl;
-    uint32_t all;
-    tstart[d] = block_excl_scan<uint32_t>(tot, &all, wtmp);   /* OS_THREADS == OS_R */
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j < OS_KPT; j++) {
-        size_t i = base + (size_t)j * 64 + lane;
-        if (i < n) {
-            unsigned dd = (unsigned)(xform(key[j], x) >> shift) & (OS_R - 1);
-            stage[tstart[dd] + wpre[w * OS_R + dd] + rank[j]] = key[j];
-        }
-    }
-    __syncthreads();
-    size_t tile0 = tile * OS_TILE;
-    uint32_t tn = (uint32_t)((n - tile0) < (size_t)OS_TILE ? (n - tile0) : (size_t)OS_TILE);
-    for (uint32_t p = threadIdx.x; p < tn; p += OS_THREADS) {
-        uint64_t kk = stage[p];
-        unsigned dd = (unsigned)(xform(kk, x) >> shift) & (OS_R - 1);
-        out[obase[dd] + (p - tstart[dd])] = kk;
-    }
-}
-
-/* unique compaction of a sorted array with a chained scan over tiles;
- * the last tile writes the total to *total */
-template <int OS_KPT>
-__global__ __launch_bounds__(OS_THREADS) void k_os_unique(const uint64_t *__restrict__ k, size_t n,
-                                                          uint64_t *__restrict__ dst,
-                                                          uint64_t *status, unsigned *tile_ctr,
-                                                          unsigned long long *total, unsigned *err)
-{
-    constexpr int OS_TILE = OS_THREADS * OS_KPT, OS_WCHUNK = 64 * OS_KPT;
-    __shared__ unsigned tile_sh;
-    __shared__ uint32_t wc[OS_WAVES];
-    __shared__ uint64_t excl_sh;
-    const int w = threadIdx.x >> 6, lane = lane_id();
-    if (threadIdx.x == 0) tile_sh = atomicAdd(tile_ctr, 1u);
-    __syncthreads();
-    const size_t tile = tile_sh;
-    const size_t base = tile * OS_TILE + (size_t)w * OS_WCHUNK;
-    const uint64_t lt = lanemask_lt();
-    uint32_t c = 0;
-    uint64_t m[OS_KPT], kv[OS_KPT];
-    /* one load per key; the predecessor comes from the neighbouring lane,
-     * lane 0 of each row reads the key before the row */
-#pragma unroll
-    for (int j = 0; j < OS_KPT; j++) {
-        size_t i = base + (size_t)j * 64 + lane;
-        kv[j] = i < n ? k[i] : 0;
-    }
-    uint64_t prev0[OS_KPT];
-#pragma unroll
-    for (int j = 0; j < OS_KPT; j++) {
-        size_t i0 = base + (size_t)j * 64;
-        prev0[j] = (lane == 0 && i0 > 0 && i0 <= n) ? k[i0 - 1] : 0;
-    }
-#pragma unroll
-    for (int j = 0; j < OS_KPT; j++) {
-        size_t i = base + (size_t)j * 64 + lane;
-        uint64_t pv = __shfl_up(kv[j], 1, 64);
-        if (lane == 0) pv = prev0[j];
-        bool f = i < n && (i == 0 || kv[j] != pv);
-        m[j] = __ballot(f);
-        c += (uint32_t)__popcll(m[j]);
-    }
-    if (lane == 0) wc[w] = c;
-    __syncthreads();
-    if (w == 0) {
-        /* wave 0 publishes and looks back 64 tiles per step (lane i reads tile t - i) */
-        uint32_t tot = wc[0] + wc[1] + wc[2] + wc[3];
-        uint64_t *my = status + tile;
-        uint64_t excl = 0;
-        if (tile == 0) {
-            if (lane == 0) st_flag(my, OS_INC | tot);
-        } else {
-            if (lane == 0) st_flag(my, OS_AGG | tot);
-            long t = (long)tile - 1;
-            while (t >= 0) {
-                long ti = t - lane;
-                uint64_t v = ti >= 0 ? ld_flag(status + ti) : OS_INC;
-                uint64_t f = v >> 62;
-                uint64_t notready = __ballot(f == 0), inc = __ballot(f == 2);
-                /* nearest lane that is not ready, and nearest inclusive one */
-                int nr = notready ? __ffsll((long long)notready) - 1 : 64;
-                int ni = inc ? __ffsll((long long)inc) - 1 : 64;
-                int stop = nr < ni ? nr : ni;          /* lanes before `stop` are aggregates */
-                uint64_t part = (lane < stop || (lane == ni && ni < nr)) ? (v & OS_VAL) : 0;
-#pragma unroll
-                for (int dd = 32; dd >= 1; dd >>= 1) part += __shfl_xor(part, dd, 64);
-                excl += part;
-                if (ni < nr) break;                    /* reached an inclusive prefix */
-                if (nr < 64) {
-                    /* wait for the nearest unpublished tile, then continue past it */
-                    uint64_t wv = 0;
-                    if (lane == 0) wv = wait_flag(status + (t - nr), err);
-                    wv = __shfl(wv, 0, 64);
-                    excl += wv & OS_VAL;
-                    if ((wv >> 62) == 2) break;
-                    t = t - nr - 1;
-                } else {
-                    t -= 64;
-                }
-            }
-            if (lane == 0) st_flag(my, OS_INC | (excl + tot));
-        }
-        if (lane == 0) {
-            excl_sh = excl;
-            if ((tile + 1) * OS_TILE >= n) *total = excl + tot;
-        }
-    }
-    __syncthreads();
-    uint64_t o = excl_sh;
-    for (int ww = 0; ww < w; ww++) o += wc[ww];
-#pragma unroll
-    for (int j = 0; j < OS_KPT; j++) {
-        size_t i = base + (size_t)j * 64 + lane;
-        if ((m[j] >> lane) & 1ULL) dst[o + __popcll(m[j] & lt)] = kv[j];
-        (void)i;
-        o += __popcll(m[j]);
-    }
-}
-
-KeyXform make_xform(uint64_t vary)
-{
-    KeyXform x;
-    for (int i = 0; i < 4; i++) x.lo[i] = 0, x.len[i] = 0, x.mask[i] = 0;
-    x.nseg = 0;
-    x.nbits = 0;
-    if (!vary) return x;
-    int lo_s[32], len_s[32], ns = 0;
-    int b = 63;
-    while (b >= 0) {   /* runs of set bits, most significant first */
-        if (!((vary >> b) & 1)) {
-            b--;
-            continue;
-        }
-        int hi = b;
-        while (b >= 0 && ((vary >> b) & 1)) b--;
-        lo_s[ns] = b + 1;
-        len_s[ns] = hi - b;
-        ns++;
-    }
-    if (ns > 4) {
-        /* too fragmented: one contiguous range from the lowest to the highest varying bit */
-        int top = 63 - __builtin_clzll(vary), bot = __builtin_ctzll(vary);
-        ns = 1;
-        lo_s[0] = bot;
-        len_s[0] = top - bot + 1;
-    }
-    for (int i = 0; i < ns; i++) {
-        x.lo[i] = lo_s[i];
-        x.len[i] = len_s[i];
-        x.mask[i] = len_s[i] >= 64 ? ~0ULL : ((1ULL << len_s[i]) - 1);
-        x.nbits += len_s[i];
-    }
-    x.nseg = ns;
-    return x;
-}
-
-size_t sort_scratch_bytes(size_t n)
-{
-    size_t tiles = (n + OS_TILE_MIN - 1) / OS_TILE_MIN + 1;
-    return 4096 + OS_MAXPASS * OS_R * (4 + 8) + tiles * OS_R * 8 + tiles * 8 + 256;
-}
-
-struct OsScratch {
-    unsigned *ctr;        /* OS_MAXPASS + 1 tile counters */
-    uint32_t *ghist;
-    uint64_t *gbase;
-    uint64_t *status;
-    uint64_t *ustatus;
-};
-
-static OsScratch carve(void *scratch, size_t n)
-{
-    size_t tiles = (n + OS_TILE_MIN - 1) / OS_TILE_MIN + 1;
-    char *p = (char *)scratch;
-    OsScratch s;
-    s.ctr = (unsigned *)p;
-    p += 256;
-    s.ghist = (uint32_t *)p;
-    p += OS_MAXPASS * OS_R * 4;
-    s.gbase = (uint64_t *)p;
-    p += OS_MAXPASS * OS_R * 8;
-    s.ustatus = (uint64_t *)p;
-    p += (tiles * 8 + 255) / 256 * 256;
-    s.status = (uint64_t *)p;
-    return s;
-}
-
-static void launch_pass(hipStream_t st, int kpt, int lbw, unsigned tiles, const uint64_t *src,
-                        uint64_t *dst, size_t n, KeyXform x, int shift, const uint64_t *gbase,
-                        uint64_t *status, unsigned *ctr, unsigned *err)
-{
-#define CG_PASS(K, W)                                                                          \
-    hipLaunchKernelGGL((k_os_pass<K, W>), dim3(tiles), dim3(OS_THREADS), 0, st, src, dst, n, x, \
-                       shift, gbase, status, ctr, err)
-    if (kpt == 32) {
-        if (lbw >= 64) CG_PASS(32, 64);
-        else if (lbw >= 32) CG_PASS(32, 32);
-        else CG_PASS(32, 16);
-    } else {
-        if (lbw >= 64) CG_PASS(16, 64);
-        else if (lbw >= 32) CG_PASS(16, 32);
-        else CG_PASS(16, 16);
-    }
-#undef CG_PASS
-}
-
-uint64_t *sort_unique(hipStream_t st, uint64_t *keys, uint64_t *tmp, size_t n, uint64_t vary,
-                      void *scratch, uint64_t *dst, unsigned long long *d_nuniq, unsigned *d_err,
-                      int *prof_unique)
-{
-    *prof_unique = -1;
-    if (n == 0) {
-        launch_fill_u64(st, (uint64_t *)d_nuniq, 1, 0);
-        return keys;
-    }
-    OsScratch s = carve(scratch, n);
-    const int kpt = g_sort_kpt == 32 ? 32 : 16;
-    const size_t tile_keys = (size_t)OS_THREADS * kpt;
-    const size_t tiles = (n + tile_keys - 1) / tile_keys;
-    KeyXform x = make_xform(n > 1 ? vary : 0);
-    int npass = (x.nbits + OS_BITS - 1) / OS_BITS;
-    /* counters, error flag and histograms (256 + 8 KiB) cleared together */
-    launch_fill_u64(st, (uint64_t *)scratch, (256 + OS_MAXPASS * OS_R * 4) / 8, 0);
-    uint64_t *src = keys, *dst2 = tmp;
-    if (npass > 0) {
-        size_t hb = (n + OS_TILE_MIN - 1) / OS_TILE_MIN;
-        if (hb > 2048) hb = 2048;
-        int h = prof_hook_begin(2);
-        hipLaunchKernelGGL(k_os_hist, dim3((unsigned)hb), dim3(OS_THREADS), 0, st, keys, n, x, npass,
-                           s.ghist);
-        prof_hook_end(h, 8.0 * n);
-        hipLaunchKernelGGL(k_os_scan, dim3(1), dim3(OS_R), 0, st, s.ghist, npass, s.gbase);
-        for (int p = 0; p < npass; p++) {
-            launch_fill_u64(st, s.status, tiles * OS_R, 0);
-            h = prof_hook_begin(3);
-            launch_pass(st, kpt, g_sort_lbw, (unsigned)tiles, src, dst2, n, x, OS_BITS * p,
-                        s.gbase + OS_R * p, s.status, s.ctr + p, d_err);
-            prof_hook_end(h, 16.0 * n);
-            uint64_t *t = src;
-            src = dst2;
-            dst2 = t;
-        }
-    }
-    launch_fill_u64(st, s.ustatus, tiles, 0);
-    *prof_unique = prof_hook_begin(4);
-    if (kpt == 32)
-        hipLaunchKernelGGL(k_os_unique<32>, dim3((unsigned)tiles), dim3(OS_THREADS), 0, st, src, n,
-                           dst, s.ustatus, s.ctr + OS_MAXPASS, d_nuniq, d_err);
-    else
-        hipLaunchKernelGGL(k_os_unique<16>, dim3((unsigned)tiles), dim3(OS_THREADS), 0, st, src, n,
-                           dst, s.ustatus, s.ctr + OS_MAXPASS, d_nuniq, d_err);
-    prof_hook_end(*prof_unique, 8.0 * n);
-    return src;
-}
-
-/* ------------------------------------------------------------------------ */
 /* prefix-bucket directory + lookup                                          */
 /* ------------------------------------------------------------------------ */
+
+static int g_dir_delta = 0;
+void set_dir_delta(int d) { g_dir_delta = d < -4 ? -4 : (d > 3 ? 3 : d); }
+int get_dir_delta() { return g_dir_delta; }
 
 LayerIndexPlan plan_index(size_t n, uint64_t key_or, uint64_t key_and)
 {
@@ -800,7 +307,9 @@ LayerIndexPlan plan_index(size_t n, uint64_t key_or, uint64_t key_and)
     p.cval = key_and & p.cmask;
     int lg = 0;
     while (((size_t)1 << (lg + 1)) <= n) lg++;
-    int bits = lg < p.x.nbits ? lg : p.x.nbits;
+    int bits = lg + g_dir_delta;
+    if (bits < 0) bits = 0;
+    if (bits > p.x.nbits) bits = p.x.nbits;
     if (bits > 30) bits = 30;
     p.bits = bits;
     p.shift = p.x.nbits - bits;
@@ -1293,13 +802,18 @@ __device__ __forceinline__ uint64_t benc(uint64_t b, const BatchTable &T, uint64
 {
     if ((b & T.fnib) != T.fnib) atomicOr(err, 2u);   /* formation left its canonical place */
     uint64_t c = 0;
-    for (int j = 0; j < T.nfree; j++) c = (c << 4) | ((b >> (60 - 4 * T.pos[j])) & 0xf);
+    if (T.runs) {
+        c = xform(b, T.fx);
+    } else {
+        for (int j = 0; j < T.nfree; j++) c = (c << 4) | ((b >> (60 - 4 * T.pos[j])) & 0xf);
+    }
     return (tid << BATCH_TID_SHIFT) | c;
 }
 
 __device__ __forceinline__ uint64_t bdec(uint64_t ck, const BatchTable &T)
 {
     uint64_t c = ck & ((1ULL << BATCH_TID_SHIFT) - 1);
+    if (T.runs) return T.fnib | xform_inv(c, T.fx);
     uint64_t b = T.fnib;
     for (int j = T.nfree - 1; j >= 0; j--) {
         b |= (c & 0xf) << (60 - 4 * T.pos[j]);
@@ -1309,6 +823,7 @@ __device__ __forceinline__ uint64_t bdec(uint64_t ck, const BatchTable &T)
 }
 
 __global__ __launch_bounds__(MV_THREADS) void k_bmove(const uint64_t *__restrict__ in, size_t n,
+                                                      const unsigned long long *dn,
                                                       uint64_t *__restrict__ out,
                                                       unsigned long long *stats,
                                                       const uint16_t *__restrict__ glut, MoveRules r,
@@ -1325,6 +840,7 @@ __global__ __launch_bounds__(MV_THREADS) void k_bmove(const uint64_t *__restrict
         uint4 *l = reinterpret_cast<uint4 *>(lut);
         for (int i = threadIdx.x; i < 65536 / 8; i += MV_THREADS) l[i] = g[i];
     }
+    if (dn && *dn < n) n = (size_t)*dn;
     __syncthreads();
     const size_t stride = (size_t)gridDim.x * MV_THREADS;
     for (size_t b0 = (size_t)blockIdx.x * MV_THREADS; b0 < n; b0 += stride) {
@@ -1362,8 +878,9 @@ __global__ __launch_bounds__(MV_THREADS) void k_bmove(const uint64_t *__restrict
 }
 
 __global__ __launch_bounds__(SP_THREADS) void k_bspawn(const uint64_t *__restrict__ in, size_t n,
+                                                       const unsigned long long *dn,
                                                        uint64_t *__restrict__ out2,
-                                                       uint64_t *__restrict__ out4,
+                                                       uint64_t *__restrict__ out4, size_t cap,
                                                        unsigned long long *stats,
                                                        const BatchTable *__restrict__ tabs, int step,
                                                        unsigned *err)
@@ -1372,6 +889,7 @@ __global__ __launch_bounds__(SP_THREADS) void k_bspawn(const uint64_t *__restric
     __shared__ unsigned long long base_sh;
     __shared__ uint64_t ro[16], ra[16];
     uint64_t or2 = 0, and2 = ~0ULL, or4 = 0, and4 = ~0ULL;
+    if (dn && *dn < n) n = (size_t)*dn;
     const size_t stride = (size_t)gridDim.x * SP_THREADS;
     for (size_t b0 = (size_t)blockIdx.x * SP_THREADS; b0 < n; b0 += stride) {
         size_t i = b0 + threadIdx.x;
@@ -1396,8 +914,10 @@ __global__ __launch_bounds__(SP_THREADS) void k_bspawn(const uint64_t *__restric
             const BatchTable &T = tabs[tid];
             uint64_t c2 = benc(cg_canon(cg_set_tile(b, x, 1)), T, tid, err);
             uint64_t c4 = benc(cg_canon(cg_set_tile(b, x, 2)), T, tid, err);
-            out2[o] = c2;
-            out4[o] = c4;
+            if (o < cap) {
+                out2[o] = c2;
+                out4[o] = c4;
+            }
             or2 |= c2;
             and2 &= c2;
             or4 |= c4;
@@ -1411,40 +931,70 @@ __global__ __launch_bounds__(SP_THREADS) void k_bspawn(const uint64_t *__restric
 }
 
 /* composite -> boards; per-table count, OR and AND of the boards
- * (tstat[3 t + 0..2]); keys are sorted, so a wave whose lanes share one
- * table reduces first and adds once */
+ * (tstat[3 t + 0..2]).  The keys are sorted, so tables are contiguous: each
+ * wave walks its own contiguous chunk of 64-key rows with a running (table,
+ * count, OR, AND) accumulator and adds it to the table's counters only when
+ * the table changes and at the end (a few thousand atomics per launch rather
+ * than three per row on the same addresses); a row that straddles a table
+ * boundary adds per key. */
 __global__ __launch_bounds__(256) void k_bdecode(const uint64_t *__restrict__ in, size_t n,
+                                                 const unsigned long long *dn,
                                                  uint64_t *__restrict__ out,
                                                  const BatchTable *__restrict__ tabs,
                                                  unsigned long long *tstat)
 {
-    size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
-    bool valid = i < n;
-    uint64_t tid = ~0ULL, b = 0;
-    if (valid) {
-        uint64_t ck = in[i];
-        tid = ck >> BATCH_TID_SHIFT;
-        b = bdec(ck, tabs[tid]);
-        out[i] = b;
-    }
-    uint64_t t0 = __shfl(tid, 0, 64);
-    bool same = __ballot(valid && tid == t0) == __ballot(valid) && __ballot(valid) == ~0ULL;
-    if (same) {
-        uint64_t o = b, a = b;
+    const int lane = lane_id();
+    if (dn && *dn < n) n = (size_t)*dn;
+    const size_t rows = (n + 63) / 64;
+    const size_t nwaves = (size_t)gridDim.x * 4;
+    const size_t wv = ((size_t)blockIdx.x * 256 + threadIdx.x) >> 6;
+    const size_t per = (rows + nwaves - 1) / nwaves;
+    const size_t r0 = wv * per, r1 = r0 + per < rows ? r0 + per : rows;
+    uint64_t cur = ~0ULL, cor = 0, cand = ~0ULL;   /* wave-uniform accumulator */
+    unsigned long long ccnt = 0;
+    for (size_t r = r0; r < r1; r++) {
+        size_t i = r * 64 + lane;
+        bool valid = i < n;
+        uint64_t tid = ~0ULL, b = 0;
+        if (valid) {
+            uint64_t ck = in[i];
+            tid = ck >> BATCH_TID_SHIFT;
+            b = bdec(ck, tabs[tid]);
+            out[i] = b;
+        }
+        uint64_t vm = __ballot(valid);
+        uint64_t t0 = __shfl(tid, 0, 64);
+        if (__ballot(valid && tid == t0) == vm) {
+            uint64_t o = valid ? b : 0, a = valid ? b : ~0ULL;
 #pragma unroll
-        for (int d = 32; d >= 1; d >>= 1) {
-            o |= __shfl_xor(o, d, 64);
-            a &= __shfl_xor(a, d, 64);
+            for (int d = 32; d >= 1; d >>= 1) {
+                o |= __shfl_xor(o, d, 64);
+                a &= __shfl_xor(a, d, 64);
+            }
+            if (t0 != cur) {
+                if (cur != ~0ULL && lane == 0) {
+                    atomicAdd(&tstat[3 * cur], ccnt);
+                    atomicOr(&tstat[3 * cur + 1], (unsigned long long)cor);
+                    atomicAnd(&tstat[3 * cur + 2], (unsigned long long)cand);
+                }
+                cur = t0;
+                ccnt = 0;
+                cor = 0;
+                cand = ~0ULL;
+            }
+            ccnt += (unsigned long long)__popcll(vm);
+            cor |= o;
+            cand &= a;
+        } else if (valid) {
+            atomicAdd(&tstat[3 * tid], 1ULL);
+            atomicOr(&tstat[3 * tid + 1], (unsigned long long)b);
+            atomicAnd(&tstat[3 * tid + 2], (unsigned long long)b);
         }
-        if (lane_id() == 0) {
-            atomicAdd(&tstat[3 * t0], 64ULL);
-            atomicOr(&tstat[3 * t0 + 1], (unsigned long long)o);
-            atomicAnd(&tstat[3 * t0 + 2], (unsigned long long)a);
-        }
-    } else if (valid) {
-        atomicAdd(&tstat[3 * tid], 1ULL);
-        atomicOr(&tstat[3 * tid + 1], (unsigned long long)b);
-        atomicAnd(&tstat[3 * tid + 2], (unsigned long long)b);
+    }
+    if (cur != ~0ULL && lane == 0) {
+        atomicAdd(&tstat[3 * cur], ccnt);
+        atomicOr(&tstat[3 * cur + 1], (unsigned long long)cor);
+        atomicAnd(&tstat[3 * cur + 2], (unsigned long long)cand);
     }
 }
 
@@ -1480,33 +1030,36 @@ __global__ __launch_bounds__(256) void k_bdir_count(const uint64_t *__restrict__
     }
 }
 
-void launch_bmove(hipStream_t st, const uint64_t *in, size_t n, uint64_t *out,
-                  unsigned long long *stats, const uint16_t *lut, MoveRules r, const BatchTable *tabs,
-                  int step, unsigned *err)
+void launch_bmove(hipStream_t st, const uint64_t *in, size_t n, const unsigned long long *dn,
+                  uint64_t *out, unsigned long long *stats, const uint16_t *lut, MoveRules r,
+                  const BatchTable *tabs, int step, unsigned *err)
 {
     if (!n) return;
     size_t blocks = (n + MV_THREADS - 1) / MV_THREADS;
     if (blocks > 512) blocks = 512;
-    hipLaunchKernelGGL(k_bmove, dim3((unsigned)blocks), dim3(MV_THREADS), 0, st, in, n, out, stats,
-                       lut, r, tabs, step, err);
+    hipLaunchKernelGGL(k_bmove, dim3((unsigned)blocks), dim3(MV_THREADS), 0, st, in, n, dn, out,
+                       stats, lut, r, tabs, step, err);
 }
 
-void launch_bspawn(hipStream_t st, const uint64_t *in, size_t n, uint64_t *out2, uint64_t *out4,
-                   unsigned long long *stats, const BatchTable *tabs, int step, unsigned *err)
+void launch_bspawn(hipStream_t st, const uint64_t *in, size_t n, const unsigned long long *dn,
+                   uint64_t *out2, uint64_t *out4, size_t cap, unsigned long long *stats,
+                   const BatchTable *tabs, int step, unsigned *err)
 {
     if (!n) return;
     size_t blocks = (n + SP_THREADS - 1) / SP_THREADS;
     if (blocks > 1024) blocks = 1024;
-    hipLaunchKernelGGL(k_bspawn, dim3((unsigned)blocks), dim3(SP_THREADS), 0, st, in, n, out2, out4,
-                       stats, tabs, step, err);
+    hipLaunchKernelGGL(k_bspawn, dim3((unsigned)blocks), dim3(SP_THREADS), 0, st, in, n, dn, out2,
+                       out4, cap, stats, tabs, step, err);
 }
 
-void launch_bdecode(hipStream_t st, const uint64_t *in, size_t n, uint64_t *out,
-                    const BatchTable *tabs, unsigned long long *tstat)
+void launch_bdecode(hipStream_t st, const uint64_t *in, size_t n, const unsigned long long *dn,
+                    uint64_t *out, const BatchTable *tabs, unsigned long long *tstat)
 {
     if (!n) return;
-    hipLaunchKernelGGL(k_bdecode, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, in, n, out,
-                       tabs, tstat);
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 1024) blocks = 1024;
+    hipLaunchKernelGGL(k_bdecode, dim3((unsigned)blocks), dim3(256), 0, st, in, n, dn, out, tabs,
+                       tstat);
 }
 
 void launch_bdir(hipStream_t st, const uint64_t *keys, size_t n, const size_t *seg, int ntab,
@@ -1561,9 +1114,3 @@ void launch_bsolve(hipStream_t st, const BSolveDesc *desc, int ntab, size_t tota
 
 } // namespace cg
 
-namespace cg {
-void set_sort_kpt(int kpt) { g_sort_kpt = kpt == 32 ? 32 : 16; }
-void set_sort_lbw(int w) { g_sort_lbw = w >= 64 ? 64 : (w >= 32 ? 32 : 16); }
-int get_sort_lbw() { return g_sort_lbw; }
-int get_sort_kpt() { return g_sort_kpt; }
-}

@@ -1,0 +1,673 @@
+/*
+ * sort.hip -- onesweep LSD radix sort + unique compaction of u64 keys (gfx950).
+ *
+ * The bits that differ between keys (OR & ~AND of the key set, produced by
+ * the kernels that wrote the keys) are packed into a dense key on the fly
+ * (an order-preserving bijection on the key set), so a DPDGap layer with 40
+ * varying bits spread over bits 0..51 sorts in 5 passes of 8 bits, or 4 of
+ * 10.  One histogram kernel computes every pass's digit counts in a single
+ * read; each pass is one kernel whose tiles take their global offsets from a
+ * chained (decoupled look-back) scan over per-tile digit counts; the unique
+ * compaction is one more chained-scan kernel.
+ *
+ * Digit width is a template parameter (8, 9 or 10 bits): a pass block has one
+ * thread per digit (256, 512 or 1024 threads), ranks its keys with one wave
+ * ballot per digit bit (match-any), stages the tile in LDS in digit order and
+ * writes each digit's run contiguously.  Keys per thread sets the tile
+ * (4096 to 16384 keys).  Which (digit width, tile, look-back window) a sort
+ * uses is looked up per (size class, number of varying bits) in a table the
+ * autotuner fills; the default is 8 bits, 8192-key tiles.
+ *
+ * Look-back flags live in a sorter-owned status array: each word holds a
+ * state (aggregate / inclusive prefix), the epoch of the launch that wrote it
+ * and the value, so a launch ignores words left by earlier launches and the
+ * array is cleared only when it grows or the epoch counter wraps.  A
+ * look-back wait is bounded: past its budget it sets an error flag and the
+ * kernel finishes; the host reports a failure.
+ */
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include "kernels.h"
+#include "devutil.h"
+
+namespace cg {
+
+constexpr int OS_MAXPASS = 8;
+constexpr int OS_MAXR = 1024;
+constexpr int OS_VALBITS = 42, OS_EPOCHBITS = 20;
+constexpr uint64_t OS_VAL = (1ULL << OS_VALBITS) - 1;
+constexpr uint64_t OS_AGG = 1ULL << 62, OS_INC = 2ULL << 62;
+constexpr uint32_t OS_EPOCH_MAX = (1u << OS_EPOCHBITS) - 1;
+constexpr int OS_SPIN_LIMIT = 1 << 22;
+constexpr int UQ_THREADS = 256, UQ_KPT = 32, UQ_TILE = UQ_THREADS * UQ_KPT;
+
+__device__ __forceinline__ uint64_t ld_flag(const uint64_t *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void st_flag(uint64_t *p, uint64_t state, uint32_t epoch, uint64_t v)
+{
+    __hip_atomic_store(p, state | ((uint64_t)epoch << OS_VALBITS) | v, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+}
+
+/* 0 (not published by this launch), 1 (aggregate) or 2 (inclusive prefix) */
+__device__ __forceinline__ uint32_t fstate(uint64_t f, uint32_t epoch)
+{
+    return (uint32_t)((f >> OS_VALBITS) & OS_EPOCH_MAX) == epoch ? (uint32_t)(f >> 62) : 0u;
+}
+
+/* waits until the flag at p is published by this launch; bounded */
+__device__ __forceinline__ uint64_t wait_flag(const uint64_t *p, uint32_t epoch, unsigned *err)
+{
+    uint64_t v;
+    int spins = 0;
+    while (fstate(v = ld_flag(p), epoch) == 0) {
+        if (++spins > OS_SPIN_LIMIT) {
+            atomicOr(err, 1u);
+            return OS_INC | ((uint64_t)epoch << OS_VALBITS);
+        }
+        __builtin_amdgcn_s_sleep(2);
+    }
+    return v;
+}
+
+/* Exclusive prefix for one chained-scan lane: status of tile i at
+ * status[i * stride].  The flags are coherent across XCDs, so every load is
+ * a long round trip and the loads themselves cost bandwidth: the window
+ * grows 1, 2, 4, ... up to LB_W predecessors per step (usually the nearest
+ * one or two already carry an inclusive prefix), sums aggregates
+ * nearest-first, and waits only on the first predecessor that has not
+ * published yet. */
+template <int LB_W>
+__device__ __forceinline__ uint64_t lookback(const uint64_t *status, size_t stride, size_t tile,
+                                             uint32_t epoch, unsigned *err)
+{
+    uint64_t excl = 0;
+    long t = (long)tile - 1;
+    int win = 1;
+    while (t >= 0) {
+        uint64_t v[LB_W];
+#pragma unroll
+        for (int j = 0; j < LB_W; j++)
+            v[j] = (j < win && t - j >= 0) ? ld_flag(status + (size_t)(t - j) * stride) : 0;
+        /* fully unrolled and predicated, so v stays in registers */
+        int used = 0;
+        bool blocked = false, done = false;
+#pragma unroll
+        for (int j = 0; j < LB_W; j++) {
+            if (!blocked && !done && j < win && t - j >= 0) {
+                uint32_t f = fstate(v[j], epoch);
+                if (f == 0) {
+                    blocked = true;
+                } else {
+                    excl += v[j] & OS_VAL;
+                    used = j + 1;
+                    done = f == 2;
+                }
+            }
+        }
+        if (done) return excl;
+        t -= used;
+        if (!blocked) {
+            win = win * 2 < LB_W ? win * 2 : LB_W;
+            continue;
+        }
+        /* t is the nearest predecessor that has not published yet */
+        uint64_t w = wait_flag(status + (size_t)t * stride, epoch, err);
+        excl += w & OS_VAL;
+        if (fstate(w, epoch) == 2) return excl;
+        t -= 1;
+    }
+    return excl;
+}
+
+/* digit counts of every pass in one read of the keys; NC LDS copies of the
+ * counters spread the atomics (the four waves share them round-robin) */
+template <int BITS>
+__global__ __launch_bounds__(256) void k_os_hist(const uint64_t *__restrict__ keys, size_t n,
+                                                 KeyXform x, int npass, uint32_t *ghist)
+{
+    constexpr int R = 1 << BITS, NC = BITS == 8 ? 4 : (BITS == 9 ? 2 : 1);
+    __shared__ uint32_t h[NC][OS_MAXPASS][R];
+    for (int i = threadIdx.x; i < NC * OS_MAXPASS * R; i += 256) (&h[0][0][0])[i] = 0;
+    __syncthreads();
+    const int c = (threadIdx.x >> 6) % NC;
+    constexpr int HB = 8;   /* keys loaded per thread before counting */
+    for (size_t b0 = (size_t)blockIdx.x * 256 * HB; b0 < n; b0 += (size_t)gridDim.x * 256 * HB) {
+        uint64_t kk[HB];
+#pragma unroll
+        for (int j = 0; j < HB; j++) {
+            size_t i = b0 + (size_t)j * 256 + threadIdx.x;
+            kk[j] = i < n ? keys[i] : 0;
+        }
+#pragma unroll
+        for (int j = 0; j < HB; j++) {
+            size_t i = b0 + (size_t)j * 256 + threadIdx.x;
+            if (i >= n) continue;
+            uint64_t v = xform(kk[j], x);
+            for (int p = 0; p < npass; p++) atomicAdd(&h[c][p][(v >> (BITS * p)) & (R - 1)], 1u);
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < npass * R; i += 256) {
+        int p = i / R, d = i % R;
+        uint32_t t = 0;
+#pragma unroll
+        for (int k = 0; k < NC; k++) t += h[k][p][d];
+        if (t) atomicAdd(&ghist[i], t);
+    }
+}
+
+/* exclusive scan of every pass's digit counts */
+template <int BITS>
+__global__ __launch_bounds__(1 << BITS) void k_os_scan(const uint32_t *ghist, int npass,
+                                                       uint64_t *gbase)
+{
+    constexpr int R = 1 << BITS;
+    __shared__ uint64_t wtmp[17];
+    for (int p = 0; p < npass; p++) {
+        uint64_t tot;
+        uint64_t ex = block_excl_scan<uint64_t>(ghist[p * R + threadIdx.x], &tot, wtmp);
+        gbase[p * R + threadIdx.x] = ex;
+    }
+}
+
+/* One LSD pass: a block of R = 2^BITS threads (thread d owns digit d) sorts
+ * a tile of R * KPT keys by digit (xform(key) >> shift) & (R - 1), stably. */
+/* at least 2 waves per SIMD: the registers of a KPT = 32 block then match the
+ * two blocks per CU its 70 KiB of LDS allows (asking for 4 with KPT = 16
+ * spills) */
+template <int BITS, int KPT, int LB_W>
+__global__ __launch_bounds__(1 << BITS, 2) void k_os_pass(const uint64_t *__restrict__ in,
+                                                       uint64_t *__restrict__ out, size_t n,
+                                                       KeyXform x, int shift,
+                                                       const uint64_t *__restrict__ gbase,
+                                                       uint64_t *status, unsigned *tile_ctr,
+                                                       uint32_t epoch, unsigned *err)
+{
+    constexpr int R = 1 << BITS, NW = R / 64, TILE = R * KPT, WCHUNK = 64 * KPT;
+    static_assert(TILE < 65536, "per-wave counters are 16-bit");
+    __shared__ uint16_t wcnt[NW * R];   /* per-wave digit counts, then wave prefixes */
+    __shared__ uint32_t tstart[R];
+    __shared__ uint64_t obase[R];
+    __shared__ uint64_t stage[TILE];
+    __shared__ uint32_t wtmp[17];
+    __shared__ unsigned tile_sh;
+
+    const int w = threadIdx.x >> 6, lane = lane_id();
+    if (threadIdx.x == 0) tile_sh = atomicAdd(tile_ctr, 1u);
+    for (int i = threadIdx.x; i < NW * R; i += R) wcnt[i] = 0;
+    __syncthreads();
+    const size_t tile = tile_sh;
+    const size_t base = tile * TILE + (size_t)w * WCHUNK;
+    /* keys of this wave: rows j with j * 64 + lane < rem */
+    const int rem = base < n ? (int)((n - base) < (size_t)WCHUNK ? (n - base) : (size_t)WCHUNK) : 0;
+    uint64_t key[KPT];
+    uint32_t rank2[(KPT + 1) / 2];   /* ranks (< 64 KPT) packed two per register */
+    const uint64_t lt = lanemask_lt();
+#pragma unroll
+    for (int j = 0; j < KPT; j++) key[j] = j * 64 + lane < rem ? in[base + j * 64 + lane] : 0;
+#pragma unroll
+    for (int j = 0; j < KPT; j++) {
+        bool valid = j * 64 + lane < rem;
+        unsigned d = (unsigned)(xform(key[j], x) >> shift) & (R - 1);
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int bit = 0; bit < BITS; bit++) {
+            bool s = (d >> bit) & 1u;
+            uint64_t bb = __ballot(s);
+            peers &= s ? bb : ~bb;
+        }
+        uint32_t before = (uint32_t)__popcll(peers & lt);
+        uint32_t cnt = wcnt[w * R + d];
+        __builtin_amdgcn_wave_barrier();
+        if (valid && before == 0) wcnt[w * R + d] = (uint16_t)(cnt + (uint32_t)__popcll(peers));
+        __builtin_amdgcn_wave_barrier();
+        uint32_t r = cnt + before;
+        if (j & 1) rank2[j >> 1] |= r << 16;
+        else rank2[j >> 1] = r;
+    }
+    __syncthreads();
+    /* thread d: wave prefixes (in place) and the tile's count of digit d */
+    const int d = threadIdx.x;
+    uint32_t tot = 0;
+#pragma unroll
+    for (int ww = 0; ww < NW; ww++) {
+        uint32_t c = wcnt[ww * R + d];
+        wcnt[ww * R + d] = (uint16_t)tot;
+        tot += c;
+    }
+    /* chained scan over tiles for digit d */
+    uint64_t *my = status + tile * R + d;
+    uint64_t excl = 0;
+    if (tile == 0) {
+        st_flag(my, OS_INC, epoch, tot);
+    } else {
+        st_flag(my, OS_AGG, epoch, tot);
+        excl = lookback<LB_W>(status + d, R, tile, epoch, err);
+        st_flag(my, OS_INC, epoch, excl + tot);
+    }
+    obase[d] = gbase[d] + excl;
+    uint32_t all;
+    tstart[d] = block_excl_scan<uint32_t>(tot, &all, wtmp);
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < KPT; j++) {
+        if (j * 64 + lane < rem) {
+            /* recompute the digit rather than keep KPT digits live across the scan */
+            uint64_t kk = key[j];
+            asm volatile("" : "+v"(kk));
+            unsigned dd = (unsigned)(xform(kk, x) >> shift) & (R - 1);
+            uint32_t r = (rank2[j >> 1] >> ((j & 1) * 16)) & 0xffffu;
+            stage[tstart[dd] + wcnt[w * R + dd] + r] = kk;
+        }
+    }
+    __syncthreads();
+    size_t tile0 = tile * TILE;
+    uint32_t tn = (uint32_t)((n - tile0) < (size_t)TILE ? (n - tile0) : (size_t)TILE);
+    for (uint32_t p = threadIdx.x; p < tn; p += R) {
+        uint64_t kk = stage[p];
+        unsigned dd = (unsigned)(xform(kk, x) >> shift) & (R - 1);
+        out[obase[dd] + (p - tstart[dd])] = kk;
+    }
+}
+
+/* unique compaction of a sorted array with a chained scan over tiles;
+ * the last tile writes the total to *total */
+__global__ __launch_bounds__(UQ_THREADS) void k_os_unique(const uint64_t *__restrict__ k, size_t n,
+                                                          uint64_t *__restrict__ dst,
+                                                          uint64_t *status, unsigned *tile_ctr,
+                                                          uint32_t epoch,
+                                                          unsigned long long *total, unsigned *err)
+{
+    constexpr int WCHUNK = 64 * UQ_KPT;
+    __shared__ unsigned tile_sh;
+    __shared__ uint32_t wc[UQ_THREADS / 64];
+    __shared__ uint64_t excl_sh;
+    const int w = threadIdx.x >> 6, lane = lane_id();
+    if (threadIdx.x == 0) tile_sh = atomicAdd(tile_ctr, 1u);
+    __syncthreads();
+    const size_t tile = tile_sh;
+    const size_t base = tile * UQ_TILE + (size_t)w * WCHUNK;
+    const uint64_t lt = lanemask_lt();
+    uint32_t c = 0;
+    uint64_t m[UQ_KPT], kv[UQ_KPT];
+    /* one load per key; the predecessor comes from the neighbouring lane,
+     * lane 0 of each row reads the key before the row */
+#pragma unroll
+    for (int j = 0; j < UQ_KPT; j++) {
+        size_t i = base + (size_t)j * 64 + lane;
+        kv[j] = i < n ? k[i] : 0;
+    }
+    uint64_t prev0[UQ_KPT];
+#pragma unroll
+    for (int j = 0; j < UQ_KPT; j++) {
+        size_t i0 = base + (size_t)j * 64;
+        prev0[j] = (lane == 0 && i0 > 0 && i0 <= n) ? k[i0 - 1] : 0;
+    }
+#pragma unroll
+    for (int j = 0; j < UQ_KPT; j++) {
+        size_t i = base + (size_t)j * 64 + lane;
+        uint64_t pv = __shfl_up(kv[j], 1, 64);
+        if (lane == 0) pv = prev0[j];
+        bool f = i < n && (i == 0 || kv[j] != pv);
+        m[j] = __ballot(f);
+        c += (uint32_t)__popcll(m[j]);
+    }
+    if (lane == 0) wc[w] = c;
+    __syncthreads();
+    if (w == 0) {
+        /* wave 0 publishes and looks back 64 tiles per step (lane i reads tile t - i) */
+        uint32_t tot = wc[0] + wc[1] + wc[2] + wc[3];
+        uint64_t *my = status + tile;
+        uint64_t excl = 0;
+        if (tile == 0) {
+            if (lane == 0) st_flag(my, OS_INC, epoch, tot);
+        } else {
+            if (lane == 0) st_flag(my, OS_AGG, epoch, tot);
+            long t = (long)tile - 1;
+            while (t >= 0) {
+                long ti = t - lane;
+                uint64_t v = ti >= 0 ? ld_flag(status + ti) : 0;
+                uint32_t f = ti >= 0 ? fstate(v, epoch) : 2u;
+                uint64_t notready = __ballot(f == 0), inc = __ballot(f == 2);
+                /* nearest lane that is not ready, and nearest inclusive one */
+                int nr = notready ? __ffsll((long long)notready) - 1 : 64;
+                int ni = inc ? __ffsll((long long)inc) - 1 : 64;
+                int stop = nr < ni ? nr : ni;          /* lanes before `stop` are aggregates */
+                uint64_t part =
+                    (ti >= 0 && (lane < stop || (lane == ni && ni < nr))) ? (v & OS_VAL) : 0;
+#pragma unroll
+                for (int dd = 32; dd >= 1; dd >>= 1) part += __shfl_xor(part, dd, 64);
+                excl += part;
+                if (ni < nr) break;                    /* reached an inclusive prefix */
+                if (nr < 64) {
+                    /* wait for the nearest unpublished tile, then continue past it */
+                    uint64_t wv = 0;
+                    if (lane == 0) wv = wait_flag(status + (t - nr), epoch, err);
+                    wv = __shfl(wv, 0, 64);
+                    excl += wv & OS_VAL;
+                    if (fstate(wv, epoch) == 2) break;
+                    t = t - nr - 1;
+                } else {
+                    t -= 64;
+                }
+            }
+            if (lane == 0) st_flag(my, OS_INC, epoch, excl + tot);
+        }
+        if (lane == 0) {
+            excl_sh = excl;
+            if ((tile + 1) * UQ_TILE >= n) *total = excl + tot;
+        }
+    }
+    __syncthreads();
+    uint64_t o = excl_sh;
+    for (int ww = 0; ww < w; ww++) o += wc[ww];
+#pragma unroll
+    for (int j = 0; j < UQ_KPT; j++) {
+        if ((m[j] >> lane) & 1ULL) dst[o + __popcll(m[j] & lt)] = kv[j];
+        o += __popcll(m[j]);
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* host side                                                                 */
+/* ------------------------------------------------------------------------ */
+
+static void snote(hipError_t e) { kernels_note_error(e); }
+
+KeyXform make_xform(uint64_t vary)
+{
+    KeyXform x;
+    for (int i = 0; i < 4; i++) x.lo[i] = 0, x.len[i] = 0, x.mask[i] = 0;
+    x.nseg = 0;
+    x.nbits = 0;
+    if (!vary) return x;
+    int lo_s[32], len_s[32], ns = 0;
+    int b = 63;
+    while (b >= 0) {   /* runs of set bits, most significant first */
+        if (!((vary >> b) & 1)) {
+            b--;
+            continue;
+        }
+        int hi = b;
+        while (b >= 0 && ((vary >> b) & 1)) b--;
+        lo_s[ns] = b + 1;
+        len_s[ns] = hi - b;
+        ns++;
+    }
+    if (ns > 4) {
+        /* too fragmented: one contiguous range from the lowest to the highest varying bit */
+        int top = 63 - __builtin_clzll(vary), bot = __builtin_ctzll(vary);
+        ns = 1;
+        lo_s[0] = bot;
+        len_s[0] = top - bot + 1;
+    }
+    for (int i = 0; i < ns; i++) {
+        x.lo[i] = lo_s[i];
+        x.len[i] = len_s[i];
+        x.mask[i] = len_s[i] >= 64 ? ~0ULL : ((1ULL << len_s[i]) - 1);
+        x.nbits += len_s[i];
+    }
+    x.nseg = ns;
+    return x;
+}
+
+/* ---- configurations ---- */
+
+static const SortCfg k_cfgs[] = {
+    {8, 32, 16}, {8, 32, 32}, {8, 16, 16}, {8, 16, 32}, {9, 16, 16},
+    {9, 16, 32}, {9, 32, 16}, {9, 32, 32}, {10, 8, 16}, {10, 8, 32},
+};
+constexpr int N_CFGS = sizeof(k_cfgs) / sizeof(k_cfgs[0]);
+
+int sort_num_configs() { return N_CFGS; }
+SortCfg sort_config(int i) { return k_cfgs[i < 0 ? 0 : (i >= N_CFGS ? N_CFGS - 1 : i)]; }
+
+static bool cfg_valid(const SortCfg &c)
+{
+    for (int i = 0; i < N_CFGS; i++)
+        if (k_cfgs[i].bits == c.bits && k_cfgs[i].kpt == c.kpt && k_cfgs[i].lbw == c.lbw) return true;
+    return false;
+}
+
+static SortCfg g_table[SORT_SIZE_CLASSES][65];
+static bool g_table_init = false;
+static SortCfg g_force = {0, 0, 0};
+
+static void table_init()
+{
+    if (g_table_init) return;
+    for (int c = 0; c < SORT_SIZE_CLASSES; c++)
+        for (int b = 0; b <= 64; b++) g_table[c][b] = k_cfgs[0];
+    g_table_init = true;
+}
+
+int sort_size_class(size_t n)
+{
+    if (n < ((size_t)1 << 20)) return 0;
+    if (n < ((size_t)1 << 23)) return 1;
+    return 2;
+}
+
+int sort_table_set(int cls, int nbits, SortCfg c)
+{
+    table_init();
+    if (cls < 0 || cls >= SORT_SIZE_CLASSES || nbits < 0 || nbits > 64 || !cfg_valid(c)) return -1;
+    g_table[cls][nbits] = c;
+    return 0;
+}
+
+SortCfg sort_table_get(int cls, int nbits)
+{
+    table_init();
+    if (cls < 0 || cls >= SORT_SIZE_CLASSES || nbits < 0 || nbits > 64) return k_cfgs[0];
+    return g_table[cls][nbits];
+}
+
+void sort_table_reset()
+{
+    g_table_init = false;
+    table_init();
+}
+
+int sort_force(int bits, int kpt, int lbw)
+{
+    g_force.bits = bits;
+    g_force.kpt = kpt;
+    g_force.lbw = lbw;
+    return 0;
+}
+
+SortCfg sort_forced() { return g_force; }
+
+/* the configuration a sort of n keys with nbits varying bits uses: the
+ * table's entry with any forced field replaced; a forced combination that has
+ * no kernel falls back to the nearest one with the forced digit width */
+SortCfg sort_choose(size_t n, int nbits)
+{
+    table_init();
+    SortCfg c = g_table[sort_size_class(n)][nbits < 0 ? 0 : (nbits > 64 ? 64 : nbits)];
+    if (g_force.bits) c.bits = g_force.bits;
+    if (g_force.kpt) c.kpt = g_force.kpt;
+    if (g_force.lbw) c.lbw = g_force.lbw;
+    if (cfg_valid(c)) return c;
+    for (int i = 0; i < N_CFGS; i++)
+        if (k_cfgs[i].bits == c.bits && k_cfgs[i].lbw == (c.lbw >= 32 ? 32 : 16)) return k_cfgs[i];
+    return k_cfgs[0];
+}
+
+/* ---- look-back status array ---- */
+
+static uint64_t *g_status = nullptr;
+static size_t g_status_cap = 0;   /* entries */
+static uint32_t g_epoch = 0;
+
+static bool status_reserve(hipStream_t st, size_t entries)
+{
+    if (entries <= g_status_cap && g_status) return true;
+    size_t cap = std::max(entries + entries / 4, (size_t)1 << 16);
+    if (g_status) {
+        snote(hipStreamSynchronize(st));
+        snote(hipFree(g_status));
+        g_status = nullptr;
+        g_status_cap = 0;
+    }
+    hipError_t e = hipMalloc((void **)&g_status, cap * 8);
+    if (e != hipSuccess) {
+        snote(e);
+        g_status = nullptr;
+        return false;
+    }
+    g_status_cap = cap;
+    snote(hipMemsetAsync(g_status, 0, cap * 8, st));
+    g_epoch = 0;
+    return true;
+}
+
+static uint32_t next_epoch(hipStream_t st)
+{
+    if (++g_epoch > OS_EPOCH_MAX) {
+        snote(hipMemsetAsync(g_status, 0, g_status_cap * 8, st));
+        g_epoch = 1;
+    }
+    return g_epoch;
+}
+
+void sort_release()
+{
+    if (g_status) hipFree(g_status);
+    g_status = nullptr;
+    g_status_cap = 0;
+    g_epoch = 0;
+}
+
+/* ---- launches ---- */
+
+struct OsScratch {
+    unsigned *ctr;        /* OS_MAXPASS + 1 tile counters */
+    uint32_t *ghist;      /* OS_MAXPASS x OS_MAXR */
+    uint64_t *gbase;      /* OS_MAXPASS x OS_MAXR */
+};
+
+constexpr size_t OS_CLEAR_BYTES = 256 + (size_t)OS_MAXPASS * OS_MAXR * 4;
+
+size_t sort_scratch_bytes(size_t) { return OS_CLEAR_BYTES + (size_t)OS_MAXPASS * OS_MAXR * 8 + 256; }
+
+static OsScratch carve(void *scratch)
+{
+    char *p = (char *)scratch;
+    OsScratch s;
+    s.ctr = (unsigned *)p;
+    s.ghist = (uint32_t *)(p + 256);
+    s.gbase = (uint64_t *)(p + OS_CLEAR_BYTES);
+    return s;
+}
+
+static void launch_hist_scan(hipStream_t st, int bits, const uint64_t *keys, size_t n, KeyXform x,
+                             int npass, const OsScratch &s)
+{
+    size_t hb = (n + 4095) / 4096;
+    if (hb > 2048) hb = 2048;
+    int h = prof_hook_begin(2);
+    switch (bits) {
+    case 9:
+        hipLaunchKernelGGL(k_os_hist<9>, dim3((unsigned)hb), dim3(256), 0, st, keys, n, x, npass,
+                           s.ghist);
+        break;
+    case 10:
+        hipLaunchKernelGGL(k_os_hist<10>, dim3((unsigned)hb), dim3(256), 0, st, keys, n, x, npass,
+                           s.ghist);
+        break;
+    default:
+        hipLaunchKernelGGL(k_os_hist<8>, dim3((unsigned)hb), dim3(256), 0, st, keys, n, x, npass,
+                           s.ghist);
+    }
+    snote(hipGetLastError());
+    prof_hook_end(h, 8.0 * n);
+    switch (bits) {
+    case 9:
+        hipLaunchKernelGGL(k_os_scan<9>, dim3(1), dim3(512), 0, st, s.ghist, npass, s.gbase);
+        break;
+    case 10:
+        hipLaunchKernelGGL(k_os_scan<10>, dim3(1), dim3(1024), 0, st, s.ghist, npass, s.gbase);
+        break;
+    default:
+        hipLaunchKernelGGL(k_os_scan<8>, dim3(1), dim3(256), 0, st, s.ghist, npass, s.gbase);
+    }
+    snote(hipGetLastError());
+}
+
+static void launch_pass(hipStream_t st, SortCfg c, unsigned tiles, const uint64_t *src,
+                        uint64_t *dst, size_t n, KeyXform x, int shift, const uint64_t *gbase,
+                        unsigned *ctr, uint32_t epoch, unsigned *err)
+{
+#define CG_PASS(B, K, W)                                                                          \
+    hipLaunchKernelGGL((k_os_pass<B, K, W>), dim3(tiles), dim3(1 << B), 0, st, src, dst, n, x,     \
+                       shift, gbase, g_status, ctr, epoch, err)
+    const bool wide = c.lbw >= 32;
+    if (c.bits == 10) {
+        if (wide) CG_PASS(10, 8, 32);
+        else CG_PASS(10, 8, 16);
+    } else if (c.bits == 9) {
+        if (c.kpt == 32) {
+            if (wide) CG_PASS(9, 32, 32);
+            else CG_PASS(9, 32, 16);
+        } else {
+            if (wide) CG_PASS(9, 16, 32);
+            else CG_PASS(9, 16, 16);
+        }
+    } else {
+        if (c.kpt == 16) {
+            if (wide) CG_PASS(8, 16, 32);
+            else CG_PASS(8, 16, 16);
+        } else {
+            if (wide) CG_PASS(8, 32, 32);
+            else CG_PASS(8, 32, 16);
+        }
+    }
+#undef CG_PASS
+    snote(hipGetLastError());
+}
+
+uint64_t *sort_unique(hipStream_t st, uint64_t *keys, uint64_t *tmp, size_t n, uint64_t vary,
+                      void *scratch, uint64_t *dst, unsigned long long *d_nuniq, unsigned *d_err,
+                      int *prof_unique)
+{
+    *prof_unique = -1;
+    if (n == 0) {
+        launch_fill_u64(st, (uint64_t *)d_nuniq, 1, 0);
+        return keys;
+    }
+    OsScratch s = carve(scratch);
+    KeyXform x = make_xform(n > 1 ? vary : 0);
+    const SortCfg c = sort_choose(n, x.nbits);
+    const int npass = (x.nbits + c.bits - 1) / c.bits;
+    const size_t tile_keys = ((size_t)1 << c.bits) * c.kpt;
+    const size_t tiles = (n + tile_keys - 1) / tile_keys;
+    const size_t utiles = (n + UQ_TILE - 1) / UQ_TILE;
+    if (!status_reserve(st, std::max(npass ? tiles << c.bits : 0, utiles))) return keys;
+    /* tile counters and histograms cleared together */
+    launch_fill_u64(st, (uint64_t *)scratch, OS_CLEAR_BYTES / 8, 0);
+    uint64_t *src = keys, *dst2 = tmp;
+    if (npass > 0) {
+        launch_hist_scan(st, c.bits, keys, n, x, npass, s);
+        for (int p = 0; p < npass; p++) {
+            int h = prof_hook_begin(3);
+            launch_pass(st, c, (unsigned)tiles, src, dst2, n, x, c.bits * p,
+                        s.gbase + ((size_t)p << c.bits), s.ctr + p, next_epoch(st), d_err);
+            prof_hook_end(h, 16.0 * n);
+            std::swap(src, dst2);
+        }
+    }
+    *prof_unique = prof_hook_begin(4);
+    hipLaunchKernelGGL(k_os_unique, dim3((unsigned)utiles), dim3(UQ_THREADS), 0, st, src, n, dst,
+                       g_status, s.ctr + OS_MAXPASS, next_epoch(st), d_nuniq, d_err);
+    snote(hipGetLastError());
+    prof_hook_end(*prof_unique, 8.0 * n);
+    return src;
+}
+
+} // namespace cg

@@ -138,13 +138,39 @@ int cg_dev_canon(uint64_t *d_keys, size_t n);
 /* synchronises the library's stream */
 int cg_dev_sync(void);
 
-/* ---------------- tuning knobs ---------------- */
+/* ---------------- tuning knobs and the autotuner ---------------- */
 
-/* "sort_kpt": keys per thread of the radix-sort and unique kernels (16 or
- * 32; tiles of 4096 or 8192 keys).  "sort_lbw": look-back window of the
- * radix passes (16, 32 or 64 tiles).  Returns 0, or -1 for an unknown knob. */
+/* Knobs (value 0 returns a forced field to the tuned table):
+ *   "sort_bits"  radix digit width of every sort: 8, 9 or 10 bits (a pass
+ *                block has one thread per digit);
+ *   "sort_kpt"   keys per thread of the passes: 32 or 16 (8 bits and 9 bits),
+ *                8 (10 bits); a tile is 2^bits * kpt keys;
+ *   "sort_lbw"   look-back window of the passes: 16 or 32 tiles;
+ *   "dir_delta"  prefix-bucket directory depth: bits = floor(log2 n) + delta
+ *                (-4..3), clamped to the layer's varying bits and 30;
+ *   "spawn_estimate"  0: spawn buffers sized for about twice the previous
+ *                layer, a step that overflows them is repeated with exact
+ *                buffers; 1: sized for half the previous layer, so that
+ *                growing layers take the repeat (for tests);
+ *   "reset"      default table (8-bit digits, 8192-key tiles, window 16),
+ *                no forced field, delta 0, spawn_estimate 0.
+ * Returns 0, or -1 for an unknown knob or value. */
 int cg_tune_set(const char *knob, int value);
 int cg_tune_get(const char *knob);
+/* the sorts read their configuration from a table indexed by size class
+ * (0: n < 2^20, 1: n < 2^23, 2: larger) and number of varying key bits
+ * (0..64); these set one entry and report the configuration a sort of n
+ * keys with nbits varying bits would use */
+int cg_tune_sort_entry(int size_class, int nbits, int bits, int kpt, int lbw);
+int cg_tune_sort_query(size_t n, int nbits, int *bits, int *kpt, int *lbw);
+/* Autotuner: times every sort configuration at every pass count on
+ * board-like keys for each size class and fills the table with the fastest
+ * (a configuration replaces the default only when at least 3% faster), then
+ * times directory depths on a lookup workload and sets "dir_delta".  effort
+ * 0 takes about a second, 1 uses larger samples.  report (may be NULL)
+ * receives the measurements and picks as JSON.  On failure the defaults are
+ * restored. */
+int cg_autotune(int effort, char *report, size_t cap);
 
 /* ---------------- kernel timers ---------------- */
 
@@ -153,7 +179,7 @@ int cg_tune_get(const char *knob);
  * algorithmic bytes of each launch. */
 void cg_prof_enable(int on);
 void cg_prof_reset(void);
-/* kernel: "move", "spawn", "sort_hist", "sort_scatter", "unique", "solve",
+/* kernel: "move", "spawn", "sort_hist", "sort_pass", "unique", "solve",
  * "adjoint", "lookup".  Returns the number of launches. */
 long cg_prof_query(const char *kernel, double *total_ms, double *total_bytes);
 

@@ -137,6 +137,82 @@ def test_sort_unique(cg, case):
     assert np.array_equal(host(out, n.value), np.unique(a))
 
 
+SORT_CONFIGS = [(8, 32, 16), (8, 32, 32), (8, 16, 16), (8, 16, 32), (9, 16, 16), (9, 16, 32),
+                (9, 32, 16), (9, 32, 32), (10, 8, 16), (10, 8, 32)]
+
+
+def _sort_check(cg, a):
+    d = dev(a) if len(a) else torch.empty(1, dtype=torch.int64, device="cuda")
+    out = torch.empty(max(len(a), 1), dtype=torch.int64, device="cuda")
+    n = C.c_size_t()
+    assert cg.lib().cg_dev_sort_unique(d.data_ptr(), len(a), out.data_ptr(), C.byref(n)) == 0
+    assert np.array_equal(host(out, n.value), np.unique(a))
+
+
+@pytest.mark.parametrize("cfg", SORT_CONFIGS, ids=lambda c: "%dx%d_w%d" % c)
+def test_sort_every_configuration(cg, cfg):
+    """Each (digit width, keys per thread, look-back window) kernel against
+    numpy's unique: tile-boundary sizes, key widths that leave a partial last
+    digit, 64-bit keys, heavy duplicates and board keys."""
+    bits, kpt, lbw = cfg
+    tile = (1 << bits) * kpt
+    rng = np.random.default_rng(bits * 1000 + kpt * 10 + lbw)
+    try:
+        cg.tune_set("sort_bits", bits)
+        cg.tune_set("sort_kpt", kpt)
+        cg.tune_set("sort_lbw", lbw)
+        assert cg.tune_sort_query(1 << 22, 40) == cfg
+        for n in (1, tile - 1, tile, tile + 1, 3 * tile + 17):
+            _sort_check(cg, rng.integers(0, 1 << 13, size=n, dtype=np.uint64) << np.uint64(7))
+        for nb in (bits + 1, 2 * bits - 1, 3 * bits, 45, 54):
+            _sort_check(cg, rng.integers(0, 1 << nb, size=200003, dtype=np.uint64))
+        full = rng.integers(0, 2**64 - 1, size=300000, dtype=np.uint64, endpoint=True)
+        full[::5] = full[3]
+        _sort_check(cg, full)
+        _sort_check(cg, rand_boards(rng, 0x00000000FFF0FFF0, 1 << 22))
+    finally:
+        cg.tune_set("reset", 0)
+
+
+@pytest.mark.parametrize("delta", [-3, -1, 1, 2])
+def test_lookup_directory_depths(cg, delta):
+    """Lookups through shallower and deeper prefix-bucket directories."""
+    rng = np.random.default_rng(100 + delta)
+    keys = np.unique(rand_boards(rng, 0x00000000FFF0FFF0, 200000))
+    vals = rng.random(len(keys))
+    q = np.concatenate([keys[rng.integers(0, len(keys), size=20000)],
+                        rand_boards(rng, 0x00000000FFF0FFF0, 5000)])
+    exp = np.where(np.isin(q, keys), vals[np.clip(np.searchsorted(keys, q), 0, len(keys) - 1)], 0.0)
+    try:
+        cg.tune_set("dir_delta", delta)
+        kd, vd, qd = dev(keys), torch.from_numpy(vals).cuda(), dev(q)
+        out = torch.empty(len(q), dtype=torch.float64, device="cuda")
+        assert cg.lib().cg_dev_lookup(kd.data_ptr(), vd.data_ptr(), len(keys), qd.data_ptr(), len(q),
+                                      out.data_ptr(), 0) == 0
+        assert np.array_equal(out.cpu().numpy(), exp)
+    finally:
+        cg.tune_set("reset", 0)
+
+
+def test_autotune(cg):
+    """The autotuner fills the table with kernels that exist and sorts stay exact."""
+    try:
+        rep = cg.autotune(0)
+        assert len(rep["sort"]) == 3 and all(len(c["pick"]) == 64 for c in rep["sort"])
+        assert rep["dir"]["pick"] in (-3, -2, -1, 0, 1, 2)
+        assert cg.tune_get("dir_delta") == rep["dir"]["pick"]
+        for cls, n in enumerate((1 << 18, 1 << 21, 1 << 24)):
+            for nb in (20, 40, 54):
+                b, k, w = cg.tune_sort_query(n, nb)
+                assert (b, k, w) in SORT_CONFIGS
+                assert rep["sort"][cls]["pick"][str(nb)] == "%dx%d/%d" % (b, k, w)
+        rng = np.random.default_rng(7)
+        _sort_check(cg, rand_boards(rng, 0x00000000FFF0FFF0, 1 << 22))
+        _sort_check(cg, rng.integers(0, 1 << 54, size=1 << 21, dtype=np.uint64))
+    finally:
+        cg.tune_set("reset", 0)
+
+
 @pytest.mark.parametrize("n", [0, 1, 50, 1023, 1024, 1025, 300000])
 def test_lookup_kernel(cg, n):
     rng = np.random.default_rng(n + 3)
@@ -198,6 +274,17 @@ def test_tablebase_vs_oracle(cg, name):
     tb.solve(win)
     check_tb(tb, layers, vals)
     tb.close()
+
+
+@pytest.mark.parametrize("name", ["c1", "dpdgap_prune", "3x4"])
+def test_tablebase_spawn_buffer_repeat(cg, name):
+    """Undersized spawn buffers: every growing step overflows them and is
+    repeated with exact buffers; layers and values stay identical."""
+    try:
+        cg.tune_set("spawn_estimate", 1)
+        test_tablebase_vs_oracle(cg, name)
+    finally:
+        cg.tune_set("reset", 0)
 
 
 def test_partial_solve_range(cg):
@@ -447,6 +534,33 @@ def test_batch_matches_single_tables(cg):
             for i in range(B.num_layers(t)):
                 assert np.array_equal(B.values(t, i).view(np.uint64), eager[t][i].view(np.uint64))
         B.solve(use_graph=False)
+    B.close()
+
+
+def test_batch_spawn_buffer_repeat(cg):
+    """Batched generation with undersized spawn buffers (every growing step is
+    repeated) gives the same layers as the single-table path."""
+    from cablegen_amd.batch_configs import batch
+    cfgs = batch(8, lo=3, hi=14)
+    r = cg.rules(ignore_f=True)
+    B = cg.Batch(r)
+    for c in cfgs:
+        B.add(c["initial"], c["start"], c["end"], c["winstates"])
+    try:
+        cg.tune_set("spawn_estimate", 1)
+        B.generate()
+        B.solve(use_graph=False)
+    finally:
+        cg.tune_set("reset", 0)
+    for t, c in enumerate(cfgs):
+        tb = cg.Tablebase(r)
+        tb.generate(c["initial"], c["start"], c["end"])
+        tb.solve(c["winstates"])
+        assert B.num_layers(t) == tb.num_layers
+        for i in range(tb.num_layers):
+            assert np.array_equal(B.keys(t, i), tb.keys(i)), (t, i)
+            assert np.array_equal(B.values(t, i).view(np.uint64), tb.values(i).view(np.uint64)), (t, i)
+        tb.close()
     B.close()
 
 
