@@ -98,7 +98,7 @@ def micro(args):
     out = {"metric": "kernel roofline microbench (HBM GB/s per kernel)", "n_boards": n,
            "queries": nq, "seed": 1, "formation_mask": "0x00000000FFF0FFF0",
            "rules": "ignore_f, no prune", "kernels": {}}
-    out["tuning"] = run_autotune(cg, args.autotune)
+    out["tuning"] = run_autotune(cg, args)
 
     def timed(name, fn):
         cg.prof_enable(True)
@@ -113,6 +113,13 @@ def micro(args):
         cg.prof_enable(False)
         out["kernels"][name] = res
 
+    # calibration for HBM counters: a plain device copy of 2^27 u64 (1 GiB read, 1 GiB written)
+    cx = torch.ones(1 << 27, dtype=torch.int64, device=dev)
+    cy = torch.empty_like(cx)
+    cy.copy_(cx)
+    torch.cuda.synchronize()
+    out["calibration_copy_bytes_each_way"] = 8 << 27
+    del cx, cy
     boards = torch.empty(n, dtype=torch.int64, device=dev)
     assert L.cg_dev_random_boards(boards.data_ptr(), n, 1, fmask, 0.4, 9, 0) == 0
     cand = torch.empty(4 * n, dtype=torch.int64, device=dev)
@@ -181,18 +188,30 @@ def micro(args):
     print(json.dumps(out))
 
 
-def run_autotune(cg, effort):
-    """Autotuner picks (untimed, before the warmup); effort < 0 keeps the defaults."""
-    if effort < 0:
-        cg.tune_set("reset", 0)
-        return {"autotune": "off (defaults: 8-bit digits, 8192-key tiles, window 16, dir delta 0)"}
+def run_autotune(cg, args):
+    """Sort / directory tuning before the warmup (never timed).  "stored"
+    installs the table the autotuner wrote on an MI355X
+    (cablegen_amd/tuned/mi355x.json; the tuner runs if it is missing), "auto"
+    runs the autotuner now, "off" keeps the defaults."""
+    mode = args.tuning
     t0 = time.perf_counter()
-    rep = cg.autotune(effort)
+    if mode == "off":
+        cg.tune_set("reset", 0)
+        return {"source": "defaults: 8-bit digits, 8192-key tiles, window 16, dir delta 0"}
+    if mode == "stored" and os.path.exists(cg.TUNED_TABLE):
+        rep = cg.load_tuning()
+        src = "stored autotuner table cablegen_amd/tuned/mi355x.json"
+    else:
+        rep = cg.autotune(1)
+        src = "autotuner run now (effort 1)"
+    if args.save_tuning:
+        with open(args.save_tuning, "w") as f:
+            json.dump(rep, f)
     sel = {}
     for cls, name in enumerate(("n<2^20", "n<2^23", "n>=2^23")):
         sel[name] = {nb: rep["sort"][cls]["pick"][nb] for nb in ("24", "32", "36", "40", "44", "48", "54")}
-    return {"autotune_s": time.perf_counter() - t0, "effort": effort,
-            "dir_delta": rep["dir"]["pick"], "sort_picks_bits_x_kpt/window": sel}
+    return {"source": src, "seconds": time.perf_counter() - t0, "dir_delta": rep["dir"]["pick"],
+            "sort_picks_bits_x_kpt/window": sel}
 
 
 def batch_bench(args):
@@ -202,7 +221,7 @@ def batch_bench(args):
     from cablegen_amd.batch_configs import batch
     cfgs = batch(args.batch_tables, args.batch_lo, args.batch_hi)
     r = cg.rules(ignore_f=True)
-    tuning = run_autotune(cg, args.autotune)
+    tuning = run_autotune(cg, args)
 
     tb = cg.Tablebase(r)   # one device-resident table reused: its arena is kept
 
@@ -266,8 +285,9 @@ def main():
     ap.add_argument("--batch-tables", type=int, default=64)
     ap.add_argument("--batch-lo", type=int, default=10, help="end = start + 2*U{lo..hi}")
     ap.add_argument("--batch-hi", type=int, default=40)
-    ap.add_argument("--autotune", type=int, default=0,
-                    help="autotuner effort before timing (0 quick, 1 full, -1 off)")
+    ap.add_argument("--tuning", choices=("stored", "auto", "off"), default="stored",
+                    help="sort/directory tuning: stored autotuner table, run the autotuner, or defaults")
+    ap.add_argument("--save-tuning", default=None, help="write the autotuner report (JSON) here")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -285,7 +305,7 @@ def main():
     r = cg.rules(**RULES)
     tb = cg.Tablebase(r)
     start = cg.lib().get_sum(INITIAL[0])
-    tuning = run_autotune(cg, args.autotune)
+    tuning = run_autotune(cg, args)
 
     def step():
         t0 = time.perf_counter()

@@ -258,6 +258,31 @@ def tune_sort_query(n, nbits):
     return b.value, k.value, w.value
 
 
+TUNED_TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned", "mi355x.json")
+
+
+def apply_tuning(rep):
+    """Installs the picks of an autotuner report (sort table, directory depth)."""
+    L = lib()
+    tune_set("reset", 0)
+    for cls, c in enumerate(rep["sort"]):
+        for nb, cfg in c["pick"].items():
+            bits, rest = cfg.split("x")
+            kpt, lbw = rest.split("/")
+            _check(L.cg_tune_sort_entry(cls, int(nb), int(bits), int(kpt), int(lbw)),
+                   "cg_tune_sort_entry")
+    tune_set("dir_delta", rep["dir"]["pick"])
+
+
+def load_tuning(path=TUNED_TABLE):
+    """Reads and installs an autotuner report saved as JSON; returns it."""
+    import json
+    with open(path) as f:
+        rep = json.load(f)
+    apply_tuning(rep)
+    return rep
+
+
 def autotune(effort=0):
     """Runs the autotuner (sort configuration per size class and key width,
     directory depth) on the GPU, installs its picks and returns its report."""

@@ -123,17 +123,23 @@ __device__ __forceinline__ uint64_t lookback(const uint64_t *status, size_t stri
     return excl;
 }
 
-/* digit counts of every pass in one read of the keys; NC LDS copies of the
- * counters spread the atomics (the four waves share them round-robin) */
+/* digit counts of every pass in one read of the keys.  Board keys are skewed
+ * (the top digits take few values), so lanes of one wave often hit the same
+ * counter: NC copies of the counters indexed by lane % NC, each starting in a
+ * different LDS bank, cap a wave's conflict on one counter at 64 / NC lanes. */
 template <int BITS>
 __global__ __launch_bounds__(256) void k_os_hist(const uint64_t *__restrict__ keys, size_t n,
-                                                 KeyXform x, int npass, uint32_t *ghist)
+                                                 KeyXform x, int npass, uint32_t *ghist,
+                                                 uint64_t vmask, uint64_t *cval)
 {
-    constexpr int R = 1 << BITS, NC = BITS == 8 ? 4 : (BITS == 9 ? 2 : 1);
-    __shared__ uint32_t h[NC][OS_MAXPASS][R];
-    for (int i = threadIdx.x; i < NC * OS_MAXPASS * R; i += 256) (&h[0][0][0])[i] = 0;
+    constexpr int R = 1 << BITS, NC = BITS == 8 ? 8 : (BITS == 9 ? 4 : 2);
+    constexpr int STRIDE = OS_MAXPASS * R + 1;   /* 64 KiB of counters */
+    __shared__ uint32_t h[NC * STRIDE];
+    /* the bits outside the packed ones are the same in every key */
+    if (blockIdx.x == 0 && threadIdx.x == 0) *cval = keys[0] & ~vmask;
+    for (int i = threadIdx.x; i < NC * STRIDE; i += 256) h[i] = 0;
     __syncthreads();
-    const int c = (threadIdx.x >> 6) % NC;
+    uint32_t *hc = h + (lane_id() % NC) * STRIDE;
     constexpr int HB = 8;   /* keys loaded per thread before counting */
     for (size_t b0 = (size_t)blockIdx.x * 256 * HB; b0 < n; b0 += (size_t)gridDim.x * 256 * HB) {
         uint64_t kk[HB];
@@ -147,15 +153,14 @@ __global__ __launch_bounds__(256) void k_os_hist(const uint64_t *__restrict__ ke
             size_t i = b0 + (size_t)j * 256 + threadIdx.x;
             if (i >= n) continue;
             uint64_t v = xform(kk[j], x);
-            for (int p = 0; p < npass; p++) atomicAdd(&h[c][p][(v >> (BITS * p)) & (R - 1)], 1u);
+            for (int p = 0; p < npass; p++) atomicAdd(&hc[p * R + ((v >> (BITS * p)) & (R - 1))], 1u);
         }
     }
     __syncthreads();
     for (int i = threadIdx.x; i < npass * R; i += 256) {
-        int p = i / R, d = i % R;
         uint32_t t = 0;
 #pragma unroll
-        for (int k = 0; k < NC; k++) t += h[k][p][d];
+        for (int k = 0; k < NC; k++) t += h[k * STRIDE + i];
         if (t) atomicAdd(&ghist[i], t);
     }
 }
@@ -175,11 +180,14 @@ __global__ __launch_bounds__(1 << BITS) void k_os_scan(const uint32_t *ghist, in
 }
 
 /* One LSD pass: a block of R = 2^BITS threads (thread d owns digit d) sorts
- * a tile of R * KPT keys by digit (xform(key) >> shift) & (R - 1), stably. */
+ * a tile of R * KPT keys by digit (k >> shift) & (R - 1), stably, where k is
+ * the packed key.  The first pass (RAW) reads the original keys and packs
+ * them (xform) once; every pass writes packed keys, so later passes extract
+ * a digit with a shift and a mask. */
 /* at least 2 waves per SIMD: the registers of a KPT = 32 block then match the
  * two blocks per CU its 70 KiB of LDS allows (asking for 4 with KPT = 16
  * spills) */
-template <int BITS, int KPT, int LB_W>
+template <int BITS, int KPT, int LB_W, bool RAW>
 __global__ __launch_bounds__(1 << BITS, 2) void k_os_pass(const uint64_t *__restrict__ in,
                                                        uint64_t *__restrict__ out, size_t n,
                                                        KeyXform x, int shift,
@@ -209,17 +217,24 @@ __global__ __launch_bounds__(1 << BITS, 2) void k_os_pass(const uint64_t *__rest
     const uint64_t lt = lanemask_lt();
 #pragma unroll
     for (int j = 0; j < KPT; j++) key[j] = j * 64 + lane < rem ? in[base + j * 64 + lane] : 0;
+    if (RAW) {
+#pragma unroll
+        for (int j = 0; j < KPT; j++) key[j] = xform(key[j], x);
+    }
 #pragma unroll
     for (int j = 0; j < KPT; j++) {
         bool valid = j * 64 + lane < rem;
-        unsigned d = (unsigned)(xform(key[j], x) >> shift) & (R - 1);
-        uint64_t peers = __ballot(valid);
+        unsigned d = (unsigned)(key[j] >> shift) & (R - 1);
+        /* lanes whose digit differs from this lane's in some bit: OR over the
+         * bits of (ballot of the bit) XOR (the lane's own bit as 0 / all ones) */
+        uint64_t mism = 0;
 #pragma unroll
         for (int bit = 0; bit < BITS; bit++) {
-            bool s = (d >> bit) & 1u;
-            uint64_t bb = __ballot(s);
-            peers &= s ? bb : ~bb;
+            uint32_t t = 0u - ((d >> bit) & 1u);
+            uint64_t bb = __ballot(t != 0);
+            mism |= bb ^ (((uint64_t)t << 32) | t);
         }
+        uint64_t peers = __ballot(valid) & ~mism;
         uint32_t before = (uint32_t)__popcll(peers & lt);
         uint32_t cnt = wcnt[w * R + d];
         __builtin_amdgcn_wave_barrier();
@@ -259,7 +274,7 @@ __global__ __launch_bounds__(1 << BITS, 2) void k_os_pass(const uint64_t *__rest
             /* recompute the digit rather than keep KPT digits live across the scan */
             uint64_t kk = key[j];
             asm volatile("" : "+v"(kk));
-            unsigned dd = (unsigned)(xform(kk, x) >> shift) & (R - 1);
+            unsigned dd = (unsigned)(kk >> shift) & (R - 1);
             uint32_t r = (rank2[j >> 1] >> ((j & 1) * 16)) & 0xffffu;
             stage[tstart[dd] + wcnt[w * R + dd] + r] = kk;
         }
@@ -269,7 +284,7 @@ __global__ __launch_bounds__(1 << BITS, 2) void k_os_pass(const uint64_t *__rest
     uint32_t tn = (uint32_t)((n - tile0) < (size_t)TILE ? (n - tile0) : (size_t)TILE);
     for (uint32_t p = threadIdx.x; p < tn; p += R) {
         uint64_t kk = stage[p];
-        unsigned dd = (unsigned)(xform(kk, x) >> shift) & (R - 1);
+        unsigned dd = (unsigned)(kk >> shift) & (R - 1);
         out[obase[dd] + (p - tstart[dd])] = kk;
     }
 }
@@ -280,7 +295,9 @@ __global__ __launch_bounds__(UQ_THREADS) void k_os_unique(const uint64_t *__rest
                                                           uint64_t *__restrict__ dst,
                                                           uint64_t *status, unsigned *tile_ctr,
                                                           uint32_t epoch,
-                                                          unsigned long long *total, unsigned *err)
+                                                          unsigned long long *total, unsigned *err,
+                                                          int packed, KeyXform x,
+                                                          const uint64_t *cval)
 {
     constexpr int WCHUNK = 64 * UQ_KPT;
     __shared__ unsigned tile_sh;
@@ -365,9 +382,12 @@ __global__ __launch_bounds__(UQ_THREADS) void k_os_unique(const uint64_t *__rest
     __syncthreads();
     uint64_t o = excl_sh;
     for (int ww = 0; ww < w; ww++) o += wc[ww];
+    /* packed keys go back to the original bits: xform_inv plus the constant bits */
+    const uint64_t cv = packed ? *cval : 0;
 #pragma unroll
     for (int j = 0; j < UQ_KPT; j++) {
-        if ((m[j] >> lane) & 1ULL) dst[o + __popcll(m[j] & lt)] = kv[j];
+        if ((m[j] >> lane) & 1ULL)
+            dst[o + __popcll(m[j] & lt)] = packed ? (xform_inv(kv[j], x) | cv) : kv[j];
         o += __popcll(m[j]);
     }
 }
@@ -548,6 +568,7 @@ void sort_release()
 
 struct OsScratch {
     unsigned *ctr;        /* OS_MAXPASS + 1 tile counters */
+    uint64_t *cval;       /* the keys' constant bits */
     uint32_t *ghist;      /* OS_MAXPASS x OS_MAXR */
     uint64_t *gbase;      /* OS_MAXPASS x OS_MAXR */
 };
@@ -561,6 +582,7 @@ static OsScratch carve(void *scratch)
     char *p = (char *)scratch;
     OsScratch s;
     s.ctr = (unsigned *)p;
+    s.cval = (uint64_t *)(p + 128);
     s.ghist = (uint32_t *)(p + 256);
     s.gbase = (uint64_t *)(p + OS_CLEAR_BYTES);
     return s;
@@ -569,21 +591,22 @@ static OsScratch carve(void *scratch)
 static void launch_hist_scan(hipStream_t st, int bits, const uint64_t *keys, size_t n, KeyXform x,
                              int npass, const OsScratch &s)
 {
+    const uint64_t vmask = xform_inv(~0ULL, x);
     size_t hb = (n + 4095) / 4096;
     if (hb > 2048) hb = 2048;
     int h = prof_hook_begin(2);
     switch (bits) {
     case 9:
         hipLaunchKernelGGL(k_os_hist<9>, dim3((unsigned)hb), dim3(256), 0, st, keys, n, x, npass,
-                           s.ghist);
+                           s.ghist, vmask, s.cval);
         break;
     case 10:
         hipLaunchKernelGGL(k_os_hist<10>, dim3((unsigned)hb), dim3(256), 0, st, keys, n, x, npass,
-                           s.ghist);
+                           s.ghist, vmask, s.cval);
         break;
     default:
         hipLaunchKernelGGL(k_os_hist<8>, dim3((unsigned)hb), dim3(256), 0, st, keys, n, x, npass,
-                           s.ghist);
+                           s.ghist, vmask, s.cval);
     }
     snote(hipGetLastError());
     prof_hook_end(h, 8.0 * n);
@@ -600,12 +623,13 @@ static void launch_hist_scan(hipStream_t st, int bits, const uint64_t *keys, siz
     snote(hipGetLastError());
 }
 
-static void launch_pass(hipStream_t st, SortCfg c, unsigned tiles, const uint64_t *src,
-                        uint64_t *dst, size_t n, KeyXform x, int shift, const uint64_t *gbase,
-                        unsigned *ctr, uint32_t epoch, unsigned *err)
+template <bool RAW>
+static void launch_pass_t(hipStream_t st, SortCfg c, unsigned tiles, const uint64_t *src,
+                          uint64_t *dst, size_t n, KeyXform x, int shift, const uint64_t *gbase,
+                          unsigned *ctr, uint32_t epoch, unsigned *err)
 {
 #define CG_PASS(B, K, W)                                                                          \
-    hipLaunchKernelGGL((k_os_pass<B, K, W>), dim3(tiles), dim3(1 << B), 0, st, src, dst, n, x,     \
+    hipLaunchKernelGGL((k_os_pass<B, K, W, RAW>), dim3(tiles), dim3(1 << B), 0, st, src, dst, n, x, \
                        shift, gbase, g_status, ctr, epoch, err)
     const bool wide = c.lbw >= 32;
     if (c.bits == 10) {
@@ -630,6 +654,14 @@ static void launch_pass(hipStream_t st, SortCfg c, unsigned tiles, const uint64_
     }
 #undef CG_PASS
     snote(hipGetLastError());
+}
+
+static void launch_pass(hipStream_t st, SortCfg c, unsigned tiles, const uint64_t *src,
+                        uint64_t *dst, size_t n, KeyXform x, int shift, const uint64_t *gbase,
+                        unsigned *ctr, uint32_t epoch, unsigned *err, bool raw)
+{
+    if (raw) launch_pass_t<true>(st, c, tiles, src, dst, n, x, shift, gbase, ctr, epoch, err);
+    else launch_pass_t<false>(st, c, tiles, src, dst, n, x, shift, gbase, ctr, epoch, err);
 }
 
 uint64_t *sort_unique(hipStream_t st, uint64_t *keys, uint64_t *tmp, size_t n, uint64_t vary,
@@ -657,14 +689,15 @@ uint64_t *sort_unique(hipStream_t st, uint64_t *keys, uint64_t *tmp, size_t n, u
         for (int p = 0; p < npass; p++) {
             int h = prof_hook_begin(3);
             launch_pass(st, c, (unsigned)tiles, src, dst2, n, x, c.bits * p,
-                        s.gbase + ((size_t)p << c.bits), s.ctr + p, next_epoch(st), d_err);
+                        s.gbase + ((size_t)p << c.bits), s.ctr + p, next_epoch(st), d_err, p == 0);
             prof_hook_end(h, 16.0 * n);
             std::swap(src, dst2);
         }
     }
     *prof_unique = prof_hook_begin(4);
     hipLaunchKernelGGL(k_os_unique, dim3((unsigned)utiles), dim3(UQ_THREADS), 0, st, src, n, dst,
-                       g_status, s.ctr + OS_MAXPASS, next_epoch(st), d_nuniq, d_err);
+                       g_status, s.ctr + OS_MAXPASS, next_epoch(st), d_nuniq, d_err, npass > 0 ? 1 : 0,
+                       x, s.cval);
     snote(hipGetLastError());
     prof_hook_end(*prof_unique, 8.0 * n);
     return src;
