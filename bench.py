@@ -188,6 +188,92 @@ def micro(args):
     print(json.dumps(out))
 
 
+PMC_KERNELS = {"move": "k_move(", "spawn": "k_spawn(", "sort_hist": "k_os_hist", "sort_pass": "k_os_pass",
+               "unique": "k_os_unique", "solve": "k_solve(", "lookup": "k_lookup(", "adjoint": "k_adjoint("}
+
+
+def _csvs(d, suffix):
+    import glob
+    return sorted(glob.glob(os.path.join(d, "**", "*" + suffix), recursive=True))
+
+
+def prof_summary(args):
+    """Summaries of rocprofv3 output (printed as markdown):
+    --prof-summary DIR            kernel table of DIR's *_kernel_stats.csv;
+    --prof-summary DIR --pmc      per-kernel HBM traffic and L2 hit rate from the
+                                  FETCH_SIZE / WRITE_SIZE / TCC_HIT_sum+TCC_MISS_sum
+                                  runs in DIR/*FETCH_SIZE*, *WRITE_SIZE*, *TCC_HIT*;
+                                  the calibration copy (__amd_rocclr_copyBuffer of
+                                  1 GiB) is taken from --pmc-calib DIR (default DIR);
+                                  --pmc-json FILE also writes per-launch averages."""
+    import csv
+    import collections
+    d = args.prof_summary
+    if not args.pmc:
+        for f in _csvs(d, "_kernel_stats.csv"):
+            rows = list(csv.DictReader(open(f)))
+            print("| kernel | calls | total ms | avg us | % | min us | max us |")
+            print("|---|---:|---:|---:|---:|---:|---:|")
+            for r in rows[:args.prof_rows]:
+                print("| %s | %s | %.1f | %.2f | %.2f | %.2f | %.2f |" % (
+                    r["Name"][:70].replace("|", "/"), r["Calls"], float(r["TotalDurationNs"]) / 1e6,
+                    float(r["AverageNs"]) / 1e3, float(r["Percentage"]), float(r["MinNs"]) / 1e3,
+                    float(r["MaxNs"]) / 1e3))
+        return
+
+    def load(sub, name):
+        out = {}
+        for f in _csvs(os.path.join(d), "_counter_collection.csv"):
+            if sub not in f or (args.pmc_tag and args.pmc_tag not in f):
+                continue
+            for r in csv.DictReader(open(f)):
+                if r["Counter_Name"] == name:
+                    out[int(r["Dispatch_Id"])] = (r["Kernel_Name"], float(r["Counter_Value"]))
+        return out
+    F, W = load("FETCH_SIZE", "FETCH_SIZE"), load("WRITE_SIZE", "WRITE_SIZE")
+    H, M = load("TCC_HIT", "TCC_HIT_sum"), load("TCC_HIT", "TCC_MISS_sum")
+    calib_dir = args.pmc_calib or d
+    cF = max(v for k, v in load_calib(calib_dir, "FETCH_SIZE"))
+    cW = max(v for k, v in load_calib(calib_dir, "WRITE_SIZE"))
+    gib_kib = float(1 << 20)
+    kf, kw = gib_kib / cF, gib_kib / cW
+    print("calibration: a 1 GiB device copy reads FETCH_SIZE %.0f KiB and WRITE_SIZE %.0f KiB; "
+          "factors %.3f (fetch) and %.3f (write)\n" % (cF, cW, kf, kw))
+    print("| kernel | launches | HBM read / launch (MB) | HBM write / launch (MB) | L2 hit rate |")
+    print("|---|---:|---:|---:|---:|")
+    js = {"calibration": {"fetch_factor": kf, "write_factor": kw}, "kernels": {}}
+    for name, pat in PMC_KERNELS.items():
+        ids = [i for i, v in F.items() if pat in v[0]]
+        if not ids:
+            continue
+        fb = sum(F[i][1] for i in ids) * 1024 * kf / len(ids)
+        wl = [i for i, v in W.items() if pat in v[0]]
+        wb = sum(W[i][1] for i in wl) * 1024 * kw / max(len(wl), 1)
+        hl = [i for i, v in H.items() if pat in v[0]]
+        h = sum(H[i][1] for i in hl)
+        m = sum(M[i][1] for i in hl if i in M)
+        hit = h / (h + m) if h + m else float("nan")
+        print("| %s | %d | %.3f | %.3f | %.3f |" % (name, len(ids), fb / 1e6, wb / 1e6, hit))
+        js["kernels"][name] = {"launches": len(ids), "read_bytes_per_launch": fb,
+                               "write_bytes_per_launch": wb, "l2_hit_rate": hit}
+    if args.pmc_json:
+        js["workload"] = args.pmc_workload
+        with open(args.pmc_json, "w") as f:
+            json.dump(js, f, indent=1)
+
+
+def load_calib(d, name):
+    import csv
+    out = []
+    for f in _csvs(d, "_counter_collection.csv"):
+        if name not in f:
+            continue
+        for r in csv.DictReader(open(f)):
+            if r["Counter_Name"] == name and "copyBuffer" in r["Kernel_Name"]:
+                out.append((r["Kernel_Name"], float(r["Counter_Value"])))
+    return out
+
+
 def run_autotune(cg, args):
     """Sort / directory tuning before the warmup (never timed).  "stored"
     installs the table the autotuner wrote on an MI355X
@@ -283,8 +369,15 @@ def main():
     ap.add_argument("--micro-log2", type=int, default=30, help="log2 of the microbench board count")
     ap.add_argument("--batch", action="store_true", help="batched-tables benchmark instead")
     ap.add_argument("--batch-tables", type=int, default=64)
-    ap.add_argument("--batch-lo", type=int, default=10, help="end = start + 2*U{lo..hi}")
-    ap.add_argument("--batch-hi", type=int, default=40)
+    ap.add_argument("--batch-lo", type=int, default=5, help="end = start + 2*U{lo..hi}")
+    ap.add_argument("--batch-hi", type=int, default=20)
+    ap.add_argument("--prof-summary", default=None, help="summarise rocprofv3 output in this directory")
+    ap.add_argument("--prof-rows", type=int, default=14)
+    ap.add_argument("--pmc", action="store_true", help="with --prof-summary: PMC traffic / L2 summary")
+    ap.add_argument("--pmc-calib", default=None, help="directory holding the calibration-copy PMC runs")
+    ap.add_argument("--pmc-tag", default=None, help="only PMC runs whose path contains this")
+    ap.add_argument("--pmc-json", default=None, help="write per-launch traffic here (JSON)")
+    ap.add_argument("--pmc-workload", default="", help="workload label stored in --pmc-json")
     ap.add_argument("--tuning", choices=("stored", "auto", "off"), default="stored",
                     help="sort/directory tuning: stored autotuner table, run the autotuner, or defaults")
     ap.add_argument("--save-tuning", default=None, help="write the autotuner report (JSON) here")
@@ -294,6 +387,9 @@ def main():
               "(the multi-GPU sharding is described in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
 
+    if args.prof_summary:
+        prof_summary(args)
+        return
     if args.micro:
         micro(args)
         return
@@ -340,6 +436,14 @@ def main():
     avg_ms = d["ms"] / max(d["launches"], 1)
     avg_bytes = d["bytes"] / max(d["launches"], 1)
     achieved = avg_bytes / (avg_ms * 1e6) if avg_ms else 0.0
+    traffic = None
+    tf = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "pmc_traffic.json")
+    if os.path.exists(tf):
+        with open(tf) as f:
+            tj = json.load(f)
+        k = tj.get("kernels", {}).get(dom)
+        if k and tj.get("workload") == "c2":
+            traffic = k["read_bytes_per_launch"] + k["write_bytes_per_launch"]
 
     out = {
         "metric": METRIC, "value": value, "unit": "boards/s", "n_gpus": 1,
@@ -352,7 +456,9 @@ def main():
                    "layers": tb.num_layers, "boards": total,
                    "gen_ms": 1e3 * sum(gen_t) / args.steps, "solve_ms": 1e3 * sum(sol_t) / args.steps},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": "profiles/pmc_traffic.json (rocprofv3 FETCH_SIZE / WRITE_SIZE, "
+                                       "calibrated by a 1 GiB copy)" if traffic else None,
                      "avg_launch_us": avg_ms * 1e3, "avg_algorithmic_bytes": avg_bytes},
         "tuning": tuning,
         "kernels": {k: {"launches": v["launches"] // max(args.steps, 1),

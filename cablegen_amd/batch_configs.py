@@ -4,10 +4,12 @@ table_config(seed) draws one table (numpy Generator seeded with `seed`):
   * formation uniform over {2x4, 3x4 (bottom row 0xF), DPDGap};
   * 2-4 tiles with exponents uniform in 1..7 in distinct random free cells;
   * goal 2^(max exponent + 2) in the top-left cell (free in all three);
-  * end tile sum = initial sum + 2 * U{lo..hi}; the default U{10..40}
-    (offset 20..80) keeps the tables small -- without pruning these
-    formations grow fast (offset 40 alone gives 4e8 boards over 64 tables),
-    so BASELINE's offset of 100..400 would make each table a full-GPU job;
+  * end tile sum = initial sum + 2 * U{lo..hi}.  Without pruning these
+    formations grow about 2-4x per 10 of tile sum (3x4 fastest), so an offset
+    of 100..400 makes every table a full-GPU job of 1e8-1e10 boards.  The
+    default U{5..20} (offset 10..40) gives 64 tables of 5e3..3.6e7 boards
+    (1.6e8 in all, median 1.5e5); U{10..40} gives 4e3..4.9e8 (1.7e9 in all,
+    five tables hold two thirds of the boards);
   * rules: formation cells as walls (ignore_f), no pruning.
 """
 import numpy as np
@@ -28,7 +30,7 @@ def tile_sum(b):
     return s
 
 
-def table_config(seed, lo=10, hi=40):
+def table_config(seed, lo=5, hi=20):
     rng = np.random.default_rng(seed)
     name = ["2x4", "3x4", "DPDGap"][int(rng.integers(0, 3))]
     fmask = FORMATIONS[name]
@@ -46,5 +48,5 @@ def table_config(seed, lo=10, hi=40):
                 rules=dict(ignore_f=True))
 
 
-def batch(n=64, lo=10, hi=40):
+def batch(n=64, lo=5, hi=20):
     return [table_config(s, lo, hi) for s in range(n)]

@@ -85,6 +85,14 @@ struct Engine {
     Buf A, B, Cb;                                 /* candidate / sort buffers */
     void *scratch = nullptr;
     size_t scratch_cap = 0;
+    /* a second stream for the carry sort, which runs beside the P_{s+2}
+     * sort: its own ping-pong buffer and scratch; ev2 orders the main
+     * stream after it */
+    hipStream_t st2 = nullptr;
+    hipEvent_t ev2 = nullptr;
+    Buf B2;
+    void *scratch2 = nullptr;
+    size_t scratch2_cap = 0;
     uint64_t *d_wmask = nullptr, *d_wval = nullptr;
     size_t wcap = 0;
 };
@@ -94,6 +102,8 @@ static int eng_init()
 {
     if (E.init) return 0;
     HC(hipStreamCreate(&E.st));
+    HC(hipStreamCreate(&E.st2));
+    HC(hipEventCreateWithFlags(&E.ev2, hipEventDisableTiming));
     HC(hipMalloc((void **)&E.d_cnt, 8 * sizeof(unsigned long long)));
     E.h_cnt = (unsigned long long *)calloc(8, sizeof(unsigned long long));
     if (!E.h_cnt) return fail_msg("host allocation failed");
@@ -107,11 +117,19 @@ static int eng_init()
     return 0;
 }
 
+/* waits for both streams (work enqueued on them may still use a buffer
+ * about to be freed) */
+static int sync_streams()
+{
+    if (E.st) HC(hipStreamSynchronize(E.st));
+    if (E.st2) HC(hipStreamSynchronize(E.st2));
+    return 0;
+}
+
 static int ensure(Buf &b, size_t n)
 {
     if (n <= b.cap && b.p) return 0;
-    /* work enqueued on the stream may still use the buffer */
-    if (b.p && E.st) HC(hipStreamSynchronize(E.st));
+    if (b.p && sync_streams()) return -1;
     if (b.p) HC(hipFree(b.p));
     b.p = nullptr;
     size_t c = std::max<size_t>(n + n / 4, 1024);
@@ -123,8 +141,8 @@ static int ensure(Buf &b, size_t n)
 static int ensure_scratch_bytes(size_t need)
 {
     if (need <= E.scratch_cap && E.scratch) return 0;
-    /* sorts or index builds enqueued on the stream may still use it */
-    if (E.scratch) HC(hipStreamSynchronize(E.st));
+    /* sorts or index builds enqueued on the streams may still use it */
+    if (E.scratch && sync_streams()) return -1;
     if (E.scratch) HC(hipFree(E.scratch));
     E.scratch = nullptr;
     size_t c = need + need / 4 + 4096;
@@ -137,12 +155,25 @@ static int ensure_scratch(size_t n)
 {
     size_t need = sort_scratch_bytes(n);
     if (need <= E.scratch_cap && E.scratch) return 0;
-    if (E.scratch) HC(hipStreamSynchronize(E.st));
+    if (E.scratch && sync_streams()) return -1;
     if (E.scratch) HC(hipFree(E.scratch));
     E.scratch = nullptr;
     size_t c = need + need / 4 + 4096;
     HC(hipMalloc(&E.scratch, c));
     E.scratch_cap = c;
+    return 0;
+}
+
+static int ensure_scratch2(size_t n)
+{
+    size_t need = sort_scratch_bytes(n);
+    if (need <= E.scratch2_cap && E.scratch2) return 0;
+    if (E.scratch2 && sync_streams()) return -1;
+    if (E.scratch2) HC(hipFree(E.scratch2));
+    E.scratch2 = nullptr;
+    size_t c = need + need / 4 + 4096;
+    HC(hipMalloc(&E.scratch2, c));
+    E.scratch2_cap = c;
     return 0;
 }
 
@@ -171,6 +202,7 @@ static const char *PK_NAMES[PK_N] = { "move", "spawn", "sort_hist", "sort_pass",
 
 struct ProfRec {
     int kind;
+    hipStream_t st;
     hipEvent_t a, b;
     double bytes;
     int stats_slot;      /* solve launches: lookup counters read at collection */
@@ -194,18 +226,19 @@ static hipEvent_t ev_get()
 static double g_ms[PK_N], g_bytes[PK_N];
 static long g_launches[PK_N];
 
-static int prof_begin(int kind)
+static int prof_begin(int kind, hipStream_t st = nullptr)
 {
     if (!g_prof) return -1;
     ProfRec r;
     r.kind = kind;
+    r.st = st ? st : E.st;
     r.bytes = 0;
     r.stats_slot = -1;
     r.base_bytes = 0;
     r.a = ev_get();
     r.b = ev_get();
     if (!r.a || !r.b) return -1;
-    hipEventRecord(r.a, E.st);
+    hipEventRecord(r.a, r.st);
     g_recs.push_back(r);
     return (int)g_recs.size() - 1;
 }
@@ -213,7 +246,7 @@ static int prof_begin(int kind)
 static void prof_end(int h, double bytes)
 {
     if (h < 0) return;
-    hipEventRecord(g_recs[h].b, E.st);
+    hipEventRecord(g_recs[h].b, g_recs[h].st);
     g_recs[h].bytes = bytes;
 }
 
@@ -230,6 +263,7 @@ static void prof_collect()
 {
     if (g_recs.empty()) return;
     hipStreamSynchronize(E.st);
+    if (E.st2) hipStreamSynchronize(E.st2);
     std::vector<unsigned long long> st;
     if (g_stats_used) {
         st.resize(2 * g_stats_used);
@@ -255,8 +289,8 @@ static void prof_collect()
 /* sort hooks: the radix passes run inside kernels.hip; these brackets let
  * the timers see each pass */
 namespace cg {
-int prof_hook_begin(int kind) { return prof_begin(kind); }
-void prof_hook_end(int h, double bytes) { prof_end(h, bytes); }
+int prof_hook_begin(int kind, hipStream_t st) { return prof_begin(kind, st); }
+void prof_hook_end(int h, double bytes, hipStream_t) { prof_end(h, bytes); }
 }
 
 extern "C" void cg_prof_enable(int on)
@@ -377,13 +411,29 @@ static int read_counters(int lo, int n)
 static uint64_t vary_of(size_t n, uint64_t o, uint64_t a) { return n > 1 ? (o & ~a) : 0; }
 
 /* enqueues sort + unique of buf[0..n) into dst; count lands in d_cnt[slot] */
-static int sort_enqueue(uint64_t *buf, size_t n, uint64_t vary, uint64_t *dst, int slot, int *hu)
+/* lane 0: the main stream; lane 1: the second stream (its own ping-pong
+ * buffer, scratch and sort context) -- the caller orders the main stream
+ * after it (join_lane1) before using its output */
+static int sort_enqueue(uint64_t *buf, size_t n, uint64_t vary, uint64_t *dst, int slot, int *hu,
+                        int lane = 0)
 {
-    if (ensure(E.B, n + 1)) return -1;
-    if (ensure_scratch(n)) return -1;
-    sort_unique(E.st, buf, E.B.p, n, vary, E.scratch, dst, E.d_cnt + slot, E.d_err, hu);
+    if (lane == 1) {
+        if (ensure(E.B2, n + 1) || ensure_scratch2(n)) return -1;
+        sort_unique(E.st2, buf, E.B2.p, n, vary, E.scratch2, dst, E.d_cnt + slot, E.d_err, hu, 1);
+    } else {
+        if (ensure(E.B, n + 1) || ensure_scratch(n)) return -1;
+        sort_unique(E.st, buf, E.B.p, n, vary, E.scratch, dst, E.d_cnt + slot, E.d_err, hu, 0);
+    }
     hipError_t e = kernels_take_error();
     if (e != hipSuccess) return fail("sort_unique", e);
+    return 0;
+}
+
+/* later work on the main stream waits for everything enqueued on the second */
+static int join_lane1()
+{
+    HC(hipEventRecord(E.ev2, E.st2));
+    HC(hipStreamWaitEvent(E.st, E.ev2, 0));
     return 0;
 }
 
@@ -510,9 +560,12 @@ static int gen_step(const cg_rules &r, const uint64_t *P, size_t nP, const uint6
     size_t n2 = nC + sumE;
     if (ensure(nxtP, n2 + 1) || ensure(nxtC, sumE + 1)) return -1;
     int hu2, hu4;
+    /* the two sorts are independent: C' runs on the second stream beside
+     * P_{s+2} (the spawns finished at the synchronisation above) */
+    if (sort_enqueue(E.Cb.p, sumE, vary_of(sumE, or4, and4), nxtC.p, 6, &hu4, 1)) return -1;
     if (sort_enqueue(E.A.p, n2, vary_of(n2, or2 | carry.o, and2 & carry.a), nxtP.p, 5, &hu2))
         return -1;
-    if (sort_enqueue(E.Cb.p, sumE, vary_of(sumE, or4, and4), nxtC.p, 6, &hu4)) return -1;
+    if (join_lane1()) return -1;
     pc.pending = true;
     pc.n2 = n2;
     pc.sumE = sumE;
@@ -1199,11 +1252,18 @@ extern "C" int cg_autotune(int effort, char *report, size_t cap)
             rc = fail_msg("autotune: allocation failed");
         const uint64_t dpd = 0x00000000FFF0FFF0ULL;
         if (!rc) {
+            /* canonical forms move the formation cells: the varying bits come
+             * from the keys themselves */
             launch_random_boards(E.st, b.src, nl, 0x5eed1000ULL, dpd, 0.4, 11, 0);
             launch_canon(E.st, b.src, nl);
+            uint64_t ko = 0, ka = ~0ULL;
+            rc = key_or_and(b.src, nl, &ko, &ka);
             int hu;
-            sort_unique(E.st, b.src, b.tmp, nl, ~dpd, E.scratch, b.dst, E.d_cnt + 7, E.d_err, &hu);
-            rc = read_counters(7, 1);
+            if (!rc) {
+                sort_unique(E.st, b.src, b.tmp, nl, vary_of(nl, ko, ka), E.scratch, b.dst, E.d_cnt + 7,
+                            E.d_err, &hu);
+                rc = read_counters(7, 1);
+            }
         }
         size_t nu = rc ? 0 : E.h_cnt[7];
         uint64_t o = 0, a = ~0ULL;
@@ -1653,9 +1713,10 @@ extern "C" int cg_batch_generate(cg_batch *b)
         size_t n2 = nC + sumE;
         if (ensure(b->P2, n2 + 1) || ensure(b->C2, sumE + 1)) return -1;
         int hu2, hu4;
+        if (sort_enqueue(E.Cb.p, sumE, vary_of(sumE, or4, and4), b->C2.p, 6, &hu4, 1)) return -1;
         if (sort_enqueue(E.A.p, n2, vary_of(n2, or2 | carry.o, and2 & carry.a), b->P2.p, 5, &hu2))
             return -1;
-        if (sort_enqueue(E.Cb.p, sumE, vary_of(sumE, or4, and4), b->C2.p, 6, &hu4)) return -1;
+        if (join_lane1()) return -1;
         /* directories of every table of this step */
         st.n = nM;
         st.off.assign(ntab + 1, 0);

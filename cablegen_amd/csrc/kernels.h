@@ -70,9 +70,10 @@ void launch_build_dir(hipStream_t st, const uint64_t *keys, size_t n, const Laye
                       uint32_t *dir, void *scratch);
 
 /* kernel-timer brackets provided by the engine (kind: 2 = radix histogram,
- * 3 = radix pass, 4 = unique); a negative handle means timing is off */
-int prof_hook_begin(int kind);
-void prof_hook_end(int h, double bytes);
+ * 3 = radix pass, 4 = unique), recorded on stream st; a negative handle
+ * means timing is off */
+int prof_hook_begin(int kind, hipStream_t st);
+void prof_hook_end(int h, double bytes, hipStream_t st);
 
 /* first device error seen by a launcher's runtime calls since the last call */
 hipError_t kernels_take_error();
@@ -103,13 +104,16 @@ void launch_spawn(hipStream_t st, const uint64_t *in, size_t n, const unsigned l
  * entirely on the stream (no host synchronisation): onesweep LSD radix over
  * the bits set in `vary` (OR & ~AND of the keys; bits outside it must be
  * constant), then a chained-scan compaction.  The unique count lands in
- * *d_nuniq.  tmp holds n keys; scratch holds sort_scratch_bytes(n).  A
+ * *d_nuniq.  tmp holds n keys; scratch holds sort_scratch_bytes(n).  Sorts of
+ * different contexts (ctx < SORT_CONTEXTS) may run concurrently on different
+ * streams, each with its own tmp and scratch.  A
  * chained-scan wait that exceeds its budget sets *d_err (never cleared
  * here); the caller checks it after synchronising.  keys and tmp are
  * clobbered.  *prof_unique receives the timer handle of the compaction. */
+constexpr int SORT_CONTEXTS = 2;
 uint64_t *sort_unique(hipStream_t st, uint64_t *keys, uint64_t *tmp, size_t n, uint64_t vary,
                       void *scratch, uint64_t *dst, unsigned long long *d_nuniq, unsigned *d_err,
-                      int *prof_unique);
+                      int *prof_unique, int ctx = 0);
 
 /* Sort configurations: digit width (8, 9 or 10 bits; the pass block has one
  * thread per digit), keys per thread (tile = 2^bits * kpt keys) and look-back

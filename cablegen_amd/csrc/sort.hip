@@ -521,47 +521,53 @@ SortCfg sort_choose(size_t n, int nbits)
 
 /* ---- look-back status array ---- */
 
-static uint64_t *g_status = nullptr;
-static size_t g_status_cap = 0;   /* entries */
-static uint32_t g_epoch = 0;
+/* one look-back status array per sort context: sorts of different contexts
+ * may run concurrently (on different streams); sorts of one context run in
+ * stream order */
+struct SortCtx {
+    uint64_t *status = nullptr;
+    size_t cap = 0;   /* entries */
+    uint32_t epoch = 0;
+};
+static SortCtx g_ctx[SORT_CONTEXTS];
 
-static bool status_reserve(hipStream_t st, size_t entries)
+static bool status_reserve(hipStream_t st, SortCtx &c, size_t entries)
 {
-    if (entries <= g_status_cap && g_status) return true;
+    if (entries <= c.cap && c.status) return true;
     size_t cap = std::max(entries + entries / 4, (size_t)1 << 16);
-    if (g_status) {
+    if (c.status) {
         snote(hipStreamSynchronize(st));
-        snote(hipFree(g_status));
-        g_status = nullptr;
-        g_status_cap = 0;
+        snote(hipFree(c.status));
+        c.status = nullptr;
+        c.cap = 0;
     }
-    hipError_t e = hipMalloc((void **)&g_status, cap * 8);
+    hipError_t e = hipMalloc((void **)&c.status, cap * 8);
     if (e != hipSuccess) {
         snote(e);
-        g_status = nullptr;
+        c.status = nullptr;
         return false;
     }
-    g_status_cap = cap;
-    snote(hipMemsetAsync(g_status, 0, cap * 8, st));
-    g_epoch = 0;
+    c.cap = cap;
+    snote(hipMemsetAsync(c.status, 0, cap * 8, st));
+    c.epoch = 0;
     return true;
 }
 
-static uint32_t next_epoch(hipStream_t st)
+static uint32_t next_epoch(hipStream_t st, SortCtx &c)
 {
-    if (++g_epoch > OS_EPOCH_MAX) {
-        snote(hipMemsetAsync(g_status, 0, g_status_cap * 8, st));
-        g_epoch = 1;
+    if (++c.epoch > OS_EPOCH_MAX) {
+        snote(hipMemsetAsync(c.status, 0, c.cap * 8, st));
+        c.epoch = 1;
     }
-    return g_epoch;
+    return c.epoch;
 }
 
 void sort_release()
 {
-    if (g_status) hipFree(g_status);
-    g_status = nullptr;
-    g_status_cap = 0;
-    g_epoch = 0;
+    for (SortCtx &c : g_ctx) {
+        if (c.status) hipFree(c.status);
+        c = SortCtx();
+    }
 }
 
 /* ---- launches ---- */
@@ -594,7 +600,7 @@ static void launch_hist_scan(hipStream_t st, int bits, const uint64_t *keys, siz
     const uint64_t vmask = xform_inv(~0ULL, x);
     size_t hb = (n + 4095) / 4096;
     if (hb > 2048) hb = 2048;
-    int h = prof_hook_begin(2);
+    int h = prof_hook_begin(2, st);
     switch (bits) {
     case 9:
         hipLaunchKernelGGL(k_os_hist<9>, dim3((unsigned)hb), dim3(256), 0, st, keys, n, x, npass,
@@ -609,7 +615,7 @@ static void launch_hist_scan(hipStream_t st, int bits, const uint64_t *keys, siz
                            s.ghist, vmask, s.cval);
     }
     snote(hipGetLastError());
-    prof_hook_end(h, 8.0 * n);
+    prof_hook_end(h, 8.0 * n, st);
     switch (bits) {
     case 9:
         hipLaunchKernelGGL(k_os_scan<9>, dim3(1), dim3(512), 0, st, s.ghist, npass, s.gbase);
@@ -626,11 +632,11 @@ static void launch_hist_scan(hipStream_t st, int bits, const uint64_t *keys, siz
 template <bool RAW>
 static void launch_pass_t(hipStream_t st, SortCfg c, unsigned tiles, const uint64_t *src,
                           uint64_t *dst, size_t n, KeyXform x, int shift, const uint64_t *gbase,
-                          unsigned *ctr, uint32_t epoch, unsigned *err)
+                          uint64_t *status, unsigned *ctr, uint32_t epoch, unsigned *err)
 {
 #define CG_PASS(B, K, W)                                                                          \
     hipLaunchKernelGGL((k_os_pass<B, K, W, RAW>), dim3(tiles), dim3(1 << B), 0, st, src, dst, n, x, \
-                       shift, gbase, g_status, ctr, epoch, err)
+                       shift, gbase, status, ctr, epoch, err)
     const bool wide = c.lbw >= 32;
     if (c.bits == 10) {
         if (wide) CG_PASS(10, 8, 32);
@@ -658,17 +664,18 @@ static void launch_pass_t(hipStream_t st, SortCfg c, unsigned tiles, const uint6
 
 static void launch_pass(hipStream_t st, SortCfg c, unsigned tiles, const uint64_t *src,
                         uint64_t *dst, size_t n, KeyXform x, int shift, const uint64_t *gbase,
-                        unsigned *ctr, uint32_t epoch, unsigned *err, bool raw)
+                        uint64_t *status, unsigned *ctr, uint32_t epoch, unsigned *err, bool raw)
 {
-    if (raw) launch_pass_t<true>(st, c, tiles, src, dst, n, x, shift, gbase, ctr, epoch, err);
-    else launch_pass_t<false>(st, c, tiles, src, dst, n, x, shift, gbase, ctr, epoch, err);
+    if (raw) launch_pass_t<true>(st, c, tiles, src, dst, n, x, shift, gbase, status, ctr, epoch, err);
+    else launch_pass_t<false>(st, c, tiles, src, dst, n, x, shift, gbase, status, ctr, epoch, err);
 }
 
 uint64_t *sort_unique(hipStream_t st, uint64_t *keys, uint64_t *tmp, size_t n, uint64_t vary,
                       void *scratch, uint64_t *dst, unsigned long long *d_nuniq, unsigned *d_err,
-                      int *prof_unique)
+                      int *prof_unique, int ctx)
 {
     *prof_unique = -1;
+    SortCtx &sc = g_ctx[ctx < 0 || ctx >= SORT_CONTEXTS ? 0 : ctx];
     if (n == 0) {
         launch_fill_u64(st, (uint64_t *)d_nuniq, 1, 0);
         return keys;
@@ -680,26 +687,27 @@ uint64_t *sort_unique(hipStream_t st, uint64_t *keys, uint64_t *tmp, size_t n, u
     const size_t tile_keys = ((size_t)1 << c.bits) * c.kpt;
     const size_t tiles = (n + tile_keys - 1) / tile_keys;
     const size_t utiles = (n + UQ_TILE - 1) / UQ_TILE;
-    if (!status_reserve(st, std::max(npass ? tiles << c.bits : 0, utiles))) return keys;
+    if (!status_reserve(st, sc, std::max(npass ? tiles << c.bits : 0, utiles))) return keys;
     /* tile counters and histograms cleared together */
     launch_fill_u64(st, (uint64_t *)scratch, OS_CLEAR_BYTES / 8, 0);
     uint64_t *src = keys, *dst2 = tmp;
     if (npass > 0) {
         launch_hist_scan(st, c.bits, keys, n, x, npass, s);
         for (int p = 0; p < npass; p++) {
-            int h = prof_hook_begin(3);
+            int h = prof_hook_begin(3, st);
             launch_pass(st, c, (unsigned)tiles, src, dst2, n, x, c.bits * p,
-                        s.gbase + ((size_t)p << c.bits), s.ctr + p, next_epoch(st), d_err, p == 0);
-            prof_hook_end(h, 16.0 * n);
+                        s.gbase + ((size_t)p << c.bits), sc.status, s.ctr + p, next_epoch(st, sc), d_err,
+                        p == 0);
+            prof_hook_end(h, 16.0 * n, st);
             std::swap(src, dst2);
         }
     }
-    *prof_unique = prof_hook_begin(4);
+    *prof_unique = prof_hook_begin(4, st);
     hipLaunchKernelGGL(k_os_unique, dim3((unsigned)utiles), dim3(UQ_THREADS), 0, st, src, n, dst,
-                       g_status, s.ctr + OS_MAXPASS, next_epoch(st), d_nuniq, d_err, npass > 0 ? 1 : 0,
-                       x, s.cval);
+                       sc.status, s.ctr + OS_MAXPASS, next_epoch(st, sc), d_nuniq, d_err,
+                       npass > 0 ? 1 : 0, x, s.cval);
     snote(hipGetLastError());
-    prof_hook_end(*prof_unique, 8.0 * n);
+    prof_hook_end(*prof_unique, 8.0 * n, st);
     return src;
 }
 
