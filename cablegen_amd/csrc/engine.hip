@@ -488,7 +488,7 @@ static void counts_arrived(StepCounts &pc, size_t nP, size_t nC)
 static int counts_finish(StepCounts &pc, size_t *nP, size_t *nC)
 {
     if (!pc.pending) return 0;
-    if (read_counters(5, 2)) return -1;
+    if (join_lane1() || read_counters(5, 2)) return -1;
     if (nP) *nP = E.h_cnt[5];
     if (nC) *nC = E.h_cnt[6];
     counts_arrived(pc, E.h_cnt[5], E.h_cnt[6]);
@@ -515,6 +515,9 @@ static int gen_step(const cg_rules &r, const uint64_t *P, size_t nP, const uint6
     int h = prof_begin(PK_MOVE);
     launch_move(E.st, P, nP, pc.pending ? E.d_cnt + 5 : nullptr, E.A.p, E.d_cnt, lut, move_rules(r));
     prof_end(h, 0);
+    /* the previous step's carry sort (second stream) overlapped the moves;
+     * its count and output are needed from here on */
+    if (pc.pending && join_lane1()) return -1;
     if (read_counters(0, pc.pending ? 7 : 3)) return -1;
     if (pc.pending) {
         nP = E.h_cnt[5];
@@ -565,7 +568,6 @@ static int gen_step(const cg_rules &r, const uint64_t *P, size_t nP, const uint6
     if (sort_enqueue(E.Cb.p, sumE, vary_of(sumE, or4, and4), nxtC.p, 6, &hu4, 1)) return -1;
     if (sort_enqueue(E.A.p, n2, vary_of(n2, or2 | carry.o, and2 & carry.a), nxtP.p, 5, &hu2))
         return -1;
-    if (join_lane1()) return -1;
     pc.pending = true;
     pc.n2 = n2;
     pc.sumE = sumE;
@@ -1668,6 +1670,7 @@ extern "C" int cg_batch_generate(cg_batch *b)
         launch_bmove(E.st, b->P.p, nP, pc.pending ? E.d_cnt + 5 : nullptr, E.A.p, E.d_cnt, lut,
                      move_rules(b->r), b->d_tabs, k, E.d_err);
         prof_end(h, 0);
+        if (pc.pending && join_lane1()) return -1;
         if (read_counters(0, pc.pending ? 7 : 3)) return -1;
         if (pc.pending) {
             nP = E.h_cnt[5];
@@ -1716,7 +1719,6 @@ extern "C" int cg_batch_generate(cg_batch *b)
         if (sort_enqueue(E.Cb.p, sumE, vary_of(sumE, or4, and4), b->C2.p, 6, &hu4, 1)) return -1;
         if (sort_enqueue(E.A.p, n2, vary_of(n2, or2 | carry.o, and2 & carry.a), b->P2.p, 5, &hu2))
             return -1;
-        if (join_lane1()) return -1;
         /* directories of every table of this step */
         st.n = nM;
         st.off.assign(ntab + 1, 0);
