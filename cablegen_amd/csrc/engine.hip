@@ -1076,7 +1076,8 @@ extern "C" int cg_tune_set(const char *knob, int value)
         return sort_force(value, f.kpt, f.lbw);
     }
     if (!strcmp(knob, "sort_kpt")) {
-        if (value && value != 8 && value != 16 && value != 32) return fail_msg("sort_kpt: 0, 8, 16 or 32");
+        if (value && value != 7 && value != 8 && value != 16 && value != 32)
+            return fail_msg("sort_kpt: 0, 7, 8, 16 or 32");
         return sort_force(f.bits, value, f.lbw);
     }
     if (!strcmp(knob, "sort_lbw")) {

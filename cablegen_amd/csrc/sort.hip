@@ -184,19 +184,35 @@ __global__ __launch_bounds__(1 << BITS) void k_os_scan(const uint32_t *ghist, in
  * the packed key.  The first pass (RAW) reads the original keys and packs
  * them (xform) once; every pass writes packed keys, so later passes extract
  * a digit with a shift and a mask. */
+/* threads of a pass block: one per digit, except the 8-keys-per-thread
+ * configurations, which use 1024 threads (the first 2^bits own the digits) so
+ * that an 8192-key tile is spread over 16 waves */
+constexpr int pass_threads(int bits, int kpt) { return kpt <= 8 ? 1024 : (1 << bits); }
+
 /* at least 2 waves per SIMD: the registers of a KPT = 32 block then match the
  * two blocks per CU its 70 KiB of LDS allows (asking for 4 with KPT = 16
  * spills) */
+/* 8-bit digits x 8 keys and 9-bit digits x 7 keys (7168-key tiles, 78 KiB of
+ * LDS) with 1024 threads and window 16: two blocks (32 waves) per CU, which
+ * needs <= 64 VGPRs (a small spill in the look-back); window 32: one block per
+ * CU, no spill.  The autotuner picks between them. */
+constexpr int pass_min_waves(int bits, int kpt, int lbw)
+{
+    return (((bits == 8 && kpt == 8) || (bits == 9 && kpt == 7)) && lbw == 16) ? 8 : 2;
+}
+
 template <int BITS, int KPT, int LB_W, bool RAW>
-__global__ __launch_bounds__(1 << BITS, 2) void k_os_pass(const uint64_t *__restrict__ in,
+__global__ __launch_bounds__(pass_threads(BITS, KPT), pass_min_waves(BITS, KPT, LB_W)) void k_os_pass(const uint64_t *__restrict__ in,
                                                        uint64_t *__restrict__ out, size_t n,
                                                        KeyXform x, int shift,
                                                        const uint64_t *__restrict__ gbase,
                                                        uint64_t *status, unsigned *tile_ctr,
                                                        uint32_t epoch, unsigned *err)
 {
-    constexpr int R = 1 << BITS, NW = R / 64, TILE = R * KPT, WCHUNK = 64 * KPT;
+    constexpr int R = 1 << BITS, T = pass_threads(BITS, KPT), NW = T / 64, TILE = T * KPT,
+                  WCHUNK = 64 * KPT;
     static_assert(TILE < 65536, "per-wave counters are 16-bit");
+    static_assert(T >= R, "a thread per digit");
     __shared__ uint16_t wcnt[NW * R];   /* per-wave digit counts, then wave prefixes */
     __shared__ uint32_t tstart[R];
     __shared__ uint64_t obase[R];
@@ -206,7 +222,7 @@ __global__ __launch_bounds__(1 << BITS, 2) void k_os_pass(const uint64_t *__rest
 
     const int w = threadIdx.x >> 6, lane = lane_id();
     if (threadIdx.x == 0) tile_sh = atomicAdd(tile_ctr, 1u);
-    for (int i = threadIdx.x; i < NW * R; i += R) wcnt[i] = 0;
+    for (int i = threadIdx.x; i < NW * R; i += T) wcnt[i] = 0;
     __syncthreads();
     const size_t tile = tile_sh;
     const size_t base = tile * TILE + (size_t)w * WCHUNK;
@@ -245,28 +261,33 @@ __global__ __launch_bounds__(1 << BITS, 2) void k_os_pass(const uint64_t *__rest
         else rank2[j >> 1] = r;
     }
     __syncthreads();
-    /* thread d: wave prefixes (in place) and the tile's count of digit d */
+    /* thread d < R: wave prefixes (in place) and the tile's count of digit d */
     const int d = threadIdx.x;
     uint32_t tot = 0;
+    if (d < R) {
 #pragma unroll
-    for (int ww = 0; ww < NW; ww++) {
-        uint32_t c = wcnt[ww * R + d];
-        wcnt[ww * R + d] = (uint16_t)tot;
-        tot += c;
+        for (int ww = 0; ww < NW; ww++) {
+            uint32_t c = wcnt[ww * R + d];
+            wcnt[ww * R + d] = (uint16_t)tot;
+            tot += c;
+        }
+        /* chained scan over tiles for digit d */
+        uint64_t *my = status + tile * R + d;
+        uint64_t excl = 0;
+        if (tile == 0) {
+            st_flag(my, OS_INC, epoch, tot);
+        } else {
+            st_flag(my, OS_AGG, epoch, tot);
+            /* the 8-waves-per-SIMD variant keeps its look-back window small */
+        constexpr int LBW = pass_min_waves(BITS, KPT, LB_W) > 2 ? LB_W / 2 : LB_W;
+        excl = lookback<LBW>(status + d, R, tile, epoch, err);
+            st_flag(my, OS_INC, epoch, excl + tot);
+        }
+        obase[d] = gbase[d] + excl;
     }
-    /* chained scan over tiles for digit d */
-    uint64_t *my = status + tile * R + d;
-    uint64_t excl = 0;
-    if (tile == 0) {
-        st_flag(my, OS_INC, epoch, tot);
-    } else {
-        st_flag(my, OS_AGG, epoch, tot);
-        excl = lookback<LB_W>(status + d, R, tile, epoch, err);
-        st_flag(my, OS_INC, epoch, excl + tot);
-    }
-    obase[d] = gbase[d] + excl;
     uint32_t all;
-    tstart[d] = block_excl_scan<uint32_t>(tot, &all, wtmp);
+    uint32_t ts = block_excl_scan<uint32_t>(tot, &all, wtmp);   /* threads >= R add 0 */
+    if (d < R) tstart[d] = ts;
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < KPT; j++) {
@@ -282,7 +303,7 @@ __global__ __launch_bounds__(1 << BITS, 2) void k_os_pass(const uint64_t *__rest
     __syncthreads();
     size_t tile0 = tile * TILE;
     uint32_t tn = (uint32_t)((n - tile0) < (size_t)TILE ? (n - tile0) : (size_t)TILE);
-    for (uint32_t p = threadIdx.x; p < tn; p += R) {
+    for (uint32_t p = threadIdx.x; p < tn; p += T) {
         uint64_t kk = stage[p];
         unsigned dd = (unsigned)(kk >> shift) & (R - 1);
         out[obase[dd] + (p - tstart[dd])] = kk;
@@ -438,8 +459,8 @@ KeyXform make_xform(uint64_t vary)
 /* ---- configurations ---- */
 
 static const SortCfg k_cfgs[] = {
-    {8, 32, 16}, {8, 32, 32}, {8, 16, 16}, {8, 16, 32}, {9, 16, 16},
-    {9, 16, 32}, {9, 32, 16}, {9, 32, 32}, {10, 8, 16}, {10, 8, 32},
+    {8, 32, 16}, {8, 32, 32}, {8, 16, 16}, {8, 16, 32}, {8, 8, 16},  {8, 8, 32},  {9, 16, 16},
+    {9, 16, 32}, {9, 32, 16}, {9, 32, 32}, {9, 7, 16},  {9, 7, 32},  {10, 8, 16}, {10, 8, 32},
 };
 constexpr int N_CFGS = sizeof(k_cfgs) / sizeof(k_cfgs[0]);
 
@@ -634,15 +655,18 @@ static void launch_pass_t(hipStream_t st, SortCfg c, unsigned tiles, const uint6
                           uint64_t *dst, size_t n, KeyXform x, int shift, const uint64_t *gbase,
                           uint64_t *status, unsigned *ctr, uint32_t epoch, unsigned *err)
 {
-#define CG_PASS(B, K, W)                                                                          \
-    hipLaunchKernelGGL((k_os_pass<B, K, W, RAW>), dim3(tiles), dim3(1 << B), 0, st, src, dst, n, x, \
-                       shift, gbase, status, ctr, epoch, err)
+#define CG_PASS(B, K, W)                                                                         \
+    hipLaunchKernelGGL((k_os_pass<B, K, W, RAW>), dim3(tiles), dim3(pass_threads(B, K)), 0, st, src, \
+                       dst, n, x, shift, gbase, status, ctr, epoch, err)
     const bool wide = c.lbw >= 32;
     if (c.bits == 10) {
         if (wide) CG_PASS(10, 8, 32);
         else CG_PASS(10, 8, 16);
     } else if (c.bits == 9) {
-        if (c.kpt == 32) {
+        if (c.kpt == 7) {
+            if (wide) CG_PASS(9, 7, 32);
+            else CG_PASS(9, 7, 16);
+        } else if (c.kpt == 32) {
             if (wide) CG_PASS(9, 32, 32);
             else CG_PASS(9, 32, 16);
         } else {
@@ -650,7 +674,10 @@ static void launch_pass_t(hipStream_t st, SortCfg c, unsigned tiles, const uint6
             else CG_PASS(9, 16, 16);
         }
     } else {
-        if (c.kpt == 16) {
+        if (c.kpt == 8) {
+            if (wide) CG_PASS(8, 8, 32);
+            else CG_PASS(8, 8, 16);
+        } else if (c.kpt == 16) {
             if (wide) CG_PASS(8, 16, 32);
             else CG_PASS(8, 16, 16);
         } else {
@@ -684,7 +711,7 @@ uint64_t *sort_unique(hipStream_t st, uint64_t *keys, uint64_t *tmp, size_t n, u
     KeyXform x = make_xform(n > 1 ? vary : 0);
     const SortCfg c = sort_choose(n, x.nbits);
     const int npass = (x.nbits + c.bits - 1) / c.bits;
-    const size_t tile_keys = ((size_t)1 << c.bits) * c.kpt;
+    const size_t tile_keys = (size_t)pass_threads(c.bits, c.kpt) * c.kpt;
     const size_t tiles = (n + tile_keys - 1) / tile_keys;
     const size_t utiles = (n + UQ_TILE - 1) / UQ_TILE;
     if (!status_reserve(st, sc, std::max(npass ? tiles << c.bits : 0, utiles))) return keys;

@@ -143,8 +143,9 @@ int cg_dev_sync(void);
 /* Knobs (value 0 returns a forced field to the tuned table):
  *   "sort_bits"  radix digit width of every sort: 8, 9 or 10 bits (a pass
  *                block has one thread per digit);
- *   "sort_kpt"   keys per thread of the passes: 32 or 16 (8 bits and 9 bits),
- *                8 (10 bits); a tile is 2^bits * kpt keys;
+ *   "sort_kpt"   keys per thread of the passes: 32 or 16 (8 and 9 bits, one
+ *                thread per digit), 8 (8 and 10 bits) or 7 (9 bits) with 1024
+ *                threads; a tile is threads * kpt keys;
  *   "sort_lbw"   look-back window of the passes: 16 or 32 tiles;
  *   "dir_delta"  prefix-bucket directory depth: bits = floor(log2 n) + delta
  *                (-4..3), clamped to the layer's varying bits and 30;

@@ -137,8 +137,9 @@ def test_sort_unique(cg, case):
     assert np.array_equal(host(out, n.value), np.unique(a))
 
 
-SORT_CONFIGS = [(8, 32, 16), (8, 32, 32), (8, 16, 16), (8, 16, 32), (9, 16, 16), (9, 16, 32),
-                (9, 32, 16), (9, 32, 32), (10, 8, 16), (10, 8, 32)]
+SORT_CONFIGS = [(8, 32, 16), (8, 32, 32), (8, 16, 16), (8, 16, 32), (8, 8, 16), (8, 8, 32),
+                (9, 16, 16), (9, 16, 32), (9, 32, 16), (9, 32, 32), (9, 7, 16), (9, 7, 32),
+                (10, 8, 16), (10, 8, 32)]
 
 
 def _sort_check(cg, a):
@@ -155,7 +156,7 @@ def test_sort_every_configuration(cg, cfg):
     numpy's unique: tile-boundary sizes, key widths that leave a partial last
     digit, 64-bit keys, heavy duplicates and board keys."""
     bits, kpt, lbw = cfg
-    tile = (1 << bits) * kpt
+    tile = (1024 if kpt <= 8 else 1 << bits) * kpt
     rng = np.random.default_rng(bits * 1000 + kpt * 10 + lbw)
     try:
         cg.tune_set("sort_bits", bits)
