@@ -48,28 +48,45 @@ CG_HD int cg_popc64(uint64_t x)
 
 CG_HD int cg_empties(uint64_t b) { return cg_popc64(cg_zmask(b)); }
 
-/* rotate_clockwise (board.c:239-248): out(r,c) = in(3-c, r) */
-CG_HD uint64_t cg_rot_cw(uint64_t b)
+/* The D4 permutations work on the two 32-bit halves (rows 0-1 and rows
+ * 2-3), where they are byte permutes, 16-bit rotates and masked nibble
+ * shifts -- a few single-cycle ALU operations each on gfx950. */
+CG_HD uint64_t cg_join(uint32_t hi, uint32_t lo) { return ((uint64_t)hi << 32) | lo; }
+
+/* rotate_clockwise (board.c:239-248): out(r,c) = in(3-c, r).  Step 1 moves
+ * bytes between the halves; step 2 moves nibbles inside each half. */
+CG_HD uint32_t cg_rot_half(uint32_t x)
 {
-    b = ((b & 0xff00ff0000000000ULL) >> 8) | ((b & 0x00ff00ff00000000ULL) >> 32) |
-        ((b & 0x00000000ff00ff00ULL) << 32) | ((b & 0x0000000000ff00ffULL) << 8);
-    b = ((b & 0xf0f00000f0f00000ULL) >> 4) | ((b & 0x0f0f00000f0f0000ULL) >> 16) |
-        ((b & 0x0000f0f00000f0f0ULL) << 16) | ((b & 0x00000f0f00000f0fULL) << 4);
-    return b;
+    uint32_t r16 = (x >> 16) | (x << 16);
+    return (r16 & 0xf0f00f0fu) | ((x >> 4) & 0x0f0f0000u) | ((x << 4) & 0x0000f0f0u);
 }
 
-/* row-order reversal (flip, board.c:376-381) */
+CG_HD uint64_t cg_rot_cw(uint64_t b)
+{
+    uint32_t hi = (uint32_t)(b >> 32), lo = (uint32_t)b;
+    uint32_t h1 = ((hi >> 8) & 0x00ff00ffu) | (lo & 0xff00ff00u);
+    uint32_t l1 = (hi & 0x00ff00ffu) | ((lo << 8) & 0xff00ff00u);
+    return cg_join(cg_rot_half(h1), cg_rot_half(l1));
+}
+
+/* row-order reversal (flip, board.c:376-381): swap the halves and the rows
+ * inside each */
 CG_HD uint64_t cg_flip_rows(uint64_t b)
 {
-    b = (b >> 32) | (b << 32);
-    return ((b & 0xffff0000ffff0000ULL) >> 16) | ((b & 0x0000ffff0000ffffULL) << 16);
+    uint32_t hi = (uint32_t)(b >> 32), lo = (uint32_t)b;
+    return cg_join((lo >> 16) | (lo << 16), (hi >> 16) | (hi << 16));
 }
 
 /* nibble reversal inside every 16-bit row (mirror left/right) */
+CG_HD uint32_t cg_mirror_half(uint32_t x)
+{
+    x = ((x >> 8) & 0x00ff00ffu) | ((x << 8) & 0xff00ff00u);
+    return ((x >> 4) & 0x0f0f0f0fu) | ((x << 4) & 0xf0f0f0f0u);
+}
+
 CG_HD uint64_t cg_mirror_rows(uint64_t b)
 {
-    b = ((b & 0xff00ff00ff00ff00ULL) >> 8) | ((b & 0x00ff00ff00ff00ffULL) << 8);
-    return ((b & 0xf0f0f0f0f0f0f0f0ULL) >> 4) | ((b & 0x0f0f0f0f0f0f0f0fULL) << 4);
+    return cg_join(cg_mirror_half((uint32_t)(b >> 32)), cg_mirror_half((uint32_t)b));
 }
 
 CG_HD uint64_t cg_umax(uint64_t a, uint64_t b) { return a > b ? a : b; }

@@ -143,13 +143,24 @@ __device__ __forceinline__ int next_cell(uint64_t &z)
     return 15 - (pos >> 2);
 }
 
+/* (board, empty cell) pairs of one 256-board chunk, in output order */
+struct SpawnPairs {
+    uint64_t sb[SP_THREADS];
+    uint8_t pb[16 * SP_THREADS], pc[16 * SP_THREADS];
+    uint32_t wtmp[17];
+};
+
+/* Each thread lists its board's empty cells (ascending) as pairs at its
+ * exclusive-scan offset; then consecutive threads take consecutive pairs, so
+ * the two output arrays are written with coalesced stores and every lane has
+ * the same work whatever the number of empty cells. */
 __global__ __launch_bounds__(SP_THREADS) void k_spawn(const uint64_t *__restrict__ in, size_t n,
                                                       const unsigned long long *dn,
                                                       uint64_t *__restrict__ out2,
                                                       uint64_t *__restrict__ out4, size_t cap,
                                                       unsigned long long *stats)
 {
-    __shared__ uint32_t wtmp[17];
+    __shared__ SpawnPairs sp;
     __shared__ unsigned long long base_sh;
     __shared__ uint64_t ro[16], ra[16];
     uint64_t or2 = 0, and2 = ~0ULL, or4 = 0, and4 = ~0ULL;
@@ -161,13 +172,21 @@ __global__ __launch_bounds__(SP_THREADS) void k_spawn(const uint64_t *__restrict
         uint64_t z = i < n ? cg_zmask(b) : 0;
         uint32_t e = (uint32_t)cg_popc64(z);
         uint32_t tot;
-        uint32_t off = block_excl_scan<uint32_t>(e, &tot, wtmp);
+        uint32_t off = block_excl_scan<uint32_t>(e, &tot, sp.wtmp);
+        sp.sb[threadIdx.x] = b;
+        while (z) {
+            sp.pb[off] = (uint8_t)threadIdx.x;
+            sp.pc[off] = (uint8_t)next_cell(z);
+            off++;
+        }
         if (threadIdx.x == 0) base_sh = tot ? atomicAdd(&stats[0], (unsigned long long)tot) : 0;
         __syncthreads();
-        size_t o = base_sh + off;
-        while (z) {
-            int x = next_cell(z);
-            uint64_t c2 = cg_canon(cg_set_tile(b, x, 1)), c4 = cg_canon(cg_set_tile(b, x, 2));
+        const size_t base = base_sh;
+        for (uint32_t p = threadIdx.x; p < tot; p += SP_THREADS) {
+            uint64_t bb = sp.sb[sp.pb[p]];
+            int x = sp.pc[p];
+            uint64_t c2 = cg_canon(cg_set_tile(bb, x, 1)), c4 = cg_canon(cg_set_tile(bb, x, 2));
+            size_t o = base + p;
             if (o < cap) {   /* past the capacity: only counted (the host retries) */
                 out2[o] = c2;
                 out4[o] = c4;
@@ -176,7 +195,6 @@ __global__ __launch_bounds__(SP_THREADS) void k_spawn(const uint64_t *__restrict
             and2 &= c2;
             or4 |= c4;
             and4 &= c4;
-            o++;
         }
         __syncthreads();
     }
@@ -885,7 +903,8 @@ __global__ __launch_bounds__(SP_THREADS) void k_bspawn(const uint64_t *__restric
                                                        const BatchTable *__restrict__ tabs, int step,
                                                        unsigned *err)
 {
-    __shared__ uint32_t wtmp[17];
+    __shared__ SpawnPairs sp;
+    __shared__ uint16_t tid_sh[SP_THREADS];   /* table ids < 2^(64 - BATCH_TID_SHIFT) */
     __shared__ unsigned long long base_sh;
     __shared__ uint64_t ro[16], ra[16];
     uint64_t or2 = 0, and2 = ~0ULL, or4 = 0, and4 = ~0ULL;
@@ -905,15 +924,26 @@ __global__ __launch_bounds__(SP_THREADS) void k_bspawn(const uint64_t *__restric
         }
         uint32_t e = (uint32_t)cg_popc64(z);
         uint32_t tot;
-        uint32_t off = block_excl_scan<uint32_t>(e, &tot, wtmp);
+        uint32_t off = block_excl_scan<uint32_t>(e, &tot, sp.wtmp);
+        sp.sb[threadIdx.x] = b;
+        tid_sh[threadIdx.x] = (uint16_t)tid;
+        while (z) {
+            sp.pb[off] = (uint8_t)threadIdx.x;
+            sp.pc[off] = (uint8_t)next_cell(z);
+            off++;
+        }
         if (threadIdx.x == 0) base_sh = tot ? atomicAdd(&stats[0], (unsigned long long)tot) : 0;
         __syncthreads();
-        size_t o = base_sh + off;
-        while (z) {
-            int x = next_cell(z);
-            const BatchTable &T = tabs[tid];
-            uint64_t c2 = benc(cg_canon(cg_set_tile(b, x, 1)), T, tid, err);
-            uint64_t c4 = benc(cg_canon(cg_set_tile(b, x, 2)), T, tid, err);
+        const size_t base = base_sh;
+        for (uint32_t p = threadIdx.x; p < tot; p += SP_THREADS) {
+            int q = sp.pb[p];
+            uint64_t bb = sp.sb[q];
+            uint64_t t = tid_sh[q];
+            const BatchTable &T = tabs[t];
+            int x = sp.pc[p];
+            uint64_t c2 = benc(cg_canon(cg_set_tile(bb, x, 1)), T, t, err);
+            uint64_t c4 = benc(cg_canon(cg_set_tile(bb, x, 2)), T, t, err);
+            size_t o = base + p;
             if (o < cap) {
                 out2[o] = c2;
                 out4[o] = c4;
@@ -922,7 +952,6 @@ __global__ __launch_bounds__(SP_THREADS) void k_bspawn(const uint64_t *__restric
             and2 &= c2;
             or4 |= c4;
             and4 &= c4;
-            o++;
         }
         __syncthreads();
     }

@@ -123,12 +123,15 @@ __device__ __forceinline__ uint64_t lookback(const uint64_t *status, size_t stri
     return excl;
 }
 
-/* digit counts of every pass in one read of the keys.  Board keys are skewed
- * (the top digits take few values), so lanes of one wave often hit the same
- * counter: NC copies of the counters indexed by lane % NC, each starting in a
- * different LDS bank, cap a wave's conflict on one counter at 64 / NC lanes. */
+/* digit counts of every pass in one read of the keys, 16 waves per block.
+ * Board keys are skewed (the top digits take few values), so lanes of one
+ * wave often hit the same counter: NC copies of the counters indexed by
+ * lane % NC, each starting in a different LDS bank, cap a wave's conflict on
+ * one counter at 64 / NC lanes. */
+constexpr int HIST_THREADS = 1024;
+
 template <int BITS>
-__global__ __launch_bounds__(256) void k_os_hist(const uint64_t *__restrict__ keys, size_t n,
+__global__ __launch_bounds__(HIST_THREADS) void k_os_hist(const uint64_t *__restrict__ keys, size_t n,
                                                  KeyXform x, int npass, uint32_t *ghist,
                                                  uint64_t vmask, uint64_t *cval)
 {
@@ -137,27 +140,28 @@ __global__ __launch_bounds__(256) void k_os_hist(const uint64_t *__restrict__ ke
     __shared__ uint32_t h[NC * STRIDE];
     /* the bits outside the packed ones are the same in every key */
     if (blockIdx.x == 0 && threadIdx.x == 0) *cval = keys[0] & ~vmask;
-    for (int i = threadIdx.x; i < NC * STRIDE; i += 256) h[i] = 0;
+    for (int i = threadIdx.x; i < NC * STRIDE; i += HIST_THREADS) h[i] = 0;
     __syncthreads();
     uint32_t *hc = h + (lane_id() % NC) * STRIDE;
     constexpr int HB = 8;   /* keys loaded per thread before counting */
-    for (size_t b0 = (size_t)blockIdx.x * 256 * HB; b0 < n; b0 += (size_t)gridDim.x * 256 * HB) {
+    for (size_t b0 = (size_t)blockIdx.x * HIST_THREADS * HB; b0 < n;
+         b0 += (size_t)gridDim.x * HIST_THREADS * HB) {
         uint64_t kk[HB];
 #pragma unroll
         for (int j = 0; j < HB; j++) {
-            size_t i = b0 + (size_t)j * 256 + threadIdx.x;
+            size_t i = b0 + (size_t)j * HIST_THREADS + threadIdx.x;
             kk[j] = i < n ? keys[i] : 0;
         }
 #pragma unroll
         for (int j = 0; j < HB; j++) {
-            size_t i = b0 + (size_t)j * 256 + threadIdx.x;
+            size_t i = b0 + (size_t)j * HIST_THREADS + threadIdx.x;
             if (i >= n) continue;
             uint64_t v = xform(kk[j], x);
             for (int p = 0; p < npass; p++) atomicAdd(&hc[p * R + ((v >> (BITS * p)) & (R - 1))], 1u);
         }
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < npass * R; i += 256) {
+    for (int i = threadIdx.x; i < npass * R; i += HIST_THREADS) {
         uint32_t t = 0;
 #pragma unroll
         for (int k = 0; k < NC; k++) t += h[k * STRIDE + i];
@@ -358,7 +362,9 @@ __global__ __launch_bounds__(UQ_THREADS) void k_os_unique(const uint64_t *__rest
     __syncthreads();
     if (w == 0) {
         /* wave 0 publishes and looks back 64 tiles per step (lane i reads tile t - i) */
-        uint32_t tot = wc[0] + wc[1] + wc[2] + wc[3];
+        uint32_t tot = 0;
+#pragma unroll
+        for (int ww = 0; ww < UQ_THREADS / 64; ww++) tot += wc[ww];
         uint64_t *my = status + tile;
         uint64_t excl = 0;
         if (tile == 0) {
@@ -619,20 +625,20 @@ static void launch_hist_scan(hipStream_t st, int bits, const uint64_t *keys, siz
                              int npass, const OsScratch &s)
 {
     const uint64_t vmask = xform_inv(~0ULL, x);
-    size_t hb = (n + 4095) / 4096;
-    if (hb > 2048) hb = 2048;
+    size_t hb = (n + 8191) / 8192;
+    if (hb > 1024) hb = 1024;
     int h = prof_hook_begin(2, st);
     switch (bits) {
     case 9:
-        hipLaunchKernelGGL(k_os_hist<9>, dim3((unsigned)hb), dim3(256), 0, st, keys, n, x, npass,
+        hipLaunchKernelGGL(k_os_hist<9>, dim3((unsigned)hb), dim3(HIST_THREADS), 0, st, keys, n, x, npass,
                            s.ghist, vmask, s.cval);
         break;
     case 10:
-        hipLaunchKernelGGL(k_os_hist<10>, dim3((unsigned)hb), dim3(256), 0, st, keys, n, x, npass,
+        hipLaunchKernelGGL(k_os_hist<10>, dim3((unsigned)hb), dim3(HIST_THREADS), 0, st, keys, n, x, npass,
                            s.ghist, vmask, s.cval);
         break;
     default:
-        hipLaunchKernelGGL(k_os_hist<8>, dim3((unsigned)hb), dim3(256), 0, st, keys, n, x, npass,
+        hipLaunchKernelGGL(k_os_hist<8>, dim3((unsigned)hb), dim3(HIST_THREADS), 0, st, keys, n, x, npass,
                            s.ghist, vmask, s.cval);
     }
     snote(hipGetLastError());
