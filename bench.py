@@ -282,8 +282,8 @@ def run_autotune(cg, args):
     mode = args.tuning
     t0 = time.perf_counter()
     if mode == "off":
-        cg.tune_set("reset", 0)
-        return {"source": "defaults: 8-bit digits, 8192-key tiles, window 16, dir delta 0"}
+        cg.tune_set("builtin", 0)
+        return {"source": "built-in defaults: 8-bit digits, 8192-key tiles, window 16, dir delta 0"}
     if mode == "stored" and os.path.exists(cg.TUNED_TABLE):
         rep = cg.load_tuning()
         src = "stored autotuner table cablegen_amd/tuned/mi355x.json"

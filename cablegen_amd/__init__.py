@@ -111,6 +111,7 @@ def lib():
     L.cg_tune_sort_query.argtypes = [C.c_size_t, C.c_int, C.POINTER(C.c_int),
                                      C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.cg_autotune.argtypes = [C.c_int, C.c_char_p, C.c_size_t]
+    L.cg_tune_load.argtypes = [C.c_char_p]
     L.cg_prof_enable.argtypes = [C.c_int]
     L.cg_prof_query.restype = C.c_long
     L.cg_prof_query.argtypes = [C.c_char_p, F64P, F64P]
@@ -264,7 +265,7 @@ TUNED_TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned", 
 def apply_tuning(rep):
     """Installs the picks of an autotuner report (sort table, directory depth)."""
     L = lib()
-    tune_set("reset", 0)
+    tune_set("builtin", 0)
     for cls, c in enumerate(rep["sort"]):
         for nb, cfg in c["pick"].items():
             bits, rest = cfg.split("x")

@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <dlfcn.h>
 #include <string>
 #include <vector>
 #include "board.h"
@@ -98,6 +99,9 @@ struct Engine {
 };
 static Engine E;
 
+static void tuning_load_default();
+static std::string g_tune_path;   /* the table installed at initialisation, if any */
+
 static int eng_init()
 {
     if (E.init) return 0;
@@ -113,6 +117,7 @@ static int eng_init()
                      hipHostMallocMapped | hipHostMallocCoherent));
     memset(E.h_pub, 0, 16 * sizeof(unsigned long long));
     HC(hipHostGetDevicePointer((void **)&E.hd_pub, E.h_pub, 0));
+    tuning_load_default();
     E.init = true;
     return 0;
 }
@@ -1094,11 +1099,12 @@ extern "C" int cg_tune_set(const char *knob, int value)
         g_spawn_est_mode = value;
         return 0;
     }
-    if (!strcmp(knob, "reset")) {
+    if (!strcmp(knob, "reset") || !strcmp(knob, "builtin")) {
         sort_table_reset();
         sort_force(0, 0, 0);
         set_dir_delta(0);
         g_spawn_est_mode = 0;
+        if (!strcmp(knob, "reset") && !g_tune_path.empty()) return cg_tune_load(g_tune_path.c_str());
         return 0;
     }
     return fail_msg("unknown tuning knob");
@@ -1113,6 +1119,94 @@ extern "C" int cg_tune_get(const char *knob)
     if (!strcmp(knob, "dir_delta")) return get_dir_delta();
     if (!strcmp(knob, "spawn_estimate")) return g_spawn_est_mode;
     return fail_msg("unknown tuning knob");
+}
+
+/* Reads the picks of an autotuner report (the JSON cg_autotune writes):
+ * the "pick" objects of the three "sort" classes ("<nbits>": "<bits>x<kpt>/
+ * <window>") and the "dir" section's "pick" (directory depth).  Anything
+ * malformed leaves the defaults in place and returns -1. */
+static int tuning_parse(const std::string &js)
+{
+    size_t dpos = js.find("\"dir\"");
+    if (dpos == std::string::npos) return fail_msg("tuning: no dir section");
+    /* position just after `"pick"` + spaces + ':' + spaces, or npos */
+    auto after_pick = [&](size_t from) -> size_t {
+        size_t q = js.find("\"pick\"", from);
+        if (q == std::string::npos) return q;
+        q += 6;
+        while (q < js.size() && (js[q] == ' ' || js[q] == ':' || js[q] == '\n')) q++;
+        return q;
+    };
+    size_t pos = 0;
+    std::vector<std::pair<int, SortCfg>> entries[SORT_SIZE_CLASSES];
+    for (int cls = 0; cls < SORT_SIZE_CLASSES; cls++) {
+        pos = after_pick(pos);
+        if (pos == std::string::npos || pos > dpos || js[pos] != '{')
+            return fail_msg("tuning: missing sort picks");
+        pos++;
+        size_t end = js.find('}', pos);
+        if (end == std::string::npos) return fail_msg("tuning: unterminated picks");
+        while (pos < end) {
+            int nb, b, k, w, used = 0;
+            if (sscanf(js.c_str() + pos, " \"%d\" : \"%dx%d/%d\"%n", &nb, &b, &k, &w, &used) != 4 || !used)
+                return fail_msg("tuning: malformed pick");
+            entries[cls].push_back({nb, SortCfg{b, k, w}});
+            pos += used;
+            while (pos < end && (js[pos] == ',' || js[pos] == ' ' || js[pos] == '\n')) pos++;
+        }
+    }
+    size_t ppos = after_pick(dpos);
+    int delta = 0;
+    if (ppos == std::string::npos || sscanf(js.c_str() + ppos, "%d", &delta) != 1 || delta < -4 ||
+        delta > 3)
+        return fail_msg("tuning: malformed dir pick");
+    sort_table_reset();
+    for (int cls = 0; cls < SORT_SIZE_CLASSES; cls++)
+        for (auto &e : entries[cls])
+            if (sort_table_set(cls, e.first, e.second)) {
+                sort_table_reset();
+                return fail_msg("tuning: a pick names no kernel");
+            }
+    set_dir_delta(delta);
+    return 0;
+}
+
+extern "C" int cg_tune_load(const char *path)
+{
+    FILE *f = fopen(path, "rb");
+    if (!f) return fail_msg("cg_tune_load: cannot open the file");
+    std::string js;
+    char buf[4096];
+    size_t r;
+    while ((r = fread(buf, 1, sizeof buf, f)) > 0) js.append(buf, r);
+    fclose(f);
+    return tuning_parse(js);
+}
+
+/* At initialisation the library installs the table the autotuner wrote on
+ * an MI355X, tuned/mi355x.json next to the library, unless CABLEGEN_TUNING
+ * names another file (an empty value keeps the built-in defaults). */
+static void tuning_load_default()
+{
+    const char *env = getenv("CABLEGEN_TUNING");
+    std::string path;
+    if (env) {
+        path = env;
+    } else {
+        Dl_info info;
+        if (!dladdr((void *)&tuning_load_default, &info) || !info.dli_fname) return;
+        path = info.dli_fname;
+        size_t slash = path.rfind('/');
+        path = (slash == std::string::npos ? std::string(".") : path.substr(0, slash)) + "/tuned/mi355x.json";
+    }
+    if (path.empty()) return;
+    FILE *f = fopen(path.c_str(), "rb");
+    if (!f) return;
+    fclose(f);
+    if (cg_tune_load(path.c_str()))
+        fprintf(stderr, "cablegen: ignoring tuning file %s (%s)\n", path.c_str(), g_err.c_str());
+    else
+        g_tune_path = path;
 }
 
 extern "C" int cg_tune_sort_entry(int size_class, int nbits, int bits, int kpt, int lbw)

@@ -153,8 +153,10 @@ int cg_dev_sync(void);
  *                layer, a step that overflows them is repeated with exact
  *                buffers; 1: sized for half the previous layer, so that
  *                growing layers take the repeat (for tests);
- *   "reset"      default table (8-bit digits, 8192-key tiles, window 16),
- *                no forced field, delta 0, spawn_estimate 0.
+ *   "builtin"    built-in defaults: 8-bit digits, 8192-key tiles, window 16
+ *                for every entry, no forced field, delta 0, spawn_estimate 0;
+ *   "reset"      the state after initialisation: the built-in defaults, then
+ *                the table loaded at initialisation if there was one.
  * Returns 0, or -1 for an unknown knob or value. */
 int cg_tune_set(const char *knob, int value);
 int cg_tune_get(const char *knob);
@@ -172,6 +174,12 @@ int cg_tune_sort_query(size_t n, int nbits, int *bits, int *kpt, int *lbw);
  * receives the measurements and picks as JSON.  On failure the defaults are
  * restored. */
 int cg_autotune(int effort, char *report, size_t cap);
+/* Installs the picks of a saved autotuner report (JSON as cg_autotune
+ * writes it).  The library does this at initialisation with tuned/
+ * mi355x.json next to libcablegen.so, or with the file the environment
+ * variable CABLEGEN_TUNING names (an empty value keeps the built-in
+ * defaults).  Returns 0, or -1 for an unreadable or malformed file. */
+int cg_tune_load(const char *path);
 
 /* ---------------- kernel timers ---------------- */
 

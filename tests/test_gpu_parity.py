@@ -195,6 +195,20 @@ def test_lookup_directory_depths(cg, delta):
         cg.tune_set("reset", 0)
 
 
+def test_stored_table_installed_at_init(cg):
+    """Initialisation installs cablegen_amd/tuned/mi355x.json, so every sort in
+    these tests runs with the configurations the bench uses."""
+    import json
+    if not os.path.exists(cg.TUNED_TABLE) or os.environ.get("CABLEGEN_TUNING") is not None:
+        pytest.skip("no stored table, or overridden")
+    assert cg.lib().cg_dev_sync() == 0   # initialises the engine
+    rep = json.load(open(cg.TUNED_TABLE))
+    for cls, n in enumerate((1 << 18, 1 << 21, 1 << 25)):
+        for nb in (24, 40, 48, 54):
+            assert rep["sort"][cls]["pick"][str(nb)] == "%dx%d/%d" % cg.tune_sort_query(n, nb)
+    assert cg.tune_get("dir_delta") == rep["dir"]["pick"]
+
+
 def test_autotune(cg):
     """The autotuner fills the table with kernels that exist and sorts stay exact."""
     try:

@@ -162,3 +162,42 @@ def test_arrays(cg):
     s = L.shrink_darr(C.byref(d))
     assert s.valid and s.size == 1000
     assert np.array_equal(np.ctypeslib.as_array(s.bp, shape=(1000,)), np.arange(1000))
+
+
+def test_stored_tuning_table_installs(cg):
+    """The autotuner's stored MI355X table names only configurations that
+    have kernels, installs through cg_tune_sort_entry and is what the sorts
+    then choose (table lookups only; nothing runs on a device)."""
+    import json
+    if not os.path.exists(cg.TUNED_TABLE):
+        pytest.skip("no stored table")
+    rep = json.load(open(cg.TUNED_TABLE))
+    kernels = {"8x32/16", "8x32/32", "8x16/16", "8x16/32", "8x8/16", "8x8/32", "9x16/16", "9x16/32",
+               "9x32/16", "9x32/32", "9x7/16", "9x7/32", "10x8/16", "10x8/32"}
+    assert len(rep["sort"]) == 3
+    for c in rep["sort"]:
+        assert set(c["pick"]) == {str(b) for b in range(1, 65)}
+        assert set(c["pick"].values()) <= kernels
+    try:
+        cg.apply_tuning(rep)
+        for cls, n in enumerate((1 << 18, 1 << 21, 1 << 25)):
+            for nb in (1, 24, 40, 54, 64):
+                b, k, w = cg.tune_sort_query(n, nb)
+                assert rep["sort"][cls]["pick"][str(nb)] == "%dx%d/%d" % (b, k, w)
+        assert cg.tune_get("dir_delta") == rep["dir"]["pick"]
+        # forcing a field overrides the table; reset restores the defaults
+        cg.tune_set("sort_bits", 10)
+        assert cg.tune_sort_query(1 << 25, 40)[0] == 10
+        cg.tune_set("builtin", 0)
+        assert cg.tune_sort_query(1 << 25, 40) == (8, 32, 16)
+        with pytest.raises(cg.CablegenError):
+            cg.tune_set("sort_bits", 7)
+        # the C loader reads the same file and "reset" returns to it
+        assert cg.lib().cg_tune_load(cg.TUNED_TABLE.encode()) == 0
+        for cls, n in enumerate((1 << 18, 1 << 21, 1 << 25)):
+            b, k, w = cg.tune_sort_query(n, 48)
+            assert rep["sort"][cls]["pick"]["48"] == "%dx%d/%d" % (b, k, w)
+        assert cg.lib().cg_tune_load(os.path.join(ROOT, "bench.py").encode()) != 0
+    finally:
+        cg.tune_set("reset", 0)
+
