@@ -518,7 +518,8 @@ static int gen_step(const cg_rules &r, const uint64_t *P, size_t nP, const uint6
     if (ensure(E.A, 4 * nP + 1)) return -1;
     launch_init_stats(E.st, E.d_cnt);
     int h = prof_begin(PK_MOVE);
-    launch_move(E.st, P, nP, pc.pending ? E.d_cnt + 5 : nullptr, E.A.p, E.d_cnt, lut, move_rules(r));
+    launch_move(E.st, P, nP, pc.pending ? E.d_cnt + 5 : nullptr, E.A.p, E.d_cnt, lut, move_rules(r),
+                1);
     prof_end(h, 0);
     /* the previous step's carry sort (second stream) overlapped the moves;
      * its count and output are needed from here on */

@@ -84,9 +84,12 @@ size_t sort_scratch_bytes(size_t n);
 /* move + lock + prune + canonicalise; appends valid canonical successors to
  * out (capacity 4n), adds their number to stats[0] and ORs / ANDs them into
  * stats[1] / stats[2].  With dn, the number of boards is min(n, *dn) read on
- * the device (n bounds the grid); likewise for launch_spawn. */
+ * the device (n bounds the grid); likewise for launch_spawn.  With dedupe, a
+ * candidate produced twice within a 1024-board chunk is appended once (the
+ * set of candidates is unchanged). */
 void launch_move(hipStream_t st, const uint64_t *in, size_t n, const unsigned long long *dn,
-                 uint64_t *out, unsigned long long *stats, const uint16_t *lut, MoveRules r);
+                 uint64_t *out, unsigned long long *stats, const uint16_t *lut, MoveRules r,
+                 int dedupe = 0);
 
 /* sum of empty cells over a layer into *count (device counter) */
 void launch_count_empties(hipStream_t st, const uint64_t *in, size_t n, unsigned long long *count);

@@ -47,28 +47,56 @@ hipError_t kernels_take_error()
 /* ------------------------------------------------------------------------ */
 
 constexpr int MV_THREADS = 1024;
+constexpr int MV_HASH = 8192;   /* hash slots per 1024-board chunk (<= 4096 candidates) */
 
-__global__ __launch_bounds__(MV_THREADS) void k_move(const uint64_t *__restrict__ in, size_t n,
-                                                     const unsigned long long *dn,
-                                                     uint64_t *__restrict__ out,
-                                                     unsigned long long *stats,
-                                                     const uint16_t *__restrict__ glut,
-                                                     MoveRules r)
+/* First occurrence of c among the candidates the block produced in this
+ * chunk: open addressing in LDS with a 64-bit compare-and-swap; 0 marks an
+ * empty slot (a moved board always holds a tile).  Bounded probing. */
+__device__ __forceinline__ bool mv_first(unsigned long long *ht, uint64_t c)
 {
-    __shared__ uint16_t lut[65536];
+    uint32_t sl = (uint32_t)((c * 0x9E3779B97F4A7C15ULL) >> (64 - 13));
+    for (int probe = 0; probe < MV_HASH; probe++) {
+        unsigned long long old = atomicCAS(&ht[sl], 0ULL, (unsigned long long)c);
+        if (old == 0ULL) return true;
+        if (old == c) return false;
+        sl = (sl + 1) & (MV_HASH - 1);
+    }
+    return true;
+}
+
+/* DEDUPE (the generation step): the row table is read through the cache and
+ * LDS holds a hash set per 1024-board chunk, so candidates a chunk produces
+ * twice are appended once (70% of move candidates are duplicates, ~40%
+ * within a chunk); the sort that follows sees fewer keys and the layer is
+ * unchanged.  Otherwise (the move primitive) every candidate is appended and
+ * the row table is staged in LDS. */
+template <bool DEDUPE>
+__global__ __launch_bounds__(MV_THREADS, DEDUPE ? 8 : 1) void k_move(
+    const uint64_t *__restrict__ in, size_t n, const unsigned long long *dn,
+    uint64_t *__restrict__ out, unsigned long long *stats, const uint16_t *__restrict__ glut,
+    MoveRules r)
+{
+    /* LDS: the row table (128 KiB) or the hash set (64 KiB) */
+    __shared__ unsigned long long smem[DEDUPE ? MV_HASH : 65536 / 4];
     __shared__ uint32_t wtmp[17];
     __shared__ unsigned long long base_sh;
     __shared__ uint64_t ro[16], ra[16];
     uint64_t kor = 0, kand = ~0ULL;
-    {
+    const uint16_t *lut = glut;
+    if (!DEDUPE) {
         const uint4 *g = reinterpret_cast<const uint4 *>(glut);
-        uint4 *l = reinterpret_cast<uint4 *>(lut);
+        uint4 *l = reinterpret_cast<uint4 *>(smem);
         for (int i = threadIdx.x; i < 65536 / 8; i += MV_THREADS) l[i] = g[i];
+        lut = reinterpret_cast<const uint16_t *>(smem);
+        __syncthreads();
     }
     if (dn && *dn < n) n = (size_t)*dn;   /* count produced on the device */
-    __syncthreads();
     const size_t stride = (size_t)gridDim.x * MV_THREADS;
     for (size_t b0 = (size_t)blockIdx.x * MV_THREADS; b0 < n; b0 += stride) {
+        if (DEDUPE) {
+            for (int i = threadIdx.x; i < MV_HASH; i += MV_THREADS) smem[i] = 0ULL;
+            __syncthreads();
+        }
         size_t i = b0 + threadIdx.x;
         uint64_t res[4];
         uint32_t k = 0;
@@ -85,26 +113,42 @@ __global__ __launch_bounds__(MV_THREADS) void k_move(const uint64_t *__restrict_
                 }
             }
         }
-        for (uint32_t j = 0; j < k; j++) kor |= res[j], kand &= res[j];
+        uint32_t keep = (1u << k) - 1u;
+        if (DEDUPE) {
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                if (j < (int)k && !mv_first(smem, res[j])) keep &= ~(1u << j);
+        }
+        uint32_t kk = (uint32_t)__builtin_popcount(keep);
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            if ((keep >> j) & 1u) kor |= res[j], kand &= res[j];
         uint32_t tot;
-        uint32_t off = block_excl_scan<uint32_t>(k, &tot, wtmp);
+        uint32_t off = block_excl_scan<uint32_t>(kk, &tot, wtmp);
         if (threadIdx.x == 0) base_sh = tot ? atomicAdd(&stats[0], (unsigned long long)tot) : 0;
         __syncthreads();
-        unsigned long long base = base_sh;
-        for (uint32_t j = 0; j < k; j++) out[base + off + j] = res[j];
+        unsigned long long o = base_sh + off;
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            if ((keep >> j) & 1u) out[o++] = res[j];
         __syncthreads();
     }
     block_or_and(kor, kand, &stats[1], &stats[2], ro, ra);
 }
 
 void launch_move(hipStream_t st, const uint64_t *in, size_t n, const unsigned long long *dn,
-                 uint64_t *out, unsigned long long *stats, const uint16_t *lut, MoveRules r)
+                 uint64_t *out, unsigned long long *stats, const uint16_t *lut, MoveRules r,
+                 int dedupe)
 {
     if (!n) return;
     size_t blocks = (n + MV_THREADS - 1) / MV_THREADS;
     if (blocks > 512) blocks = 512;
-    hipLaunchKernelGGL(k_move, dim3((unsigned)blocks), dim3(MV_THREADS), 0, st, in, n, dn, out,
-                       stats, lut, r);
+    if (dedupe)
+        hipLaunchKernelGGL(k_move<true>, dim3((unsigned)blocks), dim3(MV_THREADS), 0, st, in, n, dn,
+                           out, stats, lut, r);
+    else
+        hipLaunchKernelGGL(k_move<false>, dim3((unsigned)blocks), dim3(MV_THREADS), 0, st, in, n, dn,
+                           out, stats, lut, r);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -840,28 +884,26 @@ __device__ __forceinline__ uint64_t bdec(uint64_t ck, const BatchTable &T)
     return b;
 }
 
-__global__ __launch_bounds__(MV_THREADS) void k_bmove(const uint64_t *__restrict__ in, size_t n,
-                                                      const unsigned long long *dn,
-                                                      uint64_t *__restrict__ out,
-                                                      unsigned long long *stats,
-                                                      const uint16_t *__restrict__ glut, MoveRules r,
-                                                      const BatchTable *__restrict__ tabs, int step,
-                                                      unsigned *err)
+/* the batched move: decode, move, canonicalise, re-encode; duplicates within
+ * a 1024-key chunk are appended once (as k_move<true>) */
+__global__ __launch_bounds__(MV_THREADS, 8) void k_bmove(const uint64_t *__restrict__ in, size_t n,
+                                                         const unsigned long long *dn,
+                                                         uint64_t *__restrict__ out,
+                                                         unsigned long long *stats,
+                                                         const uint16_t *__restrict__ lut, MoveRules r,
+                                                         const BatchTable *__restrict__ tabs, int step,
+                                                         unsigned *err)
 {
-    __shared__ uint16_t lut[65536];
+    __shared__ unsigned long long ht[MV_HASH];
     __shared__ uint32_t wtmp[17];
     __shared__ unsigned long long base_sh;
     __shared__ uint64_t ro[16], ra[16];
     uint64_t kor = 0, kand = ~0ULL;
-    {
-        const uint4 *g = reinterpret_cast<const uint4 *>(glut);
-        uint4 *l = reinterpret_cast<uint4 *>(lut);
-        for (int i = threadIdx.x; i < 65536 / 8; i += MV_THREADS) l[i] = g[i];
-    }
     if (dn && *dn < n) n = (size_t)*dn;
-    __syncthreads();
     const size_t stride = (size_t)gridDim.x * MV_THREADS;
     for (size_t b0 = (size_t)blockIdx.x * MV_THREADS; b0 < n; b0 += stride) {
+        for (int i = threadIdx.x; i < MV_HASH; i += MV_THREADS) ht[i] = 0ULL;
+        __syncthreads();
         size_t i = b0 + threadIdx.x;
         uint64_t res[4];
         uint32_t k = 0;
@@ -883,13 +925,22 @@ __global__ __launch_bounds__(MV_THREADS) void k_bmove(const uint64_t *__restrict
                 }
             }
         }
-        for (uint32_t j = 0; j < k; j++) kor |= res[j], kand &= res[j];
+        uint32_t keep = (1u << k) - 1u;
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            if (j < (int)k && !mv_first(ht, res[j])) keep &= ~(1u << j);
+        uint32_t kk = (uint32_t)__builtin_popcount(keep);
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            if ((keep >> j) & 1u) kor |= res[j], kand &= res[j];
         uint32_t tot;
-        uint32_t off = block_excl_scan<uint32_t>(k, &tot, wtmp);
+        uint32_t off = block_excl_scan<uint32_t>(kk, &tot, wtmp);
         if (threadIdx.x == 0) base_sh = tot ? atomicAdd(&stats[0], (unsigned long long)tot) : 0;
         __syncthreads();
-        unsigned long long base = base_sh;
-        for (uint32_t j = 0; j < k; j++) out[base + off + j] = res[j];
+        unsigned long long o = base_sh + off;
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            if ((keep >> j) & 1u) out[o++] = res[j];
         __syncthreads();
     }
     block_or_and(kor, kand, &stats[1], &stats[2], ro, ra);
