@@ -188,8 +188,9 @@ def micro(args):
     print(json.dumps(out))
 
 
-PMC_KERNELS = {"move": "k_move(", "spawn": "k_spawn(", "sort_hist": "k_os_hist", "sort_pass": "k_os_pass",
-               "unique": "k_os_unique", "solve": "k_solve(", "lookup": "k_lookup(", "adjoint": "k_adjoint("}
+PMC_KERNELS = {"move": "::k_move", "spawn": "::k_spawn(", "sort_hist": "::k_os_hist",
+               "sort_pass": "::k_os_pass", "unique": "::k_os_unique", "solve": "::k_solve(",
+               "lookup": "::k_lookup(", "adjoint": "::k_adjoint("}
 
 
 def _csvs(d, suffix):
