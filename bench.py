@@ -284,7 +284,9 @@ def run_autotune(cg, args):
     t0 = time.perf_counter()
     if mode == "off":
         cg.tune_set("builtin", 0)
-        return {"source": "built-in defaults: 8-bit digits, 8192-key tiles, window 16, dir delta 0"}
+        cg.tune_set("cell_code", args.cell_code)
+        return {"source": "built-in defaults: 8-bit digits, 8192-key tiles, window 16, dir delta 0",
+                "cell_code": args.cell_code}
     if mode == "stored" and os.path.exists(cg.TUNED_TABLE):
         rep = cg.load_tuning()
         src = "stored autotuner table cablegen_amd/tuned/mi355x.json"
@@ -294,11 +296,12 @@ def run_autotune(cg, args):
     if args.save_tuning:
         with open(args.save_tuning, "w") as f:
             json.dump(rep, f)
+    cg.tune_set("cell_code", args.cell_code)
     sel = {}
     for cls, name in enumerate(("n<2^20", "n<2^23", "n>=2^23")):
         sel[name] = {nb: rep["sort"][cls]["pick"][nb] for nb in ("24", "32", "36", "40", "44", "48", "54")}
     return {"source": src, "seconds": time.perf_counter() - t0, "dir_delta": rep["dir"]["pick"],
-            "sort_picks_bits_x_kpt/window": sel}
+            "sort_picks_bits_x_kpt/window": sel, "cell_code": args.cell_code}
 
 
 def batch_bench(args):
@@ -335,12 +338,15 @@ def batch_bench(args):
 
     one_at_a_time()
     batched()
+    c0 = cg.tune_get("cell_code_sorts")
     n1, t_one = one_at_a_time()
+    c1 = cg.tune_get("cell_code_sorts")
     cg.prof_enable(True)
     cg.prof_reset()
     nb, tg, ts = batched()
     kern = {k: cg.prof_query(k) for k in ("move", "spawn", "sort_hist", "sort_pass", "unique")}
     cg.prof_enable(False)
+    c2 = cg.tune_get("cell_code_sorts")
     assert n1 == nb
     sizes = sorted((B.total_boards_table(t) for t in range(len(cfgs))), reverse=True)
     out = {"metric": "batched tablebases: boards/s, batched vs one at a time",
@@ -350,6 +356,8 @@ def batch_bench(args):
            "speedup": t_one / (tg + ts),
            "largest_tables": sizes[:5], "median_table": sizes[len(sizes) // 2],
            "batched_kernels_ms": {k: v[1] for k, v in kern.items()},
+           "batched_sorts": kern["sort_hist"][0], "batched_sort_passes": kern["sort_pass"][0],
+           "cell_coded_sorts": {"one_at_a_time": c1 - c0, "batched": c2 - c1},
            "tuning": tuning,
            "workload": "seeds 0..%d: formation in {2x4, 3x4, DPDGap}, 2-4 tiles of 2^1..2^7, "
                        "goal 2^(max+2) top-left, end = start + 2*U{%d..%d}, ignore_f, no prune"
@@ -382,6 +390,8 @@ def main():
     ap.add_argument("--tuning", choices=("stored", "auto", "off"), default="stored",
                     help="sort/directory tuning: stored autotuner table, run the autotuner, or defaults")
     ap.add_argument("--save-tuning", default=None, help="write the autotuner report (JSON) here")
+    ap.add_argument("--cell-code", type=int, choices=(0, 1), default=1,
+                    help="1: layer sorts code pairs of cells when the tile sum bounds them (default)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "

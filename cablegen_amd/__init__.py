@@ -83,6 +83,10 @@ def lib():
     L.cg_dev_spawn.argtypes = [C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p,
                                C.POINTER(C.c_size_t)]
     L.cg_dev_sort_unique.argtypes = [C.c_void_p, C.c_size_t, C.c_void_p, C.POINTER(C.c_size_t)]
+    L.cg_dev_sort_unique_bounded.argtypes = [C.c_void_p, C.c_size_t, C.c_void_p,
+                                             C.POINTER(C.c_size_t), C.c_uint64]
+    L.cg_sort_code_probe.argtypes = [C.c_void_p, C.c_size_t, C.c_uint64, C.c_uint64, C.c_void_p,
+                                     C.c_void_p]
     L.cg_dev_lookup.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t,
                                 C.c_void_p, C.c_int]
     L.cg_dev_solve_layer.argtypes = [C.POINTER(Rules), C.c_void_p, C.c_size_t, C.c_void_p,
@@ -131,6 +135,11 @@ def lib():
     L.cge_build_row_table.argtypes = [C.c_int, C.c_void_p]
     _lib = L
     return L
+
+
+def last_error():
+    """the library's message for its last failed call"""
+    return lib().cg_last_error().decode()
 
 
 def _check(rc, what):

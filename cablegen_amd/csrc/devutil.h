@@ -98,5 +98,55 @@ __host__ __device__ __forceinline__ uint64_t xform_inv(uint64_t c, const KeyXfor
     return b;
 }
 
+__host__ __device__ __forceinline__ uint32_t sc_cell(uint32_t hi, uint32_t lo, int at)
+{
+    return ((at >= 32 ? hi : lo) >> (at & 31)) & 15u;
+}
+
+/* the sort code of key k (SortCode); bad becomes nonzero when a cell of the
+ * cell-pair code is outside its base */
+__host__ __device__ __forceinline__ uint64_t sc_encode(uint64_t k, const SortCode &c, uint32_t &bad)
+{
+    if (!c.cells) return xform(k, c.x);
+    const uint32_t hi = (uint32_t)(k >> 32), lo = (uint32_t)k;
+    uint64_t v = 0;
+#pragma unroll
+    for (int p = 0; p < 8; p++) {
+        if (p < c.npair) {
+            const uint32_t ba = c.cbase[2 * p], bb = c.cbase[2 * p + 1];
+            uint32_t a = sc_cell(hi, lo, c.cell_lo[2 * p]), b = sc_cell(hi, lo, c.cell_lo[2 * p + 1]);
+            bad |= (uint32_t)(a >= ba) | (uint32_t)(b >= bb);
+            v = (v << c.pbits[p]) | (a * bb + b);
+        }
+    }
+    if (c.ncell & 1) {
+        uint32_t a = sc_cell(hi, lo, c.cell_lo[c.ncell - 1]);
+        bad |= (uint32_t)(a >= c.cbase[c.ncell - 1]);
+        v = (v << c.lbits) | a;
+    }
+    return v;
+}
+
+/* inverse of sc_encode on the covered bits (the other bits are zero) */
+__host__ __device__ __forceinline__ uint64_t sc_decode(uint64_t v, const SortCode &c)
+{
+    if (!c.cells) return xform_inv(v, c.x);
+    uint64_t r = 0;
+    if (c.ncell & 1) {
+        r = (v & ((1ULL << c.lbits) - 1)) << c.cell_lo[c.ncell - 1];
+        v >>= c.lbits;
+    }
+#pragma unroll
+    for (int p = 7; p >= 0; p--) {
+        if (p < c.npair) {
+            uint32_t f = (uint32_t)v & ((1u << c.pbits[p]) - 1);
+            v >>= c.pbits[p];
+            uint32_t a = (f * c.magic[p]) >> 16, b = f - a * (uint32_t)c.cbase[2 * p + 1];
+            r |= ((uint64_t)a << c.cell_lo[2 * p]) | ((uint64_t)b << c.cell_lo[2 * p + 1]);
+        }
+    }
+    return r;
+}
+
 } // namespace cg
 #endif

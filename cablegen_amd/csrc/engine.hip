@@ -86,9 +86,9 @@ struct Engine {
     Buf A, B, Cb;                                 /* candidate / sort buffers */
     void *scratch = nullptr;
     size_t scratch_cap = 0;
-    /* a second stream for the carry sort, which runs beside the P_{s+2}
-     * sort: its own ping-pong buffer and scratch; ev2 orders the main
-     * stream after it */
+    /* a second stream for the carry sort, which runs beside the next
+     * step's moves: its own ping-pong buffer and scratch; ev2 orders the
+     * main stream after it */
     hipStream_t st2 = nullptr;
     hipEvent_t ev2 = nullptr;
     Buf B2;
@@ -408,6 +408,7 @@ static int read_counters(int lo, int n)
         hipMemsetAsync(E.d_err, 0, sizeof(unsigned), E.st);
         hipStreamSynchronize(E.st);
         if (e & 2u) return fail_msg("batch: a board's formation cells left the table's canonical place");
+        if (e & SORT_ERR_CODE) return fail_msg("sort: a key's cell value exceeds the bound of its layer");
         return fail_msg("sort: a chained-scan wait exceeded its budget");
     }
     return 0;
@@ -420,14 +421,16 @@ static uint64_t vary_of(size_t n, uint64_t o, uint64_t a) { return n > 1 ? (o & 
  * buffer, scratch and sort context) -- the caller orders the main stream
  * after it (join_lane1) before using its output */
 static int sort_enqueue(uint64_t *buf, size_t n, uint64_t vary, uint64_t *dst, int slot, int *hu,
-                        int lane = 0)
+                        int lane = 0, uint64_t cell_max = CELLS_UNBOUNDED)
 {
     if (lane == 1) {
         if (ensure(E.B2, n + 1) || ensure_scratch2(n)) return -1;
-        sort_unique(E.st2, buf, E.B2.p, n, vary, E.scratch2, dst, E.d_cnt + slot, E.d_err, hu, 1);
+        sort_unique(E.st2, buf, E.B2.p, n, vary, E.scratch2, dst, E.d_cnt + slot, E.d_err, hu, 1,
+                    cell_max);
     } else {
         if (ensure(E.B, n + 1) || ensure_scratch(n)) return -1;
-        sort_unique(E.st, buf, E.B.p, n, vary, E.scratch, dst, E.d_cnt + slot, E.d_err, hu, 0);
+        sort_unique(E.st, buf, E.B.p, n, vary, E.scratch, dst, E.d_cnt + slot, E.d_err, hu, 0,
+                    cell_max);
     }
     hipError_t e = kernels_take_error();
     if (e != hipSuccess) return fail("sort_unique", e);
@@ -449,11 +452,6 @@ struct LayerAlloc {
     virtual ~LayerAlloc() {}
 };
 
-/* OR / AND of the carried spawn-4 set (same set, same statistics as S4) */
-struct Carry {
-    uint64_t o = 0, a = ~0ULL;
-};
-
 static int nonzero_nibbles(uint64_t x)
 {
     int c = 0;
@@ -461,14 +459,13 @@ static int nonzero_nibbles(uint64_t x)
     return c;
 }
 
-/* Counts of P_{s+2} and C' that are still on the device (d_cnt[5], d_cnt[6])
- * when a step returns: the next step's move kernel reads the P count there,
- * and its first synchronisation brings both to the host. */
+/* The count of the carry C' is still on the device (d_cnt[6]) when a step
+ * returns; the next step's first synchronisation brings it to the host. */
 struct StepCounts {
     bool pending = false;
-    size_t n2 = 0, sumE = 0;   /* input sizes of the two sorts (upper bounds of the counts) */
-    int hu2 = -1, hu4 = -1;    /* timer handles of the two compactions, bytes set on arrival */
-    size_t last_nM = 0;        /* previous layer size: sizes the spawn buffers */
+    size_t sumE = 0;         /* input size of the carry sort (upper bound of the count) */
+    int hu4 = -1;            /* timer handle of its compaction, bytes set on arrival */
+    size_t last_nM = 0;      /* previous layer size: sizes the spawn buffers */
 };
 
 /* spawn buffers are sized before |M_s| is known: for about twice the previous
@@ -482,53 +479,106 @@ static size_t spawn_rows_estimate(const StepCounts &pc, size_t ncand)
     return std::min(ncand, 2 * pc.last_nM + 65536);
 }
 
-static void counts_arrived(StepCounts &pc, size_t nP, size_t nC)
+static void counts_arrived(StepCounts &pc, size_t nC)
 {
-    prof_set_bytes(pc.hu2, 8.0 * pc.n2 + 8.0 * nP);
     prof_set_bytes(pc.hu4, 8.0 * pc.sumE + 8.0 * nC);
     pc.pending = false;
 }
 
-/* brings pending counts to the host (end of a generation) */
-static int counts_finish(StepCounts &pc, size_t *nP, size_t *nC)
+/* brings the pending count to the host (end of a generation) */
+static int counts_finish(StepCounts &pc)
 {
     if (!pc.pending) return 0;
-    if (join_lane1() || read_counters(5, 2)) return -1;
-    if (nP) *nP = E.h_cnt[5];
-    if (nC) *nC = E.h_cnt[6];
-    counts_arrived(pc, E.h_cnt[5], E.h_cnt[6]);
+    if (join_lane1() || read_counters(6, 1)) return -1;
+    counts_arrived(pc, E.h_cnt[6]);
     return 0;
 }
 
-/* One layer step (generate.c:409-550): P (pre-move layer) and the carried
- * spawn-4 set C give M_s (from `la`), P_{s+2} and the new carry (written to
- * nxtP / nxtC).  Two host synchronisations: after the moves (which also
- * brings the previous step's pending counts), and after the spawns (which
- * brings |M_s|).  When pc.pending, nP and nC are upper bounds and the exact
- * counts are on the device; on return *nP2 / *nC2 are upper bounds and
- * pc.pending is set. */
-static int gen_step(const cg_rules &r, const uint64_t *P, size_t nP, const uint64_t *C, size_t nC,
-                    StepCounts &pc, Carry &carry, LayerAlloc &la, uint64_t **M, size_t *nM,
-                    Buf &nxtP, size_t *nP2, Buf &nxtC, size_t *nC2, uint64_t *m_or = nullptr,
-                    uint64_t *m_and = nullptr)
+/* Bound on the cell values of a generation's keys, for the sort's cell-pair
+ * code.  Moves keep a board's sum of non-0xf tiles (0xf and 0xe cells never
+ * merge) and a spawn adds a 2 or a 4, so when every initial board has the
+ * same such sum, every board of the step that starts from sum s has sum s
+ * and no tile above 2^floor(log2 s) outside its 0xf cells.  0xf cells are
+ * never created or removed (spawns put 1 or 2), so every board keeps the
+ * initial boards' count nf of them: when the AND of a key set has nf 0xf
+ * cells, those are the 0xf cells of every key and the other cells are at
+ * most floor(log2 s). */
+struct ValueBound {
+    bool ok = false;
+    int nf = 0;
+    long s = 0;   /* tile sum of the boards the next step starts from */
+};
+
+static ValueBound value_bound_of(const uint64_t *b, size_t n)
 {
+    ValueBound v;
+    for (size_t i = 0; i < n; i++) {
+        long sum = 0;
+        int nf = 0;
+        for (int c = 0; c < 16; c++) {
+            unsigned t = (unsigned)(b[i] >> (4 * c)) & 15u;
+            if (t == 15) nf++;
+            else if (t) sum += 1L << t;
+        }
+        if (i == 0) {
+            v.s = sum;
+            v.nf = nf;
+        } else if (sum != v.s || nf != v.nf) {
+            return ValueBound();
+        }
+    }
+    v.ok = n > 0;
+    return v;
+}
+
+/* floor(log2 s): the largest tile of a board of tile sum s (s >= 1) */
+static int tile_log2(long s) { return 63 - __builtin_clzl((unsigned long)s); }
+
+/* largest values of the varying cells in a key set of tile sum s whose AND
+ * is key_and (as a cell_max word), or CELLS_UNBOUNDED */
+static uint64_t cell_bound(const ValueBound *v, long s, uint64_t key_and)
+{
+    if (!v || !v->ok || s < 1) return CELLS_UNBOUNDED;
+    int nf = 0;
+    for (int c = 0; c < 16; c++) nf += ((key_and >> (4 * c)) & 15u) == 15u;
+    if (nf != v->nf) return CELLS_UNBOUNDED;
+    int lg = tile_log2(s);
+    return lg <= 14 ? cells_bounded_by(lg) : CELLS_UNBOUNDED;
+}
+
+/* One layer step (generate.c:409-550): the pre-move boards P and the carried
+ * spawn-4 set C give M_s = uniq(moves(P)) (from `la`), the next pre-move
+ * boards P_{s+2} = C u S2 (written to nxtP) and the new carry C' = uniq(S4)
+ * (written to nxtC).  P is a multiset: the moves of a multiset have the same
+ * set of results, and M is sorted and deduplicated anyway, so P_{s+2} needs
+ * no sort of its own (its duplicates, about 15% on DPDGap, cost less in the
+ * move kernel and the M sort than sorting all of it would).  Two host
+ * synchronisations: after the moves (which also brings the previous step's
+ * carry count), and after the spawns (which brings |M_s|).  When pc.pending,
+ * nC is an upper bound and the exact count is on the device; on return *nC2
+ * is an upper bound, pc.pending is set and *nP2 is exact.  vb (optional)
+ * bounds the cell values for the sorts and advances to the next step's sum. */
+static int gen_step(const cg_rules &r, const uint64_t *P, size_t nP, const uint64_t *C, size_t nC,
+                    StepCounts &pc, LayerAlloc &la, uint64_t **M, size_t *nM,
+                    Buf &nxtP, size_t *nP2, Buf &nxtC, size_t *nC2, ValueBound *vb,
+                    uint64_t *m_or = nullptr, uint64_t *m_and = nullptr)
+{
+    const long s = vb ? vb->s : 0;
     const uint16_t *lut = get_lut(r.ignore_f);
     if (!lut) return fail_msg("row table upload failed");
     /* moves */
     if (ensure(E.A, 4 * nP + 1)) return -1;
     launch_init_stats(E.st, E.d_cnt);
     int h = prof_begin(PK_MOVE);
-    launch_move(E.st, P, nP, pc.pending ? E.d_cnt + 5 : nullptr, E.A.p, E.d_cnt, lut, move_rules(r),
-                1);
+    launch_move(E.st, P, nP, nullptr, E.A.p, E.d_cnt, lut, move_rules(r), 1);
     prof_end(h, 0);
     /* the previous step's carry sort (second stream) overlapped the moves;
      * its count and output are needed from here on */
     if (pc.pending && join_lane1()) return -1;
     if (read_counters(0, pc.pending ? 7 : 3)) return -1;
     if (pc.pending) {
-        nP = E.h_cnt[5];
         nC = E.h_cnt[6];
-        counts_arrived(pc, nP, nC);
+        counts_arrived(pc, nC);
     }
     size_t ncand = E.h_cnt[0];
     uint64_t mor = E.h_cnt[1], mand = E.h_cnt[2];
@@ -539,7 +589,8 @@ static int gen_step(const cg_rules &r, const uint64_t *P, size_t nP, const uint6
     *M = la.alloc(ncand);
     if (!*M) return fail_msg("layer allocation failed");
     int hu;
-    if (sort_enqueue(E.A.p, ncand, vary_of(ncand, mor, mand), *M, 7, &hu)) return -1;
+    if (sort_enqueue(E.A.p, ncand, vary_of(ncand, mor, mand), *M, 7, &hu, 0, cell_bound(vb, s, mand)))
+        return -1;
     /* spawns: every board has at most 16 - (cells nonzero in all of M_s)
      * empties; the kernel reads |M_s| from the device, the grid and the
      * buffers cover ncand */
@@ -548,11 +599,11 @@ static int gen_step(const cg_rules &r, const uint64_t *P, size_t nP, const uint6
     int hs = -1;
     size_t sumE = 0;
     for (int attempt = 0; attempt < 2; attempt++) {
-        if (ensure(E.A, nC + cap + 1) || ensure(E.Cb, cap + 1)) return -1;
-        if (nC) HC(hipMemcpyAsync(E.A.p, C, nC * sizeof(uint64_t), hipMemcpyDeviceToDevice, E.st));
+        if (ensure(nxtP, nC + cap + 1) || ensure(E.Cb, cap + 1)) return -1;
+        if (nC) HC(hipMemcpyAsync(nxtP.p, C, nC * sizeof(uint64_t), hipMemcpyDeviceToDevice, E.st));
         launch_init_stats(E.st, E.d_cnt);
         hs = prof_begin(PK_SPAWN);
-        launch_spawn(E.st, *M, ncand, E.d_cnt + 7, E.A.p + nC, E.Cb.p, cap, E.d_cnt);
+        launch_spawn(E.st, *M, ncand, E.d_cnt + 7, nxtP.p + nC, E.Cb.p, cap, E.d_cnt);
         prof_end(hs, 0);
         if (read_counters(0, 8)) return -1;
         *nM = E.h_cnt[7];
@@ -564,25 +615,21 @@ static int gen_step(const cg_rules &r, const uint64_t *P, size_t nP, const uint6
     prof_set_bytes(hu, 8.0 * ncand + 8.0 * *nM);
     prof_set_bytes(hs, 8.0 * *nM + 16.0 * sumE);
     la.shrink(*M, ncand, *nM);
-    uint64_t or2 = E.h_cnt[1], and2 = E.h_cnt[2], or4 = E.h_cnt[3], and4 = E.h_cnt[4];
-    /* P_{s+2} = uniq(C u S2), C' = uniq(S4); counts stay on the device */
-    size_t n2 = nC + sumE;
-    if (ensure(nxtP, n2 + 1) || ensure(nxtC, sumE + 1)) return -1;
-    int hu2, hu4;
-    /* the two sorts are independent: C' runs on the second stream beside
-     * P_{s+2} (the spawns finished at the synchronisation above) */
-    if (sort_enqueue(E.Cb.p, sumE, vary_of(sumE, or4, and4), nxtC.p, 6, &hu4, 1)) return -1;
-    if (sort_enqueue(E.A.p, n2, vary_of(n2, or2 | carry.o, and2 & carry.a), nxtP.p, 5, &hu2))
+    uint64_t or4 = E.h_cnt[3], and4 = E.h_cnt[4];
+    /* C' = uniq(S4) on the second stream (the spawns finished at the
+     * synchronisation above); it overlaps the next step's moves, and its
+     * count stays on the device until then */
+    if (ensure(nxtC, sumE + 1)) return -1;
+    int hu4;
+    if (sort_enqueue(E.Cb.p, sumE, vary_of(sumE, or4, and4), nxtC.p, 6, &hu4, 1,
+                     cell_bound(vb, s + 4, and4)))
         return -1;
+    if (vb) vb->s = s + 2;
     pc.pending = true;
-    pc.n2 = n2;
     pc.sumE = sumE;
-    pc.hu2 = hu2;
     pc.hu4 = hu4;
-    *nP2 = n2;
+    *nP2 = nC + sumE;
     *nC2 = sumE;
-    carry.o = or4;
-    carry.a = and4;
     return 0;
 }
 
@@ -758,15 +805,15 @@ extern "C" int cg_tb_generate(cg_tablebase *tb, const uint64_t *initial, size_t 
     if (ensure(tb->P, n + 1)) return -1;
     if (n) HC(hipMemcpy(tb->P.p, initial, n * 8, hipMemcpyHostToDevice));
     size_t nP = n, nC = 0;
-    Carry carry;
     StepCounts pc;
     ArenaLayerAlloc la(&tb->arena);
+    ValueBound vb = value_bound_of(initial, n);
     for (int s = start; s <= end; s += 2) {
         TbLayer l;
         size_t nP2 = 0, nC2 = 0;
         uint64_t mor = 0, mand = ~0ULL;
-        if (gen_step(tb->r, tb->P.p, nP, tb->C.p, nC, pc, carry, la, &l.keys, &l.n, tb->P2, &nP2,
-                     tb->C2, &nC2, &mor, &mand))
+        if (gen_step(tb->r, tb->P.p, nP, tb->C.p, nC, pc, la, &l.keys, &l.n, tb->P2, &nP2,
+                     tb->C2, &nC2, &vb, &mor, &mand))
             return -1;
         l.plan = plan_index(l.n, mor, mand);
         l.dir = (uint32_t *)tb->arena.alloc(l.plan.dir_entries * 4);
@@ -778,7 +825,7 @@ extern "C" int cg_tb_generate(cg_tablebase *tb, const uint64_t *initial, size_t 
         nP = nP2;
         nC = nC2;
     }
-    if (counts_finish(pc, nullptr, nullptr)) return -1;
+    if (counts_finish(pc)) return -1;
     return check_sync(E.st, "cg_tb_generate");
 }
 
@@ -993,18 +1040,38 @@ extern "C" int cg_dev_spawn(const uint64_t *d_in, size_t n, uint64_t *d_out2, ui
     return 0;
 }
 
-extern "C" int cg_dev_sort_unique(uint64_t *d_keys, size_t n, uint64_t *d_out, size_t *n_unique)
+extern "C" int cg_dev_sort_unique_bounded(uint64_t *d_keys, size_t n, uint64_t *d_out,
+                                          size_t *n_unique, uint64_t cell_max)
 {
     if (eng_init()) return -1;
     launch_init_stats(E.st, E.d_cnt);
     launch_or_and(E.st, d_keys, n, E.d_cnt);
     if (read_counters(0, 3)) return -1;
     int hu;
-    if (sort_enqueue(d_keys, n, vary_of(n, E.h_cnt[1], E.h_cnt[2]), d_out, 5, &hu)) return -1;
+    if (sort_enqueue(d_keys, n, vary_of(n, E.h_cnt[1], E.h_cnt[2]), d_out, 5, &hu, 0, cell_max))
+        return -1;
     if (read_counters(5, 1)) return -1;
     *n_unique = E.h_cnt[5];
     prof_set_bytes(hu, 8.0 * n + 8.0 * *n_unique);
     return 0;
+}
+
+extern "C" int cg_dev_sort_unique(uint64_t *d_keys, size_t n, uint64_t *d_out, size_t *n_unique)
+{
+    return cg_dev_sort_unique_bounded(d_keys, n, d_out, n_unique, CELLS_UNBOUNDED);
+}
+
+extern "C" int cg_sort_code_probe(const uint64_t *keys, size_t n, uint64_t vary, uint64_t cell_max,
+                                  uint64_t *codes, uint64_t *back)
+{
+    SortCode c = make_pair_code(vary, cell_max);
+    if (!c.cells) return -1;
+    for (size_t i = 0; i < n; i++) {
+        uint32_t dm = 0;
+        codes[i] = sc_encode(keys[i], c, dm);
+        back[i] = sc_decode(codes[i], c) | (keys[i] & ~c.vmask);
+    }
+    return c.nbits;
 }
 
 /* a temporary index for caller-owned layers */
@@ -1100,11 +1167,17 @@ extern "C" int cg_tune_set(const char *knob, int value)
         g_spawn_est_mode = value;
         return 0;
     }
+    if (!strcmp(knob, "cell_code")) {
+        if (value != 0 && value != 1) return fail_msg("cell_code: 0 or 1");
+        sort_set_cell_code(value);
+        return 0;
+    }
     if (!strcmp(knob, "reset") || !strcmp(knob, "builtin")) {
         sort_table_reset();
         sort_force(0, 0, 0);
         set_dir_delta(0);
         g_spawn_est_mode = 0;
+        sort_set_cell_code(1);
         if (!strcmp(knob, "reset") && !g_tune_path.empty()) return cg_tune_load(g_tune_path.c_str());
         return 0;
     }
@@ -1119,6 +1192,8 @@ extern "C" int cg_tune_get(const char *knob)
     if (!strcmp(knob, "sort_lbw")) return f.lbw;
     if (!strcmp(knob, "dir_delta")) return get_dir_delta();
     if (!strcmp(knob, "spawn_estimate")) return g_spawn_est_mode;
+    if (!strcmp(knob, "cell_code")) return sort_cell_code();
+    if (!strcmp(knob, "cell_code_sorts")) return (int)std::min<unsigned long long>(sort_code_count(), 0x7fffffff);
     return fail_msg("unknown tuning knob");
 }
 
@@ -1453,8 +1528,8 @@ static struct {
     cg_rules r;
     Buf P, C, P2, C2, M;
     size_t nP = 0, nC = 0, nM = 0;
-    Carry carry;
     StepCounts pc;
+    ValueBound vb;
 } G;
 
 struct BufLayerAlloc : LayerAlloc {
@@ -1474,8 +1549,8 @@ extern "C" int cge_gen_begin(const cge_rules *r, const uint64_t *initial, size_t
     G.nP = n;
     G.nC = 0;
     G.nM = 0;
-    G.carry = Carry();
     G.pc = StepCounts();
+    G.vb = value_bound_of(initial, n);
     return 0;
 }
 
@@ -1484,8 +1559,8 @@ extern "C" int cge_gen_step(size_t *n_layer)
     uint64_t *M = nullptr;
     size_t nP2 = 0, nC2 = 0;
     BufLayerAlloc la;
-    if (gen_step(G.r, G.P.p, G.nP, G.C.p, G.nC, G.pc, G.carry, la, &M, &G.nM, G.P2, &nP2, G.C2,
-                 &nC2))
+    if (gen_step(G.r, G.P.p, G.nP, G.C.p, G.nC, G.pc, la, &M, &G.nM, G.P2, &nP2, G.C2,
+                 &nC2, &G.vb))
         return -1;
     std::swap(G.P, G.P2);
     std::swap(G.C, G.C2);
@@ -1504,7 +1579,7 @@ extern "C" int cge_gen_fetch(uint64_t *host, size_t n)
 
 extern "C" void cge_gen_end(void)
 {
-    counts_finish(G.pc, nullptr, nullptr);
+    counts_finish(G.pc);
     buf_free(G.P);
     buf_free(G.C);
     buf_free(G.P2);
@@ -1580,8 +1655,8 @@ extern "C" void cge_solve_end(void)
 /* batched tables                                                            */
 /* ------------------------------------------------------------------------ */
 /* Many small tables share every kernel launch: per layer step one move, one
- * sort/dedupe, one spawn, two sorts and one directory build serve all the
- * tables (composite keys carry the table id), with four host
+ * sort/dedupe, one spawn, the carry sort and one directory build serve all the
+ * tables (composite keys carry the table id), with two host
  * synchronisations per step whatever the number of tables.  The solve of a
  * step is one kernel over a per-table descriptor array; the whole downward
  * sweep is captured once in a hipGraph and replayed. */
@@ -1750,12 +1825,38 @@ extern "C" int cg_batch_generate(cg_batch *b)
     if (ensure(b->P, p0.size() + 1)) return -1;
     HC(hipMemcpy(b->P.p, p0.data(), p0.size() * 8, hipMemcpyHostToDevice));
     size_t nP = p0.size(), nC = 0;
-    Carry carry;
     StepCounts pc;
     std::vector<uint64_t> init_tstat(3 * ntab);
     for (int t = 0; t < ntab; t++) init_tstat[3 * t + 2] = ~0ULL;
     int maxfree = 0;
     for (int t = 0; t < ntab; t++) maxfree = std::max(maxfree, b->tabs[t].dev.nfree);
+    /* cell bounds of the composite keys: the table id's nibbles take the
+     * values of 0..ntab-1; a free cell holds at most the largest tile of its
+     * table's tile sum (ValueBound) when every table's initial boards share
+     * one tile sum and hold no 0xf outside the formation (0xf never enters a
+     * free cell: a formation cell would be vacated, which the kernels
+     * report) */
+    uint64_t tid_max = 0;
+    for (int j = 0; j < 4; j++) {
+        const int lo = BATCH_TID_SHIFT + 4 * j;
+        const uint64_t rest = (uint64_t)(ntab - 1) >> (4 * j);
+        tid_max |= (rest > 15 ? 15ULL : rest) << lo;
+    }
+    long sum_max = 0;
+    bool bounded = true;
+    for (int t = 0; t < ntab && bounded; t++) {
+        const cg_batch::Tab &tab = b->tabs[t];
+        ValueBound v = value_bound_of(tab.init.data(), tab.init.size());
+        for (uint64_t c : tab.init)
+            for (int x = 0; x < 16; x++)
+                if (!((tab.dev.fnib >> (60 - 4 * x)) & 15) && ((c >> (60 - 4 * x)) & 15) == 15) bounded = false;
+        if (!v.ok) bounded = false;
+        /* the sum reached by the carry of the table's last step */
+        sum_max = std::max(sum_max, v.s + 2L * (tab.nsteps - 1) + 4);
+    }
+    uint64_t batch_cells = CELLS_UNBOUNDED;
+    if (bounded && sum_max >= 1 && tile_log2(sum_max) <= 14)
+        batch_cells = tid_max | (cells_bounded_by(tile_log2(sum_max)) >> (64 - BATCH_TID_SHIFT));
     /* per step: two host synchronisations, as in gen_step */
     for (int k = 0; k < b->K; k++) {
         cg_batch::Step st;
@@ -1763,22 +1864,22 @@ extern "C" int cg_batch_generate(cg_batch *b)
         if (ensure(E.A, 4 * nP + 1)) return -1;
         launch_init_stats(E.st, E.d_cnt);
         int h = prof_begin(PK_MOVE);
-        launch_bmove(E.st, b->P.p, nP, pc.pending ? E.d_cnt + 5 : nullptr, E.A.p, E.d_cnt, lut,
-                     move_rules(b->r), b->d_tabs, k, E.d_err);
+        launch_bmove(E.st, b->P.p, nP, nullptr, E.A.p, E.d_cnt, lut, move_rules(b->r), b->d_tabs,
+                     k, E.d_err);
         prof_end(h, 0);
         if (pc.pending && join_lane1()) return -1;
         if (read_counters(0, pc.pending ? 7 : 3)) return -1;
         if (pc.pending) {
-            nP = E.h_cnt[5];
             nC = E.h_cnt[6];
-            counts_arrived(pc, nP, nC);
+            counts_arrived(pc, nC);
         }
         size_t ncand = E.h_cnt[0];
         uint64_t mor = E.h_cnt[1], mand = E.h_cnt[2];
         prof_set_bytes(h, 8.0 * nP + 8.0 * ncand);
         if (ensure(b->M, ncand + 1)) return -1;
         int hu;
-        if (sort_enqueue(E.A.p, ncand, vary_of(ncand, mor, mand), b->M.p, 7, &hu)) return -1;
+        if (sort_enqueue(E.A.p, ncand, vary_of(ncand, mor, mand), b->M.p, 7, &hu, 0, batch_cells))
+            return -1;
         /* boards of the step (count on the device until the next synchronisation) */
         st.keys = (uint64_t *)b->arena.alloc(std::max<size_t>(ncand, 1) * 8);
         if (!st.keys) return fail_msg("batch layer allocation failed");
@@ -1789,11 +1890,11 @@ extern "C" int cg_batch_generate(cg_batch *b)
         int hs = -1;
         size_t nM = 0, sumE = 0;
         for (int attempt = 0; attempt < 2; attempt++) {
-            if (ensure(E.A, nC + cap + 1) || ensure(E.Cb, cap + 1)) return -1;
-            if (nC) HC(hipMemcpyAsync(E.A.p, b->C.p, nC * 8, hipMemcpyDeviceToDevice, E.st));
+            if (ensure(b->P2, nC + cap + 1) || ensure(E.Cb, cap + 1)) return -1;
+            if (nC) HC(hipMemcpyAsync(b->P2.p, b->C.p, nC * 8, hipMemcpyDeviceToDevice, E.st));
             launch_init_stats(E.st, E.d_cnt);
             hs = prof_begin(PK_SPAWN);
-            launch_bspawn(E.st, b->M.p, ncand, E.d_cnt + 7, E.A.p + nC, E.Cb.p, cap, E.d_cnt,
+            launch_bspawn(E.st, b->M.p, ncand, E.d_cnt + 7, b->P2.p + nC, E.Cb.p, cap, E.d_cnt,
                           b->d_tabs, k, E.d_err);
             prof_end(hs, 0);
             HC(hipMemcpyAsync(b->h_tstat.data(), b->d_tstat, 3 * ntab * 8, hipMemcpyDeviceToHost,
@@ -1808,12 +1909,12 @@ extern "C" int cg_batch_generate(cg_batch *b)
         prof_set_bytes(hu, 8.0 * ncand + 8.0 * nM);
         prof_set_bytes(hs, 8.0 * nM + 16.0 * sumE);
         b->arena.shrink_last(st.keys, std::max<size_t>(ncand, 1) * 8, std::max<size_t>(nM, 1) * 8);
-        uint64_t or2 = E.h_cnt[1], and2 = E.h_cnt[2], or4 = E.h_cnt[3], and4 = E.h_cnt[4];
+        uint64_t or4 = E.h_cnt[3], and4 = E.h_cnt[4];
+        /* next pre-move boards: C u S2 as written (a multiset, as in gen_step) */
         size_t n2 = nC + sumE;
-        if (ensure(b->P2, n2 + 1) || ensure(b->C2, sumE + 1)) return -1;
-        int hu2, hu4;
-        if (sort_enqueue(E.Cb.p, sumE, vary_of(sumE, or4, and4), b->C2.p, 6, &hu4, 1)) return -1;
-        if (sort_enqueue(E.A.p, n2, vary_of(n2, or2 | carry.o, and2 & carry.a), b->P2.p, 5, &hu2))
+        if (ensure(b->C2, sumE + 1)) return -1;
+        int hu4;
+        if (sort_enqueue(E.Cb.p, sumE, vary_of(sumE, or4, and4), b->C2.p, 6, &hu4, 1, batch_cells))
             return -1;
         /* directories of every table of this step */
         st.n = nM;
@@ -1846,18 +1947,14 @@ extern "C" int cg_batch_generate(cg_batch *b)
         launch_bdir(E.st, st.keys, nM, d_seg, ntab, d_plan, entries, st.dir, E.scratch);
         b->steps.push_back(std::move(st));
         pc.pending = true;
-        pc.n2 = n2;
         pc.sumE = sumE;
-        pc.hu2 = hu2;
         pc.hu4 = hu4;
         std::swap(b->P, b->P2);
         std::swap(b->C, b->C2);
         nP = n2;
         nC = sumE;
-        carry.o = or4;
-        carry.a = and4;
     }
-    if (counts_finish(pc, nullptr, nullptr)) return -1;
+    if (counts_finish(pc)) return -1;
     return check_sync(E.st, "cg_batch_generate");
 }
 

@@ -38,6 +38,33 @@ struct KeyXform {
 };
 KeyXform make_xform(uint64_t vary);
 
+/* The sort's key code.  cells == 0: the bit packing x.  cells == 1 (the
+ * cell-pair code): the ncell varying cells (nibbles at bit cell_lo[i], most
+ * significant first; cell i holds values below cbase[i]) are taken two at a
+ * time, a pair (a, b) as a * cbase(b) + b in a field of pbits[p] bits (npair =
+ * ncell / 2; an odd last cell in lbits bits), so two cells of values 0..10
+ * take 7 bits instead of 8.  Both codes keep the keys' order: field order is
+ * cell order, and a * cbase(b) + b orders pairs lexicographically. */
+struct SortCode {
+    KeyXform x;
+    int cells, npair, ncell, lbits;
+    int nbits;              /* bits of the code */
+    uint64_t vmask;         /* the key bits the code covers */
+    uint8_t cell_lo[16];
+    uint8_t cbase[16];
+    uint8_t pbits[8];
+    uint32_t magic[8];      /* ceil(2^16 / cbase of the pair's second cell):
+                               f / base = (f * magic) >> 16 for f < 256 */
+};
+/* bit packing of vary */
+SortCode make_bit_code(uint64_t vary);
+/* the cell-pair code of vary's cells, where the nibble of cell_max at each
+ * cell's position is the cell's largest value (bit packing when vary is 0) */
+SortCode make_pair_code(uint64_t vary, uint64_t cell_max);
+/* cell_max with every cell bounded by v (0..15) */
+constexpr uint64_t cells_bounded_by(int v) { return (uint64_t)(v & 15) * 0x1111111111111111ULL; }
+constexpr uint64_t CELLS_UNBOUNDED = ~0ULL;
+
 /* A sorted unique layer with its prefix-bucket index.  Keys of the layer
  * agree on the bits in cmask (value cval); the remaining bits are packed by
  * x and the top `bits` packed bits select a bucket: dir[b] is the index of
@@ -112,11 +139,16 @@ void launch_spawn(hipStream_t st, const uint64_t *in, size_t n, const unsigned l
  * streams, each with its own tmp and scratch.  A
  * chained-scan wait that exceeds its budget sets *d_err (never cleared
  * here); the caller checks it after synchronising.  keys and tmp are
- * clobbered.  *prof_unique receives the timer handle of the compaction. */
+ * clobbered.  *prof_unique receives the timer handle of the compaction.
+ * cell_max states, per 4-bit cell (the nibble of cell_max at the cell's
+ * position), the largest value the keys hold there; the sort uses the
+ * cell-pair code for the cells with a bit in `vary` when that needs fewer
+ * passes.  A key that breaks the bound ORs SORT_ERR_CODE into *d_err. */
 constexpr int SORT_CONTEXTS = 2;
+constexpr unsigned SORT_ERR_CODE = 4u;
 uint64_t *sort_unique(hipStream_t st, uint64_t *keys, uint64_t *tmp, size_t n, uint64_t vary,
                       void *scratch, uint64_t *dst, unsigned long long *d_nuniq, unsigned *d_err,
-                      int *prof_unique, int ctx = 0);
+                      int *prof_unique, int ctx = 0, uint64_t cell_max = CELLS_UNBOUNDED);
 
 /* Sort configurations: digit width (8, 9 or 10 bits; the pass block has one
  * thread per digit), keys per thread (tile = 2^bits * kpt keys) and look-back
@@ -139,6 +171,11 @@ SortCfg sort_forced();
 SortCfg sort_choose(size_t n, int nbits);
 /* frees the look-back status array */
 void sort_release();
+/* whether bounded sorts may use the cell-pair code (default on), and the
+ * number of sorts that used it */
+void sort_set_cell_code(int on);
+int sort_cell_code();
+unsigned long long sort_code_count();
 
 /* prefix-bucket directory depth: bits = floor(log2 n) + delta, clamped */
 void set_dir_delta(int d);

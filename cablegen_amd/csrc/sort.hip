@@ -5,7 +5,10 @@
  * the kernels that wrote the keys) are packed into a dense key on the fly
  * (an order-preserving bijection on the key set), so a DPDGap layer with 40
  * varying bits spread over bits 0..51 sorts in 5 passes of 8 bits, or 4 of
- * 10.  One histogram kernel computes every pass's digit counts in a single
+ * 10.  When the caller bounds the cell values (tiles of a layer of tile sum s
+ * are at most 2^floor(log2 s)), pairs of varying cells are coded as
+ * a * base + b: the DPDGap layer's 10 cells of values 0..10 then take 35 bits,
+ * 4 passes of 9.  One histogram kernel computes every pass's digit counts in a single
  * read; each pass is one kernel whose tiles take their global offsets from a
  * chained (decoupled look-back) scan over per-tile digit counts; the unique
  * compaction is one more chained-scan kernel.
@@ -27,6 +30,7 @@
  */
 #include <hip/hip_runtime.h>
 #include <algorithm>
+#include <string.h>
 #include "kernels.h"
 #include "devutil.h"
 
@@ -73,31 +77,33 @@ __device__ __forceinline__ uint64_t wait_flag(const uint64_t *p, uint32_t epoch,
     return v;
 }
 
-/* Exclusive prefix for one chained-scan lane: status of tile i at
- * status[i * stride].  The flags are coherent across XCDs, so every load is
- * a long round trip and the loads themselves cost bandwidth: the window
- * grows 1, 2, 4, ... up to LB_W predecessors per step (usually the nearest
- * one or two already carry an inclusive prefix), sums aggregates
- * nearest-first, and waits only on the first predecessor that has not
- * published yet. */
+/* Sum for one chained-scan lane over the entries t_from, t_from - 1, ...,
+ * lo (status of entry i at status[i * stride]), nearest first, up to and
+ * including the first inclusive prefix (*done then set).  The flags are
+ * coherent across XCDs, so every load is a long round trip and the loads
+ * themselves cost bandwidth: the window grows from win0 by doubling up to
+ * LB_W entries per step, and the walk waits only on the first entry that
+ * has not published yet. */
 template <int LB_W>
-__device__ __forceinline__ uint64_t lookback(const uint64_t *status, size_t stride, size_t tile,
-                                             uint32_t epoch, unsigned *err)
+__device__ __forceinline__ uint64_t lookback_range(const uint64_t *status, size_t stride, long t_from,
+                                                   long lo, uint32_t epoch, unsigned *err, int win0,
+                                                   bool *done_out)
 {
     uint64_t excl = 0;
-    long t = (long)tile - 1;
-    int win = 1;
-    while (t >= 0) {
+    long t = t_from;
+    int win = win0;
+    *done_out = false;
+    while (t >= lo) {
         uint64_t v[LB_W];
 #pragma unroll
         for (int j = 0; j < LB_W; j++)
-            v[j] = (j < win && t - j >= 0) ? ld_flag(status + (size_t)(t - j) * stride) : 0;
+            v[j] = (j < win && t - j >= lo) ? ld_flag(status + (size_t)(t - j) * stride) : 0;
         /* fully unrolled and predicated, so v stays in registers */
         int used = 0;
         bool blocked = false, done = false;
 #pragma unroll
         for (int j = 0; j < LB_W; j++) {
-            if (!blocked && !done && j < win && t - j >= 0) {
+            if (!blocked && !done && j < win && t - j >= lo) {
                 uint32_t f = fstate(v[j], epoch);
                 if (f == 0) {
                     blocked = true;
@@ -108,19 +114,57 @@ __device__ __forceinline__ uint64_t lookback(const uint64_t *status, size_t stri
                 }
             }
         }
-        if (done) return excl;
+        if (done) {
+            *done_out = true;
+            return excl;
+        }
         t -= used;
         if (!blocked) {
             win = win * 2 < LB_W ? win * 2 : LB_W;
             continue;
         }
-        /* t is the nearest predecessor that has not published yet */
+        /* t is the nearest entry that has not published yet */
         uint64_t w = wait_flag(status + (size_t)t * stride, epoch, err);
         excl += w & OS_VAL;
-        if (fstate(w, epoch) == 2) return excl;
+        if (fstate(w, epoch) == 2) {
+            *done_out = true;
+            return excl;
+        }
         t -= 1;
     }
     return excl;
+}
+
+/* Two-level look-back of a pass's digit lane.  Tiles form groups of
+ * LB_GROUP; the last tile of a group publishes the group's status (gst):
+ * its aggregate as soon as it has summed its group's tile aggregates, and the
+ * inclusive prefix once known.  A tile walks its in-group predecessors, then
+ * previous groups, so when all tiles start at once (a sort of a few hundred
+ * tiles) the walk takes a few round trips instead of one per LB_W tiles.
+ * Group statuses depend only on tile aggregates, which never wait, so every
+ * wait ends.  Returns the tile's exclusive prefix. */
+constexpr int LB_GROUP = 16;
+
+template <int LB_W>
+__device__ __forceinline__ uint64_t lookback_grouped(const uint64_t *tst, uint64_t *gst, size_t stride,
+                                                     size_t tile, uint64_t tot, uint32_t epoch,
+                                                     unsigned *err)
+{
+    const long g = (long)(tile / LB_GROUP), g0 = g * LB_GROUP;
+    const bool gend = (long)tile - g0 == LB_GROUP - 1;
+    bool done;
+    uint64_t excl = lookback_range<LB_W>(tst, stride, (long)tile - 1, g0, epoch, err, LB_W, &done);
+    if (gend) st_flag(gst + (size_t)g * stride, done ? OS_INC : OS_AGG, epoch, excl + tot);
+    if (done) return excl;
+    excl += lookback_range<LB_W>(gst, stride, g - 1, 0, epoch, err, LB_W, &done);
+    if (gend) st_flag(gst + (size_t)g * stride, OS_INC, epoch, excl + tot);
+    return excl;
+}
+
+/* entries of a pass's status array: tiles, then groups, per digit */
+__host__ __device__ constexpr size_t pass_status_entries(size_t tiles, int bits)
+{
+    return (tiles + (tiles + LB_GROUP - 1) / LB_GROUP) << bits;
 }
 
 /* digit counts of every pass in one read of the keys, 16 waves per block.
@@ -132,17 +176,18 @@ constexpr int HIST_THREADS = 1024;
 
 template <int BITS>
 __global__ __launch_bounds__(HIST_THREADS) void k_os_hist(const uint64_t *__restrict__ keys, size_t n,
-                                                 KeyXform x, int npass, uint32_t *ghist,
-                                                 uint64_t vmask, uint64_t *cval)
+                                                 SortCode c, int npass, uint32_t *ghist,
+                                                 uint64_t *cval, unsigned *err)
 {
     constexpr int R = 1 << BITS, NC = BITS == 8 ? 8 : (BITS == 9 ? 4 : 2);
     constexpr int STRIDE = OS_MAXPASS * R + 1;   /* 64 KiB of counters */
     __shared__ uint32_t h[NC * STRIDE];
     /* the bits outside the packed ones are the same in every key */
-    if (blockIdx.x == 0 && threadIdx.x == 0) *cval = keys[0] & ~vmask;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *cval = keys[0] & ~c.vmask;
     for (int i = threadIdx.x; i < NC * STRIDE; i += HIST_THREADS) h[i] = 0;
     __syncthreads();
     uint32_t *hc = h + (lane_id() % NC) * STRIDE;
+    uint32_t bad = 0;   /* a cell outside its base (cell-pair code) */
     constexpr int HB = 8;   /* keys loaded per thread before counting */
     for (size_t b0 = (size_t)blockIdx.x * HIST_THREADS * HB; b0 < n;
          b0 += (size_t)gridDim.x * HIST_THREADS * HB) {
@@ -156,10 +201,12 @@ __global__ __launch_bounds__(HIST_THREADS) void k_os_hist(const uint64_t *__rest
         for (int j = 0; j < HB; j++) {
             size_t i = b0 + (size_t)j * HIST_THREADS + threadIdx.x;
             if (i >= n) continue;
-            uint64_t v = xform(kk[j], x);
+            uint64_t v = sc_encode(kk[j], c, bad);
             for (int p = 0; p < npass; p++) atomicAdd(&hc[p * R + ((v >> (BITS * p)) & (R - 1))], 1u);
         }
     }
+    /* a cell value outside the code would be sorted wrongly */
+    if (bad) atomicOr(err, SORT_ERR_CODE);
     __syncthreads();
     for (int i = threadIdx.x; i < npass * R; i += HIST_THREADS) {
         uint32_t t = 0;
@@ -185,9 +232,9 @@ __global__ __launch_bounds__(1 << BITS) void k_os_scan(const uint32_t *ghist, in
 
 /* One LSD pass: a block of R = 2^BITS threads (thread d owns digit d) sorts
  * a tile of R * KPT keys by digit (k >> shift) & (R - 1), stably, where k is
- * the packed key.  The first pass (RAW) reads the original keys and packs
- * them (xform) once; every pass writes packed keys, so later passes extract
- * a digit with a shift and a mask. */
+ * the key's code.  The first pass (RAW) reads the original keys and encodes
+ * them (sc_encode) once; every pass writes codes, so later passes extract a
+ * digit with a shift and a mask. */
 /* threads of a pass block: one per digit, except the 8-keys-per-thread
  * configurations, which use 1024 threads (the first 2^bits own the digits) so
  * that an 8192-key tile is spread over 16 waves */
@@ -208,7 +255,7 @@ constexpr int pass_min_waves(int bits, int kpt, int lbw)
 template <int BITS, int KPT, int LB_W, bool RAW>
 __global__ __launch_bounds__(pass_threads(BITS, KPT), pass_min_waves(BITS, KPT, LB_W)) void k_os_pass(const uint64_t *__restrict__ in,
                                                        uint64_t *__restrict__ out, size_t n,
-                                                       KeyXform x, int shift,
+                                                       SortCode c, int shift,
                                                        const uint64_t *__restrict__ gbase,
                                                        uint64_t *status, unsigned *tile_ctr,
                                                        uint32_t epoch, unsigned *err)
@@ -238,8 +285,9 @@ __global__ __launch_bounds__(pass_threads(BITS, KPT), pass_min_waves(BITS, KPT, 
 #pragma unroll
     for (int j = 0; j < KPT; j++) key[j] = j * 64 + lane < rem ? in[base + j * 64 + lane] : 0;
     if (RAW) {
+        uint32_t bad = 0;   /* checked by the histogram kernel */
 #pragma unroll
-        for (int j = 0; j < KPT; j++) key[j] = xform(key[j], x);
+        for (int j = 0; j < KPT; j++) key[j] = sc_encode(key[j], c, bad);
     }
 #pragma unroll
     for (int j = 0; j < KPT; j++) {
@@ -284,7 +332,8 @@ __global__ __launch_bounds__(pass_threads(BITS, KPT), pass_min_waves(BITS, KPT, 
             st_flag(my, OS_AGG, epoch, tot);
             /* the 8-waves-per-SIMD variant keeps its look-back window small */
         constexpr int LBW = pass_min_waves(BITS, KPT, LB_W) > 2 ? LB_W / 2 : LB_W;
-        excl = lookback<LBW>(status + d, R, tile, epoch, err);
+        const size_t ntiles = (n + TILE - 1) / TILE;
+        excl = lookback_grouped<LBW>(status + d, status + ntiles * R + d, R, tile, tot, epoch, err);
             st_flag(my, OS_INC, epoch, excl + tot);
         }
         obase[d] = gbase[d] + excl;
@@ -321,7 +370,7 @@ __global__ __launch_bounds__(UQ_THREADS) void k_os_unique(const uint64_t *__rest
                                                           uint64_t *status, unsigned *tile_ctr,
                                                           uint32_t epoch,
                                                           unsigned long long *total, unsigned *err,
-                                                          int packed, KeyXform x,
+                                                          int packed, SortCode code,
                                                           const uint64_t *cval)
 {
     constexpr int WCHUNK = 64 * UQ_KPT;
@@ -409,12 +458,12 @@ __global__ __launch_bounds__(UQ_THREADS) void k_os_unique(const uint64_t *__rest
     __syncthreads();
     uint64_t o = excl_sh;
     for (int ww = 0; ww < w; ww++) o += wc[ww];
-    /* packed keys go back to the original bits: xform_inv plus the constant bits */
+    /* codes go back to the original bits: sc_decode plus the constant bits */
     const uint64_t cv = packed ? *cval : 0;
 #pragma unroll
     for (int j = 0; j < UQ_KPT; j++) {
         if ((m[j] >> lane) & 1ULL)
-            dst[o + __popcll(m[j] & lt)] = packed ? (xform_inv(kv[j], x) | cv) : kv[j];
+            dst[o + __popcll(m[j] & lt)] = packed ? (sc_decode(kv[j], code) | cv) : kv[j];
         o += __popcll(m[j]);
     }
 }
@@ -461,6 +510,61 @@ KeyXform make_xform(uint64_t vary)
     x.nseg = ns;
     return x;
 }
+
+SortCode make_bit_code(uint64_t vary)
+{
+    SortCode c;
+    memset(&c, 0, sizeof c);
+    c.x = make_xform(vary);
+    c.nbits = c.x.nbits;
+    c.vmask = xform_inv(~0ULL, c.x);
+    return c;
+}
+
+static int bits_for(int values)
+{
+    int b = 0;
+    while ((1 << b) < values) b++;
+    return b;
+}
+
+SortCode make_pair_code(uint64_t vary, uint64_t cell_max)
+{
+    SortCode c = make_bit_code(vary);
+    if (!vary) return c;
+    int nc = 0;
+    uint64_t vm = 0;
+    for (int cell = 0; cell < 16; cell++) {   /* most significant cell first */
+        const int lo = 60 - 4 * cell;
+        if ((vary >> lo) & 15) {
+            const int base = (int)((cell_max >> lo) & 15) + 1;
+            c.cell_lo[nc] = (uint8_t)lo;
+            c.cbase[nc] = (uint8_t)(base < 2 ? 2 : base);
+            nc++;
+            vm |= 15ULL << lo;
+        }
+    }
+    c.cells = 1;
+    c.ncell = nc;
+    c.npair = nc / 2;
+    c.nbits = 0;
+    for (int p = 0; p < c.npair; p++) {
+        const int ba = c.cbase[2 * p], bb = c.cbase[2 * p + 1];
+        c.pbits[p] = (uint8_t)bits_for(ba * bb);
+        c.magic[p] = (65536u + (uint32_t)bb - 1) / (uint32_t)bb;
+        c.nbits += c.pbits[p];
+    }
+    c.lbits = (nc & 1) ? bits_for(c.cbase[nc - 1]) : 0;
+    c.nbits += c.lbits;
+    c.vmask = vm;
+    return c;
+}
+
+static int g_cell_code = 1;
+static unsigned long long g_code_sorts = 0;
+void sort_set_cell_code(int on) { g_cell_code = on ? 1 : 0; }
+int sort_cell_code() { return g_cell_code; }
+unsigned long long sort_code_count() { return g_code_sorts; }
 
 /* ---- configurations ---- */
 
@@ -621,25 +725,24 @@ static OsScratch carve(void *scratch)
     return s;
 }
 
-static void launch_hist_scan(hipStream_t st, int bits, const uint64_t *keys, size_t n, KeyXform x,
-                             int npass, const OsScratch &s)
+static void launch_hist_scan(hipStream_t st, int bits, const uint64_t *keys, size_t n, const SortCode &c,
+                             int npass, const OsScratch &s, unsigned *err)
 {
-    const uint64_t vmask = xform_inv(~0ULL, x);
     size_t hb = (n + 8191) / 8192;
     if (hb > 1024) hb = 1024;
     int h = prof_hook_begin(2, st);
     switch (bits) {
     case 9:
-        hipLaunchKernelGGL(k_os_hist<9>, dim3((unsigned)hb), dim3(HIST_THREADS), 0, st, keys, n, x, npass,
-                           s.ghist, vmask, s.cval);
+        hipLaunchKernelGGL(k_os_hist<9>, dim3((unsigned)hb), dim3(HIST_THREADS), 0, st, keys, n, c, npass,
+                           s.ghist, s.cval, err);
         break;
     case 10:
-        hipLaunchKernelGGL(k_os_hist<10>, dim3((unsigned)hb), dim3(HIST_THREADS), 0, st, keys, n, x, npass,
-                           s.ghist, vmask, s.cval);
+        hipLaunchKernelGGL(k_os_hist<10>, dim3((unsigned)hb), dim3(HIST_THREADS), 0, st, keys, n, c, npass,
+                           s.ghist, s.cval, err);
         break;
     default:
-        hipLaunchKernelGGL(k_os_hist<8>, dim3((unsigned)hb), dim3(HIST_THREADS), 0, st, keys, n, x, npass,
-                           s.ghist, vmask, s.cval);
+        hipLaunchKernelGGL(k_os_hist<8>, dim3((unsigned)hb), dim3(HIST_THREADS), 0, st, keys, n, c, npass,
+                           s.ghist, s.cval, err);
     }
     snote(hipGetLastError());
     prof_hook_end(h, 8.0 * n, st);
@@ -658,7 +761,7 @@ static void launch_hist_scan(hipStream_t st, int bits, const uint64_t *keys, siz
 
 template <bool RAW>
 static void launch_pass_t(hipStream_t st, SortCfg c, unsigned tiles, const uint64_t *src,
-                          uint64_t *dst, size_t n, KeyXform x, int shift, const uint64_t *gbase,
+                          uint64_t *dst, size_t n, const SortCode &x, int shift, const uint64_t *gbase,
                           uint64_t *status, unsigned *ctr, uint32_t epoch, unsigned *err)
 {
 #define CG_PASS(B, K, W)                                                                         \
@@ -696,7 +799,7 @@ static void launch_pass_t(hipStream_t st, SortCfg c, unsigned tiles, const uint6
 }
 
 static void launch_pass(hipStream_t st, SortCfg c, unsigned tiles, const uint64_t *src,
-                        uint64_t *dst, size_t n, KeyXform x, int shift, const uint64_t *gbase,
+                        uint64_t *dst, size_t n, const SortCode &x, int shift, const uint64_t *gbase,
                         uint64_t *status, unsigned *ctr, uint32_t epoch, unsigned *err, bool raw)
 {
     if (raw) launch_pass_t<true>(st, c, tiles, src, dst, n, x, shift, gbase, status, ctr, epoch, err);
@@ -705,7 +808,7 @@ static void launch_pass(hipStream_t st, SortCfg c, unsigned tiles, const uint64_
 
 uint64_t *sort_unique(hipStream_t st, uint64_t *keys, uint64_t *tmp, size_t n, uint64_t vary,
                       void *scratch, uint64_t *dst, unsigned long long *d_nuniq, unsigned *d_err,
-                      int *prof_unique, int ctx)
+                      int *prof_unique, int ctx, uint64_t cell_max)
 {
     *prof_unique = -1;
     SortCtx &sc = g_ctx[ctx < 0 || ctx >= SORT_CONTEXTS ? 0 : ctx];
@@ -714,18 +817,31 @@ uint64_t *sort_unique(hipStream_t st, uint64_t *keys, uint64_t *tmp, size_t n, u
         return keys;
     }
     OsScratch s = carve(scratch);
-    KeyXform x = make_xform(n > 1 ? vary : 0);
-    const SortCfg c = sort_choose(n, x.nbits);
-    const int npass = (x.nbits + c.bits - 1) / c.bits;
+    SortCode x = make_bit_code(n > 1 ? vary : 0);
+    SortCfg c = sort_choose(n, x.nbits);
+    int npass = (x.nbits + c.bits - 1) / c.bits;
+    if (n > 1 && cell_max != CELLS_UNBOUNDED && g_cell_code) {
+        /* the cell-pair code when it saves a pass */
+        SortCode pc = make_pair_code(vary, cell_max);
+        SortCfg c2 = sort_choose(n, pc.nbits);
+        int np2 = (pc.nbits + c2.bits - 1) / c2.bits;
+        if (pc.cells && np2 < npass) {
+            g_code_sorts++;
+            x = pc;
+            c = c2;
+            npass = np2;
+        }
+    }
     const size_t tile_keys = (size_t)pass_threads(c.bits, c.kpt) * c.kpt;
     const size_t tiles = (n + tile_keys - 1) / tile_keys;
     const size_t utiles = (n + UQ_TILE - 1) / UQ_TILE;
-    if (!status_reserve(st, sc, std::max(npass ? tiles << c.bits : 0, utiles))) return keys;
+    if (!status_reserve(st, sc, std::max(npass ? pass_status_entries(tiles, c.bits) : 0, utiles)))
+        return keys;
     /* tile counters and histograms cleared together */
     launch_fill_u64(st, (uint64_t *)scratch, OS_CLEAR_BYTES / 8, 0);
     uint64_t *src = keys, *dst2 = tmp;
     if (npass > 0) {
-        launch_hist_scan(st, c.bits, keys, n, x, npass, s);
+        launch_hist_scan(st, c.bits, keys, n, x, npass, s, d_err);
         for (int p = 0; p < npass; p++) {
             int h = prof_hook_begin(3, st);
             launch_pass(st, c, (unsigned)tiles, src, dst2, n, x, c.bits * p,

@@ -117,6 +117,18 @@ int cg_dev_spawn(const uint64_t *d_in, size_t n, uint64_t *d_out2, uint64_t *d_o
 /* sorts d_keys[0..n) and writes the unique keys to d_out (capacity n);
  * d_keys is clobbered.  *n_unique receives the count. */
 int cg_dev_sort_unique(uint64_t *d_keys, size_t n, uint64_t *d_out, size_t *n_unique);
+/* as cg_dev_sort_unique, with the largest value of every 4-bit cell stated
+ * in cell_max (the nibble at the cell's position; all ones: no bound): the
+ * sort may then code pairs of varying cells as a * base + b.  A key above
+ * its bound makes the call fail. */
+int cg_dev_sort_unique_bounded(uint64_t *d_keys, size_t n, uint64_t *d_out, size_t *n_unique,
+                               uint64_t cell_max);
+/* Host-side view of the sort's cell-pair code (no GPU work): for the cells
+ * with a bit in vary, bounded by cell_max as above, codes[i] is the code of
+ * keys[i] and back[i] the key decoded from it.  Returns the code's width in
+ * bits, or -1 when vary is 0. */
+int cg_sort_code_probe(const uint64_t *keys, size_t n, uint64_t vary, uint64_t cell_max,
+                       uint64_t *codes, uint64_t *back);
 /* exact-match lookup of nq queries into a sorted unique layer (keys, values);
  * misses give 0 (lookup, src/solve.c:108-164). */
 int cg_dev_lookup(const uint64_t *d_keys, const double *d_vals, size_t n, const uint64_t *d_q,
@@ -153,11 +165,16 @@ int cg_dev_sync(void);
  *                layer, a step that overflows them is repeated with exact
  *                buffers; 1: sized for half the previous layer, so that
  *                growing layers take the repeat (for tests);
+ *   "cell_code"  1: a generation's sorts code pairs of varying cells as
+ *                a * base + b when the layer's tile sum bounds the cell
+ *                values and that saves a radix pass; 0: bit packing only;
  *   "builtin"    built-in defaults: 8-bit digits, 8192-key tiles, window 16
- *                for every entry, no forced field, delta 0, spawn_estimate 0;
+ *                for every entry, no forced field, delta 0, spawn_estimate 0,
+ *                cell_code 1;
  *   "reset"      the state after initialisation: the built-in defaults, then
  *                the table loaded at initialisation if there was one.
- * Returns 0, or -1 for an unknown knob or value. */
+ * Returns 0, or -1 for an unknown knob or value.  cg_tune_get also reads
+ * "cell_code_sorts": the number of sorts that used the cell-pair code. */
 int cg_tune_set(const char *knob, int value);
 int cg_tune_get(const char *knob);
 /* the sorts read their configuration from a table indexed by size class

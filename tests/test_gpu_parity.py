@@ -175,6 +175,119 @@ def test_sort_every_configuration(cg, cfg):
         cg.tune_set("reset", 0)
 
 
+def cell_keys(rng, n, cells, vmax, const=0xFFF0FFF000000000):
+    """n keys equal to `const` outside `cells` (0 = most significant nibble),
+    whose values are 0..vmax"""
+    k = np.full(n, const, np.uint64)
+    for c in cells:
+        sh = np.uint64(60 - 4 * c)
+        k &= ~(np.uint64(15) << sh)
+        k |= rng.integers(0, vmax + 1, n).astype(np.uint64) << sh
+    return k
+
+
+def bounded(v):
+    """a cell_max word bounding every cell by v"""
+    return v * 0x1111111111111111
+
+
+def _bounded_check(cg, a, cell_max):
+    d = dev(a) if len(a) else torch.empty(1, dtype=torch.int64, device="cuda")
+    out = torch.empty(max(len(a), 1), dtype=torch.int64, device="cuda")
+    n = C.c_size_t()
+    assert cg.lib().cg_dev_sort_unique_bounded(d.data_ptr(), len(a), out.data_ptr(), C.byref(n),
+                                               cell_max) == 0, cg.last_error()
+    assert np.array_equal(host(out, n.value), np.unique(a))
+
+
+DPD_CELLS = [3, 7, 8, 9, 10, 11, 12, 13, 14, 15]
+# per digit width: cells and bound for which the cell-pair code saves a pass
+CODE_CASES = {8: (DPD_CELLS[:9], 10), 9: (DPD_CELLS, 10), 10: (DPD_CELLS, 7)}
+
+
+@pytest.mark.parametrize("cfg", SORT_CONFIGS, ids=lambda c: "%dx%d_w%d" % c)
+def test_sort_cell_code_every_configuration(cg, cfg):
+    """Bounded sorts through the cell-pair code against numpy's unique, with
+    each pass kernel encoding in its first pass: tile-boundary sizes and
+    heavy duplicates.  Every call with enough keys for all the cells to vary
+    must use the code (two keys may differ in too few cells to save a pass)."""
+    bits, kpt, lbw = cfg
+    tile = (1024 if kpt <= 8 else 1 << bits) * kpt
+    cells, vmax = CODE_CASES[bits]
+    rng = np.random.default_rng(7 + bits * 1000 + kpt * 10 + lbw)
+    try:
+        cg.tune_set("sort_bits", bits)
+        cg.tune_set("sort_kpt", kpt)
+        cg.tune_set("sort_lbw", lbw)
+        for n in (1, 2, tile + 1, 3 * tile + 17, 200003):
+            before = cg.tune_get("cell_code_sorts")
+            a = cell_keys(rng, n, cells, vmax)
+            if n >= 3:
+                a[::3] = a[1]
+            _bounded_check(cg, a, bounded(vmax))
+            if n > tile:
+                assert cg.tune_get("cell_code_sorts") == before + 1
+    finally:
+        cg.tune_set("reset", 0)
+
+
+def test_sort_cell_code_tuned_sizes(cg):
+    """DPDGap-like keys (ten varying cells of values 0..10, constant 0xf
+    cells) across the tuned table's size classes; bit packing and the code
+    give the same unique keys."""
+    rng = np.random.default_rng(11)
+    for n in (2, 1000, 70001, 3000017, (1 << 23) + 5):
+        a = cell_keys(rng, n, DPD_CELLS, 10)
+        before = cg.tune_get("cell_code_sorts")
+        _bounded_check(cg, a, bounded(10))
+        used = cg.tune_get("cell_code_sorts") - before
+        assert used == 1 or n == 2
+        _sort_check(cg, a)
+        assert cg.tune_get("cell_code_sorts") == before + used
+
+
+def test_sort_cell_code_bound_violation(cg):
+    """A key whose cell exceeds the stated bound makes the call fail (the
+    histogram kernel checks every cell it codes); the next call succeeds."""
+    rng = np.random.default_rng(12)
+    a = cell_keys(rng, 100000, DPD_CELLS, 10)
+    a[777] = (a[777] & ~np.uint64(0xF << 8)) | np.uint64(12 << 8)
+    d, out, n = dev(a), torch.empty(len(a), dtype=torch.int64, device="cuda"), C.c_size_t()
+    assert cg.lib().cg_dev_sort_unique_bounded(d.data_ptr(), len(a), out.data_ptr(), C.byref(n),
+                                               bounded(10)) != 0
+    assert "bound" in cg.last_error()
+    _bounded_check(cg, cell_keys(rng, 5000, DPD_CELLS, 10), bounded(10))
+
+
+def test_sort_cell_code_per_cell_bases(cg):
+    """Composite keys (a table id above ten free cells, as the batch builds
+    them) with a bound per cell, against numpy's unique."""
+    rng = np.random.default_rng(13)
+    for n in (5000, 300001):
+        tid = rng.integers(0, 40, n).astype(np.uint64)
+        a = cell_keys(rng, n, list(range(6, 16)), 8, const=0) | (tid << np.uint64(48))
+        before = cg.tune_get("cell_code_sorts")
+        _bounded_check(cg, a, (2 << 52) | (15 << 48) | (bounded(8) >> 16))
+        assert cg.tune_get("cell_code_sorts") == before + 1
+
+
+@pytest.mark.parametrize("name", ["c1", "dpdgap_prune", "two_initial"])
+def test_tablebase_cell_code_on_and_off(cg, name):
+    """Generation with the cell-pair code (the default; the initial boards fix
+    each layer's tile sum and 0xf cells) and with bit packing only: both
+    match the oracle, and the default run used the code."""
+    try:
+        before = cg.tune_get("cell_code_sorts")
+        test_tablebase_vs_oracle(cg, name)
+        assert cg.tune_get("cell_code_sorts") > before
+        cg.tune_set("cell_code", 0)
+        before = cg.tune_get("cell_code_sorts")
+        test_tablebase_vs_oracle(cg, name)
+        assert cg.tune_get("cell_code_sorts") == before
+    finally:
+        cg.tune_set("reset", 0)
+
+
 @pytest.mark.parametrize("delta", [-3, -1, 1, 2])
 def test_lookup_directory_depths(cg, delta):
     """Lookups through shallower and deeper prefix-bucket directories."""
@@ -524,7 +637,10 @@ def test_batch_matches_single_tables(cg):
     B = cg.Batch(r)
     for c in cfgs:
         B.add(c["initial"], c["start"], c["end"], c["winstates"])
+    before = cg.tune_get("cell_code_sorts")
     B.generate()
+    # the composite keys' table id and free cells are bounded: the sorts code them
+    assert cg.tune_get("cell_code_sorts") > before
     B.solve(use_graph=False)
     eager = [[B.values(t, i) for i in range(B.num_layers(t))] for t in range(len(cfgs))]
     for t, c in enumerate(cfgs):
