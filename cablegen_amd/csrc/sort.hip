@@ -84,20 +84,27 @@ __device__ __forceinline__ uint64_t wait_flag(const uint64_t *p, uint32_t epoch,
  * themselves cost bandwidth: the window grows from win0 by doubling up to
  * LB_W entries per step, and the walk waits only on the first entry that
  * has not published yet. */
-template <int LB_W>
+template <int LB_W, int PF>
 __device__ __forceinline__ uint64_t lookback_range(const uint64_t *status, size_t stride, long t_from,
                                                    long lo, uint32_t epoch, unsigned *err, int win0,
-                                                   bool *done_out)
+                                                   bool *done_out, const uint64_t (&pre)[PF],
+                                                   bool use_pre)
 {
     uint64_t excl = 0;
     long t = t_from;
     int win = win0;
+    bool first = use_pre;
     *done_out = false;
     while (t >= lo) {
         uint64_t v[LB_W];
+        /* the first PF entries of the first window may have been loaded
+         * earlier (pre, same entries) */
 #pragma unroll
         for (int j = 0; j < LB_W; j++)
-            v[j] = (j < win && t - j >= lo) ? ld_flag(status + (size_t)(t - j) * stride) : 0;
+            v[j] = (j < win && t - j >= lo)
+                       ? ((j < PF && first) ? pre[j < PF ? j : 0] : ld_flag(status + (size_t)(t - j) * stride))
+                       : 0;
+        first = false;
         /* fully unrolled and predicated, so v stays in registers */
         int used = 0;
         bool blocked = false, done = false;
@@ -145,18 +152,30 @@ __device__ __forceinline__ uint64_t lookback_range(const uint64_t *status, size_
  * wait ends.  Returns the tile's exclusive prefix. */
 constexpr int LB_GROUP = 16;
 
-template <int LB_W>
+/* loads the first in-group window of lookback_grouped (issued early, used
+ * later: any snapshot of aggregates and inclusive prefixes is valid input) */
+template <int PF>
+__device__ __forceinline__ void lookback_prefetch(const uint64_t *tst, size_t stride, size_t tile,
+                                                  uint64_t (&pre)[PF])
+{
+    const long g0 = (long)(tile / LB_GROUP) * LB_GROUP, t = (long)tile - 1;
+#pragma unroll
+    for (int j = 0; j < PF; j++) pre[j] = t - j >= g0 ? ld_flag(tst + (size_t)(t - j) * stride) : 0;
+}
+
+template <int LB_W, int PF>
 __device__ __forceinline__ uint64_t lookback_grouped(const uint64_t *tst, uint64_t *gst, size_t stride,
                                                      size_t tile, uint64_t tot, uint32_t epoch,
-                                                     unsigned *err)
+                                                     unsigned *err, const uint64_t (&pre)[PF])
 {
     const long g = (long)(tile / LB_GROUP), g0 = g * LB_GROUP;
     const bool gend = (long)tile - g0 == LB_GROUP - 1;
     bool done;
-    uint64_t excl = lookback_range<LB_W>(tst, stride, (long)tile - 1, g0, epoch, err, LB_W, &done);
+    uint64_t excl =
+        lookback_range<LB_W, PF>(tst, stride, (long)tile - 1, g0, epoch, err, LB_W, &done, pre, true);
     if (gend) st_flag(gst + (size_t)g * stride, done ? OS_INC : OS_AGG, epoch, excl + tot);
     if (done) return excl;
-    excl += lookback_range<LB_W>(gst, stride, g - 1, 0, epoch, err, LB_W, &done);
+    excl += lookback_range<LB_W, PF>(gst, stride, g - 1, 0, epoch, err, LB_W, &done, pre, false);
     if (gend) st_flag(gst + (size_t)g * stride, OS_INC, epoch, excl + tot);
     return excl;
 }
@@ -316,6 +335,11 @@ __global__ __launch_bounds__(pass_threads(BITS, KPT), pass_min_waves(BITS, KPT, 
     /* thread d < R: wave prefixes (in place) and the tile's count of digit d */
     const int d = threadIdx.x;
     uint32_t tot = 0;
+    /* the 8-waves-per-SIMD variant keeps its look-back window small */
+    constexpr int LBW = pass_min_waves(BITS, KPT, LB_W) > 2 ? LB_W / 2 : LB_W;
+    constexpr int PF = LBW < 16 ? LBW : 16;   /* words loaded ahead of the staging */
+    uint64_t pre[PF];
+    uint64_t *my = status + tile * R + d;
     if (d < R) {
 #pragma unroll
         for (int ww = 0; ww < NW; ww++) {
@@ -323,20 +347,15 @@ __global__ __launch_bounds__(pass_threads(BITS, KPT), pass_min_waves(BITS, KPT, 
             wcnt[ww * R + d] = (uint16_t)tot;
             tot += c;
         }
-        /* chained scan over tiles for digit d */
-        uint64_t *my = status + tile * R + d;
-        uint64_t excl = 0;
+        /* chained scan over tiles for digit d: publish this tile's count,
+         * and start loading the predecessors' words; they are used after
+         * the staging below, which hides one round trip */
         if (tile == 0) {
             st_flag(my, OS_INC, epoch, tot);
         } else {
             st_flag(my, OS_AGG, epoch, tot);
-            /* the 8-waves-per-SIMD variant keeps its look-back window small */
-        constexpr int LBW = pass_min_waves(BITS, KPT, LB_W) > 2 ? LB_W / 2 : LB_W;
-        const size_t ntiles = (n + TILE - 1) / TILE;
-        excl = lookback_grouped<LBW>(status + d, status + ntiles * R + d, R, tile, tot, epoch, err);
-            st_flag(my, OS_INC, epoch, excl + tot);
+            lookback_prefetch<PF>(status + d, R, tile, pre);
         }
-        obase[d] = gbase[d] + excl;
     }
     uint32_t all;
     uint32_t ts = block_excl_scan<uint32_t>(tot, &all, wtmp);   /* threads >= R add 0 */
@@ -352,6 +371,16 @@ __global__ __launch_bounds__(pass_threads(BITS, KPT), pass_min_waves(BITS, KPT, 
             uint32_t r = (rank2[j >> 1] >> ((j & 1) * 16)) & 0xffffu;
             stage[tstart[dd] + wcnt[w * R + dd] + r] = kk;
         }
+    }
+    if (d < R) {
+        uint64_t excl = 0;
+        if (tile != 0) {
+            const size_t ntiles = (n + TILE - 1) / TILE;
+            excl = lookback_grouped<LBW, PF>(status + d, status + ntiles * R + d, R, tile, tot, epoch,
+                                             err, pre);
+            st_flag(my, OS_INC, epoch, excl + tot);
+        }
+        obase[d] = gbase[d] + excl;
     }
     __syncthreads();
     size_t tile0 = tile * TILE;
