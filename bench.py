@@ -11,8 +11,11 @@ value = total boards over all layers / (T_generate + T_solve), per step.
 
 Also reported:
   roofline      the kernel with the largest total time in the timed steps,
-                timed with HIP events on the library's stream; algorithmic
-                bytes per launch as documented in DESIGN.md
+                timed with HIP events on the stream it runs on; achieved is
+                over its main-stream launches (the carry sort's, on the second
+                stream, share the GPU with the next step's moves and are
+                reported beside it); algorithmic bytes per launch as
+                documented in DESIGN.md
   cpu_baseline  the oracle (a pthreads C port of the reference's CPU
                 algorithm) on a bounded sample: full generation steps and
                 solves of three consecutive mid-table layers, whose outputs
@@ -220,6 +223,27 @@ def prof_summary(args):
                     r["Name"][:70].replace("|", "/"), r["Calls"], float(r["TotalDurationNs"]) / 1e6,
                     float(r["AverageNs"]) / 1e3, float(r["Percentage"]), float(r["MinNs"]) / 1e3,
                     float(r["MaxNs"]) / 1e3))
+        # per stream: the main stream is the one that runs the move kernel;
+        # the second one runs the carry sort beside the next step's moves
+        for f in _csvs(d, "_kernel_trace.csv"):
+            rows = list(csv.DictReader(open(f)))
+            main = collections.Counter(r["Stream_Id"] for r in rows if "::k_move" in r["Kernel_Name"])
+            if not main:
+                continue
+            main_id = main.most_common(1)[0][0]
+            print("\nBy stream (main = the stream running `k_move`; times from the kernel trace):\n")
+            print("| kernel | main: calls | main: avg us | second: calls | second: avg us |")
+            print("|---|---:|---:|---:|---:|")
+            for name, pat in (("sort_hist", "::k_os_hist"), ("sort_pass", "::k_os_pass"),
+                              ("unique", "::k_os_unique")):
+                acc = {True: [], False: []}
+                for r in rows:
+                    if pat in r["Kernel_Name"]:
+                        acc[r["Stream_Id"] == main_id].append(
+                            (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
+                mn, sn = acc[True], acc[False]
+                print("| %s | %d | %.2f | %d | %.2f |" % (name, len(mn), sum(mn) / max(len(mn), 1),
+                                                          len(sn), sum(sn) / max(len(sn), 1)))
         return
 
     def load(sub, name):
@@ -229,8 +253,14 @@ def prof_summary(args):
                 continue
             for r in csv.DictReader(open(f)):
                 if r["Counter_Name"] == name:
-                    out[int(r["Dispatch_Id"])] = (r["Kernel_Name"], float(r["Counter_Value"]))
+                    out[int(r["Dispatch_Id"])] = (r["Kernel_Name"], float(r["Counter_Value"]),
+                                                  r["Queue_Id"])
         return out
+
+    def main_queue(m):
+        """queue of the move kernel's dispatches (the library's main stream)"""
+        q = collections.Counter(v[2] for v in m.values() if "::k_move" in v[0])
+        return q.most_common(1)[0][0] if q else None
     F, W = load("FETCH_SIZE", "FETCH_SIZE"), load("WRITE_SIZE", "WRITE_SIZE")
     H, M = load("TCC_HIT", "TCC_HIT_sum"), load("TCC_HIT", "TCC_MISS_sum")
     calib_dir = args.pmc_calib or d
@@ -240,23 +270,33 @@ def prof_summary(args):
     kf, kw = gib_kib / cF, gib_kib / cW
     print("calibration: a 1 GiB device copy reads FETCH_SIZE %.0f KiB and WRITE_SIZE %.0f KiB; "
           "factors %.3f (fetch) and %.3f (write)\n" % (cF, cW, kf, kw))
-    print("| kernel | launches | HBM read / launch (MB) | HBM write / launch (MB) | L2 hit rate |")
-    print("|---|---:|---:|---:|---:|")
-    js = {"calibration": {"fetch_factor": kf, "write_factor": kw}, "kernels": {}}
+    qF, qW, qH = main_queue(F), main_queue(W), main_queue(H)
+    print("| kernel | launches | HBM read / launch (MB) | HBM write / launch (MB) | L2 hit rate "
+          "| main stream: launches | read (MB) | write (MB) |")
+    print("|---|---:|---:|---:|---:|---:|---:|---:|")
+    js = {"calibration": {"fetch_factor": kf, "write_factor": kw}, "kernels": {},
+          "kernels_main_stream": {}}
+
+    def per_launch(T, pat, k, q=None):
+        ids = [i for i, v in T.items() if pat in v[0] and (q is None or v[2] == q)]
+        return len(ids), (sum(T[i][1] for i in ids) * 1024 * k / len(ids) if ids else 0.0)
     for name, pat in PMC_KERNELS.items():
-        ids = [i for i, v in F.items() if pat in v[0]]
-        if not ids:
+        nl, fb = per_launch(F, pat, kf)
+        if not nl:
             continue
-        fb = sum(F[i][1] for i in ids) * 1024 * kf / len(ids)
-        wl = [i for i, v in W.items() if pat in v[0]]
-        wb = sum(W[i][1] for i in wl) * 1024 * kw / max(len(wl), 1)
+        _, wb = per_launch(W, pat, kw)
         hl = [i for i, v in H.items() if pat in v[0]]
         h = sum(H[i][1] for i in hl)
         m = sum(M[i][1] for i in hl if i in M)
         hit = h / (h + m) if h + m else float("nan")
-        print("| %s | %d | %.3f | %.3f | %.3f |" % (name, len(ids), fb / 1e6, wb / 1e6, hit))
-        js["kernels"][name] = {"launches": len(ids), "read_bytes_per_launch": fb,
+        nm, fm = per_launch(F, pat, kf, qF)
+        _, wm = per_launch(W, pat, kw, qW)
+        print("| %s | %d | %.3f | %.3f | %.3f | %d | %.3f | %.3f |" % (
+            name, nl, fb / 1e6, wb / 1e6, hit, nm, fm / 1e6, wm / 1e6))
+        js["kernels"][name] = {"launches": nl, "read_bytes_per_launch": fb,
                                "write_bytes_per_launch": wb, "l2_hit_rate": hit}
+        js["kernels_main_stream"][name] = {"launches": nm, "read_bytes_per_launch": fm,
+                                           "write_bytes_per_launch": wm}
     if args.pmc_json:
         js["workload"] = args.pmc_workload
         with open(args.pmc_json, "w") as f:
@@ -441,18 +481,26 @@ def main():
     for k in KERNELS:
         n, ms, b = cg.prof_query(k)
         kern[k] = dict(launches=n, ms=ms, bytes=b)
-    cg.prof_enable(False)
     dom = max(KERNELS, key=lambda k: kern[k]["ms"])
-    d = kern[dom]
-    avg_ms = d["ms"] / max(d["launches"], 1)
-    avg_bytes = d["bytes"] / max(d["launches"], 1)
+    # the dominant kernel's launches on the main stream: the second stream's
+    # (the carry sort) share the GPU with the next step's moves, so their
+    # event times include that contention; they are reported beside it
+    by_stream = [cg.prof_query_stream(dom, s) for s in (0, 1)]
+    cg.prof_enable(False)
+    n0, ms0, b0 = by_stream[0]
+    avg_ms = ms0 / max(n0, 1)
+    avg_bytes = b0 / max(n0, 1)
     achieved = avg_bytes / (avg_ms * 1e6) if avg_ms else 0.0
+    n1, ms1, b1 = by_stream[1]
+    second = {"launches": n1 // max(args.steps, 1), "avg_launch_us": 1e3 * ms1 / max(n1, 1),
+              "avg_algorithmic_bytes": b1 / max(n1, 1),
+              "achieved": (b1 / ms1 / 1e6) if ms1 else 0.0}
     traffic = None
     tf = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "pmc_traffic.json")
     if os.path.exists(tf):
         with open(tf) as f:
             tj = json.load(f)
-        k = tj.get("kernels", {}).get(dom)
+        k = tj.get("kernels_main_stream", {}).get(dom)
         if k and tj.get("workload") == "c2":
             traffic = k["read_bytes_per_launch"] + k["write_bytes_per_launch"]
 
@@ -469,8 +517,13 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "traffic_source": "profiles/pmc_traffic.json (rocprofv3 FETCH_SIZE / WRITE_SIZE, "
-                                       "calibrated by a 1 GiB copy)" if traffic else None,
-                     "avg_launch_us": avg_ms * 1e3, "avg_algorithmic_bytes": avg_bytes},
+                                       "calibrated by a 1 GiB copy; main-stream launches)"
+                     if traffic else None,
+                     "launches_per_step": n0 // max(args.steps, 1),
+                     "avg_launch_us": avg_ms * 1e3, "avg_algorithmic_bytes": avg_bytes,
+                     "stream": "main (the second stream's launches, which overlap the next "
+                               "step's moves, are in second_stream)",
+                     "second_stream": second},
         "tuning": tuning,
         "kernels": {k: {"launches": v["launches"] // max(args.steps, 1),
                         "ms_per_step": v["ms"] / args.steps,

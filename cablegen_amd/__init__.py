@@ -119,6 +119,8 @@ def lib():
     L.cg_prof_enable.argtypes = [C.c_int]
     L.cg_prof_query.restype = C.c_long
     L.cg_prof_query.argtypes = [C.c_char_p, F64P, F64P]
+    L.cg_prof_query_stream.restype = C.c_long
+    L.cg_prof_query_stream.argtypes = [C.c_char_p, C.c_int, F64P, F64P]
     # drop-in API (api/cablegen.h)
     L.movedir.restype = C.c_bool
     L.movedir.argtypes = [U64P, C.c_int]
@@ -377,6 +379,13 @@ def prof_query(kernel):
     """(launches, total_ms, total_algorithmic_bytes) for one kernel class."""
     ms, b = C.c_double(), C.c_double()
     n = lib().cg_prof_query(kernel.encode(), C.byref(ms), C.byref(b))
+    return n, ms.value, b.value
+
+
+def prof_query_stream(kernel, stream):
+    """prof_query for the launches of one stream (0 main, 1 carry-sort stream)."""
+    ms, b = C.c_double(), C.c_double()
+    n = lib().cg_prof_query_stream(kernel.encode(), stream, C.byref(ms), C.byref(b))
     return n, ms.value, b.value
 
 

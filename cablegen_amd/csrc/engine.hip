@@ -230,6 +230,9 @@ static hipEvent_t ev_get()
 }
 static double g_ms[PK_N], g_bytes[PK_N];
 static long g_launches[PK_N];
+/* the same per stream: 0 the main stream, 1 the second (carry sort) stream */
+static double g_ms_s[2][PK_N], g_bytes_s[2][PK_N];
+static long g_launches_s[2][PK_N];
 
 static int prof_begin(int kind, hipStream_t st = nullptr)
 {
@@ -284,6 +287,10 @@ static void prof_collect()
         g_ms[r.kind] += ms;
         g_bytes[r.kind] += r.bytes;
         g_launches[r.kind]++;
+        const int lane = (E.st2 && r.st == E.st2) ? 1 : 0;
+        g_ms_s[lane][r.kind] += ms;
+        g_bytes_s[lane][r.kind] += r.bytes;
+        g_launches_s[lane][r.kind]++;
         g_evpool.push_back(r.a);
         g_evpool.push_back(r.b);
     }
@@ -307,7 +314,10 @@ extern "C" void cg_prof_enable(int on)
 extern "C" void cg_prof_reset(void)
 {
     prof_collect();
-    for (int i = 0; i < PK_N; i++) g_ms[i] = g_bytes[i] = 0, g_launches[i] = 0;
+    for (int i = 0; i < PK_N; i++) {
+        g_ms[i] = g_bytes[i] = 0, g_launches[i] = 0;
+        for (int l = 0; l < 2; l++) g_ms_s[l][i] = g_bytes_s[l][i] = 0, g_launches_s[l][i] = 0;
+    }
 }
 
 extern "C" long cg_prof_query(const char *kernel, double *total_ms, double *total_bytes)
@@ -318,6 +328,20 @@ extern "C" long cg_prof_query(const char *kernel, double *total_ms, double *tota
             if (total_ms) *total_ms = g_ms[i];
             if (total_bytes) *total_bytes = g_bytes[i];
             return g_launches[i];
+        }
+    return -1;
+}
+
+extern "C" long cg_prof_query_stream(const char *kernel, int stream, double *total_ms,
+                                     double *total_bytes)
+{
+    prof_collect();
+    if (stream < 0 || stream > 1) return -1;
+    for (int i = 0; i < PK_N; i++)
+        if (!strcmp(kernel, PK_NAMES[i])) {
+            if (total_ms) *total_ms = g_ms_s[stream][i];
+            if (total_bytes) *total_bytes = g_bytes_s[stream][i];
+            return g_launches_s[stream][i];
         }
     return -1;
 }

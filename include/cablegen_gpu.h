@@ -208,6 +208,9 @@ void cg_prof_reset(void);
 /* kernel: "move", "spawn", "sort_hist", "sort_pass", "unique", "solve",
  * "adjoint", "lookup".  Returns the number of launches. */
 long cg_prof_query(const char *kernel, double *total_ms, double *total_bytes);
+/* the same for the launches of one stream: 0 the library's main stream, 1
+ * the second stream, which runs the carry sort beside the next step's moves */
+long cg_prof_query_stream(const char *kernel, int stream, double *total_ms, double *total_bytes);
 
 /* ---------------- settings used by the drop-in API ---------------- */
 
