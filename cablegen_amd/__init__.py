@@ -119,6 +119,7 @@ def lib():
     L.cg_prof_enable.argtypes = [C.c_int]
     L.cg_prof_query.restype = C.c_long
     L.cg_prof_query.argtypes = [C.c_char_p, F64P, F64P]
+    L.cg_prof_select.argtypes = [C.c_char_p]
     L.cg_prof_query_stream.restype = C.c_long
     L.cg_prof_query_stream.argtypes = [C.c_char_p, C.c_int, F64P, F64P]
     # drop-in API (api/cablegen.h)
@@ -380,6 +381,11 @@ def prof_query(kernel):
     ms, b = C.c_double(), C.c_double()
     n = lib().cg_prof_query(kernel.encode(), C.byref(ms), C.byref(b))
     return n, ms.value, b.value
+
+
+def prof_select(kernel=None):
+    """time only one kernel's launches (None: every kernel)"""
+    lib().cg_prof_select(kernel.encode() if kernel else None)
 
 
 def prof_query_stream(kernel, stream):

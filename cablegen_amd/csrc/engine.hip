@@ -214,6 +214,7 @@ struct ProfRec {
     double base_bytes;
 };
 static bool g_prof = false;
+static int g_prof_only = -1;   /* >= 0: only this kernel kind is timed */
 static std::vector<ProfRec> g_recs;
 static std::vector<hipEvent_t> g_evpool;
 
@@ -236,7 +237,7 @@ static long g_launches_s[2][PK_N];
 
 static int prof_begin(int kind, hipStream_t st = nullptr)
 {
-    if (!g_prof) return -1;
+    if (!g_prof || (g_prof_only >= 0 && kind != g_prof_only)) return -1;
     ProfRec r;
     r.kind = kind;
     r.st = st ? st : E.st;
@@ -309,6 +310,20 @@ extern "C" void cg_prof_enable(int on)
 {
     eng_init();
     g_prof = on != 0;
+}
+
+extern "C" int cg_prof_select(const char *kernel)
+{
+    if (!kernel || !*kernel) {
+        g_prof_only = -1;
+        return 0;
+    }
+    for (int i = 0; i < PK_N; i++)
+        if (!strcmp(kernel, PK_NAMES[i])) {
+            g_prof_only = i;
+            return 0;
+        }
+    return fail_msg("cg_prof_select: unknown kernel");
 }
 
 extern "C" void cg_prof_reset(void)
@@ -496,6 +511,7 @@ struct StepCounts {
  * layer (never more than the candidate count); a step whose spawns do not fit
  * is repeated with exact buffers */
 static int g_spawn_est_mode = 0;   /* 1: undersized on purpose (exercises the repeat) */
+static int g_carry_lane = 1;       /* stream of the carry sort: 1 the second, 0 the main */
 
 static size_t spawn_rows_estimate(const StepCounts &pc, size_t ncand)
 {
@@ -645,7 +661,7 @@ static int gen_step(const cg_rules &r, const uint64_t *P, size_t nP, const uint6
      * count stays on the device until then */
     if (ensure(nxtC, sumE + 1)) return -1;
     int hu4;
-    if (sort_enqueue(E.Cb.p, sumE, vary_of(sumE, or4, and4), nxtC.p, 6, &hu4, 1,
+    if (sort_enqueue(E.Cb.p, sumE, vary_of(sumE, or4, and4), nxtC.p, 6, &hu4, g_carry_lane,
                      cell_bound(vb, s + 4, and4)))
         return -1;
     if (vb) vb->s = s + 2;
@@ -1196,12 +1212,18 @@ extern "C" int cg_tune_set(const char *knob, int value)
         sort_set_cell_code(value);
         return 0;
     }
+    if (!strcmp(knob, "carry_stream")) {
+        if (value != 0 && value != 1) return fail_msg("carry_stream: 0 or 1");
+        g_carry_lane = value;
+        return 0;
+    }
     if (!strcmp(knob, "reset") || !strcmp(knob, "builtin")) {
         sort_table_reset();
         sort_force(0, 0, 0);
         set_dir_delta(0);
         g_spawn_est_mode = 0;
         sort_set_cell_code(1);
+        g_carry_lane = 1;
         if (!strcmp(knob, "reset") && !g_tune_path.empty()) return cg_tune_load(g_tune_path.c_str());
         return 0;
     }
@@ -1217,6 +1239,7 @@ extern "C" int cg_tune_get(const char *knob)
     if (!strcmp(knob, "dir_delta")) return get_dir_delta();
     if (!strcmp(knob, "spawn_estimate")) return g_spawn_est_mode;
     if (!strcmp(knob, "cell_code")) return sort_cell_code();
+    if (!strcmp(knob, "carry_stream")) return g_carry_lane;
     if (!strcmp(knob, "cell_code_sorts")) return (int)std::min<unsigned long long>(sort_code_count(), 0x7fffffff);
     return fail_msg("unknown tuning knob");
 }
@@ -1938,7 +1961,8 @@ extern "C" int cg_batch_generate(cg_batch *b)
         size_t n2 = nC + sumE;
         if (ensure(b->C2, sumE + 1)) return -1;
         int hu4;
-        if (sort_enqueue(E.Cb.p, sumE, vary_of(sumE, or4, and4), b->C2.p, 6, &hu4, 1, batch_cells))
+        if (sort_enqueue(E.Cb.p, sumE, vary_of(sumE, or4, and4), b->C2.p, 6, &hu4, g_carry_lane,
+                         batch_cells))
             return -1;
         /* directories of every table of this step */
         st.n = nM;

@@ -757,8 +757,10 @@ static OsScratch carve(void *scratch)
 static void launch_hist_scan(hipStream_t st, int bits, const uint64_t *keys, size_t n, const SortCode &c,
                              int npass, const OsScratch &s, unsigned *err)
 {
+    /* at most two blocks per CU (64 KiB of counters each): each block then
+     * clears and flushes its counters once per 8192-key iteration at most */
     size_t hb = (n + 8191) / 8192;
-    if (hb > 1024) hb = 1024;
+    if (hb > 512) hb = 512;
     int h = prof_hook_begin(2, st);
     switch (bits) {
     case 9:

@@ -165,12 +165,14 @@ int cg_dev_sync(void);
  *                layer, a step that overflows them is repeated with exact
  *                buffers; 1: sized for half the previous layer, so that
  *                growing layers take the repeat (for tests);
+ *   "carry_stream"  1: the carry sort runs on a second stream beside the
+ *                next step's moves; 0: on the main stream after the spawns;
  *   "cell_code"  1: a generation's sorts code pairs of varying cells as
  *                a * base + b when the layer's tile sum bounds the cell
  *                values and that saves a radix pass; 0: bit packing only;
  *   "builtin"    built-in defaults: 8-bit digits, 8192-key tiles, window 16
  *                for every entry, no forced field, delta 0, spawn_estimate 0,
- *                cell_code 1;
+ *                cell_code 1, carry_stream 1;
  *   "reset"      the state after initialisation: the built-in defaults, then
  *                the table loaded at initialisation if there was one.
  * Returns 0, or -1 for an unknown knob or value.  cg_tune_get also reads
@@ -208,6 +210,9 @@ void cg_prof_reset(void);
 /* kernel: "move", "spawn", "sort_hist", "sort_pass", "unique", "solve",
  * "adjoint", "lookup".  Returns the number of launches. */
 long cg_prof_query(const char *kernel, double *total_ms, double *total_bytes);
+/* times only the launches of one kernel (names as for cg_prof_query; NULL or
+ * "" times every kernel): each timed launch records two events */
+int cg_prof_select(const char *kernel);
 /* the same for the launches of one stream: 0 the library's main stream, 1
  * the second stream, which runs the carry sort beside the next step's moves */
 long cg_prof_query_stream(const char *kernel, int stream, double *total_ms, double *total_bytes);
