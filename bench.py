@@ -15,8 +15,10 @@ Also reported:
                 HIP events on the stream it runs on; achieved is
                 over its main-stream launches (the carry sort's, on the second
                 stream, share the GPU with the next step's moves and are
-                reported beside it); algorithmic bytes per launch as
-                documented in DESIGN.md
+                reported beside it); algorithmic bytes per launch: the
+                payload read once and written once (a radix pass 16 B per
+                key, the histogram 8 B, the compaction 8 B per key + 8 B per
+                unique key, the move kernel 8 B per board + 8 B per candidate)
   cpu_baseline  the oracle (a pthreads C port of the reference's CPU
                 algorithm) on a bounded sample: full generation steps and
                 solves of three consecutive mid-table layers, whose outputs
@@ -411,6 +413,61 @@ def batch_bench(args):
     print(json.dumps(out))
 
 
+def dropin_bench(args):
+    """C2 through the reference's entry points (api/cablegen.h generate() and
+    solve()): layers and tables are written to and read from files in
+    args.dropin, as the reference does.  Reports end-to-end boards/s, the
+    files' bytes, and how much of the time is not GPU work."""
+    import ctypes as C
+    import shutil
+    import numpy as np
+    import cablegen_amd as cg
+    d = os.path.abspath(args.dropin)
+    os.makedirs(d + "/b", exist_ok=True)
+    os.makedirs(d + "/t", exist_ok=True)
+    conf = d + "/c2.conf"
+    with open(conf, "w") as f:
+        f.write("[Cablegen]\nignore_f = true\n[Generate]\nprune = true\nstsl = 58\nltc = 5\n"
+                "smallest_large = 6\ndir = %s/b/\nend = %d\n[Solve]\ndir = %s/t/\nend = 0\n"
+                % (d, END_GEN, d))
+    L = cg.lib()
+    L.change_config(conf.encode())
+    tuning = run_autotune(cg, args)
+    init = np.array(INITIAL, np.uint64)
+    win = np.array(WINSTATES, np.uint64)
+    sa = cg.SArr(True, init.ctypes.data_as(cg.U64P), len(init))
+    wa = cg.SArr(True, win.ctypes.data_as(cg.U64P), len(win))
+    start = L.get_sum(INITIAL[0])
+    free0 = shutil.disk_usage(d).free
+    t0 = time.perf_counter()
+    L.generate(start, END_GEN, ("%s/b/%%d.boards" % d).encode(), C.byref(sa))
+    t1 = time.perf_counter()
+    bbytes = sum(os.path.getsize(os.path.join(d, "b", f)) for f in os.listdir(d + "/b"))
+    out = {"metric": "boards/s of generate() + solve() through api/cablegen.h, files included",
+           "workload": "DPDGap-512 (the bench's C2) written to / read from %s" % d,
+           "generate_s": t1 - t0, "boards_file_bytes": bbytes, "free_disk_bytes_before": free0,
+           "tuning": tuning}
+    if not args.dropin_generate_only:
+        t2 = time.perf_counter()
+        L.solve(END_GEN, 0, ("%s/b/%%d.boards" % d).encode(), ("%s/t/%%d.tables" % d).encode(),
+                C.byref(wa))
+        t3 = time.perf_counter()
+        out["solve_s"] = t3 - t2
+        out["tables_file_bytes"] = sum(os.path.getsize(os.path.join(d, "t", f))
+                                       for f in os.listdir(d + "/t"))
+    n = 0
+    for f in os.listdir(d + "/b"):
+        n += os.path.getsize(os.path.join(d, "b", f)) // 8
+    out["boards"] = n
+    total_t = out["generate_s"] + out.get("solve_s", 0.0)
+    out["value"] = n / total_t
+    out["unit"] = "boards/s"
+    if not args.dropin_keep:
+        shutil.rmtree(d + "/b", ignore_errors=True)
+        shutil.rmtree(d + "/t", ignore_errors=True)
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -421,6 +478,10 @@ def main():
     ap.add_argument("--micro", action="store_true", help="kernel roofline microbench instead")
     ap.add_argument("--micro-log2", type=int, default=30, help="log2 of the microbench board count")
     ap.add_argument("--batch", action="store_true", help="batched-tables benchmark instead")
+    ap.add_argument("--dropin", default=None,
+                    help="C2 through generate()/solve() with files in this directory instead")
+    ap.add_argument("--dropin-generate-only", action="store_true")
+    ap.add_argument("--dropin-keep", action="store_true", help="keep the written files")
     ap.add_argument("--batch-tables", type=int, default=64)
     ap.add_argument("--batch-lo", type=int, default=5, help="end = start + 2*U{lo..hi}")
     ap.add_argument("--batch-hi", type=int, default=20)
@@ -440,8 +501,8 @@ def main():
                     help="1: layer sorts code pairs of cells when the tile sum bounds them (default)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the multi-GPU sharding is described in DESIGN.md)", file=sys.stderr)
+        print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
+              "(layers would shard by key range across GPUs, which is not built)", file=sys.stderr)
         sys.exit(2)
 
     if args.prof_summary:
@@ -452,6 +513,9 @@ def main():
         return
     if args.batch:
         batch_bench(args)
+        return
+    if args.dropin:
+        dropin_bench(args)
         return
 
     import cablegen_amd as cg

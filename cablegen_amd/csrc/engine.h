@@ -32,6 +32,10 @@ const char *cge_last_error(void);
  * (65536 entries); right moves are derived by nibble reversal. */
 void cge_build_row_table(int ignore_f, uint16_t *tab);
 
+/* pinned (page-locked) host memory for layer transfers; NULL on failure */
+void *cge_host_alloc(size_t bytes);
+void cge_host_free(void *p);
+
 /* ---- streaming generation (drives generate()) ---- */
 int cge_gen_begin(const cge_rules *r, const uint64_t *initial, size_t n);
 int cge_gen_step(size_t *n_layer);                 /* M_s for the next s */

@@ -1617,6 +1617,22 @@ extern "C" int cge_gen_step(size_t *n_layer)
     return 0;
 }
 
+extern "C" void *cge_host_alloc(size_t bytes)
+{
+    if (eng_init()) return nullptr;
+    void *p = nullptr;
+    if (hipHostMalloc(&p, bytes ? bytes : 8, hipHostMallocDefault) != hipSuccess) {
+        fail_msg("pinned host allocation failed");
+        return nullptr;
+    }
+    return p;
+}
+
+extern "C" void cge_host_free(void *p)
+{
+    if (p) (void)hipHostFree(p);
+}
+
 extern "C" int cge_gen_fetch(uint64_t *host, size_t n)
 {
     if (n > G.nM) return fail_msg("cge_gen_fetch: too many boards");

@@ -626,30 +626,99 @@ void read_table(table *t, const char *path)
 }
 
 /* ------------------------------------------------------------------------ */
-/* generate() on the GPU, writes overlapped with the next layer              */
+/* generate() and solve() on the GPU, file I/O overlapped with the GPU work  */
 /* ------------------------------------------------------------------------ */
+
+/* Layers move between the GPU and files through a ring of slots in pinned
+ * host memory: a slot is filled by a device copy (or a file read) and
+ * emptied by a writer thread, so several files are written at once while the
+ * GPU works on the next layers.  Buffers are (re)allocated on the calling
+ * thread only, after the slot's previous writer has finished. */
+#define IO_SLOTS 6
 
 typedef struct {
     pthread_t th;
     int busy;
-    uint64_t *buf;
-    size_t n;
-    const char *fmt;
-    int layer;
-} writer_t;
+    uint64_t *k;     /* keys (pinned) */
+    double *v;       /* values (pinned; solve only) */
+    size_t cap, n;
+    char *path;
+    int ok;
+} io_slot_t;
 
-static void *writer_main(void *arg)
+static void *board_writer(void *arg)
 {
-    writer_t *w = arg;
-    write_boards((static_arr_info){ .valid = true, .bp = w->buf, .size = w->n }, w->fmt, w->layer);
+    io_slot_t *w = arg;
+    FILE *f = fopen(w->path, "wb");
+    if (!f) {
+        warnf("Couldn't write to %s!", w->path);
+        return NULL;
+    }
+    if (w->n) fwrite(w->k, sizeof(uint64_t), w->n, f);
+    fclose(f);
     return NULL;
 }
 
-static void writer_wait(writer_t *w)
+static void *table_writer(void *arg)
+{
+    io_slot_t *w = arg;
+    write_table_file(w->k, w->v, w->n, w->path);
+    return NULL;
+}
+
+/* reads a %d.boards file into the slot (capacity ensured by the caller) */
+static void *board_reader(void *arg)
+{
+    io_slot_t *w = arg;
+    w->ok = 0;
+    FILE *f = fopen(w->path, "rb");
+    if (!f) return NULL;
+    if (w->n && fread(w->k, sizeof(uint64_t), w->n, f) != w->n) {
+        fclose(f);
+        return NULL;
+    }
+    fclose(f);
+    w->ok = 1;
+    if (get_settings()->delete_boards) remove(w->path);
+    return NULL;
+}
+
+static void slot_join(io_slot_t *w)
 {
     if (w->busy) {
         pthread_join(w->th, NULL);
         w->busy = 0;
+    }
+}
+
+static void slot_run(io_slot_t *w, void *(*fn)(void *))
+{
+    if (pthread_create(&w->th, NULL, fn, w)) {
+        fn(w);
+    } else {
+        w->busy = 1;
+    }
+}
+
+/* capacity for n keys (and values when with_vals) */
+static void slot_reserve(io_slot_t *w, size_t n, int with_vals, const char *what)
+{
+    if (n <= w->cap && w->k) return;
+    cge_host_free(w->k);
+    cge_host_free(w->v);
+    w->cap = n + n / 4 + 16;
+    w->k = cge_host_alloc(w->cap * sizeof(uint64_t));
+    w->v = with_vals ? cge_host_alloc(w->cap * sizeof(double)) : NULL;
+    if (!w->k || (with_vals && !w->v)) die(what);
+}
+
+static void slots_release(io_slot_t *w)
+{
+    for (int i = 0; i < IO_SLOTS; i++) {
+        slot_join(&w[i]);
+        cge_host_free(w[i].k);
+        cge_host_free(w[i].v);
+        free(w[i].path);
     }
 }
 
@@ -658,40 +727,52 @@ void generate(const int start, const int end, const char *fmt, const static_arr_
     cge_rules r = engine_rules();
     if (get_settings()->premove) warnf("premove is not supported (its reference path is undefined); ignored");
     if (cge_gen_begin(&r, initial->bp, initial->size)) die("generate");
-    writer_t w[2];
+    io_slot_t w[IO_SLOTS];
     memset(w, 0, sizeof w);
-    size_t cap[2] = { 0, 0 };
     int k = 0;
     for (int s = start; s <= end; s += 2) {
         size_t n = 0;
         if (cge_gen_step(&n)) die("generate");
-        writer_wait(&w[k]);
-        if (n > cap[k]) {
-            free(w[k].buf);
-            cap[k] = n + n / 4 + 16;
-            w[k].buf = malloc_errcheck(cap[k] * sizeof(uint64_t));
-        }
-        if (cge_gen_fetch(w[k].buf, n)) die("generate");
-        w[k].n = n;
-        w[k].fmt = fmt;
-        w[k].layer = s;
-        if (pthread_create(&w[k].th, NULL, writer_main, &w[k])) {
-            writer_main(&w[k]);
-        } else {
-            w[k].busy = 1;
-        }
-        k ^= 1;
+        io_slot_t *x = &w[k];
+        slot_join(x);
+        slot_reserve(x, n, 0, "generate");
+        if (cge_gen_fetch(x->k, n)) die("generate");
+        x->n = n;
+        free(x->path);
+        x->path = format_str(fmt, s);
+        slot_run(x, board_writer);
+        k = (k + 1) % IO_SLOTS;
     }
-    writer_wait(&w[0]);
-    writer_wait(&w[1]);
-    free(w[0].buf);
-    free(w[1].buf);
+    slots_release(w);
     cge_gen_end();
 }
 
-/* ------------------------------------------------------------------------ */
-/* solve() on the GPU                                                        */
-/* ------------------------------------------------------------------------ */
+static long path_size(const char *path)
+{
+    struct stat st;
+    if (stat(path, &st)) return -1;
+    return (long)st.st_size;
+}
+
+/* starts reading layer file `path` into slot x; a missing file reads as an
+ * empty layer (x->ok = 0) */
+static void start_read(io_slot_t *x, const char *path)
+{
+    slot_join(x);
+    free(x->path);
+    x->path = strdup(path);
+    long sz = path_size(path);
+    if (sz < 0) {
+        warnf("Couldn't read %s!", path);
+        x->n = 0;
+        x->ok = 0;
+        return;
+    }
+    if (sz % 8) warnf("sz %%8 != 0, this is probably not a real table!");
+    x->n = (size_t)sz / 8;
+    slot_reserve(x, x->n, 1, "solve");
+    slot_run(x, board_reader);
+}
 
 void solve(unsigned start, unsigned end, char *posfmt, char *tablefmt, const static_arr_info *winstates)
 {
@@ -707,20 +788,38 @@ void solve(unsigned start, unsigned end, char *posfmt, char *tablefmt, const sta
     }
     if (cge_solve_begin(&r, img, 8 * nw, 0.1)) die("solve");
     free(img);
-    char name[4096];
+    /* the layers, top down, as the reference walks them */
+    size_t nl = 0;
     for (unsigned s = start; s >= end && s; s -= 2) {
-        snprintf(name, sizeof name, posfmt, s);
-        static_arr_info keys = read_boards(name);
-        if (get_settings()->delete_boards) remove(name);
-        size_t n = keys.valid ? keys.size : 0;
-        double *vals = malloc_errcheck((n ? n : 1) * sizeof(double));
-        if (cge_solve_step(keys.bp, n, vals)) die("solve");
-        snprintf(name, sizeof name, tablefmt, s);
-        write_table_file(keys.bp, vals, n, name);
-        free(vals);
-        free(keys.bp);
+        nl++;
         if (s < 2) break;
     }
+    io_slot_t w[IO_SLOTS];
+    memset(w, 0, sizeof w);
+    char name[4096];
+    if (nl) {
+        snprintf(name, sizeof name, posfmt, start);
+        start_read(&w[0], name);
+    }
+    for (size_t i = 0; i < nl; i++) {
+        const unsigned s = start - 2 * (unsigned)i;
+        io_slot_t *x = &w[i % IO_SLOTS];
+        /* x: read of layer s (started one layer earlier) */
+        if (x->busy) slot_join(x);
+        if (!x->ok) x->n = 0;
+        if (i + 1 < nl) {
+            /* prefetch the next layer's file while this one is on the GPU */
+            snprintf(name, sizeof name, posfmt, s - 2);
+            start_read(&w[(i + 1) % IO_SLOTS], name);
+        }
+        slot_reserve(x, x->n, 1, "solve");
+        if (cge_solve_step(x->k, x->n, x->v)) die("solve");
+        snprintf(name, sizeof name, tablefmt, s);
+        free(x->path);
+        x->path = strdup(name);
+        slot_run(x, table_writer);
+    }
+    slots_release(w);
     cge_solve_end();
 }
 

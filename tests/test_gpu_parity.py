@@ -404,6 +404,19 @@ def test_tablebase_vs_oracle(cg, name):
     tb.close()
 
 
+@pytest.mark.parametrize("name", ["c1", "dpdgap_prune"])
+def test_tablebase_carry_sort_on_main_stream(cg, name):
+    """The carry sort scheduled on the main stream (carry_stream 0) instead of
+    the second one gives the same tables."""
+    try:
+        cg.tune_set("carry_stream", 0)
+        assert cg.tune_get("carry_stream") == 0
+        test_tablebase_vs_oracle(cg, name)
+    finally:
+        cg.tune_set("reset", 0)
+    assert cg.tune_get("carry_stream") == 1
+
+
 @pytest.mark.parametrize("name", ["c1", "dpdgap_prune", "3x4"])
 def test_tablebase_spawn_buffer_repeat(cg, name):
     """Undersized spawn buffers: every growing step overflows them and is
