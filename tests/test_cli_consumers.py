@@ -1,0 +1,103 @@
+"""The reference's own CLI (frontends/cli/main.c, unchanged, linked to this
+library) as a consumer of tables: its `lookup` command reads the `%d.tables`
+files of the board's sum, sum + 2 and sum + 4 and prints, for every empty
+cell and spawn, the best move and its probability (`best()`, main.c:174-189:
+a move wins only when strictly better, starting from left at 0).
+
+The tables here are written by the oracle, so the test runs on the CPU: it
+pins the library's host `read_table`, `lookup`, `movedir` and `get_sum` as the
+CLI uses them.  The expected lines come from a restatement of `best()` over the
+oracle's moves and canonical forms."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from tests import oracle_lib as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CLI = os.path.join(ROOT, "oracle", "_ref", "cablegen_cli_mi355x")
+NAMES = ("left", "right", "up", "down")
+
+pytestmark = pytest.mark.skipif(not os.path.exists(CLI), reason="reference CLI not built")
+
+
+def tile(b, i):
+    return (b >> (60 - 4 * i)) & 0xF
+
+
+def set_tile(b, i, v):
+    sh = 60 - 4 * i
+    return (b & ~(0xF << sh)) | (v << sh)
+
+
+def table_lookup(b, K, V):
+    c = O.canon(b)
+    i = int(np.searchsorted(K, np.uint64(c)))
+    return float(V[i]) if i < len(K) and int(K[i]) == c else 0.0
+
+
+def best(cfg, b, K, V):
+    """main.c:174-189: the first move whose value is strictly larger wins"""
+    maxd, maxp = 0, 0.0
+    for d in range(4):
+        ok, t = O.move(cfg, b, d)
+        if not ok:
+            continue
+        p = table_lookup(t, K, V)
+        if p > maxp:
+            maxd, maxp = d, p
+    return maxd, maxp
+
+
+@pytest.fixture(scope="module")
+def tables(tmp_path_factory):
+    d = tmp_path_factory.mktemp("cli")
+    (d / "t").mkdir()
+    cfg = O.make_cfg(ignore_f=True, cores=8)
+    start, end = 6, 80
+    initial = [0x1200000000000000 | 0xFFFF]
+    layers = O.generate(cfg, initial, start, end)
+    vals = O.solve(cfg, layers, start, end, 0, [0x6000000000000000])
+    tabs = {}
+    for i, (k, v) in enumerate(zip(layers, vals)):
+        s = start + 2 * i
+        if v is None:
+            continue
+        k = np.ascontiguousarray(k, np.uint64)
+        v = np.ascontiguousarray(v, np.float64)
+        with open(d / "t" / ("%d.tables" % s), "wb") as f:
+            f.write(k.tobytes())
+            f.write(v.tobytes())
+        tabs[s] = (k, v)
+    with open(d / "c.conf", "w") as f:
+        f.write("[Cablegen]\nignore_f = true\n[Solve]\ndir = %s/\n" % (d / "t"))
+    return d, cfg, tabs
+
+
+def test_lookup_command_best_moves(tables):
+    d, cfg, tabs = tables
+    rng = np.random.default_rng(3)
+    checked = nonzero = 0
+    for s in (62, 68, 74):
+        K, _ = tabs[s]
+        K2, V2 = tabs[s + 2]
+        K4, V4 = tabs[s + 4]
+        for b in K[rng.integers(0, len(K), size=6)]:
+            b = int(b)
+            exp = []
+            for i in range(16):
+                if tile(b, i):
+                    continue
+                for v, (Kt, Vt) in ((1, (K2, V2)), (2, (K4, V4))):
+                    md, mp = best(cfg, set_tile(b, i, v), Kt, Vt)
+                    exp.append("Best move: %s (%.10f)" % (NAMES[md], mp))
+                    nonzero += mp > 0
+            r = subprocess.run([CLI, "-C=%s" % (d / "c.conf"), "lookup", "%016x" % b], cwd=str(d),
+                               capture_output=True, text=True, timeout=60)
+            assert r.returncode == 0, r.stdout + r.stderr
+            got = [l.strip() for l in r.stdout.splitlines() if l.startswith("Best move:")]
+            assert got == exp, hex(b)
+            checked += 1
+    assert checked == 18 and nonzero > 20
