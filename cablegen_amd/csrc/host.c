@@ -672,9 +672,13 @@ static void *board_reader(void *arg)
     io_slot_t *w = arg;
     w->ok = 0;
     FILE *f = fopen(w->path, "rb");
-    if (!f) return NULL;
+    if (!f) {
+        warnf("Couldn't read %s!", w->path);
+        return NULL;
+    }
     if (w->n && fread(w->k, sizeof(uint64_t), w->n, f) != w->n) {
         fclose(f);
+        warnf("Couldn't read %s!", w->path);
         return NULL;
     }
     fclose(f);

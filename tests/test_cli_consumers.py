@@ -101,3 +101,47 @@ def test_lookup_command_best_moves(tables):
             assert got == exp, hex(b)
             checked += 1
     assert checked == 18 and nonzero > 20
+
+
+def test_lookup_spawn_command(tables):
+    """`lookup_spawn BOARD` (main.c:191-246 with spawn = true): the first cell
+    holding at most 2 (an empty cell counts) is cleared, and only the spawn
+    that rebuilds the given board prints its best move."""
+    d, cfg, tabs = tables
+    rng = np.random.default_rng(4)
+    checked = 0
+    for s in (62, 68, 74):
+        K, _ = tabs[s]
+        K2, V2 = tabs[s + 2]
+        K4, V4 = tabs[s + 4]
+        for b in K[rng.integers(0, len(K), size=4)]:
+            # a board of sum s + 2 or s + 4: a tile 2 or 4 placed in an empty cell of b
+            b = int(b)
+            empty = [i for i in range(16) if not tile(b, i)]
+            i0 = empty[int(rng.integers(0, len(empty)))]
+            v0 = 1 + int(rng.integers(0, 2))
+            q = set_tile(b, i0, v0)
+            # the command clears the first cell with a value <= 2, then tries every spawn
+            base = q
+            for i in range(16):
+                if tile(q, i) <= 2:
+                    base = set_tile(q, i, 0)
+                    break
+            s0 = sum(1 << tile(base, i) for i in range(16) if 0 < tile(base, i) < 0xE)
+            if s0 != s:
+                continue   # the cleared cell was not the spawn: other tables
+            exp = []
+            for i in range(16):
+                if tile(base, i):
+                    continue
+                for v, (Kt, Vt) in ((1, (K2, V2)), (2, (K4, V4))):
+                    if set_tile(base, i, v) == q:
+                        md, mp = best(cfg, q, Kt, Vt)
+                        exp.append("Best move: %s (%.10f)" % (NAMES[md], mp))
+            r = subprocess.run([CLI, "-C=%s" % (d / "c.conf"), "lookup_spawn", "%016x" % q],
+                               cwd=str(d), capture_output=True, text=True, timeout=60)
+            assert r.returncode == 0, r.stdout + r.stderr
+            got = [l.strip() for l in r.stdout.splitlines() if l.startswith("Best move:")]
+            assert got == exp, hex(q)
+            checked += 1
+    assert checked >= 4
