@@ -105,38 +105,24 @@ CG_HD uint64_t cg_canon(uint64_t b)
     return m;
 }
 
-/* nibble reversal of a 16-bit row (right move = rev o left o rev) */
-CG_HD unsigned cg_row_rev(unsigned r)
-{
-    return ((r & 0xfu) << 12) | ((r & 0xf0u) << 4) | ((r >> 4) & 0xf0u) | (r >> 12);
-}
-
-/* Lock rule (board.c:29-38, 117): a row move is allowed unless a 0xF cell was
- * vacated, or always with free_formation. */
-CG_HD int cg_row_locked(unsigned pre, unsigned post, int free_formation)
-{
-    uint64_t fp = cg_fmask(pre), fq = cg_fmask(post);
-    return free_formation || (fp & ~fq) == 0;
-}
-
 /* One horizontal board move using a left-move row table `L` (65536 u16).
- * dir 0 = left, 1 = right.  Returns 1 iff valid (some row changed and every
- * row is locked, movedir_hori board.c:151-169). */
+ * dir 0 = left, 1 = right.  Returns 1 iff valid (movedir_hori
+ * board.c:151-169): some row changed and every row is locked -- the lock rule
+ * (board.c:29-38, 117) allows a row move unless a 0xF cell was vacated, or
+ * always with free_formation. */
 CG_HD int cg_move_hori_tab(const uint16_t *L, uint64_t *b, int dir, int ff)
 {
-    uint64_t pre = *b, out = 0;
-    int changed = 0, ok = 1;
+    /* a right move is a left move of the mirrored board, mirrored back; the
+     * changed and lock tests of the four rows are tests on the whole board */
+    const uint64_t pre = *b;
+    const uint64_t src = dir ? cg_mirror_rows(pre) : pre;
+    uint64_t out = 0;
 #if defined(__HIP_DEVICE_COMPILE__)
 #pragma unroll
 #endif
-    for (int i = 0; i < 4; i++) {
-        unsigned row = (unsigned)(pre >> (16 * i)) & 0xffffu;
-        unsigned nr = dir ? cg_row_rev(L[cg_row_rev(row)]) : L[row];
-        changed |= nr != row;
-        ok &= cg_row_locked(row, nr, ff);
-        out |= (uint64_t)nr << (16 * i);
-    }
-    if (ok && changed) { *b = out; return 1; }
+    for (int i = 0; i < 4; i++) out |= (uint64_t)L[(src >> (16 * i)) & 0xffffu] << (16 * i);
+    if (dir) out = cg_mirror_rows(out);
+    if (out != pre && (ff || (cg_fmask(pre) & ~cg_fmask(out)) == 0)) { *b = out; return 1; }
     return 0;
 }
 
@@ -155,27 +141,47 @@ CG_HD int cg_move_tab(const uint16_t *L, uint64_t *b, int dir, int ff)
  * there are more than ltc large tiles. */
 CG_HD int cg_prune(uint64_t b, long stsl, long ltc, long sl)
 {
-    int large = 0, sts = 0, bad = 0;
-    unsigned cnt_lo = 0, cnt_hi = 0;   /* 4-bit counters: values 0-7 / 8-15 */
+    /* H: a 4-bit count of every value v at bits [4v, 4v+3].  A count reaches
+     * 16 (and carries into the next counter) only on a board whose 16 cells
+     * are equal, which is handled at the end. */
+    uint64_t H = 0;
+    /* 4 * value of the even / odd cells, one per byte (the shift amounts) */
+    const uint64_t ev = (b & 0x0F0F0F0F0F0F0F0FULL) << 2, od = (b >> 2) & 0x3C3C3C3C3C3C3C3CULL;
 #if defined(__HIP_DEVICE_COMPILE__)
 #pragma unroll
 #endif
-    for (int x = 0; x < 16; x++) {
-        int v = (int)(b >> (60 - 4 * x)) & 0xf;
-        if (v >= sl && v < 0xe) {
-            large++;
-            if (v < 8) cnt_lo += 1u << (4 * v);
-            else cnt_hi += 1u << (4 * (v - 8));
-        } else if (v < 0xe) {
-            sts += 1 << v;
-        }
+    for (int k = 0; k < 8; k++) {
+        H += 1ULL << ((ev >> (8 * k)) & 63);
+        H += 1ULL << ((od >> (8 * k)) & 63);
     }
-#if defined(__HIP_DEVICE_COMPILE__)
-#pragma unroll
-#endif
-    for (int v = 0; v < 14; v++) {
-        unsigned c = v < 8 ? (cnt_lo >> (4 * v)) & 0xf : (cnt_hi >> (4 * (v - 8))) & 0xf;
-        bad |= (v == sl) ? (c > 5) : (c > 2);
+    /* value classes: small v < slc, large slc <= v < 0xE; slc = sl clamped to [0, 14] */
+    const long slc = sl < 0 ? 0 : (sl > 14 ? 14 : sl);
+    const uint64_t small_m = (1ULL << (4 * slc)) - 1ULL;
+    const uint64_t large_m = ((1ULL << 56) - 1ULL) & ~small_m;
+    const uint64_t sl_bit = (sl >= 0 && sl < 14) ? 1ULL << (4 * sl) : 0;
+    /* bit 0 of each counter: count >= 3, count >= 6 */
+    const uint64_t ge3 = ((H >> 3) | (H >> 2) | ((H >> 1) & H)) & CG_NIB_LO;
+    const uint64_t ge6 = ((H >> 3) | ((H >> 2) & (H >> 1))) & CG_NIB_LO;
+    int bad = ((ge3 & large_m & ~sl_bit) | (ge6 & sl_bit)) != 0;
+    uint64_t t = H & large_m;
+    t = (t & 0x0F0F0F0F0F0F0F0FULL) + ((t >> 4) & 0x0F0F0F0F0F0F0F0FULL);
+    uint32_t t32 = (uint32_t)t + (uint32_t)(t >> 32);
+    t32 += t32 >> 16;
+    int large = (int)((t32 + (t32 >> 8)) & 0xff);
+    /* small-tile sum: sum over v < slc of count(v) * 2^v, folding the counter
+     * digits pairwise (fields of 8, 16, 32 bits, weights 2, 4, 16, 256) */
+    uint64_t hs = H & small_m;
+    hs = (hs & 0x0F0F0F0F0F0F0F0FULL) + 2 * ((hs >> 4) & 0x0F0F0F0F0F0F0F0FULL);
+    hs = (hs & 0x00FF00FF00FF00FFULL) + 4 * ((hs >> 8) & 0x00FF00FF00FF00FFULL);
+    hs = (hs & 0x0000FFFF0000FFFFULL) + 16 * ((hs >> 16) & 0x0000FFFF0000FFFFULL);
+    int sts = (int)((uint32_t)hs + ((uint32_t)(hs >> 32) << 8));
+    /* a board of 16 equal cells u */
+    const uint64_t u = b & 0xf;
+    if (b == u * CG_NIB_LO) {
+        const int is_large = (long)u >= sl && u < 0xe;
+        large = is_large ? 16 : 0;
+        bad = is_large;
+        sts = (!is_large && u < 0xe) ? 16 << u : 0;
     }
     return bad || sts > stsl + 64 || large > ltc;
 }

@@ -83,6 +83,44 @@ def test_move_kernel(cg, ff, ig, prune):
         assert np.array_equal(got, np.sort(np.array(exp, dtype=np.uint64)))
 
 
+@pytest.mark.parametrize("stsl,ltc,sl", [(58, 5, 6), (0, 2, 3), (300, 16, 0), (100, 8, 13),
+                                         (20, 4, 14), (2000, 16, 15), (1000, 16, -1)])
+def test_move_kernel_prune_parameters(cg, stsl, ltc, sl):
+    """prune_board (generate.c:98-131) over its parameter range: a value equal
+    to smallest_large may occur 5 times, other large values twice; boards
+    with 16 equal cells, 0xE tiles and 16 large tiles included."""
+    rng = np.random.default_rng(100 + sl)
+    cfg = O.make_cfg(ignore_f=1, prune=1, stsl=stsl, ltc=ltc, smallest_large=sl)
+    r = cg.rules(ignore_f=1, prune=1, stsl=stsl, ltc=ltc, smallest_large=sl)
+    b = np.concatenate([
+        rand_boards(rng, 0, 3000, p0=0.2, kmax=13),
+        rand_boards(rng, 0x000000000000FFFF, 1000, p0=0.1, kmax=14),
+        np.array([u * 0x1111111111111111 for u in range(16)], dtype=np.uint64),
+        # 15 equal cells and one different one, at every position
+        np.array([(u * 0x1111111111111111) ^ (w << (4 * x)) for u in (0, 1, 6, 13)
+                  for w in (1, 3) for x in range(16)], dtype=np.uint64),
+        # five / six copies of one value among small tiles
+        np.array([int(sum(v << (4 * x) for x, v in enumerate([k] * c + [1, 2] * 4)))
+                  for k in (3, 6, 7, 13) for c in (2, 3, 5, 6)], dtype=np.uint64)])
+    exp, pruned = [], 0
+    for x in b:
+        x = int(x)
+        if O.orc().orc_prune(x, stsl, ltc, sl):
+            pruned += 1
+            continue
+        for d in range(4):
+            ok, m = O.move(cfg, x, d)
+            if ok and not O.orc().orc_prune(m, stsl, ltc, sl):
+                exp.append(O.canon(m))
+    assert 0 < pruned < len(b)
+    dout = torch.empty(4 * len(b) + 1, dtype=torch.int64, device="cuda")
+    n = C.c_size_t()
+    assert cg.lib().cg_dev_move(C.byref(r), dev(b).data_ptr(), len(b), dout.data_ptr(),
+                                C.byref(n)) == 0
+    got = np.sort(host(dout, n.value))
+    assert np.array_equal(got, np.sort(np.array(exp, dtype=np.uint64)))
+
+
 def test_spawn_kernel(cg):
     rng = np.random.default_rng(7)
     for fm in FORMATIONS:
