@@ -92,16 +92,19 @@ __global__ __launch_bounds__(MV_THREADS, DEDUPE ? 8 : 1) void k_move(
     }
     if (dn && *dn < n) n = (size_t)*dn;   /* count produced on the device */
     const size_t stride = (size_t)gridDim.x * MV_THREADS;
-    for (size_t b0 = (size_t)blockIdx.x * MV_THREADS; b0 < n; b0 += stride) {
+    /* the next chunk's board is loaded while this chunk is processed */
+    size_t i = (size_t)blockIdx.x * MV_THREADS + threadIdx.x;
+    uint64_t nb = i < n ? in[i] : 0;
+    for (size_t b0 = (size_t)blockIdx.x * MV_THREADS; b0 < n; b0 += stride, i += stride) {
         if (DEDUPE) {
-            for (int i = threadIdx.x; i < MV_HASH; i += MV_THREADS) smem[i] = 0ULL;
+            for (int h = threadIdx.x; h < MV_HASH; h += MV_THREADS) smem[h] = 0ULL;
             __syncthreads();
         }
-        size_t i = b0 + threadIdx.x;
+        const uint64_t b = nb;
+        nb = i + stride < n ? in[i + stride] : 0;
         uint64_t res[4];
         uint32_t k = 0;
         if (i < n) {
-            uint64_t b = in[i];
             if (!(r.prune && cg_prune(b, r.stsl, r.ltc, r.sl))) {
                 uint64_t y = cg_rot_cw(b);
 #pragma unroll
@@ -197,7 +200,9 @@ struct SpawnPairs {
 /* Each thread lists its board's empty cells (ascending) as pairs at its
  * exclusive-scan offset; then consecutive threads take consecutive pairs, so
  * the two output arrays are written with coalesced stores and every lane has
- * the same work whatever the number of empty cells. */
+ * the same work whatever the number of empty cells.  A block first counts the
+ * pairs of all its chunks and reserves its output range with one atomic (one
+ * per chunk made the shared counter the bottleneck). */
 __global__ __launch_bounds__(SP_THREADS) void k_spawn(const uint64_t *__restrict__ in, size_t n,
                                                       const unsigned long long *dn,
                                                       uint64_t *__restrict__ out2,
@@ -210,9 +215,20 @@ __global__ __launch_bounds__(SP_THREADS) void k_spawn(const uint64_t *__restrict
     uint64_t or2 = 0, and2 = ~0ULL, or4 = 0, and4 = ~0ULL;
     if (dn && *dn < n) n = (size_t)*dn;   /* count produced on the device */
     const size_t stride = (size_t)gridDim.x * SP_THREADS;
-    for (size_t b0 = (size_t)blockIdx.x * SP_THREADS; b0 < n; b0 += stride) {
-        size_t i = b0 + threadIdx.x;
-        uint64_t b = i < n ? in[i] : 0;
+    const size_t first = (size_t)blockIdx.x * SP_THREADS + threadIdx.x;
+    uint32_t mine = 0;
+    for (size_t i = first; i < n; i += stride) mine += (uint32_t)cg_empties(in[i]);
+    uint32_t all;
+    block_excl_scan<uint32_t>(mine, &all, sp.wtmp);
+    if (threadIdx.x == 0) base_sh = all ? atomicAdd(&stats[0], (unsigned long long)all) : 0;
+    __syncthreads();
+    size_t base = base_sh;
+    /* the next chunk's board is loaded while this chunk is processed */
+    size_t i = first;
+    uint64_t nb = i < n ? in[i] : 0;
+    for (size_t b0 = (size_t)blockIdx.x * SP_THREADS; b0 < n; b0 += stride, i += stride) {
+        const uint64_t b = nb;
+        nb = i + stride < n ? in[i + stride] : 0;
         uint64_t z = i < n ? cg_zmask(b) : 0;
         uint32_t e = (uint32_t)cg_popc64(z);
         uint32_t tot;
@@ -223,9 +239,7 @@ __global__ __launch_bounds__(SP_THREADS) void k_spawn(const uint64_t *__restrict
             sp.pc[off] = (uint8_t)next_cell(z);
             off++;
         }
-        if (threadIdx.x == 0) base_sh = tot ? atomicAdd(&stats[0], (unsigned long long)tot) : 0;
         __syncthreads();
-        const size_t base = base_sh;
         for (uint32_t p = threadIdx.x; p < tot; p += SP_THREADS) {
             uint64_t bb = sp.sb[sp.pb[p]];
             int x = sp.pc[p];
@@ -240,6 +254,7 @@ __global__ __launch_bounds__(SP_THREADS) void k_spawn(const uint64_t *__restrict
             or4 |= c4;
             and4 &= c4;
         }
+        base += tot;
         __syncthreads();
     }
     block_or_and(or2, and2, &stats[1], &stats[2], ro, ra);
@@ -1028,13 +1043,25 @@ __global__ __launch_bounds__(SP_THREADS) void k_bspawn(const uint64_t *__restric
     uint64_t or2 = 0, and2 = ~0ULL, or4 = 0, and4 = ~0ULL;
     if (dn && *dn < n) n = (size_t)*dn;
     const size_t stride = (size_t)gridDim.x * SP_THREADS;
+    /* one output reservation per block, as in k_spawn; the spawns of a
+     * board feed step + 1 of its table */
+    uint32_t mine = 0;
+    for (size_t i = (size_t)blockIdx.x * SP_THREADS + threadIdx.x; i < n; i += stride) {
+        const uint64_t ck = in[i];
+        const uint64_t t = ck >> BATCH_TID_SHIFT;
+        if (step + 1 < tabs[t].nsteps) mine += (uint32_t)cg_empties(bdec(ck, tabs[t]));
+    }
+    uint32_t all;
+    block_excl_scan<uint32_t>(mine, &all, sp.wtmp);
+    if (threadIdx.x == 0) base_sh = all ? atomicAdd(&stats[0], (unsigned long long)all) : 0;
+    __syncthreads();
+    size_t base = base_sh;
     for (size_t b0 = (size_t)blockIdx.x * SP_THREADS; b0 < n; b0 += stride) {
         size_t i = b0 + threadIdx.x;
         uint64_t b = 0, z = 0, tid = 0;
         if (i < n) {
             uint64_t ck = in[i];
             tid = ck >> BATCH_TID_SHIFT;
-            /* the spawns feed step + 1 */
             if (step + 1 < tabs[tid].nsteps) {
                 b = bdec(ck, tabs[tid]);
                 z = cg_zmask(b);
@@ -1050,9 +1077,7 @@ __global__ __launch_bounds__(SP_THREADS) void k_bspawn(const uint64_t *__restric
             sp.pc[off] = (uint8_t)next_cell(z);
             off++;
         }
-        if (threadIdx.x == 0) base_sh = tot ? atomicAdd(&stats[0], (unsigned long long)tot) : 0;
         __syncthreads();
-        const size_t base = base_sh;
         for (uint32_t p = threadIdx.x; p < tot; p += SP_THREADS) {
             int q = sp.pb[p];
             uint64_t bb = sp.sb[q];
@@ -1071,6 +1096,7 @@ __global__ __launch_bounds__(SP_THREADS) void k_bspawn(const uint64_t *__restric
             or4 |= c4;
             and4 &= c4;
         }
+        base += tot;
         __syncthreads();
     }
     block_or_and(or2, and2, &stats[1], &stats[2], ro, ra);
