@@ -342,10 +342,14 @@ def run_autotune(cg, args):
             json.dump(rep, f)
     cg.tune_set("cell_code", args.cell_code)
     cg.tune_set("carry_stream", args.carry_stream)
+    dir_delta = rep["dir"]["pick"]
+    if args.dir_delta is not None:
+        dir_delta = args.dir_delta
+        cg.tune_set("dir_delta", dir_delta)
     sel = {}
     for cls, name in enumerate(("n<2^20", "n<2^23", "n>=2^23")):
         sel[name] = {nb: rep["sort"][cls]["pick"][nb] for nb in ("24", "32", "36", "40", "44", "48", "54")}
-    return {"source": src, "seconds": time.perf_counter() - t0, "dir_delta": rep["dir"]["pick"],
+    return {"source": src, "seconds": time.perf_counter() - t0, "dir_delta": dir_delta,
             "sort_picks_bits_x_kpt/window": sel, "cell_code": args.cell_code,
             "carry_stream": args.carry_stream}
 
@@ -497,6 +501,8 @@ def main():
     ap.add_argument("--save-tuning", default=None, help="write the autotuner report (JSON) here")
     ap.add_argument("--carry-stream", type=int, choices=(0, 1), default=1,
                     help="0: the carry sort on the main stream instead of a second one")
+    ap.add_argument("--dir-delta", type=int, choices=range(-4, 4), default=None,
+                    help="layer directory size: 2^(floor(log2 n) + delta) buckets (overrides the tuned pick)")
     ap.add_argument("--cell-code", type=int, choices=(0, 1), default=1,
                     help="1: layer sorts code pairs of cells when the tile sum bounds them (default)")
     args = ap.parse_args()

@@ -628,6 +628,54 @@ __device__ __forceinline__ bool win_satisfied(uint64_t b, const WinRules &w)
 /* two zero words: the directory range [0, 0), key 0 and value 0.0 */
 __device__ const uint64_t g_zero2[2] = { 0, 0 };
 
+/* Lockstep lookups of N keys for the adjoint: slot s looks for q[s] in t2
+ * (even s) or t4 (odd s), nothing when live[s] is false, as layer_find does
+ * (constant bits checked by the caller, bucket range from the directory,
+ * lower bound), with the loads of all N slots in flight in every round and
+ * a zero word read where a slot has nothing to read -- the scheme of
+ * best_successors.  Returns found[s] and the key's index pos[s]. */
+template <int N>
+__device__ __forceinline__ void find_lockstep(const uint64_t (&q)[N], const bool (&live)[N],
+                                              const LayerView &t2, const LayerView &t4,
+                                              uint32_t (&pos)[N], bool (&found)[N])
+{
+#define CG_SLOT_VIEW(s) (((s) & 1) ? t4 : t2)
+    uint32_t hi[N];
+#pragma unroll
+    for (int s = 0; s < N; s++) {
+        const LayerView &t = CG_SLOT_VIEW(s);
+        const uint32_t *p = reinterpret_cast<const uint32_t *>(g_zero2);
+        if (live[s]) p = t.dir + (xform(q[s], t.x) >> t.shift);
+        pos[s] = p[0];
+        hi[s] = p[1];
+        found[s] = false;
+    }
+    for (;;) {
+        uint64_t k[N];
+        bool any = false;
+#pragma unroll
+        for (int s = 0; s < N; s++) {
+            const LayerView &t = CG_SLOT_VIEW(s);
+            k[s] = *(pos[s] < hi[s] ? t.keys + (pos[s] + ((hi[s] - pos[s]) >> 1)) : g_zero2);
+        }
+#pragma unroll
+        for (int s = 0; s < N; s++) {
+            if (pos[s] < hi[s]) {
+                const uint32_t mid = pos[s] + ((hi[s] - pos[s]) >> 1);
+                if (k[s] < q[s]) {
+                    pos[s] = mid + 1;
+                } else {
+                    hi[s] = mid;
+                    found[s] = k[s] == q[s];
+                }
+                any |= pos[s] < hi[s];
+            }
+        }
+        if (!any) break;
+    }
+#undef CG_SLOT_VIEW
+}
+
 /* The eight successors of one (board, cell) pair -- the spawned 2 (c2, layer
  * t2) and the spawned 4 (c4, layer t4), four moves each -- are looked up in
  * lockstep, so each round (the directory, every step of the lower-bound
@@ -863,21 +911,44 @@ __global__ __launch_bounds__(AD_THREADS) void k_adjoint(const uint64_t *__restri
         if (!E) continue;
         uint64_t am = argmax[i];
         double a2 = 0.0, a4 = 0.0;
+        /* two cells per round: slot s = 2 * (cell of the round) + (spawn 4?),
+         * the four lookups in lockstep; the sums still run over the cells in
+         * ascending order */
         while (z) {
-            int x = next_cell(z);
+            int xs[2];
+            xs[0] = next_cell(z);
+            const bool two = z != 0;
+            xs[1] = two ? next_cell(z) : xs[0];
+            uint64_t q[4];
+            bool live[4];
 #pragma unroll
-            for (int k = 0; k < 2; k++) {
+            for (int s = 0; s < 4; s++) {
+                const int k = s & 1, x = xs[s >> 1];
                 const LayerView &t = k ? t4 : t2;
-                int d = (int)(am >> (4 * x + 2 * k)) & 3;
+                const int d = (int)(am >> (4 * x + 2 * k)) & 3;
                 uint64_t m = cg_set_tile(b, x, (unsigned)(k + 1));
                 if (d >= 2) m = cg_rot_cw(m);
-                if (!cg_move_hori_tab(lut, &m, d == 1 || d == 2, ff)) continue;
-                long j = layer_find(cg_canon(m), t);
-                if (j < 0) continue;
-                double v = t.vals[j];
-                if (k) a4 += v;
-                else a2 += v;
-                if (v > 0.0) atomicAdd(k ? &occ4[j] : &occ2[j], o * (k ? p4 : w2) / (double)E);
+                const bool ok = cg_move_hori_tab(lut, &m, d == 1 || d == 2, ff) && (s < 2 || two);
+                q[s] = cg_canon(m);
+                live[s] = ok && t.n != 0 && (q[s] & t.cmask) == t.cval;
+            }
+            uint32_t pos[4];
+            bool f[4];
+            find_lockstep<4>(q, live, t2, t4, pos, f);
+            double v[4];
+#pragma unroll
+            for (int s = 0; s < 4; s++) {
+                const LayerView &t = (s & 1) ? t4 : t2;
+                v[s] = *(f[s] ? t.vals + pos[s] : reinterpret_cast<const double *>(g_zero2));
+            }
+#pragma unroll
+            for (int s = 0; s < 4; s++) {
+                if (!f[s]) continue;
+                const int k = s & 1;
+                if (k) a4 += v[s];
+                else a2 += v[s];
+                if (v[s] > 0.0)
+                    atomicAdd(k ? &occ4[pos[s]] : &occ2[pos[s]], o * (k ? p4 : w2) / (double)E);
             }
         }
         contrib += o * (a4 - a2) / (double)E;
