@@ -235,9 +235,15 @@ static long g_launches[PK_N];
 static double g_ms_s[2][PK_N], g_bytes_s[2][PK_N];
 static long g_launches_s[2][PK_N];
 
+/* whether launches of this kernel kind are being timed */
+static bool prof_wants(int kind)
+{
+    return g_prof && (g_prof_only < 0 || kind == g_prof_only);
+}
+
 static int prof_begin(int kind, hipStream_t st = nullptr)
 {
-    if (!g_prof || (g_prof_only >= 0 && kind != g_prof_only)) return -1;
+    if (!prof_wants(kind)) return -1;
     ProfRec r;
     r.kind = kind;
     r.st = st ? st : E.st;
@@ -876,7 +882,9 @@ static int solve_layer_dev(const cg_rules &r, const uint64_t *M, size_t n, doubl
     if (!lut) return fail_msg("row table upload failed");
     unsigned long long *stats = nullptr;
     int slot = -1;
-    if (g_prof) {
+    /* lookup counters (the algorithmic bytes) only when the solve is timed:
+     * they cost atomics at the end of every block */
+    if (prof_wants(PK_SOLVE)) {
         if (g_stats_used == g_stats_cap) {
             /* grow: collect pending records first so the old array can go */
             prof_collect();
