@@ -47,11 +47,12 @@ hipError_t kernels_take_error()
 /* ------------------------------------------------------------------------ */
 
 constexpr int MV_THREADS = 1024;
-constexpr int MV_HASH = 8192;   /* hash slots per 1024-board chunk (<= 4096 candidates) */
+constexpr int MV_HASH = 8192;   /* hash slots per 2048-board round (<= 8192 candidates) */
 
-/* First occurrence of c among the candidates the block produced in this
- * chunk: open addressing in LDS with a 64-bit compare-and-swap; 0 marks an
- * empty slot (a moved board always holds a tile).  Bounded probing. */
+/* Inserts c into the block's LDS hash set (open addressing with a 64-bit
+ * compare-and-swap; 0 marks an empty slot, a moved board always holds a
+ * tile); true on its first occurrence.  A set that holds at most MV_HASH
+ * distinct keys always finds c or a free slot. */
 __device__ __forceinline__ bool mv_first(unsigned long long *ht, uint64_t c)
 {
     uint32_t sl = (uint32_t)((c * 0x9E3779B97F4A7C15ULL) >> (64 - 13));
@@ -64,47 +65,32 @@ __device__ __forceinline__ bool mv_first(unsigned long long *ht, uint64_t c)
     return true;
 }
 
-/* DEDUPE (the generation step): the row table is read through the cache and
- * LDS holds a hash set per 1024-board chunk, so candidates a chunk produces
- * twice are appended once (70% of move candidates are duplicates, ~40%
- * within a chunk); the sort that follows sees fewer keys and the layer is
- * unchanged.  Otherwise (the move primitive) every candidate is appended and
- * the row table is staged in LDS. */
-template <bool DEDUPE>
-__global__ __launch_bounds__(MV_THREADS, DEDUPE ? 8 : 1) void k_move(
+/* The move primitive (cg_dev_move): every candidate is appended, and the
+ * row table is staged in LDS. */
+__global__ __launch_bounds__(MV_THREADS, 1) void k_move(
     const uint64_t *__restrict__ in, size_t n, const unsigned long long *dn,
     uint64_t *__restrict__ out, unsigned long long *stats, const uint16_t *__restrict__ glut,
     MoveRules r)
 {
-    /* LDS: the row table (128 KiB) or the hash set (64 KiB) */
-    __shared__ unsigned long long smem[DEDUPE ? MV_HASH : 65536 / 4];
+    __shared__ uint4 lds_lut[65536 / 8];   /* the row table, 128 KiB */
     __shared__ uint32_t wtmp[17];
     __shared__ unsigned long long base_sh;
     __shared__ uint64_t ro[16], ra[16];
     uint64_t kor = 0, kand = ~0ULL;
-    const uint16_t *lut = glut;
-    if (!DEDUPE) {
+    {
         const uint4 *g = reinterpret_cast<const uint4 *>(glut);
-        uint4 *l = reinterpret_cast<uint4 *>(smem);
-        for (int i = threadIdx.x; i < 65536 / 8; i += MV_THREADS) l[i] = g[i];
-        lut = reinterpret_cast<const uint16_t *>(smem);
+        for (int i = threadIdx.x; i < 65536 / 8; i += MV_THREADS) lds_lut[i] = g[i];
         __syncthreads();
     }
+    const uint16_t *lut = reinterpret_cast<const uint16_t *>(lds_lut);
     if (dn && *dn < n) n = (size_t)*dn;   /* count produced on the device */
     const size_t stride = (size_t)gridDim.x * MV_THREADS;
-    /* the next chunk's board is loaded while this chunk is processed */
-    size_t i = (size_t)blockIdx.x * MV_THREADS + threadIdx.x;
-    uint64_t nb = i < n ? in[i] : 0;
-    for (size_t b0 = (size_t)blockIdx.x * MV_THREADS; b0 < n; b0 += stride, i += stride) {
-        if (DEDUPE) {
-            for (int h = threadIdx.x; h < MV_HASH; h += MV_THREADS) smem[h] = 0ULL;
-            __syncthreads();
-        }
-        const uint64_t b = nb;
-        nb = i + stride < n ? in[i + stride] : 0;
+    for (size_t b0 = (size_t)blockIdx.x * MV_THREADS; b0 < n; b0 += stride) {
+        const size_t i = b0 + threadIdx.x;
         uint64_t res[4];
         uint32_t k = 0;
         if (i < n) {
+            const uint64_t b = in[i];
             if (!(r.prune && cg_prune(b, r.stsl, r.ltc, r.sl))) {
                 uint64_t y = cg_rot_cw(b);
 #pragma unroll
@@ -116,25 +102,98 @@ __global__ __launch_bounds__(MV_THREADS, DEDUPE ? 8 : 1) void k_move(
                 }
             }
         }
-        uint32_t keep = (1u << k) - 1u;
-        if (DEDUPE) {
-#pragma unroll
-            for (int j = 0; j < 4; j++)
-                if (j < (int)k && !mv_first(smem, res[j])) keep &= ~(1u << j);
-        }
-        uint32_t kk = (uint32_t)__builtin_popcount(keep);
 #pragma unroll
         for (int j = 0; j < 4; j++)
-            if ((keep >> j) & 1u) kor |= res[j], kand &= res[j];
+            if (j < (int)k) kor |= res[j], kand &= res[j];
         uint32_t tot;
-        uint32_t off = block_excl_scan<uint32_t>(kk, &tot, wtmp);
+        uint32_t off = block_excl_scan<uint32_t>(k, &tot, wtmp);
         if (threadIdx.x == 0) base_sh = tot ? atomicAdd(&stats[0], (unsigned long long)tot) : 0;
         __syncthreads();
         unsigned long long o = base_sh + off;
 #pragma unroll
         for (int j = 0; j < 4; j++)
-            if ((keep >> j) & 1u) out[o++] = res[j];
+            if (j < (int)k) out[o++] = res[j];
         __syncthreads();
+    }
+    block_or_and(kor, kand, &stats[1], &stats[2], ro, ra);
+}
+
+/* The generation step's move kernel.  A block takes 2048 input boards per
+ * round (two per thread) and inserts every candidate into its LDS hash set
+ * (8192 slots: at most 8192 candidates, about 4,000 distinct), so each
+ * candidate repeated within the 2048 boards is kept once.  The set itself is
+ * then compacted: wave w reads slots (s * 16 + w) * 64 + lane in round s,
+ * so a ballot per round gives each occupied slot its place in a run the
+ * wave writes contiguously, and the block reserves its output with one
+ * atomic per 2048 boards. */
+constexpr int MG_BPT = 2;   /* boards per thread per round */
+
+__global__ __launch_bounds__(MV_THREADS, 8) void k_move_gen(
+    const uint64_t *__restrict__ in, size_t n, const unsigned long long *dn,
+    uint64_t *__restrict__ out, unsigned long long *stats, const uint16_t *__restrict__ lut,
+    MoveRules r)
+{
+    constexpr int NW = MV_THREADS / 64, ROUNDS = MV_HASH / MV_THREADS;
+    __shared__ unsigned long long ht[MV_HASH];
+    __shared__ uint32_t wpre[NW + 1];
+    __shared__ unsigned long long base_sh;
+    __shared__ uint64_t ro[16], ra[16];
+    uint64_t kor = 0, kand = ~0ULL;
+    if (dn && *dn < n) n = (size_t)*dn;   /* count produced on the device */
+    const int w = threadIdx.x >> 6;
+    const size_t chunk = (size_t)MV_THREADS * MG_BPT;
+    const size_t stride = (size_t)gridDim.x * chunk;
+    for (size_t b0 = (size_t)blockIdx.x * chunk; b0 < n; b0 += stride) {
+        for (int h = threadIdx.x; h < MV_HASH; h += MV_THREADS) ht[h] = 0ULL;
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < MG_BPT; k++) {
+            const size_t i = b0 + (size_t)k * MV_THREADS + threadIdx.x;
+            if (i >= n) continue;
+            const uint64_t b = in[i];
+            if (r.prune && cg_prune(b, r.stsl, r.ltc, r.sl)) continue;
+            const uint64_t y = cg_rot_cw(b);
+#pragma unroll
+            for (int d = 0; d < 4; d++) {
+                uint64_t t = d < 2 ? b : y;
+                if (cg_move_hori_tab(lut, &t, d == 1 || d == 2, r.ff) &&
+                    !(r.prune && cg_prune(t, r.stsl, r.ltc, r.sl)))
+                    mv_first(ht, cg_canon(t));
+            }
+        }
+        __syncthreads();
+        uint64_t v[ROUNDS], m[ROUNDS];
+        uint32_t wtot = 0;
+#pragma unroll
+        for (int s = 0; s < ROUNDS; s++) {
+            v[s] = ht[(s * NW + w) * 64 + lane_id()];
+            m[s] = __ballot(v[s] != 0ULL);
+            wtot += (uint32_t)__popcll(m[s]);
+        }
+        if (lane_id() == 0) wpre[w] = wtot;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t acc = 0;
+            for (int j = 0; j < NW; j++) {
+                const uint32_t c = wpre[j];
+                wpre[j] = acc;
+                acc += c;
+            }
+            base_sh = acc ? atomicAdd(&stats[0], (unsigned long long)acc) : 0;
+        }
+        __syncthreads();
+        unsigned long long o = base_sh + wpre[w];
+        const uint64_t lt = lanemask_lt();
+#pragma unroll
+        for (int s = 0; s < ROUNDS; s++) {
+            if (v[s]) {
+                out[o + __popcll(m[s] & lt)] = v[s];
+                kor |= v[s];
+                kand &= v[s];
+            }
+            o += __popcll(m[s]);
+        }
+        __syncthreads();   /* the set and wpre are reused by the next round */
     }
     block_or_and(kor, kand, &stats[1], &stats[2], ro, ra);
 }
@@ -147,10 +206,10 @@ void launch_move(hipStream_t st, const uint64_t *in, size_t n, const unsigned lo
     size_t blocks = (n + MV_THREADS - 1) / MV_THREADS;
     if (blocks > 512) blocks = 512;
     if (dedupe)
-        hipLaunchKernelGGL(k_move<true>, dim3((unsigned)blocks), dim3(MV_THREADS), 0, st, in, n, dn,
+        hipLaunchKernelGGL(k_move_gen, dim3((unsigned)blocks), dim3(MV_THREADS), 0, st, in, n, dn,
                            out, stats, lut, r);
     else
-        hipLaunchKernelGGL(k_move<false>, dim3((unsigned)blocks), dim3(MV_THREADS), 0, st, in, n, dn,
+        hipLaunchKernelGGL(k_move, dim3((unsigned)blocks), dim3(MV_THREADS), 0, st, in, n, dn,
                            out, stats, lut, r);
 }
 
@@ -1049,7 +1108,8 @@ __device__ __forceinline__ uint64_t bdec(uint64_t ck, const BatchTable &T)
 }
 
 /* the batched move: decode, move, canonicalise, re-encode; duplicates within
- * a 1024-key chunk are appended once (as k_move<true>) */
+ * a 1024-key chunk are appended once (through the LDS hash set of
+ * k_move_gen, one chunk at a time) */
 __global__ __launch_bounds__(MV_THREADS, 8) void k_bmove(const uint64_t *__restrict__ in, size_t n,
                                                          const unsigned long long *dn,
                                                          uint64_t *__restrict__ out,

@@ -39,7 +39,7 @@ def test_trace_summary_splits_streams(tmp_path):
                 "TotalDurationNs": 600000, "AverageNs": 200000, "Percentage": 100.0,
                 "MinNs": 100000, "MaxNs": 300000}])
     trace = []
-    for stream, name, dur in (("1", "void cg::k_move<true>(...)", 50), ("1", "void cg::k_os_pass<8>(...)", 100),
+    for stream, name, dur in (("1", "cg::k_move_gen(...)", 50), ("1", "void cg::k_os_pass<8>(...)", 100),
                               ("1", "void cg::k_os_pass<8>(...)", 300), ("2", "void cg::k_os_pass<8>(...)", 200)):
         trace.append({"Kernel_Name": name, "Stream_Id": stream, "Start_Timestamp": 1000,
                       "End_Timestamp": 1000 + dur * 1000})
@@ -57,7 +57,7 @@ def test_pmc_summary_calibrates_and_splits_streams(tmp_path):
     # a 1 GiB copy reads FETCH_SIZE 512 MiB and WRITE_SIZE 1 GiB (in KiB): factors 2 and 1
     rows_f = [{"Dispatch_Id": 1, "Queue_Id": 1, "Kernel_Name": "__amd_rocclr_copyBuffer",
                "Counter_Name": "FETCH_SIZE", "Counter_Value": 524288.0},
-              {"Dispatch_Id": 2, "Queue_Id": 1, "Kernel_Name": "void cg::k_move<true>(...)",
+              {"Dispatch_Id": 2, "Queue_Id": 1, "Kernel_Name": "cg::k_move_gen(...)",
                "Counter_Name": "FETCH_SIZE", "Counter_Value": 1000.0},
               {"Dispatch_Id": 3, "Queue_Id": 1, "Kernel_Name": "void cg::k_os_pass<8>(...)",
                "Counter_Name": "FETCH_SIZE", "Counter_Value": 3000.0},
@@ -65,7 +65,7 @@ def test_pmc_summary_calibrates_and_splits_streams(tmp_path):
                "Counter_Name": "FETCH_SIZE", "Counter_Value": 1000.0}]
     rows_w = [{"Dispatch_Id": 1, "Queue_Id": 1, "Kernel_Name": "__amd_rocclr_copyBuffer",
                "Counter_Name": "WRITE_SIZE", "Counter_Value": 1048576.0},
-              {"Dispatch_Id": 2, "Queue_Id": 1, "Kernel_Name": "void cg::k_move<true>(...)",
+              {"Dispatch_Id": 2, "Queue_Id": 1, "Kernel_Name": "cg::k_move_gen(...)",
                "Counter_Name": "WRITE_SIZE", "Counter_Value": 500.0},
               {"Dispatch_Id": 3, "Queue_Id": 1, "Kernel_Name": "void cg::k_os_pass<8>(...)",
                "Counter_Name": "WRITE_SIZE", "Counter_Value": 4000.0},
