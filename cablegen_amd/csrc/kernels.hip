@@ -118,14 +118,57 @@ __global__ __launch_bounds__(MV_THREADS, 1) void k_move(
     block_or_and(kor, kand, &stats[1], &stats[2], ro, ra);
 }
 
+/* Appends the occupied slots of a block's hash set to out (one atomic on
+ * stats[0] reserves the block's range) and folds them into kor / kand.  Wave
+ * w reads slots (s * 16 + w) * 64 + lane in round s, so a ballot per round
+ * gives each occupied slot its place in a run the wave writes contiguously.
+ * Ends with a barrier: the set and wpre may be reused at once. */
+__device__ __forceinline__ void hash_set_flush(const unsigned long long *ht, uint64_t *__restrict__ out,
+                                               unsigned long long *stats, uint32_t *wpre,
+                                               unsigned long long *base_sh, uint64_t &kor,
+                                               uint64_t &kand)
+{
+    constexpr int NW = MV_THREADS / 64, ROUNDS = MV_HASH / MV_THREADS;
+    const int w = threadIdx.x >> 6;
+    uint64_t v[ROUNDS], m[ROUNDS];
+    uint32_t wtot = 0;
+#pragma unroll
+    for (int s = 0; s < ROUNDS; s++) {
+        v[s] = ht[(s * NW + w) * 64 + lane_id()];
+        m[s] = __ballot(v[s] != 0ULL);
+        wtot += (uint32_t)__popcll(m[s]);
+    }
+    if (lane_id() == 0) wpre[w] = wtot;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t acc = 0;
+        for (int j = 0; j < NW; j++) {
+            const uint32_t c = wpre[j];
+            wpre[j] = acc;
+            acc += c;
+        }
+        *base_sh = acc ? atomicAdd(&stats[0], (unsigned long long)acc) : 0;
+    }
+    __syncthreads();
+    unsigned long long o = *base_sh + wpre[w];
+    const uint64_t lt = lanemask_lt();
+#pragma unroll
+    for (int s = 0; s < ROUNDS; s++) {
+        if (v[s]) {
+            out[o + __popcll(m[s] & lt)] = v[s];
+            kor |= v[s];
+            kand &= v[s];
+        }
+        o += __popcll(m[s]);
+    }
+    __syncthreads();
+}
+
 /* The generation step's move kernel.  A block takes 2048 input boards per
  * round (two per thread) and inserts every candidate into its LDS hash set
  * (8192 slots: at most 8192 candidates, about 4,000 distinct), so each
- * candidate repeated within the 2048 boards is kept once.  The set itself is
- * then compacted: wave w reads slots (s * 16 + w) * 64 + lane in round s,
- * so a ballot per round gives each occupied slot its place in a run the
- * wave writes contiguously, and the block reserves its output with one
- * atomic per 2048 boards. */
+ * candidate repeated within the 2048 boards is kept once; the set is then
+ * appended with one output reservation per round (hash_set_flush). */
 constexpr int MG_BPT = 2;   /* boards per thread per round */
 
 __global__ __launch_bounds__(MV_THREADS, 8) void k_move_gen(
@@ -133,14 +176,12 @@ __global__ __launch_bounds__(MV_THREADS, 8) void k_move_gen(
     uint64_t *__restrict__ out, unsigned long long *stats, const uint16_t *__restrict__ lut,
     MoveRules r)
 {
-    constexpr int NW = MV_THREADS / 64, ROUNDS = MV_HASH / MV_THREADS;
     __shared__ unsigned long long ht[MV_HASH];
-    __shared__ uint32_t wpre[NW + 1];
+    __shared__ uint32_t wpre[MV_THREADS / 64];
     __shared__ unsigned long long base_sh;
     __shared__ uint64_t ro[16], ra[16];
     uint64_t kor = 0, kand = ~0ULL;
     if (dn && *dn < n) n = (size_t)*dn;   /* count produced on the device */
-    const int w = threadIdx.x >> 6;
     const size_t chunk = (size_t)MV_THREADS * MG_BPT;
     const size_t stride = (size_t)gridDim.x * chunk;
     for (size_t b0 = (size_t)blockIdx.x * chunk; b0 < n; b0 += stride) {
@@ -162,38 +203,7 @@ __global__ __launch_bounds__(MV_THREADS, 8) void k_move_gen(
             }
         }
         __syncthreads();
-        uint64_t v[ROUNDS], m[ROUNDS];
-        uint32_t wtot = 0;
-#pragma unroll
-        for (int s = 0; s < ROUNDS; s++) {
-            v[s] = ht[(s * NW + w) * 64 + lane_id()];
-            m[s] = __ballot(v[s] != 0ULL);
-            wtot += (uint32_t)__popcll(m[s]);
-        }
-        if (lane_id() == 0) wpre[w] = wtot;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            uint32_t acc = 0;
-            for (int j = 0; j < NW; j++) {
-                const uint32_t c = wpre[j];
-                wpre[j] = acc;
-                acc += c;
-            }
-            base_sh = acc ? atomicAdd(&stats[0], (unsigned long long)acc) : 0;
-        }
-        __syncthreads();
-        unsigned long long o = base_sh + wpre[w];
-        const uint64_t lt = lanemask_lt();
-#pragma unroll
-        for (int s = 0; s < ROUNDS; s++) {
-            if (v[s]) {
-                out[o + __popcll(m[s] & lt)] = v[s];
-                kor |= v[s];
-                kand &= v[s];
-            }
-            o += __popcll(m[s]);
-        }
-        __syncthreads();   /* the set and wpre are reused by the next round */
+        hash_set_flush(ht, out, stats, wpre, &base_sh, kor, kand);
     }
     block_or_and(kor, kand, &stats[1], &stats[2], ro, ra);
 }
@@ -1107,9 +1117,9 @@ __device__ __forceinline__ uint64_t bdec(uint64_t ck, const BatchTable &T)
     return b;
 }
 
-/* the batched move: decode, move, canonicalise, re-encode; duplicates within
- * a 1024-key chunk are appended once (through the LDS hash set of
- * k_move_gen, one chunk at a time) */
+/* the batched move: decode, move, canonicalise, re-encode; as k_move_gen,
+ * 2048 keys per round go through one LDS hash set, so duplicates among them
+ * are appended once */
 __global__ __launch_bounds__(MV_THREADS, 8) void k_bmove(const uint64_t *__restrict__ in, size_t n,
                                                          const unsigned long long *dn,
                                                          uint64_t *__restrict__ out,
@@ -1119,53 +1129,37 @@ __global__ __launch_bounds__(MV_THREADS, 8) void k_bmove(const uint64_t *__restr
                                                          unsigned *err)
 {
     __shared__ unsigned long long ht[MV_HASH];
-    __shared__ uint32_t wtmp[17];
+    __shared__ uint32_t wpre[MV_THREADS / 64];
     __shared__ unsigned long long base_sh;
     __shared__ uint64_t ro[16], ra[16];
     uint64_t kor = 0, kand = ~0ULL;
     if (dn && *dn < n) n = (size_t)*dn;
-    const size_t stride = (size_t)gridDim.x * MV_THREADS;
-    for (size_t b0 = (size_t)blockIdx.x * MV_THREADS; b0 < n; b0 += stride) {
-        for (int i = threadIdx.x; i < MV_HASH; i += MV_THREADS) ht[i] = 0ULL;
+    const size_t chunk = (size_t)MV_THREADS * MG_BPT;
+    const size_t stride = (size_t)gridDim.x * chunk;
+    for (size_t b0 = (size_t)blockIdx.x * chunk; b0 < n; b0 += stride) {
+        for (int h = threadIdx.x; h < MV_HASH; h += MV_THREADS) ht[h] = 0ULL;
         __syncthreads();
-        size_t i = b0 + threadIdx.x;
-        uint64_t res[4];
-        uint32_t k = 0;
-        if (i < n) {
-            uint64_t ck = in[i];
-            uint64_t tid = ck >> BATCH_TID_SHIFT;
-            const BatchTable &T = tabs[tid];
-            if (step < T.nsteps) {
-                uint64_t b = bdec(ck, T);
-                if (!(r.prune && cg_prune(b, r.stsl, r.ltc, r.sl))) {
-                    uint64_t y = cg_rot_cw(b);
 #pragma unroll
-                    for (int d = 0; d < 4; d++) {
-                        uint64_t t = d < 2 ? b : y;
-                        if (cg_move_hori_tab(lut, &t, d == 1 || d == 2, r.ff) &&
-                            !(r.prune && cg_prune(t, r.stsl, r.ltc, r.sl)))
-                            res[k++] = benc(cg_canon(t), T, tid, err);
-                    }
-                }
+        for (int k = 0; k < MG_BPT; k++) {
+            const size_t i = b0 + (size_t)k * MV_THREADS + threadIdx.x;
+            if (i >= n) continue;
+            const uint64_t ck = in[i];
+            const uint64_t tid = ck >> BATCH_TID_SHIFT;
+            const BatchTable &T = tabs[tid];
+            if (step >= T.nsteps) continue;
+            const uint64_t b = bdec(ck, T);
+            if (r.prune && cg_prune(b, r.stsl, r.ltc, r.sl)) continue;
+            const uint64_t y = cg_rot_cw(b);
+#pragma unroll
+            for (int d = 0; d < 4; d++) {
+                uint64_t t = d < 2 ? b : y;
+                if (cg_move_hori_tab(lut, &t, d == 1 || d == 2, r.ff) &&
+                    !(r.prune && cg_prune(t, r.stsl, r.ltc, r.sl)))
+                    mv_first(ht, benc(cg_canon(t), T, tid, err));
             }
         }
-        uint32_t keep = (1u << k) - 1u;
-#pragma unroll
-        for (int j = 0; j < 4; j++)
-            if (j < (int)k && !mv_first(ht, res[j])) keep &= ~(1u << j);
-        uint32_t kk = (uint32_t)__builtin_popcount(keep);
-#pragma unroll
-        for (int j = 0; j < 4; j++)
-            if ((keep >> j) & 1u) kor |= res[j], kand &= res[j];
-        uint32_t tot;
-        uint32_t off = block_excl_scan<uint32_t>(kk, &tot, wtmp);
-        if (threadIdx.x == 0) base_sh = tot ? atomicAdd(&stats[0], (unsigned long long)tot) : 0;
         __syncthreads();
-        unsigned long long o = base_sh + off;
-#pragma unroll
-        for (int j = 0; j < 4; j++)
-            if ((keep >> j) & 1u) out[o++] = res[j];
-        __syncthreads();
+        hash_set_flush(ht, out, stats, wpre, &base_sh, kor, kand);
     }
     block_or_and(kor, kand, &stats[1], &stats[2], ro, ra);
 }
