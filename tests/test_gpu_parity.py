@@ -121,6 +121,30 @@ def test_move_kernel_prune_parameters(cg, stsl, ltc, sl):
     assert np.array_equal(got, np.sort(np.array(exp, dtype=np.uint64)))
 
 
+def test_generation_move_full_hash_sets(cg):
+    """The generation step's move kernel takes 2048 boards per round into one
+    8192-slot LDS hash set.  Boards with four empty cells and distinct tiles
+    give about four distinct candidates each, so the sets run ~95% full;
+    a block of one repeated board gives the opposite extreme.  The first layer
+    of a one-step generation is uniq(canon(moves(initial))) (generate.c:409-550),
+    checked against the oracle."""
+    rng = np.random.default_rng(11)
+    n = 1 << 19
+    b = np.zeros(n, dtype=np.uint64)
+    for x in range(16):
+        b |= rng.integers(1, 12, size=n).astype(np.uint64) << np.uint64(60 - 4 * x)
+    for _ in range(4):
+        pos = rng.integers(0, 16, size=n).astype(np.uint64)
+        b &= ~(np.uint64(0xF) << (np.uint64(60) - np.uint64(4) * pos))
+    b[: 1 << 14] = b[0]
+    cfg = O.make_cfg(cores=8)
+    M, _, _ = O.gen_layer(cfg, b, np.zeros(0, np.uint64))
+    assert len(M) > 3 * (n - (1 << 14))
+    tb = cg.Tablebase(cg.rules())
+    tb.generate(b, 100, 100)
+    assert np.array_equal(tb.keys(0), M)
+
+
 def test_spawn_kernel(cg):
     rng = np.random.default_rng(7)
     for fm in FORMATIONS:
