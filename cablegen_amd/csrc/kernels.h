@@ -123,9 +123,11 @@ void launch_count_empties(hipStream_t st, const uint64_t *in, size_t n, unsigned
 
 /* spawn expansion: for each board and empty cell x (ascending), canon(b[x<-1])
  * into out2 and canon(b[x<-2]) into out4 at the same positions; stats[0]
- * receives the number of spawns, stats[1..2] / stats[3..4] the OR / AND of
- * the out2 / out4 keys.  Only positions below cap are written: a count above
- * cap means the buffers were too small and the launch must be repeated. */
+ * receives the number of spawns, stats[3..4] the OR / AND of the out4 keys
+ * (the carry sort's key bits; the out2 keys are only moved, never sorted, so
+ * stats[1..2] are left as they are).  Only positions below cap are written: a
+ * count above cap means the buffers were too small and the launch must be
+ * repeated. */
 void launch_spawn(hipStream_t st, const uint64_t *in, size_t n, const unsigned long long *dn,
                   uint64_t *out2, uint64_t *out4, size_t cap, unsigned long long *stats);
 

@@ -281,7 +281,7 @@ __global__ __launch_bounds__(SP_THREADS) void k_spawn(const uint64_t *__restrict
     __shared__ SpawnPairs sp;
     __shared__ unsigned long long base_sh;
     __shared__ uint64_t ro[16], ra[16];
-    uint64_t or2 = 0, and2 = ~0ULL, or4 = 0, and4 = ~0ULL;
+    uint64_t or4 = 0, and4 = ~0ULL;
     if (dn && *dn < n) n = (size_t)*dn;   /* count produced on the device */
     const size_t stride = (size_t)gridDim.x * SP_THREADS;
     const size_t first = (size_t)blockIdx.x * SP_THREADS + threadIdx.x;
@@ -318,15 +318,12 @@ __global__ __launch_bounds__(SP_THREADS) void k_spawn(const uint64_t *__restrict
                 out2[o] = c2;
                 out4[o] = c4;
             }
-            or2 |= c2;
-            and2 &= c2;
             or4 |= c4;
             and4 &= c4;
         }
         base += tot;
         __syncthreads();
     }
-    block_or_and(or2, and2, &stats[1], &stats[2], ro, ra);
     block_or_and(or4, and4, &stats[3], &stats[4], ro, ra);
 }
 
@@ -336,8 +333,9 @@ void launch_spawn(hipStream_t st, const uint64_t *in, size_t n, const unsigned l
     if (!n) return;
     size_t blocks = (n + SP_THREADS - 1) / SP_THREADS;
     /* 512 blocks: the OR/AND atomics at the end of every block bound the
-     * kernel (C2 spawn 22.4 ms per step with 512, 25.1 with 1024, 38.7 with
-     * 2048, 29.7 with 256) */
+     * kernel (with two OR/AND pairs per block, C2 spawn took 22.4 ms per step
+     * with 512 blocks, 25.1 with 1024, 38.7 with 2048, 29.7 with 256; with
+     * the one pair now, 19.1 with 512 and 18.8 with 1024) */
     if (blocks > 512) blocks = 512;
     hipLaunchKernelGGL(k_spawn, dim3((unsigned)blocks), dim3(SP_THREADS), 0, st, in, n, dn, out2,
                        out4, cap, stats);
