@@ -51,6 +51,20 @@ def build_lib():
     return OUT
 
 
+NATIVE = os.path.join(ROOT, "tests", "native")
+CALLER = os.path.join(NATIVE, "_build", "dropin_caller")
+
+
+def build_caller():
+    """tests/native/dropin_caller: a C caller of the drop-in ABI linked to
+    libcablegen.so (GPU tests run it; it needs no reference file)"""
+    os.makedirs(os.path.dirname(CALLER), exist_ok=True)
+    _run(["gcc", "-O2", "-std=gnu11", "-Wall", "-I", os.path.join(ROOT, "include"), "-o", CALLER,
+          os.path.join(NATIVE, "dropin_caller.c"), "-L", HERE, "-lcablegen",
+          "-Wl,-rpath,$ORIGIN/../../../cablegen_amd"])
+    return CALLER
+
+
 def build_oracle():
     _run(["make", "-s", "-B", "-f", "oracle/Makefile"])
     if os.path.isdir("/root/reference/src"):
@@ -60,6 +74,7 @@ def build_oracle():
 
 def build_all():
     build_lib()
+    build_caller()
     build_oracle()
 
 

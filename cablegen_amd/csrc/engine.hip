@@ -758,10 +758,19 @@ static LayerView empty_view()
     return v;
 }
 
+/* a layer the 32-bit directory can index (LAYER_INDEX_MAX) */
+static int check_layer_size(size_t n)
+{
+    if (n >= LAYER_INDEX_MAX)
+        return fail_msg("layer index: a layer of 2^32 or more boards does not fit the 32-bit directory");
+    return 0;
+}
+
 /* builds the prefix-bucket directory of a sorted unique layer into dir
  * (plan.dir_entries u32) */
 static int build_index(const uint64_t *keys, size_t n, const LayerIndexPlan &plan, uint32_t *dir)
 {
+    if (check_layer_size(n)) return -1;
     if (ensure_scratch_bytes(dir_scratch_bytes(plan.dir_entries))) return -1;
     launch_build_dir(E.st, keys, n, plan, dir, E.scratch);
     return 0;
@@ -1120,6 +1129,14 @@ extern "C" int cg_sort_code_probe(const uint64_t *keys, size_t n, uint64_t vary,
         back[i] = sc_decode(codes[i], c) | (keys[i] & ~c.vmask);
     }
     return c.nbits;
+}
+
+extern "C" int cg_layer_index_probe(size_t n, uint64_t key_or, uint64_t key_and, int *bits)
+{
+    if (check_layer_size(n)) return -1;
+    LayerIndexPlan p = plan_index(n, key_or, key_and);
+    if (bits) *bits = p.bits;
+    return 0;
 }
 
 /* a temporary index for caller-owned layers */
@@ -2015,7 +2032,7 @@ extern "C" int cg_batch_generate(cg_batch *b)
                           E.st));
         HC(hipMemcpyAsync(d_plan, st.bplan.data(), ntab * sizeof(BatchIndexPlan),
                           hipMemcpyHostToDevice, E.st));
-        if (ensure_scratch_bytes(dir_scratch_bytes(entries + 1))) return -1;
+        if (check_layer_size(nM) || ensure_scratch_bytes(dir_scratch_bytes(entries + 1))) return -1;
         launch_bdir(E.st, st.keys, nM, d_seg, ntab, d_plan, entries, st.dir, E.scratch);
         b->steps.push_back(std::move(st));
         pc.pending = true;

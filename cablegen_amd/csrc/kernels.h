@@ -83,6 +83,12 @@ struct LayerView {
     KeyXform x;
 };
 
+/* Directory entries, and the positions the solve and the adjoint derive from
+ * them, are 32-bit: a layer (or a batch step's concatenated layers) of
+ * LAYER_INDEX_MAX boards or more cannot be indexed, and the launchers refuse
+ * it (kernels_note_error) instead of wrapping. */
+constexpr size_t LAYER_INDEX_MAX = (size_t)1 << 32;
+
 /* index of a layer: directory sizing from n and the OR / AND of its keys */
 struct LayerIndexPlan {
     KeyXform x;

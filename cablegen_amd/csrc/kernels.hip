@@ -550,6 +550,10 @@ size_t dir_scratch_bytes(size_t entries)
 void launch_build_dir(hipStream_t st, const uint64_t *keys, size_t n, const LayerIndexPlan &p,
                       uint32_t *dir, void *scratch)
 {
+    if (n >= LAYER_INDEX_MAX) {   /* 32-bit entries would wrap */
+        kernels_note_error(hipErrorInvalidValue);
+        return;
+    }
     size_t m = p.dir_entries - 1;   /* buckets */
     uint32_t *cnt = (uint32_t *)scratch;
     uint32_t *sums = cnt + ((m + 2 + 63) / 64) * 64;
@@ -1375,6 +1379,10 @@ void launch_bdecode(hipStream_t st, const uint64_t *in, size_t n, const unsigned
 void launch_bdir(hipStream_t st, const uint64_t *keys, size_t n, const size_t *seg, int ntab,
                  const BatchIndexPlan *plan, size_t total_entries, uint32_t *dir, void *scratch)
 {
+    if (n >= LAYER_INDEX_MAX) {   /* 32-bit entries would wrap */
+        kernels_note_error(hipErrorInvalidValue);
+        return;
+    }
     size_t m = total_entries;
     uint32_t *cnt = (uint32_t *)scratch;
     uint32_t *sums = cnt + ((m + 2 + 63) / 64) * 64;

@@ -195,7 +195,7 @@ constexpr int HIST_THREADS = 1024;
 
 template <int BITS>
 __global__ __launch_bounds__(HIST_THREADS) void k_os_hist(const uint64_t *__restrict__ keys, size_t n,
-                                                 SortCode c, int npass, uint32_t *ghist,
+                                                 SortCode c, int npass, unsigned long long *ghist,
                                                  uint64_t *cval, unsigned *err)
 {
     constexpr int R = 1 << BITS, NC = BITS == 8 ? 8 : (BITS == 9 ? 4 : 2);
@@ -231,13 +231,14 @@ __global__ __launch_bounds__(HIST_THREADS) void k_os_hist(const uint64_t *__rest
         uint32_t t = 0;
 #pragma unroll
         for (int k = 0; k < NC; k++) t += h[k * STRIDE + i];
-        if (t) atomicAdd(&ghist[i], t);
+        if (t) atomicAdd(&ghist[i], (unsigned long long)t);
     }
 }
 
-/* exclusive scan of every pass's digit counts */
+/* exclusive scan of every pass's digit counts (64-bit: a sort may hold
+ * 2^32 keys or more) */
 template <int BITS>
-__global__ __launch_bounds__(1 << BITS) void k_os_scan(const uint32_t *ghist, int npass,
+__global__ __launch_bounds__(1 << BITS) void k_os_scan(const unsigned long long *ghist, int npass,
                                                        uint64_t *gbase)
 {
     constexpr int R = 1 << BITS;
@@ -735,11 +736,11 @@ void sort_release()
 struct OsScratch {
     unsigned *ctr;        /* OS_MAXPASS + 1 tile counters */
     uint64_t *cval;       /* the keys' constant bits */
-    uint32_t *ghist;      /* OS_MAXPASS x OS_MAXR */
+    unsigned long long *ghist;   /* OS_MAXPASS x OS_MAXR */
     uint64_t *gbase;      /* OS_MAXPASS x OS_MAXR */
 };
 
-constexpr size_t OS_CLEAR_BYTES = 256 + (size_t)OS_MAXPASS * OS_MAXR * 4;
+constexpr size_t OS_CLEAR_BYTES = 256 + (size_t)OS_MAXPASS * OS_MAXR * 8;
 
 size_t sort_scratch_bytes(size_t) { return OS_CLEAR_BYTES + (size_t)OS_MAXPASS * OS_MAXR * 8 + 256; }
 
@@ -749,7 +750,7 @@ static OsScratch carve(void *scratch)
     OsScratch s;
     s.ctr = (unsigned *)p;
     s.cval = (uint64_t *)(p + 128);
-    s.ghist = (uint32_t *)(p + 256);
+    s.ghist = (unsigned long long *)(p + 256);
     s.gbase = (uint64_t *)(p + OS_CLEAR_BYTES);
     return s;
 }

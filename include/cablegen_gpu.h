@@ -129,6 +129,11 @@ int cg_dev_sort_unique_bounded(uint64_t *d_keys, size_t n, uint64_t *d_out, size
  * bits, or -1 when vary is 0. */
 int cg_sort_code_probe(const uint64_t *keys, size_t n, uint64_t vary, uint64_t cell_max,
                        uint64_t *codes, uint64_t *back);
+/* Host-side view of a layer's index plan (no GPU work): the directory
+ * depth in bits for a sorted unique layer of n keys with the given OR / AND.
+ * Fails (-1, with a message) for n >= 2^32, which the 32-bit directory
+ * cannot index -- the same check every layer build makes. */
+int cg_layer_index_probe(size_t n, uint64_t key_or, uint64_t key_and, int *bits);
 /* exact-match lookup of nq queries into a sorted unique layer (keys, values);
  * misses give 0 (lookup, src/solve.c:108-164). */
 int cg_dev_lookup(const uint64_t *d_keys, const double *d_vals, size_t n, const uint64_t *d_q,

@@ -340,3 +340,70 @@ def read_table_file(path):
     raw = np.fromfile(path, dtype=np.uint64) if os.path.exists(path) else np.zeros(0, np.uint64)
     n = len(raw) // 2
     return raw[:n].copy(), raw[n:2 * n].view(np.float64).copy()
+
+
+# --------------------------------------------------------------------------
+# table consumers: the CLI's best() and lookup output (frontends/cli/main.c)
+# --------------------------------------------------------------------------
+
+MOVE_NAMES = ("left", "right", "up", "down")
+
+
+def tile(b, i):
+    return (int(b) >> (60 - 4 * i)) & 0xF
+
+
+def set_tile(b, i, v):
+    sh = 60 - 4 * i
+    return (int(b) & ~(0xF << sh)) | (int(v) << sh)
+
+
+def tile_sum(b):
+    return sum(1 << tile(b, i) for i in range(16) if 0 < tile(b, i) < 0xE)
+
+
+def table_lookup(b, K, V):
+    """lookup(b, t, canonicalize=true) (solve.c:108-164): 0.0 on a miss"""
+    c = canon(b)
+    i = int(np.searchsorted(K, np.uint64(c)))
+    return float(V[i]) if i < len(K) and int(K[i]) == c else 0.0
+
+
+def best(cfg, b, K, V):
+    """main.c:174-189: the first move whose value is strictly larger wins,
+    starting from left at 0"""
+    maxd, maxp = 0, 0.0
+    for d in range(4):
+        ok, t = move(cfg, b, d)
+        if not ok:
+            continue
+        p = table_lookup(t, K, V)
+        if p > maxp:
+            maxd, maxp = d, p
+    return maxd, maxp
+
+
+def lookup_lines(cfg, q, tabs, spawn=False):
+    """The "Best move" lines of the CLI's lookup (spawn=False) or
+    lookup_spawn (spawn=True) of board q (main.c:191-265); tabs maps a tile
+    sum to its (keys, values)."""
+    b = int(q)
+    if spawn:
+        for i in range(16):
+            if tile(b, i) <= 2:
+                b = set_tile(b, i, 0)
+                break
+    s = tile_sum(b)
+    empty = (np.zeros(0, np.uint64), np.zeros(0, np.float64))
+    out = []
+    for i in range(16):
+        if tile(b, i):
+            continue
+        for v in (1, 2):
+            c = set_tile(b, i, v)
+            if spawn and c != int(q):
+                continue
+            K, V = tabs.get(s + 2 * v, empty)
+            md, mp = best(cfg, c, K, V)
+            out.append("Best move: %s (%.10f)" % (MOVE_NAMES[md], mp))
+    return out

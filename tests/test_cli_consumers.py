@@ -18,37 +18,12 @@ from tests import oracle_lib as O
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 CLI = os.path.join(ROOT, "oracle", "_ref", "cablegen_cli_mi355x")
-NAMES = ("left", "right", "up", "down")
+NAMES = O.MOVE_NAMES
 
 pytestmark = pytest.mark.skipif(not os.path.exists(CLI), reason="reference CLI not built")
 
 
-def tile(b, i):
-    return (b >> (60 - 4 * i)) & 0xF
-
-
-def set_tile(b, i, v):
-    sh = 60 - 4 * i
-    return (b & ~(0xF << sh)) | (v << sh)
-
-
-def table_lookup(b, K, V):
-    c = O.canon(b)
-    i = int(np.searchsorted(K, np.uint64(c)))
-    return float(V[i]) if i < len(K) and int(K[i]) == c else 0.0
-
-
-def best(cfg, b, K, V):
-    """main.c:174-189: the first move whose value is strictly larger wins"""
-    maxd, maxp = 0, 0.0
-    for d in range(4):
-        ok, t = O.move(cfg, b, d)
-        if not ok:
-            continue
-        p = table_lookup(t, K, V)
-        if p > maxp:
-            maxd, maxp = d, p
-    return maxd, maxp
+tile, set_tile, best = O.tile, O.set_tile, O.best
 
 
 @pytest.fixture(scope="module")

@@ -674,35 +674,6 @@ def test_c2_adjoint_fd(cg, c2_table):
     assert abs(fd - dP) <= 1e-6 * abs(dP), (fd, dP)
 
 
-CLI = os.path.join(os.path.dirname(HERE), "oracle", "_ref", "cablegen_cli_mi355x")
-
-
-@pytest.mark.skipif(not os.path.exists(CLI), reason="reference CLI not built against the library")
-def test_reference_cli_on_library(cg, tmp_path):
-    """The reference's own CLI (frontends/cli/main.c, unchanged) linked to
-    libcablegen.so: `generate` and `solve` driven by a config file must write
-    the reference's layers and tables."""
-    import subprocess
-    d = str(tmp_path)
-    os.makedirs(d + "/b")
-    os.makedirs(d + "/t")
-    np.array([0x11000000FFFFFFFF], np.uint64).tofile(d + "/initial")
-    np.array([0x7000000000000000], np.uint64).tofile(d + "/winstates")
-    with open(d + "/c.conf", "w") as f:
-        f.write(f"[Cablegen]\nignore_f = true\ncores = 2\n[Generate]\ndir = {d}/b/\n"
-                f"initial = {d}/initial\nend = 260\n[Solve]\ndir = {d}/t/\n"
-                f"winstates = {d}/winstates\nend = 0\n")
-    for cmd in ("generate", "solve"):
-        r = subprocess.run([CLI, f"-C={d}/c.conf", cmd], cwd=d, capture_output=True, text=True,
-                           timeout=120)
-        assert r.returncode == 0, r.stdout + r.stderr
-    rows = json.load(open(os.path.join(HERE, "golden", "c1_ref.json")))["variants"]["ignore_f"]
-    for s, n, hk, hv in rows:
-        k, v = O.read_table_file(f"{d}/t/{s}.tables")
-        assert len(k) == n and "%016x" % O.hash_u64(k) == hk, s
-        assert "%016x" % O.hash_u64(v.view(np.uint64)) == hv, s
-
-
 # ---------------------------------------------------------------- batched tables
 
 def test_batch_matches_single_tables(cg):

@@ -201,3 +201,21 @@ def test_stored_tuning_table_installs(cg):
     finally:
         cg.tune_set("reset", 0)
 
+
+
+def test_layer_index_refuses_2e32_boards(cg):
+    """The directory, and the positions the solve and the adjoint derive from
+    it, are 32-bit.  A layer of 2^32 boards or more must fail loudly (the
+    check every layer build makes), not wrap; one below the limit plans a
+    directory of at most 30 bits.  (Host-side planning only.)"""
+    L = cg.lib()
+    bits = C.c_int(-1)
+    key_or, key_and = 0xFFFAFFFBFFFFFFFF, 0xFFF0FFF000000000   # a DPDGap layer's 40 varying bits
+    assert L.cg_layer_index_probe((1 << 32) - 1, key_or, key_and, C.byref(bits)) == 0
+    assert bits.value == 30
+    for n in (1 << 32, (1 << 32) + 1, 1 << 40):
+        bits.value = -1
+        assert L.cg_layer_index_probe(n, key_or, key_and, C.byref(bits)) != 0
+        assert bits.value == -1
+        assert "2^32" in cg.last_error()
+    assert L.cg_layer_index_probe(1000, key_or, key_and, C.byref(bits)) == 0 and 0 < bits.value <= 12
