@@ -10,20 +10,26 @@ board) is on the device before the timed region starts.
 value = total boards over all layers / (T_generate + T_solve), per step.
 
 Also reported:
-  roofline      the kernel with the largest total time (from one untimed step
-                with every kernel timed), then timed in the timed steps with
-                HIP events on the stream it runs on; achieved is
-                over its main-stream launches (the carry sort's, on the second
-                stream, share the GPU with the next step's moves and are
-                reported beside it); algorithmic bytes per launch: the
-                payload read once and written once (a radix pass 16 B per
-                key, the histogram 8 B, the compaction 8 B per key + 8 B per
-                unique key, the move kernel 8 B per board + 8 B per candidate)
+  roofline      the stage with the largest total time (move, spawn, solve, or
+                the sort stage = every kernel of a sort + dedupe), from one
+                untimed step with every kernel timed, then timed in the timed
+                steps with HIP events on the stream each launch runs on.
+                achieved = algorithmic bytes / time over its main-stream units
+                (the carry sorts, on the second stream, share the GPU with the
+                next step's moves and are reported beside it).  Algorithmic
+                bytes: the payload read once and written once -- a sort
+                8 B per key + 8 B per unique key (extra radix passes are
+                implementation overhead), the move kernel 8 B per board + 8 B
+                per candidate, the spawn 8 B per board + 16 B per spawn, the
+                solve 16 B per board + 24 B per lookup + 8 B per hit.
+                traffic = calibrated rocprofv3 HBM bytes per unit
   cpu_baseline  the oracle (a pthreads C port of the reference's CPU
-                algorithm) on a bounded sample: full generation steps and
-                solves of three consecutive mid-table layers, whose outputs
-                are also checked bit-for-bit against the GPU's
-  adjoint       time of the exact dP(win)/dp4 sweep relative to a plain solve
+                algorithm) on min(nproc, OMP_NUM_THREADS, 127) threads, on
+                windows of three consecutive layers spread over the table
+                (about --cpu-seconds of CPU work; --cpu-full: the whole
+                table), outputs checked bit-for-bit against the GPU's
+  adjoint       time of the exact dP(win)/dp4 sweep relative to a plain
+                solve, mean of --adjoint-reps sweeps
 """
 import argparse
 import json
@@ -43,47 +49,112 @@ END_GEN = 1100
 RULES = dict(ignore_f=True, prune=True, stsl=58, ltc=5, smallest_large=6)
 HBM_PEAK_GBS = 8000.0
 KERNELS = ["move", "spawn", "sort_hist", "sort_pass", "unique", "solve"]
+# the sort stage's kernels; the last one of each sort (the compaction) carries
+# the stage's algorithmic bytes 8 B per key in + 8 B per unique key out
+SORT_KINDS = ["sort_hist", "sort_pass", "unique"]
+SORT_FINAL = ["unique"]
+# roofline candidates: the move, spawn and solve kernels, and the sort stage
+STAGES = ["move", "spawn", "sort", "solve"]
 
 
-def cpu_baseline(tb, cg, cores):
-    """Oracle timing on three consecutive full-size layers in the middle of
-    the table (generation step + solve of each), checked against the GPU."""
+def host_cpu():
+    """CPU model, the cores this process may run on (what nproc prints), the
+    machine's logical CPUs, and the thread budget of the job (OMP_NUM_THREADS
+    when the environment sets one)"""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    nproc = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    share = os.environ.get("OMP_NUM_THREADS")
+    share = int(share) if share and share.isdigit() and int(share) > 0 else None
+    return model, nproc, os.cpu_count() or nproc, share
+
+
+def baseline_cores():
+    """Every core the job may use, capped at 127: the reference merges
+    cores + 1 sorted arrays and allows at most 128 (src/array.c:247-251)"""
+    _, nproc, _, share = host_cpu()
+    return max(1, min(nproc, share or nproc, 127))
+
+
+def cpu_baseline(tb, cg, cores, budget_s=20.0, full=False):
+    """The oracle (the reference's CPU algorithm, -O2, pthreads on `cores`
+    threads) timed on windows of consecutive full-size layers spread over
+    the table, until about budget_s of CPU work; each window's pre-move
+    layer and carry are rebuilt from the GPU's two previous layers, and every
+    output is compared bit-for-bit with the GPU's.  full=True times the
+    whole table instead (generate() then solve(), as the reference runs it)."""
     from tests import oracle_lib as O
     cfg = O.make_cfg(cores=cores, **RULES)
-    sizes = [tb.layer_size(i) for i in range(tb.num_layers)]
-    i0 = int(np.argmax(sizes[4:-4])) + 4 - 1
-    idx = [i0, i0 + 1, i0 + 2]
-    # pre-move layer P_s = uniq(S2(M_{s-2}) u S4(M_{s-4})) and the carry S4(M_{s-2})
-    m2 = tb.keys(idx[0] - 1)
-    m4 = tb.keys(idx[0] - 2)
-    s2_a, s4_a = O.spawn_sets(cfg, m2)
-    _, s4_b = O.spawn_sets(cfg, m4)
-    P = np.union1d(s2_a, s4_b).astype(np.uint64)
-    Cc = s4_a
+    model, nproc, ncpu, share = host_cpu()
+    total = tb.total_boards()
     W = O.expand_winstates(WINSTATES)
+    host = {"cpu_model": model, "nproc": nproc, "cpu_count": ncpu, "omp_num_threads": share,
+            "cores_rule": "min(nproc, OMP_NUM_THREADS, 127)"}
+    if full:
+        start = tb.start
+        t0 = time.perf_counter()
+        layers = O.generate(cfg, INITIAL, start, END_GEN)
+        t1 = time.perf_counter()
+        vals = O.solve(cfg, layers, start, END_GEN, 0, WINSTATES)
+        t2 = time.perf_counter()
+        exact = len(layers) == tb.num_layers and all(
+            np.array_equal(layers[i], tb.keys(i)) and
+            np.array_equal(vals[i].view(np.uint64), tb.values(i).view(np.uint64))
+            for i in range(len(layers)))
+        boards = sum(len(x) for x in layers)
+        return dict(host, value=boards / (t2 - t0), unit="boards/s", cores=cores, kind="port",
+                    sample=f"the whole table: generate {t1 - t0:.1f}s + solve {t2 - t1:.1f}s of "
+                           f"{boards} boards; outputs bit-identical to the GPU's: {exact}",
+                    sample_fraction=boards / total)
+    sizes = [tb.layer_size(i) for i in range(tb.num_layers)]
+    nl = tb.num_layers
+    # window starts spread over the table's bulk (layers of at least a tenth of the largest)
+    big = [i for i in range(4, nl - 3) if sizes[i] >= max(sizes) / 10]
+    order = []
+    if big:
+        lo, hi = big[0], big[-1] - 2
+        for k in range(8):
+            i0 = lo + (hi - lo) * ((5 * k) % 8) // 7   # a spread order: middle first
+            if i0 not in order:
+                order.append(i0)
     t_gen = t_solve = 0.0
     boards = 0
     exact = True
-    for i in idx:
-        t0 = time.perf_counter()
-        M, P, Cc = O.gen_layer(cfg, P, Cc)
-        t_gen += time.perf_counter() - t0
-        gk = tb.keys(i)
-        exact &= bool(np.array_equal(M, gk))
-        K2, V2 = tb.keys(i + 1), tb.values(i + 1)
-        K4, V4 = tb.keys(i + 2), tb.values(i + 2)
-        t0 = time.perf_counter()
-        V = O.solve_layer(cfg, M, K2, V2, K4, V4, W)
-        t_solve += time.perf_counter() - t0
-        exact &= bool(np.array_equal(V.view(np.uint64), tb.values(i).view(np.uint64)))
-        boards += len(M)
-    s0 = tb.start + 2 * idx[0]
-    return {
-        "value": boards / (t_gen + t_solve), "unit": "boards/s", "cores": cores, "kind": "port",
-        "sample": f"generation step + solve of the layers s={s0},{s0 + 2},{s0 + 4} "
-                  f"({boards} boards; gen {t_gen:.2f}s, solve {t_solve:.2f}s); "
-                  f"outputs bit-identical to the GPU's: {exact}",
-    }
+    done = []
+    for i0 in order:
+        if t_gen + t_solve >= budget_s:
+            break
+        # pre-move layer P_s = S2(M_{s-2}) u S4(M_{s-4}) and the carry S4(M_{s-2})
+        s2_a, s4_a = O.spawn_sets(cfg, tb.keys(i0 - 1))
+        _, s4_b = O.spawn_sets(cfg, tb.keys(i0 - 2))
+        P = np.union1d(s2_a, s4_b).astype(np.uint64)
+        Cc = s4_a
+        for i in (i0, i0 + 1, i0 + 2):
+            t0 = time.perf_counter()
+            M, P, Cc = O.gen_layer(cfg, P, Cc)
+            t_gen += time.perf_counter() - t0
+            exact &= bool(np.array_equal(M, tb.keys(i)))
+            K2, V2 = tb.keys(i + 1), tb.values(i + 1)
+            K4, V4 = tb.keys(i + 2), tb.values(i + 2)
+            t0 = time.perf_counter()
+            V = O.solve_layer(cfg, M, K2, V2, K4, V4, W)
+            t_solve += time.perf_counter() - t0
+            exact &= bool(np.array_equal(V.view(np.uint64), tb.values(i).view(np.uint64)))
+            boards += len(M)
+        done.append(tb.start + 2 * i0)
+    return dict(host, value=boards / (t_gen + t_solve), unit="boards/s", cores=cores, kind="port",
+                sample=f"generation step + solve of 3 consecutive layers from each of s={done} "
+                       f"({boards} of the table's {total} boards, {100.0 * boards / total:.1f}%; "
+                       f"gen {t_gen:.2f}s, solve {t_solve:.2f}s); outputs bit-identical to the "
+                       f"GPU's: {exact}",
+                sample_fraction=boards / total)
 
 
 def micro(args):
@@ -472,6 +543,74 @@ def dropin_bench(args):
     print(json.dumps(out))
 
 
+def stage_roofline(cg, dom, steps):
+    """roofline entry of the dominant stage over its timed launches, per
+    stream.  For the sort stage, a sort's algorithmic bytes are 8 B per key
+    in and 8 B per unique key out (carried by its final kernel), over the
+    time of all of its kernels; the others are single kernels."""
+    kinds = SORT_KINDS if dom == "sort" else [dom]
+    final = SORT_FINAL if dom == "sort" else [dom]
+    lanes = []
+    for st in (0, 1):
+        ms = b = 0.0
+        n = 0
+        per = {}
+        for k in kinds:
+            c, m, by = cg.prof_query_stream(k, st)
+            if c <= 0:
+                continue
+            ms += m
+            per[k] = {"launches": c // max(steps, 1), "ms_per_step": m / max(steps, 1)}
+            if k in final:
+                b += by
+                n += c
+        lanes.append({"units": n, "ms": ms, "bytes": b, "kernels": per})
+    m0 = lanes[0]
+    avg_ms = m0["ms"] / max(m0["units"], 1)
+    avg_bytes = m0["bytes"] / max(m0["units"], 1)
+    achieved = avg_bytes / (avg_ms * 1e6) if avg_ms else 0.0
+    s1 = lanes[1]
+    traffic, tsrc = stage_traffic(kinds, final)
+    out = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+           "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+           "traffic_source": tsrc,
+           "traffic_over_algorithmic": (traffic / avg_bytes) if traffic and avg_bytes else None,
+           "units_per_step": m0["units"] // max(steps, 1),
+           "avg_unit_us": avg_ms * 1e3, "avg_algorithmic_bytes": avg_bytes,
+           "stream": "main (the second stream's units, which overlap the next step's moves, are "
+                     "in second_stream)",
+           "kernels_main_stream": m0["kernels"],
+           "second_stream": {"units": s1["units"] // max(steps, 1),
+                             "avg_unit_us": 1e3 * s1["ms"] / max(s1["units"], 1),
+                             "avg_algorithmic_bytes": s1["bytes"] / max(s1["units"], 1),
+                             "achieved": (s1["bytes"] / s1["ms"] / 1e6) if s1["ms"] else 0.0,
+                             "kernels": s1["kernels"]}}
+    if dom == "sort":
+        out["unit_of_work"] = ("one sort + dedupe (main stream: the move candidates of a layer "
+                               "step -> M_s); achieved = (8 N_cand + 8 N_uniq) / time of all of "
+                               "the sort's kernels")
+    return out
+
+
+def stage_traffic(kinds, final):
+    """HBM bytes per unit of the stage (main stream) from the calibrated PMC
+    record in profiles/pmc_traffic.json (C2 workload), or None"""
+    tf = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    if not os.path.exists(tf):
+        return None, None
+    with open(tf) as f:
+        tj = json.load(f)
+    km = tj.get("kernels_main_stream", {})
+    if tj.get("workload") != "c2" or not all(k in km for k in final):
+        return None, None
+    units = sum(km[k]["launches"] for k in final)
+    tot = sum((km[k]["read_bytes_per_launch"] + km[k]["write_bytes_per_launch"]) * km[k]["launches"]
+              for k in kinds if k in km)
+    return (tot / units if units else None,
+            "profiles/pmc_traffic.json (rocprofv3 FETCH_SIZE / WRITE_SIZE per dispatch, calibrated "
+            "by a 1 GiB copy; main-stream dispatches of the stage's kernels per unit)")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -479,6 +618,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adjoint-reps", type=int, default=3, help="sweeps averaged for the adjoint ratio")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0,
+                    help="CPU baseline: about this much oracle time on sampled layer windows")
+    ap.add_argument("--cpu-full", action="store_true",
+                    help="CPU baseline: the whole table (generate + solve) instead of a sample")
     ap.add_argument("--micro", action="store_true", help="kernel roofline microbench instead")
     ap.add_argument("--micro-log2", type=int, default=30, help="log2 of the microbench board count")
     ap.add_argument("--batch", action="store_true", help="batched-tables benchmark instead")
@@ -541,8 +685,8 @@ def main():
     for _ in range(args.warmup):
         step()
     # one untimed step with every kernel timed gives the per-kernel breakdown
-    # and names the dominant kernel; the timed steps then time only that
-    # kernel's launches (two events per launch), for the roofline
+    # and names the dominant stage; the timed steps then time only that
+    # stage's launches (two events per launch), for the roofline
     cg.prof_enable(True)
     cg.prof_select(None)
     cg.prof_reset()
@@ -552,7 +696,9 @@ def main():
     for k in KERNELS:
         n, ms, b = cg.prof_query(k)
         kern[k] = dict(launches=n, ms=ms, bytes=b)
-    dom = max(KERNELS, key=lambda k: kern[k]["ms"])
+    stage_ms = {k: kern[k]["ms"] for k in ("move", "spawn", "solve")}
+    stage_ms["sort"] = sum(kern[k]["ms"] for k in SORT_KINDS)
+    dom = max(STAGES, key=lambda k: stage_ms[k])
     cg.prof_select(dom)
     cg.prof_reset()
     cg.lib().cg_dev_sync()
@@ -566,28 +712,9 @@ def main():
     T = (sum(gen_t) + sum(sol_t)) / args.steps
     value = total / T
 
-    # the dominant kernel's launches on the main stream: the second stream's
-    # (the carry sort) share the GPU with the next step's moves, so their
-    # event times include that contention; they are reported beside it
-    by_stream = [cg.prof_query_stream(dom, s) for s in (0, 1)]
+    roof = stage_roofline(cg, dom, args.steps)
     cg.prof_select(None)
     cg.prof_enable(False)
-    n0, ms0, b0 = by_stream[0]
-    avg_ms = ms0 / max(n0, 1)
-    avg_bytes = b0 / max(n0, 1)
-    achieved = avg_bytes / (avg_ms * 1e6) if avg_ms else 0.0
-    n1, ms1, b1 = by_stream[1]
-    second = {"launches": n1 // max(args.steps, 1), "avg_launch_us": 1e3 * ms1 / max(n1, 1),
-              "avg_algorithmic_bytes": b1 / max(n1, 1),
-              "achieved": (b1 / ms1 / 1e6) if ms1 else 0.0}
-    traffic = None
-    tf = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "pmc_traffic.json")
-    if os.path.exists(tf):
-        with open(tf) as f:
-            tj = json.load(f)
-        k = tj.get("kernels_main_stream", {}).get(dom)
-        if k and tj.get("workload") == "c2":
-            traffic = k["read_bytes_per_launch"] + k["write_bytes_per_launch"]
 
     out = {
         "metric": METRIC, "value": value, "unit": "boards/s", "n_gpus": 1,
@@ -599,42 +726,42 @@ def main():
                                "generate 492..1100, solve 1100..0, winstate 0x0000900000000000",
                    "layers": tb.num_layers, "boards": total,
                    "gen_ms": 1e3 * sum(gen_t) / args.steps, "solve_ms": 1e3 * sum(sol_t) / args.steps},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "traffic_source": "profiles/pmc_traffic.json (rocprofv3 FETCH_SIZE / WRITE_SIZE, "
-                                       "calibrated by a 1 GiB copy; main-stream launches)"
-                     if traffic else None,
-                     "launches_per_step": n0 // max(args.steps, 1),
-                     "avg_launch_us": avg_ms * 1e3, "avg_algorithmic_bytes": avg_bytes,
-                     "stream": "main (the second stream's launches, which overlap the next "
-                               "step's moves, are in second_stream)",
-                     "second_stream": second},
+        "roofline": roof,
         "tuning": tuning,
         "kernels": {k: {"launches": v["launches"], "ms_per_step": v["ms"],
                         "GB_s": (v["bytes"] / v["ms"] / 1e6) if v["ms"] else 0.0}
                     for k, v in kern.items()},
+        "stages_ms_per_step": stage_ms,
         "kernels_source": "one untimed step after the warm-up with every kernel timed "
-                          "(the timed steps time only the roofline kernel)",
+                          "(the timed steps time only the roofline stage's kernels)",
     }
 
     if not args.no_adjoint:
+        # mean over several sweeps: plain solve, solve keeping the argmax, adjoint
         tb.solve(WINSTATES)
         cg.lib().cg_dev_sync()
-        t0 = time.perf_counter()
-        tb.solve(WINSTATES)
-        t1 = time.perf_counter()
-        tb.solve(WINSTATES, keep_argmax=True)
-        t2 = time.perf_counter()
-        P, dP, D = tb.adjoint(INITIAL)
-        t3 = time.perf_counter()
-        out["adjoint"] = {"P_win": P, "dP_dp4": dP, "plain_solve_ms": 1e3 * (t1 - t0),
-                          "solve_with_argmax_ms": 1e3 * (t2 - t1), "adjoint_sweep_ms": 1e3 * (t3 - t2),
-                          "adjoint_over_solve": (t3 - t2) / (t1 - t0),
-                          "argmax_solve_plus_adjoint_over_solve": (t3 - t1) / (t1 - t0)}
+        reps = max(1, args.adjoint_reps)
+        ts, ta, tj = [], [], []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            tb.solve(WINSTATES)
+            t1 = time.perf_counter()
+            tb.solve(WINSTATES, keep_argmax=True)
+            t2 = time.perf_counter()
+            P, dP, D = tb.adjoint(INITIAL)
+            t3 = time.perf_counter()
+            ts.append(t1 - t0)
+            ta.append(t2 - t1)
+            tj.append(t3 - t2)
+        ms, ma, mj = (1e3 * sum(x) / reps for x in (ts, ta, tj))
+        out["adjoint"] = {"P_win": P, "dP_dp4": dP, "sweeps": reps, "plain_solve_ms": ms,
+                          "solve_with_argmax_ms": ma, "adjoint_sweep_ms": mj,
+                          "adjoint_sweep_ms_each": [1e3 * x for x in tj],
+                          "adjoint_over_solve": mj / ms,
+                          "argmax_solve_plus_adjoint_over_solve": (ma + mj) / ms}
 
     if not args.no_cpu:
-        cores = min(os.cpu_count() or 1, 16)
-        out["cpu_baseline"] = cpu_baseline(tb, cg, cores)
+        out["cpu_baseline"] = cpu_baseline(tb, cg, baseline_cores(), args.cpu_seconds, args.cpu_full)
         out["speedup_vs_cpu"] = value / out["cpu_baseline"]["value"]
     tb.close()
     print(json.dumps(out))

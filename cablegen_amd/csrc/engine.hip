@@ -215,6 +215,12 @@ struct ProfRec {
 };
 static bool g_prof = false;
 static int g_prof_only = -1;   /* >= 0: only this kernel kind is timed */
+/* cg_prof_select("sort"): every kernel of the sort stage is timed */
+static constexpr int PK_GROUP_SORT = 1000;
+static bool is_sort_kind(int kind)
+{
+    return kind == PK_SORT_HIST || kind == PK_SORT_PASS || kind == PK_UNIQUE;
+}
 static std::vector<ProfRec> g_recs;
 static std::vector<hipEvent_t> g_evpool;
 
@@ -238,7 +244,8 @@ static long g_launches_s[2][PK_N];
 /* whether launches of this kernel kind are being timed */
 static bool prof_wants(int kind)
 {
-    return g_prof && (g_prof_only < 0 || kind == g_prof_only);
+    return g_prof && (g_prof_only < 0 || kind == g_prof_only ||
+                      (g_prof_only == PK_GROUP_SORT && is_sort_kind(kind)));
 }
 
 static int prof_begin(int kind, hipStream_t st = nullptr)
@@ -329,6 +336,10 @@ extern "C" int cg_prof_select(const char *kernel)
             g_prof_only = i;
             return 0;
         }
+    if (!strcmp(kernel, "sort")) {
+        g_prof_only = PK_GROUP_SORT;
+        return 0;
+    }
     return fail_msg("cg_prof_select: unknown kernel");
 }
 
@@ -798,6 +809,9 @@ struct cg_tablebase {
     int solve_lo = 0;          /* lowest solved layer index */
     std::vector<uint64_t> W;   /* expanded winstates */
     Buf P, C, P2, C2;
+    /* adjoint workspace, kept between calls: occupation of every board,
+     * D per layer + P, the initial boards */
+    Buf occ, dD, init;
 };
 
 static LayerView view_of(const cg_tablebase *tb, int idx)
@@ -845,6 +859,9 @@ extern "C" void cg_tb_free(cg_tablebase *tb)
     buf_free(tb->C);
     buf_free(tb->P2);
     buf_free(tb->C2);
+    buf_free(tb->occ);
+    buf_free(tb->dD);
+    buf_free(tb->init);
     delete tb;
 }
 
@@ -954,18 +971,17 @@ extern "C" int cg_tb_adjoint(cg_tablebase *tb, const uint64_t *initial, size_t n
     int nl = (int)tb->L.size();
     WinRules w;
     if (upload_win(tb->W.data(), tb->W.size(), tb->r, &w)) return -1;
-    /* occupation per layer + D per layer + P, in one scratch allocation */
+    /* occupation per layer + D per layer + P: workspace of the tablebase,
+     * allocated on the first call (and when a larger table needs more) */
     size_t tot = 0;
     std::vector<size_t> off(nl + 1);
     for (int i = 0; i < nl; i++) off[i] = tot, tot += tb->L[i].n;
     off[nl] = tot;
-    double *occ = nullptr, *dD = nullptr;
-    uint64_t *dinit = nullptr;
-    HC(hipMalloc((void **)&occ, std::max<size_t>(tot, 1) * 8));
-    HC(hipMalloc((void **)&dD, (nl + 2) * 8));
-    HC(hipMalloc((void **)&dinit, std::max<size_t>(n, 1) * 8));
-    HC(hipMemsetAsync(occ, 0, std::max<size_t>(tot, 1) * 8, E.st));
-    HC(hipMemsetAsync(dD, 0, (nl + 2) * 8, E.st));
+    if (ensure(tb->occ, tot + 1) || ensure(tb->dD, nl + 2) || ensure(tb->init, n + 1)) return -1;
+    double *occ = (double *)tb->occ.p, *dD = (double *)tb->dD.p;
+    uint64_t *dinit = tb->init.p;
+    launch_fill_u64(E.st, tb->occ.p, tot + 1, 0);
+    launch_fill_u64(E.st, tb->dD.p, nl + 2, 0);
     if (n) HC(hipMemcpyAsync(dinit, initial, n * 8, hipMemcpyHostToDevice, E.st));
     double *dPp = dD + nl;
     if (nl > 0 && tb->solve_lo == 0)
@@ -983,9 +999,6 @@ extern "C" int cg_tb_adjoint(cg_tablebase *tb, const uint64_t *initial, size_t n
     std::vector<double> h(nl + 1);
     HC(hipMemcpyAsync(h.data(), dD, (nl + 1) * 8, hipMemcpyDeviceToHost, E.st));
     if (check_sync(E.st, "cg_tb_adjoint")) return -1;
-    hipFree(occ);
-    hipFree(dD);
-    hipFree(dinit);
     double s = 0;
     for (int i = 0; i < nl; i++) {
         s += h[i];

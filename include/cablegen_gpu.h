@@ -216,7 +216,8 @@ void cg_prof_reset(void);
  * "adjoint", "lookup".  Returns the number of launches. */
 long cg_prof_query(const char *kernel, double *total_ms, double *total_bytes);
 /* times only the launches of one kernel (names as for cg_prof_query; NULL or
- * "" times every kernel): each timed launch records two events */
+ * "" times every kernel; "sort" times every kernel of the sort stage): each
+ * timed launch records two events */
 int cg_prof_select(const char *kernel);
 /* the same for the launches of one stream: 0 the library's main stream, 1
  * the second stream, which runs the carry sort beside the next step's moves */
