@@ -425,13 +425,49 @@ def run_autotune(cg, args):
             "carry_stream": args.carry_stream}
 
 
+def census(args):
+    """Counts every table of the specified batched draw (seeds 0..63, end =
+    start + U[100, 400]) by a bounded device-resident generation: a table
+    that passes --census-max boards before its end sum is reported with the
+    boards counted so far and the sum reached."""
+    import cablegen_amd as cg
+    from cablegen_amd.batch_configs import spec_batch, census_sizes, SPEC_LO, SPEC_HI
+    cfgs = spec_batch(args.batch_tables)
+    t0 = time.perf_counter()
+    sizes = census_sizes(cg, cfgs, int(args.census_max))
+    rows = []
+    for c, (n, done, last) in zip(cfgs, sizes):
+        rows.append({"seed": c["seed"], "formation": c["formation"], "initial": "0x%016x" % c["initial"][0],
+                     "start": c["start"], "end": c["end"], "boards": n, "complete": done,
+                     "last_sum": last})
+    fit = [r for r in rows if r["complete"]]
+    out = {"metric": "census of the batched draw", "tables": len(rows),
+           "draw": "seeds 0..%d, end = start + 2*U{%d..%d}" % (len(rows) - 1, SPEC_LO, SPEC_HI),
+           "bound_boards": int(args.census_max), "complete": len(fit),
+           "complete_boards": sum(r["boards"] for r in fit),
+           "seconds": time.perf_counter() - t0, "rows": rows}
+    print(json.dumps(out))
+
+
 def batch_bench(args):
     """Batching: 64 small tables (cablegen_amd.batch_configs, seeds 0..63)
-    generated and solved together vs one at a time on the same GPU."""
+    generated and solved together vs one at a time on the same GPU.  With
+    --batch-spec the draw is the specified one (end = start + U[100, 400])
+    and the run takes the tables a bounded count completes under
+    --batch-fit boards each (the others cannot be kept resident)."""
     import cablegen_amd as cg
-    from cablegen_amd.batch_configs import batch
-    cfgs = batch(args.batch_tables, args.batch_lo, args.batch_hi)
+    from cablegen_amd.batch_configs import batch, spec_batch, census_sizes
     r = cg.rules(ignore_f=True)
+    excluded = []
+    if args.batch_spec:
+        allc = spec_batch(args.batch_tables)
+        sizes = census_sizes(cg, allc, int(args.batch_fit))
+        cfgs = [c for c, (n, done, _) in zip(allc, sizes) if done]
+        excluded = [{"seed": c["seed"], "formation": c["formation"], "boards_counted": n,
+                     "last_sum": last, "end": c["end"]}
+                    for c, (n, done, last) in zip(allc, sizes) if not done]
+    else:
+        cfgs = batch(args.batch_tables, args.batch_lo, args.batch_hi)
     tuning = run_autotune(cg, args)
 
     tb = cg.Tablebase(r)   # one device-resident table reused: its arena is kept
@@ -480,9 +516,14 @@ def batch_bench(args):
            "batched_sorts": kern["sort_hist"][0], "batched_sort_passes": kern["sort_pass"][0],
            "cell_coded_sorts": {"one_at_a_time": c1 - c0, "batched": c2 - c1},
            "tuning": tuning,
-           "workload": "seeds 0..%d: formation in {2x4, 3x4, DPDGap}, 2-4 tiles of 2^1..2^7, "
-                       "goal 2^(max+2) top-left, end = start + 2*U{%d..%d}, ignore_f, no prune"
-                       % (len(cfgs) - 1, args.batch_lo, args.batch_hi)}
+           "workload": ("specified draw, seeds 0..%d: end = start + 2*U{50..200}; the %d tables "
+                        "whose bounded count completed under %d boards each (the others are listed "
+                        "in excluded)" % (args.batch_tables - 1, len(cfgs), int(args.batch_fit))
+                        if args.batch_spec else
+                        "seeds 0..%d: formation in {2x4, 3x4, DPDGap}, 2-4 tiles of 2^1..2^7, "
+                        "goal 2^(max+2) top-left, end = start + 2*U{%d..%d}, ignore_f, no prune"
+                        % (len(cfgs) - 1, args.batch_lo, args.batch_hi)),
+           "excluded": excluded}
     B.close()
     tb.close()
     print(json.dumps(out))
@@ -633,6 +674,12 @@ def main():
     ap.add_argument("--batch-tables", type=int, default=64)
     ap.add_argument("--batch-lo", type=int, default=5, help="end = start + 2*U{lo..hi}")
     ap.add_argument("--batch-hi", type=int, default=20)
+    ap.add_argument("--batch-spec", action="store_true",
+                    help="the specified draw (end = start + U[100,400]), tables that fit --batch-fit")
+    ap.add_argument("--batch-fit", type=float, default=2e8,
+                    help="--batch-spec: per-table bound on the boards of a table that is kept")
+    ap.add_argument("--census", action="store_true", help="count the specified batched draw's tables")
+    ap.add_argument("--census-max", type=float, default=4e9, help="--census: per-table bound (boards)")
     ap.add_argument("--prof-summary", default=None, help="summarise rocprofv3 output in this directory")
     ap.add_argument("--prof-rows", type=int, default=14)
     ap.add_argument("--pmc", action="store_true", help="with --prof-summary: PMC traffic / L2 summary")
@@ -663,6 +710,9 @@ def main():
         return
     if args.batch:
         batch_bench(args)
+        return
+    if args.census:
+        census(args)
         return
     if args.dropin:
         dropin_bench(args)

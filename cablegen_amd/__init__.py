@@ -66,6 +66,8 @@ def lib():
     L.cg_tb_create.argtypes = [C.POINTER(Rules)]
     L.cg_tb_free.argtypes = [C.c_void_p]
     L.cg_tb_generate.argtypes = [C.c_void_p, U64P, C.c_size_t, C.c_int, C.c_int]
+    L.cg_tb_generate_bounded.argtypes = [C.c_void_p, U64P, C.c_size_t, C.c_int, C.c_int, C.c_size_t,
+                                         C.POINTER(C.c_int)]
     L.cg_tb_solve.argtypes = [C.c_void_p, U64P, C.c_size_t, C.c_int, C.c_double, C.c_int]
     L.cg_tb_adjoint.argtypes = [C.c_void_p, U64P, C.c_size_t, F64P, F64P, F64P]
     L.cg_tb_num_layers.argtypes = [C.c_void_p]
@@ -210,6 +212,15 @@ class Tablebase:
     def generate(self, initial, start, end):
         a, p = _u64(initial)
         _check(self.L.cg_tb_generate(self.h, p, len(a), int(start), int(end)), "cg_tb_generate")
+
+    def generate_bounded(self, initial, start, end, max_boards):
+        """generates up to end, or stops after the first layer that takes the
+        total above max_boards; returns the tile sum of the last layer"""
+        a, p = _u64(initial)
+        last = C.c_int()
+        _check(self.L.cg_tb_generate_bounded(self.h, p, len(a), int(start), int(end), int(max_boards),
+                                             C.byref(last)), "cg_tb_generate_bounded")
+        return last.value
 
     def solve(self, winstates, end_solve=0, p4=0.1, keep_argmax=False):
         a, p = _u64(winstates)

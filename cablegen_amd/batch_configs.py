@@ -4,15 +4,19 @@ table_config(seed) draws one table (numpy Generator seeded with `seed`):
   * formation uniform over {2x4, 3x4 (bottom row 0xF), DPDGap};
   * 2-4 tiles with exponents uniform in 1..7 in distinct random free cells;
   * goal 2^(max exponent + 2) in the top-left cell (free in all three);
-  * end tile sum = initial sum + 2 * U{lo..hi}.  Without pruning these
-    formations grow about 2-4x per 10 of tile sum (3x4 fastest), so an offset
-    of 100..400 makes every table a full-GPU job of 1e8-1e10 boards.  The
-    default U{5..20} (offset 10..40) gives 64 tables of 5e3..3.6e7 boards
-    (1.6e8 in all, median 1.5e5); U{10..40} gives 4e3..4.9e8 (1.7e9 in all,
-    five tables hold two thirds of the boards);
+  * end tile sum = initial sum + 2 * U{lo..hi}.  The specified batch draws
+    the offset from U[100, 400] (SPEC_LO, SPEC_HI = 50, 200).  Without
+    pruning, the 3x4 and DPDGap formations grow about 2-4x per 10 of tile
+    sum, so most of those tables hold far more boards than one GPU can keep
+    resident: `python bench.py --census` counts every table of the draw
+    (bounded), and spec_fitting() keeps the tables whose count completed
+    under a bound.  The smaller draw U{5..20} (offset 10..40; 64 tables of
+    5e3..3.6e7 boards) is kept as a second workload;
   * rules: formation cells as walls (ignore_f), no pruning.
 """
 import numpy as np
+
+SPEC_LO, SPEC_HI = 50, 200   # end = start + 2 * U{50..200}, i.e. start + U[100, 400] in steps of 2
 
 FORMATIONS = {
     "2x4": 0x00000000FFFFFFFF,
@@ -50,3 +54,23 @@ def table_config(seed, lo=5, hi=20):
 
 def batch(n=64, lo=5, hi=20):
     return [table_config(s, lo, hi) for s in range(n)]
+
+
+def spec_batch(n=64):
+    """the specified draw: seeds 0..n-1, end = start + 2 * U{50..200}"""
+    return batch(n, SPEC_LO, SPEC_HI)
+
+
+def census_sizes(cg, cfgs, max_boards):
+    """per table: (total boards, complete) from a bounded device-resident
+    generation; complete is False when the table passed max_boards before
+    its end sum (the total then counts the layers generated so far)"""
+    out = []
+    tb = cg.Tablebase(cg.rules(**cfgs[0]["rules"]))
+    try:
+        for c in cfgs:
+            last = tb.generate_bounded(c["initial"], c["start"], c["end"], max_boards)
+            out.append((tb.total_boards(), last >= c["end"], last))
+    finally:
+        tb.close()
+    return out

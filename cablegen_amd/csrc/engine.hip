@@ -867,6 +867,13 @@ extern "C" void cg_tb_free(cg_tablebase *tb)
 
 extern "C" int cg_tb_generate(cg_tablebase *tb, const uint64_t *initial, size_t n, int start, int end)
 {
+    return cg_tb_generate_bounded(tb, initial, n, start, end, 0, nullptr);
+}
+
+extern "C" int cg_tb_generate_bounded(cg_tablebase *tb, const uint64_t *initial, size_t n, int start,
+                                      int end, size_t max_boards, int *last_sum)
+{
+    if (last_sum) *last_sum = start - 2;
     if (eng_init()) return -1;
     HC(hipStreamSynchronize(E.st));
     tb->arena.reset();
@@ -880,6 +887,7 @@ extern "C" int cg_tb_generate(cg_tablebase *tb, const uint64_t *initial, size_t 
     StepCounts pc;
     ArenaLayerAlloc la(&tb->arena);
     ValueBound vb = value_bound_of(initial, n);
+    size_t total = 0;
     for (int s = start; s <= end; s += 2) {
         TbLayer l;
         size_t nP2 = 0, nC2 = 0;
@@ -892,10 +900,13 @@ extern "C" int cg_tb_generate(cg_tablebase *tb, const uint64_t *initial, size_t 
         if (!l.dir) return fail_msg("index allocation failed");
         if (build_index(l.keys, l.n, l.plan, l.dir)) return -1;
         tb->L.push_back(l);
+        total += l.n;
+        if (last_sum) *last_sum = s;
         std::swap(tb->P, tb->P2);
         std::swap(tb->C, tb->C2);
         nP = nP2;
         nC = nC2;
+        if (max_boards && total > max_boards) break;
     }
     if (counts_finish(pc)) return -1;
     return check_sync(E.st, "cg_tb_generate");

@@ -48,6 +48,12 @@ void cg_tb_free(cg_tablebase *tb);
  * initial boards (host memory), keeping every layer in HBM.  Same recurrence
  * as generate() (src/generate.c:551-592). */
 int cg_tb_generate(cg_tablebase *tb, const uint64_t *initial, size_t n, int start, int end);
+/* As cg_tb_generate, but stops after the first layer that brings the total
+ * above max_boards (0: no bound), so a table too large to keep resident is
+ * counted up to that point instead of exhausting HBM; *last_sum (may be NULL)
+ * receives the tile sum of the last layer generated. */
+int cg_tb_generate_bounded(cg_tablebase *tb, const uint64_t *initial, size_t n, int start, int end,
+                           size_t max_boards, int *last_sum);
 
 /* Solves every stored layer with s >= end_solve from the top down, as
  * solve() does (src/solve.c:172-224).  winstates are the unexpanded boards
