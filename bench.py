@@ -679,7 +679,7 @@ def main():
     ap.add_argument("--batch-fit", type=float, default=2e8,
                     help="--batch-spec: per-table bound on the boards of a table that is kept")
     ap.add_argument("--census", action="store_true", help="count the specified batched draw's tables")
-    ap.add_argument("--census-max", type=float, default=4e9, help="--census: per-table bound (boards)")
+    ap.add_argument("--census-max", type=float, default=1e9, help="--census: per-table bound (boards)")
     ap.add_argument("--prof-summary", default=None, help="summarise rocprofv3 output in this directory")
     ap.add_argument("--prof-rows", type=int, default=14)
     ap.add_argument("--pmc", action="store_true", help="with --prof-summary: PMC traffic / L2 summary")

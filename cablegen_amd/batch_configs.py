@@ -66,11 +66,14 @@ def census_sizes(cg, cfgs, max_boards):
     generation; complete is False when the table passed max_boards before
     its end sum (the total then counts the layers generated so far)"""
     out = []
-    tb = cg.Tablebase(cg.rules(**cfgs[0]["rules"]))
-    try:
-        for c in cfgs:
+    for c in cfgs:
+        tb = cg.Tablebase(cg.rules(**c["rules"]))
+        try:
             last = tb.generate_bounded(c["initial"], c["start"], c["end"], max_boards)
             out.append((tb.total_boards(), last >= c["end"], last))
-    finally:
-        tb.close()
+        except cg.CablegenError:
+            # device memory ran out inside a step: the layers before it are counted
+            out.append((tb.total_boards(), False, c["start"] + 2 * (tb.num_layers - 1)))
+        finally:
+            tb.close()
     return out
