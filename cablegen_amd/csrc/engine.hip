@@ -96,6 +96,11 @@ struct Engine {
     size_t scratch2_cap = 0;
     uint64_t *d_wmask = nullptr, *d_wval = nullptr;
     size_t wcap = 0;
+    /* the sample sort's bucket buffers of the two lanes */
+    Buf Pt, Pt2;
+    /* a sample sort enqueued before the last counter read had an oversize
+     * bucket: its output is incomplete and the sort must be repeated */
+    bool oversize = false;
 };
 static Engine E;
 
@@ -200,10 +205,12 @@ static const uint16_t *get_lut(int ignore_f)
 /* kernel timers                                                             */
 /* ------------------------------------------------------------------------ */
 
+/* the order matches the kinds the sort's launchers pass (kernels.h) */
 enum { PK_MOVE, PK_SPAWN, PK_SORT_HIST, PK_SORT_PASS, PK_UNIQUE, PK_SOLVE, PK_ADJOINT, PK_LOOKUP,
-       PK_N };
+       PK_SORT_SPLIT, PK_SORT_PART, PK_SORT_LOCAL, PK_N };
 static const char *PK_NAMES[PK_N] = { "move", "spawn", "sort_hist", "sort_pass",
-                                      "unique", "solve", "adjoint", "lookup" };
+                                      "unique", "solve", "adjoint", "lookup",
+                                      "sort_split", "sort_part", "sort_local" };
 
 struct ProfRec {
     int kind;
@@ -219,7 +226,8 @@ static int g_prof_only = -1;   /* >= 0: only this kernel kind is timed */
 static constexpr int PK_GROUP_SORT = 1000;
 static bool is_sort_kind(int kind)
 {
-    return kind == PK_SORT_HIST || kind == PK_SORT_PASS || kind == PK_UNIQUE;
+    return kind == PK_SORT_HIST || kind == PK_SORT_PASS || kind == PK_UNIQUE || kind == PK_SORT_SPLIT ||
+           kind == PK_SORT_PART || kind == PK_SORT_LOCAL;
 }
 static std::vector<ProfRec> g_recs;
 static std::vector<hipEvent_t> g_evpool;
@@ -461,8 +469,12 @@ static int read_counters(int lo, int n)
     for (int i = lo; i < lo + n; i++) E.h_cnt[i] = ((volatile unsigned long long *)E.h_pub)[i];
     unsigned e = (unsigned)((volatile unsigned long long *)E.h_pub)[8];
     if (e) {
-        hipMemsetAsync(E.d_err, 0, sizeof(unsigned), E.st);
+        launch_fill_u64(E.st, (uint64_t *)E.d_err, 1, 0);
         hipStreamSynchronize(E.st);
+        if (e == SORT_ERR_OVERSIZE) {   /* the caller repeats that sort */
+            E.oversize = true;
+            return 0;
+        }
         if (e & 2u) return fail_msg("batch: a board's formation cells left the table's canonical place");
         if (e & SORT_ERR_CODE) return fail_msg("sort: a key's cell value exceeds the bound of its layer");
         return fail_msg("sort: a chained-scan wait exceeded its budget");
@@ -476,17 +488,21 @@ static uint64_t vary_of(size_t n, uint64_t o, uint64_t a) { return n > 1 ? (o & 
 /* lane 0: the main stream; lane 1: the second stream (its own ping-pong
  * buffer, scratch and sort context) -- the caller orders the main stream
  * after it (join_lane1) before using its output */
+/* allow_sample = false: the LSD path (the repeat of a sample sort whose
+ * bucket overflowed); the sample sort leaves buf unchanged, so a repeat can
+ * read it again */
 static int sort_enqueue(uint64_t *buf, size_t n, uint64_t vary, uint64_t *dst, int slot, int *hu,
-                        int lane = 0, uint64_t cell_max = CELLS_UNBOUNDED)
+                        int lane = 0, uint64_t cell_max = CELLS_UNBOUNDED, bool allow_sample = true)
 {
+    const size_t pk = allow_sample ? sort_part_keys(n) : 0;
     if (lane == 1) {
-        if (ensure(E.B2, n + 1) || ensure_scratch2(n)) return -1;
+        if (ensure(E.B2, n + 1) || ensure_scratch2(n) || (pk && ensure(E.Pt2, pk))) return -1;
         sort_unique(E.st2, buf, E.B2.p, n, vary, E.scratch2, dst, E.d_cnt + slot, E.d_err, hu, 1,
-                    cell_max);
+                    cell_max, E.Pt2.p, E.Pt2.cap, allow_sample);
     } else {
-        if (ensure(E.B, n + 1) || ensure_scratch(n)) return -1;
+        if (ensure(E.B, n + 1) || ensure_scratch(n) || (pk && ensure(E.Pt, pk))) return -1;
         sort_unique(E.st, buf, E.B.p, n, vary, E.scratch, dst, E.d_cnt + slot, E.d_err, hu, 0,
-                    cell_max);
+                    cell_max, E.Pt.p, E.Pt.cap, allow_sample);
     }
     hipError_t e = kernels_take_error();
     if (e != hipSuccess) return fail("sort_unique", e);
@@ -522,6 +538,9 @@ struct StepCounts {
     size_t sumE = 0;         /* input size of the carry sort (upper bound of the count) */
     int hu4 = -1;            /* timer handle of its compaction, bytes set on arrival */
     size_t last_nM = 0;      /* previous layer size: sizes the spawn buffers */
+    /* the pending carry sort's input (in E.Cb until the next spawn), key bits
+     * and cell bound: what its repeat needs */
+    uint64_t vary4 = 0, cell4 = CELLS_UNBOUNDED;
 };
 
 /* spawn buffers are sized before |M_s| is known: for about twice the previous
@@ -542,11 +561,25 @@ static void counts_arrived(StepCounts &pc, size_t nC)
     pc.pending = false;
 }
 
+/* The pending carry sort (input still in E.Cb) had an oversize bucket: it
+ * is repeated on the LSD path into C, on the main stream. */
+static int carry_repeat(StepCounts &pc, uint64_t *C)
+{
+    E.oversize = false;
+    int hu4;
+    if (sort_enqueue(E.Cb.p, pc.sumE, pc.vary4, C, 6, &hu4, 0, pc.cell4, false) || read_counters(6, 1))
+        return -1;
+    if (E.oversize) return fail_msg("sort: an LSD sort reported an oversize bucket");
+    pc.hu4 = hu4;
+    return 0;
+}
+
 /* brings the pending count to the host (end of a generation) */
-static int counts_finish(StepCounts &pc)
+static int counts_finish(StepCounts &pc, uint64_t *C)
 {
     if (!pc.pending) return 0;
     if (join_lane1() || read_counters(6, 1)) return -1;
+    if (E.oversize && carry_repeat(pc, C)) return -1;
     counts_arrived(pc, E.h_cnt[6]);
     return 0;
 }
@@ -615,7 +648,7 @@ static uint64_t cell_bound(const ValueBound *v, long s, uint64_t key_and)
  * nC is an upper bound and the exact count is on the device; on return *nC2
  * is an upper bound, pc.pending is set and *nP2 is exact.  vb (optional)
  * bounds the cell values for the sorts and advances to the next step's sum. */
-static int gen_step(const cg_rules &r, const uint64_t *P, size_t nP, const uint64_t *C, size_t nC,
+static int gen_step(const cg_rules &r, const uint64_t *P, size_t nP, uint64_t *C, size_t nC,
                     StepCounts &pc, LayerAlloc &la, uint64_t **M, size_t *nM,
                     Buf &nxtP, size_t *nP2, Buf &nxtC, size_t *nC2, ValueBound *vb,
                     uint64_t *m_or = nullptr, uint64_t *m_and = nullptr)
@@ -633,12 +666,17 @@ static int gen_step(const cg_rules &r, const uint64_t *P, size_t nP, const uint6
      * its count and output are needed from here on */
     if (pc.pending && join_lane1()) return -1;
     if (read_counters(0, pc.pending ? 7 : 3)) return -1;
+    size_t ncand = E.h_cnt[0];
+    uint64_t mor = E.h_cnt[1], mand = E.h_cnt[2];
+    if (E.oversize) {
+        /* only the carry sort can be pending here (the M sort of the last
+         * step was checked at its spawns) */
+        if (!pc.pending || carry_repeat(pc, C)) return pc.pending ? -1 : fail_msg("sort: unexpected oversize");
+    }
     if (pc.pending) {
         nC = E.h_cnt[6];
         counts_arrived(pc, nC);
     }
-    size_t ncand = E.h_cnt[0];
-    uint64_t mor = E.h_cnt[1], mand = E.h_cnt[2];
     if (m_or) *m_or = mor;
     if (m_and) *m_and = mand;
     prof_set_bytes(h, 8.0 * nP + 8.0 * ncand);
@@ -646,8 +684,8 @@ static int gen_step(const cg_rules &r, const uint64_t *P, size_t nP, const uint6
     *M = la.alloc(ncand);
     if (!*M) return fail_msg("layer allocation failed");
     int hu;
-    if (sort_enqueue(E.A.p, ncand, vary_of(ncand, mor, mand), *M, 7, &hu, 0, cell_bound(vb, s, mand)))
-        return -1;
+    const uint64_t mvary = vary_of(ncand, mor, mand), mcell = cell_bound(vb, s, mand);
+    if (sort_enqueue(E.A.p, ncand, mvary, *M, 7, &hu, 0, mcell)) return -1;
     /* spawns: every board has at most 16 - (cells nonzero in all of M_s)
      * empties; the kernel reads |M_s| from the device, the grid and the
      * buffers cover ncand */
@@ -655,7 +693,7 @@ static int gen_step(const cg_rules &r, const uint64_t *P, size_t nP, const uint6
     size_t cap = emax * spawn_rows_estimate(pc, ncand);
     int hs = -1;
     size_t sumE = 0;
-    for (int attempt = 0; attempt < 2; attempt++) {
+    for (int attempt = 0; attempt < 3; attempt++) {
         if (ensure(nxtP, nC + cap + 1) || ensure(E.Cb, cap + 1)) return -1;
         if (nC) HC(hipMemcpyAsync(nxtP.p, C, nC * sizeof(uint64_t), hipMemcpyDeviceToDevice, E.st));
         launch_init_stats(E.st, E.d_cnt);
@@ -663,11 +701,19 @@ static int gen_step(const cg_rules &r, const uint64_t *P, size_t nP, const uint6
         launch_spawn(E.st, *M, ncand, E.d_cnt + 7, nxtP.p + nC, E.Cb.p, cap, E.d_cnt);
         prof_end(hs, 0);
         if (read_counters(0, 8)) return -1;
+        if (E.oversize) {
+            /* the M sort had an oversize bucket: repeat it on the LSD path
+             * (the candidates are unchanged), then the spawns */
+            E.oversize = false;
+            if (sort_enqueue(E.A.p, ncand, mvary, *M, 7, &hu, 0, mcell, false)) return -1;
+            continue;
+        }
         *nM = E.h_cnt[7];
         sumE = E.h_cnt[0];
         if (sumE <= cap) break;
         cap = emax * *nM;   /* exact bound, always enough */
     }
+    if (E.oversize || sumE > cap) return fail_msg("generation step: spawns or sort not settled");
     pc.last_nM = *nM;
     prof_set_bytes(hu, 8.0 * ncand + 8.0 * *nM);
     prof_set_bytes(hs, 8.0 * *nM + 16.0 * sumE);
@@ -678,9 +724,9 @@ static int gen_step(const cg_rules &r, const uint64_t *P, size_t nP, const uint6
      * count stays on the device until then */
     if (ensure(nxtC, sumE + 1)) return -1;
     int hu4;
-    if (sort_enqueue(E.Cb.p, sumE, vary_of(sumE, or4, and4), nxtC.p, 6, &hu4, g_carry_lane,
-                     cell_bound(vb, s + 4, and4)))
-        return -1;
+    pc.vary4 = vary_of(sumE, or4, and4);
+    pc.cell4 = cell_bound(vb, s + 4, and4);
+    if (sort_enqueue(E.Cb.p, sumE, pc.vary4, nxtC.p, 6, &hu4, g_carry_lane, pc.cell4)) return -1;
     if (vb) vb->s = s + 2;
     pc.pending = true;
     pc.sumE = sumE;
@@ -908,7 +954,7 @@ extern "C" int cg_tb_generate_bounded(cg_tablebase *tb, const uint64_t *initial,
         nC = nC2;
         if (max_boards && total > max_boards) break;
     }
-    if (counts_finish(pc)) return -1;
+    if (counts_finish(pc, tb->C.p)) return -1;
     return check_sync(E.st, "cg_tb_generate");
 }
 
@@ -1129,9 +1175,14 @@ extern "C" int cg_dev_sort_unique_bounded(uint64_t *d_keys, size_t n, uint64_t *
     launch_or_and(E.st, d_keys, n, E.d_cnt);
     if (read_counters(0, 3)) return -1;
     int hu;
-    if (sort_enqueue(d_keys, n, vary_of(n, E.h_cnt[1], E.h_cnt[2]), d_out, 5, &hu, 0, cell_max))
-        return -1;
+    const uint64_t vary = vary_of(n, E.h_cnt[1], E.h_cnt[2]);
+    if (sort_enqueue(d_keys, n, vary, d_out, 5, &hu, 0, cell_max)) return -1;
     if (read_counters(5, 1)) return -1;
+    if (E.oversize) {   /* a bucket of the sample sort overflowed: the LSD path */
+        E.oversize = false;
+        if (sort_enqueue(d_keys, n, vary, d_out, 5, &hu, 0, cell_max, false) || read_counters(5, 1))
+            return -1;
+    }
     *n_unique = E.h_cnt[5];
     prof_set_bytes(hu, 8.0 * n + 8.0 * *n_unique);
     return 0;
@@ -1266,6 +1317,21 @@ extern "C" int cg_tune_set(const char *knob, int value)
         g_carry_lane = value;
         return 0;
     }
+    if (!strcmp(knob, "sample_sort")) {
+        if (value != 0 && value != 1) return fail_msg("sample_sort: 0 or 1");
+        sort_set_sample(value, sort_sample_bucket(), sort_sample_cap());
+        return 0;
+    }
+    if (!strcmp(knob, "sample_bucket")) {
+        if (value < 1000 || value > 16384) return fail_msg("sample_bucket: 1000..16384");
+        sort_set_sample(sort_sample_on(), value, sort_sample_cap());
+        return 0;
+    }
+    if (!strcmp(knob, "sample_cap")) {
+        if (value < 0 || value > 16384) return fail_msg("sample_cap: 0 (16384) or 1..16384");
+        sort_set_sample(sort_sample_on(), sort_sample_bucket(), value);
+        return 0;
+    }
     if (!strcmp(knob, "reset") || !strcmp(knob, "builtin")) {
         sort_table_reset();
         sort_force(0, 0, 0);
@@ -1273,6 +1339,7 @@ extern "C" int cg_tune_set(const char *knob, int value)
         g_spawn_est_mode = 0;
         sort_set_cell_code(1);
         g_carry_lane = 1;
+        sort_set_sample(1, 6000, 0);
         if (!strcmp(knob, "reset") && !g_tune_path.empty()) return cg_tune_load(g_tune_path.c_str());
         return 0;
     }
@@ -1290,6 +1357,10 @@ extern "C" int cg_tune_get(const char *knob)
     if (!strcmp(knob, "cell_code")) return sort_cell_code();
     if (!strcmp(knob, "carry_stream")) return g_carry_lane;
     if (!strcmp(knob, "cell_code_sorts")) return (int)std::min<unsigned long long>(sort_code_count(), 0x7fffffff);
+    if (!strcmp(knob, "sample_sort")) return sort_sample_on();
+    if (!strcmp(knob, "sample_bucket")) return sort_sample_bucket();
+    if (!strcmp(knob, "sample_cap")) return sort_sample_cap();
+    if (!strcmp(knob, "sample_sorts")) return (int)std::min<unsigned long long>(sort_sample_count(), 0x7fffffff);
     return fail_msg("unknown tuning knob");
 }
 
@@ -1421,7 +1492,8 @@ int tune_sort_once(TuneBufs &b, size_t n, uint64_t mask, int reps, double *best)
         launch_copy_and_u64(E.st, b.work, b.src, n, mask);
         cg_prof_reset();
         int hu;
-        sort_unique(E.st, b.work, b.tmp, n, mask, E.scratch, b.dst, E.d_cnt + 7, E.d_err, &hu);
+        sort_unique(E.st, b.work, b.tmp, n, mask, E.scratch, b.dst, E.d_cnt + 7, E.d_err, &hu, 0,
+                    CELLS_UNBOUNDED, nullptr, 0, false);   /* the LSD configurations */
         hipError_t e = kernels_take_error();
         if (e != hipSuccess) return fail("autotune sort", e);
         double mh = 0, mp = 0;
@@ -1691,7 +1763,7 @@ extern "C" int cge_gen_fetch(uint64_t *host, size_t n)
 
 extern "C" void cge_gen_end(void)
 {
-    counts_finish(G.pc);
+    counts_finish(G.pc, G.C.p);
     buf_free(G.P);
     buf_free(G.C);
     buf_free(G.P2);
@@ -1981,17 +2053,19 @@ extern "C" int cg_batch_generate(cg_batch *b)
         prof_end(h, 0);
         if (pc.pending && join_lane1()) return -1;
         if (read_counters(0, pc.pending ? 7 : 3)) return -1;
+        size_t ncand = E.h_cnt[0];
+        uint64_t mor = E.h_cnt[1], mand = E.h_cnt[2];
+        if (E.oversize && (!pc.pending || carry_repeat(pc, b->C.p)))
+            return pc.pending ? -1 : fail_msg("sort: unexpected oversize");
         if (pc.pending) {
             nC = E.h_cnt[6];
             counts_arrived(pc, nC);
         }
-        size_t ncand = E.h_cnt[0];
-        uint64_t mor = E.h_cnt[1], mand = E.h_cnt[2];
         prof_set_bytes(h, 8.0 * nP + 8.0 * ncand);
         if (ensure(b->M, ncand + 1)) return -1;
         int hu;
-        if (sort_enqueue(E.A.p, ncand, vary_of(ncand, mor, mand), b->M.p, 7, &hu, 0, batch_cells))
-            return -1;
+        const uint64_t mvary = vary_of(ncand, mor, mand);
+        if (sort_enqueue(E.A.p, ncand, mvary, b->M.p, 7, &hu, 0, batch_cells)) return -1;
         /* boards of the step (count on the device until the next synchronisation) */
         st.keys = (uint64_t *)b->arena.alloc(std::max<size_t>(ncand, 1) * 8);
         if (!st.keys) return fail_msg("batch layer allocation failed");
@@ -2001,7 +2075,7 @@ extern "C" int cg_batch_generate(cg_batch *b)
         size_t cap = (size_t)maxfree * spawn_rows_estimate(pc, ncand);
         int hs = -1;
         size_t nM = 0, sumE = 0;
-        for (int attempt = 0; attempt < 2; attempt++) {
+        for (int attempt = 0; attempt < 3; attempt++) {
             if (ensure(b->P2, nC + cap + 1) || ensure(E.Cb, cap + 1)) return -1;
             if (nC) HC(hipMemcpyAsync(b->P2.p, b->C.p, nC * 8, hipMemcpyDeviceToDevice, E.st));
             launch_init_stats(E.st, E.d_cnt);
@@ -2012,11 +2086,22 @@ extern "C" int cg_batch_generate(cg_batch *b)
             HC(hipMemcpyAsync(b->h_tstat.data(), b->d_tstat, 3 * ntab * 8, hipMemcpyDeviceToHost,
                               E.st));
             if (read_counters(0, 8)) return -1;
+            if (E.oversize) {
+                /* the M sort had an oversize bucket: repeat it on the LSD path,
+                 * the decode and the spawns */
+                E.oversize = false;
+                if (sort_enqueue(E.A.p, ncand, mvary, b->M.p, 7, &hu, 0, batch_cells, false)) return -1;
+                HC(hipMemcpyAsync(b->d_tstat, init_tstat.data(), 3 * ntab * 8, hipMemcpyHostToDevice,
+                                  E.st));
+                launch_bdecode(E.st, b->M.p, ncand, E.d_cnt + 7, st.keys, b->d_tabs, b->d_tstat);
+                continue;
+            }
             nM = E.h_cnt[7];
             sumE = E.h_cnt[0];
             if (sumE <= cap) break;
             cap = (size_t)maxfree * nM;   /* exact bound, always enough */
         }
+        if (E.oversize || sumE > cap) return fail_msg("batch step: spawns or sort not settled");
         pc.last_nM = nM;
         prof_set_bytes(hu, 8.0 * ncand + 8.0 * nM);
         prof_set_bytes(hs, 8.0 * nM + 16.0 * sumE);
@@ -2026,9 +2111,9 @@ extern "C" int cg_batch_generate(cg_batch *b)
         size_t n2 = nC + sumE;
         if (ensure(b->C2, sumE + 1)) return -1;
         int hu4;
-        if (sort_enqueue(E.Cb.p, sumE, vary_of(sumE, or4, and4), b->C2.p, 6, &hu4, g_carry_lane,
-                         batch_cells))
-            return -1;
+        pc.vary4 = vary_of(sumE, or4, and4);
+        pc.cell4 = batch_cells;
+        if (sort_enqueue(E.Cb.p, sumE, pc.vary4, b->C2.p, 6, &hu4, g_carry_lane, batch_cells)) return -1;
         /* directories of every table of this step */
         st.n = nM;
         st.off.assign(ntab + 1, 0);
@@ -2067,7 +2152,7 @@ extern "C" int cg_batch_generate(cg_batch *b)
         nP = n2;
         nC = sumE;
     }
-    if (counts_finish(pc)) return -1;
+    if (counts_finish(pc, b->C.p)) return -1;
     return check_sync(E.st, "cg_batch_generate");
 }
 

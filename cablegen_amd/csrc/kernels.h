@@ -103,7 +103,8 @@ void launch_build_dir(hipStream_t st, const uint64_t *keys, size_t n, const Laye
                       uint32_t *dir, void *scratch);
 
 /* kernel-timer brackets provided by the engine (kind: 2 = radix histogram,
- * 3 = radix pass, 4 = unique), recorded on stream st; a negative handle
+ * 3 = radix pass, 4 = unique, 8 = sample-sort splitters, 9 = its partition,
+ * 10 = its bucket sort + dedupe), recorded on stream st; a negative handle
  * means timing is off */
 int prof_hook_begin(int kind, hipStream_t st);
 void prof_hook_end(int h, double bytes, hipStream_t st);
@@ -154,9 +155,28 @@ void launch_spawn(hipStream_t st, const uint64_t *in, size_t n, const unsigned l
  * passes.  A key that breaks the bound ORs SORT_ERR_CODE into *d_err. */
 constexpr int SORT_CONTEXTS = 2;
 constexpr unsigned SORT_ERR_CODE = 4u;
+/* a bucket of the sample sort held more keys than one block sorts in LDS:
+ * the output is incomplete and the sort must be repeated on the LSD path */
+constexpr unsigned SORT_ERR_OVERSIZE = 8u;
+/* part (capacity part_cap keys, may be NULL) holds the buckets of the
+ * sample sort, which sorts of up to about 10^7 keys take when part has
+ * sort_part_keys(n) keys (sorts of at most 16384 keys need none) and
+ * allow_sample is set; it only reads keys, and a bucket above its capacity
+ * ORs SORT_ERR_OVERSIZE into *d_err: the caller then repeats the sort with
+ * allow_sample = false (the LSD path, which takes any size). */
 uint64_t *sort_unique(hipStream_t st, uint64_t *keys, uint64_t *tmp, size_t n, uint64_t vary,
                       void *scratch, uint64_t *dst, unsigned long long *d_nuniq, unsigned *d_err,
-                      int *prof_unique, int ctx = 0, uint64_t cell_max = CELLS_UNBOUNDED);
+                      int *prof_unique, int ctx = 0, uint64_t cell_max = CELLS_UNBOUNDED,
+                      uint64_t *part = nullptr, size_t part_cap = 0, bool allow_sample = true);
+/* keys of the part buffer a sort of n keys needs (0: none) */
+size_t sort_part_keys(size_t n);
+/* sample sort: on / off, mean keys per bucket, bucket capacity (lowered only
+ * to exercise the repeat; 0 restores 16384) */
+void sort_set_sample(int on, int bucket_avg, int cap);
+int sort_sample_on();
+int sort_sample_bucket();
+int sort_sample_cap();
+unsigned long long sort_sample_count();
 
 /* Sort configurations: digit width (8, 9 or 10 bits; the pass block has one
  * thread per digit), keys per thread (tile = 2^bits * kpt keys) and look-back

@@ -499,6 +499,362 @@ __global__ __launch_bounds__(UQ_THREADS) void k_os_unique(const uint64_t *__rest
 }
 
 /* ------------------------------------------------------------------------ */
+/* sample sort: one partition pass into LDS-sized buckets, then each bucket  */
+/* sorted and deduplicated in LDS                                            */
+/* ------------------------------------------------------------------------ */
+/* For mid-size sorts (the move candidates and spawn-4 sets of a layer step,
+ * 10^4 to 10^7 keys) the LSD passes are bound by tile latency rather than
+ * bandwidth, and every duplicate crosses HBM once per pass.  Here:
+ *   k_ss_sample  one block sorts a regular sample of SS_S keys and writes
+ *                B - 1 splitters (every (S/B)-th sample);
+ *   k_ss_part    one read of the keys: each key goes to the bucket of the
+ *                splitters around it (all copies of a key to one bucket),
+ *                tiles rank their keys per bucket in LDS and reserve room
+ *                with one atomic per (tile, bucket) in a bucket-major
+ *                buffer of SS_LCAP keys per bucket;
+ *   k_ss_local   one block per bucket (in ticket order): the bucket's keys
+ *                are sorted in LDS by stable 8-bit LSD passes over the bits
+ *                in which they differ, deduplicated, and written at their
+ *                place, which a chained scan of unique counts over the
+ *                buckets gives.
+ * So a key is read twice and written once, and only unique keys reach the
+ * output.  A bucket above SS_LCAP keys (a sample far from its data, or one
+ * key repeated tens of thousands of times) sets SORT_ERR_OVERSIZE: the
+ * caller then repeats the sort on the LSD path. */
+constexpr int SS_T = 1024;                  /* threads of every sample-sort block */
+constexpr int SS_NW = SS_T / 64;
+constexpr int SS_K = 16;                    /* keys per thread of a bucket */
+constexpr int SS_LCAP = SS_T * SS_K;        /* 16384 keys: one bucket in LDS */
+constexpr int SS_WCHUNK = 64 * SS_K;        /* keys of one wave */
+constexpr int SS_PK = 8;                    /* keys per thread of a partition tile */
+constexpr int SS_TILE = SS_T * SS_PK;       /* 8192 */
+constexpr int SS_BMAX = 2048;               /* buckets */
+constexpr int SS_S = SS_LCAP;               /* samples */
+
+struct SsLds {
+    uint64_t stage[SS_LCAP];                /* 128 KiB */
+    uint16_t wcnt[SS_NW * 256];             /* per-wave digit counts, then wave prefixes */
+    uint32_t dstart[256];
+    uint32_t wtmp[17];
+    uint64_t red[2 * SS_NW];
+    uint32_t wc[SS_NW];
+    uint64_t excl;
+    unsigned ticket;
+};
+
+/* minimum and maximum of the valid keys over the block */
+__device__ __forceinline__ void ss_min_max(const uint64_t (&key)[SS_K], int rem, SsLds &L, uint64_t *mn,
+                                           uint64_t *mx)
+{
+    const int w = threadIdx.x >> 6, lane = lane_id();
+    uint64_t a = ~0ULL, b = 0;
+#pragma unroll
+    for (int j = 0; j < SS_K; j++)
+        if (j * 64 + lane < rem) {
+            a = key[j] < a ? key[j] : a;
+            b = key[j] > b ? key[j] : b;
+        }
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        uint64_t x = __shfl_xor(a, d, 64), y = __shfl_xor(b, d, 64);
+        a = x < a ? x : a;
+        b = y > b ? y : b;
+    }
+    if (lane == 0) L.red[w] = a, L.red[SS_NW + w] = b;
+    __syncthreads();
+    a = ~0ULL;
+    b = 0;
+#pragma unroll
+    for (int ww = 0; ww < SS_NW; ww++) {
+        a = L.red[ww] < a ? L.red[ww] : a;
+        b = L.red[SS_NW + ww] > b ? L.red[SS_NW + ww] : b;
+    }
+    *mn = a;
+    *mx = b;
+    __syncthreads();   /* red may be reused */
+}
+
+/* Sorts the block's m values (key[j] of thread (w, lane) is value number
+ * w * SS_WCHUNK + j * 64 + lane; the first m are valid) ascending over their
+ * low nb bits, which must hold every valid value (the callers sort keys
+ * relative to the block's smallest).  Stable 8-bit LSD passes: each wave
+ * ranks its values with one ballot per digit bit, the block turns the
+ * per-wave counts into positions and scatters into L.stage.  On return
+ * L.stage[0..m) holds the sorted values and key[] the values at the
+ * thread's positions of that order. */
+__device__ __forceinline__ void ss_block_sort(uint64_t (&key)[SS_K], int m, int nb, SsLds &L)
+{
+    const int w = threadIdx.x >> 6, lane = lane_id();
+    const uint64_t lt = lanemask_lt();
+    const int rem = m - w * SS_WCHUNK < 0 ? 0 : (m - w * SS_WCHUNK > SS_WCHUNK ? SS_WCHUNK : m - w * SS_WCHUNK);
+    if (nb <= 0) {   /* every valid key is equal: any order is sorted */
+#pragma unroll
+        for (int j = 0; j < SS_K; j++)
+            if (j * 64 + lane < rem) L.stage[w * SS_WCHUNK + j * 64 + lane] = key[j];
+        __syncthreads();
+        return;
+    }
+    for (int sh = 0; sh < nb; sh += 8) {
+        for (int i = threadIdx.x; i < SS_NW * 256; i += SS_T) L.wcnt[i] = 0;
+        __syncthreads();
+        uint32_t rank2[SS_K / 2];
+#pragma unroll
+        for (int j = 0; j < SS_K; j++) {
+            const bool valid = j * 64 + lane < rem;
+            const unsigned d = (unsigned)(key[j] >> sh) & 255u;
+            uint64_t mism = 0;
+#pragma unroll
+            for (int bit = 0; bit < 8; bit++) {
+                uint32_t t = 0u - ((d >> bit) & 1u);
+                uint64_t bb = __ballot(t != 0);
+                mism |= bb ^ (((uint64_t)t << 32) | t);
+            }
+            const uint64_t peers = __ballot(valid) & ~mism;
+            const uint32_t before = (uint32_t)__popcll(peers & lt);
+            const uint32_t cnt = L.wcnt[w * 256 + d];
+            __builtin_amdgcn_wave_barrier();
+            if (valid && before == 0) L.wcnt[w * 256 + d] = (uint16_t)(cnt + (uint32_t)__popcll(peers));
+            __builtin_amdgcn_wave_barrier();
+            const uint32_t r = cnt + before;
+            if (j & 1) rank2[j >> 1] |= r << 16;
+            else rank2[j >> 1] = r;
+            __builtin_amdgcn_sched_barrier(0);   /* one key at a time: registers */
+        }
+        __syncthreads();
+        uint32_t tot = 0;
+        if (threadIdx.x < 256) {
+#pragma unroll
+            for (int ww = 0; ww < SS_NW; ww++) {
+                const uint32_t c = L.wcnt[ww * 256 + threadIdx.x];
+                L.wcnt[ww * 256 + threadIdx.x] = (uint16_t)tot;
+                tot += c;
+            }
+        }
+        uint32_t all;
+        const uint32_t ex = block_excl_scan<uint32_t>(tot, &all, L.wtmp);   /* threads >= 256 add 0 */
+        if (threadIdx.x < 256) L.dstart[threadIdx.x] = ex;
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < SS_K; j++) {
+            if (j * 64 + lane < rem) {
+                const unsigned d = (unsigned)(key[j] >> sh) & 255u;
+                const uint32_t r = (rank2[j >> 1] >> ((j & 1) * 16)) & 0xffffu;
+                L.stage[L.dstart[d] + L.wcnt[w * 256 + d] + r] = key[j];
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < SS_K; j++)
+            if (j * 64 + lane < rem) key[j] = L.stage[w * SS_WCHUNK + j * 64 + lane];
+    }
+    __syncthreads();
+}
+
+/* B - 1 splitters from a regular sample of SS_S keys (n > SS_S) */
+__global__ __launch_bounds__(SS_T) void k_ss_sample(const uint64_t *__restrict__ keys, size_t n, int B,
+                                                    uint64_t *__restrict__ spl)
+{
+    __shared__ SsLds L;
+    const int w = threadIdx.x >> 6, lane = lane_id();
+    uint64_t key[SS_K];
+#pragma unroll
+    for (int j = 0; j < SS_K; j++) {
+        const size_t s = (size_t)w * SS_WCHUNK + j * 64 + lane;
+        key[j] = keys[((2 * s + 1) * n) / (2 * (size_t)SS_S)];
+    }
+    uint64_t mn, mx;
+    ss_min_max(key, SS_WCHUNK, L, &mn, &mx);
+#pragma unroll
+    for (int j = 0; j < SS_K; j++) key[j] -= mn;
+    ss_block_sort(key, SS_S, mx > mn ? 64 - __clzll((long long)(mx - mn)) : 0, L);
+    for (int i = threadIdx.x; i < B - 1; i += SS_T) spl[i] = mn + L.stage[((size_t)(i + 1) * SS_S) / B];
+}
+
+struct SsPartLds {
+    uint64_t spl[SS_BMAX];
+    uint64_t wbase[SS_BMAX];
+    uint64_t stage[SS_TILE];
+    uint32_t cnt[SS_BMAX];        /* counts, then the tile's start of each bucket */
+    uint16_t bid[SS_TILE];
+    uint32_t wtmp[17];
+};
+
+/* Bucket of every key (the number of splitters <= key), ranked per bucket in
+ * LDS; one reservation per (tile, bucket) in gcur; the tile staged in bucket
+ * order and written to part[b * SS_LCAP + position].  Room beyond SS_LCAP in
+ * a bucket is not written (the bucket is then oversize). */
+__global__ __launch_bounds__(SS_T) void k_ss_part(const uint64_t *__restrict__ keys, size_t n,
+                                                  const uint64_t *__restrict__ spl_g, int B,
+                                                  uint64_t *__restrict__ part,
+                                                  unsigned long long *gcur, unsigned *err, int cap)
+{
+    __shared__ SsPartLds P;
+    const int nspl = B - 1;
+    for (int i = threadIdx.x; i < nspl; i += SS_T) P.spl[i] = spl_g[i];
+    for (int i = threadIdx.x; i < B; i += SS_T) P.cnt[i] = 0;
+    __syncthreads();
+    const size_t tile0 = (size_t)blockIdx.x * SS_TILE;
+    uint64_t k[SS_PK];
+    uint32_t br[SS_PK];   /* bucket << 16 | rank in the tile's bucket */
+#pragma unroll
+    for (int j = 0; j < SS_PK; j++) {
+        const size_t i = tile0 + (size_t)j * SS_T + threadIdx.x;
+        k[j] = i < n ? keys[i] : 0;
+    }
+#pragma unroll
+    for (int j = 0; j < SS_PK; j++) {
+        const size_t i = tile0 + (size_t)j * SS_T + threadIdx.x;
+        if (i < n) {
+            int lo = 0, hi = nspl;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (P.spl[mid] <= k[j]) lo = mid + 1;
+                else hi = mid;
+            }
+            br[j] = ((uint32_t)lo << 16) | atomicAdd(&P.cnt[lo], 1u);
+        }
+    }
+    __syncthreads();
+    /* two buckets per thread: the tile's start of each, and the reservation */
+    const int b0 = 2 * threadIdx.x, b1 = b0 + 1;
+    const uint32_t c0 = b0 < B ? P.cnt[b0] : 0, c1 = b1 < B ? P.cnt[b1] : 0;
+    uint32_t all;
+    const uint32_t ex = block_excl_scan<uint32_t>(c0 + c1, &all, P.wtmp);
+    bool over = false;
+    if (c0) {
+        const unsigned long long g = atomicAdd(&gcur[b0], (unsigned long long)c0);
+        over |= g + c0 > (unsigned long long)cap;
+        P.wbase[b0] = (uint64_t)b0 * SS_LCAP + g - ex;
+    }
+    if (c1) {
+        const unsigned long long g = atomicAdd(&gcur[b1], (unsigned long long)c1);
+        over |= g + c1 > (unsigned long long)cap;
+        P.wbase[b1] = (uint64_t)b1 * SS_LCAP + g - (ex + c0);
+    }
+    if (over) atomicOr(err, SORT_ERR_OVERSIZE);
+    if (b0 < B) P.cnt[b0] = ex;
+    if (b1 < B) P.cnt[b1] = ex + c0;
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < SS_PK; j++) {
+        const size_t i = tile0 + (size_t)j * SS_T + threadIdx.x;
+        if (i < n) {
+            const uint32_t b = br[j] >> 16, p = P.cnt[b] + (br[j] & 0xffffu);
+            P.stage[p] = k[j];
+            P.bid[p] = (uint16_t)b;
+        }
+    }
+    __syncthreads();
+    const uint32_t tn = (uint32_t)((n - tile0) < (size_t)SS_TILE ? (n - tile0) : (size_t)SS_TILE);
+    for (uint32_t p = threadIdx.x; p < tn; p += SS_T) {
+        const uint32_t b = P.bid[p];
+        const uint64_t o = P.wbase[b] + p;
+        if (o < (uint64_t)(b + 1) * SS_LCAP) part[o] = P.stage[p];
+    }
+}
+
+/* One bucket per block, buckets taken in ticket order: bucket t holds
+ * counts[t] keys at src + t * stride (B == 1: the n1 keys at src).  Sorted
+ * and deduplicated in LDS; the unique keys go to dst at the bucket's place
+ * (a chained scan of the buckets' unique counts; the last bucket writes the
+ * total). */
+__global__ __launch_bounds__(SS_T) void k_ss_local(const uint64_t *__restrict__ src, size_t stride,
+                                                   const unsigned long long *counts, size_t n1, int B,
+                                                   uint64_t *__restrict__ dst, uint64_t *status,
+                                                   unsigned *ticket, uint32_t epoch,
+                                                   unsigned long long *total, unsigned *err, int cap)
+{
+    __shared__ SsLds L;
+    const int w = threadIdx.x >> 6, lane = lane_id();
+    const uint64_t lt = lanemask_lt();
+    if (threadIdx.x == 0) L.ticket = B > 1 ? atomicAdd(ticket, 1u) : 0u;
+    __syncthreads();
+    const size_t t = L.ticket;
+    size_t m = B > 1 ? (size_t)counts[t] : n1;
+    if (m > (size_t)cap) {   /* oversize: published as empty, the caller repeats the sort */
+        if (threadIdx.x == 0) atomicOr(err, SORT_ERR_OVERSIZE);
+        m = 0;
+    }
+    const uint64_t *s = src + t * stride;
+    const int rem = (int)m - w * SS_WCHUNK < 0 ? 0
+                    : ((int)m - w * SS_WCHUNK > SS_WCHUNK ? SS_WCHUNK : (int)m - w * SS_WCHUNK);
+    uint64_t key[SS_K];
+#pragma unroll
+    for (int j = 0; j < SS_K; j++) key[j] = j * 64 + lane < rem ? s[w * SS_WCHUNK + j * 64 + lane] : 0;
+    uint64_t mn, mx;
+    ss_min_max(key, rem, L, &mn, &mx);
+#pragma unroll
+    for (int j = 0; j < SS_K; j++) key[j] -= mn;
+    ss_block_sort(key, (int)m, (m > 1 && mx > mn) ? 64 - __clzll((long long)(mx - mn)) : 0, L);
+    /* unique flags (bit j: key j is the first of its value in the sorted
+     * order) */
+    uint32_t fbits = 0, c = 0;
+#pragma unroll
+    for (int j = 0; j < SS_K; j++) {
+        const int i = w * SS_WCHUNK + j * 64 + lane;
+        const bool f = j * 64 + lane < rem && (i == 0 || L.stage[i - 1] != key[j]);
+        fbits |= (uint32_t)f << j;
+        c += (uint32_t)__popcll(__ballot(f));
+    }
+    if (lane == 0) L.wc[w] = c;
+    __syncthreads();
+    if (w == 0) {
+        uint32_t tot = 0;
+#pragma unroll
+        for (int ww = 0; ww < SS_NW; ww++) tot += L.wc[ww];
+        uint64_t excl = 0;
+        if (B > 1) {
+            uint64_t *my = status + t;
+            if (t == 0) {
+                if (lane == 0) st_flag(my, OS_INC, epoch, tot);
+            } else {
+                if (lane == 0) st_flag(my, OS_AGG, epoch, tot);
+                long tt = (long)t - 1;
+                while (tt >= 0) {
+                    const long ti = tt - lane;
+                    const uint64_t v = ti >= 0 ? ld_flag(status + ti) : 0;
+                    const uint32_t f = ti >= 0 ? fstate(v, epoch) : 2u;
+                    const uint64_t notready = __ballot(f == 0), inc = __ballot(f == 2);
+                    const int nr = notready ? __ffsll((long long)notready) - 1 : 64;
+                    const int ni = inc ? __ffsll((long long)inc) - 1 : 64;
+                    const int stop = nr < ni ? nr : ni;
+                    uint64_t part = (ti >= 0 && (lane < stop || (lane == ni && ni < nr))) ? (v & OS_VAL) : 0;
+#pragma unroll
+                    for (int dd = 32; dd >= 1; dd >>= 1) part += __shfl_xor(part, dd, 64);
+                    excl += part;
+                    if (ni < nr) break;
+                    if (nr < 64) {
+                        uint64_t wv = 0;
+                        if (lane == 0) wv = wait_flag(status + (tt - nr), epoch, err);
+                        wv = __shfl(wv, 0, 64);
+                        excl += wv & OS_VAL;
+                        if (fstate(wv, epoch) == 2) break;
+                        tt = tt - nr - 1;
+                    } else {
+                        tt -= 64;
+                    }
+                }
+                if (lane == 0) st_flag(my, OS_INC, epoch, excl + tot);
+            }
+        }
+        if (lane == 0) {
+            L.excl = excl;
+            if (t + 1 == (size_t)B) *total = excl + tot;
+        }
+    }
+    __syncthreads();
+    uint64_t o = L.excl;
+    for (int ww = 0; ww < w; ww++) o += L.wc[ww];
+#pragma unroll
+    for (int j = 0; j < SS_K; j++) {
+        const uint64_t fm = __ballot((fbits >> j) & 1u);
+        if ((fbits >> j) & 1u) dst[o + __popcll(fm & lt)] = key[j] + mn;
+        o += __popcll(fm);
+    }
+}
+
+/* ------------------------------------------------------------------------ */
 /* host side                                                                 */
 /* ------------------------------------------------------------------------ */
 
@@ -738,11 +1094,18 @@ struct OsScratch {
     uint64_t *cval;       /* the keys' constant bits */
     unsigned long long *ghist;   /* OS_MAXPASS x OS_MAXR */
     uint64_t *gbase;      /* OS_MAXPASS x OS_MAXR */
+    /* sample sort: bucket ticket, then the bucket counts (cleared together),
+     * then the splitters */
+    unsigned *ss_ticket;
+    unsigned long long *ss_cnt;
+    uint64_t *ss_spl;
 };
 
 constexpr size_t OS_CLEAR_BYTES = 256 + (size_t)OS_MAXPASS * OS_MAXR * 8;
+constexpr size_t SS_OFF = OS_CLEAR_BYTES + (size_t)OS_MAXPASS * OS_MAXR * 8;
+constexpr size_t SS_CLEAR_BYTES = 256 + (size_t)SS_BMAX * 8;
 
-size_t sort_scratch_bytes(size_t) { return OS_CLEAR_BYTES + (size_t)OS_MAXPASS * OS_MAXR * 8 + 256; }
+size_t sort_scratch_bytes(size_t) { return SS_OFF + SS_CLEAR_BYTES + (size_t)SS_BMAX * 8 + 256; }
 
 static OsScratch carve(void *scratch)
 {
@@ -752,7 +1115,45 @@ static OsScratch carve(void *scratch)
     s.cval = (uint64_t *)(p + 128);
     s.ghist = (unsigned long long *)(p + 256);
     s.gbase = (uint64_t *)(p + OS_CLEAR_BYTES);
+    s.ss_ticket = (unsigned *)(p + SS_OFF);
+    s.ss_cnt = (unsigned long long *)(p + SS_OFF + 256);
+    s.ss_spl = (uint64_t *)(p + SS_OFF + SS_CLEAR_BYTES);
     return s;
+}
+
+/* ---- sample sort: configuration ---- */
+
+static int g_ss_on = 1;              /* sample sort for sizes it handles */
+static int g_ss_bavg = 6000;         /* mean keys per bucket */
+static int g_ss_cap = SS_LCAP;       /* bucket capacity (lowered only to test the repeat) */
+static unsigned long long g_ss_sorts = 0;
+
+void sort_set_sample(int on, int bucket_avg, int cap)
+{
+    g_ss_on = on ? 1 : 0;
+    if (bucket_avg > 0) g_ss_bavg = bucket_avg;
+    g_ss_cap = (cap > 0 && cap <= SS_LCAP) ? cap : SS_LCAP;
+}
+int sort_sample_on() { return g_ss_on; }
+int sort_sample_bucket() { return g_ss_bavg; }
+int sort_sample_cap() { return g_ss_cap; }
+unsigned long long sort_sample_count() { return g_ss_sorts; }
+
+/* buckets of a sample sort of n keys (1: one block), or 0 when the sort
+ * takes the LSD path */
+static size_t ss_buckets(size_t n)
+{
+    if (!g_ss_on || n < 2) return 0;
+    if (n <= (size_t)SS_LCAP) return 1;
+    size_t B = (n + (size_t)g_ss_bavg - 1) / (size_t)g_ss_bavg;
+    if (B < 2) B = 2;
+    return B <= (size_t)SS_BMAX ? B : 0;
+}
+
+size_t sort_part_keys(size_t n)
+{
+    size_t B = ss_buckets(n);
+    return B > 1 ? B * (size_t)SS_LCAP : 0;
 }
 
 static void launch_hist_scan(hipStream_t st, int bits, const uint64_t *keys, size_t n, const SortCode &c,
@@ -838,9 +1239,39 @@ static void launch_pass(hipStream_t st, SortCfg c, unsigned tiles, const uint64_
     else launch_pass_t<false>(st, c, tiles, src, dst, n, x, shift, gbase, status, ctr, epoch, err);
 }
 
+/* The sample sort of n keys into dst (B buckets, part holding B * SS_LCAP
+ * keys when B > 1); keys are only read. */
+static void ss_sort_unique(hipStream_t st, const uint64_t *keys, size_t n, size_t B, uint64_t *part,
+                           const OsScratch &s, SortCtx &sc, uint64_t *dst, unsigned long long *d_nuniq,
+                           unsigned *d_err, int *prof_unique)
+{
+    g_ss_sorts++;
+    const uint32_t epoch = next_epoch(st, sc);
+    if (B > 1) {
+        launch_fill_u64(st, (uint64_t *)s.ss_ticket, SS_CLEAR_BYTES / 8, 0);
+        int h = prof_hook_begin(8, st);
+        hipLaunchKernelGGL(k_ss_sample, dim3(1), dim3(SS_T), 0, st, keys, n, (int)B, s.ss_spl);
+        snote(hipGetLastError());
+        prof_hook_end(h, 8.0 * SS_S + 8.0 * (B - 1), st);
+        h = prof_hook_begin(9, st);
+        const size_t tiles = (n + SS_TILE - 1) / SS_TILE;
+        hipLaunchKernelGGL(k_ss_part, dim3((unsigned)tiles), dim3(SS_T), 0, st, keys, n,
+                           (const uint64_t *)s.ss_spl, (int)B, part, s.ss_cnt, d_err, g_ss_cap);
+        snote(hipGetLastError());
+        prof_hook_end(h, 16.0 * n, st);
+    }
+    *prof_unique = prof_hook_begin(10, st);
+    hipLaunchKernelGGL(k_ss_local, dim3((unsigned)B), dim3(SS_T), 0, st, B > 1 ? part : keys,
+                       (size_t)SS_LCAP, (const unsigned long long *)s.ss_cnt, n, (int)B, dst, sc.status,
+                       s.ss_ticket, epoch, d_nuniq, d_err, g_ss_cap);
+    snote(hipGetLastError());
+    prof_hook_end(*prof_unique, 8.0 * n, st);
+}
+
 uint64_t *sort_unique(hipStream_t st, uint64_t *keys, uint64_t *tmp, size_t n, uint64_t vary,
                       void *scratch, uint64_t *dst, unsigned long long *d_nuniq, unsigned *d_err,
-                      int *prof_unique, int ctx, uint64_t cell_max)
+                      int *prof_unique, int ctx, uint64_t cell_max, uint64_t *part, size_t part_cap,
+                      bool allow_sample)
 {
     *prof_unique = -1;
     SortCtx &sc = g_ctx[ctx < 0 || ctx >= SORT_CONTEXTS ? 0 : ctx];
@@ -849,6 +1280,12 @@ uint64_t *sort_unique(hipStream_t st, uint64_t *keys, uint64_t *tmp, size_t n, u
         return keys;
     }
     OsScratch s = carve(scratch);
+    const size_t B = allow_sample ? ss_buckets(n) : 0;
+    if (B == 1 || (B > 1 && part && part_cap >= B * (size_t)SS_LCAP)) {
+        if (!status_reserve(st, sc, B)) return keys;
+        ss_sort_unique(st, keys, n, B, part, s, sc, dst, d_nuniq, d_err, prof_unique);
+        return keys;
+    }
     SortCode x = make_bit_code(n > 1 ? vary : 0);
     SortCfg c = sort_choose(n, x.nbits);
     int npass = (x.nbits + c.bits - 1) / c.bits;

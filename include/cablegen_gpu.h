@@ -181,13 +181,23 @@ int cg_dev_sync(void);
  *   "cell_code"  1: a generation's sorts code pairs of varying cells as
  *                a * base + b when the layer's tile sum bounds the cell
  *                values and that saves a radix pass; 0: bit packing only;
+ *   "sample_sort"  1: sorts of up to about 10^7 keys partition into
+ *                LDS-sized buckets by splitters from a sample and sort and
+ *                deduplicate each bucket in LDS (a sort of at most 16384 keys
+ *                is one block); 0: every sort takes the LSD radix path;
+ *   "sample_bucket"  mean keys per bucket of the sample sort (1000..16384);
+ *   "sample_cap" bucket capacity of the sample sort (0: 16384); a bucket
+ *                above it makes the library repeat that sort on the LSD path
+ *                (lowered only to exercise the repeat);
  *   "builtin"    built-in defaults: 8-bit digits, 8192-key tiles, window 16
  *                for every entry, no forced field, delta 0, spawn_estimate 0,
- *                cell_code 1, carry_stream 1;
+ *                cell_code 1, carry_stream 1, sample_sort 1 with 6000 keys
+ *                per bucket and capacity 16384;
  *   "reset"      the state after initialisation: the built-in defaults, then
  *                the table loaded at initialisation if there was one.
  * Returns 0, or -1 for an unknown knob or value.  cg_tune_get also reads
- * "cell_code_sorts": the number of sorts that used the cell-pair code. */
+ * "cell_code_sorts": the number of sorts that used the cell-pair code, and
+ * "sample_sorts": the number of sample sorts. */
 int cg_tune_set(const char *knob, int value);
 int cg_tune_get(const char *knob);
 /* the sorts read their configuration from a table indexed by size class

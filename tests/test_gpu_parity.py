@@ -221,6 +221,7 @@ def test_sort_every_configuration(cg, cfg):
     tile = (1024 if kpt <= 8 else 1 << bits) * kpt
     rng = np.random.default_rng(bits * 1000 + kpt * 10 + lbw)
     try:
+        cg.tune_set("sample_sort", 0)   # the LSD kernels themselves
         cg.tune_set("sort_bits", bits)
         cg.tune_set("sort_kpt", kpt)
         cg.tune_set("sort_lbw", lbw)
@@ -278,6 +279,7 @@ def test_sort_cell_code_every_configuration(cg, cfg):
     cells, vmax = CODE_CASES[bits]
     rng = np.random.default_rng(7 + bits * 1000 + kpt * 10 + lbw)
     try:
+        cg.tune_set("sample_sort", 0)
         cg.tune_set("sort_bits", bits)
         cg.tune_set("sort_kpt", kpt)
         cg.tune_set("sort_lbw", lbw)
@@ -298,14 +300,18 @@ def test_sort_cell_code_tuned_sizes(cg):
     cells) across the tuned table's size classes; bit packing and the code
     give the same unique keys."""
     rng = np.random.default_rng(11)
-    for n in (2, 1000, 70001, 3000017, (1 << 23) + 5):
-        a = cell_keys(rng, n, DPD_CELLS, 10)
-        before = cg.tune_get("cell_code_sorts")
-        _bounded_check(cg, a, bounded(10))
-        used = cg.tune_get("cell_code_sorts") - before
-        assert used == 1 or n == 2
-        _sort_check(cg, a)
-        assert cg.tune_get("cell_code_sorts") == before + used
+    try:
+        cg.tune_set("sample_sort", 0)
+        for n in (2, 1000, 70001, 3000017, (1 << 23) + 5):
+            a = cell_keys(rng, n, DPD_CELLS, 10)
+            before = cg.tune_get("cell_code_sorts")
+            _bounded_check(cg, a, bounded(10))
+            used = cg.tune_get("cell_code_sorts") - before
+            assert used == 1 or n == 2
+            _sort_check(cg, a)
+            assert cg.tune_get("cell_code_sorts") == before + used
+    finally:
+        cg.tune_set("reset", 0)
 
 
 def test_sort_cell_code_bound_violation(cg):
@@ -315,22 +321,30 @@ def test_sort_cell_code_bound_violation(cg):
     a = cell_keys(rng, 100000, DPD_CELLS, 10)
     a[777] = (a[777] & ~np.uint64(0xF << 8)) | np.uint64(12 << 8)
     d, out, n = dev(a), torch.empty(len(a), dtype=torch.int64, device="cuda"), C.c_size_t()
-    assert cg.lib().cg_dev_sort_unique_bounded(d.data_ptr(), len(a), out.data_ptr(), C.byref(n),
-                                               bounded(10)) != 0
-    assert "bound" in cg.last_error()
-    _bounded_check(cg, cell_keys(rng, 5000, DPD_CELLS, 10), bounded(10))
+    try:
+        cg.tune_set("sample_sort", 0)   # the code is an LSD-path encoding
+        assert cg.lib().cg_dev_sort_unique_bounded(d.data_ptr(), len(a), out.data_ptr(), C.byref(n),
+                                                   bounded(10)) != 0
+        assert "bound" in cg.last_error()
+        _bounded_check(cg, cell_keys(rng, 5000, DPD_CELLS, 10), bounded(10))
+    finally:
+        cg.tune_set("reset", 0)
 
 
 def test_sort_cell_code_per_cell_bases(cg):
     """Composite keys (a table id above ten free cells, as the batch builds
     them) with a bound per cell, against numpy's unique."""
     rng = np.random.default_rng(13)
-    for n in (5000, 300001):
-        tid = rng.integers(0, 40, n).astype(np.uint64)
-        a = cell_keys(rng, n, list(range(6, 16)), 8, const=0) | (tid << np.uint64(48))
-        before = cg.tune_get("cell_code_sorts")
-        _bounded_check(cg, a, (2 << 52) | (15 << 48) | (bounded(8) >> 16))
-        assert cg.tune_get("cell_code_sorts") == before + 1
+    try:
+        cg.tune_set("sample_sort", 0)
+        for n in (5000, 300001):
+            tid = rng.integers(0, 40, n).astype(np.uint64)
+            a = cell_keys(rng, n, list(range(6, 16)), 8, const=0) | (tid << np.uint64(48))
+            before = cg.tune_get("cell_code_sorts")
+            _bounded_check(cg, a, (2 << 52) | (15 << 48) | (bounded(8) >> 16))
+            assert cg.tune_get("cell_code_sorts") == before + 1
+    finally:
+        cg.tune_set("reset", 0)
 
 
 @pytest.mark.parametrize("name", ["c1", "dpdgap_prune", "two_initial"])
@@ -339,6 +353,7 @@ def test_tablebase_cell_code_on_and_off(cg, name):
     each layer's tile sum and 0xf cells) and with bit packing only: both
     match the oracle, and the default run used the code."""
     try:
+        cg.tune_set("sample_sort", 0)   # every layer sort on the LSD path
         before = cg.tune_get("cell_code_sorts")
         test_tablebase_vs_oracle(cg, name)
         assert cg.tune_get("cell_code_sorts") > before
@@ -683,10 +698,10 @@ def test_batch_matches_single_tables(cg):
     B = cg.Batch(r)
     for c in cfgs:
         B.add(c["initial"], c["start"], c["end"], c["winstates"])
-    before = cg.tune_get("cell_code_sorts")
+    before = cg.tune_get("sample_sorts")
     B.generate()
-    # the composite keys' table id and free cells are bounded: the sorts code them
-    assert cg.tune_get("cell_code_sorts") > before
+    # the batch's layer sorts take the sample sort (the default path)
+    assert cg.tune_get("sample_sorts") > before
     B.solve(use_graph=False)
     eager = [[B.values(t, i) for i in range(B.num_layers(t))] for t in range(len(cfgs))]
     for t, c in enumerate(cfgs):
@@ -745,4 +760,98 @@ def test_batch_rejects_bad_tables(cg):
     B = cg.Batch(cg.rules(ignore_f=True))
     with pytest.raises(cg.CablegenError):
         B.add([0x1200000000000000], 4, 20, [0x5000000000000000])   # 16 free cells
+    B.close()
+
+
+# ---------------------------------------------------------------- sample sort
+
+def _sample_sort_check(cg, a, expect_sample=True):
+    before = cg.tune_get("sample_sorts")
+    _sort_check(cg, a)
+    if expect_sample:
+        assert cg.tune_get("sample_sorts") > before
+
+
+@pytest.mark.parametrize("n", [2, 3, 1000, 16383, 16384, 16385, 70001, 1000003, 5000011, 11500007])
+def test_sample_sort_sizes(cg, n):
+    """The sample sort (splitters from a sample, one partition pass into
+    LDS-sized buckets, each bucket sorted and deduplicated in LDS) against
+    numpy's unique: one-block sizes around 16384 keys and multi-bucket sizes
+    up to ~1.2e7, on DPDGap boards with the duplicate rate of a move-candidate
+    set and on full 64-bit keys."""
+    rng = np.random.default_rng(n)
+    b = rand_boards(rng, 0x00000000FFF0FFF0, max(n // 3, 1))
+    a = b[rng.integers(0, len(b), size=n)]
+    _sample_sort_check(cg, a)
+    if n <= 1000003:
+        _sample_sort_check(cg, rng.integers(0, 2**64 - 1, size=n, dtype=np.uint64, endpoint=True))
+
+
+def test_sample_sort_key_shapes(cg):
+    """All-equal keys, two values, keys spanning the whole 64-bit range with
+    the extremes, keys differing only in bit 63 and only in bit 0, and a
+    bucket whose keys span more than 2^32 (relative keys are 64-bit)."""
+    rng = np.random.default_rng(5)
+    n = 200001
+    _sample_sort_check(cg, np.full(n, 0x1234, np.uint64))
+    _sample_sort_check(cg, np.where(rng.random(n) < 0.5, 7, 2**64 - 1).astype(np.uint64))
+    x = rng.integers(0, 2**64 - 1, size=n, dtype=np.uint64, endpoint=True)
+    x[:3] = [0, 2**64 - 1, 2**63]
+    _sample_sort_check(cg, x)
+    _sample_sort_check(cg, (rng.integers(0, 2, n).astype(np.uint64) << np.uint64(63)) | np.uint64(5))
+    _sample_sort_check(cg, rng.integers(0, 2, n).astype(np.uint64) | np.uint64(0xABC0))
+    # sparse keys: a few clusters far apart
+    c = rng.integers(0, 4, n).astype(np.uint64) << np.uint64(50)
+    _sample_sort_check(cg, c | rng.integers(0, 1 << 40, n).astype(np.uint64))
+
+
+def test_sample_sort_oversize_repeats_on_lsd(cg):
+    """A bucket above the capacity (one key repeated 60000 times, or every
+    bucket once the capacity is lowered) makes the library repeat the sort on
+    the LSD path: the result is exact either way."""
+    rng = np.random.default_rng(6)
+    a = rand_boards(rng, 0x00000000FFF0FFF0, 300000)
+    a[rng.integers(0, len(a), 60000)] = a[17]
+    _sort_check(cg, a)
+    try:
+        cg.tune_set("sample_cap", 64)
+        for n in (1000, 70001, 2000003):
+            _sort_check(cg, rand_boards(rng, 0x00000000FFF0FFF0, n))
+    finally:
+        cg.tune_set("reset", 0)
+
+
+@pytest.mark.parametrize("name", ["c1", "dpdgap_prune"])
+def test_tablebase_sample_sort_repeat_path(cg, name):
+    """Generation with every sample sort overflowing (capacity 64): each
+    step's M sort and carry sort are repeated on the LSD path and the table
+    still matches the oracle."""
+    try:
+        cg.tune_set("sample_cap", 64)
+        test_tablebase_vs_oracle(cg, name)
+    finally:
+        cg.tune_set("reset", 0)
+
+
+def test_batch_sample_sort_repeat_path(cg):
+    from cablegen_amd.batch_configs import batch
+    cfgs = batch(6, lo=3, hi=10)
+    r = cg.rules(ignore_f=True)
+    B = cg.Batch(r)
+    for c in cfgs:
+        B.add(c["initial"], c["start"], c["end"], c["winstates"])
+    try:
+        cg.tune_set("sample_cap", 64)
+        B.generate()
+        B.solve(use_graph=False)
+    finally:
+        cg.tune_set("reset", 0)
+    for t, c in enumerate(cfgs):
+        tb = cg.Tablebase(r)
+        tb.generate(c["initial"], c["start"], c["end"])
+        tb.solve(c["winstates"])
+        for i in range(tb.num_layers):
+            assert np.array_equal(B.keys(t, i), tb.keys(i)), (t, i)
+            assert np.array_equal(B.values(t, i).view(np.uint64), tb.values(i).view(np.uint64))
+        tb.close()
     B.close()
