@@ -48,11 +48,13 @@ WINSTATES = [0x0000900000000000]
 END_GEN = 1100
 RULES = dict(ignore_f=True, prune=True, stsl=58, ltc=5, smallest_large=6)
 HBM_PEAK_GBS = 8000.0
-KERNELS = ["move", "spawn", "sort_hist", "sort_pass", "unique", "solve"]
-# the sort stage's kernels; the last one of each sort (the compaction) carries
-# the stage's algorithmic bytes 8 B per key in + 8 B per unique key out
-SORT_KINDS = ["sort_hist", "sort_pass", "unique"]
-SORT_FINAL = ["unique"]
+KERNELS = ["move", "spawn", "sort_hist", "sort_pass", "unique", "sort_split", "sort_part",
+           "sort_local", "solve"]
+# the sort stage's kernels (LSD radix: histogram, passes, compaction; sample
+# sort: splitters, partition, bucket sort + dedupe); the last one of each sort
+# carries the stage's algorithmic bytes 8 B per key in + 8 B per unique key out
+SORT_KINDS = ["sort_hist", "sort_pass", "unique", "sort_split", "sort_part", "sort_local"]
+SORT_FINAL = ["unique", "sort_local"]
 # roofline candidates: the move, spawn and solve kernels, and the sort stage
 STAGES = ["move", "spawn", "sort", "solve"]
 
@@ -400,8 +402,12 @@ def run_autotune(cg, args):
         cg.tune_set("builtin", 0)
         cg.tune_set("cell_code", args.cell_code)
         cg.tune_set("carry_stream", args.carry_stream)
+        cg.tune_set("sample_sort", args.sample_sort)
+        if args.sample_bucket:
+            cg.tune_set("sample_bucket", args.sample_bucket)
         return {"source": "built-in defaults: 8-bit digits, 8192-key tiles, window 16, dir delta 0",
-                "cell_code": args.cell_code, "carry_stream": args.carry_stream}
+                "cell_code": args.cell_code, "carry_stream": args.carry_stream,
+                "sample_sort": args.sample_sort, "sample_bucket": cg.tune_get("sample_bucket")}
     if mode == "stored" and os.path.exists(cg.TUNED_TABLE):
         rep = cg.load_tuning()
         src = "stored autotuner table cablegen_amd/tuned/mi355x.json"
@@ -413,6 +419,9 @@ def run_autotune(cg, args):
             json.dump(rep, f)
     cg.tune_set("cell_code", args.cell_code)
     cg.tune_set("carry_stream", args.carry_stream)
+    cg.tune_set("sample_sort", args.sample_sort)
+    if args.sample_bucket:
+        cg.tune_set("sample_bucket", args.sample_bucket)
     dir_delta = rep["dir"]["pick"]
     if args.dir_delta is not None:
         dir_delta = args.dir_delta
@@ -422,7 +431,8 @@ def run_autotune(cg, args):
         sel[name] = {nb: rep["sort"][cls]["pick"][nb] for nb in ("24", "32", "36", "40", "44", "48", "54")}
     return {"source": src, "seconds": time.perf_counter() - t0, "dir_delta": dir_delta,
             "sort_picks_bits_x_kpt/window": sel, "cell_code": args.cell_code,
-            "carry_stream": args.carry_stream}
+            "carry_stream": args.carry_stream, "sample_sort": args.sample_sort,
+            "sample_bucket": cg.tune_get("sample_bucket")}
 
 
 def census(args):
@@ -694,6 +704,10 @@ def main():
                     help="0: the carry sort on the main stream instead of a second one")
     ap.add_argument("--dir-delta", type=int, choices=range(-4, 4), default=None,
                     help="layer directory size: 2^(floor(log2 n) + delta) buckets (overrides the tuned pick)")
+    ap.add_argument("--sample-sort", type=int, choices=(0, 1), default=1,
+                    help="1: mid-size sorts take the sample sort (default); 0: LSD radix only")
+    ap.add_argument("--sample-bucket", type=int, default=0,
+                    help="mean keys per bucket of the sample sort (0: the library's default)")
     ap.add_argument("--cell-code", type=int, choices=(0, 1), default=1,
                     help="1: layer sorts code pairs of cells when the tile sum bounds them (default)")
     args = ap.parse_args()
@@ -734,6 +748,7 @@ def main():
 
     for _ in range(args.warmup):
         step()
+    ss0, sr0 = cg.tune_get("sample_sorts"), cg.tune_get("sample_repeats")
     # one untimed step with every kernel timed gives the per-kernel breakdown
     # and names the dominant stage; the timed steps then time only that
     # stage's launches (two events per launch), for the roofline
@@ -782,6 +797,8 @@ def main():
                         "GB_s": (v["bytes"] / v["ms"] / 1e6) if v["ms"] else 0.0}
                     for k, v in kern.items()},
         "stages_ms_per_step": stage_ms,
+        "sorts": {"sample_sorts_per_step": (cg.tune_get("sample_sorts") - ss0) / (args.steps + 1),
+                  "lsd_repeats_per_step": (cg.tune_get("sample_repeats") - sr0) / (args.steps + 1)},
         "kernels_source": "one untimed step after the warm-up with every kernel timed "
                           "(the timed steps time only the roofline stage's kernels)",
     }

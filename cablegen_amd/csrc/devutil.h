@@ -50,6 +50,38 @@ __device__ __forceinline__ T block_excl_scan(T v, T *total, T *wtmp)
     return r;
 }
 
+/* Inclusive scan of a 32-bit value over the 64 lanes with DPP moves (no LDS
+ * round trip): shifts by 1, 2, 4 and 8 inside each row of 16 lanes, then the
+ * last lane of row 0 (2) added to row 1 (3) and lane 31 to rows 2 and 3.
+ * Lanes whose DPP source lies outside the row read 0. */
+__device__ __forceinline__ uint32_t wave_incl_scan_dpp(uint32_t x)
+{
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xf, 0xf, false);   /* row_shr:1 */
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xf, 0xf, false);   /* row_shr:2 */
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xf, 0xf, false);   /* row_shr:4 */
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xf, 0xf, false);   /* row_shr:8 */
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xa, 0xf, false);   /* row_bcast:15 */
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xc, 0xf, false);   /* row_bcast:31 */
+    return x;
+}
+
+/* Exclusive scan over a block of up to 16 waves with one barrier: each wave
+ * scans with DPP, lane 63 publishes the wave total to wtmp[w], and every
+ * wave scans the totals of the waves before it.  The caller separates two
+ * uses of wtmp by a barrier. */
+__device__ __forceinline__ uint32_t block_excl_scan_dpp(uint32_t v, uint32_t *total, uint32_t *wtmp)
+{
+    const int w = threadIdx.x >> 6, lane = lane_id(), nw = (blockDim.x + 63) >> 6;
+    const uint32_t inc = wave_incl_scan_dpp(v);
+    if (lane == 63) wtmp[w] = inc;
+    __syncthreads();
+    const uint32_t t = lane < nw ? wtmp[lane] : 0u;
+    const uint32_t ts = wave_incl_scan_dpp(t);
+    const uint32_t before = w ? (uint32_t)__shfl(ts, w - 1, 64) : 0u;
+    *total = (uint32_t)__shfl(ts, nw - 1, 64);
+    return before + inc - v;
+}
+
 /* OR and AND of per-thread values over the block, one atomic pair per block */
 __device__ __forceinline__ void block_or_and(uint64_t o, uint64_t a, unsigned long long *dst_or,
                                              unsigned long long *dst_and, uint64_t *ro, uint64_t *ra)

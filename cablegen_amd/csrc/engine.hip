@@ -458,6 +458,9 @@ static MoveRules move_rules(const cg_rules &r)
     return m;
 }
 
+/* sorts repeated on the LSD path after an oversize bucket */
+static unsigned long long g_oversize_repeats = 0;
+
 /* copies device counters [lo, lo+n) to E.h_cnt, synchronises and checks the
  * sort error flag */
 static int read_counters(int lo, int n)
@@ -473,6 +476,7 @@ static int read_counters(int lo, int n)
         hipStreamSynchronize(E.st);
         if (e == SORT_ERR_OVERSIZE) {   /* the caller repeats that sort */
             E.oversize = true;
+            g_oversize_repeats++;
             return 0;
         }
         if (e & 2u) return fail_msg("batch: a board's formation cells left the table's canonical place");
@@ -1339,7 +1343,7 @@ extern "C" int cg_tune_set(const char *knob, int value)
         g_spawn_est_mode = 0;
         sort_set_cell_code(1);
         g_carry_lane = 1;
-        sort_set_sample(1, 6000, 0);
+        sort_set_sample(1, 5000, 0);
         if (!strcmp(knob, "reset") && !g_tune_path.empty()) return cg_tune_load(g_tune_path.c_str());
         return 0;
     }
@@ -1361,6 +1365,7 @@ extern "C" int cg_tune_get(const char *knob)
     if (!strcmp(knob, "sample_bucket")) return sort_sample_bucket();
     if (!strcmp(knob, "sample_cap")) return sort_sample_cap();
     if (!strcmp(knob, "sample_sorts")) return (int)std::min<unsigned long long>(sort_sample_count(), 0x7fffffff);
+    if (!strcmp(knob, "sample_repeats")) return (int)std::min<unsigned long long>(g_oversize_repeats, 0x7fffffff);
     return fail_msg("unknown tuning knob");
 }
 

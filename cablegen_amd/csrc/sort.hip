@@ -505,8 +505,8 @@ __global__ __launch_bounds__(UQ_THREADS) void k_os_unique(const uint64_t *__rest
 /* For mid-size sorts (the move candidates and spawn-4 sets of a layer step,
  * 10^4 to 10^7 keys) the LSD passes are bound by tile latency rather than
  * bandwidth, and every duplicate crosses HBM once per pass.  Here:
- *   k_ss_sample  one block sorts a regular sample of SS_S keys and writes
- *                B - 1 splitters (every (S/B)-th sample);
+ *   k_ss_gather, k_ss_sample  a regular sample of SS_S keys, sorted by one
+ *                block, gives B - 1 splitters (every (S/B)-th sample);
  *   k_ss_part    one read of the keys: each key goes to the bucket of the
  *                splitters around it (all copies of a key to one bucket),
  *                tiles rank their keys per bucket in LDS and reserve room
@@ -523,18 +523,24 @@ __global__ __launch_bounds__(UQ_THREADS) void k_os_unique(const uint64_t *__rest
  * caller then repeats the sort on the LSD path. */
 constexpr int SS_T = 1024;                  /* threads of every sample-sort block */
 constexpr int SS_NW = SS_T / 64;
-constexpr int SS_K = 16;                    /* keys per thread of a bucket */
-constexpr int SS_LCAP = SS_T * SS_K;        /* 16384 keys: one bucket in LDS */
-constexpr int SS_WCHUNK = 64 * SS_K;        /* keys of one wave */
+constexpr int SS_LCAP = 16384;              /* keys of a bucket (32-bit relative values) */
+constexpr int SS_LCAP64 = 8192;             /* keys of a bucket that spans 2^32 or more */
+constexpr int SS_KS32 = 16;                 /* rows per thread of a bucket's sort: 16384 values */
+constexpr int SS_KS64 = 8;                  /* 64-bit values: 8192 */
 constexpr int SS_PK = 8;                    /* keys per thread of a partition tile */
 constexpr int SS_TILE = SS_T * SS_PK;       /* 8192 */
 constexpr int SS_BMAX = 2048;               /* buckets */
 constexpr int SS_S = SS_LCAP;               /* samples */
 
+/* 64 KiB of values plus counters: two blocks per CU */
 struct SsLds {
-    uint64_t stage[SS_LCAP];                /* 128 KiB */
-    uint16_t wcnt[SS_NW * 256];             /* per-wave digit counts, then wave prefixes */
-    uint32_t dstart[256];
+    union {
+        uint32_t v32[SS_LCAP];
+        uint64_t v64[SS_LCAP64];
+    } stage;
+    /* digit-major counters: [digit][wave], 8-byte aligned for the scan's
+     * 4-counter loads; counts, then each (digit, wave)'s first position */
+    uint16_t cnt[256 * SS_NW] __attribute__((aligned(8)));
     uint32_t wtmp[17];
     uint64_t red[2 * SS_NW];
     uint32_t wc[SS_NW];
@@ -542,18 +548,14 @@ struct SsLds {
     unsigned ticket;
 };
 
-/* minimum and maximum of the valid keys over the block */
-__device__ __forceinline__ void ss_min_max(const uint64_t (&key)[SS_K], int rem, SsLds &L, uint64_t *mn,
-                                           uint64_t *mx)
+template <typename V> __device__ __forceinline__ V *ss_stage(SsLds &L);
+template <> __device__ __forceinline__ uint32_t *ss_stage<uint32_t>(SsLds &L) { return L.stage.v32; }
+template <> __device__ __forceinline__ uint64_t *ss_stage<uint64_t>(SsLds &L) { return L.stage.v64; }
+
+/* minimum and maximum of this thread's keys folded over the block */
+__device__ __forceinline__ void ss_min_max(uint64_t a, uint64_t b, SsLds &L, uint64_t *mn, uint64_t *mx)
 {
     const int w = threadIdx.x >> 6, lane = lane_id();
-    uint64_t a = ~0ULL, b = 0;
-#pragma unroll
-    for (int j = 0; j < SS_K; j++)
-        if (j * 64 + lane < rem) {
-            a = key[j] < a ? key[j] : a;
-            b = key[j] > b ? key[j] : b;
-        }
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) {
         uint64_t x = __shfl_xor(a, d, 64), y = __shfl_xor(b, d, 64);
@@ -574,34 +576,54 @@ __device__ __forceinline__ void ss_min_max(const uint64_t (&key)[SS_K], int rem,
     __syncthreads();   /* red may be reused */
 }
 
-/* Sorts the block's m values (key[j] of thread (w, lane) is value number
- * w * SS_WCHUNK + j * 64 + lane; the first m are valid) ascending over their
- * low nb bits, which must hold every valid value (the callers sort keys
- * relative to the block's smallest).  Stable 8-bit LSD passes: each wave
- * ranks its values with one ballot per digit bit, the block turns the
- * per-wave counts into positions and scatters into L.stage.  On return
- * L.stage[0..m) holds the sorted values and key[] the values at the
- * thread's positions of that order. */
-__device__ __forceinline__ void ss_block_sort(uint64_t (&key)[SS_K], int m, int nb, SsLds &L)
+/* The block's m values are spread over the waves in chunks of `chunk` (a
+ * multiple of 64): value i belongs to wave i / chunk, row (i % chunk) / 64,
+ * lane i % 64; v[j] of a thread is its row j.  ss_chunk gives the chunk
+ * size for m values and K rows per wave at most. */
+template <int K>
+__device__ __forceinline__ int ss_chunk(int m)
 {
+    const int rows = (m + SS_NW * 64 - 1) / (SS_NW * 64);
+    return (rows < 1 ? 1 : (rows > K ? K : rows)) * 64;
+}
+
+__device__ __forceinline__ int ss_rem(int m, int chunk)
+{
+    const int w = threadIdx.x >> 6;
+    return m - w * chunk < 0 ? 0 : (m - w * chunk > chunk ? chunk : m - w * chunk);
+}
+
+/* Sorts the m values (layout as above) ascending over their low nb bits,
+ * which hold every valid value.  Stable 8-bit LSD passes: each wave ranks its
+ * values with one ballot per digit bit against its own counter of each digit
+ * (the rows of a wave in order), one block scan over the (digit, wave)
+ * counters gives every (digit, wave) its first position, and the values are
+ * scattered into the stage.  On return the stage's first m entries hold the
+ * sorted values and v[] the values at the thread's positions. */
+template <typename V, int K>
+__device__ __forceinline__ void ss_block_sort(V (&v)[K], int m, int chunk, int nb, SsLds &L)
+{
+    V *stage = ss_stage<V>(L);
     const int w = threadIdx.x >> 6, lane = lane_id();
     const uint64_t lt = lanemask_lt();
-    const int rem = m - w * SS_WCHUNK < 0 ? 0 : (m - w * SS_WCHUNK > SS_WCHUNK ? SS_WCHUNK : m - w * SS_WCHUNK);
-    if (nb <= 0) {   /* every valid key is equal: any order is sorted */
+    const int rem = ss_rem(m, chunk);
+    if (nb <= 0) {   /* every valid value is equal: any order is sorted */
 #pragma unroll
-        for (int j = 0; j < SS_K; j++)
-            if (j * 64 + lane < rem) L.stage[w * SS_WCHUNK + j * 64 + lane] = key[j];
+        for (int j = 0; j < K; j++)
+            if (j * 64 + lane < rem) stage[w * chunk + j * 64 + lane] = v[j];
         __syncthreads();
         return;
     }
+    uint64_t *cnt64 = reinterpret_cast<uint64_t *>(L.cnt);
     for (int sh = 0; sh < nb; sh += 8) {
-        for (int i = threadIdx.x; i < SS_NW * 256; i += SS_T) L.wcnt[i] = 0;
+        cnt64[threadIdx.x] = 0;   /* 4096 counters: four per thread */
         __syncthreads();
-        uint32_t rank2[SS_K / 2];
+        uint32_t rank2[K / 2];
 #pragma unroll
-        for (int j = 0; j < SS_K; j++) {
+        for (int j = 0; j < K; j++) {
+            if (j * 64 >= rem) break;   /* wave-uniform */
             const bool valid = j * 64 + lane < rem;
-            const unsigned d = (unsigned)(key[j] >> sh) & 255u;
+            const unsigned d = (unsigned)(v[j] >> sh) & 255u;
             uint64_t mism = 0;
 #pragma unroll
             for (int bit = 0; bit < 8; bit++) {
@@ -611,64 +633,89 @@ __device__ __forceinline__ void ss_block_sort(uint64_t (&key)[SS_K], int m, int 
             }
             const uint64_t peers = __ballot(valid) & ~mism;
             const uint32_t before = (uint32_t)__popcll(peers & lt);
-            const uint32_t cnt = L.wcnt[w * 256 + d];
-            __builtin_amdgcn_wave_barrier();
-            if (valid && before == 0) L.wcnt[w * 256 + d] = (uint16_t)(cnt + (uint32_t)__popcll(peers));
-            __builtin_amdgcn_wave_barrier();
-            const uint32_t r = cnt + before;
+            /* the wave's LDS operations execute in order: the leader's
+             * update is seen by the next row's read */
+            const uint32_t c = L.cnt[d * SS_NW + w];
+            if (valid && before == 0) L.cnt[d * SS_NW + w] = (uint16_t)(c + (uint32_t)__popcll(peers));
+            const uint32_t r = c + before;
             if (j & 1) rank2[j >> 1] |= r << 16;
             else rank2[j >> 1] = r;
-            __builtin_amdgcn_sched_barrier(0);   /* one key at a time: registers */
         }
         __syncthreads();
-        uint32_t tot = 0;
-        if (threadIdx.x < 256) {
-#pragma unroll
-            for (int ww = 0; ww < SS_NW; ww++) {
-                const uint32_t c = L.wcnt[ww * 256 + threadIdx.x];
-                L.wcnt[ww * 256 + threadIdx.x] = (uint16_t)tot;
-                tot += c;
-            }
-        }
-        uint32_t all;
-        const uint32_t ex = block_excl_scan<uint32_t>(tot, &all, L.wtmp);   /* threads >= 256 add 0 */
-        if (threadIdx.x < 256) L.dstart[threadIdx.x] = ex;
-        __syncthreads();
-#pragma unroll
-        for (int j = 0; j < SS_K; j++) {
-            if (j * 64 + lane < rem) {
-                const unsigned d = (unsigned)(key[j] >> sh) & 255u;
-                const uint32_t r = (rank2[j >> 1] >> ((j & 1) * 16)) & 0xffffu;
-                L.stage[L.dstart[d] + L.wcnt[w * 256 + d] + r] = key[j];
-            }
-            __builtin_amdgcn_sched_barrier(0);
+        {
+            /* thread t: digit t / 4, waves 4 (t % 4) .. + 3 */
+            const uint64_t q = cnt64[threadIdx.x];
+            const uint32_t c0 = (uint32_t)(q & 0xffff), c1 = (uint32_t)((q >> 16) & 0xffff),
+                           c2 = (uint32_t)((q >> 32) & 0xffff), c3 = (uint32_t)(q >> 48);
+            uint32_t all;
+            const uint32_t ex = block_excl_scan_dpp(c0 + c1 + c2 + c3, &all, L.wtmp);
+            cnt64[threadIdx.x] = (uint64_t)ex | ((uint64_t)(ex + c0) << 16) | ((uint64_t)(ex + c0 + c1) << 32) |
+                                 ((uint64_t)(ex + c0 + c1 + c2) << 48);
         }
         __syncthreads();
+        /* positions (< 16384) replace the ranks in place; all counter reads
+         * are issued before the first store */
 #pragma unroll
-        for (int j = 0; j < SS_K; j++)
-            if (j * 64 + lane < rem) key[j] = L.stage[w * SS_WCHUNK + j * 64 + lane];
+        for (int j = 0; j < K; j++) {
+            if (j * 64 >= rem) break;
+            const unsigned d = (unsigned)(v[j] >> sh) & 255u;
+            const uint32_t p = L.cnt[d * SS_NW + w] + ((rank2[j >> 1] >> ((j & 1) * 16)) & 0xffffu);
+            rank2[j >> 1] = (j & 1) ? ((rank2[j >> 1] & 0xffffu) | (p << 16)) : ((rank2[j >> 1] & 0xffff0000u) | p);
+        }
+#pragma unroll
+        for (int j = 0; j < K; j++) {
+            if (j * 64 >= rem) break;
+            if (j * 64 + lane < rem) stage[(rank2[j >> 1] >> ((j & 1) * 16)) & 0xffffu] = v[j];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < K; j++)
+            if (j * 64 + lane < rem) v[j] = stage[w * chunk + j * 64 + lane];
     }
     __syncthreads();
 }
 
-/* B - 1 splitters from a regular sample of SS_S keys (n > SS_S) */
-__global__ __launch_bounds__(SS_T) void k_ss_sample(const uint64_t *__restrict__ keys, size_t n, int B,
+/* the regular sample of SS_S keys, gathered by many blocks (one block's
+ * scattered loads would take ~20 us) into smp */
+__global__ __launch_bounds__(256) void k_ss_gather(const uint64_t *__restrict__ keys, size_t n,
+                                                   uint64_t *__restrict__ smp)
+{
+    const size_t s = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (s < (size_t)SS_S) smp[s] = keys[((2 * s + 1) * n) / (2 * (size_t)SS_S)];
+}
+
+/* B - 1 splitters from a regular sample of SS_S keys (n > SS_S).  The
+ * sample is sorted on the top 32 bits of its range (key - min) >> z, and a
+ * splitter is min + (value << z): monotone, which is all a splitter needs
+ * (ties in the top bits only move a bucket boundary). */
+__global__ __launch_bounds__(SS_T) void k_ss_sample(const uint64_t *__restrict__ smp, int B,
                                                     uint64_t *__restrict__ spl)
 {
     __shared__ SsLds L;
+    constexpr int K = SS_S / SS_T;
     const int w = threadIdx.x >> 6, lane = lane_id();
-    uint64_t key[SS_K];
+    constexpr int chunk = K * 64;
+    uint64_t a = ~0ULL, b = 0, k[K];
 #pragma unroll
-    for (int j = 0; j < SS_K; j++) {
-        const size_t s = (size_t)w * SS_WCHUNK + j * 64 + lane;
-        key[j] = keys[((2 * s + 1) * n) / (2 * (size_t)SS_S)];
+    for (int j = 0; j < K; j++) {
+        k[j] = smp[w * chunk + j * 64 + lane];
+        a = k[j] < a ? k[j] : a;
+        b = k[j] > b ? k[j] : b;
     }
     uint64_t mn, mx;
-    ss_min_max(key, SS_WCHUNK, L, &mn, &mx);
+    ss_min_max(a, b, L, &mn, &mx);
+    const int nb = mx > mn ? 64 - __clzll((long long)(mx - mn)) : 0;
+    const int z = nb > 32 ? nb - 32 : 0;
+    uint32_t v[K];
 #pragma unroll
-    for (int j = 0; j < SS_K; j++) key[j] -= mn;
-    ss_block_sort(key, SS_S, mx > mn ? 64 - __clzll((long long)(mx - mn)) : 0, L);
-    for (int i = threadIdx.x; i < B - 1; i += SS_T) spl[i] = mn + L.stage[((size_t)(i + 1) * SS_S) / B];
+    for (int j = 0; j < K; j++) v[j] = (uint32_t)((k[j] - mn) >> z);
+    /* a run-time count keeps the sort's loops rolled as in k_ss_local (a
+     * constant one lets the compiler interleave all rows and spill) */
+    int m = SS_S;
+    asm volatile("" : "+s"(m));
+    ss_block_sort<uint32_t, K>(v, m, chunk, nb - z, L);
+    for (int i = threadIdx.x; i < B - 1; i += SS_T)
+        spl[i] = mn + ((uint64_t)L.stage.v32[((size_t)(i + 1) * SS_S) / B] << z);
 }
 
 struct SsPartLds {
@@ -754,50 +801,149 @@ __global__ __launch_bounds__(SS_T) void k_ss_part(const uint64_t *__restrict__ k
     }
 }
 
+/* Sorts the bucket's m keys relative to mn and counts the unique values per
+ * wave into L.wc; the sorted values stay in the stage (*ms of them, at most
+ * SS_T * K).  The keys are first deduplicated through an LDS hash set (the
+ * stage as open-addressing slots with linear probing; all ones marks an
+ * empty slot and is never a value), inserted as they are read, so only
+ * distinct values are sorted: a move-candidate bucket holds each key about
+ * 3.6 times.  Returns the bits that mark, per row j of this thread, a value
+ * starting a run of equal values in the sorted order, or -1 when the bucket
+ * has more distinct values than the set or one block's sort holds (the
+ * caller reports it oversize). */
+template <typename V, int K>
+__device__ __forceinline__ int64_t ss_sort_bucket(const uint64_t *__restrict__ s, int m, uint64_t mn, int nb,
+                                                  SsLds &L, int *ms)
+{
+    const int w = threadIdx.x >> 6, lane = lane_id();
+    /* the stage as a hash set of V: 16384 slots of 32 bits or 8192 of 64 */
+    constexpr int NS = (int)(sizeof(SsLds::stage) / sizeof(V));
+    constexpr V EMPTY = (V)~(V)0;
+    V *T = ss_stage<V>(L);
+    for (int i = threadIdx.x; i < NS; i += SS_T) T[i] = EMPTY;
+    __syncthreads();
+    bool full = false;
+    for (int i = threadIdx.x; i < m; i += SS_T) {
+        const V x = (V)(s[i] - mn);
+        uint32_t h = (sizeof(V) == 4) ? (((uint32_t)x * 2654435761u) >> 18)
+                                      : (uint32_t)(((uint64_t)x * 0x9E3779B97F4A7C15ULL) >> 51);   /* 13 bits */
+        for (int probe = 0;; probe++) {
+            if (probe == NS) {   /* every slot holds another value: too many distinct values */
+                full = true;
+                break;
+            }
+            const V old = atomicCAS(&T[h], EMPTY, x);
+            if (old == EMPTY || old == x) break;
+            h = (h + 1) & (NS - 1);
+        }
+    }
+    constexpr int PER = NS / SS_T;
+    V x[PER];
+    uint32_t c = 0;
+    const bool any_full = __syncthreads_or(full);
+    if (any_full) return -1;
+#pragma unroll
+    for (int j = 0; j < PER; j++) {
+        x[j] = T[threadIdx.x + j * SS_T];
+        c += x[j] != EMPTY;
+    }
+    uint32_t all;
+    uint32_t o = block_excl_scan_dpp(c, &all, L.wtmp);   /* its barrier ends the reads */
+#pragma unroll
+    for (int j = 0; j < PER; j++)
+        if (x[j] != EMPTY) T[o++] = x[j];
+    __syncthreads();
+    const int n = (int)all;
+    if (n > SS_T * K) return -1;
+    const int chunk = ss_chunk<K>(n), rem = ss_rem(n, chunk);
+    V v[K];
+#pragma unroll
+    for (int j = 0; j < K; j++) v[j] = j * 64 + lane < rem ? T[w * chunk + j * 64 + lane] : (V)0;
+    __syncthreads();
+    ss_block_sort<V, K>(v, n, chunk, nb, L);
+    const V *stage = ss_stage<V>(L);
+    uint32_t fbits = 0, nu = 0;
+#pragma unroll
+    for (int j = 0; j < K; j++) {
+        const int i = w * chunk + j * 64 + lane;
+        const bool f = j * 64 + lane < rem && (i == 0 || stage[i - 1] != stage[i]);
+        fbits |= (uint32_t)f << j;
+        nu += (uint32_t)__popcll(__ballot(f));
+    }
+    if (lane == 0) L.wc[w] = nu;
+    *ms = n;
+    return fbits;
+}
+
+/* writes the unique values of the sorted stage (fbits from ss_sort_bucket,
+ * ms values) to dst + o, as keys mn + value */
+template <typename V, int K>
+__device__ __forceinline__ void ss_emit(uint32_t fbits, int ms, uint64_t mn, uint64_t o, uint64_t *__restrict__ dst,
+                                        SsLds &L)
+{
+    const int w = threadIdx.x >> 6, lane = lane_id();
+    const uint64_t lt = lanemask_lt();
+    const int chunk = ss_chunk<K>(ms);
+    const V *stage = ss_stage<V>(L);
+    for (int ww = 0; ww < w; ww++) o += L.wc[ww];
+#pragma unroll
+    for (int j = 0; j < K; j++) {
+        const uint64_t fm = __ballot((fbits >> j) & 1u);
+        if ((fbits >> j) & 1u) dst[o + __popcll(fm & lt)] = mn + (uint64_t)stage[w * chunk + j * 64 + lane];
+        o += __popcll(fm);
+    }
+}
+
 /* One bucket per block, buckets taken in ticket order: bucket t holds
- * counts[t] keys at src + t * stride (B == 1: the n1 keys at src).  Sorted
- * and deduplicated in LDS; the unique keys go to dst at the bucket's place
- * (a chained scan of the buckets' unique counts; the last bucket writes the
- * total). */
-__global__ __launch_bounds__(SS_T) void k_ss_local(const uint64_t *__restrict__ src, size_t stride,
-                                                   const unsigned long long *counts, size_t n1, int B,
-                                                   uint64_t *__restrict__ dst, uint64_t *status,
-                                                   unsigned *ticket, uint32_t epoch,
-                                                   unsigned long long *total, unsigned *err, int cap)
+ * counts[t] keys at src + t * stride (B == 1: the n1 keys at src).  A bucket
+ * spanning less than 2^32 - 1 is deduplicated as 32-bit values relative to
+ * its smallest key (up to cap keys) and its distinct values sorted, a wider
+ * one sorted as 64-bit values (up to 8192 keys) and
+ * deduplicated; and the unique keys go to dst at the bucket's
+ * place (a chained scan of the buckets' unique counts; the last bucket
+ * writes the total). */
+__global__ __launch_bounds__(SS_T, 8) void k_ss_local(const uint64_t *__restrict__ src, size_t stride,
+                                                      const unsigned long long *counts, size_t n1, int B,
+                                                      uint64_t *__restrict__ dst, uint64_t *status,
+                                                      unsigned *ticket, uint32_t epoch,
+                                                      unsigned long long *total, unsigned *err, int cap)
 {
     __shared__ SsLds L;
     const int w = threadIdx.x >> 6, lane = lane_id();
-    const uint64_t lt = lanemask_lt();
     if (threadIdx.x == 0) L.ticket = B > 1 ? atomicAdd(ticket, 1u) : 0u;
     __syncthreads();
     const size_t t = L.ticket;
     size_t m = B > 1 ? (size_t)counts[t] : n1;
-    if (m > (size_t)cap) {   /* oversize: published as empty, the caller repeats the sort */
+    const uint64_t *s = src + t * stride;
+    /* smallest and largest key (the keys are read again below, from L2) */
+    uint64_t a = ~0ULL, b = 0;
+    if (m <= (size_t)SS_LCAP)
+        for (size_t i = threadIdx.x; i < m; i += SS_T) {
+            const uint64_t k = s[i];
+            a = k < a ? k : a;
+            b = k > b ? k : b;
+        }
+    uint64_t mn, mx;
+    ss_min_max(a, b, L, &mn, &mx);
+    /* 32-bit values below 0xFFFFFFFF (the hash set's empty mark) */
+    const bool wide = (mx - mn) >= 0xffffffffULL && m > 1;
+    /* a range of 2^64 - 1 would make the largest key's value the empty
+     * mark of the 64-bit hash set */
+    if (m > (size_t)cap || (mx - mn) == ~0ULL) {   /* oversize: published as empty, the caller repeats the sort */
         if (threadIdx.x == 0) atomicOr(err, SORT_ERR_OVERSIZE);
         m = 0;
     }
-    const uint64_t *s = src + t * stride;
-    const int rem = (int)m - w * SS_WCHUNK < 0 ? 0
-                    : ((int)m - w * SS_WCHUNK > SS_WCHUNK ? SS_WCHUNK : (int)m - w * SS_WCHUNK);
-    uint64_t key[SS_K];
-#pragma unroll
-    for (int j = 0; j < SS_K; j++) key[j] = j * 64 + lane < rem ? s[w * SS_WCHUNK + j * 64 + lane] : 0;
-    uint64_t mn, mx;
-    ss_min_max(key, rem, L, &mn, &mx);
-#pragma unroll
-    for (int j = 0; j < SS_K; j++) key[j] -= mn;
-    ss_block_sort(key, (int)m, (m > 1 && mx > mn) ? 64 - __clzll((long long)(mx - mn)) : 0, L);
-    /* unique flags (bit j: key j is the first of its value in the sorted
-     * order) */
-    uint32_t fbits = 0, c = 0;
-#pragma unroll
-    for (int j = 0; j < SS_K; j++) {
-        const int i = w * SS_WCHUNK + j * 64 + lane;
-        const bool f = j * 64 + lane < rem && (i == 0 || L.stage[i - 1] != key[j]);
-        fbits |= (uint32_t)f << j;
-        c += (uint32_t)__popcll(__ballot(f));
+    const int nb = (m > 1 && mx > mn) ? 64 - __clzll((long long)(mx - mn)) : 0;
+    int ms = 0;
+    int64_t fb = wide ? ss_sort_bucket<uint64_t, SS_KS64>(s, (int)m, mn, nb, L, &ms)
+                      : ss_sort_bucket<uint32_t, SS_KS32>(s, (int)m, mn, nb, L, &ms);
+    if (fb < 0) {   /* more distinct values than one block sorts: oversize */
+        if (threadIdx.x == 0) atomicOr(err, SORT_ERR_OVERSIZE);
+        if (lane == 0) L.wc[w] = 0;
+        fb = 0;
+        ms = 0;
     }
-    if (lane == 0) L.wc[w] = c;
+    const uint32_t fbits = (uint32_t)fb;
     __syncthreads();
     if (w == 0) {
         uint32_t tot = 0;
@@ -844,14 +990,8 @@ __global__ __launch_bounds__(SS_T) void k_ss_local(const uint64_t *__restrict__ 
         }
     }
     __syncthreads();
-    uint64_t o = L.excl;
-    for (int ww = 0; ww < w; ww++) o += L.wc[ww];
-#pragma unroll
-    for (int j = 0; j < SS_K; j++) {
-        const uint64_t fm = __ballot((fbits >> j) & 1u);
-        if ((fbits >> j) & 1u) dst[o + __popcll(fm & lt)] = key[j] + mn;
-        o += __popcll(fm);
-    }
+    if (wide) ss_emit<uint64_t, SS_KS64>(fbits, ms, mn, L.excl, dst, L);
+    else ss_emit<uint32_t, SS_KS32>(fbits, ms, mn, L.excl, dst, L);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1124,7 +1264,7 @@ static OsScratch carve(void *scratch)
 /* ---- sample sort: configuration ---- */
 
 static int g_ss_on = 1;              /* sample sort for sizes it handles */
-static int g_ss_bavg = 6000;         /* mean keys per bucket */
+static int g_ss_bavg = 5000;         /* mean keys per bucket */
 static int g_ss_cap = SS_LCAP;       /* bucket capacity (lowered only to test the repeat) */
 static unsigned long long g_ss_sorts = 0;
 
@@ -1141,13 +1281,19 @@ unsigned long long sort_sample_count() { return g_ss_sorts; }
 
 /* buckets of a sample sort of n keys (1: one block), or 0 when the sort
  * takes the LSD path */
+/* up to SS_BMAX buckets of about g_ss_bavg keys; a larger sort takes
+ * SS_BMAX buckets of up to SS_BAVG_MAX keys (the largest sorts are the
+ * layers' move candidates, which hold each key about 3.6 times, so their
+ * buckets have few distinct values), and beyond that the LSD path */
+constexpr size_t SS_BAVG_MAX = 7000;
 static size_t ss_buckets(size_t n)
 {
     if (!g_ss_on || n < 2) return 0;
     if (n <= (size_t)SS_LCAP) return 1;
     size_t B = (n + (size_t)g_ss_bavg - 1) / (size_t)g_ss_bavg;
     if (B < 2) B = 2;
-    return B <= (size_t)SS_BMAX ? B : 0;
+    if (B <= (size_t)SS_BMAX) return B;
+    return n <= (size_t)SS_BMAX * SS_BAVG_MAX ? (size_t)SS_BMAX : 0;
 }
 
 size_t sort_part_keys(size_t n)
@@ -1250,7 +1396,11 @@ static void ss_sort_unique(hipStream_t st, const uint64_t *keys, size_t n, size_
     if (B > 1) {
         launch_fill_u64(st, (uint64_t *)s.ss_ticket, SS_CLEAR_BYTES / 8, 0);
         int h = prof_hook_begin(8, st);
-        hipLaunchKernelGGL(k_ss_sample, dim3(1), dim3(SS_T), 0, st, keys, n, (int)B, s.ss_spl);
+        /* the sample goes to the start of part, which the partition
+         * overwrites only after the splitters are made */
+        hipLaunchKernelGGL(k_ss_gather, dim3(SS_S / 256), dim3(256), 0, st, keys, n, part);
+        hipLaunchKernelGGL(k_ss_sample, dim3(1), dim3(SS_T), 0, st, (const uint64_t *)part, (int)B,
+                           s.ss_spl);
         snote(hipGetLastError());
         prof_hook_end(h, 8.0 * SS_S + 8.0 * (B - 1), st);
         h = prof_hook_begin(9, st);

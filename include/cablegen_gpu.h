@@ -191,13 +191,14 @@ int cg_dev_sync(void);
  *                (lowered only to exercise the repeat);
  *   "builtin"    built-in defaults: 8-bit digits, 8192-key tiles, window 16
  *                for every entry, no forced field, delta 0, spawn_estimate 0,
- *                cell_code 1, carry_stream 1, sample_sort 1 with 6000 keys
+ *                cell_code 1, carry_stream 1, sample_sort 1 with 5000 keys
  *                per bucket and capacity 16384;
  *   "reset"      the state after initialisation: the built-in defaults, then
  *                the table loaded at initialisation if there was one.
  * Returns 0, or -1 for an unknown knob or value.  cg_tune_get also reads
  * "cell_code_sorts": the number of sorts that used the cell-pair code, and
- * "sample_sorts": the number of sample sorts. */
+ * "sample_sorts": the number of sample sorts, and "sample_repeats": the
+ * number of sample sorts repeated on the LSD path after an oversize bucket. */
 int cg_tune_set(const char *knob, int value);
 int cg_tune_get(const char *knob);
 /* the sorts read their configuration from a table indexed by size class
