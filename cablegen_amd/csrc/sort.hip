@@ -511,12 +511,13 @@ __global__ __launch_bounds__(UQ_THREADS) void k_os_unique(const uint64_t *__rest
  *                splitters around it (all copies of a key to one bucket),
  *                tiles rank their keys per bucket in LDS and reserve room
  *                with one atomic per (tile, bucket) in a bucket-major
- *                buffer of SS_LCAP keys per bucket;
+ *                buffer (SS_NX regions per bucket, one per XCD);
  *   k_ss_local   one block per bucket (in ticket order): the bucket's keys
- *                are sorted in LDS by stable 8-bit LSD passes over the bits
- *                in which they differ, deduplicated, and written at their
- *                place, which a chained scan of unique counts over the
- *                buckets gives.
+ *                are deduplicated through an LDS hash set, the distinct
+ *                values sorted in LDS by stable 8-bit LSD passes over the
+ *                bits in which they differ, and written at their place,
+ *                which a chained scan of distinct counts over the buckets
+ *                gives.
  * So a key is read twice and written once, and only unique keys reach the
  * output.  A bucket above SS_LCAP keys (a sample far from its data, or one
  * key repeated tens of thousands of times) sets SORT_ERR_OVERSIZE: the
@@ -530,6 +531,18 @@ constexpr int SS_KS64 = 8;                  /* 64-bit values: 8192 */
 constexpr int SS_PK = 8;                    /* keys per thread of a partition tile */
 constexpr int SS_TILE = SS_T * SS_PK;       /* 8192 */
 constexpr int SS_BMAX = 2048;               /* buckets */
+/* A bucket's keys land in SS_NX regions, one per group of partition blocks
+ * (blockIdx % SS_NX, which on MI355X is the XCD the block runs on): SS_NX
+ * counters per bucket, so no counter takes every tile's reservation, and a
+ * region's partly written lines stay in one XCD's L2.  Each region holds a
+ * whole bucket, since the keys of one bucket may all come from a few
+ * neighbouring tiles (moves of a sorted layer keep its order). */
+constexpr int SS_NX = 8;
+constexpr int SS_SUBCAP = SS_LCAP;
+constexpr int SS_BSTRIDE = SS_NX * SS_SUBCAP;   /* keys of a bucket's regions */
+/* splitter guide: the number of splitters below smin + (p << gshift) for
+ * p = 0..SS_GUIDE, which narrows a key's splitter search to a few entries */
+constexpr int SS_GUIDE = 4096;
 constexpr int SS_S = SS_LCAP;               /* samples */
 
 /* 64 KiB of values plus counters: two blocks per CU */
@@ -543,7 +556,8 @@ struct SsLds {
     uint16_t cnt[256 * SS_NW] __attribute__((aligned(8)));
     uint32_t wtmp[17];
     uint64_t red[2 * SS_NW];
-    uint32_t wc[SS_NW];
+    uint64_t vlo, vhi;      /* extremes of a bucket's values */
+    uint32_t segc[SS_NX];   /* keys of the bucket's regions */
     uint64_t excl;
     unsigned ticket;
 };
@@ -689,7 +703,8 @@ __global__ __launch_bounds__(256) void k_ss_gather(const uint64_t *__restrict__ 
  * splitter is min + (value << z): monotone, which is all a splitter needs
  * (ties in the top bits only move a bucket boundary). */
 __global__ __launch_bounds__(SS_T) void k_ss_sample(const uint64_t *__restrict__ smp, int B,
-                                                    uint64_t *__restrict__ spl)
+                                                    uint64_t *__restrict__ spl, uint16_t *__restrict__ guide,
+                                                    uint64_t *__restrict__ gparm)
 {
     __shared__ SsLds L;
     constexpr int K = SS_S / SS_T;
@@ -714,8 +729,36 @@ __global__ __launch_bounds__(SS_T) void k_ss_sample(const uint64_t *__restrict__
     int m = SS_S;
     asm volatile("" : "+s"(m));
     ss_block_sort<uint32_t, K>(v, m, chunk, nb - z, L);
-    for (int i = threadIdx.x; i < B - 1; i += SS_T)
-        spl[i] = mn + ((uint64_t)L.stage.v32[((size_t)(i + 1) * SS_S) / B] << z);
+    const int nspl = B - 1;
+    /* the splitters' sample values, kept in the counters' room (2048 words) */
+    uint32_t *sv = reinterpret_cast<uint32_t *>(L.cnt);
+    for (int i = threadIdx.x; i < nspl; i += SS_T) {
+        const uint32_t x = L.stage.v32[((size_t)(i + 1) * SS_S) / B];
+        sv[i] = x;
+        spl[i] = mn + ((uint64_t)x << z);
+    }
+    __syncthreads();
+    /* guide[p]: splitters below mn + (p << gs); the sample's range fits
+     * SS_GUIDE steps of 2^gs */
+    const int gs = nb > 12 ? nb - 12 : 0;
+    for (int p = threadIdx.x; p <= SS_GUIDE; p += SS_T) {
+        int lo = 0, hi = nspl;
+        if (p == SS_GUIDE) {
+            lo = nspl;
+        } else {
+            const uint64_t off = (uint64_t)p << gs;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (((uint64_t)sv[mid] << z) < off) lo = mid + 1;
+                else hi = mid;
+            }
+        }
+        guide[p] = (uint16_t)lo;
+    }
+    if (threadIdx.x == 0) {
+        gparm[0] = mn;
+        gparm[1] = (uint64_t)gs;
+    }
 }
 
 struct SsPartLds {
@@ -724,22 +767,31 @@ struct SsPartLds {
     uint64_t stage[SS_TILE];
     uint32_t cnt[SS_BMAX];        /* counts, then the tile's start of each bucket */
     uint16_t bid[SS_TILE];
+    uint16_t guide[SS_GUIDE + 1];
     uint32_t wtmp[17];
 };
 
-/* Bucket of every key (the number of splitters <= key), ranked per bucket in
- * LDS; one reservation per (tile, bucket) in gcur; the tile staged in bucket
- * order and written to part[b * SS_LCAP + position].  Room beyond SS_LCAP in
- * a bucket is not written (the bucket is then oversize). */
+/* Bucket of every key (the number of splitters <= key, searched between the
+ * guide's bounds), ranked per bucket in LDS; one reservation per (tile,
+ * bucket) on the bucket's counter of this block's region group; the tile
+ * staged in bucket order and written to part[b * SS_BSTRIDE + group *
+ * SS_SUBCAP + position].  A region that fills up is not written past its
+ * end (the bucket is then oversize). */
 __global__ __launch_bounds__(SS_T) void k_ss_part(const uint64_t *__restrict__ keys, size_t n,
-                                                  const uint64_t *__restrict__ spl_g, int B,
+                                                  const uint64_t *__restrict__ spl_g,
+                                                  const uint16_t *__restrict__ guide_g,
+                                                  const uint64_t *__restrict__ gparm, int B,
                                                   uint64_t *__restrict__ part,
-                                                  unsigned long long *gcur, unsigned *err, int cap)
+                                                  unsigned long long *gcur, unsigned *err)
 {
     __shared__ SsPartLds P;
     const int nspl = B - 1;
+    const int grp = blockIdx.x % SS_NX;
     for (int i = threadIdx.x; i < nspl; i += SS_T) P.spl[i] = spl_g[i];
+    for (int i = threadIdx.x; i <= SS_GUIDE; i += SS_T) P.guide[i] = guide_g[i];
     for (int i = threadIdx.x; i < B; i += SS_T) P.cnt[i] = 0;
+    const uint64_t smin = gparm[0];
+    const int gs = (int)gparm[1];
     __syncthreads();
     const size_t tile0 = (size_t)blockIdx.x * SS_TILE;
     uint64_t k[SS_PK];
@@ -753,7 +805,9 @@ __global__ __launch_bounds__(SS_T) void k_ss_part(const uint64_t *__restrict__ k
     for (int j = 0; j < SS_PK; j++) {
         const size_t i = tile0 + (size_t)j * SS_T + threadIdx.x;
         if (i < n) {
-            int lo = 0, hi = nspl;
+            uint64_t pp = k[j] < smin ? 0 : (k[j] - smin) >> gs;
+            const int p = pp < (uint64_t)SS_GUIDE ? (int)pp : SS_GUIDE - 1;
+            int lo = P.guide[p], hi = P.guide[p + 1];
             while (lo < hi) {
                 const int mid = (lo + hi) >> 1;
                 if (P.spl[mid] <= k[j]) lo = mid + 1;
@@ -767,17 +821,17 @@ __global__ __launch_bounds__(SS_T) void k_ss_part(const uint64_t *__restrict__ k
     const int b0 = 2 * threadIdx.x, b1 = b0 + 1;
     const uint32_t c0 = b0 < B ? P.cnt[b0] : 0, c1 = b1 < B ? P.cnt[b1] : 0;
     uint32_t all;
-    const uint32_t ex = block_excl_scan<uint32_t>(c0 + c1, &all, P.wtmp);
+    const uint32_t ex = block_excl_scan_dpp(c0 + c1, &all, P.wtmp);
     bool over = false;
     if (c0) {
-        const unsigned long long g = atomicAdd(&gcur[b0], (unsigned long long)c0);
-        over |= g + c0 > (unsigned long long)cap;
-        P.wbase[b0] = (uint64_t)b0 * SS_LCAP + g - ex;
+        const unsigned long long g = atomicAdd(&gcur[b0 * SS_NX + grp], (unsigned long long)c0);
+        over |= g + c0 > (unsigned long long)SS_SUBCAP;
+        P.wbase[b0] = (uint64_t)b0 * SS_BSTRIDE + (uint64_t)grp * SS_SUBCAP + g - ex;
     }
     if (c1) {
-        const unsigned long long g = atomicAdd(&gcur[b1], (unsigned long long)c1);
-        over |= g + c1 > (unsigned long long)cap;
-        P.wbase[b1] = (uint64_t)b1 * SS_LCAP + g - (ex + c0);
+        const unsigned long long g = atomicAdd(&gcur[b1 * SS_NX + grp], (unsigned long long)c1);
+        over |= g + c1 > (unsigned long long)SS_SUBCAP;
+        P.wbase[b1] = (uint64_t)b1 * SS_BSTRIDE + (uint64_t)grp * SS_SUBCAP + g - (ex + c0);
     }
     if (over) atomicOr(err, SORT_ERR_OVERSIZE);
     if (b0 < B) P.cnt[b0] = ex;
@@ -797,51 +851,67 @@ __global__ __launch_bounds__(SS_T) void k_ss_part(const uint64_t *__restrict__ k
     for (uint32_t p = threadIdx.x; p < tn; p += SS_T) {
         const uint32_t b = P.bid[p];
         const uint64_t o = P.wbase[b] + p;
-        if (o < (uint64_t)(b + 1) * SS_LCAP) part[o] = P.stage[p];
+        if (o < (uint64_t)b * SS_BSTRIDE + (uint64_t)(grp + 1) * SS_SUBCAP) part[o] = P.stage[p];
     }
 }
 
-/* Sorts the bucket's m keys relative to mn and counts the unique values per
- * wave into L.wc; the sorted values stay in the stage (*ms of them, at most
- * SS_T * K).  The keys are first deduplicated through an LDS hash set (the
- * stage as open-addressing slots with linear probing; all ones marks an
- * empty slot and is never a value), inserted as they are read, so only
+/* Deduplicates the bucket's keys (the SS_NX regions of L.segc[x] keys at s +
+ * x * SS_SUBCAP) as values k - base of type V through an LDS hash set: the
+ * stage as open-addressing slots with linear probing, all ones marking an
+ * empty slot (never a value).  Keys are inserted as they are read, so only
  * distinct values are sorted: a move-candidate bucket holds each key about
- * 3.6 times.  Returns the bits that mark, per row j of this thread, a value
- * starting a run of equal values in the sorted order, or -1 when the bucket
- * has more distinct values than the set or one block's sort holds (the
- * caller reports it oversize). */
+ * 3.6 times.  The distinct values are compacted to the front of the stage
+ * relative to the smallest of them (*vmin), and *nbits is the width of
+ * their range.  Returns their count, or -1 when the set or one block's sort
+ * (SS_T * K values) cannot hold them. */
 template <typename V, int K>
-__device__ __forceinline__ int64_t ss_sort_bucket(const uint64_t *__restrict__ s, int m, uint64_t mn, int nb,
-                                                  SsLds &L, int *ms)
+__device__ __forceinline__ int ss_dedupe(const uint64_t *__restrict__ s, uint64_t base, SsLds &L, uint64_t *vmin,
+                                         int *nbits)
 {
-    const int w = threadIdx.x >> 6, lane = lane_id();
-    /* the stage as a hash set of V: 16384 slots of 32 bits or 8192 of 64 */
     constexpr int NS = (int)(sizeof(SsLds::stage) / sizeof(V));
     constexpr V EMPTY = (V)~(V)0;
     V *T = ss_stage<V>(L);
     for (int i = threadIdx.x; i < NS; i += SS_T) T[i] = EMPTY;
+    if (threadIdx.x == 0) {
+        L.vlo = ~0ULL;
+        L.vhi = 0;
+    }
     __syncthreads();
     bool full = false;
-    for (int i = threadIdx.x; i < m; i += SS_T) {
-        const V x = (V)(s[i] - mn);
-        uint32_t h = (sizeof(V) == 4) ? (((uint32_t)x * 2654435761u) >> 18)
-                                      : (uint32_t)(((uint64_t)x * 0x9E3779B97F4A7C15ULL) >> 51);   /* 13 bits */
-        for (int probe = 0;; probe++) {
-            if (probe == NS) {   /* every slot holds another value: too many distinct values */
-                full = true;
-                break;
+    V lo = EMPTY, hi = 0;
+    for (int r = 0; r < SS_NX; r++)
+        for (int i = threadIdx.x; i < (int)L.segc[r]; i += SS_T) {
+            const V x = (V)(s[(size_t)r * SS_SUBCAP + i] - base);
+            lo = x < lo ? x : lo;
+            hi = x > hi ? x : hi;
+            uint32_t h = (sizeof(V) == 4) ? (((uint32_t)x * 2654435761u) >> 18)
+                                          : (uint32_t)(((uint64_t)x * 0x9E3779B97F4A7C15ULL) >> 51);   /* 13 bits */
+            for (int probe = 0;; probe++) {
+                if (probe == NS) {   /* every slot holds another value: too many distinct values */
+                    full = true;
+                    break;
+                }
+                const V old = atomicCAS(&T[h], EMPTY, x);
+                if (old == EMPTY || old == x) break;
+                h = (h + 1) & (NS - 1);
             }
-            const V old = atomicCAS(&T[h], EMPTY, x);
-            if (old == EMPTY || old == x) break;
-            h = (h + 1) & (NS - 1);
         }
+    uint64_t l64 = lo, h64 = hi;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        const uint64_t a = __shfl_xor(l64, d, 64), b = __shfl_xor(h64, d, 64);
+        l64 = a < l64 ? a : l64;
+        h64 = b > h64 ? b : h64;
     }
+    if (lane_id() == 0) {
+        atomicMin((unsigned long long *)&L.vlo, (unsigned long long)l64);
+        atomicMax((unsigned long long *)&L.vhi, (unsigned long long)h64);
+    }
+    if (__syncthreads_or(full)) return -1;
+    const V vl = (V)L.vlo, vh = (V)L.vhi;
     constexpr int PER = NS / SS_T;
     V x[PER];
     uint32_t c = 0;
-    const bool any_full = __syncthreads_or(full);
-    if (any_full) return -1;
 #pragma unroll
     for (int j = 0; j < PER; j++) {
         x[j] = T[threadIdx.x + j * SS_T];
@@ -851,59 +921,46 @@ __device__ __forceinline__ int64_t ss_sort_bucket(const uint64_t *__restrict__ s
     uint32_t o = block_excl_scan_dpp(c, &all, L.wtmp);   /* its barrier ends the reads */
 #pragma unroll
     for (int j = 0; j < PER; j++)
-        if (x[j] != EMPTY) T[o++] = x[j];
+        if (x[j] != EMPTY) T[o++] = x[j] - vl;
     __syncthreads();
     const int n = (int)all;
     if (n > SS_T * K) return -1;
+    *vmin = n ? (uint64_t)vl : 0;
+    *nbits = (n > 1 && vh > vl) ? 64 - __clzll((long long)(uint64_t)(vh - vl)) : 0;
+    return n;
+}
+
+/* Sorts the n distinct values at the front of the stage (ss_dedupe's
+ * output) over their low nb bits; on return they are in ascending order at
+ * the stage's first n entries. */
+template <typename V, int K>
+__device__ __forceinline__ void ss_sort_distinct(int n, int nb, SsLds &L)
+{
+    const int w = threadIdx.x >> 6, lane = lane_id();
     const int chunk = ss_chunk<K>(n), rem = ss_rem(n, chunk);
+    const V *T = ss_stage<V>(L);
     V v[K];
 #pragma unroll
     for (int j = 0; j < K; j++) v[j] = j * 64 + lane < rem ? T[w * chunk + j * 64 + lane] : (V)0;
     __syncthreads();
     ss_block_sort<V, K>(v, n, chunk, nb, L);
-    const V *stage = ss_stage<V>(L);
-    uint32_t fbits = 0, nu = 0;
-#pragma unroll
-    for (int j = 0; j < K; j++) {
-        const int i = w * chunk + j * 64 + lane;
-        const bool f = j * 64 + lane < rem && (i == 0 || stage[i - 1] != stage[i]);
-        fbits |= (uint32_t)f << j;
-        nu += (uint32_t)__popcll(__ballot(f));
-    }
-    if (lane == 0) L.wc[w] = nu;
-    *ms = n;
-    return fbits;
 }
 
-/* writes the unique values of the sorted stage (fbits from ss_sort_bucket,
- * ms values) to dst + o, as keys mn + value */
-template <typename V, int K>
-__device__ __forceinline__ void ss_emit(uint32_t fbits, int ms, uint64_t mn, uint64_t o, uint64_t *__restrict__ dst,
-                                        SsLds &L)
-{
-    const int w = threadIdx.x >> 6, lane = lane_id();
-    const uint64_t lt = lanemask_lt();
-    const int chunk = ss_chunk<K>(ms);
-    const V *stage = ss_stage<V>(L);
-    for (int ww = 0; ww < w; ww++) o += L.wc[ww];
-#pragma unroll
-    for (int j = 0; j < K; j++) {
-        const uint64_t fm = __ballot((fbits >> j) & 1u);
-        if ((fbits >> j) & 1u) dst[o + __popcll(fm & lt)] = mn + (uint64_t)stage[w * chunk + j * 64 + lane];
-        o += __popcll(fm);
-    }
-}
-
-/* One bucket per block, buckets taken in ticket order: bucket t holds
- * counts[t] keys at src + t * stride (B == 1: the n1 keys at src).  A bucket
- * spanning less than 2^32 - 1 is deduplicated as 32-bit values relative to
- * its smallest key (up to cap keys) and its distinct values sorted, a wider
- * one sorted as 64-bit values (up to 8192 keys) and
- * deduplicated; and the unique keys go to dst at the bucket's
- * place (a chained scan of the buckets' unique counts; the last bucket
- * writes the total). */
-__global__ __launch_bounds__(SS_T, 8) void k_ss_local(const uint64_t *__restrict__ src, size_t stride,
-                                                      const unsigned long long *counts, size_t n1, int B,
+/* One bucket per block, buckets taken in ticket order: bucket t holds the
+ * counts[t * SS_NX + x] keys of region x at src + t * SS_BSTRIDE + x *
+ * SS_SUBCAP (B == 1: the n1 keys at src).  An inner bucket's keys lie in
+ * [spl[t - 1], spl[t]): when that range is below 2^32 - 1 the keys are taken
+ * relative to spl[t - 1] as 32-bit values without a pass for their extremes;
+ * else the smallest and largest key are found first and the values are
+ * 32-bit when the keys span less than 2^32 - 1, 64-bit otherwise.  The keys
+ * are deduplicated in LDS and the distinct values sorted; the bucket's
+ * distinct count is published as soon as it is known, and the chained scan
+ * of those counts over the buckets places the unique keys in dst (the last
+ * bucket writes the total).  More than cap keys, or more distinct values
+ * than the hash set holds, is an oversize bucket (published as empty). */
+__global__ __launch_bounds__(SS_T, 8) void k_ss_local(const uint64_t *__restrict__ src,
+                                                      const unsigned long long *counts,
+                                                      const uint64_t *__restrict__ spl, size_t n1, int B,
                                                       uint64_t *__restrict__ dst, uint64_t *status,
                                                       unsigned *ticket, uint32_t epoch,
                                                       unsigned long long *total, unsigned *err, int cap)
@@ -913,85 +970,105 @@ __global__ __launch_bounds__(SS_T, 8) void k_ss_local(const uint64_t *__restrict
     if (threadIdx.x == 0) L.ticket = B > 1 ? atomicAdd(ticket, 1u) : 0u;
     __syncthreads();
     const size_t t = L.ticket;
-    size_t m = B > 1 ? (size_t)counts[t] : n1;
-    const uint64_t *s = src + t * stride;
-    /* smallest and largest key (the keys are read again below, from L2) */
-    uint64_t a = ~0ULL, b = 0;
-    if (m <= (size_t)SS_LCAP)
-        for (size_t i = threadIdx.x; i < m; i += SS_T) {
-            const uint64_t k = s[i];
-            a = k < a ? k : a;
-            b = k > b ? k : b;
+    const uint64_t *s = B > 1 ? src + t * SS_BSTRIDE : src;
+    size_t m = 0;
+    if (B > 1) {
+        for (int x = 0; x < SS_NX; x++) m += counts[t * SS_NX + x];
+        if (threadIdx.x < SS_NX) {
+            const unsigned long long c = counts[t * SS_NX + threadIdx.x];
+            L.segc[threadIdx.x] = (uint32_t)(c < (unsigned long long)SS_SUBCAP ? c : SS_SUBCAP);
         }
-    uint64_t mn, mx;
-    ss_min_max(a, b, L, &mn, &mx);
-    /* 32-bit values below 0xFFFFFFFF (the hash set's empty mark) */
-    const bool wide = (mx - mn) >= 0xffffffffULL && m > 1;
-    /* a range of 2^64 - 1 would make the largest key's value the empty
-     * mark of the 64-bit hash set */
-    if (m > (size_t)cap || (mx - mn) == ~0ULL) {   /* oversize: published as empty, the caller repeats the sort */
-        if (threadIdx.x == 0) atomicOr(err, SORT_ERR_OVERSIZE);
-        m = 0;
+    } else {
+        m = n1;
+        if (threadIdx.x < SS_NX) L.segc[threadIdx.x] = threadIdx.x ? 0u : (uint32_t)n1;
     }
-    const int nb = (m > 1 && mx > mn) ? 64 - __clzll((long long)(mx - mn)) : 0;
-    int ms = 0;
-    int64_t fb = wide ? ss_sort_bucket<uint64_t, SS_KS64>(s, (int)m, mn, nb, L, &ms)
-                      : ss_sort_bucket<uint32_t, SS_KS32>(s, (int)m, mn, nb, L, &ms);
-    if (fb < 0) {   /* more distinct values than one block sorts: oversize */
-        if (threadIdx.x == 0) atomicOr(err, SORT_ERR_OVERSIZE);
-        if (lane == 0) L.wc[w] = 0;
-        fb = 0;
-        ms = 0;
+    uint64_t base = 0;
+    bool wide = false, bounded = false;
+    if (B > 1 && t > 0 && t + 1 < (size_t)B) {
+        const uint64_t lo = spl[t - 1], up = spl[t];
+        bounded = up > lo && up - lo - 1 < 0xffffffffULL;
+        base = lo;
     }
-    const uint32_t fbits = (uint32_t)fb;
     __syncthreads();
-    if (w == 0) {
-        uint32_t tot = 0;
-#pragma unroll
-        for (int ww = 0; ww < SS_NW; ww++) tot += L.wc[ww];
-        uint64_t excl = 0;
-        if (B > 1) {
-            uint64_t *my = status + t;
-            if (t == 0) {
-                if (lane == 0) st_flag(my, OS_INC, epoch, tot);
-            } else {
-                if (lane == 0) st_flag(my, OS_AGG, epoch, tot);
-                long tt = (long)t - 1;
-                while (tt >= 0) {
-                    const long ti = tt - lane;
-                    const uint64_t v = ti >= 0 ? ld_flag(status + ti) : 0;
-                    const uint32_t f = ti >= 0 ? fstate(v, epoch) : 2u;
-                    const uint64_t notready = __ballot(f == 0), inc = __ballot(f == 2);
-                    const int nr = notready ? __ffsll((long long)notready) - 1 : 64;
-                    const int ni = inc ? __ffsll((long long)inc) - 1 : 64;
-                    const int stop = nr < ni ? nr : ni;
-                    uint64_t part = (ti >= 0 && (lane < stop || (lane == ni && ni < nr))) ? (v & OS_VAL) : 0;
-#pragma unroll
-                    for (int dd = 32; dd >= 1; dd >>= 1) part += __shfl_xor(part, dd, 64);
-                    excl += part;
-                    if (ni < nr) break;
-                    if (nr < 64) {
-                        uint64_t wv = 0;
-                        if (lane == 0) wv = wait_flag(status + (tt - nr), epoch, err);
-                        wv = __shfl(wv, 0, 64);
-                        excl += wv & OS_VAL;
-                        if (fstate(wv, epoch) == 2) break;
-                        tt = tt - nr - 1;
-                    } else {
-                        tt -= 64;
-                    }
+    bool over = m > (size_t)cap;
+    if (!bounded) {
+        uint64_t a = ~0ULL, b = 0;
+        if (!over)
+            for (int r = 0; r < SS_NX; r++)
+                for (int i = threadIdx.x; i < (int)L.segc[r]; i += SS_T) {
+                    const uint64_t k = s[(size_t)r * SS_SUBCAP + i];
+                    a = k < a ? k : a;
+                    b = k > b ? k : b;
                 }
-                if (lane == 0) st_flag(my, OS_INC, epoch, excl + tot);
+        uint64_t mx;
+        ss_min_max(a, b, L, &base, &mx);
+        /* 32-bit values stay below 0xFFFFFFFF (the hash set's empty mark);
+         * a range of 2^64 - 1 would make the largest key's value the empty
+         * mark of the 64-bit set */
+        wide = m > 1 && mx > base && (mx - base) >= 0xffffffffULL;
+        over |= m > 1 && mx > base && (mx - base) == ~0ULL;
+    }
+    if (over) {   /* oversize: published as empty */
+        if (threadIdx.x == 0) atomicOr(err, SORT_ERR_OVERSIZE);
+        __syncthreads();   /* every thread has read segc */
+        if (threadIdx.x < SS_NX) L.segc[threadIdx.x] = 0;
+        __syncthreads();
+    }
+    uint64_t vmin = 0;
+    int nb = 0;
+    int n = wide ? ss_dedupe<uint64_t, SS_KS64>(s, base, L, &vmin, &nb)
+                 : ss_dedupe<uint32_t, SS_KS32>(s, base, L, &vmin, &nb);
+    if (n < 0) {   /* more distinct values than one block sorts: oversize */
+        if (threadIdx.x == 0) atomicOr(err, SORT_ERR_OVERSIZE);
+        n = 0;
+    }
+    uint64_t *my = status + t;
+    if (B > 1 && threadIdx.x == 0) st_flag(my, t == 0 ? OS_INC : OS_AGG, epoch, (uint64_t)n);
+    if (wide) ss_sort_distinct<uint64_t, SS_KS64>(n, nb, L);
+    else ss_sort_distinct<uint32_t, SS_KS32>(n, nb, L);
+    if (w == 0) {
+        uint64_t excl = 0;
+        if (B > 1 && t > 0) {
+            long tt = (long)t - 1;
+            while (tt >= 0) {
+                const long ti = tt - lane;
+                const uint64_t v = ti >= 0 ? ld_flag(status + ti) : 0;
+                const uint32_t f = ti >= 0 ? fstate(v, epoch) : 2u;
+                const uint64_t notready = __ballot(f == 0), inc = __ballot(f == 2);
+                const int nr = notready ? __ffsll((long long)notready) - 1 : 64;
+                const int ni = inc ? __ffsll((long long)inc) - 1 : 64;
+                const int stop = nr < ni ? nr : ni;
+                uint64_t part = (ti >= 0 && (lane < stop || (lane == ni && ni < nr))) ? (v & OS_VAL) : 0;
+#pragma unroll
+                for (int dd = 32; dd >= 1; dd >>= 1) part += __shfl_xor(part, dd, 64);
+                excl += part;
+                if (ni < nr) break;
+                if (nr < 64) {
+                    uint64_t wv = 0;
+                    if (lane == 0) wv = wait_flag(status + (tt - nr), epoch, err);
+                    wv = __shfl(wv, 0, 64);
+                    excl += wv & OS_VAL;
+                    if (fstate(wv, epoch) == 2) break;
+                    tt = tt - nr - 1;
+                } else {
+                    tt -= 64;
+                }
             }
+            if (lane == 0) st_flag(my, OS_INC, epoch, excl + (uint64_t)n);
         }
         if (lane == 0) {
             L.excl = excl;
-            if (t + 1 == (size_t)B) *total = excl + tot;
+            if (t + 1 == (size_t)B) *total = excl + (uint64_t)n;
         }
     }
     __syncthreads();
-    if (wide) ss_emit<uint64_t, SS_KS64>(fbits, ms, mn, L.excl, dst, L);
-    else ss_emit<uint32_t, SS_KS32>(fbits, ms, mn, L.excl, dst, L);
+    /* the sorted distinct values, in order, at the bucket's place */
+    uint64_t *o = dst + L.excl;
+    const uint64_t add = base + vmin;
+    if (wide)
+        for (int i = threadIdx.x; i < n; i += SS_T) o[i] = add + L.stage.v64[i];
+    else
+        for (int i = threadIdx.x; i < n; i += SS_T) o[i] = add + (uint64_t)L.stage.v32[i];
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1234,18 +1311,24 @@ struct OsScratch {
     uint64_t *cval;       /* the keys' constant bits */
     unsigned long long *ghist;   /* OS_MAXPASS x OS_MAXR */
     uint64_t *gbase;      /* OS_MAXPASS x OS_MAXR */
-    /* sample sort: bucket ticket, then the bucket counts (cleared together),
-     * then the splitters */
+    /* sample sort: bucket ticket, then the counts of the buckets' regions
+     * (cleared together), the splitters, their guide and its parameters */
     unsigned *ss_ticket;
     unsigned long long *ss_cnt;
     uint64_t *ss_spl;
+    uint16_t *ss_guide;
+    uint64_t *ss_gparm;
 };
 
 constexpr size_t OS_CLEAR_BYTES = 256 + (size_t)OS_MAXPASS * OS_MAXR * 8;
 constexpr size_t SS_OFF = OS_CLEAR_BYTES + (size_t)OS_MAXPASS * OS_MAXR * 8;
-constexpr size_t SS_CLEAR_BYTES = 256 + (size_t)SS_BMAX * 8;
+constexpr size_t SS_CLEAR_BYTES = 256 + (size_t)SS_BMAX * SS_NX * 8;
+constexpr size_t SS_GUIDE_BYTES = ((size_t)(SS_GUIDE + 1) * 2 + 255) & ~(size_t)255;
 
-size_t sort_scratch_bytes(size_t) { return SS_OFF + SS_CLEAR_BYTES + (size_t)SS_BMAX * 8 + 256; }
+size_t sort_scratch_bytes(size_t)
+{
+    return SS_OFF + SS_CLEAR_BYTES + (size_t)SS_BMAX * 8 + SS_GUIDE_BYTES + 256;
+}
 
 static OsScratch carve(void *scratch)
 {
@@ -1258,6 +1341,8 @@ static OsScratch carve(void *scratch)
     s.ss_ticket = (unsigned *)(p + SS_OFF);
     s.ss_cnt = (unsigned long long *)(p + SS_OFF + 256);
     s.ss_spl = (uint64_t *)(p + SS_OFF + SS_CLEAR_BYTES);
+    s.ss_guide = (uint16_t *)(p + SS_OFF + SS_CLEAR_BYTES + (size_t)SS_BMAX * 8);
+    s.ss_gparm = (uint64_t *)(p + SS_OFF + SS_CLEAR_BYTES + (size_t)SS_BMAX * 8 + SS_GUIDE_BYTES);
     return s;
 }
 
@@ -1299,7 +1384,7 @@ static size_t ss_buckets(size_t n)
 size_t sort_part_keys(size_t n)
 {
     size_t B = ss_buckets(n);
-    return B > 1 ? B * (size_t)SS_LCAP : 0;
+    return B > 1 ? B * (size_t)SS_BSTRIDE : 0;
 }
 
 static void launch_hist_scan(hipStream_t st, int bits, const uint64_t *keys, size_t n, const SortCode &c,
@@ -1385,7 +1470,7 @@ static void launch_pass(hipStream_t st, SortCfg c, unsigned tiles, const uint64_
     else launch_pass_t<false>(st, c, tiles, src, dst, n, x, shift, gbase, status, ctr, epoch, err);
 }
 
-/* The sample sort of n keys into dst (B buckets, part holding B * SS_LCAP
+/* The sample sort of n keys into dst (B buckets, part holding B * SS_BSTRIDE
  * keys when B > 1); keys are only read. */
 static void ss_sort_unique(hipStream_t st, const uint64_t *keys, size_t n, size_t B, uint64_t *part,
                            const OsScratch &s, SortCtx &sc, uint64_t *dst, unsigned long long *d_nuniq,
@@ -1394,26 +1479,28 @@ static void ss_sort_unique(hipStream_t st, const uint64_t *keys, size_t n, size_
     g_ss_sorts++;
     const uint32_t epoch = next_epoch(st, sc);
     if (B > 1) {
-        launch_fill_u64(st, (uint64_t *)s.ss_ticket, SS_CLEAR_BYTES / 8, 0);
+        launch_fill_u64(st, (uint64_t *)s.ss_ticket, (256 + B * SS_NX * 8) / 8, 0);
         int h = prof_hook_begin(8, st);
         /* the sample goes to the start of part, which the partition
          * overwrites only after the splitters are made */
         hipLaunchKernelGGL(k_ss_gather, dim3(SS_S / 256), dim3(256), 0, st, keys, n, part);
         hipLaunchKernelGGL(k_ss_sample, dim3(1), dim3(SS_T), 0, st, (const uint64_t *)part, (int)B,
-                           s.ss_spl);
+                           s.ss_spl, s.ss_guide, s.ss_gparm);
         snote(hipGetLastError());
         prof_hook_end(h, 8.0 * SS_S + 8.0 * (B - 1), st);
         h = prof_hook_begin(9, st);
         const size_t tiles = (n + SS_TILE - 1) / SS_TILE;
         hipLaunchKernelGGL(k_ss_part, dim3((unsigned)tiles), dim3(SS_T), 0, st, keys, n,
-                           (const uint64_t *)s.ss_spl, (int)B, part, s.ss_cnt, d_err, g_ss_cap);
+                           (const uint64_t *)s.ss_spl, (const uint16_t *)s.ss_guide,
+                           (const uint64_t *)s.ss_gparm, (int)B, part, s.ss_cnt, d_err);
         snote(hipGetLastError());
         prof_hook_end(h, 16.0 * n, st);
     }
     *prof_unique = prof_hook_begin(10, st);
     hipLaunchKernelGGL(k_ss_local, dim3((unsigned)B), dim3(SS_T), 0, st, B > 1 ? part : keys,
-                       (size_t)SS_LCAP, (const unsigned long long *)s.ss_cnt, n, (int)B, dst, sc.status,
-                       s.ss_ticket, epoch, d_nuniq, d_err, g_ss_cap);
+                       (const unsigned long long *)s.ss_cnt, (const uint64_t *)s.ss_spl, n, (int)B, dst,
+                       sc.status, s.ss_ticket, epoch,
+                       d_nuniq, d_err, g_ss_cap);
     snote(hipGetLastError());
     prof_hook_end(*prof_unique, 8.0 * n, st);
 }
@@ -1431,7 +1518,7 @@ uint64_t *sort_unique(hipStream_t st, uint64_t *keys, uint64_t *tmp, size_t n, u
     }
     OsScratch s = carve(scratch);
     const size_t B = allow_sample ? ss_buckets(n) : 0;
-    if (B == 1 || (B > 1 && part && part_cap >= B * (size_t)SS_LCAP)) {
+    if (B == 1 || (B > 1 && part && part_cap >= B * (size_t)SS_BSTRIDE)) {
         if (!status_reserve(st, sc, B)) return keys;
         ss_sort_unique(st, keys, n, B, part, s, sc, dst, d_nuniq, d_err, prof_unique);
         return keys;
