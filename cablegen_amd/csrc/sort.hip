@@ -540,6 +540,7 @@ constexpr int SS_BMAX = 2048;               /* buckets */
 constexpr int SS_NX = 8;
 constexpr int SS_SUBCAP = SS_LCAP;
 constexpr int SS_BSTRIDE = SS_NX * SS_SUBCAP;   /* keys of a bucket's regions */
+static_assert((uint64_t)SS_BMAX * SS_BSTRIDE < 0xffffffffULL, "places in part are 32-bit");
 /* splitter guide: the number of splitters below smin + (p << gshift) for
  * p = 0..SS_GUIDE, which narrows a key's splitter search to a few entries */
 constexpr int SS_GUIDE = 4096;
@@ -699,9 +700,11 @@ __global__ __launch_bounds__(256) void k_ss_gather(const uint64_t *__restrict__ 
 }
 
 /* B - 1 splitters from a regular sample of SS_S keys (n > SS_S).  The
- * sample is sorted on the top 32 bits of its range (key - min) >> z, and a
- * splitter is min + (value << z): monotone, which is all a splitter needs
- * (ties in the top bits only move a bucket boundary). */
+ * sample is sorted on the top 32 bits of its range, (key - min) >> z, and a
+ * splitter is min + (value << z): monotone, which is all a splitter needs.
+ * Ties in those bits only move a bucket boundary to a multiple of 2^z.
+ * Fewer bits would not do: a layer's keys fill a tiny part of their range,
+ * and one 2^24-th of it can hold more keys than a bucket. */
 __global__ __launch_bounds__(SS_T) void k_ss_sample(const uint64_t *__restrict__ smp, int B,
                                                     uint64_t *__restrict__ spl, uint16_t *__restrict__ guide,
                                                     uint64_t *__restrict__ gparm)
@@ -763,26 +766,25 @@ __global__ __launch_bounds__(SS_T) void k_ss_sample(const uint64_t *__restrict__
 
 struct SsPartLds {
     uint64_t spl[SS_BMAX];
-    uint64_t wbase[SS_BMAX];
-    uint64_t stage[SS_TILE];
-    uint32_t cnt[SS_BMAX];        /* counts, then the tile's start of each bucket */
-    uint16_t bid[SS_TILE];
+    uint32_t cnt[SS_BMAX];        /* counts, then each bucket's place in part (~0u: region full) */
     uint16_t guide[SS_GUIDE + 1];
-    uint32_t wtmp[17];
 };
 
 /* Bucket of every key (the number of splitters <= key, searched between the
- * guide's bounds), ranked per bucket in LDS; one reservation per (tile,
- * bucket) on the bucket's counter of this block's region group; the tile
- * staged in bucket order and written to part[b * SS_BSTRIDE + group *
- * SS_SUBCAP + position].  A region that fills up is not written past its
- * end (the bucket is then oversize). */
-__global__ __launch_bounds__(SS_T) void k_ss_part(const uint64_t *__restrict__ keys, size_t n,
-                                                  const uint64_t *__restrict__ spl_g,
-                                                  const uint16_t *__restrict__ guide_g,
-                                                  const uint64_t *__restrict__ gparm, int B,
-                                                  uint64_t *__restrict__ part,
-                                                  unsigned long long *gcur, unsigned *err)
+ * guide's bounds) and its rank among the tile's keys of that bucket (an LDS
+ * atomic); one reservation per (tile, bucket) on the bucket's counter of
+ * this block's region group; every key written from registers to part[b *
+ * SS_BSTRIDE + group * SS_SUBCAP + reservation + rank].  Without a staging
+ * tile the LDS holds only the splitters, guide and counters, so two blocks
+ * share a CU; a tile's run of one bucket is a few keys either way.  A tile
+ * whose reservation would pass the region's end writes none of that
+ * bucket's keys (the bucket is then oversize). */
+__global__ __launch_bounds__(SS_T, 2) void k_ss_part(const uint64_t *__restrict__ keys, size_t n,
+                                                     const uint64_t *__restrict__ spl_g,
+                                                     const uint16_t *__restrict__ guide_g,
+                                                     const uint64_t *__restrict__ gparm, int B,
+                                                     uint64_t *__restrict__ part,
+                                                     unsigned long long *gcur, unsigned *err)
 {
     __shared__ SsPartLds P;
     const int nspl = B - 1;
@@ -817,41 +819,26 @@ __global__ __launch_bounds__(SS_T) void k_ss_part(const uint64_t *__restrict__ k
         }
     }
     __syncthreads();
-    /* two buckets per thread: the tile's start of each, and the reservation */
-    const int b0 = 2 * threadIdx.x, b1 = b0 + 1;
+    /* two buckets per thread (B <= 2 SS_T): the reservations */
+    static_assert(SS_BMAX <= 2 * SS_T, "two buckets per thread");
+    const int b0 = threadIdx.x, b1 = threadIdx.x + SS_T;
     const uint32_t c0 = b0 < B ? P.cnt[b0] : 0, c1 = b1 < B ? P.cnt[b1] : 0;
-    uint32_t all;
-    const uint32_t ex = block_excl_scan_dpp(c0 + c1, &all, P.wtmp);
-    bool over = false;
-    if (c0) {
-        const unsigned long long g = atomicAdd(&gcur[b0 * SS_NX + grp], (unsigned long long)c0);
-        over |= g + c0 > (unsigned long long)SS_SUBCAP;
-        P.wbase[b0] = (uint64_t)b0 * SS_BSTRIDE + (uint64_t)grp * SS_SUBCAP + g - ex;
-    }
-    if (c1) {
-        const unsigned long long g = atomicAdd(&gcur[b1 * SS_NX + grp], (unsigned long long)c1);
-        over |= g + c1 > (unsigned long long)SS_SUBCAP;
-        P.wbase[b1] = (uint64_t)b1 * SS_BSTRIDE + (uint64_t)grp * SS_SUBCAP + g - (ex + c0);
-    }
-    if (over) atomicOr(err, SORT_ERR_OVERSIZE);
-    if (b0 < B) P.cnt[b0] = ex;
-    if (b1 < B) P.cnt[b1] = ex + c0;
+    unsigned long long g0 = 0, g1 = 0;
+    if (c0) g0 = atomicAdd(&gcur[b0 * SS_NX + grp], (unsigned long long)c0);
+    if (c1) g1 = atomicAdd(&gcur[b1 * SS_NX + grp], (unsigned long long)c1);
+    const bool o0 = c0 && g0 + c0 > (unsigned long long)SS_SUBCAP;
+    const bool o1 = c1 && g1 + c1 > (unsigned long long)SS_SUBCAP;
+    if (o0 || o1) atomicOr(err, SORT_ERR_OVERSIZE);
+    if (c0) P.cnt[b0] = o0 ? ~0u : (uint32_t)((uint64_t)b0 * SS_BSTRIDE + (uint64_t)grp * SS_SUBCAP + g0);
+    if (c1) P.cnt[b1] = o1 ? ~0u : (uint32_t)((uint64_t)b1 * SS_BSTRIDE + (uint64_t)grp * SS_SUBCAP + g1);
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < SS_PK; j++) {
         const size_t i = tile0 + (size_t)j * SS_T + threadIdx.x;
         if (i < n) {
-            const uint32_t b = br[j] >> 16, p = P.cnt[b] + (br[j] & 0xffffu);
-            P.stage[p] = k[j];
-            P.bid[p] = (uint16_t)b;
+            const uint32_t base = P.cnt[br[j] >> 16];
+            if (base != ~0u) part[base + (br[j] & 0xffffu)] = k[j];
         }
-    }
-    __syncthreads();
-    const uint32_t tn = (uint32_t)((n - tile0) < (size_t)SS_TILE ? (n - tile0) : (size_t)SS_TILE);
-    for (uint32_t p = threadIdx.x; p < tn; p += SS_T) {
-        const uint32_t b = P.bid[p];
-        const uint64_t o = P.wbase[b] + p;
-        if (o < (uint64_t)b * SS_BSTRIDE + (uint64_t)(grp + 1) * SS_SUBCAP) part[o] = P.stage[p];
     }
 }
 
