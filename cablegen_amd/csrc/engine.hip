@@ -16,6 +16,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <dlfcn.h>
+#include <mutex>
 #include <string>
 #include <vector>
 #include "board.h"
@@ -1247,6 +1248,82 @@ extern "C" int cg_dev_lookup(const uint64_t *d_keys, const double *d_vals, size_
     launch_lookup(E.st, t, d_q, nq, d_out, canonicalize);
     prof_end(h, 16.0 * nq);
     return check_sync(E.st, "cg_dev_lookup");
+}
+
+/* ------------------------------------------------------------------------ */
+/* lookup without allocation                                                 */
+/* ------------------------------------------------------------------------ */
+/* cg_dev_lookup builds a temporary directory (hipMalloc) and its count and
+ * scan buffers in the engine's workspace.  The variant below keeps all of it
+ * in one file-scope device array, so it calls no allocator: the layer's OR /
+ * AND counters (64 bytes), a directory of at most 2^NA_DIR_BITS buckets,
+ * and the directory build's counts and scan sums.  The array lives on every
+ * GPU the library runs on until the process ends; its address on the
+ * current device is taken once per device with hipGetSymbolAddress (the
+ * array's name is a host-side stand-in, never a device address).  Work goes
+ * to the null stream; calls that overlap on one GPU share the array and
+ * race. */
+constexpr int NA_DIR_BITS = 20;
+constexpr size_t NA_DIR_ENTRIES = ((size_t)1 << NA_DIR_BITS) + 1;
+constexpr size_t NA_STATS_BYTES = 256;
+constexpr size_t NA_DIR_BYTES = (NA_DIR_ENTRIES * 4 + 255) & ~(size_t)255;
+/* the build's counts (one word per bucket, rounded up) and per-tile sums:
+ * at most two words per bucket plus slack, whatever the scan's tile */
+constexpr size_t NA_BUILD_BYTES = (NA_DIR_ENTRIES * 8 + 4096 + 255) & ~(size_t)255;
+constexpr size_t NA_SCRATCH_BYTES = NA_STATS_BYTES + NA_DIR_BYTES + NA_BUILD_BYTES;
+
+static __device__ __align__(256) unsigned char scratch[NA_SCRATCH_BYTES];
+
+static std::mutex g_na_mu;
+static void *g_na_ptr[64];   /* scratch's address per device */
+
+/* the array's device address on the current device */
+static int na_scratch(unsigned char **p)
+{
+    int dev = 0;
+    HC(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return fail_msg("cg_dev_lookup_noalloc: device ordinal out of range");
+    std::lock_guard<std::mutex> g(g_na_mu);
+    if (!g_na_ptr[dev]) {
+        void *q = nullptr;
+        HC(hipGetSymbolAddress(&q, HIP_SYMBOL(scratch)));
+        g_na_ptr[dev] = q;
+    }
+    *p = (unsigned char *)g_na_ptr[dev];
+    return 0;
+}
+
+extern "C" int cg_dev_lookup_noalloc(const uint64_t *d_keys, const double *d_vals, size_t n,
+                                     const uint64_t *d_q, size_t nq, double *d_out, int canonicalize)
+{
+    if (check_layer_size(n)) return -1;
+    unsigned char *base = nullptr;
+    if (na_scratch(&base)) return -1;
+    unsigned long long *stats = (unsigned long long *)base;
+    uint32_t *dir = (uint32_t *)(base + NA_STATS_BYTES);
+    void *build = base + NA_STATS_BYTES + NA_DIR_BYTES;
+    const hipStream_t st = 0;
+    (void)kernels_take_error();
+    uint64_t o = 0, a = ~0ULL;
+    if (n) {
+        launch_init_stats(st, stats);
+        launch_or_and(st, d_keys, n, stats);
+        unsigned long long h[3];
+        HC(hipMemcpy(h, stats, sizeof h, hipMemcpyDeviceToHost));
+        o = h[1];
+        a = h[2];
+    }
+    LayerIndexPlan plan = plan_index(n, o, a);
+    if (plan.bits > NA_DIR_BITS) {   /* a coarser directory: longer searches, the same results */
+        plan.bits = NA_DIR_BITS;
+        plan.shift = plan.x.nbits - NA_DIR_BITS;
+        plan.dir_entries = NA_DIR_ENTRIES;
+    }
+    if (plan.dir_entries > NA_DIR_ENTRIES || dir_scratch_bytes(plan.dir_entries) > NA_BUILD_BYTES)
+        return fail_msg("cg_dev_lookup_noalloc: the directory exceeds the fixed scratch array");
+    launch_build_dir(st, d_keys, n, plan, dir, build);
+    launch_lookup(st, make_view(d_keys, d_vals, n, dir, plan), d_q, nq, d_out, canonicalize);
+    return check_sync(st, "cg_dev_lookup_noalloc");
 }
 
 extern "C" int cg_dev_solve_layer(const cg_rules *rules, const uint64_t *d_M, size_t n, double *d_V,

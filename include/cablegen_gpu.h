@@ -144,6 +144,19 @@ int cg_layer_index_probe(size_t n, uint64_t key_or, uint64_t key_and, int *bits)
  * misses give 0 (lookup, src/solve.c:108-164). */
 int cg_dev_lookup(const uint64_t *d_keys, const double *d_vals, size_t n, const uint64_t *d_q,
                   size_t nq, double *d_out, int canonicalize);
+/* cg_dev_lookup without allocation.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * directory and its build buffers live in one fixed device array of about
+ * 12.6 MB per GPU (held until the process ends), with a directory of at most
+ * 2^20 buckets: a layer above ~2^18 keys gets a coarser directory than
+ * cg_dev_lookup's (longer searches inside a bucket) and the same results.
+ * n >= 2^32 fails with a message, as in cg_dev_lookup.  Runs on the null
+ * stream and returns after it has synchronised (one 24-byte read-back of
+ * the layer's OR / AND).  Calls that overlap on one GPU share the array and
+ * race. */
+int cg_dev_lookup_noalloc(const uint64_t *d_keys, const double *d_vals, size_t n, const uint64_t *d_q,
+                          size_t nq, double *d_out, int canonicalize);
 /* solve one layer against two solved successor layers (device pointers);
  * win_images are the already-expanded winstates (host). */
 int cg_dev_solve_layer(const cg_rules *rules, const uint64_t *d_M, size_t n, double *d_V,
