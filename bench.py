@@ -403,11 +403,13 @@ def run_autotune(cg, args):
         cg.tune_set("cell_code", args.cell_code)
         cg.tune_set("carry_stream", args.carry_stream)
         cg.tune_set("sample_sort", args.sample_sort)
+        cg.tune_set("canon_stab", args.canon_stab)
         if args.sample_bucket:
             cg.tune_set("sample_bucket", args.sample_bucket)
         return {"source": "built-in defaults: 8-bit digits, 8192-key tiles, window 16, dir delta 0",
                 "cell_code": args.cell_code, "carry_stream": args.carry_stream,
-                "sample_sort": args.sample_sort, "sample_bucket": cg.tune_get("sample_bucket")}
+                "sample_sort": args.sample_sort, "sample_bucket": cg.tune_get("sample_bucket"),
+                "canon_stab": args.canon_stab}
     if mode == "stored" and os.path.exists(cg.TUNED_TABLE):
         rep = cg.load_tuning()
         src = "stored autotuner table cablegen_amd/tuned/mi355x.json"
@@ -420,6 +422,7 @@ def run_autotune(cg, args):
     cg.tune_set("cell_code", args.cell_code)
     cg.tune_set("carry_stream", args.carry_stream)
     cg.tune_set("sample_sort", args.sample_sort)
+    cg.tune_set("canon_stab", args.canon_stab)
     if args.sample_bucket:
         cg.tune_set("sample_bucket", args.sample_bucket)
     dir_delta = rep["dir"]["pick"]
@@ -432,7 +435,7 @@ def run_autotune(cg, args):
     return {"source": src, "seconds": time.perf_counter() - t0, "dir_delta": dir_delta,
             "sort_picks_bits_x_kpt/window": sel, "cell_code": args.cell_code,
             "carry_stream": args.carry_stream, "sample_sort": args.sample_sort,
-            "sample_bucket": cg.tune_get("sample_bucket")}
+            "sample_bucket": cg.tune_get("sample_bucket"), "canon_stab": args.canon_stab}
 
 
 def census(args):
@@ -710,6 +713,8 @@ def main():
                     help="mean keys per bucket of the sample sort (0: the library's default)")
     ap.add_argument("--cell-code", type=int, choices=(0, 1), default=1,
                     help="1: layer sorts code pairs of cells when the tile sum bounds them (default)")
+    ap.add_argument("--canon-stab", type=int, choices=(0, 1), default=1,
+                    help="1: canonical forms through the formation's stabiliser (default); 0: max of 8 images")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "

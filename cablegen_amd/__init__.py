@@ -90,6 +90,8 @@ def lib():
     L.cg_sort_code_probe.argtypes = [C.c_void_p, C.c_size_t, C.c_uint64, C.c_uint64, C.c_void_p,
                                      C.c_void_p]
     L.cg_layer_index_probe.argtypes = [C.c_size_t, C.c_uint64, C.c_uint64, C.POINTER(C.c_int)]
+    L.cg_canon_sel_probe.argtypes = [C.c_uint64, C.c_int, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p,
+                                     C.c_void_p]
     L.cg_dev_lookup.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t,
                                 C.c_void_p, C.c_int]
     L.cg_dev_lookup_noalloc.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t,

@@ -105,6 +105,101 @@ CG_HD uint64_t cg_canon(uint64_t b)
     return m;
 }
 
+/* Image i of b in cg_canon's order: bit 0 turns it clockwise, then bit 1
+ * mirrors the rows, then bit 2 reverses the row order (0 b, 1 cw b, 2
+ * mirror b, 3 mirror cw b, 4 flip b, 5 flip cw b, 6 flip mirror b, 7 flip
+ * mirror cw b). */
+CG_HD uint64_t cg_image(uint64_t b, int i)
+{
+    uint64_t x = (i & 1) ? cg_rot_cw(b) : b;
+    if (i & 2) x = cg_mirror_rows(x);
+    if (i & 4) x = cg_flip_rows(x);
+    return x;
+}
+
+/* The max over the images whose bit is set in sel (0xff: cg_canon). */
+CG_HD uint64_t cg_canon_sel(uint64_t b, unsigned sel)
+{
+    if (sel == 0xffu) return cg_canon(b);
+    const uint64_t y = (sel & 0xaau) ? cg_rot_cw(b) : 0;
+    const uint64_t mb = (sel & 0x44u) ? cg_mirror_rows(b) : 0;
+    const uint64_t my = (sel & 0x88u) ? cg_mirror_rows(y) : 0;
+    uint64_t m = 0;
+    if (sel & 0x01u) m = cg_umax(m, b);
+    if (sel & 0x02u) m = cg_umax(m, y);
+    if (sel & 0x04u) m = cg_umax(m, mb);
+    if (sel & 0x08u) m = cg_umax(m, my);
+    if (sel & 0x10u) m = cg_umax(m, cg_flip_rows(b));
+    if (sel & 0x20u) m = cg_umax(m, cg_flip_rows(y));
+    if (sel & 0x40u) m = cg_umax(m, cg_flip_rows(mb));
+    if (sel & 0x80u) m = cg_umax(m, cg_flip_rows(my));
+    return m;
+}
+
+/* Canonical form through a formation's stabiliser.  Let F be the 0xF cells
+ * of a board whose other cells are all below 0xF, and A the canonical place
+ * of F (the largest of its images, 0xF being the largest nibble).  If every
+ * image P != A of F, read from the top nibble, first differs from A at a
+ * cell where A is 0xF after cells that are all 0xF in A, then an image of
+ * the board with its formation at P is below one with its formation at A:
+ * equal nibbles up to that cell, where one holds 0xF and the other a free
+ * cell.  The canonical form is then the max over the images that map F to
+ * A only -- one image for DPDGap, two for the 2x4 formations.
+ *   f[o]:   the fmask of the boards of orientation o (0: as the layer holds
+ *           them; 1: turned clockwise, as an up or down move leaves them)
+ *   sel[o]: the images of those boards that put the formation at A
+ * A board whose fmask is not f[o] takes the full max of 8 images, so the
+ * result is cg_canon's for every board. */
+typedef struct {
+    uint64_t f[2];
+    uint32_t sel[2];
+} cg_canon_sel_t;
+
+CG_HD uint64_t cg_canon_fast(uint64_t b, cg_canon_sel_t cs, int o)
+{
+    return cg_fmask(b) == cs.f[o] ? cg_canon_sel(b, cs.sel[o]) : cg_canon(b);
+}
+
+/* every board takes the full max of 8 images */
+CG_HD cg_canon_sel_t cg_canon_all(void)
+{
+    cg_canon_sel_t cs;
+    cs.f[0] = cs.f[1] = ~0ULL;
+    cs.sel[0] = cs.sel[1] = 0xffu;
+    return cs;
+}
+
+/* the selection for boards whose 0xF cells are those of e, in e's
+ * orientation; with free_formation (formation cells move) or a formation
+ * whose images are not decided by it, f never matches and every board takes
+ * the full max */
+CG_HD cg_canon_sel_t cg_make_canon_sel(uint64_t e, int ff)
+{
+    cg_canon_sel_t cs = cg_canon_all();
+    if (ff) return cs;
+    const uint64_t fe = cg_fmask(e) * 0xfULL;   /* e's formation cells as 0xF, the rest 0 */
+    const uint64_t A = cg_canon(fe);
+    for (int i = 0; i < 8; i++) {
+        const uint64_t P = cg_image(A, i);
+        for (int x = 0; x < 16 && P != A; x++) {
+            const unsigned a = (unsigned)(A >> (60 - 4 * x)) & 0xfu, p = (unsigned)(P >> (60 - 4 * x)) & 0xfu;
+            if (a != 0xfu) return cs;   /* a free cell before the first difference */
+            if (p != 0xfu) break;      /* decided: A holds 0xF, P a free cell */
+        }
+    }
+    const uint64_t fr = cg_rot_cw(fe);
+    uint32_t s0 = 0, s1 = 0;
+    for (int i = 0; i < 8; i++) {
+        if (cg_image(fe, i) == A) s0 |= 1u << i;
+        if (cg_image(fr, i) == A) s1 |= 1u << i;
+    }
+    cs.f[0] = cg_fmask(fe);
+    cs.f[1] = cg_fmask(fr);
+    cs.sel[0] = s0;
+    cs.sel[1] = s1;
+    return cs;
+}
+
 /* One horizontal board move using a left-move row table `L` (65536 u16).
  * dir 0 = left, 1 = right.  Returns 1 iff valid (movedir_hori
  * board.c:151-169): some row changed and every row is locked -- the lock rule

@@ -448,6 +448,16 @@ struct Arena {
 /* the generation step                                                       */
 /* ------------------------------------------------------------------------ */
 
+/* canonical forms through the formation's stabiliser (knob canon_stab) */
+static int g_canon_stab = 1;
+
+/* the selection for a table whose boards have the formation of the
+ * canonical board e */
+static cg_canon_sel_t canon_sel_of(uint64_t e, const cg_rules &r)
+{
+    return g_canon_stab ? cg_make_canon_sel(e, r.free_formation) : cg_canon_all();
+}
+
 static MoveRules move_rules(const cg_rules &r)
 {
     MoveRules m;
@@ -456,6 +466,7 @@ static MoveRules move_rules(const cg_rules &r)
     m.stsl = r.stsl;
     m.ltc = r.ltc;
     m.sl = r.smallest_large;
+    m.cs = cg_canon_all();
     return m;
 }
 
@@ -656,7 +667,7 @@ static uint64_t cell_bound(const ValueBound *v, long s, uint64_t key_and)
 static int gen_step(const cg_rules &r, const uint64_t *P, size_t nP, uint64_t *C, size_t nC,
                     StepCounts &pc, LayerAlloc &la, uint64_t **M, size_t *nM,
                     Buf &nxtP, size_t *nP2, Buf &nxtC, size_t *nC2, ValueBound *vb,
-                    uint64_t *m_or = nullptr, uint64_t *m_and = nullptr)
+                    uint64_t *m_or = nullptr, uint64_t *m_and = nullptr, cg_canon_sel_t cs = cg_canon_all())
 {
     const long s = vb ? vb->s : 0;
     const uint16_t *lut = get_lut(r.ignore_f);
@@ -665,7 +676,9 @@ static int gen_step(const cg_rules &r, const uint64_t *P, size_t nP, uint64_t *C
     if (ensure(E.A, 4 * nP + 1)) return -1;
     launch_init_stats(E.st, E.d_cnt);
     int h = prof_begin(PK_MOVE);
-    launch_move(E.st, P, nP, nullptr, E.A.p, E.d_cnt, lut, move_rules(r), 1);
+    MoveRules mr = move_rules(r);
+    mr.cs = cs;
+    launch_move(E.st, P, nP, nullptr, E.A.p, E.d_cnt, lut, mr, 1);
     prof_end(h, 0);
     /* the previous step's carry sort (second stream) overlapped the moves;
      * its count and output are needed from here on */
@@ -703,7 +716,7 @@ static int gen_step(const cg_rules &r, const uint64_t *P, size_t nP, uint64_t *C
         if (nC) HC(hipMemcpyAsync(nxtP.p, C, nC * sizeof(uint64_t), hipMemcpyDeviceToDevice, E.st));
         launch_init_stats(E.st, E.d_cnt);
         hs = prof_begin(PK_SPAWN);
-        launch_spawn(E.st, *M, ncand, E.d_cnt + 7, nxtP.p + nC, E.Cb.p, cap, E.d_cnt);
+        launch_spawn(E.st, *M, ncand, E.d_cnt + 7, nxtP.p + nC, E.Cb.p, cap, E.d_cnt, cs);
         prof_end(hs, 0);
         if (read_counters(0, 8)) return -1;
         if (E.oversize) {
@@ -863,6 +876,9 @@ struct cg_tablebase {
     /* adjoint workspace, kept between calls: occupation of every board,
      * D per layer + P, the initial boards */
     Buf occ, dD, init;
+    /* the generation's canonical forms through the formation's stabiliser
+     * (from the canonical form of the first initial board) */
+    cg_canon_sel_t cs = cg_canon_all();
 };
 
 static LayerView view_of(const cg_tablebase *tb, int idx)
@@ -934,6 +950,7 @@ extern "C" int cg_tb_generate_bounded(cg_tablebase *tb, const uint64_t *initial,
     tb->have_arg = false;
     if (ensure(tb->P, n + 1)) return -1;
     if (n) HC(hipMemcpy(tb->P.p, initial, n * 8, hipMemcpyHostToDevice));
+    tb->cs = n ? canon_sel_of(cg_canon(initial[0]), tb->r) : cg_canon_all();
     size_t nP = n, nC = 0;
     StepCounts pc;
     ArenaLayerAlloc la(&tb->arena);
@@ -944,7 +961,7 @@ extern "C" int cg_tb_generate_bounded(cg_tablebase *tb, const uint64_t *initial,
         size_t nP2 = 0, nC2 = 0;
         uint64_t mor = 0, mand = ~0ULL;
         if (gen_step(tb->r, tb->P.p, nP, tb->C.p, nC, pc, la, &l.keys, &l.n, tb->P2, &nP2,
-                     tb->C2, &nC2, &vb, &mor, &mand))
+                     tb->C2, &nC2, &vb, &mor, &mand, tb->cs))
             return -1;
         l.plan = plan_index(l.n, mor, mand);
         l.dir = (uint32_t *)tb->arena.alloc(l.plan.dir_entries * 4);
@@ -1211,6 +1228,18 @@ extern "C" int cg_sort_code_probe(const uint64_t *keys, size_t n, uint64_t vary,
     return c.nbits;
 }
 
+extern "C" int cg_canon_sel_probe(uint64_t e, int ff, const uint64_t *b, size_t n, int o, uint64_t *out,
+                                  uint32_t *sel)
+{
+    const cg_canon_sel_t cs = cg_make_canon_sel(e, ff);
+    for (size_t i = 0; i < n; i++) out[i] = cg_canon_fast(b[i], cs, o & 1);
+    if (sel) {
+        sel[0] = cs.sel[0];
+        sel[1] = cs.sel[1];
+    }
+    return 0;
+}
+
 extern "C" int cg_layer_index_probe(size_t n, uint64_t key_or, uint64_t key_and, int *bits)
 {
     if (check_layer_size(n)) return -1;
@@ -1398,6 +1427,11 @@ extern "C" int cg_tune_set(const char *knob, int value)
         g_carry_lane = value;
         return 0;
     }
+    if (!strcmp(knob, "canon_stab")) {
+        if (value != 0 && value != 1) return fail_msg("canon_stab: 0 or 1");
+        g_canon_stab = value;
+        return 0;
+    }
     if (!strcmp(knob, "sample_sort")) {
         if (value != 0 && value != 1) return fail_msg("sample_sort: 0 or 1");
         sort_set_sample(value, sort_sample_bucket(), sort_sample_cap());
@@ -1420,6 +1454,7 @@ extern "C" int cg_tune_set(const char *knob, int value)
         g_spawn_est_mode = 0;
         sort_set_cell_code(1);
         g_carry_lane = 1;
+        g_canon_stab = 1;
         sort_set_sample(1, 5000, 0);
         if (!strcmp(knob, "reset") && !g_tune_path.empty()) return cg_tune_load(g_tune_path.c_str());
         return 0;
@@ -1437,6 +1472,7 @@ extern "C" int cg_tune_get(const char *knob)
     if (!strcmp(knob, "spawn_estimate")) return g_spawn_est_mode;
     if (!strcmp(knob, "cell_code")) return sort_cell_code();
     if (!strcmp(knob, "carry_stream")) return g_carry_lane;
+    if (!strcmp(knob, "canon_stab")) return g_canon_stab;
     if (!strcmp(knob, "cell_code_sorts")) return (int)std::min<unsigned long long>(sort_code_count(), 0x7fffffff);
     if (!strcmp(knob, "sample_sort")) return sort_sample_on();
     if (!strcmp(knob, "sample_bucket")) return sort_sample_bucket();
@@ -1780,6 +1816,7 @@ static struct {
     size_t nP = 0, nC = 0, nM = 0;
     StepCounts pc;
     ValueBound vb;
+    cg_canon_sel_t cs;
 } G;
 
 struct BufLayerAlloc : LayerAlloc {
@@ -1801,6 +1838,7 @@ extern "C" int cge_gen_begin(const cge_rules *r, const uint64_t *initial, size_t
     G.nM = 0;
     G.pc = StepCounts();
     G.vb = value_bound_of(initial, n);
+    G.cs = n ? canon_sel_of(cg_canon(initial[0]), G.r) : cg_canon_all();
     return 0;
 }
 
@@ -1810,7 +1848,7 @@ extern "C" int cge_gen_step(size_t *n_layer)
     size_t nP2 = 0, nC2 = 0;
     BufLayerAlloc la;
     if (gen_step(G.r, G.P.p, G.nP, G.C.p, G.nC, G.pc, la, &M, &G.nM, G.P2, &nP2, G.C2,
-                 &nC2, &G.vb))
+                 &nC2, &G.vb, nullptr, nullptr, G.cs))
         return -1;
     std::swap(G.P, G.P2);
     std::swap(G.C, G.C2);

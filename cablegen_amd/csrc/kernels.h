@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <stddef.h>
+#include "board.h"
 
 namespace cg {
 
@@ -14,6 +15,7 @@ struct MoveRules {
     int ff;          /* free_formation */
     int prune;
     long stsl, ltc, sl;
+    cg_canon_sel_t cs;   /* canonical form through the formation's stabiliser */
 };
 
 /* Winstate test as used by the solve: `nox` (without score) makes "holds the
@@ -136,7 +138,8 @@ void launch_count_empties(hipStream_t st, const uint64_t *in, size_t n, unsigned
  * count above cap means the buffers were too small and the launch must be
  * repeated. */
 void launch_spawn(hipStream_t st, const uint64_t *in, size_t n, const unsigned long long *dn,
-                  uint64_t *out2, uint64_t *out4, size_t cap, unsigned long long *stats);
+                  uint64_t *out2, uint64_t *out4, size_t cap, unsigned long long *stats,
+                  cg_canon_sel_t cs = cg_canon_all());
 
 
 /* Sorts keys[0..n) ascending and writes the unique keys to dst (capacity n),

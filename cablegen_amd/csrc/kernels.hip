@@ -98,7 +98,7 @@ __global__ __launch_bounds__(MV_THREADS, 1) void k_move(
                     uint64_t t = d < 2 ? b : y;
                     if (cg_move_hori_tab(lut, &t, d == 1 || d == 2, r.ff) &&
                         !(r.prune && cg_prune(t, r.stsl, r.ltc, r.sl)))
-                        res[k++] = cg_canon(t);
+                        res[k++] = cg_canon_fast(t, r.cs, d >= 2);
                 }
             }
         }
@@ -199,7 +199,7 @@ __global__ __launch_bounds__(MV_THREADS, 8) void k_move_gen(
                 uint64_t t = d < 2 ? b : y;
                 if (cg_move_hori_tab(lut, &t, d == 1 || d == 2, r.ff) &&
                     !(r.prune && cg_prune(t, r.stsl, r.ltc, r.sl)))
-                    mv_first(ht, cg_canon(t));
+                    mv_first(ht, cg_canon_fast(t, r.cs, d >= 2));
             }
         }
         __syncthreads();
@@ -276,7 +276,7 @@ __global__ __launch_bounds__(SP_THREADS) void k_spawn(const uint64_t *__restrict
                                                       const unsigned long long *dn,
                                                       uint64_t *__restrict__ out2,
                                                       uint64_t *__restrict__ out4, size_t cap,
-                                                      unsigned long long *stats)
+                                                      unsigned long long *stats, cg_canon_sel_t cs)
 {
     __shared__ SpawnPairs sp;
     __shared__ unsigned long long base_sh;
@@ -312,7 +312,8 @@ __global__ __launch_bounds__(SP_THREADS) void k_spawn(const uint64_t *__restrict
         for (uint32_t p = threadIdx.x; p < tot; p += SP_THREADS) {
             uint64_t bb = sp.sb[sp.pb[p]];
             int x = sp.pc[p];
-            uint64_t c2 = cg_canon(cg_set_tile(bb, x, 1)), c4 = cg_canon(cg_set_tile(bb, x, 2));
+            uint64_t c2 = cg_canon_fast(cg_set_tile(bb, x, 1), cs, 0),
+                     c4 = cg_canon_fast(cg_set_tile(bb, x, 2), cs, 0);
             size_t o = base + p;
             if (o < cap) {   /* past the capacity: only counted (the host retries) */
                 out2[o] = c2;
@@ -328,7 +329,7 @@ __global__ __launch_bounds__(SP_THREADS) void k_spawn(const uint64_t *__restrict
 }
 
 void launch_spawn(hipStream_t st, const uint64_t *in, size_t n, const unsigned long long *dn,
-                  uint64_t *out2, uint64_t *out4, size_t cap, unsigned long long *stats)
+                  uint64_t *out2, uint64_t *out4, size_t cap, unsigned long long *stats, cg_canon_sel_t cs)
 {
     if (!n) return;
     size_t blocks = (n + SP_THREADS - 1) / SP_THREADS;
@@ -338,7 +339,7 @@ void launch_spawn(hipStream_t st, const uint64_t *in, size_t n, const unsigned l
      * the one pair now, 19.1 with 512 and 18.8 with 1024) */
     if (blocks > 512) blocks = 512;
     hipLaunchKernelGGL(k_spawn, dim3((unsigned)blocks), dim3(SP_THREADS), 0, st, in, n, dn, out2,
-                       out4, cap, stats);
+                       out4, cap, stats, cs);
 }
 
 /* ------------------------------------------------------------------------ */

@@ -135,6 +135,14 @@ int cg_dev_sort_unique_bounded(uint64_t *d_keys, size_t n, uint64_t *d_out, size
  * bits, or -1 when vary is 0. */
 int cg_sort_code_probe(const uint64_t *keys, size_t n, uint64_t vary, uint64_t cell_max,
                        uint64_t *codes, uint64_t *back);
+/* Host-side view of the canonical form through a formation's stabiliser (no
+ * GPU work): with the selection made for boards whose 0xF cells are those of
+ * e (free_formation ff), out[i] is the canonical form of b[i] taken as a
+ * board of orientation o (0 as stored, 1 turned clockwise) -- always equal
+ * to the full max of 8 images; sel[0..1] receive the images each orientation
+ * keeps (0xff: all, the formation does not decide). */
+int cg_canon_sel_probe(uint64_t e, int ff, const uint64_t *b, size_t n, int o, uint64_t *out,
+                       uint32_t *sel);
 /* Host-side view of a layer's index plan (no GPU work): the directory
  * depth in bits for a sorted unique layer of n keys with the given OR / AND.
  * Fails (-1, with a message) for n >= 2^32, which the 32-bit directory

@@ -348,6 +348,20 @@ def test_sort_cell_code_per_cell_bases(cg):
 
 
 @pytest.mark.parametrize("name", ["c1", "dpdgap_prune", "two_initial"])
+def test_tablebase_canon_stabiliser_off(cg, name):
+    """Generation with every canonical form the full max of 8 images (knob
+    canon_stab 0) matches the oracle as the default (the formation's
+    stabiliser, which every other tablebase test runs) does."""
+    try:
+        cg.tune_set("canon_stab", 0)
+        assert cg.tune_get("canon_stab") == 0
+        test_tablebase_vs_oracle(cg, name)
+    finally:
+        cg.tune_set("reset", 0)
+    assert cg.tune_get("canon_stab") == 1
+
+
+@pytest.mark.parametrize("name", ["c1", "dpdgap_prune", "two_initial"])
 def test_tablebase_cell_code_on_and_off(cg, name):
     """Generation with the cell-pair code (the default; the initial boards fix
     each layer's tile sum and 0xf cells) and with bit packing only: both
