@@ -268,8 +268,17 @@ def micro(args):
 
 
 PMC_KERNELS = {"move": "::k_move", "spawn": "::k_spawn(", "sort_hist": "::k_os_hist",
-               "sort_pass": "::k_os_pass", "unique": "::k_os_unique", "solve": "::k_solve(",
+               "sort_pass": "::k_os_pass", "unique": "::k_os_unique",
+               "sort_split:gather": "::k_ss_gather(", "sort_split:sample": "::k_ss_sample(",
+               "sort_part": "::k_ss_part(", "sort_local": "::k_ss_local(", "solve": "::k_solve(",
                "lookup": "::k_lookup(", "adjoint": "::k_adjoint("}
+# SQ counter sets (one rocprofv3 run each) and the ratios printed from them:
+# per-wave cycles issuing VALU / LDS / VMEM, waiting on LDS or on anything,
+# LDS bank-conflict cycles per LDS-active cycle, instructions per wave
+SQ_SETS = [["SQ_WAVE_CYCLES", "SQ_BUSY_CYCLES", "SQ_ACTIVE_INST_VALU", "SQ_ACTIVE_INST_LDS",
+            "SQ_ACTIVE_INST_VMEM", "SQ_WAIT_INST_LDS", "SQ_WAIT_ANY", "SQ_LDS_BANK_CONFLICT"],
+           ["SQ_WAVES", "SQ_INSTS_VALU", "SQ_INSTS_LDS", "SQ_INSTS_VMEM_RD", "SQ_INSTS_VMEM_WR",
+            "SQ_INSTS_SALU", "SQ_INSTS_SMEM", "SQ_LDS_ADDR_CONFLICT"]]
 
 
 def _csvs(d, suffix):
@@ -285,7 +294,9 @@ def prof_summary(args):
                                   runs in DIR/*FETCH_SIZE*, *WRITE_SIZE*, *TCC_HIT*;
                                   the calibration copy (__amd_rocclr_copyBuffer of
                                   1 GiB) is taken from --pmc-calib DIR (default DIR);
-                                  --pmc-json FILE also writes per-launch averages."""
+                                  --pmc-json FILE also writes per-launch averages;
+                                  with runs of the SQ_SETS counters (DIR/*SQ_WAVE_CYCLES*,
+                                  *SQ_WAVES*), a table of per-wave issue and wait ratios."""
     import csv
     import collections
     d = args.prof_summary
@@ -377,6 +388,31 @@ def prof_summary(args):
         js["workload"] = args.pmc_workload
         with open(args.pmc_json, "w") as f:
             json.dump(js, f, indent=1)
+    sq = {}
+    for grp in SQ_SETS:
+        for c in grp:
+            sq[c] = load(grp[0], c)
+    if not sq["SQ_WAVE_CYCLES"] or not sq["SQ_WAVES"]:
+        return
+    print("\nPer-wave issue and wait (SQ counters summed over a kernel's dispatches; fractions of "
+          "SQ_WAVE_CYCLES; bank conflicts per SQ_ACTIVE_INST_LDS cycle; instructions per wave):\n")
+    print("| kernel | VALU | LDS | VMEM | wait LDS | wait any | LDS bank confl. | VALU / wave "
+          "| LDS / wave | VMEM rd / wave | VMEM wr / wave | SALU / wave |")
+    print("|---|---:|---:|---:|---:|---:|---:|---:|---:|---:|---:|---:|")
+
+    def tot(c, pat):
+        return sum(v[1] for v in sq[c].values() if pat in v[0])
+    for name, pat in PMC_KERNELS.items():
+        wc, nw = tot("SQ_WAVE_CYCLES", pat), tot("SQ_WAVES", pat)
+        if not wc or not nw:
+            continue
+        lds = tot("SQ_ACTIVE_INST_LDS", pat)
+        print("| %s | %.3f | %.3f | %.3f | %.3f | %.3f | %.3f | %.0f | %.0f | %.1f | %.1f | %.0f |" % (
+            name, tot("SQ_ACTIVE_INST_VALU", pat) / wc, lds / wc, tot("SQ_ACTIVE_INST_VMEM", pat) / wc,
+            tot("SQ_WAIT_INST_LDS", pat) / wc, tot("SQ_WAIT_ANY", pat) / wc,
+            tot("SQ_LDS_BANK_CONFLICT", pat) / lds if lds else 0.0, tot("SQ_INSTS_VALU", pat) / nw,
+            tot("SQ_INSTS_LDS", pat) / nw, tot("SQ_INSTS_VMEM_RD", pat) / nw,
+            tot("SQ_INSTS_VMEM_WR", pat) / nw, tot("SQ_INSTS_SALU", pat) / nw))
 
 
 def load_calib(d, name):
@@ -658,8 +694,8 @@ def stage_traffic(kinds, final):
     if tj.get("workload") != "c2" or not all(k in km for k in final):
         return None, None
     units = sum(km[k]["launches"] for k in final)
-    tot = sum((km[k]["read_bytes_per_launch"] + km[k]["write_bytes_per_launch"]) * km[k]["launches"]
-              for k in kinds if k in km)
+    tot = sum((v["read_bytes_per_launch"] + v["write_bytes_per_launch"]) * v["launches"]
+              for k, v in km.items() if k.split(":")[0] in kinds)
     return (tot / units if units else None,
             "profiles/pmc_traffic.json (rocprofv3 FETCH_SIZE / WRITE_SIZE per dispatch, calibrated "
             "by a 1 GiB copy; main-stream dispatches of the stage's kernels per unit)")

@@ -14,8 +14,13 @@ FNV-1a-64 hash of the sorted u64 keys and of the f64 value bits, exactly as the 
                    (1,0)): produced by the oracle (oracle/liborc.so), keys and
                    values.
 
+  c5_oracle.json   config 5: the gradient dP/dp4 at p4 = 0.1 on the C2
+                   table and the per-layer sensitivities D_s, by the oracle's
+                   adjoint (FP64).
+
     python tests/golden/make_manifests.py c1
     python tests/golden/make_manifests.py c2 [cores]
+    python tests/golden/make_manifests.py c5 [cores]
 """
 import json
 import os
@@ -94,9 +99,37 @@ def make_c2(cores=8):
         f.write(dump(out))
 
 
+def make_c5(cores=8):
+    """C5: the exact gradient dP/dp4 of the win probability at p4 = 0.1 on the
+    C2 table, with the per-layer sensitivities D_s, by the oracle's adjoint
+    (orc_adjoint: occupation pushed down the optimal-play successors,
+    following the solve of src/solve.c:246-281); P is the initial board's
+    value."""
+    cfg = O.make_cfg(cores=cores, **C2["rules"])
+    start = O.orc().orc_get_sum(C2["initial"][0])
+    t0 = time.time()
+    layers = O.generate(cfg, C2["initial"], start, C2["end"])
+    vals = O.solve(cfg, layers, start, C2["end"], 0, C2["win"])
+    t1 = time.time()
+    print("generated and solved in %.1fs" % (t1 - t0), flush=True)
+    P, dP, D = O.adjoint(cfg, layers, start, vals, C2["win"], C2["initial"], p4=0.1)
+    t2 = time.time()
+    print("adjoint in %.1fs: P %.17g dP/dp4 %.17g" % (t2 - t1, P, dP), flush=True)
+    out = {"config": "C2 table (DPDGap initial 0x00008765FFF3FFF2 winstate 0x0000900000000000 "
+                     "ignore_f prune stsl=58 ltc=5 smallest_large=6 end 1100), p4 = 0.1",
+           "source": "oracle (orc_adjoint, FP64, one thread)", "cores": cores,
+           "gen_solve_seconds": t1 - t0, "adjoint_seconds": t2 - t1,
+           "P": P, "dP_dp4": dP,
+           "D": [[start + 2 * i, float(d)] for i, d in enumerate(D) if d != 0.0]}
+    with open(os.path.join(HERE, "c5_oracle.json"), "w") as f:
+        f.write(dump(out))
+
+
 if __name__ == "__main__":
     what = sys.argv[1] if len(sys.argv) > 1 else "c1"
     if what == "c1":
         make_c1()
+    elif what == "c5":
+        make_c5(int(sys.argv[2]) if len(sys.argv) > 2 else 8)
     else:
         make_c2(int(sys.argv[2]) if len(sys.argv) > 2 else 8)

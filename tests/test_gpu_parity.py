@@ -686,6 +686,27 @@ def test_c2_manifest(cg, c2_table):
         assert "%016x" % O.hash_u64(tb.values(i).view(np.uint64)) == hv, s
 
 
+@pytest.mark.skipif(not os.path.exists(os.path.join(HERE, "golden", "c5_oracle.json")),
+                    reason="C5 oracle gradient not generated")
+def test_c5_adjoint_vs_oracle(cg, c2_table):
+    """C5: dP/dp4 and the per-layer sensitivities D_s on the full DPDGap-512
+    table against the oracle's FP64 adjoint (tests/golden/c5_oracle.json,
+    written by make_manifests.py c5): P bit for bit (the solve is exact),
+    dP/dp4 and every D_s within 1e-6 relative (the GPU pushes occupation with
+    FP64 atomics, so its sums run in another order)."""
+    tb = c2_table
+    ref = json.load(open(os.path.join(HERE, "golden", "c5_oracle.json")))
+    P, dP, D = tb.adjoint(C2["initial"])
+    assert P == ref["P"]
+    assert abs(dP - ref["dP_dp4"]) <= 1e-6 * abs(ref["dP_dp4"]), (dP, ref["dP_dp4"])
+    exp = np.zeros(tb.num_layers)
+    for s, d in ref["D"]:
+        exp[(s - 492) // 2] = d
+    assert len(ref["D"]) > 100
+    assert np.allclose(D, exp, rtol=1e-6, atol=1e-300), np.max(np.abs(D - exp))
+    assert np.all((D == 0) == (exp == 0))
+
+
 def test_c2_adjoint_fd(cg, c2_table):
     """dP/dp4 on the DPDGap-512 table against central differences of FP64
     re-solves at p4 = 0.1 +- 1e-5."""
