@@ -21,14 +21,16 @@ class Rules(C.Structure):
     """cg_rules: the settings that shape layers and values (inc/settings.h:15-31)."""
     _fields_ = [("free_formation", C.c_int), ("ignore_f", C.c_int), ("prune", C.c_int),
                 ("stsl", C.c_long), ("ltc", C.c_long), ("smallest_large", C.c_long),
-                ("nox", C.c_int), ("score", C.c_int)]
+                ("nox", C.c_int), ("score", C.c_int), ("premove", C.c_int)]
 
 
 def rules(free_formation=False, ignore_f=False, prune=False, stsl=200, ltc=5, smallest_large=6,
-          nox=0, score=False):
-    """Defaults are the reference's (src/settings.c:35-59)."""
+          nox=0, score=False, premove=False):
+    """Defaults are the reference's (src/settings.c:35-59).  premove: the
+    initial boards are positions after the move (the first layer is their
+    canonical set)."""
     return Rules(int(free_formation), int(ignore_f), int(prune), int(stsl), int(ltc),
-                 int(smallest_large), int(nox), int(score))
+                 int(smallest_large), int(nox), int(score), int(premove))
 
 
 class SArr(C.Structure):

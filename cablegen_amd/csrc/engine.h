@@ -24,6 +24,7 @@ typedef struct {
     long smallest_large;
     int nox;
     int score;
+    int premove;   /* the initial boards are taken as the first layer's positions */
 } cge_rules;
 
 const char *cge_last_error(void);

@@ -667,7 +667,8 @@ static uint64_t cell_bound(const ValueBound *v, long s, uint64_t key_and)
 static int gen_step(const cg_rules &r, const uint64_t *P, size_t nP, uint64_t *C, size_t nC,
                     StepCounts &pc, LayerAlloc &la, uint64_t **M, size_t *nM,
                     Buf &nxtP, size_t *nP2, Buf &nxtC, size_t *nC2, ValueBound *vb,
-                    uint64_t *m_or = nullptr, uint64_t *m_and = nullptr, cg_canon_sel_t cs = cg_canon_all())
+                    uint64_t *m_or = nullptr, uint64_t *m_and = nullptr, cg_canon_sel_t cs = cg_canon_all(),
+                    bool premoved = false)
 {
     const long s = vb ? vb->s : 0;
     const uint16_t *lut = get_lut(r.ignore_f);
@@ -676,9 +677,19 @@ static int gen_step(const cg_rules &r, const uint64_t *P, size_t nP, uint64_t *C
     if (ensure(E.A, 4 * nP + 1)) return -1;
     launch_init_stats(E.st, E.d_cnt);
     int h = prof_begin(PK_MOVE);
-    MoveRules mr = move_rules(r);
-    mr.cs = cs;
-    launch_move(E.st, P, nP, nullptr, E.A.p, E.d_cnt, lut, mr, 1);
+    if (premoved) {
+        /* premove: the boards of P are the layer's positions -- their
+         * canonical forms are the candidates, counted as the move kernel
+         * counts its own (no move, no prune) */
+        if (nP) HC(hipMemcpyAsync(E.A.p, P, nP * 8, hipMemcpyDeviceToDevice, E.st));
+        launch_canon(E.st, E.A.p, nP);
+        launch_fill_u64(E.st, (uint64_t *)E.d_cnt, 1, nP);
+        launch_or_and(E.st, E.A.p, nP, E.d_cnt);
+    } else {
+        MoveRules mr = move_rules(r);
+        mr.cs = cs;
+        launch_move(E.st, P, nP, nullptr, E.A.p, E.d_cnt, lut, mr, 1);
+    }
     prof_end(h, 0);
     /* the previous step's carry sort (second stream) overlapped the moves;
      * its count and output are needed from here on */
@@ -961,7 +972,7 @@ extern "C" int cg_tb_generate_bounded(cg_tablebase *tb, const uint64_t *initial,
         size_t nP2 = 0, nC2 = 0;
         uint64_t mor = 0, mand = ~0ULL;
         if (gen_step(tb->r, tb->P.p, nP, tb->C.p, nC, pc, la, &l.keys, &l.n, tb->P2, &nP2,
-                     tb->C2, &nC2, &vb, &mor, &mand, tb->cs))
+                     tb->C2, &nC2, &vb, &mor, &mand, tb->cs, tb->r.premove && s == start))
             return -1;
         l.plan = plan_index(l.n, mor, mand);
         l.dir = (uint32_t *)tb->arena.alloc(l.plan.dir_entries * 4);
@@ -1064,7 +1075,8 @@ extern "C" int cg_tb_adjoint(cg_tablebase *tb, const uint64_t *initial, size_t n
     if (n) HC(hipMemcpyAsync(dinit, initial, n * 8, hipMemcpyHostToDevice, E.st));
     double *dPp = dD + nl;
     if (nl > 0 && tb->solve_lo == 0)
-        launch_adjoint_init(E.st, dinit, n, view_of(tb, 0), occ, lut, tb->r.free_formation, dPp);
+        launch_adjoint_init(E.st, dinit, n, view_of(tb, 0), occ, lut, tb->r.free_formation, dPp,
+                            tb->r.premove);
     for (int i = 0; i < nl; i++) {
         if (!tb->L[i].vals) continue;
         LayerView t2 = view_of(tb, i + 1), t4 = view_of(tb, i + 2);
@@ -1807,6 +1819,7 @@ static cg_rules to_rules(const cge_rules *r)
     x.smallest_large = r->smallest_large;
     x.nox = r->nox;
     x.score = r->score;
+    x.premove = r->premove;
     return x;
 }
 
@@ -1817,6 +1830,7 @@ static struct {
     StepCounts pc;
     ValueBound vb;
     cg_canon_sel_t cs;
+    bool first;   /* the next step is the first (premove applies to it) */
 } G;
 
 struct BufLayerAlloc : LayerAlloc {
@@ -1839,6 +1853,7 @@ extern "C" int cge_gen_begin(const cge_rules *r, const uint64_t *initial, size_t
     G.pc = StepCounts();
     G.vb = value_bound_of(initial, n);
     G.cs = n ? canon_sel_of(cg_canon(initial[0]), G.r) : cg_canon_all();
+    G.first = true;
     return 0;
 }
 
@@ -1847,8 +1862,10 @@ extern "C" int cge_gen_step(size_t *n_layer)
     uint64_t *M = nullptr;
     size_t nP2 = 0, nC2 = 0;
     BufLayerAlloc la;
+    const bool premoved = G.r.premove && G.first;
+    G.first = false;
     if (gen_step(G.r, G.P.p, G.nP, G.C.p, G.nC, G.pc, la, &M, &G.nM, G.P2, &nP2, G.C2,
-                 &nC2, &G.vb, nullptr, nullptr, G.cs))
+                 &nC2, &G.vb, nullptr, nullptr, G.cs, premoved))
         return -1;
     std::swap(G.P, G.P2);
     std::swap(G.C, G.C2);
@@ -2006,6 +2023,10 @@ extern "C" cg_batch *cg_batch_create(const cg_rules *rules)
     if (eng_init()) return nullptr;
     if (rules->free_formation) {
         fail_msg("batched tables need free_formation = false (formation cells must stay put)");
+        return nullptr;
+    }
+    if (rules->premove) {
+        fail_msg("batched tables do not take premove (run such a table alone)");
         return nullptr;
     }
     cg_batch *b = new cg_batch();

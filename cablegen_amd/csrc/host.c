@@ -261,6 +261,7 @@ void cg_rules_from_settings(cg_rules *r)
     /* the reference passes nox as a char (solve.c:172-175) */
     r->nox = (int)(signed char)s->min.nox;
     r->score = s->score;
+    r->premove = s->premove;
 }
 
 static cge_rules engine_rules(void)
@@ -268,7 +269,7 @@ static cge_rules engine_rules(void)
     cg_rules r;
     cg_rules_from_settings(&r);
     cge_rules e = { r.free_formation, r.ignore_f, r.prune, r.stsl, r.ltc, r.smallest_large,
-                    r.nox, r.score };
+                    r.nox, r.score, r.premove };
     return e;
 }
 
@@ -729,7 +730,6 @@ static void slots_release(io_slot_t *w)
 void generate(const int start, const int end, const char *fmt, const static_arr_info *initial)
 {
     cge_rules r = engine_rules();
-    if (get_settings()->premove) warnf("premove is not supported (its reference path is undefined); ignored");
     if (cge_gen_begin(&r, initial->bp, initial->size)) die("generate");
     io_slot_t w[IO_SLOTS];
     memset(w, 0, sizeof w);

@@ -241,7 +241,7 @@ void launch_adjoint(hipStream_t st, const uint64_t *M, size_t n, const double *o
 /* initial boards: best move over layer t0; adds 1/n to occ0 of the argmax
  * successor and the mean win probability to *P */
 void launch_adjoint_init(hipStream_t st, const uint64_t *init, size_t n, LayerView t0, double *occ0,
-                         const uint16_t *lut, int ff, double *P);
+                         const uint16_t *lut, int ff, double *P, int premove = 0);
 
 /* random boards under a formation mask (see k_random_boards) */
 void launch_random_boards(hipStream_t st, uint64_t *out, size_t n, uint64_t seed, uint64_t fmask,

@@ -1062,13 +1062,17 @@ void launch_adjoint(hipStream_t st, const uint64_t *M, size_t n, const double *o
 }
 
 __global__ void k_adjoint_init(const uint64_t *init, size_t n, LayerView t0, double *occ0,
-                               const uint16_t *lut, int ff, double *P)
+                               const uint16_t *lut, int ff, double *P, int premove)
 {
     for (size_t i = threadIdx.x; i < n; i += blockDim.x) {
         uint64_t b = init[i];
         double best = 0.0;
         long bj = -1;
-        for (int d = 0; d < 4; d++) {
+        if (premove) {   /* the initial boards are layer 0's positions */
+            bj = layer_find(cg_canon(b), t0);
+            best = bj >= 0 ? t0.vals[bj] : 0.0;
+        }
+        for (int d = 0; d < 4 && !premove; d++) {
             uint64_t m = b;
             if (!cg_move_tab(lut, &m, d, ff)) continue;
             long j = layer_find(cg_canon(m), t0);
@@ -1084,10 +1088,10 @@ __global__ void k_adjoint_init(const uint64_t *init, size_t n, LayerView t0, dou
 }
 
 void launch_adjoint_init(hipStream_t st, const uint64_t *init, size_t n, LayerView t0, double *occ0,
-                         const uint16_t *lut, int ff, double *P)
+                         const uint16_t *lut, int ff, double *P, int premove)
 {
     if (!n) return;
-    hipLaunchKernelGGL(k_adjoint_init, dim3(1), dim3(256), 0, st, init, n, t0, occ0, lut, ff, P);
+    hipLaunchKernelGGL(k_adjoint_init, dim3(1), dim3(256), 0, st, init, n, t0, occ0, lut, ff, P, premove);
 }
 
 /* ------------------------------------------------------------------------ */

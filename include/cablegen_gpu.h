@@ -33,6 +33,11 @@ typedef struct {
     long smallest_large;    /* [Generate] smallest_large            */
     int nox;                /* [Cablegen] nox                       */
     int score;              /* [Solve] score                        */
+    int premove;            /* [Generate] premove: the initial boards are
+                             * positions after the move, so the first layer
+                             * is their canonical set (no move step); the
+                             * win probability of the adjoint is then the
+                             * mean of their values */
 } cg_rules;
 
 const char *cg_last_error(void);

@@ -535,7 +535,23 @@ int orc_generate(const orc_cfg *cfg, const uint64_t *initial, size_t n_init,
     size_t nP = n_init;
     uint64_t *C = NULL;
     size_t nC = 0;
-    for (int i = 0; i < nl; i++) {
+    int first = 0;
+    if (cfg->premove && nl > 0) {
+        /* layer 0 = the canonical set of the initial boards; the recurrence
+         * continues from its spawns */
+        for (size_t i = 0; i < nP; i++) P[i] = orc_canon(P[i]);
+        size_t nM = 0;
+        uint64_t *arr[1] = { P };
+        size_t sz[1] = { nP };
+        orc_sort_u64(P, nP);
+        uint64_t *M = orc_kway_dedupe(arr, sz, 1, &nM);
+        free(P);
+        L[0] = M;
+        S[0] = nM;
+        orc_spawn_sets(cfg, M, nM, &P, &nP, &C, &nC);
+        first = 1;
+    }
+    for (int i = first; i < nl; i++) {
         uint64_t *M, *P2, *C2;
         size_t nM, nP2, nC2;
         orc_gen_layer(cfg, P, nP, C, nC, &M, &nM, &P2, &nP2, &C2, &nC2);
@@ -728,7 +744,15 @@ int orc_adjoint(const orc_cfg *cfg, uint64_t **layers, const size_t *sizes, int 
         for (size_t i = 0; i < n_init; i++) {
             uint64_t succ = 0;
             int found;
-            double v = best_move(L, initial[i], layers[0], values[0], sizes[0], &succ, &found);
+            double v;
+            if (cfg->premove) {   /* the initial board is a layer-0 position */
+                succ = orc_canon(initial[i]);
+                long j = find_index(succ, layers[0], sizes[0]);
+                found = j >= 0;
+                v = found ? values[0][j] : 0.0;
+            } else {
+                v = best_move(L, initial[i], layers[0], values[0], sizes[0], &succ, &found);
+            }
             P += v / (double)n_init;
             if (found) {
                 long j = find_index(succ, layers[0], sizes[0]);

@@ -30,6 +30,13 @@ typedef struct {
     long ltc;                /* settings.c:50  [Generate] ltc            */
     long smallest_large;     /* settings.c:51  [Generate] smallest_large */
     int score;               /* settings.c:57  [Solve] score             */
+    int premove;             /* settings.c:45,72 [Generate] premove: its
+                              * reference block (generate.c:571-579) moves
+                              * the initial boards into a buffer nothing
+                              * reads; defined here as "the initial boards
+                              * are positions after the move": the first
+                              * layer is their canonical set, the adjoint's
+                              * P the mean of their values */
 } orc_cfg;
 
 /* --- board primitives (board.c) --- */

@@ -23,14 +23,14 @@ class OrcCfg(C.Structure):
     _fields_ = [
         ("free_formation", C.c_int), ("ignore_f", C.c_int), ("cores", C.c_int),
         ("nox", C.c_int), ("prune", C.c_int), ("stsl", C.c_long), ("ltc", C.c_long),
-        ("smallest_large", C.c_long), ("score", C.c_int),
+        ("smallest_large", C.c_long), ("score", C.c_int), ("premove", C.c_int),
     ]
 
 
 def make_cfg(free_formation=False, ignore_f=False, cores=1, nox=0, prune=False,
-             stsl=200, ltc=5, smallest_large=6, score=False):
+             stsl=200, ltc=5, smallest_large=6, score=False, premove=False):
     return OrcCfg(int(free_formation), int(ignore_f), int(cores), int(nox), int(prune),
-                  int(stsl), int(ltc), int(smallest_large), int(score))
+                  int(stsl), int(ltc), int(smallest_large), int(score), int(premove))
 
 
 def _u64(a):

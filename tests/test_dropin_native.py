@@ -27,7 +27,7 @@ needs_caller = pytest.mark.skipif(not os.path.exists(CALLER),
                                   reason="tests/native/dropin_caller not built (run build())")
 
 
-def write_conf(d, initial, win, end_gen, end_solve=0, ignore_f=True, delete_boards=False):
+def write_conf(d, initial, win, end_gen, end_solve=0, ignore_f=True, delete_boards=False, premove=False):
     os.makedirs(f"{d}/b", exist_ok=True)
     os.makedirs(f"{d}/t", exist_ok=True)
     np.array(initial, np.uint64).tofile(f"{d}/initial")
@@ -36,6 +36,7 @@ def write_conf(d, initial, win, end_gen, end_solve=0, ignore_f=True, delete_boar
     with open(conf, "w") as f:
         f.write(f"[Cablegen]\nignore_f = {'true' if ignore_f else 'false'}\ncores = 2\n"
                 f"[Generate]\ndir = {d}/b/\ninitial = {d}/initial\nend = {end_gen}\n"
+                f"premove = {'true' if premove else 'false'}\n"
                 f"[Solve]\ndir = {d}/t/\nwinstates = {d}/winstates\nend = {end_solve}\n"
                 f"delete_boards = {'true' if delete_boards else 'false'}\n")
     return conf
@@ -151,6 +152,28 @@ def test_native_caller_delete_boards(tmp_path):
     conf2 = write_conf(d2, [0x11000000FFFFFFFF], [0x7000000000000000], 120)
     run(CALLER, "gensolve", conf2)
     assert len(os.listdir(f"{d2}/b")) == 59
+
+
+@pytest.mark.gpu
+@needs_caller
+def test_native_caller_premove(tmp_path):
+    """[Generate] premove = true through generate()/solve() from C: the
+    initial boards are positions after the move, so the first layer is their
+    canonical set; every table equals the oracle's premove run."""
+    cfg = O.make_cfg(ignore_f=True, cores=8)
+    plain = O.generate(cfg, [0x11000000FFFFFFFF], 4, 120)
+    b0 = [O.orc().orc_rotate_cw(int(x)) for x in plain[0]]
+    cfgp = O.make_cfg(ignore_f=True, cores=8, premove=True)
+    layers = O.generate(cfgp, b0, 4, 120)
+    vals = O.solve(cfgp, layers, 4, 120, 0, [0x7000000000000000])
+    d = str(tmp_path)
+    conf = write_conf(d, b0, [0x7000000000000000], 120, premove=True)
+    run(CALLER, "gensolve", conf)
+    for i, (K, V) in enumerate(zip(layers, vals)):
+        k, v = O.read_table_file(f"{d}/t/{4 + 2 * i}.tables")
+        assert np.array_equal(k, K) and np.array_equal(v.view(np.uint64), V.view(np.uint64)), i
+        assert np.array_equal(O.read_boards_file(f"{d}/b/{4 + 2 * i}.boards"), K), i
+    assert np.array_equal(layers[0], plain[0])
 
 
 # ---------------------------------------------------------------- CPU: host-only consumers

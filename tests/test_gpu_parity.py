@@ -602,6 +602,42 @@ def test_adjoint_vs_oracle_and_fd(cg):
     tb.close()
 
 
+@pytest.mark.parametrize("name", ["c1_ignore_f", "dpdgap_prune"])
+def test_tablebase_premove_vs_oracle(cg, name):
+    """premove = 1 (the initial boards are positions after the move, taken in
+    any orientation): layers and values against the oracle's premove
+    generation, and the GPU's premove run from a plain run's first layer
+    equals the plain run."""
+    rules, initial, win, start, end = TB_CASES[name]
+    end = min(end, start + 80)
+    cfg = O.make_cfg(cores=8, **rules)
+    plain = O.generate(cfg, initial, start, end)
+    b0 = [O.orc().orc_rotate_cw(int(x)) for x in plain[0][::3]] + [int(x) for x in plain[0]]
+    cfgp = O.make_cfg(cores=8, premove=True, **rules)
+    layers = O.generate(cfgp, b0, start, end)
+    vals = O.solve(cfgp, layers, start, end, 0, win)
+    tb = cg.Tablebase(cg.rules(premove=True, **rules))
+    tb.generate(b0, start, end)
+    tb.solve(win, keep_argmax=True)
+    assert tb.num_layers == len(layers)
+    for i in range(len(layers)):
+        assert np.array_equal(tb.keys(i), layers[i]), i
+        assert np.array_equal(tb.keys(i), plain[i]), i
+        assert np.array_equal(tb.values(i).view(np.uint64), vals[i].view(np.uint64)), i
+    Po, dPo, Do = O.adjoint(cfgp, layers, start, vals, win, b0)
+    P, dP, D = tb.adjoint(b0)
+    assert P == Po
+    assert abs(dP - dPo) <= 1e-12 * abs(dPo)
+    assert np.allclose(D, Do, rtol=1e-10, atol=1e-15)
+    tb.close()
+
+
+def test_batch_rejects_premove(cg):
+    r = cg.rules(ignore_f=True, premove=True)
+    assert cg.lib().cg_batch_create(C.byref(r)) is None
+    assert "premove" in cg.last_error()
+
+
 def test_adjoint_autograd(cg):
     from cablegen_amd.autograd import win_probability
     rules, initial, win, start, end = TB_CASES["c1_ignore_f"]

@@ -122,6 +122,12 @@ def test_settings_defaults_and_keys(cg, tmp_path):
     r = cg.Rules()
     L.cg_rules_from_settings(C.byref(r))
     assert r.free_formation == 1 and r.nox == 9 and r.stsl == 200 and r.smallest_large == 6
+    assert r.premove == 0
+    with open(tmp_path / "p.conf", "w") as f:
+        f.write("[Generate]\npremove = true\n")
+    L.change_config(str(tmp_path / "p.conf").encode())
+    L.cg_rules_from_settings(C.byref(r))
+    assert r.premove == 1
 
 
 def test_files_round_trip(cg, tmp_path):

@@ -464,3 +464,49 @@ def test_spawn_sets_consistent_with_gen_layer():
         assert np.array_equal(P2, np.union1d(S2, Cc))
         assert np.array_equal(C2, S4)
         P, Cc = P2, C2
+
+
+# ---------------------------------------------------------------- premove
+
+@pytest.mark.parametrize("rules,initial,start,end", [
+    (dict(ignore_f=True), [0x11000000FFFFFFFF], 4, 80),
+    (dict(ignore_f=True, prune=True, stsl=58, ltc=5, smallest_large=6), [0x00008765FFF3FFF2], 492, 530),
+])
+def test_premove_identity(rules, initial, start, end):
+    """premove (the initial boards are positions after the move; the first
+    layer is their canonical set): started from the first layer of a plain
+    run, it gives that run's layers exactly -- the plain recurrence's state
+    after its first move step is the premove run's state at the start."""
+    cfg = O.make_cfg(cores=4, **rules)
+    plain = O.generate(cfg, initial, start, end)
+    # the first layer's boards in non-canonical orientations, with repeats
+    b0 = [O.orc().orc_rotate_cw(int(x)) for x in plain[0][::3]] + [int(x) for x in plain[0]]
+    pre = O.generate(O.make_cfg(cores=4, premove=True, **rules), b0, start, end)
+    assert len(pre) == len(plain)
+    for a, b in zip(plain, pre):
+        assert np.array_equal(a, b)
+
+
+def test_premove_adjoint():
+    """With premove the win probability is the mean value of the initial
+    boards' canonical forms in the first layer, and its p4-derivative matches
+    central differences."""
+    rules = dict(ignore_f=True)
+    cfgp = O.make_cfg(cores=4, premove=True, **rules)
+    win, start, end = [0x7000000000000000], 4, 200
+    initial = [0x1100000FFFFFFFF0 & 0xFFFFFFFFFFFFFFFF, 0x11000000FFFFFFFF, 0x10100000FFFFFFFF]
+    initial = [x for x in initial if O.orc().orc_get_sum(x) == 4]
+    layers = O.generate(cfgp, initial, start, end)
+    vals = O.solve(cfgp, layers, start, end, 0, win, p4=0.1)
+    P, dP, D = O.adjoint(cfgp, layers, start, vals, win, initial, p4=0.1)
+    pos = np.searchsorted(layers[0], [O.canon(x) for x in initial])
+    assert P == sum(vals[0][pos] / len(initial)) or np.isclose(P, vals[0][pos].mean(), rtol=1e-15)
+    assert np.isclose(D.sum(), dP, rtol=1e-12, atol=0)
+
+    def p_at(p4):
+        v = O.solve(cfgp, layers, start, end, 0, win, p4=p4)
+        return O.adjoint(cfgp, layers, start, v, win, initial, p4=p4)[0]
+
+    h = 1e-6
+    fd = (p_at(0.1 + h) - p_at(0.1 - h)) / (2 * h)
+    assert abs(fd - dP) <= 1e-6 * abs(dP), (fd, dP)
