@@ -487,19 +487,42 @@ __global__ __launch_bounds__(256) void k_dir_count(const uint64_t *__restrict__ 
     }
 }
 
-constexpr int DS_THREADS = 1024, DS_IPT = 4, DS_TILE = DS_THREADS * DS_IPT;
+/* Exclusive scan of the bucket counts into the directory: tiles of 16384
+ * counts, 16 consecutive per thread (four 16-byte loads and stores);
+ * k_dscan_reduce sums each tile, and each block of k_dscan_down adds up the
+ * tile sums before its own (at most DS_FOLD tiles; beyond that k_dscan_top
+ * scans them first) -- two launches for a directory of up to 2^27 buckets. */
+constexpr int DS_THREADS = 1024, DS_IPT = 16, DS_TILE = DS_THREADS * DS_IPT;
+constexpr size_t DS_FOLD = 8192;
+
+__device__ __forceinline__ void ds_load(const uint32_t *__restrict__ in, size_t base, size_t m,
+                                        uint32_t (&v)[DS_IPT])
+{
+    if (base + DS_IPT <= m) {
+        const uint4 *q = reinterpret_cast<const uint4 *>(in + base);
+#pragma unroll
+        for (int j = 0; j < DS_IPT / 4; j++) {
+            const uint4 x = q[j];
+            v[4 * j] = x.x;
+            v[4 * j + 1] = x.y;
+            v[4 * j + 2] = x.z;
+            v[4 * j + 3] = x.w;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < DS_IPT; j++) v[j] = base + j < m ? in[base + j] : 0u;
+    }
+}
 
 __global__ __launch_bounds__(DS_THREADS) void k_dscan_reduce(const uint32_t *__restrict__ in, size_t m,
                                                              uint32_t *sums)
 {
     __shared__ uint32_t wtmp[17];
-    size_t base = (size_t)blockIdx.x * DS_TILE;
+    uint32_t v[DS_IPT];
+    ds_load(in, (size_t)blockIdx.x * DS_TILE + (size_t)threadIdx.x * DS_IPT, m, v);
     uint32_t s = 0;
 #pragma unroll
-    for (int j = 0; j < DS_IPT; j++) {
-        size_t i = base + (size_t)j * DS_THREADS + threadIdx.x;
-        if (i < m) s += in[i];
-    }
+    for (int j = 0; j < DS_IPT; j++) s += v[j];
     uint32_t tot;
     block_excl_scan<uint32_t>(s, &tot, wtmp);
     if (threadIdx.x == 0) sums[blockIdx.x] = tot;
@@ -519,28 +542,57 @@ __global__ __launch_bounds__(DS_THREADS) void k_dscan_top(uint32_t *sums, size_t
     }
 }
 
-/* exclusive scan of cnt[0..m) into dir[0..m) plus dir[m] = total */
+/* exclusive scan of cnt[0..m) into dir[0..m) plus dir[m] = total; sums are
+ * the tile sums (nb <= DS_FOLD) or their exclusive scan (nb > DS_FOLD) */
 __global__ __launch_bounds__(DS_THREADS) void k_dscan_down(const uint32_t *__restrict__ in, size_t m,
-                                                           const uint32_t *sums, uint32_t *out,
+                                                           const uint32_t *sums, size_t nb, uint32_t *out,
                                                            uint32_t total)
 {
-    __shared__ uint32_t wtmp[17];
-    size_t base = (size_t)blockIdx.x * DS_TILE + (size_t)threadIdx.x * DS_IPT;
+    __shared__ uint32_t wtmp[17], wtmp2[17];
+    uint32_t pre;
+    if (nb <= DS_FOLD) {
+        uint32_t p = 0;
+        for (size_t t = threadIdx.x; t < blockIdx.x; t += DS_THREADS) p += sums[t];
+        block_excl_scan<uint32_t>(p, &pre, wtmp2);
+    } else {
+        pre = sums[blockIdx.x];
+    }
+    const size_t base = (size_t)blockIdx.x * DS_TILE + (size_t)threadIdx.x * DS_IPT;
     uint32_t v[DS_IPT];
+    ds_load(in, base, m, v);
     uint32_t s = 0;
 #pragma unroll
-    for (int j = 0; j < DS_IPT; j++) {
-        v[j] = base + j < m ? in[base + j] : 0;
-        s += v[j];
-    }
+    for (int j = 0; j < DS_IPT; j++) s += v[j];
     uint32_t tot;
-    uint32_t ex = block_excl_scan<uint32_t>(s, &tot, wtmp) + sums[blockIdx.x];
+    uint32_t ex = block_excl_scan<uint32_t>(s, &tot, wtmp) + pre;
 #pragma unroll
     for (int j = 0; j < DS_IPT; j++) {
-        if (base + j < m) out[base + j] = ex;
-        ex += v[j];
+        const uint32_t x = v[j];
+        v[j] = ex;
+        ex += x;
+    }
+    if (base + DS_IPT <= m) {
+        uint4 *q = reinterpret_cast<uint4 *>(out + base);
+#pragma unroll
+        for (int j = 0; j < DS_IPT / 4; j++) q[j] = make_uint4(v[4 * j], v[4 * j + 1], v[4 * j + 2], v[4 * j + 3]);
+    } else {
+#pragma unroll
+        for (int j = 0; j < DS_IPT; j++)
+            if (base + j < m) out[base + j] = v[j];
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) out[m] = total;
+}
+
+/* dir[0..m] from the counts cnt[0..m) (sums: scratch for the tile sums) */
+static void launch_dscan(hipStream_t st, const uint32_t *cnt, size_t m, uint32_t *sums, uint32_t *dir,
+                         uint32_t total)
+{
+    const size_t nb = (m + DS_TILE - 1) / DS_TILE;
+    if (nb) hipLaunchKernelGGL(k_dscan_reduce, dim3((unsigned)nb), dim3(DS_THREADS), 0, st, cnt, m, sums);
+    if (nb > DS_FOLD) hipLaunchKernelGGL(k_dscan_top, dim3(1), dim3(DS_THREADS), 0, st, sums, nb);
+    /* m == 0: one block writes dir[0] = total */
+    hipLaunchKernelGGL(k_dscan_down, dim3((unsigned)(nb ? nb : 1)), dim3(DS_THREADS), 0, st, cnt, m,
+                       (const uint32_t *)sums, nb, dir, total);
 }
 
 size_t dir_scratch_bytes(size_t entries)
@@ -564,11 +616,7 @@ void launch_build_dir(hipStream_t st, const uint64_t *keys, size_t n, const Laye
         hipLaunchKernelGGL(k_dir_count, dim3((unsigned)blocks), dim3(256), 0, st, keys, n, p.x,
                            p.shift, cnt);
     }
-    size_t nb = (m + DS_TILE - 1) / DS_TILE;
-    hipLaunchKernelGGL(k_dscan_reduce, dim3((unsigned)nb), dim3(DS_THREADS), 0, st, cnt, m, sums);
-    hipLaunchKernelGGL(k_dscan_top, dim3(1), dim3(DS_THREADS), 0, st, sums, nb);
-    hipLaunchKernelGGL(k_dscan_down, dim3((unsigned)nb), dim3(DS_THREADS), 0, st, cnt, m, sums, dir,
-                       (uint32_t)n);
+    launch_dscan(st, cnt, m, sums, dir, (uint32_t)n);
 }
 
 /* index of q in the layer, or -1: constant bits must match; the bucket's
@@ -1395,11 +1443,7 @@ void launch_bdir(hipStream_t st, const uint64_t *keys, size_t n, const size_t *s
     if (n)
         hipLaunchKernelGGL(k_bdir_count, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, keys, n,
                            seg, ntab, plan, cnt);
-    size_t nb = (m + DS_TILE - 1) / DS_TILE;
-    hipLaunchKernelGGL(k_dscan_reduce, dim3((unsigned)nb), dim3(DS_THREADS), 0, st, cnt, m, sums);
-    hipLaunchKernelGGL(k_dscan_top, dim3(1), dim3(DS_THREADS), 0, st, sums, nb);
-    hipLaunchKernelGGL(k_dscan_down, dim3((unsigned)nb), dim3(DS_THREADS), 0, st, cnt, m, sums, dir,
-                       (uint32_t)n);
+    launch_dscan(st, cnt, m, sums, dir, (uint32_t)n);
 }
 
 /* batched solve: block b works on table t with desc[t].tile0 <= b <
