@@ -269,7 +269,8 @@ def micro(args):
 
 PMC_KERNELS = {"move": "::k_move", "spawn": "::k_spawn(", "sort_hist": "::k_os_hist",
                "sort_pass": "::k_os_pass", "unique": "::k_os_unique",
-               "sort_split:gather": "::k_ss_gather(", "sort_split:sample": "::k_ss_sample(",
+               "sort_split:gather": "::k_ss_gather(", "sort_split:runs": "::k_ss_runs(",
+               "sort_split:select": "::k_ss_select(",
                "sort_part": "::k_ss_part(", "sort_local": "::k_ss_local(", "solve": "::k_solve(",
                "lookup": "::k_lookup(", "adjoint": "::k_adjoint("}
 # SQ counter sets (one rocprofv3 run each) and the ratios printed from them:

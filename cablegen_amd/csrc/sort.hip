@@ -505,8 +505,9 @@ __global__ __launch_bounds__(UQ_THREADS) void k_os_unique(const uint64_t *__rest
 /* For mid-size sorts (the move candidates and spawn-4 sets of a layer step,
  * 10^4 to 10^7 keys) the LSD passes are bound by tile latency rather than
  * bandwidth, and every duplicate crosses HBM once per pass.  Here:
- *   k_ss_gather, k_ss_sample  a regular sample of SS_S keys, sorted by one
- *                block, gives B - 1 splitters (every (S/B)-th sample);
+ *   k_ss_gather, k_ss_runs, k_ss_select  a regular sample of SS_S keys,
+ *                sorted in 16 runs of 1024 by 16 blocks; B - 1 splitters
+ *                (every (S/B)-th sample) by each value's rank over the runs;
  *   k_ss_part    one read of the keys: each key goes to the bucket of the
  *                splitters around it (all copies of a key to one bucket),
  *                tiles rank their keys per bucket in LDS and reserve room
@@ -691,76 +692,110 @@ __device__ __forceinline__ void ss_block_sort(V (&v)[K], int m, int chunk, int n
 }
 
 /* the regular sample of SS_S keys, gathered by many blocks (one block's
- * scattered loads would take ~20 us) into smp */
-__global__ __launch_bounds__(256) void k_ss_gather(const uint64_t *__restrict__ keys, size_t n,
-                                                   uint64_t *__restrict__ smp)
+ * scattered loads would take ~20 us) into smp, with its smallest and
+ * largest key folded into mnx[0] (as ~min) and mnx[1] by atomic max (both
+ * cleared to 0 beforehand) */
+__global__ __launch_bounds__(SS_T) void k_ss_gather(const uint64_t *__restrict__ keys, size_t n,
+                                                    uint64_t *__restrict__ smp, unsigned long long *mnx)
 {
-    const size_t s = (size_t)blockIdx.x * 256 + threadIdx.x;
-    if (s < (size_t)SS_S) smp[s] = keys[((2 * s + 1) * n) / (2 * (size_t)SS_S)];
+    __shared__ uint64_t ra[SS_NW], rb[SS_NW];
+    const size_t s = (size_t)blockIdx.x * SS_T + threadIdx.x;
+    uint64_t a = 0, b = 0;   /* ~min and max */
+    if (s < (size_t)SS_S) {
+        const uint64_t k = keys[((2 * s + 1) * n) / (2 * (size_t)SS_S)];
+        smp[s] = k;
+        a = ~k;
+        b = k;
+    }
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        const uint64_t x = __shfl_xor(a, d, 64), y = __shfl_xor(b, d, 64);
+        a = x > a ? x : a;
+        b = y > b ? y : b;
+    }
+    /* one atomic pair per block: same-address atomics from every wave of the
+     * grid serialise */
+    if (lane_id() == 0) ra[threadIdx.x >> 6] = a, rb[threadIdx.x >> 6] = b;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < SS_NW; w++) {
+            a = ra[w] > a ? ra[w] : a;
+            b = rb[w] > b ? rb[w] : b;
+        }
+        atomicMax(&mnx[0], (unsigned long long)a);
+        atomicMax(&mnx[1], (unsigned long long)b);
+    }
 }
 
-/* B - 1 splitters from a regular sample of SS_S keys (n > SS_S).  The
- * sample is sorted on the top 32 bits of its range, (key - min) >> z, and a
- * splitter is min + (value << z): monotone, which is all a splitter needs.
- * Ties in those bits only move a bucket boundary to a multiple of 2^z.
- * Fewer bits would not do: a layer's keys fill a tiny part of their range,
- * and one 2^24-th of it can hold more keys than a bucket. */
-__global__ __launch_bounds__(SS_T) void k_ss_sample(const uint64_t *__restrict__ smp, int B,
-                                                    uint64_t *__restrict__ spl, uint16_t *__restrict__ guide,
-                                                    uint64_t *__restrict__ gparm)
+/* The sample's order: its values are the top 32 bits of its range, (key -
+ * min) >> z, and a splitter is min + (value << z): monotone, which is all a
+ * splitter needs.  Ties in those bits only move a bucket boundary to a
+ * multiple of 2^z.  Fewer bits would not do: a layer's keys fill a tiny part
+ * of their range, and one 2^24-th of it can hold more keys than a bucket. */
+__device__ __forceinline__ void ss_sample_range(const unsigned long long *mnx, uint64_t *mn, int *nb, int *z)
+{
+    const uint64_t lo = ~(uint64_t)mnx[0], hi = mnx[1];
+    *mn = lo;
+    *nb = hi > lo ? 64 - __clzll((long long)(hi - lo)) : 0;
+    *z = *nb > 32 ? *nb - 32 : 0;
+}
+
+/* SS_NRUN blocks each sort SS_RUN of the sample's values into a run */
+constexpr int SS_RUN = SS_T;
+constexpr int SS_NRUN = SS_S / SS_RUN;
+
+__global__ __launch_bounds__(SS_T) void k_ss_runs(const uint64_t *__restrict__ smp, const unsigned long long *mnx,
+                                                  uint32_t *__restrict__ runs)
 {
     __shared__ SsLds L;
-    constexpr int K = SS_S / SS_T;
-    const int w = threadIdx.x >> 6, lane = lane_id();
-    constexpr int chunk = K * 64;
-    uint64_t a = ~0ULL, b = 0, k[K];
-#pragma unroll
-    for (int j = 0; j < K; j++) {
-        k[j] = smp[w * chunk + j * 64 + lane];
-        a = k[j] < a ? k[j] : a;
-        b = k[j] > b ? k[j] : b;
-    }
-    uint64_t mn, mx;
-    ss_min_max(a, b, L, &mn, &mx);
-    const int nb = mx > mn ? 64 - __clzll((long long)(mx - mn)) : 0;
-    const int z = nb > 32 ? nb - 32 : 0;
-    uint32_t v[K];
-#pragma unroll
-    for (int j = 0; j < K; j++) v[j] = (uint32_t)((k[j] - mn) >> z);
-    /* a run-time count keeps the sort's loops rolled as in k_ss_local (a
-     * constant one lets the compiler interleave all rows and spill) */
-    int m = SS_S;
+    uint64_t mn;
+    int nb, z;
+    ss_sample_range(mnx, &mn, &nb, &z);
+    /* one row per wave: value t of the run is thread t's */
+    uint32_t v[2];
+    v[0] = (uint32_t)((smp[(size_t)blockIdx.x * SS_RUN + threadIdx.x] - mn) >> z);
+    v[1] = 0;
+    int m = SS_RUN;
     asm volatile("" : "+s"(m));
-    ss_block_sort<uint32_t, K>(v, m, chunk, nb - z, L);
-    const int nspl = B - 1;
-    /* the splitters' sample values, kept in the counters' room (2048 words) */
-    uint32_t *sv = reinterpret_cast<uint32_t *>(L.cnt);
-    for (int i = threadIdx.x; i < nspl; i += SS_T) {
-        const uint32_t x = L.stage.v32[((size_t)(i + 1) * SS_S) / B];
-        sv[i] = x;
-        spl[i] = mn + ((uint64_t)x << z);
-    }
+    ss_block_sort<uint32_t, 2>(v, m, 64, nb - z, L);
+    runs[(size_t)blockIdx.x * SS_RUN + threadIdx.x] = L.stage.v32[threadIdx.x];
+}
+
+/* Splitter i is the sample value of rank ((i + 1) SS_S) / B.  Each thread
+ * takes one value of one run and finds its rank in the whole sample: its
+ * place in its run plus, in every other run, the values before it (equal
+ * values of earlier runs count as before, of later runs as after, so the
+ * ranks are 0..SS_S - 1 once each); the thread holding a splitter's rank
+ * writes it. */
+__global__ __launch_bounds__(SS_T) void k_ss_select(const uint32_t *__restrict__ runs,
+                                                    const unsigned long long *mnx, int B,
+                                                    uint64_t *__restrict__ spl)
+{
+    __shared__ uint32_t R[SS_S];
+    for (int i = threadIdx.x; i < SS_S; i += SS_T) R[i] = runs[i];
     __syncthreads();
-    /* guide[p]: splitters below mn + (p << gs); the sample's range fits
-     * SS_GUIDE steps of 2^gs */
-    const int gs = nb > 12 ? nb - 12 : 0;
-    for (int p = threadIdx.x; p <= SS_GUIDE; p += SS_T) {
-        int lo = 0, hi = nspl;
-        if (p == SS_GUIDE) {
-            lo = nspl;
-        } else {
-            const uint64_t off = (uint64_t)p << gs;
-            while (lo < hi) {
-                const int mid = (lo + hi) >> 1;
-                if (((uint64_t)sv[mid] << z) < off) lo = mid + 1;
-                else hi = mid;
-            }
+    const int r = blockIdx.x;
+    const uint32_t e = R[r * SS_RUN + threadIdx.x];
+    uint32_t rank = threadIdx.x;
+    for (int q = 0; q < SS_NRUN; q++) {
+        if (q == r) continue;
+        const uint32_t *X = R + q * SS_RUN;
+        uint32_t lo = 0;
+#pragma unroll
+        for (uint32_t step = SS_RUN / 2; step; step >>= 1) {
+            const uint32_t x = X[lo + step - 1];
+            if (q < r ? x <= e : x < e) lo += step;
         }
-        guide[p] = (uint16_t)lo;
+        const uint32_t x = X[lo < SS_RUN ? lo : SS_RUN - 1];
+        if (lo < SS_RUN && (q < r ? x <= e : x < e)) lo++;
+        rank += lo;
     }
-    if (threadIdx.x == 0) {
-        gparm[0] = mn;
-        gparm[1] = (uint64_t)gs;
+    const uint64_t c = ((uint64_t)rank * (uint64_t)B + SS_S - 1) / SS_S;   /* i + 1 */
+    if (c >= 1 && c + 1 <= (uint64_t)B && (c * SS_S) / (uint64_t)B == rank) {
+        uint64_t mn;
+        int nb, z;
+        ss_sample_range(mnx, &mn, &nb, &z);
+        spl[c - 1] = mn + ((uint64_t)e << z);
     }
 }
 
@@ -781,8 +816,7 @@ struct SsPartLds {
  * bucket's keys (the bucket is then oversize). */
 __global__ __launch_bounds__(SS_T, 2) void k_ss_part(const uint64_t *__restrict__ keys, size_t n,
                                                      const uint64_t *__restrict__ spl_g,
-                                                     const uint16_t *__restrict__ guide_g,
-                                                     const uint64_t *__restrict__ gparm, int B,
+                                                     const unsigned long long *mnx, int B,
                                                      uint64_t *__restrict__ part,
                                                      unsigned long long *gcur, unsigned *err)
 {
@@ -790,10 +824,28 @@ __global__ __launch_bounds__(SS_T, 2) void k_ss_part(const uint64_t *__restrict_
     const int nspl = B - 1;
     const int grp = blockIdx.x % SS_NX;
     for (int i = threadIdx.x; i < nspl; i += SS_T) P.spl[i] = spl_g[i];
-    for (int i = threadIdx.x; i <= SS_GUIDE; i += SS_T) P.guide[i] = guide_g[i];
     for (int i = threadIdx.x; i < B; i += SS_T) P.cnt[i] = 0;
-    const uint64_t smin = gparm[0];
-    const int gs = (int)gparm[1];
+    uint64_t smin;
+    int nbs, zs;
+    ss_sample_range(mnx, &smin, &nbs, &zs);
+    /* guide[p]: splitters below smin + (p << gs); the sample's range fits
+     * SS_GUIDE steps of 2^gs */
+    const int gs = nbs > 12 ? nbs - 12 : 0;
+    __syncthreads();
+    for (int p = threadIdx.x; p <= SS_GUIDE; p += SS_T) {
+        int lo = 0, hi = nspl;
+        if (p == SS_GUIDE) {
+            lo = nspl;
+        } else {
+            const uint64_t bound = smin + ((uint64_t)p << gs);
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (P.spl[mid] < bound) lo = mid + 1;
+                else hi = mid;
+            }
+        }
+        P.guide[p] = (uint16_t)lo;
+    }
     __syncthreads();
     const size_t tile0 = (size_t)blockIdx.x * SS_TILE;
     uint64_t k[SS_PK];
@@ -1303,18 +1355,16 @@ struct OsScratch {
     unsigned *ss_ticket;
     unsigned long long *ss_cnt;
     uint64_t *ss_spl;
-    uint16_t *ss_guide;
-    uint64_t *ss_gparm;
+    unsigned long long *ss_mnx;   /* the sample's ~min and max (in the cleared head) */
 };
 
 constexpr size_t OS_CLEAR_BYTES = 256 + (size_t)OS_MAXPASS * OS_MAXR * 8;
 constexpr size_t SS_OFF = OS_CLEAR_BYTES + (size_t)OS_MAXPASS * OS_MAXR * 8;
 constexpr size_t SS_CLEAR_BYTES = 256 + (size_t)SS_BMAX * SS_NX * 8;
-constexpr size_t SS_GUIDE_BYTES = ((size_t)(SS_GUIDE + 1) * 2 + 255) & ~(size_t)255;
 
 size_t sort_scratch_bytes(size_t)
 {
-    return SS_OFF + SS_CLEAR_BYTES + (size_t)SS_BMAX * 8 + SS_GUIDE_BYTES + 256;
+    return SS_OFF + SS_CLEAR_BYTES + (size_t)SS_BMAX * 8 + 256;
 }
 
 static OsScratch carve(void *scratch)
@@ -1328,8 +1378,7 @@ static OsScratch carve(void *scratch)
     s.ss_ticket = (unsigned *)(p + SS_OFF);
     s.ss_cnt = (unsigned long long *)(p + SS_OFF + 256);
     s.ss_spl = (uint64_t *)(p + SS_OFF + SS_CLEAR_BYTES);
-    s.ss_guide = (uint16_t *)(p + SS_OFF + SS_CLEAR_BYTES + (size_t)SS_BMAX * 8);
-    s.ss_gparm = (uint64_t *)(p + SS_OFF + SS_CLEAR_BYTES + (size_t)SS_BMAX * 8 + SS_GUIDE_BYTES);
+    s.ss_mnx = (unsigned long long *)(p + SS_OFF + 64);
     return s;
 }
 
@@ -1468,18 +1517,21 @@ static void ss_sort_unique(hipStream_t st, const uint64_t *keys, size_t n, size_
     if (B > 1) {
         launch_fill_u64(st, (uint64_t *)s.ss_ticket, (256 + B * SS_NX * 8) / 8, 0);
         int h = prof_hook_begin(8, st);
-        /* the sample goes to the start of part, which the partition
-         * overwrites only after the splitters are made */
-        hipLaunchKernelGGL(k_ss_gather, dim3(SS_S / 256), dim3(256), 0, st, keys, n, part);
-        hipLaunchKernelGGL(k_ss_sample, dim3(1), dim3(SS_T), 0, st, (const uint64_t *)part, (int)B,
-                           s.ss_spl, s.ss_guide, s.ss_gparm);
+        /* the sample and its sorted runs go to the start of part, which the
+         * partition overwrites only after the splitters are made */
+        uint32_t *runs = reinterpret_cast<uint32_t *>(part + SS_S);
+        hipLaunchKernelGGL(k_ss_gather, dim3(SS_S / SS_T), dim3(SS_T), 0, st, keys, n, part, s.ss_mnx);
+        hipLaunchKernelGGL(k_ss_runs, dim3(SS_NRUN), dim3(SS_T), 0, st, (const uint64_t *)part,
+                           (const unsigned long long *)s.ss_mnx, runs);
+        hipLaunchKernelGGL(k_ss_select, dim3(SS_NRUN), dim3(SS_T), 0, st, (const uint32_t *)runs,
+                           (const unsigned long long *)s.ss_mnx, (int)B, s.ss_spl);
         snote(hipGetLastError());
         prof_hook_end(h, 8.0 * SS_S + 8.0 * (B - 1), st);
         h = prof_hook_begin(9, st);
         const size_t tiles = (n + SS_TILE - 1) / SS_TILE;
         hipLaunchKernelGGL(k_ss_part, dim3((unsigned)tiles), dim3(SS_T), 0, st, keys, n,
-                           (const uint64_t *)s.ss_spl, (const uint16_t *)s.ss_guide,
-                           (const uint64_t *)s.ss_gparm, (int)B, part, s.ss_cnt, d_err);
+                           (const uint64_t *)s.ss_spl, (const unsigned long long *)s.ss_mnx, (int)B, part,
+                           s.ss_cnt, d_err);
         snote(hipGetLastError());
         prof_hook_end(h, 16.0 * n, st);
     }
