@@ -441,12 +441,13 @@ def run_autotune(cg, args):
         cg.tune_set("carry_stream", args.carry_stream)
         cg.tune_set("sample_sort", args.sample_sort)
         cg.tune_set("canon_stab", args.canon_stab)
+        cg.tune_set("one_sync", args.one_sync)
         if args.sample_bucket:
             cg.tune_set("sample_bucket", args.sample_bucket)
         return {"source": "built-in defaults: 8-bit digits, 8192-key tiles, window 16, dir delta 0",
                 "cell_code": args.cell_code, "carry_stream": args.carry_stream,
                 "sample_sort": args.sample_sort, "sample_bucket": cg.tune_get("sample_bucket"),
-                "canon_stab": args.canon_stab}
+                "canon_stab": args.canon_stab, "one_sync": args.one_sync}
     if mode == "stored" and os.path.exists(cg.TUNED_TABLE):
         rep = cg.load_tuning()
         src = "stored autotuner table cablegen_amd/tuned/mi355x.json"
@@ -460,6 +461,7 @@ def run_autotune(cg, args):
     cg.tune_set("carry_stream", args.carry_stream)
     cg.tune_set("sample_sort", args.sample_sort)
     cg.tune_set("canon_stab", args.canon_stab)
+    cg.tune_set("one_sync", args.one_sync)
     if args.sample_bucket:
         cg.tune_set("sample_bucket", args.sample_bucket)
     dir_delta = rep["dir"]["pick"]
@@ -472,7 +474,8 @@ def run_autotune(cg, args):
     return {"source": src, "seconds": time.perf_counter() - t0, "dir_delta": dir_delta,
             "sort_picks_bits_x_kpt/window": sel, "cell_code": args.cell_code,
             "carry_stream": args.carry_stream, "sample_sort": args.sample_sort,
-            "sample_bucket": cg.tune_get("sample_bucket"), "canon_stab": args.canon_stab}
+            "sample_bucket": cg.tune_get("sample_bucket"), "canon_stab": args.canon_stab,
+            "one_sync": args.one_sync}
 
 
 def census(args):
@@ -546,23 +549,55 @@ def batch_bench(args):
     one_at_a_time()
     batched()
     c0 = cg.tune_get("cell_code_sorts")
+    r0 = cg.tune_get("sample_repeats")
     n1, t_one = one_at_a_time()
     c1 = cg.tune_get("cell_code_sorts")
+    r1 = cg.tune_get("sample_repeats")
+    # the one-at-a-time leg's kernels (an untimed pass with every kernel timed)
     cg.prof_enable(True)
+    cg.prof_select(None)
     cg.prof_reset()
-    nb, tg, ts = batched()
-    kern = {k: cg.prof_query(k) for k in ("move", "spawn", "sort_hist", "sort_pass", "unique")}
+    one_at_a_time()
+    kern_one = {k: cg.prof_query(k) for k in ("move", "spawn", "sort_split", "sort_part", "sort_local",
+                                              "sort_hist", "sort_pass", "unique", "solve")}
     cg.prof_enable(False)
+    g0, i0 = cg.tune_get("graph_steps"), cg.tune_get("graph_instantiations")
+    nb, tg, ts = batched()
+    graph_steps = cg.tune_get("graph_steps") - g0
+    graph_inst = cg.tune_get("graph_instantiations") - i0
     c2 = cg.tune_get("cell_code_sorts")
+    r2 = cg.tune_get("sample_repeats")
     assert n1 == nb
+    # A/B: the same batched launches enqueued one by one instead of one graph
+    # per layer step; then one such run with every kernel timed (event
+    # records stay out of graphs) for the per-kernel breakdown
+    try:
+        cg.tune_set("batch_graph", 0)
+        _, tg_eager, ts_eager = batched()
+        cg.prof_enable(True)
+        cg.prof_select(None)
+        cg.prof_reset()
+        batched()
+        kern = {k: cg.prof_query(k) for k in ("move", "spawn", "sort_split", "sort_part", "sort_local",
+                                              "sort_hist", "sort_pass", "unique")}
+        cg.prof_enable(False)
+    finally:
+        cg.tune_set("batch_graph", 1)
     sizes = sorted((B.total_boards_table(t) for t in range(len(cfgs))), reverse=True)
     out = {"metric": "batched tablebases: boards/s, batched vs one at a time",
            "tables": len(cfgs), "boards": nb, "one_at_a_time_s": t_one,
            "batched_generate_s": tg, "batched_solve_s": ts,
            "one_at_a_time_boards_per_s": n1 / t_one, "batched_boards_per_s": nb / (tg + ts),
            "speedup": t_one / (tg + ts),
+           "batched_step_graphs": graph_steps, "graph_instantiations": graph_inst,
+           "batched_eager_generate_s": tg_eager, "batched_eager_solve_s": ts_eager,
+           "speedup_eager_steps": t_one / (tg_eager + ts_eager),
            "largest_tables": sizes[:5], "median_table": sizes[len(sizes) // 2],
            "batched_kernels_ms": {k: v[1] for k, v in kern.items()},
+           "one_at_a_time_kernels": {k: [v[0], v[1]] for k, v in kern_one.items()},
+           "sample_sort_repeats": {"one_at_a_time": r1 - r0, "batched": r2 - r1},
+           "one_sync": cg.tune_get("one_sync"),
+           "batched_kernels_source": "one batched run with its launches enqueued one by one, every kernel timed",
            "batched_sorts": kern["sort_hist"][0], "batched_sort_passes": kern["sort_pass"][0],
            "cell_coded_sorts": {"one_at_a_time": c1 - c0, "batched": c2 - c1},
            "tuning": tuning,
@@ -750,6 +785,9 @@ def main():
                     help="mean keys per bucket of the sample sort (0: the library's default)")
     ap.add_argument("--cell-code", type=int, choices=(0, 1), default=1,
                     help="1: layer sorts code pairs of cells when the tile sum bounds them (default)")
+    ap.add_argument("--one-sync", type=int, choices=(0, 1), default=1,
+                    help="1: layer steps with one host synchronisation (M sort sized on the device; "
+                         "default); 0: the candidate count read before the M sort")
     ap.add_argument("--canon-stab", type=int, choices=(0, 1), default=1,
                     help="1: canonical forms through the formation's stabiliser (default); 0: max of 8 images")
     args = ap.parse_args()

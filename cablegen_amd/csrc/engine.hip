@@ -79,12 +79,21 @@ struct Engine {
     bool init = false;
     hipStream_t st = nullptr;
     uint16_t *lut[2] = { nullptr, nullptr };     /* left-move table per ignore_f */
-    unsigned long long *d_cnt = nullptr;          /* device counters [8] */
+    /* device counters [CNT_N]: [0..2] a producer's count / OR / AND, [5..7]
+     * sort counts (6: the carry C', 7: the layer M_s), [8..12] the spawn's
+     * count / OR / AND pairs of a layer step */
+    unsigned long long *d_cnt = nullptr;
     unsigned long long *h_cnt = nullptr;          /* host copy of the counters */
-    unsigned *d_err = nullptr;                    /* sort wait-budget / batch formation flags */
-    unsigned long long *h_pub = nullptr;          /* pinned, mapped: counters [0..8), flags [8] */
+    /* error words: [0] the main stream's sorts and the batch's formation
+     * flags, [1] the second stream's sorts */
+    unsigned *d_err = nullptr;
+    unsigned long long *h_pub = nullptr;          /* pinned, mapped: counters [0..CNT_N), flags [CNT_N..+2) */
     unsigned long long *hd_pub = nullptr;         /* its device address */
     Buf A, B, Cb;                                 /* candidate / sort buffers */
+    /* spawn-4 buffers of the layer steps, used in turn: the pending carry
+     * sort's input stays intact while the next step's spawns write the
+     * other one, so a carry sort found oversize after them can be repeated */
+    Buf S4[2];
     void *scratch = nullptr;
     size_t scratch_cap = 0;
     /* a second stream for the carry sort, which runs beside the next
@@ -92,6 +101,7 @@ struct Engine {
      * main stream after it */
     hipStream_t st2 = nullptr;
     hipEvent_t ev2 = nullptr;
+    hipEvent_t evf = nullptr;   /* orders the second stream after the main one (a fork) */
     Buf B2;
     void *scratch2 = nullptr;
     size_t scratch2_cap = 0;
@@ -100,9 +110,13 @@ struct Engine {
     /* the sample sort's bucket buffers of the two lanes */
     Buf Pt, Pt2;
     /* a sample sort enqueued before the last counter read had an oversize
-     * bucket: its output is incomplete and the sort must be repeated */
-    bool oversize = false;
+     * bucket: its output is incomplete and the sort must be repeated
+     * (oversize: a sort reporting into either error word; oversize1: the
+     * first word's, oversize2: the carry sorts' word).  Every read sets all
+     * three afresh. */
+    bool oversize = false, oversize1 = false, oversize2 = false;
 };
+constexpr int CNT_N = 24;
 static Engine E;
 
 static void tuning_load_default();
@@ -114,14 +128,16 @@ static int eng_init()
     HC(hipStreamCreate(&E.st));
     HC(hipStreamCreate(&E.st2));
     HC(hipEventCreateWithFlags(&E.ev2, hipEventDisableTiming));
-    HC(hipMalloc((void **)&E.d_cnt, 8 * sizeof(unsigned long long)));
-    E.h_cnt = (unsigned long long *)calloc(8, sizeof(unsigned long long));
+    HC(hipEventCreateWithFlags(&E.evf, hipEventDisableTiming));
+    HC(hipMalloc((void **)&E.d_cnt, CNT_N * sizeof(unsigned long long)));
+    HC(hipMemset(E.d_cnt, 0, CNT_N * sizeof(unsigned long long)));
+    E.h_cnt = (unsigned long long *)calloc(CNT_N, sizeof(unsigned long long));
     if (!E.h_cnt) return fail_msg("host allocation failed");
     HC(hipMalloc((void **)&E.d_err, 64));
     HC(hipMemset(E.d_err, 0, 64));
-    HC(hipHostMalloc((void **)&E.h_pub, 16 * sizeof(unsigned long long),
+    HC(hipHostMalloc((void **)&E.h_pub, (CNT_N + 8) * sizeof(unsigned long long),
                      hipHostMallocMapped | hipHostMallocCoherent));
-    memset(E.h_pub, 0, 16 * sizeof(unsigned long long));
+    memset(E.h_pub, 0, (CNT_N + 8) * sizeof(unsigned long long));
     HC(hipHostGetDevicePointer((void **)&E.hd_pub, E.h_pub, 0));
     tuning_load_default();
     E.init = true;
@@ -251,9 +267,12 @@ static double g_ms_s[2][PK_N], g_bytes_s[2][PK_N];
 static long g_launches_s[2][PK_N];
 
 /* whether launches of this kernel kind are being timed */
+/* a stream capture is open: kernel timers (event records) stay out of it */
+static bool g_capturing = false;
+
 static bool prof_wants(int kind)
 {
-    return g_prof && (g_prof_only < 0 || kind == g_prof_only ||
+    return g_prof && !g_capturing && (g_prof_only < 0 || kind == g_prof_only ||
                       (g_prof_only == PK_GROUP_SORT && is_sort_kind(kind)));
 }
 
@@ -473,29 +492,69 @@ static MoveRules move_rules(const cg_rules &r)
 /* sorts repeated on the LSD path after an oversize bucket */
 static unsigned long long g_oversize_repeats = 0;
 
+/* the last sort enqueued on each error word, named in an error message */
+struct SortErrNote {
+    int slot = -1;
+    size_t n = 0;
+    uint64_t vary = 0, cell_max = 0;
+    int sample = 0;
+};
+static SortErrNote g_err_note[2];
+
+static int fail_sort_word(const char *what, unsigned e1, unsigned e2)
+{
+    char buf[512];
+    const int w = e1 ? 0 : 1;
+    const SortErrNote &en = g_err_note[w];
+    snprintf(buf, sizeof buf,
+             "%s (error words %#x %#x; last sort on word %d: count slot %d, %zu keys, key bits %#llx, "
+             "cell bound %#llx, mode %d)",
+             what, e1, e2, w, en.slot, en.n, (unsigned long long)en.vary, (unsigned long long)en.cell_max,
+             en.sample);
+    return fail_msg(buf);
+}
+
 /* copies device counters [lo, lo+n) to E.h_cnt, synchronises and checks the
  * sort error flag */
-static int read_counters(int lo, int n)
+static void publish_counters(int lo, int n)
 {
     /* one small kernel stores the counters and the flags into mapped host
      * memory (instead of two copy dispatches) */
-    launch_publish(E.st, E.d_cnt, lo, n, E.d_err, E.hd_pub);
+    launch_publish(E.st, E.d_cnt, lo, n, E.d_err, E.hd_pub, CNT_N);
+}
+
+/* waits for the main stream and takes the counters [lo, lo+n) and the error
+ * words that publish_counters (or a graph holding it) stored */
+static int collect_counters(int lo, int n)
+{
+    /* the oversize flags describe this read only */
+    E.oversize = E.oversize1 = E.oversize2 = false;
     if (check_sync(E.st, "counters")) return -1;
     for (int i = lo; i < lo + n; i++) E.h_cnt[i] = ((volatile unsigned long long *)E.h_pub)[i];
-    unsigned e = (unsigned)((volatile unsigned long long *)E.h_pub)[8];
+    const unsigned e1 = (unsigned)((volatile unsigned long long *)E.h_pub)[CNT_N];
+    const unsigned e2 = (unsigned)((volatile unsigned long long *)E.h_pub)[CNT_N + 1];
+    const unsigned e = e1 | e2;
     if (e) {
         launch_fill_u64(E.st, (uint64_t *)E.d_err, 1, 0);
         hipStreamSynchronize(E.st);
         if (e == SORT_ERR_OVERSIZE) {   /* the caller repeats that sort */
             E.oversize = true;
-            g_oversize_repeats++;
+            E.oversize1 = e1 != 0;
+            E.oversize2 = e2 != 0;
+            g_oversize_repeats += (e1 != 0) + (e2 != 0);
             return 0;
         }
         if (e & 2u) return fail_msg("batch: a board's formation cells left the table's canonical place");
-        if (e & SORT_ERR_CODE) return fail_msg("sort: a key's cell value exceeds the bound of its layer");
-        return fail_msg("sort: a chained-scan wait exceeded its budget");
+        if (e & SORT_ERR_CODE) return fail_sort_word("sort: a key's cell value exceeds the bound of its layer", e1, e2);
+        return fail_sort_word("sort: a chained-scan wait exceeded its budget", e1, e2);
     }
     return 0;
+}
+
+static int read_counters(int lo, int n)
+{
+    publish_counters(lo, n);
+    return collect_counters(lo, n);
 }
 
 static uint64_t vary_of(size_t n, uint64_t o, uint64_t a) { return n > 1 ? (o & ~a) : 0; }
@@ -504,21 +563,33 @@ static uint64_t vary_of(size_t n, uint64_t o, uint64_t a) { return n > 1 ? (o & 
 /* lane 0: the main stream; lane 1: the second stream (its own ping-pong
  * buffer, scratch and sort context) -- the caller orders the main stream
  * after it (join_lane1) before using its output */
-/* allow_sample = false: the LSD path (the repeat of a sample sort whose
+/* sample = SORT_LSD: the LSD path (the repeat of a sample sort whose
  * bucket overflowed); the sample sort leaves buf unchanged, so a repeat can
  * read it again */
 static int sort_enqueue(uint64_t *buf, size_t n, uint64_t vary, uint64_t *dst, int slot, int *hu,
-                        int lane = 0, uint64_t cell_max = CELLS_UNBOUNDED, bool allow_sample = true)
+                        int lane = 0, uint64_t cell_max = CELLS_UNBOUNDED, int sample = SORT_SAMPLE)
 {
-    const size_t pk = allow_sample ? sort_part_keys(n) : 0;
+    const size_t pk = sort_part_keys(n, sample);
+    /* the carry sorts (count slot 6) report into the second error word, the
+     * others into the first, whichever stream runs them */
+    const int ew = slot == 6 ? 1 : 0;
+    unsigned *err = E.d_err + ew;
+    {
+        SortErrNote &en = g_err_note[ew];
+        en.slot = slot;
+        en.n = n;
+        en.vary = vary;
+        en.cell_max = cell_max;
+        en.sample = sample;
+    }
     if (lane == 1) {
         if (ensure(E.B2, n + 1) || ensure_scratch2(n) || (pk && ensure(E.Pt2, pk))) return -1;
-        sort_unique(E.st2, buf, E.B2.p, n, vary, E.scratch2, dst, E.d_cnt + slot, E.d_err, hu, 1,
-                    cell_max, E.Pt2.p, E.Pt2.cap, allow_sample);
+        sort_unique(E.st2, buf, E.B2.p, n, vary, E.scratch2, dst, E.d_cnt + slot, err, hu, 1,
+                    cell_max, E.Pt2.p, E.Pt2.cap, sample);
     } else {
         if (ensure(E.B, n + 1) || ensure_scratch(n) || (pk && ensure(E.Pt, pk))) return -1;
-        sort_unique(E.st, buf, E.B.p, n, vary, E.scratch, dst, E.d_cnt + slot, E.d_err, hu, 0,
-                    cell_max, E.Pt.p, E.Pt.cap, allow_sample);
+        sort_unique(E.st, buf, E.B.p, n, vary, E.scratch, dst, E.d_cnt + slot, err, hu, 0,
+                    cell_max, E.Pt.p, E.Pt.cap, sample);
     }
     hipError_t e = kernels_take_error();
     if (e != hipSuccess) return fail("sort_unique", e);
@@ -530,6 +601,66 @@ static int join_lane1()
 {
     HC(hipEventRecord(E.ev2, E.st2));
     HC(hipStreamWaitEvent(E.st, E.ev2, 0));
+    return 0;
+}
+
+/* later work on the second stream waits for everything enqueued on the main
+ * one (inside a capture this brings the second stream into the graph) */
+static int fork_lane1()
+{
+    HC(hipEventRecord(E.evf, E.st));
+    HC(hipStreamWaitEvent(E.st2, E.evf, 0));
+    return 0;
+}
+
+/* ---- layer steps captured in a graph ---- */
+
+/* batched steps as one graph launch each (knob batch_graph; 0: the same
+ * launches enqueued one by one) */
+static int g_batch_graph = 1;
+static unsigned long long g_graph_steps = 0, g_graph_instantiations = 0;
+
+static int capture_begin()
+{
+    HC(hipStreamBeginCapture(E.st, hipStreamCaptureModeRelaxed));
+    g_capturing = true;
+    return 0;
+}
+
+/* ends the capture on the main stream and launches the graph through *exec:
+ * updated in place when the new graph has the same shape, else instantiated
+ * anew.  With ok = false the capture is only ended and dropped. */
+static int capture_end_launch(hipGraphExec_t *exec, bool ok)
+{
+    g_capturing = false;
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(E.st, &g);
+    if (!ok) {
+        if (g) hipGraphDestroy(g);
+        return -1;
+    }
+    if (e != hipSuccess) return fail("hipStreamEndCapture", e);
+    if (*exec) {
+        hipGraphNode_t bad = nullptr;
+        hipGraphExecUpdateResult res = hipGraphExecUpdateSuccess;
+        if (hipGraphExecUpdate(*exec, g, &bad, &res) != hipSuccess || res != hipGraphExecUpdateSuccess) {
+            (void)hipGetLastError();
+            hipGraphExecDestroy(*exec);
+            *exec = nullptr;
+        }
+    }
+    if (!*exec) {
+        e = hipGraphInstantiate(exec, g, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            hipGraphDestroy(g);
+            *exec = nullptr;
+            return fail("hipGraphInstantiate", e);
+        }
+        g_graph_instantiations++;
+    }
+    hipGraphDestroy(g);
+    HC(hipGraphLaunch(*exec, E.st));
+    g_graph_steps++;
     return 0;
 }
 
@@ -554,9 +685,12 @@ struct StepCounts {
     size_t sumE = 0;         /* input size of the carry sort (upper bound of the count) */
     int hu4 = -1;            /* timer handle of its compaction, bytes set on arrival */
     size_t last_nM = 0;      /* previous layer size: sizes the spawn buffers */
-    /* the pending carry sort's input (in E.Cb until the next spawn), key bits
-     * and cell bound: what its repeat needs */
+    size_t emax = 16;        /* empty cells a board of the previous layer may have */
+    /* the pending carry sort's input, key bits and cell bound: what its
+     * repeat needs; s4 = the spawn-4 buffer the next step writes */
+    const uint64_t *cin = nullptr;
     uint64_t vary4 = 0, cell4 = CELLS_UNBOUNDED;
+    int s4 = 0;
 };
 
 /* spawn buffers are sized before |M_s| is known: for about twice the previous
@@ -577,13 +711,14 @@ static void counts_arrived(StepCounts &pc, size_t nC)
     pc.pending = false;
 }
 
-/* The pending carry sort (input still in E.Cb) had an oversize bucket: it
+/* The pending carry sort (input still at pc.cin) had an oversize bucket: it
  * is repeated on the LSD path into C, on the main stream. */
 static int carry_repeat(StepCounts &pc, uint64_t *C)
 {
     E.oversize = false;
     int hu4;
-    if (sort_enqueue(E.Cb.p, pc.sumE, pc.vary4, C, 6, &hu4, 0, pc.cell4, false) || read_counters(6, 1))
+    if (sort_enqueue(const_cast<uint64_t *>(pc.cin), pc.sumE, pc.vary4, C, 6, &hu4, 0, pc.cell4, SORT_LSD) ||
+        read_counters(6, 1))
         return -1;
     if (E.oversize) return fail_msg("sort: an LSD sort reported an oversize bucket");
     pc.hu4 = hu4;
@@ -652,18 +787,50 @@ static uint64_t cell_bound(const ValueBound *v, long s, uint64_t key_and)
     return lg <= 14 ? cells_bounded_by(lg) : CELLS_UNBOUNDED;
 }
 
+/* the sample sort of the *dn keys at buf (at most nmax of them) into dst,
+ * sized on the device, on the main stream; its count lands in d_cnt[slot] */
+static int sort_enqueue_dev(uint64_t *buf, const unsigned long long *dn, size_t nmax, uint64_t *dst, int slot,
+                            int *hu)
+{
+    const size_t pk = sort_part_keys(nmax, SORT_SAMPLE);
+    if (ensure_scratch(nmax) || (pk && ensure(E.Pt, pk))) return -1;
+    g_err_note[0] = SortErrNote();
+    g_err_note[0].slot = slot;
+    g_err_note[0].n = nmax;
+    g_err_note[0].sample = 3;   /* sized on the device */
+    if (sort_unique_dev(E.st, buf, dn, nmax, E.scratch, dst, E.d_cnt + slot, E.d_err, hu, 0, E.Pt.p, E.Pt.cap)) {
+        hipError_t e = kernels_take_error();
+        return e != hipSuccess ? fail("sort_unique_dev", e) : fail_msg("sort: device-sized sort refused");
+    }
+    hipError_t e = kernels_take_error();
+    if (e != hipSuccess) return fail("sort_unique_dev", e);
+    return 0;
+}
+
+/* one host synchronisation per layer step when the M sort can be sized on
+ * the device (knob one_sync; 0: the count of the candidates comes to the
+ * host before their sort, two synchronisations per step) */
+static int g_one_sync = 1;
+
 /* One layer step (generate.c:409-550): the pre-move boards P and the carried
  * spawn-4 set C give M_s = uniq(moves(P)) (from `la`), the next pre-move
- * boards P_{s+2} = C u S2 (written to nxtP) and the new carry C' = uniq(S4)
+ * boards P_{s+2} = S2 u C (written to nxtP) and the new carry C' = uniq(S4)
  * (written to nxtC).  P is a multiset: the moves of a multiset have the same
  * set of results, and M is sorted and deduplicated anyway, so P_{s+2} needs
  * no sort of its own (its duplicates, about 15% on DPDGap, cost less in the
- * move kernel and the M sort than sorting all of it would).  Two host
- * synchronisations: after the moves (which also brings the previous step's
- * carry count), and after the spawns (which brings |M_s|).  When pc.pending,
- * nC is an upper bound and the exact count is on the device; on return *nC2
- * is an upper bound, pc.pending is set and *nP2 is exact.  vb (optional)
- * bounds the cell values for the sorts and advances to the next step's sum. */
+ * move kernel and the M sort than sorting all of it would).
+ *
+ * The M sort takes the candidate count from the device (the sample sort
+ * sized for 4 |P| candidates), M is allocated for that bound and shrunk
+ * later, and the spawns read |M| on the device, so the step synchronises
+ * once, after the spawns: that read brings the candidate count, |M|, the
+ * spawn count and the previous step's carry count together, and a sort with
+ * an oversize bucket or spawn buffers that were too small are repeated then.
+ * When the bound is beyond the sample sort, the candidate count comes to the
+ * host first (two synchronisations).  When pc.pending, nC is an upper bound
+ * and the exact count is on the device; on return *nC2 is an upper bound,
+ * pc.pending is set and *nP2 is exact.  vb (optional) bounds the cell values
+ * for the LSD sorts and advances to the next step's sum. */
 static int gen_step(const cg_rules &r, const uint64_t *P, size_t nP, uint64_t *C, size_t nC,
                     StepCounts &pc, LayerAlloc &la, uint64_t **M, size_t *nM,
                     Buf &nxtP, size_t *nP2, Buf &nxtC, size_t *nC2, ValueBound *vb,
@@ -673,8 +840,9 @@ static int gen_step(const cg_rules &r, const uint64_t *P, size_t nP, uint64_t *C
     const long s = vb ? vb->s : 0;
     const uint16_t *lut = get_lut(r.ignore_f);
     if (!lut) return fail_msg("row table upload failed");
-    /* moves */
-    if (ensure(E.A, 4 * nP + 1)) return -1;
+    /* moves: at most 4 candidates per board (premove: the boards themselves) */
+    const size_t cmax = premoved ? nP : 4 * nP;
+    if (ensure(E.A, cmax + 1)) return -1;
     launch_init_stats(E.st, E.d_cnt);
     int h = prof_begin(PK_MOVE);
     if (premoved) {
@@ -691,63 +859,98 @@ static int gen_step(const cg_rules &r, const uint64_t *P, size_t nP, uint64_t *C
         launch_move(E.st, P, nP, nullptr, E.A.p, E.d_cnt, lut, mr, 1);
     }
     prof_end(h, 0);
-    /* the previous step's carry sort (second stream) overlapped the moves;
-     * its count and output are needed from here on */
-    if (pc.pending && join_lane1()) return -1;
-    if (read_counters(0, pc.pending ? 7 : 3)) return -1;
-    size_t ncand = E.h_cnt[0];
-    uint64_t mor = E.h_cnt[1], mand = E.h_cnt[2];
-    if (E.oversize) {
-        /* only the carry sort can be pending here (the M sort of the last
-         * step was checked at its spawns) */
-        if (!pc.pending || carry_repeat(pc, C)) return pc.pending ? -1 : fail_msg("sort: unexpected oversize");
-    }
-    if (pc.pending) {
-        nC = E.h_cnt[6];
-        counts_arrived(pc, nC);
-    }
-    if (m_or) *m_or = mor;
-    if (m_and) *m_and = mand;
-    prof_set_bytes(h, 8.0 * nP + 8.0 * ncand);
     /* M_s = uniq(candidates); its count goes to d_cnt[7] */
-    *M = la.alloc(ncand);
-    if (!*M) return fail_msg("layer allocation failed");
-    int hu;
-    const uint64_t mvary = vary_of(ncand, mor, mand), mcell = cell_bound(vb, s, mand);
-    if (sort_enqueue(E.A.p, ncand, mvary, *M, 7, &hu, 0, mcell)) return -1;
-    /* spawns: every board has at most 16 - (cells nonzero in all of M_s)
-     * empties; the kernel reads |M_s| from the device, the grid and the
-     * buffers cover ncand */
-    size_t emax = (size_t)(16 - nonzero_nibbles(ncand ? mand : ~0ULL));
-    size_t cap = emax * spawn_rows_estimate(pc, ncand);
+    const bool dev = g_one_sync && cmax <= sort_sample_limit(SORT_SAMPLE);
+    size_t ncand = 0, mcap = 0;
+    uint64_t mor = 0, mand = ~0ULL;
+    int hu = -1;
+    if (dev) {
+        mcap = cmax;
+        *M = la.alloc(mcap);
+        if (!*M) return fail_msg("layer allocation failed");
+        if (sort_enqueue_dev(E.A.p, E.d_cnt, cmax, *M, 7, &hu)) return -1;
+    } else {
+        /* the candidate count first; the previous step's carry sort
+         * (second stream) overlapped the moves and is settled here too */
+        if (pc.pending && join_lane1()) return -1;
+        if (read_counters(0, pc.pending ? 7 : 3)) return -1;
+        ncand = E.h_cnt[0];
+        mor = E.h_cnt[1];
+        mand = E.h_cnt[2];
+        if (pc.pending) {
+            if (E.oversize && E.oversize1) return fail_msg("sort: unexpected oversize");
+            if (E.oversize && E.oversize2 && carry_repeat(pc, C)) return -1;
+            nC = E.h_cnt[6];
+            counts_arrived(pc, nC);
+        }
+        mcap = ncand;
+        *M = la.alloc(mcap);
+        if (!*M) return fail_msg("layer allocation failed");
+        if (sort_enqueue(E.A.p, ncand, vary_of(ncand, mor, mand), *M, 7, &hu, 0, cell_bound(vb, s, mand)))
+            return -1;
+    }
+    /* spawns of M_s (the kernel reads |M_s| on the device; the grid covers
+     * the bound): S2 to the front of nxtP, S4 to the spawn-4 buffer the
+     * pending carry sort does not read */
+    Buf &S4 = E.S4[pc.s4];
+    const size_t mrows = mcap;
+    size_t emax = dev ? pc.emax : (size_t)(16 - nonzero_nibbles(ncand ? mand : ~0ULL));
+    size_t cap = emax * spawn_rows_estimate(pc, mrows);
+    const size_t cbound = pc.pending ? pc.sumE : nC;   /* bound on the carry count */
     int hs = -1;
     size_t sumE = 0;
-    for (int attempt = 0; attempt < 3; attempt++) {
-        if (ensure(nxtP, nC + cap + 1) || ensure(E.Cb, cap + 1)) return -1;
-        if (nC) HC(hipMemcpyAsync(nxtP.p, C, nC * sizeof(uint64_t), hipMemcpyDeviceToDevice, E.st));
-        launch_init_stats(E.st, E.d_cnt);
+    bool settled = false;
+    for (int attempt = 0; attempt < 4 && !settled; attempt++) {
+        if (ensure(nxtP, cap + cbound + 1) || ensure(S4, cap + 1)) return -1;
+        launch_init_stats(E.st, E.d_cnt + 8);
         hs = prof_begin(PK_SPAWN);
-        launch_spawn(E.st, *M, ncand, E.d_cnt + 7, nxtP.p + nC, E.Cb.p, cap, E.d_cnt, cs);
+        launch_spawn(E.st, *M, mrows, E.d_cnt + 7, nxtP.p, S4.p, cap, E.d_cnt + 8, cs);
         prof_end(hs, 0);
-        if (read_counters(0, 8)) return -1;
-        if (E.oversize) {
+        /* the previous step's carry sort (second stream) overlapped this
+         * step so far; its count is read with the others */
+        if (pc.pending && join_lane1()) return -1;
+        if (read_counters(0, 13)) return -1;
+        const bool ov_m = E.oversize && E.oversize1, ov_c = E.oversize && E.oversize2;
+        E.oversize = false;
+        if (dev && attempt == 0) {
+            ncand = E.h_cnt[0];
+            mor = E.h_cnt[1];
+            mand = E.h_cnt[2];
+        }
+        if (pc.pending) {
+            /* the carry sort of the previous step: repeated on the LSD path
+             * when a bucket overflowed */
+            if (ov_c && carry_repeat(pc, C)) return -1;
+            nC = E.h_cnt[6];
+            counts_arrived(pc, nC);
+        } else if (ov_c) {
+            return fail_msg("sort: unexpected oversize");
+        }
+        if (ov_m) {
             /* the M sort had an oversize bucket: repeat it on the LSD path
              * (the candidates are unchanged), then the spawns */
-            E.oversize = false;
-            if (sort_enqueue(E.A.p, ncand, mvary, *M, 7, &hu, 0, mcell, false)) return -1;
+            if (sort_enqueue(E.A.p, ncand, vary_of(ncand, mor, mand), *M, 7, &hu, 0, cell_bound(vb, s, mand),
+                             SORT_LSD))
+                return -1;
             continue;
         }
         *nM = E.h_cnt[7];
-        sumE = E.h_cnt[0];
-        if (sumE <= cap) break;
-        cap = emax * *nM;   /* exact bound, always enough */
+        sumE = E.h_cnt[8];
+        if (sumE <= cap) settled = true;
+        else cap = (size_t)(16 - nonzero_nibbles(*nM ? mand : ~0ULL)) * *nM;   /* exact bound, always enough */
     }
-    if (E.oversize || sumE > cap) return fail_msg("generation step: spawns or sort not settled");
+    if (!settled) return fail_msg("generation step: spawns or sort not settled");
+    if (m_or) *m_or = mor;
+    if (m_and) *m_and = mand;
     pc.last_nM = *nM;
+    pc.emax = (size_t)(16 - nonzero_nibbles(*nM ? mand : ~0ULL));
+    prof_set_bytes(h, 8.0 * nP + 8.0 * ncand);
     prof_set_bytes(hu, 8.0 * ncand + 8.0 * *nM);
     prof_set_bytes(hs, 8.0 * *nM + 16.0 * sumE);
-    la.shrink(*M, ncand, *nM);
-    uint64_t or4 = E.h_cnt[3], and4 = E.h_cnt[4];
+    la.shrink(*M, mcap, *nM);
+    /* P_{s+2} = S2 u C: the carry after the 2-spawns */
+    if (nC) HC(hipMemcpyAsync(nxtP.p + sumE, C, nC * sizeof(uint64_t), hipMemcpyDeviceToDevice, E.st));
+    const uint64_t or4 = E.h_cnt[11], and4 = E.h_cnt[12];
     /* C' = uniq(S4) on the second stream (the spawns finished at the
      * synchronisation above); it overlaps the next step's moves, and its
      * count stays on the device until then */
@@ -755,12 +958,14 @@ static int gen_step(const cg_rules &r, const uint64_t *P, size_t nP, uint64_t *C
     int hu4;
     pc.vary4 = vary_of(sumE, or4, and4);
     pc.cell4 = cell_bound(vb, s + 4, and4);
-    if (sort_enqueue(E.Cb.p, sumE, pc.vary4, nxtC.p, 6, &hu4, g_carry_lane, pc.cell4)) return -1;
+    if (sort_enqueue(S4.p, sumE, pc.vary4, nxtC.p, 6, &hu4, g_carry_lane, pc.cell4)) return -1;
+    pc.cin = S4.p;
+    pc.s4 ^= 1;
     if (vb) vb->s = s + 2;
     pc.pending = true;
     pc.sumE = sumE;
     pc.hu4 = hu4;
-    *nP2 = nC + sumE;
+    *nP2 = sumE + nC;
     *nC2 = sumE;
     return 0;
 }
@@ -1214,7 +1419,7 @@ extern "C" int cg_dev_sort_unique_bounded(uint64_t *d_keys, size_t n, uint64_t *
     if (read_counters(5, 1)) return -1;
     if (E.oversize) {   /* a bucket of the sample sort overflowed: the LSD path */
         E.oversize = false;
-        if (sort_enqueue(d_keys, n, vary, d_out, 5, &hu, 0, cell_max, false) || read_counters(5, 1))
+        if (sort_enqueue(d_keys, n, vary, d_out, 5, &hu, 0, cell_max, SORT_LSD) || read_counters(5, 1))
             return -1;
     }
     *n_unique = E.h_cnt[5];
@@ -1444,6 +1649,26 @@ extern "C" int cg_tune_set(const char *knob, int value)
         g_canon_stab = value;
         return 0;
     }
+    if (!strcmp(knob, "one_sync")) {
+        if (value != 0 && value != 1) return fail_msg("one_sync: 0 or 1");
+        g_one_sync = value;
+        return 0;
+    }
+    if (!strcmp(knob, "sample_bmax")) {
+        if (value != 2048 && value != 8192) return fail_msg("sample_bmax: 2048 or 8192");
+        sort_set_sample_limits(value, sort_sample_probes());
+        return 0;
+    }
+    if (!strcmp(knob, "sample_probes")) {
+        if (value < 0 || value > 16384) return fail_msg("sample_probes: 0 (default) or 1..16384");
+        sort_set_sample_limits(sort_sample_bmax(), value);
+        return 0;
+    }
+    if (!strcmp(knob, "batch_graph")) {
+        if (value != 0 && value != 1) return fail_msg("batch_graph: 0 or 1");
+        g_batch_graph = value;
+        return 0;
+    }
     if (!strcmp(knob, "sample_sort")) {
         if (value != 0 && value != 1) return fail_msg("sample_sort: 0 or 1");
         sort_set_sample(value, sort_sample_bucket(), sort_sample_cap());
@@ -1455,7 +1680,7 @@ extern "C" int cg_tune_set(const char *knob, int value)
         return 0;
     }
     if (!strcmp(knob, "sample_cap")) {
-        if (value < 0 || value > 16384) return fail_msg("sample_cap: 0 (16384) or 1..16384");
+        if (value < 0 || value > 131072) return fail_msg("sample_cap: 0 (131072) or 1..131072");
         sort_set_sample(sort_sample_on(), sort_sample_bucket(), value);
         return 0;
     }
@@ -1467,7 +1692,10 @@ extern "C" int cg_tune_set(const char *knob, int value)
         sort_set_cell_code(1);
         g_carry_lane = 1;
         g_canon_stab = 1;
+        g_one_sync = 1;
+        g_batch_graph = 1;
         sort_set_sample(1, 5000, 0);
+        sort_set_sample_limits(2048, 0);
         if (!strcmp(knob, "reset") && !g_tune_path.empty()) return cg_tune_load(g_tune_path.c_str());
         return 0;
     }
@@ -1485,6 +1713,13 @@ extern "C" int cg_tune_get(const char *knob)
     if (!strcmp(knob, "cell_code")) return sort_cell_code();
     if (!strcmp(knob, "carry_stream")) return g_carry_lane;
     if (!strcmp(knob, "canon_stab")) return g_canon_stab;
+    if (!strcmp(knob, "one_sync")) return g_one_sync;
+    if (!strcmp(knob, "batch_graph")) return g_batch_graph;
+    if (!strcmp(knob, "sample_bmax")) return sort_sample_bmax();
+    if (!strcmp(knob, "sample_probes")) return sort_sample_probes();
+    if (!strcmp(knob, "graph_steps")) return (int)std::min<unsigned long long>(g_graph_steps, 0x7fffffff);
+    if (!strcmp(knob, "graph_instantiations"))
+        return (int)std::min<unsigned long long>(g_graph_instantiations, 0x7fffffff);
     if (!strcmp(knob, "cell_code_sorts")) return (int)std::min<unsigned long long>(sort_code_count(), 0x7fffffff);
     if (!strcmp(knob, "sample_sort")) return sort_sample_on();
     if (!strcmp(knob, "sample_bucket")) return sort_sample_bucket();
@@ -2016,6 +2251,11 @@ struct cg_batch {
     hipGraphExec_t gexec = nullptr;
     bool solved = false;
     double graph_p4 = -1;
+    /* generation: the layer step's graph, and the per-table statistics in
+     * mapped host memory (3 words per table) */
+    hipGraphExec_t gstep = nullptr;
+    unsigned long long *h_tpub = nullptr, *hd_tpub = nullptr;
+    size_t tpub_cap = 0;
 };
 
 extern "C" cg_batch *cg_batch_create(const cg_rules *rules)
@@ -2047,6 +2287,8 @@ extern "C" void cg_batch_free(cg_batch *b)
     if (!b) return;
     hipStreamSynchronize(E.st);
     batch_drop_graph(b);
+    if (b->gstep) hipGraphExecDestroy(b->gstep);
+    if (b->h_tpub) hipHostFree(b->h_tpub);
     b->arena.release();
     buf_free(b->P);
     buf_free(b->C);
@@ -2182,80 +2424,194 @@ extern "C" int cg_batch_generate(cg_batch *b)
     uint64_t batch_cells = CELLS_UNBOUNDED;
     if (bounded && sum_max >= 1 && tile_log2(sum_max) <= 14)
         batch_cells = tid_max | (cells_bounded_by(tile_log2(sum_max)) >> (64 - BATCH_TID_SHIFT));
-    /* per step: two host synchronisations, as in gen_step */
+    if (b->tpub_cap < (size_t)3 * ntab) {
+        if (b->h_tpub) hipHostFree(b->h_tpub);
+        b->h_tpub = nullptr;
+        b->tpub_cap = 0;
+        HC(hipHostMalloc((void **)&b->h_tpub, 3 * ntab * sizeof(unsigned long long),
+                         hipHostMallocMapped | hipHostMallocCoherent));
+        HC(hipHostGetDevicePointer((void **)&b->hd_tpub, b->h_tpub, 0));
+        b->tpub_cap = 3 * ntab;
+    }
+    /* One host synchronisation per step.  Everything a step enqueues is one
+     * graph launch (knob batch_graph), which also carries the work the
+     * previous step left: the copy of the carry C into the pre-move boards,
+     * the carry sort of the previous spawn-4 boards on the second stream (a
+     * branch of the graph, joined before the spawns) and the directories of
+     * the previous layer.  Sizes the host cannot know before the launch (the
+     * candidate count, |M|, the spawn count) are read on the device: the M
+     * sort is the sample sort sized for 4 |P| candidates, the layer and the
+     * spawn buffers are allocated for bounds. */
+    struct Deferred {
+        /* the carry sort of the previous step's spawn-4 boards */
+        bool carry = false;
+        const uint64_t *c_in = nullptr;
+        size_t c_n = 0;
+        uint64_t c_vary = 0;
+        /* the copy of the carry before it into the pre-move boards */
+        const uint64_t *cp_src = nullptr;
+        uint64_t *cp_dst = nullptr;
+        size_t cp_n = 0;
+        /* this step's launch carried a carry sort (settled at its read) */
+        bool carry_done = false;
+        /* the directories of the previous layer */
+        int dir_step = -1;
+        size_t dir_entries = 0;
+        const size_t *d_seg = nullptr;
+        const BatchIndexPlan *d_plan = nullptr;
+    } df;
+    /* enqueues the deferred work (main stream, and the carry sort forked to
+     * the second one) */
+    /* carry buffers: the carry sort writes cbuf[cw]; the carry copied into
+     * the pre-move boards is in the other */
+    Buf *cbuf[2] = { &b->C, &b->C2 };
+    int cw = 0;
+    auto enqueue_deferred = [&](void) -> int {
+        if (df.cp_n) launch_copy_and_u64(E.st, df.cp_dst, df.cp_src, df.cp_n, ~0ULL);
+        if (df.dir_step >= 0) {
+            const cg_batch::Step &ps = b->steps[df.dir_step];
+            launch_bdir(E.st, ps.keys, ps.n, df.d_seg, ntab, df.d_plan, df.dir_entries, ps.dir, E.scratch);
+        }
+        if (df.carry) {
+            if (g_carry_lane == 1 && fork_lane1()) return -1;
+            int hu4;
+            if (sort_enqueue(const_cast<uint64_t *>(df.c_in), df.c_n, df.c_vary, cbuf[cw]->p, 6, &hu4,
+                             g_carry_lane, batch_cells))
+                return -1;
+            pc.hu4 = hu4;
+        }
+        hipError_t e = kernels_take_error();
+        return e != hipSuccess ? fail("batch step", e) : 0;
+    };
     for (int k = 0; k < b->K; k++) {
         cg_batch::Step st;
-        /* moves of every table (the previous step's counts are still on the device) */
-        if (ensure(E.A, 4 * nP + 1)) return -1;
-        launch_init_stats(E.st, E.d_cnt);
-        int h = prof_begin(PK_MOVE);
-        launch_bmove(E.st, b->P.p, nP, nullptr, E.A.p, E.d_cnt, lut, move_rules(b->r), b->d_tabs,
-                     k, E.d_err);
-        prof_end(h, 0);
-        if (pc.pending && join_lane1()) return -1;
-        if (read_counters(0, pc.pending ? 7 : 3)) return -1;
-        size_t ncand = E.h_cnt[0];
-        uint64_t mor = E.h_cnt[1], mand = E.h_cnt[2];
-        if (E.oversize && (!pc.pending || carry_repeat(pc, b->C.p)))
-            return pc.pending ? -1 : fail_msg("sort: unexpected oversize");
-        if (pc.pending) {
-            nC = E.h_cnt[6];
-            counts_arrived(pc, nC);
-        }
-        prof_set_bytes(h, 8.0 * nP + 8.0 * ncand);
-        if (ensure(b->M, ncand + 1)) return -1;
-        int hu;
-        const uint64_t mvary = vary_of(ncand, mor, mand);
-        if (sort_enqueue(E.A.p, ncand, mvary, b->M.p, 7, &hu, 0, batch_cells)) return -1;
-        /* boards of the step (count on the device until the next synchronisation) */
-        st.keys = (uint64_t *)b->arena.alloc(std::max<size_t>(ncand, 1) * 8);
+        const size_t cmax = 4 * nP;
+        const bool dev = g_one_sync && cmax <= sort_sample_limit(SORT_SAMPLE);
+        const bool graph = dev && g_batch_graph;
+        /* buffers and allocations first: nothing may allocate inside a
+         * capture */
+        const size_t cbound = df.carry ? df.c_n : 0;   /* the carry this step's spawns sit before */
+        size_t cap = (size_t)maxfree * spawn_rows_estimate(pc, cmax);
+        Buf &S4 = E.S4[pc.s4];
+        st.keys = (uint64_t *)b->arena.alloc(std::max<size_t>(cmax, 1) * 8);
         if (!st.keys) return fail_msg("batch layer allocation failed");
-        HC(hipMemcpyAsync(b->d_tstat, init_tstat.data(), 3 * ntab * 8, hipMemcpyHostToDevice, E.st));
-        launch_bdecode(E.st, b->M.p, ncand, E.d_cnt + 7, st.keys, b->d_tabs, b->d_tstat);
-        /* spawns of every table: at most maxfree empty cells per board */
-        size_t cap = (size_t)maxfree * spawn_rows_estimate(pc, ncand);
-        int hs = -1;
-        size_t nM = 0, sumE = 0;
-        for (int attempt = 0; attempt < 3; attempt++) {
-            if (ensure(b->P2, nC + cap + 1) || ensure(E.Cb, cap + 1)) return -1;
-            if (nC) HC(hipMemcpyAsync(b->P2.p, b->C.p, nC * 8, hipMemcpyDeviceToDevice, E.st));
-            launch_init_stats(E.st, E.d_cnt);
-            hs = prof_begin(PK_SPAWN);
-            launch_bspawn(E.st, b->M.p, ncand, E.d_cnt + 7, b->P2.p + nC, E.Cb.p, cap, E.d_cnt,
-                          b->d_tabs, k, E.d_err);
-            prof_end(hs, 0);
-            HC(hipMemcpyAsync(b->h_tstat.data(), b->d_tstat, 3 * ntab * 8, hipMemcpyDeviceToHost,
-                              E.st));
-            if (read_counters(0, 8)) return -1;
-            if (E.oversize) {
-                /* the M sort had an oversize bucket: repeat it on the LSD path,
-                 * the decode and the spawns */
-                E.oversize = false;
-                if (sort_enqueue(E.A.p, ncand, mvary, b->M.p, 7, &hu, 0, batch_cells, false)) return -1;
-                HC(hipMemcpyAsync(b->d_tstat, init_tstat.data(), 3 * ntab * 8, hipMemcpyHostToDevice,
-                                  E.st));
-                launch_bdecode(E.st, b->M.p, ncand, E.d_cnt + 7, st.keys, b->d_tabs, b->d_tstat);
-                continue;
-            }
-            nM = E.h_cnt[7];
-            sumE = E.h_cnt[0];
-            if (sumE <= cap) break;
-            cap = (size_t)maxfree * nM;   /* exact bound, always enough */
+        if (ensure(E.A, cmax + 1) || ensure(b->M, cmax + 1) || ensure(b->P2, cap + cbound + 1) ||
+            ensure(S4, cap + 1) || ensure(*cbuf[cw], cbound + 1))
+            return -1;
+        {
+            size_t scr = sort_scratch_bytes(cmax);
+            if (df.dir_step >= 0) scr = std::max(scr, dir_scratch_bytes(df.dir_entries + 1));
+            if (ensure_scratch_bytes(scr)) return -1;
         }
-        if (E.oversize || sumE > cap) return fail_msg("batch step: spawns or sort not settled");
+        if (dev) {
+            const size_t pk = sort_part_keys(cmax, SORT_SAMPLE);
+            /* look-back words for up to 8192 buckets (the sample sort's
+             * largest), epochs for its launch */
+            if ((pk && ensure(E.Pt, pk)) || sort_reserve(E.st, 0, 8192, 4)) return -1;
+        }
+        if (df.carry) {
+            /* the carry sort's buffers and look-back words (second stream) */
+            const int lane = g_carry_lane;
+            const size_t pk = sort_part_keys(df.c_n, SORT_SAMPLE);
+            if (lane == 1 ? (ensure(E.B2, df.c_n + 1) || ensure_scratch2(df.c_n) || (pk && ensure(E.Pt2, pk)))
+                          : (ensure(E.B, df.c_n + 1) || (pk && ensure(E.Pt, pk))))
+                return -1;
+            if (sort_prepare(lane == 1 ? E.st2 : E.st, lane, df.c_n, df.c_vary, batch_cells, SORT_SAMPLE))
+                return fail_msg("sort: look-back words");
+        }
+        if (graph && capture_begin()) return -1;
+        int rc = enqueue_deferred();
+        if (!rc) {
+            launch_init_stats(E.st, E.d_cnt);
+            launch_bmove(E.st, b->P.p, nP, nullptr, E.A.p, E.d_cnt, lut, move_rules(b->r), b->d_tabs, k, E.d_err);
+        }
+        size_t ncand = 0;
+        uint64_t mor = 0, mand = ~0ULL;
+        int hu = -1;
+        bool carry_ov = false;
+        if (!rc && dev) {
+            rc = sort_enqueue_dev(E.A.p, E.d_cnt, cmax, b->M.p, 7, &hu);
+        } else if (!rc) {
+            /* the bound is beyond the sample sort: the candidate count first */
+            if (df.carry && g_carry_lane == 1) rc = join_lane1();
+            if (!rc) rc = read_counters(0, 3);
+            if (!rc && E.oversize) {
+                /* only the carry sort can have overflowed; it is repeated
+                 * with the counts of the step below */
+                if (E.oversize1) rc = fail_msg("sort: unexpected oversize");
+                carry_ov = E.oversize && E.oversize2;
+                E.oversize = false;
+            }
+            if (!rc) {
+                ncand = E.h_cnt[0];
+                mor = E.h_cnt[1];
+                mand = E.h_cnt[2];
+                rc = sort_enqueue(E.A.p, ncand, vary_of(ncand, mor, mand), b->M.p, 7, &hu, 0, batch_cells);
+            }
+        }
+        const size_t mrows = dev ? cmax : ncand;
+        auto enqueue_decode_spawn = [&](size_t capk) {
+            launch_tstat_init(E.st, b->d_tstat, ntab);
+            launch_bdecode(E.st, b->M.p, mrows, E.d_cnt + 7, st.keys, b->d_tabs, b->d_tstat);
+            launch_init_stats(E.st, E.d_cnt + 8);
+            launch_bspawn(E.st, b->M.p, mrows, E.d_cnt + 7, b->P2.p, S4.p, capk, E.d_cnt + 8, b->d_tabs, k,
+                          E.d_err);
+            launch_copy_and_u64(E.st, (uint64_t *)b->hd_tpub, (const uint64_t *)b->d_tstat, 3 * ntab, ~0ULL);
+        };
+        if (!rc) {
+            enqueue_decode_spawn(cap);
+            if (df.carry && g_carry_lane == 1) rc = join_lane1();
+        }
+        if (!rc) publish_counters(0, 13);
+        if (graph && capture_end_launch(&b->gstep, rc == 0)) return -1;
+        if (rc || collect_counters(0, 13)) return -1;
+        /* the counts of the step (and of the carry sort it carried) */
+        size_t nM = 0, sumE = 0;
+        bool settled = false;
+        for (int attempt = 0; attempt < 4; attempt++) {
+            const bool ov_m = E.oversize && E.oversize1, ov_c = (E.oversize && E.oversize2) || carry_ov;
+            E.oversize = false;
+            carry_ov = false;
+            if (attempt == 0 && dev) {
+                ncand = E.h_cnt[0];
+                mor = E.h_cnt[1];
+                mand = E.h_cnt[2];
+            }
+            if (df.carry) {
+                if (ov_c && carry_repeat(pc, cbuf[cw]->p)) return -1;
+                nC = E.h_cnt[6];
+                counts_arrived(pc, nC);
+                df.carry = false;
+                df.carry_done = true;
+            } else if (ov_c) {
+                return fail_msg("sort: unexpected oversize");
+            }
+            if (!ov_m) {
+                nM = E.h_cnt[7];
+                sumE = E.h_cnt[8];
+                if (sumE <= cap) {
+                    settled = true;
+                    break;
+                }
+                cap = (size_t)maxfree * nM;   /* exact bound, always enough */
+                if (ensure(b->P2, cap + cbound + 1) || ensure(S4, cap + 1)) return -1;
+            } else if (sort_enqueue(E.A.p, ncand, vary_of(ncand, mor, mand), b->M.p, 7, &hu, 0, batch_cells,
+                                    SORT_LSD)) {
+                /* the M sort had an oversize bucket: repeated on the LSD path */
+                return -1;
+            }
+            enqueue_decode_spawn(cap);
+            if (read_counters(0, 13)) return -1;
+        }
+        if (!settled) return fail_msg("batch step: spawns or sort not settled");
         pc.last_nM = nM;
         prof_set_bytes(hu, 8.0 * ncand + 8.0 * nM);
-        prof_set_bytes(hs, 8.0 * nM + 16.0 * sumE);
-        b->arena.shrink_last(st.keys, std::max<size_t>(ncand, 1) * 8, std::max<size_t>(nM, 1) * 8);
-        uint64_t or4 = E.h_cnt[3], and4 = E.h_cnt[4];
-        /* next pre-move boards: C u S2 as written (a multiset, as in gen_step) */
-        size_t n2 = nC + sumE;
-        if (ensure(b->C2, sumE + 1)) return -1;
-        int hu4;
-        pc.vary4 = vary_of(sumE, or4, and4);
-        pc.cell4 = batch_cells;
-        if (sort_enqueue(E.Cb.p, sumE, pc.vary4, b->C2.p, 6, &hu4, g_carry_lane, batch_cells)) return -1;
-        /* directories of every table of this step */
+        b->arena.shrink_last(st.keys, std::max<size_t>(cmax, 1) * 8, std::max<size_t>(nM, 1) * 8);
+        for (int t = 0; t < ntab; t++)
+            for (int j = 0; j < 3; j++) b->h_tstat[3 * t + j] = ((volatile unsigned long long *)b->h_tpub)[3 * t + j];
+        const uint64_t or4 = E.h_cnt[11], and4 = E.h_cnt[12];
+        /* directories of every table of this step: planned now, built at the
+         * head of the next launch */
         st.n = nM;
         st.off.assign(ntab + 1, 0);
         st.plan.resize(ntab);
@@ -2273,27 +2629,49 @@ extern "C" int cg_batch_generate(cg_batch *b)
             entries += st.plan[t].dir_entries;
         }
         if (st.off[ntab] != nM) return fail_msg("batch: per-table counts disagree with the layer");
+        if (check_layer_size(nM)) return -1;
         st.dir = (uint32_t *)b->arena.alloc((entries + 1) * 4);
         size_t *d_seg = (size_t *)b->arena.alloc((ntab + 1) * sizeof(size_t));
         BatchIndexPlan *d_plan = (BatchIndexPlan *)b->arena.alloc(ntab * sizeof(BatchIndexPlan));
         if (!st.dir || !d_seg || !d_plan) return fail_msg("batch index allocation failed");
-        /* the host arrays live in the step, so the asynchronous copies may read them later */
-        HC(hipMemcpyAsync(d_seg, st.off.data(), (ntab + 1) * sizeof(size_t), hipMemcpyHostToDevice,
-                          E.st));
-        HC(hipMemcpyAsync(d_plan, st.bplan.data(), ntab * sizeof(BatchIndexPlan),
-                          hipMemcpyHostToDevice, E.st));
-        if (check_layer_size(nM) || ensure_scratch_bytes(dir_scratch_bytes(entries + 1))) return -1;
-        launch_bdir(E.st, st.keys, nM, d_seg, ntab, d_plan, entries, st.dir, E.scratch);
+        /* the host arrays live in the step, so the asynchronous copies may
+         * read them later; they precede the next launch on the stream */
+        HC(hipMemcpyAsync(d_seg, st.off.data(), (ntab + 1) * sizeof(size_t), hipMemcpyHostToDevice, E.st));
+        HC(hipMemcpyAsync(d_plan, st.bplan.data(), ntab * sizeof(BatchIndexPlan), hipMemcpyHostToDevice, E.st));
         b->steps.push_back(std::move(st));
+        /* what the next launch carries: P_{k+1} = S2 u C (the carry before
+         * this step's, now settled), the carry sort of this step's spawn-4
+         * boards, this layer's directories */
+        const bool had_carry = df.carry_done;
+        df.carry_done = false;
+        df.cp_src = had_carry ? cbuf[cw]->p : nullptr;
+        df.cp_dst = b->P2.p + sumE;
+        df.cp_n = had_carry ? nC : 0;
+        if (had_carry) cw ^= 1;
+        df.carry = true;
+        df.c_in = S4.p;
+        df.c_n = sumE;
+        df.c_vary = vary_of(sumE, or4, and4);
+        df.dir_step = k;
+        df.dir_entries = entries;
+        df.d_seg = d_seg;
+        df.d_plan = d_plan;
         pc.pending = true;
         pc.sumE = sumE;
-        pc.hu4 = hu4;
+        pc.cin = S4.p;
+        pc.vary4 = df.c_vary;
+        pc.cell4 = batch_cells;
+        pc.s4 ^= 1;
+        const size_t n2 = sumE + df.cp_n;
         std::swap(b->P, b->P2);
-        std::swap(b->C, b->C2);
         nP = n2;
-        nC = sumE;
+        nC = 0;
     }
-    if (counts_finish(pc, b->C.p)) return -1;
+    /* the last layer's directories (its carry sort has no use) */
+    df.carry = false;
+    df.cp_n = 0;
+    pc.pending = false;
+    if (enqueue_deferred()) return -1;
     return check_sync(E.st, "cg_batch_generate");
 }
 

@@ -167,15 +167,44 @@ constexpr unsigned SORT_ERR_OVERSIZE = 8u;
  * allow_sample is set; it only reads keys, and a bucket above its capacity
  * ORs SORT_ERR_OVERSIZE into *d_err: the caller then repeats the sort with
  * allow_sample = false (the LSD path, which takes any size). */
+/* sample: SORT_LSD (the LSD path) or SORT_SAMPLE (the sample sort for the
+ * sizes it takes, up to sort_sample_limit(SORT_SAMPLE) keys) */
+constexpr int SORT_LSD = 0, SORT_SAMPLE = 1;
 uint64_t *sort_unique(hipStream_t st, uint64_t *keys, uint64_t *tmp, size_t n, uint64_t vary,
                       void *scratch, uint64_t *dst, unsigned long long *d_nuniq, unsigned *d_err,
                       int *prof_unique, int ctx = 0, uint64_t cell_max = CELLS_UNBOUNDED,
-                      uint64_t *part = nullptr, size_t part_cap = 0, bool allow_sample = true);
+                      uint64_t *part = nullptr, size_t part_cap = 0, int sample = SORT_SAMPLE);
 /* keys of the part buffer a sort of n keys needs (0: none) */
-size_t sort_part_keys(size_t n);
-/* sample sort: on / off, mean keys per bucket, bucket capacity (lowered only
- * to exercise the repeat; 0 restores 16384) */
+size_t sort_part_keys(size_t n, int sample = SORT_SAMPLE);
+/* The sample sort of *dn keys, a count known only on the device (nmax bounds
+ * it and sizes the grids and buffers: part holds sort_part_keys(nmax) keys,
+ * scratch sort_scratch_bytes): the same result as sort_unique, with no host
+ * synchronisation and no host decision that depends on the count.  Returns
+ * -1 (nothing enqueued) when nmax is above sort_sample_limit() or part is
+ * too small.  A bucket above its capacity ORs SORT_ERR_OVERSIZE into *d_err;
+ * the caller then repeats the sort with sort_unique(allow_sample = false). */
+int sort_unique_dev(hipStream_t st, const uint64_t *keys, const unsigned long long *dn, size_t nmax,
+                    void *scratch, uint64_t *dst, unsigned long long *d_nuniq, unsigned *d_err,
+                    int *prof_unique, int ctx, uint64_t *part, size_t part_cap);
+/* the largest sort the sample sort takes in a mode (0 when it is switched
+ * off, or for SORT_LSD) */
+size_t sort_sample_limit(int mode);
+/* Makes room for `entries` look-back words in context ctx and for
+ * `launches` more launches before the epoch counter wraps, so that the
+ * launches that follow clear nothing (no memset, e.g. inside a graph
+ * capture). */
+int sort_reserve(hipStream_t st, int ctx, size_t entries, unsigned launches);
+/* the same for a sort_unique of n keys (part assumed large enough) */
+int sort_prepare(hipStream_t st, int ctx, size_t n, uint64_t vary, uint64_t cell_max, int sample);
+/* sample sort: on / off, mean keys per bucket, keys a bucket may hold
+ * (lowered only to exercise the repeat; 0 restores the default, 131072: the
+ * keys of a bucket's regions, whose distinct values must fit the LDS) */
 void sort_set_sample(int on, int bucket_avg, int cap);
+/* most buckets of a sort (2048 or 8192) and the longest hash-set probe of a
+ * bucket the direct sort can take (0: the default) */
+void sort_set_sample_limits(int bmax, int probes);
+int sort_sample_bmax();
+int sort_sample_probes();
 int sort_sample_on();
 int sort_sample_bucket();
 int sort_sample_cap();
@@ -216,10 +245,10 @@ int get_dir_delta();
 void kernels_note_error(hipError_t e);
 
 void launch_fill_u64(hipStream_t st, uint64_t *p, size_t n, uint64_t v);
-/* hpub[lo..lo+n) = cnt[lo..lo+n), hpub[8] = *err (hpub: mapped host memory,
- * n <= 8) */
+/* hpub[lo..lo+n) = cnt[lo..lo+n), hpub[err_at + j] = err[j] for j = 0, 1
+ * (hpub: mapped host memory; lo + n <= err_at, n <= 62) */
 void launch_publish(hipStream_t st, const unsigned long long *cnt, int lo, int n, const unsigned *err,
-                    unsigned long long *hpub);
+                    unsigned long long *hpub, int err_at);
 /* dst[i] = src[i] & mask */
 void launch_copy_and_u64(hipStream_t st, uint64_t *dst, const uint64_t *src, size_t n, uint64_t mask);
 /* stats[0] = 0 (count), stats[1,3] = 0 (OR), stats[2,4] = ~0 (AND) */
@@ -293,6 +322,8 @@ void launch_bspawn(hipStream_t st, const uint64_t *in, size_t n, const unsigned 
                    const BatchTable *tabs, int step, unsigned *err);
 /* composite -> boards, with per-table count / OR / AND (tstat[3 t + 0..2],
  * AND entries preset to all ones) */
+/* per-table statistics (count, OR, AND) set to 0, 0, all ones */
+void launch_tstat_init(hipStream_t st, unsigned long long *t, int ntab);
 void launch_bdecode(hipStream_t st, const uint64_t *in, size_t n, const unsigned long long *dn,
                     uint64_t *out, const BatchTable *tabs, unsigned long long *tstat);
 /* directories of every table of a step in one concatenated array; entries

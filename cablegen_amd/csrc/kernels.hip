@@ -377,18 +377,33 @@ void launch_copy_and_u64(hipStream_t st, uint64_t *dst, const uint64_t *src, siz
     hipLaunchKernelGGL(k_copy_and_u64, dim3((unsigned)blocks), dim3(256), 0, st, dst, src, n, mask);
 }
 
+/* per-table statistics of a batched layer, [3t] count, [3t+1] OR, [3t+2]
+ * AND: 0, 0, all ones */
+__global__ void k_tstat_init(unsigned long long *t, int ntab)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < 3 * ntab; i += gridDim.x * blockDim.x)
+        t[i] = i % 3 == 2 ? ~0ULL : 0ULL;
+}
+
+void launch_tstat_init(hipStream_t st, unsigned long long *t, int ntab)
+{
+    if (ntab <= 0) return;
+    const int blocks = (3 * ntab + 255) / 256;
+    hipLaunchKernelGGL(k_tstat_init, dim3((unsigned)(blocks > 64 ? 64 : blocks)), dim3(256), 0, st, t, ntab);
+}
+
 __global__ void k_publish(const unsigned long long *cnt, int lo, int n, const unsigned *err,
-                          unsigned long long *hpub)
+                          unsigned long long *hpub, int nerr_at)
 {
     int i = threadIdx.x;
     if (i < n) hpub[lo + i] = cnt[lo + i];
-    if (i == 63) hpub[8] = *err;
+    if (i >= 62) hpub[nerr_at + (i - 62)] = err[i - 62];
 }
 
 void launch_publish(hipStream_t st, const unsigned long long *cnt, int lo, int n, const unsigned *err,
-                    unsigned long long *hpub)
+                    unsigned long long *hpub, int err_at)
 {
-    hipLaunchKernelGGL(k_publish, dim3(1), dim3(64), 0, st, cnt, lo, n, err, hpub);
+    hipLaunchKernelGGL(k_publish, dim3(1), dim3(64), 0, st, cnt, lo, n, err, hpub, err_at);
 }
 
 /* producer statistics: [0] count, then (OR, AND) pairs; AND starts all-ones */

@@ -531,7 +531,8 @@ constexpr int SS_KS32 = 16;                 /* rows per thread of a bucket's sor
 constexpr int SS_KS64 = 8;                  /* 64-bit values: 8192 */
 constexpr int SS_PK = 8;                    /* keys per thread of a partition tile */
 constexpr int SS_TILE = SS_T * SS_PK;       /* 8192 */
-constexpr int SS_BMAX = 2048;               /* buckets */
+constexpr int SS_BMAX = 8192;               /* buckets */
+constexpr int SS_BSMALL = 2048;             /* buckets of the partition with two blocks per CU */
 /* A bucket's keys land in SS_NX regions, one per group of partition blocks
  * (blockIdx % SS_NX, which on MI355X is the XCD the block runs on): SS_NX
  * counters per bucket, so no counter takes every tile's reservation, and a
@@ -545,7 +546,39 @@ static_assert((uint64_t)SS_BMAX * SS_BSTRIDE < 0xffffffffULL, "places in part ar
 /* splitter guide: the number of splitters below smin + (p << gshift) for
  * p = 0..SS_GUIDE, which narrows a key's splitter search to a few entries */
 constexpr int SS_GUIDE = 4096;
-constexpr int SS_S = SS_LCAP;               /* samples */
+constexpr int SS_S = SS_LCAP;               /* samples up to SS_BSMALL buckets (8 per bucket) */
+constexpr int SS_SMAX = 4 * SS_S;            /* samples of the largest sorts */
+/* keys a bucket's regions hold; the default cap on a bucket's keys (only its
+ * distinct values must fit the LDS) */
+constexpr int SS_KEYCAP = SS_NX * SS_SUBCAP;
+
+/* Buckets of a sample sort of n keys with a mean of bavg keys per bucket: 1
+ * (one block takes every key) up to SS_LCAP64 keys, which a bucket of keys
+ * spanning 2^32 or more holds as distinct 64-bit values, else ceil(n / bavg)
+ * with at least 2 and at most sh.bmax (larger sorts have larger buckets).
+ * The host and the kernels compute it alike, so a sort whose size is only
+ * known on the device sizes its grids for the bound and the kernels agree on
+ * B. */
+struct SsShape {
+    int bavg;   /* mean keys per bucket */
+    int bmax;   /* most buckets (larger sorts have larger buckets) */
+};
+
+__host__ __device__ __forceinline__ size_t ss_nbuckets(size_t n, SsShape sh)
+{
+    if (n <= (size_t)SS_LCAP64) return 1;
+    size_t b = (n + (size_t)sh.bavg - 1) / (size_t)sh.bavg;
+    b = b < 2 ? 2 : b;
+    return b > (size_t)sh.bmax ? (size_t)sh.bmax : b;
+}
+
+/* the sort's key count: *dn (read on the device) bounded by nmax, or nmax */
+__device__ __forceinline__ size_t ss_count(const unsigned long long *dn, size_t nmax)
+{
+    if (!dn) return nmax;
+    const size_t v = (size_t)*dn;
+    return v < nmax ? v : nmax;
+}
 
 /* 64 KiB of values plus counters: two blocks per CU */
 struct SsLds {
@@ -695,14 +728,18 @@ __device__ __forceinline__ void ss_block_sort(V (&v)[K], int m, int chunk, int n
  * scattered loads would take ~20 us) into smp, with its smallest and
  * largest key folded into mnx[0] (as ~min) and mnx[1] by atomic max (both
  * cleared to 0 beforehand) */
-__global__ __launch_bounds__(SS_T) void k_ss_gather(const uint64_t *__restrict__ keys, size_t n,
-                                                    uint64_t *__restrict__ smp, unsigned long long *mnx)
+__global__ __launch_bounds__(SS_T) void k_ss_gather(const uint64_t *__restrict__ keys,
+                                                    const unsigned long long *dn, size_t nmax, SsShape sh,
+                                                    int S, uint64_t *__restrict__ smp,
+                                                    unsigned long long *mnx)
 {
     __shared__ uint64_t ra[SS_NW], rb[SS_NW];
+    const size_t n = ss_count(dn, nmax);
+    if (ss_nbuckets(n, sh) <= 1) return;   /* one block sorts them: no sample */
     const size_t s = (size_t)blockIdx.x * SS_T + threadIdx.x;
     uint64_t a = 0, b = 0;   /* ~min and max */
-    if (s < (size_t)SS_S) {
-        const uint64_t k = keys[((2 * s + 1) * n) / (2 * (size_t)SS_S)];
+    if (s < (size_t)S) {
+        const uint64_t k = keys[((2 * s + 1) * n) / (2 * (size_t)S)];
         smp[s] = k;
         a = ~k;
         b = k;
@@ -745,9 +782,11 @@ constexpr int SS_RUN = SS_T;
 constexpr int SS_NRUN = SS_S / SS_RUN;
 
 __global__ __launch_bounds__(SS_T) void k_ss_runs(const uint64_t *__restrict__ smp, const unsigned long long *mnx,
-                                                  uint32_t *__restrict__ runs)
+                                                  uint32_t *__restrict__ runs, const unsigned long long *dn,
+                                                  size_t nmax, SsShape sh)
 {
     __shared__ SsLds L;
+    if (ss_nbuckets(ss_count(dn, nmax), sh) <= 1) return;
     uint64_t mn;
     int nb, z;
     ss_sample_range(mnx, &mn, &nb, &z);
@@ -768,10 +807,13 @@ __global__ __launch_bounds__(SS_T) void k_ss_runs(const uint64_t *__restrict__ s
  * ranks are 0..SS_S - 1 once each); the thread holding a splitter's rank
  * writes it. */
 __global__ __launch_bounds__(SS_T) void k_ss_select(const uint32_t *__restrict__ runs,
-                                                    const unsigned long long *mnx, int B,
+                                                    const unsigned long long *mnx,
+                                                    const unsigned long long *dn, size_t nmax, SsShape sh,
                                                     uint64_t *__restrict__ spl)
 {
     __shared__ uint32_t R[SS_S];
+    const int B = (int)ss_nbuckets(ss_count(dn, nmax), sh);
+    if (B <= 1) return;
     for (int i = threadIdx.x; i < SS_S; i += SS_T) R[i] = runs[i];
     __syncthreads();
     const int r = blockIdx.x;
@@ -799,9 +841,68 @@ __global__ __launch_bounds__(SS_T) void k_ss_select(const uint32_t *__restrict__
     }
 }
 
+/* The splitters of a larger sample (S = 32768 or 65536 values in S / 1024
+ * sorted runs), whose values do not fit one block's LDS: block (r, g) loads
+ * the SS_RGRP runs of group g and adds to each value of run r its count of
+ * values before it in those runs (the tie rule of k_ss_select); k_ss_pick
+ * then writes the values whose summed rank is a splitter's. */
+constexpr int SS_RGRP = SS_S / SS_RUN;   /* runs per group: 64 KiB of LDS */
+
+__global__ __launch_bounds__(SS_T) void k_ss_rank(const uint32_t *__restrict__ runs, int nrun,
+                                                  const unsigned long long *dn, size_t nmax, SsShape sh,
+                                                  uint32_t *__restrict__ rank)
+{
+    __shared__ uint32_t R[SS_RGRP * SS_RUN];
+    if (ss_nbuckets(ss_count(dn, nmax), sh) <= 1) return;
+    const int r = (int)(blockIdx.x % (unsigned)nrun), g = (int)(blockIdx.x / (unsigned)nrun);
+    const int q0 = g * SS_RGRP;
+    for (int i = threadIdx.x; i < SS_RGRP * SS_RUN; i += SS_T) R[i] = runs[(size_t)q0 * SS_RUN + i];
+    __syncthreads();
+    const uint32_t e = runs[(size_t)r * SS_RUN + threadIdx.x];
+    uint32_t c = 0;
+    for (int qq = 0; qq < SS_RGRP; qq++) {
+        const int q = q0 + qq;
+        if (q == r) {
+            c += threadIdx.x;
+            continue;
+        }
+        const uint32_t *X = R + qq * SS_RUN;
+        uint32_t lo = 0;
+#pragma unroll
+        for (uint32_t step = SS_RUN / 2; step; step >>= 1) {
+            const uint32_t x = X[lo + step - 1];
+            if (q < r ? x <= e : x < e) lo += step;
+        }
+        const uint32_t x = X[lo < SS_RUN ? lo : SS_RUN - 1];
+        if (lo < SS_RUN && (q < r ? x <= e : x < e)) lo++;
+        c += lo;
+    }
+    atomicAdd(&rank[(size_t)r * SS_RUN + threadIdx.x], c);
+}
+
+__global__ __launch_bounds__(SS_T) void k_ss_pick(const uint32_t *__restrict__ runs,
+                                                  const uint32_t *__restrict__ rank,
+                                                  const unsigned long long *mnx,
+                                                  const unsigned long long *dn, size_t nmax, SsShape sh, int S,
+                                                  uint64_t *__restrict__ spl)
+{
+    const int B = (int)ss_nbuckets(ss_count(dn, nmax), sh);
+    if (B <= 1) return;
+    const size_t i = (size_t)blockIdx.x * SS_T + threadIdx.x;
+    const uint32_t e = runs[i], rk = rank[i];
+    const uint64_t c = ((uint64_t)rk * (uint64_t)B + S - 1) / (uint64_t)S;   /* splitter c - 1 */
+    if (c >= 1 && c + 1 <= (uint64_t)B && (c * (uint64_t)S) / (uint64_t)B == rk) {
+        uint64_t mn;
+        int nb, z;
+        ss_sample_range(mnx, &mn, &nb, &z);
+        spl[c - 1] = mn + ((uint64_t)e << z);
+    }
+}
+
+template <int BM>
 struct SsPartLds {
-    uint64_t spl[SS_BMAX];
-    uint32_t cnt[SS_BMAX];        /* counts, then each bucket's place in part (~0u: region full) */
+    uint64_t spl[BM];
+    uint32_t cnt[BM];        /* counts, then each bucket's place in part (~0u: region full) */
     uint16_t guide[SS_GUIDE + 1];
 };
 
@@ -814,13 +915,20 @@ struct SsPartLds {
  * share a CU; a tile's run of one bucket is a few keys either way.  A tile
  * whose reservation would pass the region's end writes none of that
  * bucket's keys (the bucket is then oversize). */
-__global__ __launch_bounds__(SS_T, 2) void k_ss_part(const uint64_t *__restrict__ keys, size_t n,
+template <int BM>
+__global__ __launch_bounds__(SS_T, BM <= SS_BSMALL ? 2 : 1) void k_ss_part(const uint64_t *__restrict__ keys,
+                                                     const unsigned long long *dn, size_t nmax, SsShape sh,
                                                      const uint64_t *__restrict__ spl_g,
-                                                     const unsigned long long *mnx, int B,
+                                                     const unsigned long long *mnx,
                                                      uint64_t *__restrict__ part,
                                                      unsigned long long *gcur, unsigned *err)
 {
-    __shared__ SsPartLds P;
+    __shared__ SsPartLds<BM> P;
+    const size_t n = ss_count(dn, nmax);
+    const int B = (int)ss_nbuckets(n, sh);
+    /* the grid covers nmax keys: blocks past n (and every block of a
+     * one-block sort) have nothing to do */
+    if (B <= 1 || (size_t)blockIdx.x * SS_TILE >= n) return;
     const int nspl = B - 1;
     const int grp = blockIdx.x % SS_NX;
     for (int i = threadIdx.x; i < nspl; i += SS_T) P.spl[i] = spl_g[i];
@@ -871,18 +979,22 @@ __global__ __launch_bounds__(SS_T, 2) void k_ss_part(const uint64_t *__restrict_
         }
     }
     __syncthreads();
-    /* two buckets per thread (B <= 2 SS_T): the reservations */
-    static_assert(SS_BMAX <= 2 * SS_T, "two buckets per thread");
-    const int b0 = threadIdx.x, b1 = threadIdx.x + SS_T;
-    const uint32_t c0 = b0 < B ? P.cnt[b0] : 0, c1 = b1 < B ? P.cnt[b1] : 0;
-    unsigned long long g0 = 0, g1 = 0;
-    if (c0) g0 = atomicAdd(&gcur[b0 * SS_NX + grp], (unsigned long long)c0);
-    if (c1) g1 = atomicAdd(&gcur[b1 * SS_NX + grp], (unsigned long long)c1);
-    const bool o0 = c0 && g0 + c0 > (unsigned long long)SS_SUBCAP;
-    const bool o1 = c1 && g1 + c1 > (unsigned long long)SS_SUBCAP;
-    if (o0 || o1) atomicOr(err, SORT_ERR_OVERSIZE);
-    if (c0) P.cnt[b0] = o0 ? ~0u : (uint32_t)((uint64_t)b0 * SS_BSTRIDE + (uint64_t)grp * SS_SUBCAP + g0);
-    if (c1) P.cnt[b1] = o1 ? ~0u : (uint32_t)((uint64_t)b1 * SS_BSTRIDE + (uint64_t)grp * SS_SUBCAP + g1);
+    /* BM / SS_T buckets per thread: the reservations */
+    constexpr int PER = BM / SS_T;
+    static_assert(BM % SS_T == 0 && BM <= 65536, "buckets per thread; bucket << 16 in br");
+    bool over = false;
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+        const int b = threadIdx.x + q * SS_T;
+        const uint32_t c = b < B ? P.cnt[b] : 0;
+        if (c) {
+            const unsigned long long g = atomicAdd(&gcur[b * SS_NX + grp], (unsigned long long)c);
+            const bool o = g + c > (unsigned long long)SS_SUBCAP;
+            over |= o;
+            P.cnt[b] = o ? ~0u : (uint32_t)((uint64_t)b * SS_BSTRIDE + (uint64_t)grp * SS_SUBCAP + g);
+        }
+    }
+    if (over) atomicOr(err, SORT_ERR_OVERSIZE);
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < SS_PK; j++) {
@@ -893,6 +1005,11 @@ __global__ __launch_bounds__(SS_T, 2) void k_ss_part(const uint64_t *__restrict_
         }
     }
 }
+
+/* Longest probe sequence of a hash-set insert.  Linear probing stays far
+ * below it up to a load of about 3/4; a set filled closer than that would
+ * make each insert walk long runs of slots (quadratic in the keys). */
+constexpr int SS_PROBES = 64;
 
 /* Deduplicates the bucket's keys (the SS_NX regions of L.segc[x] keys at s +
  * x * SS_SUBCAP) as values k - base of type V through an LDS hash set: the
@@ -905,7 +1022,7 @@ __global__ __launch_bounds__(SS_T, 2) void k_ss_part(const uint64_t *__restrict_
  * (SS_T * K values) cannot hold them. */
 template <typename V, int K>
 __device__ __forceinline__ int ss_dedupe(const uint64_t *__restrict__ s, uint64_t base, SsLds &L, uint64_t *vmin,
-                                         int *nbits)
+                                         int *nbits, int probes)
 {
     constexpr int NS = (int)(sizeof(SsLds::stage) / sizeof(V));
     constexpr V EMPTY = (V)~(V)0;
@@ -926,7 +1043,9 @@ __device__ __forceinline__ int ss_dedupe(const uint64_t *__restrict__ s, uint64_
             uint32_t h = (sizeof(V) == 4) ? (((uint32_t)x * 2654435761u) >> 18)
                                           : (uint32_t)(((uint64_t)x * 0x9E3779B97F4A7C15ULL) >> 51);   /* 13 bits */
             for (int probe = 0;; probe++) {
-                if (probe == NS) {   /* every slot holds another value: too many distinct values */
+                if (probe == probes) {
+                    /* the set is nearly full (mostly distinct keys), or
+                     * every slot holds another value */
                     full = true;
                     break;
                 }
@@ -969,6 +1088,62 @@ __device__ __forceinline__ int ss_dedupe(const uint64_t *__restrict__ s, uint64_
     return n;
 }
 
+/* The bucket of m keys (its SS_NX regions, L.segc[r] keys each) when the
+ * hash set could not take its distinct values: all m keys, as values k -
+ * base of type V less the smallest of them (L.vlo, which the hash-set pass
+ * found, as it found the largest, L.vhi), are sorted in one block and the
+ * first of each run of equal values is kept.  m <= SS_T * K.  On return the
+ * stage's first n entries hold the distinct values ascending, relative to
+ * *vmin; returns n. */
+template <typename V, int K>
+__device__ __forceinline__ int ss_direct(const uint64_t *__restrict__ s, uint64_t base, int m, SsLds &L,
+                                         uint64_t *vmin)
+{
+    const int w = threadIdx.x >> 6, lane = lane_id();
+    const V vl = (V)L.vlo, vh = (V)L.vhi;
+    const int nb = vh > vl ? 64 - __clzll((long long)(uint64_t)(vh - vl)) : 0;
+    const int chunk = ss_chunk<K>(m), rem = ss_rem(m, chunk);
+    uint32_t pre[SS_NX + 1];
+    pre[0] = 0;
+#pragma unroll
+    for (int r = 0; r < SS_NX; r++) pre[r + 1] = pre[r] + L.segc[r];
+    V v[K];
+#pragma unroll
+    for (int j = 0; j < K; j++) {
+        v[j] = 0;
+        if (j * 64 + lane < rem) {
+            const uint32_t i = (uint32_t)(w * chunk + j * 64 + lane);
+            int r = 0;
+#pragma unroll
+            for (int q = 1; q < SS_NX; q++) r += i >= pre[q];
+            v[j] = (V)((V)(s[(size_t)r * SS_SUBCAP + (i - pre[r])] - base) - vl);
+        }
+    }
+    ss_block_sort<V, K>(v, m, chunk, nb, L);
+    /* thread t keeps the run starts among positions tK .. tK + K - 1 */
+    V *T = ss_stage<V>(L);
+    V x[K];
+    uint32_t keep = 0, c = 0;
+#pragma unroll
+    for (int j = 0; j < K; j++) {
+        const int p = (int)threadIdx.x * K + j;
+        x[j] = p < m ? T[p] : (V)0;
+        const V before = j ? x[j - 1] : (p > 0 && p <= m ? T[p - 1] : (V)0);
+        if (p < m && (p == 0 || x[j] != before)) {
+            keep |= 1u << j;
+            c++;
+        }
+    }
+    uint32_t all;
+    uint32_t o = block_excl_scan_dpp(c, &all, L.wtmp);   /* its barrier ends the reads */
+#pragma unroll
+    for (int j = 0; j < K; j++)
+        if ((keep >> j) & 1u) T[o++] = x[j];
+    __syncthreads();
+    *vmin = (uint64_t)vl;
+    return (int)all;
+}
+
 /* Sorts the n distinct values at the front of the stage (ss_dedupe's
  * output) over their low nb bits; on return they are in ascending order at
  * the stage's first n entries. */
@@ -997,19 +1172,27 @@ __device__ __forceinline__ void ss_sort_distinct(int n, int nb, SsLds &L)
  * of those counts over the buckets places the unique keys in dst (the last
  * bucket writes the total).  More than cap keys, or more distinct values
  * than the hash set holds, is an oversize bucket (published as empty). */
-__global__ __launch_bounds__(SS_T, 8) void k_ss_local(const uint64_t *__restrict__ src,
+__global__ __launch_bounds__(SS_T, 8) void k_ss_local(const uint64_t *__restrict__ part,
+                                                      const uint64_t *__restrict__ keys,
                                                       const unsigned long long *counts,
-                                                      const uint64_t *__restrict__ spl, size_t n1, int B,
+                                                      const uint64_t *__restrict__ spl,
+                                                      const unsigned long long *dn, size_t nmax, SsShape sh,
                                                       uint64_t *__restrict__ dst, uint64_t *status,
                                                       unsigned *ticket, uint32_t epoch,
-                                                      unsigned long long *total, unsigned *err, int cap)
+                                                      unsigned long long *total, unsigned *err, int cap,
+                                                      int probes_direct)
 {
     __shared__ SsLds L;
     const int w = threadIdx.x >> 6, lane = lane_id();
-    if (threadIdx.x == 0) L.ticket = B > 1 ? atomicAdd(ticket, 1u) : 0u;
+    const size_t n1 = ss_count(dn, nmax);
+    const int B = (int)ss_nbuckets(n1, sh);
+    /* the grid has a block for each bucket of nmax keys: tickets past B
+     * (and every block but the first of a one-block sort) leave */
+    if (threadIdx.x == 0) L.ticket = B > 1 ? atomicAdd(ticket, 1u) : blockIdx.x;
     __syncthreads();
     const size_t t = L.ticket;
-    const uint64_t *s = B > 1 ? src + t * SS_BSTRIDE : src;
+    if (t >= (size_t)B) return;
+    const uint64_t *s = B > 1 ? part + t * SS_BSTRIDE : keys;
     size_t m = 0;
     if (B > 1) {
         for (int x = 0; x < SS_NX; x++) m += counts[t * SS_NX + x];
@@ -1055,16 +1238,30 @@ __global__ __launch_bounds__(SS_T, 8) void k_ss_local(const uint64_t *__restrict
     }
     uint64_t vmin = 0;
     int nb = 0;
-    int n = wide ? ss_dedupe<uint64_t, SS_KS64>(s, base, L, &vmin, &nb)
-                 : ss_dedupe<uint32_t, SS_KS32>(s, base, L, &vmin, &nb);
+    /* a bucket the direct sort can take gives up on a nearly full set (a
+     * longer probe costs more than sorting every key); a larger one probes
+     * on, since its only other way is a repeat of the whole sort */
+    const bool direct_ok = !over && m <= (size_t)SS_T * (wide ? SS_KS64 : SS_KS32);
+    const int probes = direct_ok ? probes_direct : (wide ? SS_LCAP64 : SS_LCAP);
+    int n = wide ? ss_dedupe<uint64_t, SS_KS64>(s, base, L, &vmin, &nb, probes)
+                 : ss_dedupe<uint32_t, SS_KS32>(s, base, L, &vmin, &nb, probes);
+    bool sorted = false;
+    if (n < 0 && direct_ok) {
+        /* mostly distinct keys: sorted all together, equal runs dropped */
+        n = wide ? ss_direct<uint64_t, SS_KS64>(s, base, (int)m, L, &vmin)
+                 : ss_direct<uint32_t, SS_KS32>(s, base, (int)m, L, &vmin);
+        sorted = true;
+    }
     if (n < 0) {   /* more distinct values than one block sorts: oversize */
         if (threadIdx.x == 0) atomicOr(err, SORT_ERR_OVERSIZE);
         n = 0;
     }
     uint64_t *my = status + t;
     if (B > 1 && threadIdx.x == 0) st_flag(my, t == 0 ? OS_INC : OS_AGG, epoch, (uint64_t)n);
-    if (wide) ss_sort_distinct<uint64_t, SS_KS64>(n, nb, L);
-    else ss_sort_distinct<uint32_t, SS_KS32>(n, nb, L);
+    if (!sorted) {
+        if (wide) ss_sort_distinct<uint64_t, SS_KS64>(n, nb, L);
+        else ss_sort_distinct<uint32_t, SS_KS32>(n, nb, L);
+    }
     if (w == 0) {
         uint64_t excl = 0;
         if (B > 1 && t > 0) {
@@ -1386,40 +1583,53 @@ static OsScratch carve(void *scratch)
 
 static int g_ss_on = 1;              /* sample sort for sizes it handles */
 static int g_ss_bavg = 5000;         /* mean keys per bucket */
-static int g_ss_cap = SS_LCAP;       /* bucket capacity (lowered only to test the repeat) */
+static int g_ss_cap = SS_KEYCAP;     /* keys of a bucket (lowered only to test the repeat) */
+/* most buckets of a sort: SS_BSMALL (the default: C2's layer steps and the
+ * batched steps measured faster with it) or SS_BMAX */
+static int g_ss_bmax = SS_BSMALL;
+static int g_ss_probes = SS_PROBES;  /* longest hash-set probe of a bucket the direct sort can take */
 static unsigned long long g_ss_sorts = 0;
 
 void sort_set_sample(int on, int bucket_avg, int cap)
 {
     g_ss_on = on ? 1 : 0;
     if (bucket_avg > 0) g_ss_bavg = bucket_avg;
-    g_ss_cap = (cap > 0 && cap <= SS_LCAP) ? cap : SS_LCAP;
+    g_ss_cap = (cap > 0 && cap <= SS_KEYCAP) ? cap : SS_KEYCAP;
 }
+void sort_set_sample_limits(int bmax, int probes)
+{
+    g_ss_bmax = bmax == SS_BSMALL ? SS_BSMALL : SS_BMAX;
+    g_ss_probes = probes > 0 ? probes : SS_PROBES;
+}
+int sort_sample_bmax() { return g_ss_bmax; }
+int sort_sample_probes() { return g_ss_probes; }
 int sort_sample_on() { return g_ss_on; }
 int sort_sample_bucket() { return g_ss_bavg; }
 int sort_sample_cap() { return g_ss_cap; }
 unsigned long long sort_sample_count() { return g_ss_sorts; }
 
-/* buckets of a sample sort of n keys (1: one block), or 0 when the sort
- * takes the LSD path */
-/* up to SS_BMAX buckets of about g_ss_bavg keys; a larger sort takes
- * SS_BMAX buckets of up to SS_BAVG_MAX keys (the largest sorts are the
- * layers' move candidates, which hold each key about 3.6 times, so their
- * buckets have few distinct values), and beyond that the LSD path */
+/* The largest sort the sample sort takes: SS_BMAX buckets of up to
+ * SS_BAVG_MAX keys on average (the largest sorts are the layers' move
+ * candidates, which hold each key about 3.6 times, so their buckets have few
+ * distinct values); beyond it, the LSD path. */
 constexpr size_t SS_BAVG_MAX = 7000;
-static size_t ss_buckets(size_t n)
+size_t sort_sample_limit(int mode)
 {
-    if (!g_ss_on || n < 2) return 0;
-    if (n <= (size_t)SS_LCAP) return 1;
-    size_t B = (n + (size_t)g_ss_bavg - 1) / (size_t)g_ss_bavg;
-    if (B < 2) B = 2;
-    if (B <= (size_t)SS_BMAX) return B;
-    return n <= (size_t)SS_BMAX * SS_BAVG_MAX ? (size_t)SS_BMAX : 0;
+    if (!g_ss_on || mode <= 0) return 0;
+    return (size_t)g_ss_bmax * SS_BAVG_MAX;
 }
 
-size_t sort_part_keys(size_t n)
+/* buckets of a sample sort of n keys (1: one block), or 0 when the sort
+ * takes the LSD path */
+static size_t ss_buckets(size_t n, int mode = 1)
 {
-    size_t B = ss_buckets(n);
+    if (n < 2 || n > sort_sample_limit(mode)) return 0;
+    return ss_nbuckets(n, SsShape{g_ss_bavg, g_ss_bmax});
+}
+
+size_t sort_part_keys(size_t n, int mode)
+{
+    size_t B = ss_buckets(n, mode);
     return B > 1 ? B * (size_t)SS_BSTRIDE : 0;
 }
 
@@ -1506,48 +1716,149 @@ static void launch_pass(hipStream_t st, SortCfg c, unsigned tiles, const uint64_
     else launch_pass_t<false>(st, c, tiles, src, dst, n, x, shift, gbase, status, ctr, epoch, err);
 }
 
-/* The sample sort of n keys into dst (B buckets, part holding B * SS_BSTRIDE
- * keys when B > 1); keys are only read. */
-static void ss_sort_unique(hipStream_t st, const uint64_t *keys, size_t n, size_t B, uint64_t *part,
-                           const OsScratch &s, SortCtx &sc, uint64_t *dst, unsigned long long *d_nuniq,
-                           unsigned *d_err, int *prof_unique)
+/* The sample sort of n keys into dst: *dn keys when dn is set (read on the
+ * device; nmax bounds them and sizes the grids), else nmax.  Bmax =
+ * ss_nbuckets(nmax) buckets at most, part holding Bmax * SS_BSTRIDE keys when
+ * Bmax > 1; keys are only read. */
+static void ss_sort_unique(hipStream_t st, const uint64_t *keys, const unsigned long long *dn, size_t nmax,
+                           size_t Bmax, uint64_t *part, const OsScratch &s, SortCtx &sc, uint64_t *dst,
+                           unsigned long long *d_nuniq, unsigned *d_err, int *prof_unique)
 {
     g_ss_sorts++;
     const uint32_t epoch = next_epoch(st, sc);
-    if (B > 1) {
-        launch_fill_u64(st, (uint64_t *)s.ss_ticket, (256 + B * SS_NX * 8) / 8, 0);
+    const SsShape sh{g_ss_bavg, g_ss_bmax};
+    if (Bmax > 1) {
+        launch_fill_u64(st, (uint64_t *)s.ss_ticket, (256 + Bmax * SS_NX * 8) / 8, 0);
         int h = prof_hook_begin(8, st);
-        /* the sample and its sorted runs go to the start of part, which the
-         * partition overwrites only after the splitters are made */
-        uint32_t *runs = reinterpret_cast<uint32_t *>(part + SS_S);
-        hipLaunchKernelGGL(k_ss_gather, dim3(SS_S / SS_T), dim3(SS_T), 0, st, keys, n, part, s.ss_mnx);
-        hipLaunchKernelGGL(k_ss_runs, dim3(SS_NRUN), dim3(SS_T), 0, st, (const uint64_t *)part,
-                           (const unsigned long long *)s.ss_mnx, runs);
-        hipLaunchKernelGGL(k_ss_select, dim3(SS_NRUN), dim3(SS_T), 0, st, (const uint32_t *)runs,
-                           (const unsigned long long *)s.ss_mnx, (int)B, s.ss_spl);
+        /* 8 samples per bucket: SS_S up to SS_BSMALL buckets, SS_SMAX for
+         * the largest sorts.  The sample, its sorted runs and (for a larger
+         * sample) the ranks go to the start of part, which the partition
+         * overwrites only after the splitters are made. */
+        const int S = Bmax <= (size_t)SS_BSMALL ? SS_S : (Bmax <= 2 * (size_t)SS_BSMALL ? 2 * SS_S : SS_SMAX);
+        const int nrun = S / SS_RUN;
+        uint32_t *runs = reinterpret_cast<uint32_t *>(part + S);
+        hipLaunchKernelGGL(k_ss_gather, dim3(S / SS_T), dim3(SS_T), 0, st, keys, dn, nmax, sh, S, part,
+                           s.ss_mnx);
+        hipLaunchKernelGGL(k_ss_runs, dim3(nrun), dim3(SS_T), 0, st, (const uint64_t *)part,
+                           (const unsigned long long *)s.ss_mnx, runs, dn, nmax, sh);
+        if (S == SS_S) {
+            hipLaunchKernelGGL(k_ss_select, dim3(SS_NRUN), dim3(SS_T), 0, st, (const uint32_t *)runs,
+                               (const unsigned long long *)s.ss_mnx, dn, nmax, sh, s.ss_spl);
+        } else {
+            uint32_t *rank = runs + S;
+            launch_fill_u64(st, reinterpret_cast<uint64_t *>(rank), (size_t)S / 2, 0);
+            hipLaunchKernelGGL(k_ss_rank, dim3((unsigned)(nrun * (nrun / SS_RGRP))), dim3(SS_T), 0, st,
+                               (const uint32_t *)runs, nrun, dn, nmax, sh, rank);
+            hipLaunchKernelGGL(k_ss_pick, dim3((unsigned)(S / SS_T)), dim3(SS_T), 0, st, (const uint32_t *)runs,
+                               (const uint32_t *)rank, (const unsigned long long *)s.ss_mnx, dn, nmax, sh,
+                               S, s.ss_spl);
+        }
         snote(hipGetLastError());
-        prof_hook_end(h, 8.0 * SS_S + 8.0 * (B - 1), st);
+        prof_hook_end(h, 8.0 * S + 8.0 * (Bmax - 1), st);
         h = prof_hook_begin(9, st);
-        const size_t tiles = (n + SS_TILE - 1) / SS_TILE;
-        hipLaunchKernelGGL(k_ss_part, dim3((unsigned)tiles), dim3(SS_T), 0, st, keys, n,
-                           (const uint64_t *)s.ss_spl, (const unsigned long long *)s.ss_mnx, (int)B, part,
-                           s.ss_cnt, d_err);
+        const size_t tiles = (nmax + SS_TILE - 1) / SS_TILE;
+        if (Bmax <= (size_t)SS_BSMALL)
+            hipLaunchKernelGGL(k_ss_part<SS_BSMALL>, dim3((unsigned)tiles), dim3(SS_T), 0, st, keys, dn, nmax,
+                               sh, (const uint64_t *)s.ss_spl, (const unsigned long long *)s.ss_mnx, part,
+                               s.ss_cnt, d_err);
+        else
+            hipLaunchKernelGGL(k_ss_part<SS_BMAX>, dim3((unsigned)tiles), dim3(SS_T), 0, st, keys, dn, nmax,
+                               sh, (const uint64_t *)s.ss_spl, (const unsigned long long *)s.ss_mnx, part,
+                               s.ss_cnt, d_err);
         snote(hipGetLastError());
-        prof_hook_end(h, 16.0 * n, st);
+        prof_hook_end(h, 16.0 * nmax, st);
     }
     *prof_unique = prof_hook_begin(10, st);
-    hipLaunchKernelGGL(k_ss_local, dim3((unsigned)B), dim3(SS_T), 0, st, B > 1 ? part : keys,
-                       (const unsigned long long *)s.ss_cnt, (const uint64_t *)s.ss_spl, n, (int)B, dst,
-                       sc.status, s.ss_ticket, epoch,
-                       d_nuniq, d_err, g_ss_cap);
+    hipLaunchKernelGGL(k_ss_local, dim3((unsigned)Bmax), dim3(SS_T), 0, st, (const uint64_t *)part, keys,
+                       (const unsigned long long *)s.ss_cnt, (const uint64_t *)s.ss_spl, dn, nmax, sh, dst,
+                       sc.status, s.ss_ticket, epoch, d_nuniq, d_err, g_ss_cap, g_ss_probes);
     snote(hipGetLastError());
-    prof_hook_end(*prof_unique, 8.0 * n, st);
+    prof_hook_end(*prof_unique, 8.0 * nmax, st);
+}
+
+int sort_unique_dev(hipStream_t st, const uint64_t *keys, const unsigned long long *dn, size_t nmax,
+                    void *scratch, uint64_t *dst, unsigned long long *d_nuniq, unsigned *d_err,
+                    int *prof_unique, int ctx, uint64_t *part, size_t part_cap)
+{
+    *prof_unique = -1;
+    if (nmax > sort_sample_limit(SORT_SAMPLE)) return -1;
+    SortCtx &sc = g_ctx[ctx < 0 || ctx >= SORT_CONTEXTS ? 0 : ctx];
+    const size_t Bmax = ss_nbuckets(nmax, SsShape{g_ss_bavg, g_ss_bmax});
+    if (Bmax > 1 && (!part || part_cap < Bmax * (size_t)SS_BSTRIDE)) return -1;
+    if (!status_reserve(st, sc, Bmax)) return -1;
+    ss_sort_unique(st, keys, dn, nmax, Bmax, part, carve(scratch), sc, dst, d_nuniq, d_err, prof_unique);
+    return 0;
+}
+
+int sort_reserve(hipStream_t st, int ctx, size_t entries, unsigned launches)
+{
+    SortCtx &sc = g_ctx[ctx < 0 || ctx >= SORT_CONTEXTS ? 0 : ctx];
+    if (!status_reserve(st, sc, entries)) return -1;
+    if (sc.epoch + launches > OS_EPOCH_MAX) {
+        snote(hipMemsetAsync(sc.status, 0, sc.cap * 8, st));
+        sc.epoch = 0;
+    }
+    return 0;
+}
+
+/* what sort_unique does for n keys: the sample sort with B buckets (B >= 1),
+ * or the LSD path with code x, configuration c, npass passes over `tiles`
+ * tiles and a compaction over `utiles` tiles */
+struct SortPlan {
+    size_t B = 0;
+    SortCode x;
+    SortCfg c = {0, 0, 0};
+    int npass = 0;
+    bool coded = false;
+    size_t tiles = 0, utiles = 0;
+    size_t entries = 0;      /* look-back words */
+    unsigned launches = 0;   /* epochs */
+};
+
+static SortPlan sort_plan(size_t n, uint64_t vary, uint64_t cell_max, int sample)
+{
+    SortPlan P;
+    const size_t B = ss_buckets(n, sample);
+    if (B >= 1) {
+        P.B = B;
+        P.entries = B;
+        P.launches = 1;
+        return P;
+    }
+    P.x = make_bit_code(n > 1 ? vary : 0);
+    P.c = sort_choose(n, P.x.nbits);
+    P.npass = (P.x.nbits + P.c.bits - 1) / P.c.bits;
+    if (n > 1 && cell_max != CELLS_UNBOUNDED && g_cell_code) {
+        /* the cell-pair code when it saves a pass */
+        SortCode pc = make_pair_code(vary, cell_max);
+        SortCfg c2 = sort_choose(n, pc.nbits);
+        int np2 = (pc.nbits + c2.bits - 1) / c2.bits;
+        if (pc.cells && np2 < P.npass) {
+            P.coded = true;
+            P.x = pc;
+            P.c = c2;
+            P.npass = np2;
+        }
+    }
+    const size_t tile_keys = (size_t)pass_threads(P.c.bits, P.c.kpt) * P.c.kpt;
+    P.tiles = (n + tile_keys - 1) / tile_keys;
+    P.utiles = (n + UQ_TILE - 1) / UQ_TILE;
+    P.entries = std::max(P.npass ? pass_status_entries(P.tiles, P.c.bits) : 0, P.utiles);
+    P.launches = (unsigned)P.npass + 1;
+    return P;
+}
+
+int sort_prepare(hipStream_t st, int ctx, size_t n, uint64_t vary, uint64_t cell_max, int sample)
+{
+    if (n == 0) return 0;
+    const SortPlan P = sort_plan(n, vary, cell_max, sample);
+    return sort_reserve(st, ctx, P.entries, P.launches);
 }
 
 uint64_t *sort_unique(hipStream_t st, uint64_t *keys, uint64_t *tmp, size_t n, uint64_t vary,
                       void *scratch, uint64_t *dst, unsigned long long *d_nuniq, unsigned *d_err,
                       int *prof_unique, int ctx, uint64_t cell_max, uint64_t *part, size_t part_cap,
-                      bool allow_sample)
+                      int sample)
 {
     *prof_unique = -1;
     SortCtx &sc = g_ctx[ctx < 0 || ctx >= SORT_CONTEXTS ? 0 : ctx];
@@ -1556,32 +1867,18 @@ uint64_t *sort_unique(hipStream_t st, uint64_t *keys, uint64_t *tmp, size_t n, u
         return keys;
     }
     OsScratch s = carve(scratch);
-    const size_t B = allow_sample ? ss_buckets(n) : 0;
-    if (B == 1 || (B > 1 && part && part_cap >= B * (size_t)SS_BSTRIDE)) {
-        if (!status_reserve(st, sc, B)) return keys;
-        ss_sort_unique(st, keys, n, B, part, s, sc, dst, d_nuniq, d_err, prof_unique);
+    const size_t Bs = ss_buckets(n, sample);
+    const bool sample_ok = Bs == 1 || (Bs > 1 && part && part_cap >= Bs * (size_t)SS_BSTRIDE);
+    const SortPlan P = sort_plan(n, vary, cell_max, sample_ok ? sample : 0);
+    if (!status_reserve(st, sc, P.entries)) return keys;
+    if (P.B >= 1) {
+        ss_sort_unique(st, keys, nullptr, n, P.B, part, s, sc, dst, d_nuniq, d_err, prof_unique);
         return keys;
     }
-    SortCode x = make_bit_code(n > 1 ? vary : 0);
-    SortCfg c = sort_choose(n, x.nbits);
-    int npass = (x.nbits + c.bits - 1) / c.bits;
-    if (n > 1 && cell_max != CELLS_UNBOUNDED && g_cell_code) {
-        /* the cell-pair code when it saves a pass */
-        SortCode pc = make_pair_code(vary, cell_max);
-        SortCfg c2 = sort_choose(n, pc.nbits);
-        int np2 = (pc.nbits + c2.bits - 1) / c2.bits;
-        if (pc.cells && np2 < npass) {
-            g_code_sorts++;
-            x = pc;
-            c = c2;
-            npass = np2;
-        }
-    }
-    const size_t tile_keys = (size_t)pass_threads(c.bits, c.kpt) * c.kpt;
-    const size_t tiles = (n + tile_keys - 1) / tile_keys;
-    const size_t utiles = (n + UQ_TILE - 1) / UQ_TILE;
-    if (!status_reserve(st, sc, std::max(npass ? pass_status_entries(tiles, c.bits) : 0, utiles)))
-        return keys;
+    if (P.coded) g_code_sorts++;
+    const SortCode &x = P.x;
+    const SortCfg &c = P.c;
+    const int npass = P.npass;
     /* tile counters and histograms cleared together */
     launch_fill_u64(st, (uint64_t *)scratch, OS_CLEAR_BYTES / 8, 0);
     uint64_t *src = keys, *dst2 = tmp;
@@ -1589,7 +1886,7 @@ uint64_t *sort_unique(hipStream_t st, uint64_t *keys, uint64_t *tmp, size_t n, u
         launch_hist_scan(st, c.bits, keys, n, x, npass, s, d_err);
         for (int p = 0; p < npass; p++) {
             int h = prof_hook_begin(3, st);
-            launch_pass(st, c, (unsigned)tiles, src, dst2, n, x, c.bits * p,
+            launch_pass(st, c, (unsigned)P.tiles, src, dst2, n, x, c.bits * p,
                         s.gbase + ((size_t)p << c.bits), sc.status, s.ctr + p, next_epoch(st, sc), d_err,
                         p == 0);
             prof_hook_end(h, 16.0 * n, st);
@@ -1597,7 +1894,7 @@ uint64_t *sort_unique(hipStream_t st, uint64_t *keys, uint64_t *tmp, size_t n, u
         }
     }
     *prof_unique = prof_hook_begin(4, st);
-    hipLaunchKernelGGL(k_os_unique, dim3((unsigned)utiles), dim3(UQ_THREADS), 0, st, src, n, dst,
+    hipLaunchKernelGGL(k_os_unique, dim3((unsigned)P.utiles), dim3(UQ_THREADS), 0, st, src, n, dst,
                        sc.status, s.ctr + OS_MAXPASS, next_epoch(st, sc), d_nuniq, d_err,
                        npass > 0 ? 1 : 0, x, s.cval);
     snote(hipGetLastError());

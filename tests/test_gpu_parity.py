@@ -509,6 +509,53 @@ def test_tablebase_carry_sort_on_main_stream(cg, name):
 
 
 @pytest.mark.parametrize("name", ["c1", "dpdgap_prune", "3x4"])
+def test_tablebase_two_syncs_per_step(cg, name):
+    """The layer step with the candidate count brought to the host before
+    the M sort (one_sync 0, the path of steps whose bound is beyond the
+    sample sort) gives the same tables as the default step, whose M sort is
+    sized on the device and which synchronises once."""
+    try:
+        cg.tune_set("one_sync", 0)
+        assert cg.tune_get("one_sync") == 0
+        test_tablebase_vs_oracle(cg, name)
+    finally:
+        cg.tune_set("reset", 0)
+    assert cg.tune_get("one_sync") == 1
+
+
+@pytest.mark.parametrize("name", ["c1", "3x4"])
+def test_tablebase_oversize_carry_on_main_stream(cg, name):
+    """Every sample sort overflowing (capacity 64) with the carry sort on the
+    main stream: the M sort's and the carry sort's repeats are told apart by
+    their error words and both are redone on the LSD path."""
+    try:
+        cg.tune_set("sample_cap", 64)
+        cg.tune_set("carry_stream", 0)
+        test_tablebase_vs_oracle(cg, name)
+    finally:
+        cg.tune_set("reset", 0)
+
+
+def test_lsd_generation_after_oversize(cg):
+    """An oversize bucket seen by one generation leaves nothing behind: a
+    later generation on the LSD path (whose counter reads must not see that
+    oversize again and repeat a carry sort whose input the LSD passes have
+    overwritten) still matches the oracle."""
+    try:
+        cg.tune_set("sample_cap", 64)
+        test_tablebase_vs_oracle(cg, "3x4")   # every sample sort overflows and is repeated
+    finally:
+        cg.tune_set("reset", 0)
+    assert cg.tune_get("sample_repeats") > 0
+    try:
+        cg.tune_set("sample_sort", 0)
+        test_tablebase_vs_oracle(cg, "c1")
+        test_tablebase_vs_oracle(cg, "two_initial")
+    finally:
+        cg.tune_set("reset", 0)
+
+
+@pytest.mark.parametrize("name", ["c1", "dpdgap_prune", "3x4"])
 def test_tablebase_spawn_buffer_repeat(cg, name):
     """Undersized spawn buffers: every growing step overflows them and is
     repeated with exact buffers; layers and values stay identical."""
@@ -770,9 +817,14 @@ def test_batch_matches_single_tables(cg):
     for c in cfgs:
         B.add(c["initial"], c["start"], c["end"], c["winstates"])
     before = cg.tune_get("sample_sorts")
+    g0 = cg.tune_get("graph_steps")
     B.generate()
-    # the batch's layer sorts take the sample sort (the default path)
+    # the batch's layer sorts take the sample sort (the default path), and a
+    # layer step is one graph launch (those whose candidate bound is beyond
+    # the sample sort read the candidate count first, outside a graph)
     assert cg.tune_get("sample_sorts") > before
+    steps = max(B.num_layers(t) for t in range(len(cfgs)))
+    assert steps // 2 <= cg.tune_get("graph_steps") - g0 <= steps
     B.solve(use_graph=False)
     eager = [[B.values(t, i) for i in range(B.num_layers(t))] for t in range(len(cfgs))]
     for t, c in enumerate(cfgs):
@@ -827,6 +879,40 @@ def test_batch_spawn_buffer_repeat(cg):
     B.close()
 
 
+@pytest.mark.parametrize("mode", ["eager", "two_syncs", "carry_main"])
+def test_batch_step_modes(cg, mode):
+    """The batched step's launches enqueued one by one (batch_graph 0), with
+    the candidate count read before the M sort (one_sync 0), or with the
+    carry sort on the main stream inside the step's graph: the same layers
+    and values as each table alone."""
+    from cablegen_amd.batch_configs import batch
+    cfgs = batch(10, lo=3, hi=14)
+    r = cg.rules(ignore_f=True)
+    B = cg.Batch(r)
+    for c in cfgs:
+        B.add(c["initial"], c["start"], c["end"], c["winstates"])
+    knob = {"eager": ("batch_graph", 0), "two_syncs": ("one_sync", 0), "carry_main": ("carry_stream", 0)}[mode]
+    try:
+        cg.tune_set(*knob)
+        g0 = cg.tune_get("graph_steps")
+        B.generate()
+        launched = cg.tune_get("graph_steps") - g0
+        B.solve(use_graph=False)
+    finally:
+        cg.tune_set("reset", 0)
+    assert (launched == 0) == (mode != "carry_main")
+    for t, c in enumerate(cfgs):
+        tb = cg.Tablebase(r)
+        tb.generate(c["initial"], c["start"], c["end"])
+        tb.solve(c["winstates"])
+        assert B.num_layers(t) == tb.num_layers
+        for i in range(tb.num_layers):
+            assert np.array_equal(B.keys(t, i), tb.keys(i)), (t, i)
+            assert np.array_equal(B.values(t, i).view(np.uint64), tb.values(i).view(np.uint64)), (t, i)
+        tb.close()
+    B.close()
+
+
 def test_batch_rejects_bad_tables(cg):
     B = cg.Batch(cg.rules(ignore_f=True))
     with pytest.raises(cg.CablegenError):
@@ -843,19 +929,36 @@ def _sample_sort_check(cg, a, expect_sample=True):
         assert cg.tune_get("sample_sorts") > before
 
 
-@pytest.mark.parametrize("n", [2, 3, 1000, 16383, 16384, 16385, 70001, 1000003, 5000011, 9500009])
+@pytest.mark.parametrize("n", [2, 3, 1000, 8191, 8192, 8193, 16383, 16384, 16385, 70001, 1000003, 5000011,
+                               9500009, 14000009])
 def test_sample_sort_sizes(cg, n):
     """The sample sort (splitters from a sample, one partition pass into
     LDS-sized buckets, each bucket deduplicated and sorted in LDS) against
-    numpy's unique: one-block sizes around 16384 keys and multi-bucket sizes
-    up to ~1e7, on DPDGap boards with the duplicate rate of a move-candidate
-    set and on full 64-bit keys (up to 2048 buckets of 5000 keys)."""
+    numpy's unique: one-block sizes around 8192 keys and multi-bucket sizes
+    up to 1.4e7, on DPDGap boards with the duplicate rate of a move-candidate
+    set and on full 64-bit keys (up to 2048 buckets of 5000 to 7000 keys)."""
     rng = np.random.default_rng(n)
     b = rand_boards(rng, 0x00000000FFF0FFF0, max(n // 3, 1))
     a = b[rng.integers(0, len(b), size=n)]
     _sample_sort_check(cg, a)
     if n <= 1000003:
         _sample_sort_check(cg, rng.integers(0, 2**64 - 1, size=n, dtype=np.uint64, endpoint=True))
+
+
+@pytest.mark.parametrize("n", [12000007, 30000001])
+def test_sample_sort_8192_buckets(cg, n):
+    """Up to 8192 buckets (knob sample_bmax): splitters from 32768 or 65536
+    samples ranked in groups of runs, the partition with 8192 splitters in
+    LDS; against numpy's unique on DPDGap boards with repeats."""
+    rng = np.random.default_rng(n + 1)
+    b = rand_boards(rng, 0x00000000FFF0FFF0, n // 3)
+    a = b[rng.integers(0, len(b), size=n)]
+    try:
+        cg.tune_set("sample_bmax", 8192)
+        _sample_sort_check(cg, a)
+    finally:
+        cg.tune_set("reset", 0)
+    assert cg.tune_get("sample_bmax") == 2048
 
 
 def test_sample_sort_key_shapes(cg):
@@ -874,6 +977,13 @@ def test_sample_sort_key_shapes(cg):
     # sparse keys: a few clusters far apart
     c = rng.integers(0, 4, n).astype(np.uint64) << np.uint64(50)
     _sample_sort_check(cg, c | rng.integers(0, 1 << 40, n).astype(np.uint64))
+    # buckets of nearly all distinct keys spanning 2^32 or more (one block of
+    # 8190, and 5000-key buckets): the LDS hash set gives up and the bucket is
+    # sorted whole, equal runs dropped; a few repeats among them
+    for m in (8190, 40000):
+        x = rng.integers(0, 2**64 - 1, size=m, dtype=np.uint64, endpoint=True)
+        x[rng.integers(0, m, 50)] = x[0]
+        _sample_sort_check(cg, x)
 
 
 def test_sample_sort_oversize_repeats_on_lsd(cg):
