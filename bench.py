@@ -858,6 +858,23 @@ def main():
     value = total / T
 
     roof = stage_roofline(cg, dom, args.steps)
+    if dom == "sort" and args.carry_stream == 1:
+        # the same stage with nothing beside it: one untimed step with the
+        # carry sort on the main stream, so no kernel of another stream
+        # shares the GPU with a sort's kernels while they are timed
+        cg.prof_reset()
+        cg.tune_set("carry_stream", 0)
+        step()
+        cg.lib().cg_dev_sync()
+        iso = stage_roofline(cg, dom, 1)
+        cg.tune_set("carry_stream", 1)
+        roof["isolated"] = {"achieved": iso["achieved"], "frac": iso["frac"],
+                            "avg_unit_us": iso["avg_unit_us"], "units": iso["units_per_step"],
+                            "avg_algorithmic_bytes": iso["avg_algorithmic_bytes"],
+                            "source": "one untimed step with the carry sort on the main stream: "
+                                      "every sort of the step (move candidates and carries) timed "
+                                      "with no other stream's kernels beside it (the timed steps "
+                                      "overlap the candidate sorts with the carry sorts)"}
     cg.prof_select(None)
     cg.prof_enable(False)
 

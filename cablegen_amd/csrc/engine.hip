@@ -958,7 +958,8 @@ static int gen_step(const cg_rules &r, const uint64_t *P, size_t nP, uint64_t *C
     int hu4;
     pc.vary4 = vary_of(sumE, or4, and4);
     pc.cell4 = cell_bound(vb, s + 4, and4);
-    if (sort_enqueue(S4.p, sumE, pc.vary4, nxtC.p, 6, &hu4, g_carry_lane, pc.cell4)) return -1;
+    if (sort_enqueue(S4.p, sumE, pc.vary4, nxtC.p, 6, &hu4, g_carry_lane, pc.cell4, SORT_SAMPLE_SPARSE))
+        return -1;
     pc.cin = S4.p;
     pc.s4 ^= 1;
     if (vb) vb->s = s + 2;
@@ -1695,7 +1696,7 @@ extern "C" int cg_tune_set(const char *knob, int value)
         g_one_sync = 1;
         g_batch_graph = 1;
         sort_set_sample(1, 5000, 0);
-        sort_set_sample_limits(2048, 0);
+        sort_set_sample_limits(8192, 0);
         if (!strcmp(knob, "reset") && !g_tune_path.empty()) return cg_tune_load(g_tune_path.c_str());
         return 0;
     }
@@ -2476,7 +2477,7 @@ extern "C" int cg_batch_generate(cg_batch *b)
             if (g_carry_lane == 1 && fork_lane1()) return -1;
             int hu4;
             if (sort_enqueue(const_cast<uint64_t *>(df.c_in), df.c_n, df.c_vary, cbuf[cw]->p, 6, &hu4,
-                             g_carry_lane, batch_cells))
+                             g_carry_lane, batch_cells, SORT_SAMPLE_SPARSE))
                 return -1;
             pc.hu4 = hu4;
         }
@@ -2512,18 +2513,21 @@ extern "C" int cg_batch_generate(cg_batch *b)
         if (df.carry) {
             /* the carry sort's buffers and look-back words (second stream) */
             const int lane = g_carry_lane;
-            const size_t pk = sort_part_keys(df.c_n, SORT_SAMPLE);
+            const size_t pk = sort_part_keys(df.c_n, SORT_SAMPLE_SPARSE);
             if (lane == 1 ? (ensure(E.B2, df.c_n + 1) || ensure_scratch2(df.c_n) || (pk && ensure(E.Pt2, pk)))
                           : (ensure(E.B, df.c_n + 1) || (pk && ensure(E.Pt, pk))))
                 return -1;
-            if (sort_prepare(lane == 1 ? E.st2 : E.st, lane, df.c_n, df.c_vary, batch_cells, SORT_SAMPLE))
+            if (sort_prepare(lane == 1 ? E.st2 : E.st, lane, df.c_n, df.c_vary, batch_cells, SORT_SAMPLE_SPARSE))
                 return fail_msg("sort: look-back words");
         }
         if (graph && capture_begin()) return -1;
         int rc = enqueue_deferred();
+        int hm = -1;
         if (!rc) {
             launch_init_stats(E.st, E.d_cnt);
+            hm = prof_begin(PK_MOVE);
             launch_bmove(E.st, b->P.p, nP, nullptr, E.A.p, E.d_cnt, lut, move_rules(b->r), b->d_tabs, k, E.d_err);
+            prof_end(hm, 0);
         }
         size_t ncand = 0;
         uint64_t mor = 0, mand = ~0ULL;
@@ -2550,12 +2554,15 @@ extern "C" int cg_batch_generate(cg_batch *b)
             }
         }
         const size_t mrows = dev ? cmax : ncand;
+        int hs = -1;
         auto enqueue_decode_spawn = [&](size_t capk) {
             launch_tstat_init(E.st, b->d_tstat, ntab);
             launch_bdecode(E.st, b->M.p, mrows, E.d_cnt + 7, st.keys, b->d_tabs, b->d_tstat);
             launch_init_stats(E.st, E.d_cnt + 8);
+            hs = prof_begin(PK_SPAWN);
             launch_bspawn(E.st, b->M.p, mrows, E.d_cnt + 7, b->P2.p, S4.p, capk, E.d_cnt + 8, b->d_tabs, k,
                           E.d_err);
+            prof_end(hs, 0);
             launch_copy_and_u64(E.st, (uint64_t *)b->hd_tpub, (const uint64_t *)b->d_tstat, 3 * ntab, ~0ULL);
         };
         if (!rc) {
@@ -2605,7 +2612,9 @@ extern "C" int cg_batch_generate(cg_batch *b)
         }
         if (!settled) return fail_msg("batch step: spawns or sort not settled");
         pc.last_nM = nM;
+        prof_set_bytes(hm, 8.0 * nP + 8.0 * ncand);
         prof_set_bytes(hu, 8.0 * ncand + 8.0 * nM);
+        prof_set_bytes(hs, 8.0 * nM + 16.0 * sumE);
         b->arena.shrink_last(st.keys, std::max<size_t>(cmax, 1) * 8, std::max<size_t>(nM, 1) * 8);
         for (int t = 0; t < ntab; t++)
             for (int j = 0; j < 3; j++) b->h_tstat[3 * t + j] = ((volatile unsigned long long *)b->h_tpub)[3 * t + j];

@@ -167,9 +167,12 @@ constexpr unsigned SORT_ERR_OVERSIZE = 8u;
  * allow_sample is set; it only reads keys, and a bucket above its capacity
  * ORs SORT_ERR_OVERSIZE into *d_err: the caller then repeats the sort with
  * allow_sample = false (the LSD path, which takes any size). */
-/* sample: SORT_LSD (the LSD path) or SORT_SAMPLE (the sample sort for the
- * sizes it takes, up to sort_sample_limit(SORT_SAMPLE) keys) */
-constexpr int SORT_LSD = 0, SORT_SAMPLE = 1;
+/* sample: SORT_LSD (the LSD path), SORT_SAMPLE (the sample sort for the
+ * sizes it takes, up to sort_sample_limit(SORT_SAMPLE) keys, with buckets
+ * for keys that repeat as move candidates do) or SORT_SAMPLE_SPARSE (the
+ * same with smaller buckets, for keys with few repeats, such as a step's
+ * spawn-4 boards) */
+constexpr int SORT_LSD = 0, SORT_SAMPLE = 1, SORT_SAMPLE_SPARSE = 2;
 uint64_t *sort_unique(hipStream_t st, uint64_t *keys, uint64_t *tmp, size_t n, uint64_t vary,
                       void *scratch, uint64_t *dst, unsigned long long *d_nuniq, unsigned *d_err,
                       int *prof_unique, int ctx = 0, uint64_t cell_max = CELLS_UNBOUNDED,

@@ -552,16 +552,17 @@ constexpr int SS_SMAX = 4 * SS_S;            /* samples of the largest sorts */
  * distinct values must fit the LDS) */
 constexpr int SS_KEYCAP = SS_NX * SS_SUBCAP;
 
-/* Buckets of a sample sort of n keys with a mean of bavg keys per bucket: 1
- * (one block takes every key) up to SS_LCAP64 keys, which a bucket of keys
- * spanning 2^32 or more holds as distinct 64-bit values, else ceil(n / bavg)
- * with at least 2 and at most sh.bmax (larger sorts have larger buckets).
+/* Buckets of a sample sort of n keys: 1 (one block takes every key) up to
+ * SS_LCAP64 keys, which a bucket of keys spanning 2^32 or more holds as
+ * distinct 64-bit values, else ceil(n / sh.bavg) with at least 2, and for
+ * the largest sorts buckets of sh.bhi keys up to sh.bmax of them.
  * The host and the kernels compute it alike, so a sort whose size is only
  * known on the device sizes its grids for the bound and the kernels agree on
  * B. */
 struct SsShape {
-    int bavg;   /* mean keys per bucket */
-    int bmax;   /* most buckets (larger sorts have larger buckets) */
+    int bavg;   /* mean keys per bucket up to SS_BSMALL buckets */
+    int bhi;    /* mean keys per bucket of the larger sorts (>= bavg) */
+    int bmax;   /* most buckets */
 };
 
 __host__ __device__ __forceinline__ size_t ss_nbuckets(size_t n, SsShape sh)
@@ -569,6 +570,11 @@ __host__ __device__ __forceinline__ size_t ss_nbuckets(size_t n, SsShape sh)
     if (n <= (size_t)SS_LCAP64) return 1;
     size_t b = (n + (size_t)sh.bavg - 1) / (size_t)sh.bavg;
     b = b < 2 ? 2 : b;
+    if (b <= (size_t)SS_BSMALL) return b;
+    /* SS_BSMALL buckets while their mean stays within bhi, then buckets of
+     * bhi keys: B never decreases as n grows */
+    if (n <= (size_t)SS_BSMALL * (size_t)sh.bhi) return SS_BSMALL;
+    b = (n + (size_t)sh.bhi - 1) / (size_t)sh.bhi;
     return b > (size_t)sh.bmax ? (size_t)sh.bmax : b;
 }
 
@@ -1010,6 +1016,7 @@ __global__ __launch_bounds__(SS_T, BM <= SS_BSMALL ? 2 : 1) void k_ss_part(const
  * below it up to a load of about 3/4; a set filled closer than that would
  * make each insert walk long runs of slots (quadratic in the keys). */
 constexpr int SS_PROBES = 64;
+constexpr int SS_PROBES_FAR = 512;
 
 /* Deduplicates the bucket's keys (the SS_NX regions of L.segc[x] keys at s +
  * x * SS_SUBCAP) as values k - base of type V through an LDS hash set: the
@@ -1240,9 +1247,10 @@ __global__ __launch_bounds__(SS_T, 8) void k_ss_local(const uint64_t *__restrict
     int nb = 0;
     /* a bucket the direct sort can take gives up on a nearly full set (a
      * longer probe costs more than sorting every key); a larger one probes
-     * on, since its only other way is a repeat of the whole sort */
+     * further, since its only other way is a repeat of the whole sort, but
+     * not without bound (at a load near 1 every insert would walk the set) */
     const bool direct_ok = !over && m <= (size_t)SS_T * (wide ? SS_KS64 : SS_KS32);
-    const int probes = direct_ok ? probes_direct : (wide ? SS_LCAP64 : SS_LCAP);
+    const int probes = direct_ok ? probes_direct : SS_PROBES_FAR;
     int n = wide ? ss_dedupe<uint64_t, SS_KS64>(s, base, L, &vmin, &nb, probes)
                  : ss_dedupe<uint32_t, SS_KS32>(s, base, L, &vmin, &nb, probes);
     bool sorted = false;
@@ -1582,11 +1590,9 @@ static OsScratch carve(void *scratch)
 /* ---- sample sort: configuration ---- */
 
 static int g_ss_on = 1;              /* sample sort for sizes it handles */
-static int g_ss_bavg = 5000;         /* mean keys per bucket */
+static int g_ss_bavg = 5000;         /* mean keys per bucket (move candidates) */
 static int g_ss_cap = SS_KEYCAP;     /* keys of a bucket (lowered only to test the repeat) */
-/* most buckets of a sort: SS_BSMALL (the default: C2's layer steps and the
- * batched steps measured faster with it) or SS_BMAX */
-static int g_ss_bmax = SS_BSMALL;
+static int g_ss_bmax = SS_BMAX;      /* most buckets of a sort (SS_BSMALL or SS_BMAX) */
 static int g_ss_probes = SS_PROBES;  /* longest hash-set probe of a bucket the direct sort can take */
 static unsigned long long g_ss_sorts = 0;
 
@@ -1608,23 +1614,35 @@ int sort_sample_bucket() { return g_ss_bavg; }
 int sort_sample_cap() { return g_ss_cap; }
 unsigned long long sort_sample_count() { return g_ss_sorts; }
 
-/* The largest sort the sample sort takes: SS_BMAX buckets of up to
- * SS_BAVG_MAX keys on average (the largest sorts are the layers' move
- * candidates, which hold each key about 3.6 times, so their buckets have few
- * distinct values); beyond it, the LSD path. */
-constexpr size_t SS_BAVG_MAX = 7000;
+/* Bucket shapes.  Move candidates hold each key about 3.6 times, so a
+ * bucket of 5000 to 7000 keys has some 1400 to 2000 distinct values;
+ * spawn-4 keys repeat far less, and a bucket of them whose keys span 2^32
+ * or more must keep its distinct values within the 8192 of a 64-bit set:
+ * 3000 to 3500 keys. */
+static SsShape ss_shape(int mode)
+{
+    if (mode == SORT_SAMPLE_SPARSE) return SsShape{3000, 3500, g_ss_bmax};
+    return SsShape{g_ss_bavg, g_ss_bavg > 7000 ? g_ss_bavg : 7000, g_ss_bmax};
+}
+
+/* The largest sort a shape takes: sh.bmax buckets of sh.bhi keys.  Larger
+ * buckets were measured: batched candidate sets of 57M to 115M keys in
+ * buckets of up to 14000 keys (their moves are already deduplicated per
+ * round, so the buckets hold mostly distinct keys spanning 2^32 or more)
+ * took the batched generation from 88 to 290 ms against the LSD path. */
 size_t sort_sample_limit(int mode)
 {
     if (!g_ss_on || mode <= 0) return 0;
-    return (size_t)g_ss_bmax * SS_BAVG_MAX;
+    const SsShape sh = ss_shape(mode);
+    return (size_t)sh.bmax * (size_t)sh.bhi;
 }
 
 /* buckets of a sample sort of n keys (1: one block), or 0 when the sort
  * takes the LSD path */
-static size_t ss_buckets(size_t n, int mode = 1)
+static size_t ss_buckets(size_t n, int mode = SORT_SAMPLE)
 {
     if (n < 2 || n > sort_sample_limit(mode)) return 0;
-    return ss_nbuckets(n, SsShape{g_ss_bavg, g_ss_bmax});
+    return ss_nbuckets(n, ss_shape(mode));
 }
 
 size_t sort_part_keys(size_t n, int mode)
@@ -1721,12 +1739,11 @@ static void launch_pass(hipStream_t st, SortCfg c, unsigned tiles, const uint64_
  * ss_nbuckets(nmax) buckets at most, part holding Bmax * SS_BSTRIDE keys when
  * Bmax > 1; keys are only read. */
 static void ss_sort_unique(hipStream_t st, const uint64_t *keys, const unsigned long long *dn, size_t nmax,
-                           size_t Bmax, uint64_t *part, const OsScratch &s, SortCtx &sc, uint64_t *dst,
-                           unsigned long long *d_nuniq, unsigned *d_err, int *prof_unique)
+                           SsShape sh, size_t Bmax, uint64_t *part, const OsScratch &s, SortCtx &sc,
+                           uint64_t *dst, unsigned long long *d_nuniq, unsigned *d_err, int *prof_unique)
 {
     g_ss_sorts++;
     const uint32_t epoch = next_epoch(st, sc);
-    const SsShape sh{g_ss_bavg, g_ss_bmax};
     if (Bmax > 1) {
         launch_fill_u64(st, (uint64_t *)s.ss_ticket, (256 + Bmax * SS_NX * 8) / 8, 0);
         int h = prof_hook_begin(8, st);
@@ -1783,10 +1800,11 @@ int sort_unique_dev(hipStream_t st, const uint64_t *keys, const unsigned long lo
     *prof_unique = -1;
     if (nmax > sort_sample_limit(SORT_SAMPLE)) return -1;
     SortCtx &sc = g_ctx[ctx < 0 || ctx >= SORT_CONTEXTS ? 0 : ctx];
-    const size_t Bmax = ss_nbuckets(nmax, SsShape{g_ss_bavg, g_ss_bmax});
+    const SsShape sh = ss_shape(SORT_SAMPLE);
+    const size_t Bmax = ss_nbuckets(nmax, sh);
     if (Bmax > 1 && (!part || part_cap < Bmax * (size_t)SS_BSTRIDE)) return -1;
     if (!status_reserve(st, sc, Bmax)) return -1;
-    ss_sort_unique(st, keys, dn, nmax, Bmax, part, carve(scratch), sc, dst, d_nuniq, d_err, prof_unique);
+    ss_sort_unique(st, keys, dn, nmax, sh, Bmax, part, carve(scratch), sc, dst, d_nuniq, d_err, prof_unique);
     return 0;
 }
 
@@ -1872,7 +1890,8 @@ uint64_t *sort_unique(hipStream_t st, uint64_t *keys, uint64_t *tmp, size_t n, u
     const SortPlan P = sort_plan(n, vary, cell_max, sample_ok ? sample : 0);
     if (!status_reserve(st, sc, P.entries)) return keys;
     if (P.B >= 1) {
-        ss_sort_unique(st, keys, nullptr, n, P.B, part, s, sc, dst, d_nuniq, d_err, prof_unique);
+        ss_sort_unique(st, keys, nullptr, n, ss_shape(sample), P.B, part, s, sc, dst, d_nuniq, d_err,
+                       prof_unique);
         return keys;
     }
     if (P.coded) g_code_sorts++;

@@ -945,20 +945,24 @@ def test_sample_sort_sizes(cg, n):
         _sample_sort_check(cg, rng.integers(0, 2**64 - 1, size=n, dtype=np.uint64, endpoint=True))
 
 
-@pytest.mark.parametrize("n", [12000007, 30000001])
-def test_sample_sort_8192_buckets(cg, n):
-    """Up to 8192 buckets (knob sample_bmax): splitters from 32768 or 65536
-    samples ranked in groups of runs, the partition with 8192 splitters in
-    LDS; against numpy's unique on DPDGap boards with repeats."""
+@pytest.mark.parametrize("n", [12000007, 20000003, 30000001])
+def test_sample_sort_large(cg, n):
+    """Sorts above 2048 buckets of 5000 keys: 2048 buckets up to 14.3M keys,
+    then buckets of 7000 keys (up to 8192): splitters from 16384, 32768 or
+    65536 samples (the larger samples ranked in groups of runs), the
+    partition with up to 8192 splitters in LDS; against numpy's unique on
+    DPDGap boards with repeats.  With at most 2048 buckets (knob
+    sample_bmax) the largest take the LSD path."""
     rng = np.random.default_rng(n + 1)
     b = rand_boards(rng, 0x00000000FFF0FFF0, n // 3)
     a = b[rng.integers(0, len(b), size=n)]
+    _sample_sort_check(cg, a)
     try:
-        cg.tune_set("sample_bmax", 8192)
-        _sample_sort_check(cg, a)
+        cg.tune_set("sample_bmax", 2048)
+        _sample_sort_check(cg, a, expect_sample=n <= 14336000)
     finally:
         cg.tune_set("reset", 0)
-    assert cg.tune_get("sample_bmax") == 2048
+    assert cg.tune_get("sample_bmax") == 8192
 
 
 def test_sample_sort_key_shapes(cg):
