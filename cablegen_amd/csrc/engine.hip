@@ -1660,6 +1660,11 @@ extern "C" int cg_tune_set(const char *knob, int value)
         sort_set_sample_limits(value, sort_sample_probes());
         return 0;
     }
+    if (!strcmp(knob, "bucket_bitonic")) {
+        if (value != 0 && value != 1) return fail_msg("bucket_bitonic: 0 or 1");
+        sort_set_bucket_bitonic(value);
+        return 0;
+    }
     if (!strcmp(knob, "sample_probes")) {
         if (value < 0 || value > 16384) return fail_msg("sample_probes: 0 (default) or 1..16384");
         sort_set_sample_limits(sort_sample_bmax(), value);
@@ -1697,6 +1702,7 @@ extern "C" int cg_tune_set(const char *knob, int value)
         g_batch_graph = 1;
         sort_set_sample(1, 5000, 0);
         sort_set_sample_limits(8192, 0);
+        sort_set_bucket_bitonic(0);
         if (!strcmp(knob, "reset") && !g_tune_path.empty()) return cg_tune_load(g_tune_path.c_str());
         return 0;
     }
@@ -1717,6 +1723,7 @@ extern "C" int cg_tune_get(const char *knob)
     if (!strcmp(knob, "one_sync")) return g_one_sync;
     if (!strcmp(knob, "batch_graph")) return g_batch_graph;
     if (!strcmp(knob, "sample_bmax")) return sort_sample_bmax();
+    if (!strcmp(knob, "bucket_bitonic")) return sort_bucket_bitonic();
     if (!strcmp(knob, "sample_probes")) return sort_sample_probes();
     if (!strcmp(knob, "graph_steps")) return (int)std::min<unsigned long long>(g_graph_steps, 0x7fffffff);
     if (!strcmp(knob, "graph_instantiations"))

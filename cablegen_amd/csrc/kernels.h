@@ -207,6 +207,11 @@ void sort_set_sample(int on, int bucket_avg, int cap);
  * bucket the direct sort can take (0: the default) */
 void sort_set_sample_limits(int bmax, int probes);
 int sort_sample_bmax();
+/* buckets that fit one block sorted whole by a bitonic network first (1),
+ * or through the LDS hash set and an LSD sort of the distinct values, with
+ * the network only when the set fills (0, the default) */
+void sort_set_bucket_bitonic(int on);
+int sort_bucket_bitonic();
 int sort_sample_probes();
 int sort_sample_on();
 int sort_sample_bucket();

@@ -1095,60 +1095,139 @@ __device__ __forceinline__ int ss_dedupe(const uint64_t *__restrict__ s, uint64_
     return n;
 }
 
-/* The bucket of m keys (its SS_NX regions, L.segc[r] keys each) when the
- * hash set could not take its distinct values: all m keys, as values k -
- * base of type V less the smallest of them (L.vlo, which the hash-set pass
- * found, as it found the largest, L.vhi), are sorted in one block and the
- * first of each run of equal values is kept.  m <= SS_T * K.  On return the
- * stage's first n entries hold the distinct values ascending, relative to
- * *vmin; returns n. */
-template <typename V, int K>
-__device__ __forceinline__ int ss_direct(const uint64_t *__restrict__ s, uint64_t base, int m, SsLds &L,
-                                         uint64_t *vmin)
+/* the value a bitonic stage keeps of a and its partner o: the smaller when
+ * keep_lo */
+template <typename V>
+__device__ __forceinline__ V ss_pick(V a, V o, bool keep_lo)
 {
-    const int w = threadIdx.x >> 6, lane = lane_id();
-    const V vl = (V)L.vlo, vh = (V)L.vhi;
-    const int nb = vh > vl ? 64 - __clzll((long long)(uint64_t)(vh - vl)) : 0;
-    const int chunk = ss_chunk<K>(m), rem = ss_rem(m, chunk);
+    return keep_lo ? (a < o ? a : o) : (a < o ? o : a);
+}
+
+/* the stages of distance D < R, within each thread: pairs (r, r + D) */
+template <typename V, int R, int D>
+__device__ __forceinline__ void ss_intra(V (&v)[R], uint32_t first, uint32_t size)
+{
+#pragma unroll
+    for (int r = 0; r < R; r++)
+        if (!(r & D)) {
+            const bool up = ((first + (uint32_t)r) & size) == 0;
+            const V a = v[r], b = v[r + D];
+            const V lo = a < b ? a : b, hi = a < b ? b : a;
+            v[r] = up ? lo : hi;
+            v[r + D] = up ? hi : lo;
+        }
+}
+
+/* Bitonic sort of SS_T * R values held R per thread: value i = thread * R +
+ * r is v[r].  Pairs less than R apart are exchanged within a thread, less
+ * than 64 R apart between the lanes of a wave (shuffles), and the rest
+ * through the stage in LDS (r-major, so every access is conflict free), one
+ * barrier pair each: 10 of the network's stages whatever R is.  For
+ * distances of R or more the pair side and the direction are the thread's.
+ * The stage loops run at run time (unrolled, every stage's constants would
+ * stay live and spill); only the exchanges within a thread are unrolled,
+ * one instance per distance. */
+template <typename V, int R>
+__device__ __forceinline__ void ss_bitonic(V (&v)[R], SsLds &L)
+{
+    constexpr int LR = R == 1 ? 0 : R == 2 ? 1 : R == 4 ? 2 : R == 8 ? 3 : 4;
+    static_assert((1 << LR) == R && SS_T == 1024, "R a power of two up to 16, 1024 threads");
+    constexpr int LOGN = 10 + LR;
+    V *T = ss_stage<V>(L);
+    const uint32_t tid = threadIdx.x, first = tid * R;
+    for (int ls = 1; ls <= LOGN; ls++) {
+        const uint32_t size = 1u << ls;
+        for (int ld = ls - 1; ld >= 0; ld--) {
+            const uint32_t d = 1u << ld;
+            if (ld >= LR + 6) {
+                const uint32_t tx = d >> LR;
+                const bool keep_lo = ((first & d) == 0) == ((first & size) == 0);
+                __syncthreads();   /* the stage's last readers are done */
+#pragma unroll
+                for (int r = 0; r < R; r++) T[r * SS_T + tid] = v[r];
+                __syncthreads();
+#pragma unroll
+                for (int r = 0; r < R; r++) v[r] = ss_pick(v[r], T[r * SS_T + (tid ^ tx)], keep_lo);
+            } else if (ld >= LR) {
+                const int lx = 1 << (ld - LR);
+                const bool keep_lo = ((first & d) == 0) == ((first & size) == 0);
+#pragma unroll
+                for (int r = 0; r < R; r++) v[r] = ss_pick(v[r], (V)__shfl_xor(v[r], lx, 64), keep_lo);
+            } else {
+                switch (ld) {
+                case 0: ss_intra<V, R, 1>(v, first, size); break;
+                case 1: if constexpr (R > 2) ss_intra<V, R, 2>(v, first, size); break;
+                case 2: if constexpr (R > 4) ss_intra<V, R, 4>(v, first, size); break;
+                default: if constexpr (R > 8) ss_intra<V, R, 8>(v, first, size); break;
+                }
+            }
+        }
+    }
+    __syncthreads();
+}
+
+/* The bucket of m <= SS_T * R keys (its SS_NX regions, L.segc[r] keys
+ * each), as values k - base of type V, sorted by the bitonic network (the
+ * positions past m hold the all-ones value, which no key's value reaches);
+ * the first of each run of equal values is kept.  On return the stage's
+ * first n entries hold the distinct values ascending (relative to base);
+ * returns n. */
+template <typename V, int R>
+__device__ __forceinline__ int ss_bitonic_bucket(const uint64_t *__restrict__ s, uint64_t base, int m, SsLds &L)
+{
+    const uint32_t tid = threadIdx.x;
     uint32_t pre[SS_NX + 1];
     pre[0] = 0;
 #pragma unroll
     for (int r = 0; r < SS_NX; r++) pre[r + 1] = pre[r] + L.segc[r];
-    V v[K];
+    V v[R];
 #pragma unroll
-    for (int j = 0; j < K; j++) {
-        v[j] = 0;
-        if (j * 64 + lane < rem) {
-            const uint32_t i = (uint32_t)(w * chunk + j * 64 + lane);
+    for (int j = 0; j < R; j++) {
+        const uint32_t i = (uint32_t)j * SS_T + tid;   /* loads in key order: coalesced */
+        v[j] = (V)~(V)0;
+        if (i < (uint32_t)m) {
             int r = 0;
 #pragma unroll
             for (int q = 1; q < SS_NX; q++) r += i >= pre[q];
-            v[j] = (V)((V)(s[(size_t)r * SS_SUBCAP + (i - pre[r])] - base) - vl);
+            v[j] = (V)(s[(size_t)r * SS_SUBCAP + (i - pre[r])] - base);
         }
     }
-    ss_block_sort<V, K>(v, m, chunk, nb, L);
-    /* thread t keeps the run starts among positions tK .. tK + K - 1 */
-    V *T = ss_stage<V>(L);
-    V x[K];
+    ss_bitonic<V, R>(v, L);
+    /* thread t holds positions tR .. tR + R - 1; the last value of the
+     * thread before comes through LDS */
+    uint64_t *last = reinterpret_cast<uint64_t *>(L.cnt);
+    last[tid] = (uint64_t)v[R - 1];
+    __syncthreads();
+    const V before0 = tid ? (V)last[tid - 1] : (V)0;
     uint32_t keep = 0, c = 0;
 #pragma unroll
-    for (int j = 0; j < K; j++) {
-        const int p = (int)threadIdx.x * K + j;
-        x[j] = p < m ? T[p] : (V)0;
-        const V before = j ? x[j - 1] : (p > 0 && p <= m ? T[p - 1] : (V)0);
-        if (p < m && (p == 0 || x[j] != before)) {
-            keep |= 1u << j;
+    for (int r = 0; r < R; r++) {
+        const uint32_t p = tid * R + (uint32_t)r;
+        const V prev = r ? v[r - 1] : before0;
+        if (p < (uint32_t)m && (p == 0 || v[r] != prev)) {
+            keep |= 1u << r;
             c++;
         }
     }
     uint32_t all;
-    uint32_t o = block_excl_scan_dpp(c, &all, L.wtmp);   /* its barrier ends the reads */
+    uint32_t o = block_excl_scan_dpp(c, &all, L.wtmp);
+    V *T = ss_stage<V>(L);
 #pragma unroll
-    for (int j = 0; j < K; j++)
-        if ((keep >> j) & 1u) T[o++] = x[j];
+    for (int r = 0; r < R; r++)
+        if ((keep >> r) & 1u) T[o++] = v[r];
     __syncthreads();
-    *vmin = (uint64_t)vl;
     return (int)all;
+}
+
+/* the bitonic bucket sort with the fewest rows per thread that hold m keys */
+template <typename V>
+__device__ __forceinline__ int ss_bitonic_any(const uint64_t *__restrict__ s, uint64_t base, int m, SsLds &L)
+{
+    if (m <= SS_T) return ss_bitonic_bucket<V, 1>(s, base, m, L);
+    if (m <= 2 * SS_T) return ss_bitonic_bucket<V, 2>(s, base, m, L);
+    if (m <= 4 * SS_T) return ss_bitonic_bucket<V, 4>(s, base, m, L);
+    if (sizeof(V) == 8 || m <= 8 * SS_T) return ss_bitonic_bucket<V, 8>(s, base, m, L);
+    return ss_bitonic_bucket<V, sizeof(V) == 8 ? 8 : 16>(s, base, m, L);
 }
 
 /* Sorts the n distinct values at the front of the stage (ss_dedupe's
@@ -1187,7 +1266,7 @@ __global__ __launch_bounds__(SS_T, 8) void k_ss_local(const uint64_t *__restrict
                                                       uint64_t *__restrict__ dst, uint64_t *status,
                                                       unsigned *ticket, uint32_t epoch,
                                                       unsigned long long *total, unsigned *err, int cap,
-                                                      int probes_direct)
+                                                      int probes_direct, int bitonic)
 {
     __shared__ SsLds L;
     const int w = threadIdx.x >> 6, lane = lane_id();
@@ -1245,19 +1324,31 @@ __global__ __launch_bounds__(SS_T, 8) void k_ss_local(const uint64_t *__restrict
     }
     uint64_t vmin = 0;
     int nb = 0;
-    /* a bucket the direct sort can take gives up on a nearly full set (a
-     * longer probe costs more than sorting every key); a larger one probes
-     * further, since its only other way is a repeat of the whole sort, but
-     * not without bound (at a load near 1 every insert would walk the set) */
+    /* The keys go through the LDS hash set and the distinct values are
+     * sorted.  A bucket whose keys fit one block's registers (16384 32-bit
+     * or 8192 64-bit values) gives up on a nearly full set (a longer probe
+     * costs more than sorting every key) and is then sorted whole by the
+     * bitonic network, equal runs dropped; a larger one probes further,
+     * since its only other way is a repeat of the whole sort, but not
+     * without bound (at a load near 1 every insert would walk the set).
+     * With `bitonic` set, buckets that fit are sorted by the network
+     * straight away. */
     const bool direct_ok = !over && m <= (size_t)SS_T * (wide ? SS_KS64 : SS_KS32);
-    const int probes = direct_ok ? probes_direct : SS_PROBES_FAR;
-    int n = wide ? ss_dedupe<uint64_t, SS_KS64>(s, base, L, &vmin, &nb, probes)
-                 : ss_dedupe<uint32_t, SS_KS32>(s, base, L, &vmin, &nb, probes);
+    int n = -1;
     bool sorted = false;
+    if (bitonic && direct_ok && m > 0) {
+        n = wide ? ss_bitonic_any<uint64_t>(s, base, (int)m, L) : ss_bitonic_any<uint32_t>(s, base, (int)m, L);
+        sorted = true;
+    } else {
+        const int probes = direct_ok ? probes_direct : SS_PROBES_FAR;
+        n = wide ? ss_dedupe<uint64_t, SS_KS64>(s, base, L, &vmin, &nb, probes)
+                 : ss_dedupe<uint32_t, SS_KS32>(s, base, L, &vmin, &nb, probes);
+    }
     if (n < 0 && direct_ok) {
-        /* mostly distinct keys: sorted all together, equal runs dropped */
-        n = wide ? ss_direct<uint64_t, SS_KS64>(s, base, (int)m, L, &vmin)
-                 : ss_direct<uint32_t, SS_KS32>(s, base, (int)m, L, &vmin);
+        /* the hash set filled up (mostly distinct keys): every key sorted
+         * by the bitonic network, equal runs dropped */
+        n = wide ? ss_bitonic_any<uint64_t>(s, base, (int)m, L) : ss_bitonic_any<uint32_t>(s, base, (int)m, L);
+        vmin = 0;
         sorted = true;
     }
     if (n < 0) {   /* more distinct values than one block sorts: oversize */
@@ -1594,6 +1685,11 @@ static int g_ss_bavg = 5000;         /* mean keys per bucket (move candidates) *
 static int g_ss_cap = SS_KEYCAP;     /* keys of a bucket (lowered only to test the repeat) */
 static int g_ss_bmax = SS_BMAX;      /* most buckets of a sort (SS_BSMALL or SS_BMAX) */
 static int g_ss_probes = SS_PROBES;  /* longest hash-set probe of a bucket the direct sort can take */
+/* buckets that fit one block: bitonic sort of every key first (1), or the
+ * hash set first with the bitonic sort when the set fills (0, the default:
+ * C2 generated in 243-245 ms against 288-300 ms bitonic-first, since a
+ * candidate bucket's 5000 keys hold only ~1400 distinct values) */
+static int g_ss_bitonic = 0;
 static unsigned long long g_ss_sorts = 0;
 
 void sort_set_sample(int on, int bucket_avg, int cap)
@@ -1607,6 +1703,8 @@ void sort_set_sample_limits(int bmax, int probes)
     g_ss_bmax = bmax == SS_BSMALL ? SS_BSMALL : SS_BMAX;
     g_ss_probes = probes > 0 ? probes : SS_PROBES;
 }
+void sort_set_bucket_bitonic(int on) { g_ss_bitonic = on ? 1 : 0; }
+int sort_bucket_bitonic() { return g_ss_bitonic; }
 int sort_sample_bmax() { return g_ss_bmax; }
 int sort_sample_probes() { return g_ss_probes; }
 int sort_sample_on() { return g_ss_on; }
@@ -1788,7 +1886,7 @@ static void ss_sort_unique(hipStream_t st, const uint64_t *keys, const unsigned 
     *prof_unique = prof_hook_begin(10, st);
     hipLaunchKernelGGL(k_ss_local, dim3((unsigned)Bmax), dim3(SS_T), 0, st, (const uint64_t *)part, keys,
                        (const unsigned long long *)s.ss_cnt, (const uint64_t *)s.ss_spl, dn, nmax, sh, dst,
-                       sc.status, s.ss_ticket, epoch, d_nuniq, d_err, g_ss_cap, g_ss_probes);
+                       sc.status, s.ss_ticket, epoch, d_nuniq, d_err, g_ss_cap, g_ss_probes, g_ss_bitonic);
     snote(hipGetLastError());
     prof_hook_end(*prof_unique, 8.0 * nmax, st);
 }

@@ -990,6 +990,30 @@ def test_sample_sort_key_shapes(cg):
         _sample_sort_check(cg, x)
 
 
+@pytest.mark.parametrize("n", [700, 1500, 3000, 8190, 16000, 40000, 300000])
+def test_sample_sort_bitonic_buckets(cg, n):
+    """Buckets sorted whole by the bitonic network (knob bucket_bitonic; by
+    default only buckets whose hash set fills take it): every row count per
+    thread (1 to 16 for 32-bit values, up to 8 for 64-bit ones), one-block
+    and multi-bucket sorts, with repeats, against numpy's unique."""
+    rng = np.random.default_rng(n + 7)
+    b = rand_boards(rng, 0x00000000FFF0FFF0, max(n // 2, 1))
+    a = b[rng.integers(0, len(b), size=n)]
+    x = rng.integers(0, 2**64 - 1, size=n, dtype=np.uint64, endpoint=True)
+    x[rng.integers(0, n, n // 10)] = x[1]
+    try:
+        cg.tune_set("bucket_bitonic", 1)
+        _sample_sort_check(cg, a)
+        _sample_sort_check(cg, x)
+        if n >= 40000:
+            # buckets of ~12000 32-bit values: 16 rows per thread
+            cg.tune_set("sample_bucket", 12000)
+            _sample_sort_check(cg, a)
+    finally:
+        cg.tune_set("reset", 0)
+    assert cg.tune_get("bucket_bitonic") == 0
+
+
 def test_sample_sort_oversize_repeats_on_lsd(cg):
     """A bucket above the capacity (one key repeated 60000 times, or every
     bucket once the capacity is lowered) makes the library repeat the sort on
