@@ -270,8 +270,9 @@ def micro(args):
 PMC_KERNELS = {"move": "::k_move", "spawn": "::k_spawn(", "sort_hist": "::k_os_hist",
                "sort_pass": "::k_os_pass", "unique": "::k_os_unique",
                "sort_split:gather": "::k_ss_gather(", "sort_split:runs": "::k_ss_runs(",
-               "sort_split:select": "::k_ss_select(",
-               "sort_part": "::k_ss_part(", "sort_local": "::k_ss_local(", "solve": "::k_solve(",
+               "sort_split:select": "::k_ss_select(", "sort_split:rank": "::k_ss_rank(",
+               "sort_split:pick": "::k_ss_pick(",
+               "sort_part": "::k_ss_part", "sort_local": "::k_ss_local(", "solve": "::k_solve(",
                "lookup": "::k_lookup(", "adjoint": "::k_adjoint("}
 # SQ counter sets (one rocprofv3 run each) and the ratios printed from them:
 # per-wave cycles issuing VALU / LDS / VMEM, waiting on LDS or on anything,
@@ -322,8 +323,12 @@ def prof_summary(args):
             print("\nBy stream (main = the stream running `k_move`; times from the kernel trace):\n")
             print("| kernel | main: calls | main: avg us | second: calls | second: avg us |")
             print("|---|---:|---:|---:|---:|")
-            for name, pat in (("sort_hist", "::k_os_hist"), ("sort_pass", "::k_os_pass"),
-                              ("unique", "::k_os_unique")):
+            for name, pat in (("sort_split", "::k_ss_gather"), ("sort_split", "::k_ss_runs"),
+                              ("sort_split", "::k_ss_select"), ("sort_part", "::k_ss_part"),
+                              ("sort_local", "::k_ss_local"), ("sort_hist", "::k_os_hist"),
+                              ("sort_pass", "::k_os_pass"), ("unique", "::k_os_unique"),
+                              ("move", "::k_move_gen"), ("spawn", "::k_spawn("), ("solve", "::k_solve(")):
+                name = "%s (%s)" % (name, pat.strip(":(")) if name.startswith("sort_split") else name
                 acc = {True: [], False: []}
                 for r in rows:
                     if pat in r["Kernel_Name"]:
@@ -727,9 +732,9 @@ def stage_traffic(kinds, final):
     with open(tf) as f:
         tj = json.load(f)
     km = tj.get("kernels_main_stream", {})
-    if tj.get("workload") != "c2" or not all(k in km for k in final):
+    if tj.get("workload") != "c2" or not any(k in km for k in final):
         return None, None
-    units = sum(km[k]["launches"] for k in final)
+    units = sum(km[k]["launches"] for k in final if k in km)
     tot = sum((v["read_bytes_per_launch"] + v["write_bytes_per_launch"]) * v["launches"]
               for k, v in km.items() if k.split(":")[0] in kinds)
     return (tot / units if units else None,

@@ -860,7 +860,7 @@ static int gen_step(const cg_rules &r, const uint64_t *P, size_t nP, uint64_t *C
     }
     prof_end(h, 0);
     /* M_s = uniq(candidates); its count goes to d_cnt[7] */
-    const bool dev = g_one_sync && cmax <= sort_sample_limit(SORT_SAMPLE);
+    const bool dev = g_one_sync && cmax <= sort_sample_dev_limit();
     size_t ncand = 0, mcap = 0;
     uint64_t mor = 0, mand = ~0ULL;
     int hu = -1;
@@ -2494,7 +2494,7 @@ extern "C" int cg_batch_generate(cg_batch *b)
     for (int k = 0; k < b->K; k++) {
         cg_batch::Step st;
         const size_t cmax = 4 * nP;
-        const bool dev = g_one_sync && cmax <= sort_sample_limit(SORT_SAMPLE);
+        const bool dev = g_one_sync && cmax <= sort_sample_dev_limit();
         const bool graph = dev && g_batch_graph;
         /* buffers and allocations first: nothing may allocate inside a
          * capture */

@@ -183,8 +183,8 @@ size_t sort_part_keys(size_t n, int sample = SORT_SAMPLE);
  * it and sizes the grids and buffers: part holds sort_part_keys(nmax) keys,
  * scratch sort_scratch_bytes): the same result as sort_unique, with no host
  * synchronisation and no host decision that depends on the count.  Returns
- * -1 (nothing enqueued) when nmax is above sort_sample_limit() or part is
- * too small.  A bucket above its capacity ORs SORT_ERR_OVERSIZE into *d_err;
+ * -1 (nothing enqueued) when nmax is above sort_sample_dev_limit() or part
+ * is too small.  A bucket above its capacity ORs SORT_ERR_OVERSIZE into *d_err;
  * the caller then repeats the sort with sort_unique(allow_sample = false). */
 int sort_unique_dev(hipStream_t st, const uint64_t *keys, const unsigned long long *dn, size_t nmax,
                     void *scratch, uint64_t *dst, unsigned long long *d_nuniq, unsigned *d_err,
@@ -192,6 +192,8 @@ int sort_unique_dev(hipStream_t st, const uint64_t *keys, const unsigned long lo
 /* the largest sort the sample sort takes in a mode (0 when it is switched
  * off, or for SORT_LSD) */
 size_t sort_sample_limit(int mode);
+/* the largest bound sort_unique_dev takes (2048 buckets) */
+size_t sort_sample_dev_limit();
 /* Makes room for `entries` look-back words in context ctx and for
  * `launches` more launches before the epoch counter wraps, so that the
  * launches that follow clear nothing (no memset, e.g. inside a graph

@@ -1735,6 +1735,12 @@ size_t sort_sample_limit(int mode)
     return (size_t)sh.bmax * (size_t)sh.bhi;
 }
 
+/* the largest bound sort_unique_dev takes: SS_BSMALL buckets of bhi keys */
+size_t sort_sample_dev_limit()
+{
+    return g_ss_on ? (size_t)SS_BSMALL * (size_t)ss_shape(SORT_SAMPLE).bhi : 0;
+}
+
 /* buckets of a sample sort of n keys (1: one block), or 0 when the sort
  * takes the LSD path */
 static size_t ss_buckets(size_t n, int mode = SORT_SAMPLE)
@@ -1896,9 +1902,15 @@ int sort_unique_dev(hipStream_t st, const uint64_t *keys, const unsigned long lo
                     int *prof_unique, int ctx, uint64_t *part, size_t part_cap)
 {
     *prof_unique = -1;
-    if (nmax > sort_sample_limit(SORT_SAMPLE)) return -1;
+    if (nmax > sort_sample_dev_limit()) return -1;
     SortCtx &sc = g_ctx[ctx < 0 || ctx >= SORT_CONTEXTS ? 0 : ctx];
-    const SsShape sh = ss_shape(SORT_SAMPLE);
+    /* the grids, the partition variant and the sample size follow the
+     * bound, which may be several times the count: bounds of at most
+     * SS_BSMALL buckets' worth of keys keep the two-blocks-per-CU partition
+     * and the 16384-key sample (larger ones measured slower when the count
+     * needed neither) */
+    SsShape sh = ss_shape(SORT_SAMPLE);
+    sh.bmax = SS_BSMALL;
     const size_t Bmax = ss_nbuckets(nmax, sh);
     if (Bmax > 1 && (!part || part_cap < Bmax * (size_t)SS_BSTRIDE)) return -1;
     if (!status_reserve(st, sc, Bmax)) return -1;
