@@ -749,6 +749,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-isolated", action="store_true",
+                    help="skip the untimed step that times the sorts with no second-stream overlap")
     ap.add_argument("--adjoint-reps", type=int, default=3, help="sweeps averaged for the adjoint ratio")
     ap.add_argument("--cpu-seconds", type=float, default=20.0,
                     help="CPU baseline: about this much oracle time on sampled layer windows")
@@ -863,7 +865,7 @@ def main():
     value = total / T
 
     roof = stage_roofline(cg, dom, args.steps)
-    if dom == "sort" and args.carry_stream == 1:
+    if dom == "sort" and args.carry_stream == 1 and not args.no_isolated:
         # the same stage with nothing beside it: one untimed step with the
         # carry sort on the main stream, so no kernel of another stream
         # shares the GPU with a sort's kernels while they are timed
