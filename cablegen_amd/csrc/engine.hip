@@ -1665,6 +1665,16 @@ extern "C" int cg_tune_set(const char *knob, int value)
         sort_set_bucket_bitonic(value);
         return 0;
     }
+    if (!strcmp(knob, "distinct_bitonic")) {
+        if (value < 0 || value > 16384) return fail_msg("distinct_bitonic: 0 (off) or at most 16384 values");
+        sort_set_distinct_bitonic(value, sort_distinct_bitonic_bits());
+        return 0;
+    }
+    if (!strcmp(knob, "distinct_bitonic_bits")) {
+        if (value < 0 || value > 64) return fail_msg("distinct_bitonic_bits: 0..64");
+        sort_set_distinct_bitonic(sort_distinct_bitonic_max(), value);
+        return 0;
+    }
     if (!strcmp(knob, "sample_probes")) {
         if (value < 0 || value > 16384) return fail_msg("sample_probes: 0 (default) or 1..16384");
         sort_set_sample_limits(sort_sample_bmax(), value);
@@ -1703,6 +1713,7 @@ extern "C" int cg_tune_set(const char *knob, int value)
         sort_set_sample(1, 5000, 0);
         sort_set_sample_limits(8192, 0);
         sort_set_bucket_bitonic(0);
+        sort_set_distinct_bitonic(2048, 9);
         if (!strcmp(knob, "reset") && !g_tune_path.empty()) return cg_tune_load(g_tune_path.c_str());
         return 0;
     }
@@ -1724,6 +1735,8 @@ extern "C" int cg_tune_get(const char *knob)
     if (!strcmp(knob, "batch_graph")) return g_batch_graph;
     if (!strcmp(knob, "sample_bmax")) return sort_sample_bmax();
     if (!strcmp(knob, "bucket_bitonic")) return sort_bucket_bitonic();
+    if (!strcmp(knob, "distinct_bitonic")) return sort_distinct_bitonic_max();
+    if (!strcmp(knob, "distinct_bitonic_bits")) return sort_distinct_bitonic_bits();
     if (!strcmp(knob, "sample_probes")) return sort_sample_probes();
     if (!strcmp(knob, "graph_steps")) return (int)std::min<unsigned long long>(g_graph_steps, 0x7fffffff);
     if (!strcmp(knob, "graph_instantiations"))

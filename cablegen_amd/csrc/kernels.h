@@ -214,6 +214,12 @@ int sort_sample_bmax();
  * the network only when the set fills (0, the default) */
 void sort_set_bucket_bitonic(int on);
 int sort_bucket_bitonic();
+/* a bucket's distinct values sorted by a bitonic network instead of 8-bit
+ * LSD passes when there are at most dmax of them (0: never) and they span
+ * at least dnb bits */
+void sort_set_distinct_bitonic(int dmax, int dnb);
+int sort_distinct_bitonic_max();
+int sort_distinct_bitonic_bits();
 int sort_sample_probes();
 int sort_sample_on();
 int sort_sample_bucket();

@@ -1230,6 +1230,45 @@ __device__ __forceinline__ int ss_bitonic_any(const uint64_t *__restrict__ s, ui
     return ss_bitonic_bucket<V, sizeof(V) == 8 ? 8 : 16>(s, base, m, L);
 }
 
+/* Sorts the n <= SS_T * R distinct values at the front of the stage
+ * (ss_dedupe's output) by the bitonic network, the positions past n holding
+ * the all-ones value (no distinct value reaches it: they lie below the hash
+ * set's empty mark, or span less than 2^64 - 1); on return they are in
+ * ascending order at the stage's first n entries.  Its cost depends on n
+ * only, the LSD passes' on the bits the values span. */
+template <typename V, int R>
+__device__ __forceinline__ void ss_bitonic_distinct(int n, SsLds &L)
+{
+    V *T = ss_stage<V>(L);
+    const uint32_t tid = threadIdx.x;
+    V v[R];
+    /* any placement sorts: r-major loads are free of bank conflicts */
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const uint32_t i = (uint32_t)r * SS_T + tid;
+        v[r] = i < (uint32_t)n ? T[i] : (V)~(V)0;
+    }
+    /* the network's first stage exchange through LDS starts with a
+     * barrier, and its last ends with one */
+    ss_bitonic<V, R>(v, L);
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const uint32_t p = tid * R + (uint32_t)r;
+        if (p < (uint32_t)n) T[p] = v[r];
+    }
+    __syncthreads();
+}
+
+template <typename V>
+__device__ __forceinline__ void ss_bitonic_distinct_any(int n, SsLds &L)
+{
+    if (n <= SS_T) ss_bitonic_distinct<V, 1>(n, L);
+    else if (n <= 2 * SS_T) ss_bitonic_distinct<V, 2>(n, L);
+    else if (n <= 4 * SS_T) ss_bitonic_distinct<V, 4>(n, L);
+    else if (sizeof(V) == 8 || n <= 8 * SS_T) ss_bitonic_distinct<V, 8>(n, L);
+    else ss_bitonic_distinct<V, sizeof(V) == 8 ? 8 : 16>(n, L);
+}
+
 /* Sorts the n distinct values at the front of the stage (ss_dedupe's
  * output) over their low nb bits; on return they are in ascending order at
  * the stage's first n entries. */
@@ -1266,7 +1305,7 @@ __global__ __launch_bounds__(SS_T, 8) void k_ss_local(const uint64_t *__restrict
                                                       uint64_t *__restrict__ dst, uint64_t *status,
                                                       unsigned *ticket, uint32_t epoch,
                                                       unsigned long long *total, unsigned *err, int cap,
-                                                      int probes_direct, int bitonic)
+                                                      int probes_direct, int bitonic, int dmax, int dnb)
 {
     __shared__ SsLds L;
     const int w = threadIdx.x >> 6, lane = lane_id();
@@ -1358,8 +1397,17 @@ __global__ __launch_bounds__(SS_T, 8) void k_ss_local(const uint64_t *__restrict
     uint64_t *my = status + t;
     if (B > 1 && threadIdx.x == 0) st_flag(my, t == 0 ? OS_INC : OS_AGG, epoch, (uint64_t)n);
     if (!sorted) {
-        if (wide) ss_sort_distinct<uint64_t, SS_KS64>(n, nb, L);
-        else ss_sort_distinct<uint32_t, SS_KS32>(n, nb, L);
+        /* the distinct values: by the bitonic network when there are at
+         * most dmax of them spanning at least dnb bits, else by 8-bit LSD
+         * passes over their nb bits (n and nb are the block's) */
+        if (n <= dmax && nb >= dnb) {
+            if (wide) ss_bitonic_distinct_any<uint64_t>(n, L);
+            else ss_bitonic_distinct_any<uint32_t>(n, L);
+        } else if (wide) {
+            ss_sort_distinct<uint64_t, SS_KS64>(n, nb, L);
+        } else {
+            ss_sort_distinct<uint32_t, SS_KS32>(n, nb, L);
+        }
     }
     if (w == 0) {
         uint64_t excl = 0;
@@ -1690,6 +1738,12 @@ static int g_ss_probes = SS_PROBES;  /* longest hash-set probe of a bucket the d
  * C2 generated in 243-245 ms against 288-300 ms bitonic-first, since a
  * candidate bucket's 5000 keys hold only ~1400 distinct values) */
 static int g_ss_bitonic = 0;
+/* a bucket's distinct values sorted by the bitonic network when there are
+ * at most g_ss_dmax of them (0: never) spanning at least g_ss_dnb bits (two
+ * or more 8-bit LSD passes).  C2 bucket kernels, per step: 140.3-141.5 ms
+ * with 2048 and 9, 142.6-143.3 ms with LSD passes only, 142.6-145.0 ms
+ * with 4096 and 17, 147.2-147.6 ms with 16384 (every bucket). */
+static int g_ss_dmax = 2048, g_ss_dnb = 9;
 static unsigned long long g_ss_sorts = 0;
 
 void sort_set_sample(int on, int bucket_avg, int cap)
@@ -1704,6 +1758,13 @@ void sort_set_sample_limits(int bmax, int probes)
     g_ss_probes = probes > 0 ? probes : SS_PROBES;
 }
 void sort_set_bucket_bitonic(int on) { g_ss_bitonic = on ? 1 : 0; }
+void sort_set_distinct_bitonic(int dmax, int dnb)
+{
+    g_ss_dmax = dmax < 0 ? 0 : (dmax > SS_LCAP ? SS_LCAP : dmax);
+    g_ss_dnb = dnb < 0 ? 0 : (dnb > 64 ? 64 : dnb);
+}
+int sort_distinct_bitonic_max() { return g_ss_dmax; }
+int sort_distinct_bitonic_bits() { return g_ss_dnb; }
 int sort_bucket_bitonic() { return g_ss_bitonic; }
 int sort_sample_bmax() { return g_ss_bmax; }
 int sort_sample_probes() { return g_ss_probes; }
@@ -1892,7 +1953,8 @@ static void ss_sort_unique(hipStream_t st, const uint64_t *keys, const unsigned 
     *prof_unique = prof_hook_begin(10, st);
     hipLaunchKernelGGL(k_ss_local, dim3((unsigned)Bmax), dim3(SS_T), 0, st, (const uint64_t *)part, keys,
                        (const unsigned long long *)s.ss_cnt, (const uint64_t *)s.ss_spl, dn, nmax, sh, dst,
-                       sc.status, s.ss_ticket, epoch, d_nuniq, d_err, g_ss_cap, g_ss_probes, g_ss_bitonic);
+                       sc.status, s.ss_ticket, epoch, d_nuniq, d_err, g_ss_cap, g_ss_probes, g_ss_bitonic,
+                       g_ss_dmax, g_ss_dnb);
     snote(hipGetLastError());
     prof_hook_end(*prof_unique, 8.0 * nmax, st);
 }

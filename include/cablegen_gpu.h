@@ -207,24 +207,49 @@ int cg_dev_sync(void);
  *   "cell_code"  1: a generation's sorts code pairs of varying cells as
  *                a * base + b when the layer's tile sum bounds the cell
  *                values and that saves a radix pass; 0: bit packing only;
- *   "sample_sort"  1: sorts of up to about 10^7 keys partition into
- *                LDS-sized buckets by splitters from a sample and sort and
- *                deduplicate each bucket in LDS (a sort of at most 16384 keys
- *                is one block); 0: every sort takes the LSD radix path;
- *   "sample_bucket"  mean keys per bucket of the sample sort (1000..16384);
- *   "sample_cap" bucket capacity of the sample sort (0: 16384); a bucket
+ *   "canon_stab" 1: the generation's move and spawn kernels take the
+ *                canonical form through the formation's stabiliser (the
+ *                images that put the formation at its canonical place; the
+ *                full max of 8 for any other board); 0: always the max of 8;
+ *   "sample_sort"  1: sorts of up to 8192 buckets of 7000 keys (5.7e7)
+ *                partition into LDS-sized buckets by splitters from a
+ *                regular sample and deduplicate and sort each bucket in LDS
+ *                (a sort of at most 8192 keys is one block); 0: every sort
+ *                takes the LSD radix path;
+ *   "sample_bucket"  mean keys per bucket up to 2048 buckets (1000..16384);
+ *   "sample_cap" keys a bucket may hold (0: 131072, its 8 regions); a bucket
  *                above it makes the library repeat that sort on the LSD path
  *                (lowered only to exercise the repeat);
+ *   "sample_bmax"  most buckets of a sort: 8192 (sorts up to 5.7e7 keys),
+ *                or 2048 (up to 1.4e7; larger sorts take the LSD path);
+ *   "sample_probes"  longest hash-set probe of a bucket that one block can
+ *                also sort whole (0: 64); past it the bucket is sorted by a
+ *                bitonic network, equal runs dropped;
+ *   "bucket_bitonic"  1: buckets that fit one block are sorted whole by the
+ *                bitonic network first; 0: through the LDS hash set first;
+ *   "distinct_bitonic"  a bucket's distinct values are sorted by a bitonic
+ *                network when there are at most this many (0: never; at
+ *                most 16384) and they span at least "distinct_bitonic_bits"
+ *                bits (0..64), else by 8-bit LSD passes;
+ *   "one_sync"   1: a layer step reads its counters from the device once
+ *                (the candidate sort sized on the device); 0: twice;
+ *   "batch_graph"  1: each batched layer step is captured and launched as
+ *                one graph; 0: its launches are enqueued one by one;
  *   "builtin"    built-in defaults: 8-bit digits, 8192-key tiles, window 16
  *                for every entry, no forced field, delta 0, spawn_estimate 0,
- *                cell_code 1, carry_stream 1, sample_sort 1 with 5000 keys
- *                per bucket and capacity 16384;
+ *                cell_code 1, carry_stream 1, canon_stab 1, sample_sort 1
+ *                with 5000 keys per bucket, capacity 131072, 8192 buckets,
+ *                probes 64, bucket_bitonic 0, distinct_bitonic 2048 with 9
+ *                bits, one_sync 1, batch_graph 1;
  *   "reset"      the state after initialisation: the built-in defaults, then
  *                the table loaded at initialisation if there was one.
  * Returns 0, or -1 for an unknown knob or value.  cg_tune_get also reads
- * "cell_code_sorts": the number of sorts that used the cell-pair code, and
- * "sample_sorts": the number of sample sorts, and "sample_repeats": the
- * number of sample sorts repeated on the LSD path after an oversize bucket. */
+ * "cell_code_sorts": the number of sorts that used the cell-pair code,
+ * "sample_sorts": the number of sample sorts, "sample_repeats": the number
+ * of sample sorts repeated on the LSD path after an oversize bucket,
+ * "graph_steps": the number of batched layer steps launched as a graph, and
+ * "graph_instantiations": the number of graphs instantiated for them (a
+ * step whose graph has the shape of the previous one updates it in place). */
 int cg_tune_set(const char *knob, int value);
 int cg_tune_get(const char *knob);
 /* the sorts read their configuration from a table indexed by size class

@@ -1014,6 +1014,41 @@ def test_sample_sort_bitonic_buckets(cg, n):
     assert cg.tune_get("bucket_bitonic") == 0
 
 
+@pytest.mark.parametrize("n", [700, 3000, 8190, 40000, 300000, 2000003])
+def test_sample_sort_bitonic_distinct(cg, n):
+    """A bucket's distinct values sorted by the bitonic network instead of
+    LSD passes (knobs distinct_bitonic, distinct_bitonic_bits; by default
+    up to 2048 values spanning 9 bits or more): every row count per thread
+    from 1 to 16 (distinct counts up to ~12000 in buckets of 12000 keys),
+    32- and 64-bit values, values spanning few bits (one LSD pass) and many,
+    against numpy's unique, with the threshold at every count, at 2048, and
+    with LSD passes only."""
+    rng = np.random.default_rng(n + 11)
+    b = rand_boards(rng, 0x00000000FFF0FFF0, max(n // 3, 1))
+    a = b[rng.integers(0, len(b), size=n)]
+    x = rng.integers(0, 2**64 - 1, size=n, dtype=np.uint64, endpoint=True)
+    x[rng.integers(0, n, n // 10)] = x[1]
+    narrow = (rng.integers(0, 200, size=n).astype(np.uint64) << np.uint64(20)) | np.uint64(0x5000000000)
+    try:
+        for dmax, dnb in ((16384, 0), (2048, 9)):
+            cg.tune_set("distinct_bitonic", dmax)
+            cg.tune_set("distinct_bitonic_bits", dnb)
+            _sample_sort_check(cg, a)
+            _sample_sort_check(cg, x)
+            _sample_sort_check(cg, narrow)
+            if n >= 40000:
+                cg.tune_set("sample_bucket", 12000)
+                _sample_sort_check(cg, a)
+                _sample_sort_check(cg, np.unique(a))
+                cg.tune_set("sample_bucket", 5000)
+        cg.tune_set("distinct_bitonic", 0)
+        _sample_sort_check(cg, a)
+        _sample_sort_check(cg, x)
+    finally:
+        cg.tune_set("reset", 0)
+    assert cg.tune_get("distinct_bitonic") == 2048 and cg.tune_get("distinct_bitonic_bits") == 9
+
+
 def test_sample_sort_oversize_repeats_on_lsd(cg):
     """A bucket above the capacity (one key repeated 60000 times, or every
     bucket once the capacity is lowered) makes the library repeat the sort on
