@@ -482,6 +482,7 @@ static MoveRules move_rules(const cg_rules &r)
     MoveRules m;
     m.ff = r.free_formation;
     m.prune = r.prune;
+    m.prune_moves = 1;
     m.stsl = r.stsl;
     m.ltc = r.ltc;
     m.sl = r.smallest_large;
@@ -567,7 +568,8 @@ static uint64_t vary_of(size_t n, uint64_t o, uint64_t a) { return n > 1 ? (o & 
  * bucket overflowed); the sample sort leaves buf unchanged, so a repeat can
  * read it again */
 static int sort_enqueue(uint64_t *buf, size_t n, uint64_t vary, uint64_t *dst, int slot, int *hu,
-                        int lane = 0, uint64_t cell_max = CELLS_UNBOUNDED, int sample = SORT_SAMPLE)
+                        int lane = 0, uint64_t cell_max = CELLS_UNBOUNDED, int sample = SORT_SAMPLE,
+                        KeyPrune kp = KeyPrune())
 {
     const size_t pk = sort_part_keys(n, sample);
     /* the carry sorts (count slot 6) report into the second error word, the
@@ -585,11 +587,11 @@ static int sort_enqueue(uint64_t *buf, size_t n, uint64_t vary, uint64_t *dst, i
     if (lane == 1) {
         if (ensure(E.B2, n + 1) || ensure_scratch2(n) || (pk && ensure(E.Pt2, pk))) return -1;
         sort_unique(E.st2, buf, E.B2.p, n, vary, E.scratch2, dst, E.d_cnt + slot, err, hu, 1,
-                    cell_max, E.Pt2.p, E.Pt2.cap, sample);
+                    cell_max, E.Pt2.p, E.Pt2.cap, sample, kp);
     } else {
         if (ensure(E.B, n + 1) || ensure_scratch(n) || (pk && ensure(E.Pt, pk))) return -1;
         sort_unique(E.st, buf, E.B.p, n, vary, E.scratch, dst, E.d_cnt + slot, err, hu, 0,
-                    cell_max, E.Pt.p, E.Pt.cap, sample);
+                    cell_max, E.Pt.p, E.Pt.cap, sample, kp);
     }
     hipError_t e = kernels_take_error();
     if (e != hipSuccess) return fail("sort_unique", e);
@@ -790,7 +792,7 @@ static uint64_t cell_bound(const ValueBound *v, long s, uint64_t key_and)
 /* the sample sort of the *dn keys at buf (at most nmax of them) into dst,
  * sized on the device, on the main stream; its count lands in d_cnt[slot] */
 static int sort_enqueue_dev(uint64_t *buf, const unsigned long long *dn, size_t nmax, uint64_t *dst, int slot,
-                            int *hu)
+                            int *hu, KeyPrune kp = KeyPrune())
 {
     const size_t pk = sort_part_keys(nmax, SORT_SAMPLE);
     if (ensure_scratch(nmax) || (pk && ensure(E.Pt, pk))) return -1;
@@ -798,7 +800,8 @@ static int sort_enqueue_dev(uint64_t *buf, const unsigned long long *dn, size_t 
     g_err_note[0].slot = slot;
     g_err_note[0].n = nmax;
     g_err_note[0].sample = 3;   /* sized on the device */
-    if (sort_unique_dev(E.st, buf, dn, nmax, E.scratch, dst, E.d_cnt + slot, E.d_err, hu, 0, E.Pt.p, E.Pt.cap)) {
+    if (sort_unique_dev(E.st, buf, dn, nmax, E.scratch, dst, E.d_cnt + slot, E.d_err, hu, 0, E.Pt.p, E.Pt.cap,
+                        kp)) {
         hipError_t e = kernels_take_error();
         return e != hipSuccess ? fail("sort_unique_dev", e) : fail_msg("sort: device-sized sort refused");
     }
@@ -811,6 +814,10 @@ static int sort_enqueue_dev(uint64_t *buf, const unsigned long long *dn, size_t 
  * the device (knob one_sync; 0: the count of the candidates comes to the
  * host before their sort, two synchronisations per step) */
 static int g_one_sync = 1;
+
+/* with prune: the moved boards are tested by the candidate sort, once per
+ * distinct board, instead of in the move kernel (knob prune_in_sort) */
+static int g_prune_in_sort = 1;
 
 /* One layer step (generate.c:409-550): the pre-move boards P and the carried
  * spawn-4 set C give M_s = uniq(moves(P)) (from `la`), the next pre-move
@@ -843,6 +850,16 @@ static int gen_step(const cg_rules &r, const uint64_t *P, size_t nP, uint64_t *C
     /* moves: at most 4 candidates per board (premove: the boards themselves) */
     const size_t cmax = premoved ? nP : 4 * nP;
     if (ensure(E.A, cmax + 1)) return -1;
+    /* the moved boards' prune test, in the sort of the candidates: a layer
+     * step's candidates hold each distinct board about 3.5 times, and the
+     * test depends only on the board's cells (premove takes no prune) */
+    KeyPrune kp = KeyPrune();
+    if (r.prune && g_prune_in_sort && !premoved) {
+        kp.on = 1;
+        kp.stsl = r.stsl;
+        kp.ltc = r.ltc;
+        kp.sl = r.smallest_large;
+    }
     launch_init_stats(E.st, E.d_cnt);
     int h = prof_begin(PK_MOVE);
     if (premoved) {
@@ -856,6 +873,7 @@ static int gen_step(const cg_rules &r, const uint64_t *P, size_t nP, uint64_t *C
     } else {
         MoveRules mr = move_rules(r);
         mr.cs = cs;
+        mr.prune_moves = kp.on ? 0 : 1;
         launch_move(E.st, P, nP, nullptr, E.A.p, E.d_cnt, lut, mr, 1);
     }
     prof_end(h, 0);
@@ -868,7 +886,7 @@ static int gen_step(const cg_rules &r, const uint64_t *P, size_t nP, uint64_t *C
         mcap = cmax;
         *M = la.alloc(mcap);
         if (!*M) return fail_msg("layer allocation failed");
-        if (sort_enqueue_dev(E.A.p, E.d_cnt, cmax, *M, 7, &hu)) return -1;
+        if (sort_enqueue_dev(E.A.p, E.d_cnt, cmax, *M, 7, &hu, kp)) return -1;
     } else {
         /* the candidate count first; the previous step's carry sort
          * (second stream) overlapped the moves and is settled here too */
@@ -886,7 +904,8 @@ static int gen_step(const cg_rules &r, const uint64_t *P, size_t nP, uint64_t *C
         mcap = ncand;
         *M = la.alloc(mcap);
         if (!*M) return fail_msg("layer allocation failed");
-        if (sort_enqueue(E.A.p, ncand, vary_of(ncand, mor, mand), *M, 7, &hu, 0, cell_bound(vb, s, mand)))
+        if (sort_enqueue(E.A.p, ncand, vary_of(ncand, mor, mand), *M, 7, &hu, 0, cell_bound(vb, s, mand),
+                         SORT_SAMPLE, kp))
             return -1;
     }
     /* spawns of M_s (the kernel reads |M_s| on the device; the grid covers
@@ -930,7 +949,7 @@ static int gen_step(const cg_rules &r, const uint64_t *P, size_t nP, uint64_t *C
             /* the M sort had an oversize bucket: repeat it on the LSD path
              * (the candidates are unchanged), then the spawns */
             if (sort_enqueue(E.A.p, ncand, vary_of(ncand, mor, mand), *M, 7, &hu, 0, cell_bound(vb, s, mand),
-                             SORT_LSD))
+                             SORT_LSD, kp))
                 return -1;
             continue;
         }
@@ -1665,6 +1684,11 @@ extern "C" int cg_tune_set(const char *knob, int value)
         sort_set_bucket_bitonic(value);
         return 0;
     }
+    if (!strcmp(knob, "prune_in_sort")) {
+        if (value != 0 && value != 1) return fail_msg("prune_in_sort: 0 or 1");
+        g_prune_in_sort = value;
+        return 0;
+    }
     if (!strcmp(knob, "distinct_bitonic")) {
         if (value < 0 || value > 16384) return fail_msg("distinct_bitonic: 0 (off) or at most 16384 values");
         sort_set_distinct_bitonic(value, sort_distinct_bitonic_bits());
@@ -1709,6 +1733,7 @@ extern "C" int cg_tune_set(const char *knob, int value)
         g_carry_lane = 1;
         g_canon_stab = 1;
         g_one_sync = 1;
+        g_prune_in_sort = 1;
         g_batch_graph = 1;
         sort_set_sample(1, 5000, 0);
         sort_set_sample_limits(8192, 0);
@@ -1732,6 +1757,7 @@ extern "C" int cg_tune_get(const char *knob)
     if (!strcmp(knob, "carry_stream")) return g_carry_lane;
     if (!strcmp(knob, "canon_stab")) return g_canon_stab;
     if (!strcmp(knob, "one_sync")) return g_one_sync;
+    if (!strcmp(knob, "prune_in_sort")) return g_prune_in_sort;
     if (!strcmp(knob, "batch_graph")) return g_batch_graph;
     if (!strcmp(knob, "sample_bmax")) return sort_sample_bmax();
     if (!strcmp(knob, "bucket_bitonic")) return sort_bucket_bitonic();

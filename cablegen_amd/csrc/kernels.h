@@ -14,8 +14,21 @@ namespace cg {
 struct MoveRules {
     int ff;          /* free_formation */
     int prune;
+    /* with prune: 1 tests every moved board in the move kernel; 0 leaves
+     * the moved boards to the sort of the candidates (KeyPrune), which tests
+     * each distinct board once */
+    int prune_moves;
     long stsl, ltc, sl;
     cg_canon_sel_t cs;   /* canonical form through the formation's stabiliser */
+};
+
+/* prune_board (generate.c:98-131) applied by a sort whose keys are boards:
+ * distinct keys that fail the test are left out of its output and count.
+ * The test depends only on the multiset of cell values, so it gives the
+ * same set on canonical forms and after deduplication. */
+struct KeyPrune {
+    int on;
+    long stsl, ltc, sl;
 };
 
 /* Winstate test as used by the solve: `nox` (without score) makes "holds the
@@ -176,7 +189,8 @@ constexpr int SORT_LSD = 0, SORT_SAMPLE = 1, SORT_SAMPLE_SPARSE = 2;
 uint64_t *sort_unique(hipStream_t st, uint64_t *keys, uint64_t *tmp, size_t n, uint64_t vary,
                       void *scratch, uint64_t *dst, unsigned long long *d_nuniq, unsigned *d_err,
                       int *prof_unique, int ctx = 0, uint64_t cell_max = CELLS_UNBOUNDED,
-                      uint64_t *part = nullptr, size_t part_cap = 0, int sample = SORT_SAMPLE);
+                      uint64_t *part = nullptr, size_t part_cap = 0, int sample = SORT_SAMPLE,
+                      KeyPrune kp = KeyPrune());
 /* keys of the part buffer a sort of n keys needs (0: none) */
 size_t sort_part_keys(size_t n, int sample = SORT_SAMPLE);
 /* The sample sort of *dn keys, a count known only on the device (nmax bounds
@@ -188,7 +202,7 @@ size_t sort_part_keys(size_t n, int sample = SORT_SAMPLE);
  * the caller then repeats the sort with sort_unique(allow_sample = false). */
 int sort_unique_dev(hipStream_t st, const uint64_t *keys, const unsigned long long *dn, size_t nmax,
                     void *scratch, uint64_t *dst, unsigned long long *d_nuniq, unsigned *d_err,
-                    int *prof_unique, int ctx, uint64_t *part, size_t part_cap);
+                    int *prof_unique, int ctx, uint64_t *part, size_t part_cap, KeyPrune kp = KeyPrune());
 /* the largest sort the sample sort takes in a mode (0 when it is switched
  * off, or for SORT_LSD) */
 size_t sort_sample_limit(int mode);

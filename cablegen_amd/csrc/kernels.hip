@@ -97,7 +97,7 @@ __global__ __launch_bounds__(MV_THREADS, 1) void k_move(
                 for (int d = 0; d < 4; d++) {
                     uint64_t t = d < 2 ? b : y;
                     if (cg_move_hori_tab(lut, &t, d == 1 || d == 2, r.ff) &&
-                        !(r.prune && cg_prune(t, r.stsl, r.ltc, r.sl)))
+                        !(r.prune && r.prune_moves && cg_prune(t, r.stsl, r.ltc, r.sl)))
                         res[k++] = cg_canon_fast(t, r.cs, d >= 2);
                 }
             }
@@ -198,7 +198,7 @@ __global__ __launch_bounds__(MV_THREADS, 8) void k_move_gen(
             for (int d = 0; d < 4; d++) {
                 uint64_t t = d < 2 ? b : y;
                 if (cg_move_hori_tab(lut, &t, d == 1 || d == 2, r.ff) &&
-                    !(r.prune && cg_prune(t, r.stsl, r.ltc, r.sl)))
+                    !(r.prune && r.prune_moves && cg_prune(t, r.stsl, r.ltc, r.sl)))
                     mv_first(ht, cg_canon_fast(t, r.cs, d >= 2));
             }
         }
@@ -1227,7 +1227,7 @@ __global__ __launch_bounds__(MV_THREADS, 8) void k_bmove(const uint64_t *__restr
             for (int d = 0; d < 4; d++) {
                 uint64_t t = d < 2 ? b : y;
                 if (cg_move_hori_tab(lut, &t, d == 1 || d == 2, r.ff) &&
-                    !(r.prune && cg_prune(t, r.stsl, r.ltc, r.sl)))
+                    !(r.prune && r.prune_moves && cg_prune(t, r.stsl, r.ltc, r.sl)))
                     mv_first(ht, benc(cg_canon(t), T, tid, err));
             }
         }

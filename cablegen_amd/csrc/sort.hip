@@ -395,13 +395,16 @@ __global__ __launch_bounds__(pass_threads(BITS, KPT), pass_min_waves(BITS, KPT, 
 
 /* unique compaction of a sorted array with a chained scan over tiles;
  * the last tile writes the total to *total */
+/* PRUNE: the sort drops the distinct keys (boards) that fail kp's prune
+ * test (a separate instance keeps the plain compaction's registers) */
+template <bool PRUNE>
 __global__ __launch_bounds__(UQ_THREADS) void k_os_unique(const uint64_t *__restrict__ k, size_t n,
                                                           uint64_t *__restrict__ dst,
                                                           uint64_t *status, unsigned *tile_ctr,
                                                           uint32_t epoch,
                                                           unsigned long long *total, unsigned *err,
                                                           int packed, SortCode code,
-                                                          const uint64_t *cval)
+                                                          const uint64_t *cval, KeyPrune kp)
 {
     constexpr int WCHUNK = 64 * UQ_KPT;
     __shared__ unsigned tile_sh;
@@ -428,12 +431,15 @@ __global__ __launch_bounds__(UQ_THREADS) void k_os_unique(const uint64_t *__rest
         size_t i0 = base + (size_t)j * 64;
         prev0[j] = (lane == 0 && i0 > 0 && i0 <= n) ? k[i0 - 1] : 0;
     }
+    /* codes go back to the original bits: sc_decode plus the constant bits */
+    const uint64_t cv = packed ? *cval : 0;
 #pragma unroll
     for (int j = 0; j < UQ_KPT; j++) {
         size_t i = base + (size_t)j * 64 + lane;
         uint64_t pv = __shfl_up(kv[j], 1, 64);
         if (lane == 0) pv = prev0[j];
         bool f = i < n && (i == 0 || kv[j] != pv);
+        if (PRUNE && f) f = !cg_prune(packed ? (sc_decode(kv[j], code) | cv) : kv[j], kp.stsl, kp.ltc, kp.sl);
         m[j] = __ballot(f);
         c += (uint32_t)__popcll(m[j]);
     }
@@ -488,8 +494,6 @@ __global__ __launch_bounds__(UQ_THREADS) void k_os_unique(const uint64_t *__rest
     __syncthreads();
     uint64_t o = excl_sh;
     for (int ww = 0; ww < w; ww++) o += wc[ww];
-    /* codes go back to the original bits: sc_decode plus the constant bits */
-    const uint64_t cv = packed ? *cval : 0;
 #pragma unroll
     for (int j = 0; j < UQ_KPT; j++) {
         if ((m[j] >> lane) & 1ULL)
@@ -1018,6 +1022,35 @@ __global__ __launch_bounds__(SS_T, BM <= SS_BSMALL ? 2 : 1) void k_ss_part(const
 constexpr int SS_PROBES = 64;
 constexpr int SS_PROBES_FAR = 512;
 
+/* Drops the n distinct values at the front of the stage whose board (add +
+ * value) fails the prune test; returns the count kept.  A pruned board is
+ * rare, so the values are tested first and compacted only when one is
+ * found: row by row, kept values moving left into rows already read. */
+template <typename V>
+__device__ __forceinline__ int ss_prune_distinct(int n, uint64_t add, const KeyPrune &kp, SsLds &L)
+{
+    V *T = ss_stage<V>(L);
+    bool any = false;
+    for (int i = threadIdx.x; i < n; i += SS_T) any |= cg_prune(add + (uint64_t)T[i], kp.stsl, kp.ltc, kp.sl) != 0;
+    if (!__syncthreads_or(any)) return n;
+    uint32_t kept = 0;
+    for (int r0 = 0; r0 < n; r0 += SS_T) {
+        const int i = r0 + (int)threadIdx.x;
+        V x = 0;
+        uint32_t k = 0;
+        if (i < n) {
+            x = T[i];
+            k = cg_prune(add + (uint64_t)x, kp.stsl, kp.ltc, kp.sl) ? 0u : 1u;
+        }
+        uint32_t tot;
+        const uint32_t o = block_excl_scan_dpp(k, &tot, L.wtmp);   /* its barrier ends the row's reads */
+        if (k) T[kept + o] = x;
+        kept += tot;
+        __syncthreads();   /* the row's writes, and wtmp free again */
+    }
+    return (int)kept;
+}
+
 /* Deduplicates the bucket's keys (the SS_NX regions of L.segc[x] keys at s +
  * x * SS_SUBCAP) as values k - base of type V through an LDS hash set: the
  * stage as open-addressing slots with linear probing, all ones marking an
@@ -1029,7 +1062,7 @@ constexpr int SS_PROBES_FAR = 512;
  * (SS_T * K values) cannot hold them. */
 template <typename V, int K>
 __device__ __forceinline__ int ss_dedupe(const uint64_t *__restrict__ s, uint64_t base, SsLds &L, uint64_t *vmin,
-                                         int *nbits, int probes)
+                                         int *nbits, int probes, const KeyPrune &kp)
 {
     constexpr int NS = (int)(sizeof(SsLds::stage) / sizeof(V));
     constexpr V EMPTY = (V)~(V)0;
@@ -1088,10 +1121,12 @@ __device__ __forceinline__ int ss_dedupe(const uint64_t *__restrict__ s, uint64_
     for (int j = 0; j < PER; j++)
         if (x[j] != EMPTY) T[o++] = x[j] - vl;
     __syncthreads();
-    const int n = (int)all;
+    int n = (int)all;
     if (n > SS_T * K) return -1;
+    /* the range of every distinct value bounds the kept ones */
     *vmin = n ? (uint64_t)vl : 0;
     *nbits = (n > 1 && vh > vl) ? 64 - __clzll((long long)(uint64_t)(vh - vl)) : 0;
+    if (kp.on) n = ss_prune_distinct<V>(n, base + (uint64_t)vl, kp, L);
     return n;
 }
 
@@ -1173,7 +1208,8 @@ __device__ __forceinline__ void ss_bitonic(V (&v)[R], SsLds &L)
  * first n entries hold the distinct values ascending (relative to base);
  * returns n. */
 template <typename V, int R>
-__device__ __forceinline__ int ss_bitonic_bucket(const uint64_t *__restrict__ s, uint64_t base, int m, SsLds &L)
+__device__ __forceinline__ int ss_bitonic_bucket(const uint64_t *__restrict__ s, uint64_t base, int m, SsLds &L,
+                                                 const KeyPrune &kp)
 {
     const uint32_t tid = threadIdx.x;
     uint32_t pre[SS_NX + 1];
@@ -1204,7 +1240,8 @@ __device__ __forceinline__ int ss_bitonic_bucket(const uint64_t *__restrict__ s,
     for (int r = 0; r < R; r++) {
         const uint32_t p = tid * R + (uint32_t)r;
         const V prev = r ? v[r - 1] : before0;
-        if (p < (uint32_t)m && (p == 0 || v[r] != prev)) {
+        if (p < (uint32_t)m && (p == 0 || v[r] != prev) &&
+            !(kp.on && cg_prune(base + (uint64_t)v[r], kp.stsl, kp.ltc, kp.sl))) {
             keep |= 1u << r;
             c++;
         }
@@ -1221,13 +1258,14 @@ __device__ __forceinline__ int ss_bitonic_bucket(const uint64_t *__restrict__ s,
 
 /* the bitonic bucket sort with the fewest rows per thread that hold m keys */
 template <typename V>
-__device__ __forceinline__ int ss_bitonic_any(const uint64_t *__restrict__ s, uint64_t base, int m, SsLds &L)
+__device__ __forceinline__ int ss_bitonic_any(const uint64_t *__restrict__ s, uint64_t base, int m, SsLds &L,
+                                              const KeyPrune &kp)
 {
-    if (m <= SS_T) return ss_bitonic_bucket<V, 1>(s, base, m, L);
-    if (m <= 2 * SS_T) return ss_bitonic_bucket<V, 2>(s, base, m, L);
-    if (m <= 4 * SS_T) return ss_bitonic_bucket<V, 4>(s, base, m, L);
-    if (sizeof(V) == 8 || m <= 8 * SS_T) return ss_bitonic_bucket<V, 8>(s, base, m, L);
-    return ss_bitonic_bucket<V, sizeof(V) == 8 ? 8 : 16>(s, base, m, L);
+    if (m <= SS_T) return ss_bitonic_bucket<V, 1>(s, base, m, L, kp);
+    if (m <= 2 * SS_T) return ss_bitonic_bucket<V, 2>(s, base, m, L, kp);
+    if (m <= 4 * SS_T) return ss_bitonic_bucket<V, 4>(s, base, m, L, kp);
+    if (sizeof(V) == 8 || m <= 8 * SS_T) return ss_bitonic_bucket<V, 8>(s, base, m, L, kp);
+    return ss_bitonic_bucket<V, sizeof(V) == 8 ? 8 : 16>(s, base, m, L, kp);
 }
 
 /* Sorts the n <= SS_T * R distinct values at the front of the stage
@@ -1305,7 +1343,8 @@ __global__ __launch_bounds__(SS_T, 8) void k_ss_local(const uint64_t *__restrict
                                                       uint64_t *__restrict__ dst, uint64_t *status,
                                                       unsigned *ticket, uint32_t epoch,
                                                       unsigned long long *total, unsigned *err, int cap,
-                                                      int probes_direct, int bitonic, int dmax, int dnb)
+                                                      int probes_direct, int bitonic, int dmax, int dnb,
+                                                      KeyPrune kp)
 {
     __shared__ SsLds L;
     const int w = threadIdx.x >> 6, lane = lane_id();
@@ -1376,17 +1415,17 @@ __global__ __launch_bounds__(SS_T, 8) void k_ss_local(const uint64_t *__restrict
     int n = -1;
     bool sorted = false;
     if (bitonic && direct_ok && m > 0) {
-        n = wide ? ss_bitonic_any<uint64_t>(s, base, (int)m, L) : ss_bitonic_any<uint32_t>(s, base, (int)m, L);
+        n = wide ? ss_bitonic_any<uint64_t>(s, base, (int)m, L, kp) : ss_bitonic_any<uint32_t>(s, base, (int)m, L, kp);
         sorted = true;
     } else {
         const int probes = direct_ok ? probes_direct : SS_PROBES_FAR;
-        n = wide ? ss_dedupe<uint64_t, SS_KS64>(s, base, L, &vmin, &nb, probes)
-                 : ss_dedupe<uint32_t, SS_KS32>(s, base, L, &vmin, &nb, probes);
+        n = wide ? ss_dedupe<uint64_t, SS_KS64>(s, base, L, &vmin, &nb, probes, kp)
+                 : ss_dedupe<uint32_t, SS_KS32>(s, base, L, &vmin, &nb, probes, kp);
     }
     if (n < 0 && direct_ok) {
         /* the hash set filled up (mostly distinct keys): every key sorted
          * by the bitonic network, equal runs dropped */
-        n = wide ? ss_bitonic_any<uint64_t>(s, base, (int)m, L) : ss_bitonic_any<uint32_t>(s, base, (int)m, L);
+        n = wide ? ss_bitonic_any<uint64_t>(s, base, (int)m, L, kp) : ss_bitonic_any<uint32_t>(s, base, (int)m, L, kp);
         vmin = 0;
         sorted = true;
     }
@@ -1905,7 +1944,8 @@ static void launch_pass(hipStream_t st, SortCfg c, unsigned tiles, const uint64_
  * Bmax > 1; keys are only read. */
 static void ss_sort_unique(hipStream_t st, const uint64_t *keys, const unsigned long long *dn, size_t nmax,
                            SsShape sh, size_t Bmax, uint64_t *part, const OsScratch &s, SortCtx &sc,
-                           uint64_t *dst, unsigned long long *d_nuniq, unsigned *d_err, int *prof_unique)
+                           uint64_t *dst, unsigned long long *d_nuniq, unsigned *d_err, int *prof_unique,
+                           KeyPrune kp)
 {
     g_ss_sorts++;
     const uint32_t epoch = next_epoch(st, sc);
@@ -1954,14 +1994,14 @@ static void ss_sort_unique(hipStream_t st, const uint64_t *keys, const unsigned 
     hipLaunchKernelGGL(k_ss_local, dim3((unsigned)Bmax), dim3(SS_T), 0, st, (const uint64_t *)part, keys,
                        (const unsigned long long *)s.ss_cnt, (const uint64_t *)s.ss_spl, dn, nmax, sh, dst,
                        sc.status, s.ss_ticket, epoch, d_nuniq, d_err, g_ss_cap, g_ss_probes, g_ss_bitonic,
-                       g_ss_dmax, g_ss_dnb);
+                       g_ss_dmax, g_ss_dnb, kp);
     snote(hipGetLastError());
     prof_hook_end(*prof_unique, 8.0 * nmax, st);
 }
 
 int sort_unique_dev(hipStream_t st, const uint64_t *keys, const unsigned long long *dn, size_t nmax,
                     void *scratch, uint64_t *dst, unsigned long long *d_nuniq, unsigned *d_err,
-                    int *prof_unique, int ctx, uint64_t *part, size_t part_cap)
+                    int *prof_unique, int ctx, uint64_t *part, size_t part_cap, KeyPrune kp)
 {
     *prof_unique = -1;
     if (nmax > sort_sample_dev_limit()) return -1;
@@ -1976,7 +2016,7 @@ int sort_unique_dev(hipStream_t st, const uint64_t *keys, const unsigned long lo
     const size_t Bmax = ss_nbuckets(nmax, sh);
     if (Bmax > 1 && (!part || part_cap < Bmax * (size_t)SS_BSTRIDE)) return -1;
     if (!status_reserve(st, sc, Bmax)) return -1;
-    ss_sort_unique(st, keys, dn, nmax, sh, Bmax, part, carve(scratch), sc, dst, d_nuniq, d_err, prof_unique);
+    ss_sort_unique(st, keys, dn, nmax, sh, Bmax, part, carve(scratch), sc, dst, d_nuniq, d_err, prof_unique, kp);
     return 0;
 }
 
@@ -2048,7 +2088,7 @@ int sort_prepare(hipStream_t st, int ctx, size_t n, uint64_t vary, uint64_t cell
 uint64_t *sort_unique(hipStream_t st, uint64_t *keys, uint64_t *tmp, size_t n, uint64_t vary,
                       void *scratch, uint64_t *dst, unsigned long long *d_nuniq, unsigned *d_err,
                       int *prof_unique, int ctx, uint64_t cell_max, uint64_t *part, size_t part_cap,
-                      int sample)
+                      int sample, KeyPrune kp)
 {
     *prof_unique = -1;
     SortCtx &sc = g_ctx[ctx < 0 || ctx >= SORT_CONTEXTS ? 0 : ctx];
@@ -2063,7 +2103,7 @@ uint64_t *sort_unique(hipStream_t st, uint64_t *keys, uint64_t *tmp, size_t n, u
     if (!status_reserve(st, sc, P.entries)) return keys;
     if (P.B >= 1) {
         ss_sort_unique(st, keys, nullptr, n, ss_shape(sample), P.B, part, s, sc, dst, d_nuniq, d_err,
-                       prof_unique);
+                       prof_unique, kp);
         return keys;
     }
     if (P.coded) g_code_sorts++;
@@ -2085,9 +2125,14 @@ uint64_t *sort_unique(hipStream_t st, uint64_t *keys, uint64_t *tmp, size_t n, u
         }
     }
     *prof_unique = prof_hook_begin(4, st);
-    hipLaunchKernelGGL(k_os_unique, dim3((unsigned)P.utiles), dim3(UQ_THREADS), 0, st, src, n, dst,
-                       sc.status, s.ctr + OS_MAXPASS, next_epoch(st, sc), d_nuniq, d_err,
-                       npass > 0 ? 1 : 0, x, s.cval);
+    if (kp.on)
+        hipLaunchKernelGGL(k_os_unique<true>, dim3((unsigned)P.utiles), dim3(UQ_THREADS), 0, st, src, n, dst,
+                           sc.status, s.ctr + OS_MAXPASS, next_epoch(st, sc), d_nuniq, d_err,
+                           npass > 0 ? 1 : 0, x, s.cval, kp);
+    else
+        hipLaunchKernelGGL(k_os_unique<false>, dim3((unsigned)P.utiles), dim3(UQ_THREADS), 0, st, src, n, dst,
+                           sc.status, s.ctr + OS_MAXPASS, next_epoch(st, sc), d_nuniq, d_err,
+                           npass > 0 ? 1 : 0, x, s.cval, kp);
     snote(hipGetLastError());
     prof_hook_end(*prof_unique, 8.0 * n, st);
     return src;

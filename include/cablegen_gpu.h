@@ -231,6 +231,9 @@ int cg_dev_sync(void);
  *                network when there are at most this many (0: never; at
  *                most 16384) and they span at least "distinct_bitonic_bits"
  *                bits (0..64), else by 8-bit LSD passes;
+ *   "prune_in_sort"  with prune, 1: the moved boards of a layer step are
+ *                tested by the sort of its candidates, once per distinct
+ *                board; 0: each moved board is tested in the move kernel;
  *   "one_sync"   1: a layer step reads its counters from the device once
  *                (the candidate sort sized on the device); 0: twice;
  *   "batch_graph"  1: each batched layer step is captured and launched as
@@ -240,7 +243,7 @@ int cg_dev_sync(void);
  *                cell_code 1, carry_stream 1, canon_stab 1, sample_sort 1
  *                with 5000 keys per bucket, capacity 131072, 8192 buckets,
  *                probes 64, bucket_bitonic 0, distinct_bitonic 2048 with 9
- *                bits, one_sync 1, batch_graph 1;
+ *                bits, prune_in_sort 1, one_sync 1, batch_graph 1;
  *   "reset"      the state after initialisation: the built-in defaults, then
  *                the table loaded at initialisation if there was one.
  * Returns 0, or -1 for an unknown knob or value.  cg_tune_get also reads

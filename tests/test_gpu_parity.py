@@ -480,6 +480,12 @@ TB_CASES = {
     "two_initial": (dict(ignore_f=True), [0x11000000FFFFFFFF, 0x10100000FFFFFFFF],
                     [0x7000000000000000, 0x0000000000007000], 4, 160),
     "3x4": (dict(ignore_f=True), [0x1200000000000000 | 0xFFFF], [0x6000000000000000], 6, 90),
+    # tight prune settings: about 20% (C1) and 85% (DPDGap) of the moved
+    # boards fail the test, so the sorts drop boards on every path
+    "c1_hard_prune": (dict(ignore_f=True, prune=True, stsl=0, ltc=3, smallest_large=4),
+                      [0x11000000FFFFFFFF], [0x7000000000000000], 4, 260),
+    "dpdgap_hard_prune": (dict(ignore_f=True, prune=True, stsl=10, ltc=4, smallest_large=5),
+                          [0x00008765FFF3FFF2], [0x0000900000000000], 492, 600),
 }
 
 
@@ -493,6 +499,34 @@ def test_tablebase_vs_oracle(cg, name):
     tb.solve(win)
     check_tb(tb, layers, vals)
     tb.close()
+
+
+PRUNE_PATHS = {
+    "move_kernel": [("prune_in_sort", 0)],
+    "bitonic_buckets": [("bucket_bitonic", 1)],
+    "lsd": [("sample_sort", 0)],
+    "oversize_repeat": [("sample_cap", 64)],
+    "two_syncs": [("one_sync", 0)],
+    "lsd_distinct": [("distinct_bitonic", 0)],
+}
+
+
+@pytest.mark.parametrize("path", list(PRUNE_PATHS))
+@pytest.mark.parametrize("name", ["dpdgap_prune", "c1_hard_prune", "dpdgap_hard_prune"])
+def test_tablebase_prune_paths(cg, name, path):
+    """With prune, the moved boards are tested by the candidate sort once per
+    distinct board (the test depends only on the board's cells); every path
+    that sort can take drops the same boards as the test in the move kernel
+    (prune_in_sort 0): the hash-set buckets, buckets sorted whole by the
+    bitonic network, the LSD compaction, the LSD repeat of an oversize
+    bucket, the sort sized on the host, LSD passes over the distinct values."""
+    try:
+        for k, v in PRUNE_PATHS[path]:
+            cg.tune_set(k, v)
+        test_tablebase_vs_oracle(cg, name)
+    finally:
+        cg.tune_set("reset", 0)
+    assert cg.tune_get("prune_in_sort") == 1
 
 
 @pytest.mark.parametrize("name", ["c1", "dpdgap_prune"])
