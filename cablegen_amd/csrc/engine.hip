@@ -482,6 +482,7 @@ static MoveRules move_rules(const cg_rules &r)
     MoveRules m;
     m.ff = r.free_formation;
     m.prune = r.prune;
+    m.prune_inputs = 1;
     m.prune_moves = 1;
     m.stsl = r.stsl;
     m.ltc = r.ltc;
@@ -693,6 +694,11 @@ struct StepCounts {
     const uint64_t *cin = nullptr;
     uint64_t vary4 = 0, cell4 = CELLS_UNBOUNDED;
     int s4 = 0;
+    /* layer steps so far, and whether all of their spawn launches tested
+     * the spawned boards (then the next pre-move set needs no test) */
+    int steps = 0;
+    bool spawns_pruned = true;
+    KeyPrune kp4 = KeyPrune();   /* the prune test the carry sort applies */
 };
 
 /* spawn buffers are sized before |M_s| is known: for about twice the previous
@@ -719,7 +725,8 @@ static int carry_repeat(StepCounts &pc, uint64_t *C)
 {
     E.oversize = false;
     int hu4;
-    if (sort_enqueue(const_cast<uint64_t *>(pc.cin), pc.sumE, pc.vary4, C, 6, &hu4, 0, pc.cell4, SORT_LSD) ||
+    if (sort_enqueue(const_cast<uint64_t *>(pc.cin), pc.sumE, pc.vary4, C, 6, &hu4, 0, pc.cell4, SORT_LSD,
+                     pc.kp4) ||
         read_counters(6, 1))
         return -1;
     if (E.oversize) return fail_msg("sort: an LSD sort reported an oversize bucket");
@@ -815,8 +822,10 @@ static int sort_enqueue_dev(uint64_t *buf, const unsigned long long *dn, size_t 
  * host before their sort, two synchronisations per step) */
 static int g_one_sync = 1;
 
-/* with prune: the moved boards are tested by the candidate sort, once per
- * distinct board, instead of in the move kernel (knob prune_in_sort) */
+/* with prune: the moved boards are tested by the candidate sort and the
+ * spawn-4 boards by the carry sort, once per distinct board, and the
+ * spawn-2 boards by the spawn kernel, once per board, instead of all of
+ * them in the move kernel (knob prune_in_sort) */
 static int g_prune_in_sort = 1;
 
 /* One layer step (generate.c:409-550): the pre-move boards P and the carried
@@ -853,12 +862,13 @@ static int gen_step(const cg_rules &r, const uint64_t *P, size_t nP, uint64_t *C
     /* the moved boards' prune test, in the sort of the candidates: a layer
      * step's candidates hold each distinct board about 3.5 times, and the
      * test depends only on the board's cells (premove takes no prune) */
-    KeyPrune kp = KeyPrune();
-    if (r.prune && g_prune_in_sort && !premoved) {
-        kp.on = 1;
-        kp.stsl = r.stsl;
-        kp.ltc = r.ltc;
-        kp.sl = r.smallest_large;
+    KeyPrune kp = KeyPrune(), kps = KeyPrune();
+    if (r.prune && g_prune_in_sort) {
+        kps.on = 1;
+        kps.stsl = r.stsl;
+        kps.ltc = r.ltc;
+        kps.sl = r.smallest_large;
+        if (!premoved) kp = kps;
     }
     launch_init_stats(E.st, E.d_cnt);
     int h = prof_begin(PK_MOVE);
@@ -874,6 +884,10 @@ static int gen_step(const cg_rules &r, const uint64_t *P, size_t nP, uint64_t *C
         MoveRules mr = move_rules(r);
         mr.cs = cs;
         mr.prune_moves = kp.on ? 0 : 1;
+        /* after the first step the pre-move boards are this generation's
+         * spawns, tested by the spawn kernel and the carry sort when every
+         * step had them do so */
+        mr.prune_inputs = (kps.on && pc.steps > 0 && pc.spawns_pruned) ? 0 : 1;
         launch_move(E.st, P, nP, nullptr, E.A.p, E.d_cnt, lut, mr, 1);
     }
     prof_end(h, 0);
@@ -923,7 +937,7 @@ static int gen_step(const cg_rules &r, const uint64_t *P, size_t nP, uint64_t *C
         if (ensure(nxtP, cap + cbound + 1) || ensure(S4, cap + 1)) return -1;
         launch_init_stats(E.st, E.d_cnt + 8);
         hs = prof_begin(PK_SPAWN);
-        launch_spawn(E.st, *M, mrows, E.d_cnt + 7, nxtP.p, S4.p, cap, E.d_cnt + 8, cs);
+        launch_spawn(E.st, *M, mrows, E.d_cnt + 7, nxtP.p, S4.p, cap, E.d_cnt + 8, cs, kps);
         prof_end(hs, 0);
         /* the previous step's carry sort (second stream) overlapped this
          * step so far; its count is read with the others */
@@ -977,10 +991,13 @@ static int gen_step(const cg_rules &r, const uint64_t *P, size_t nP, uint64_t *C
     int hu4;
     pc.vary4 = vary_of(sumE, or4, and4);
     pc.cell4 = cell_bound(vb, s + 4, and4);
-    if (sort_enqueue(S4.p, sumE, pc.vary4, nxtC.p, 6, &hu4, g_carry_lane, pc.cell4, SORT_SAMPLE_SPARSE))
+    pc.kp4 = kps;
+    if (sort_enqueue(S4.p, sumE, pc.vary4, nxtC.p, 6, &hu4, g_carry_lane, pc.cell4, SORT_SAMPLE_SPARSE, kps))
         return -1;
     pc.cin = S4.p;
     pc.s4 ^= 1;
+    pc.steps++;
+    pc.spawns_pruned = pc.spawns_pruned && kps.on;
     if (vb) vb->s = s + 2;
     pc.pending = true;
     pc.sumE = sumE;

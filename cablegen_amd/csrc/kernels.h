@@ -14,6 +14,9 @@ namespace cg {
 struct MoveRules {
     int ff;          /* free_formation */
     int prune;
+    /* with prune: 1 tests every input board in the move kernel; 0 when the
+     * inputs are spawns that the spawn kernel already tested */
+    int prune_inputs;
     /* with prune: 1 tests every moved board in the move kernel; 0 leaves
      * the moved boards to the sort of the candidates (KeyPrune), which tests
      * each distinct board once */
@@ -149,10 +152,15 @@ void launch_count_empties(hipStream_t st, const uint64_t *in, size_t n, unsigned
  * (the carry sort's key bits; the out2 keys are only moved, never sorted, so
  * stats[1..2] are left as they are).  Only positions below cap are written: a
  * count above cap means the buffers were too small and the launch must be
- * repeated. */
+ * repeated.  With kp.on, a 2-spawn that fails the prune test is written to
+ * out2 as 0, a board with no tile, which has no legal move: the 2-spawns
+ * are only moved, so they need no prune test of their own.  The test
+ * depends only on the multiset of cells, so it is taken once per board, not
+ * once per empty cell.  (The 4-spawns are sorted first; their sort drops the
+ * boards that fail the test, KeyPrune: zeros would pile up in one bucket.) */
 void launch_spawn(hipStream_t st, const uint64_t *in, size_t n, const unsigned long long *dn,
                   uint64_t *out2, uint64_t *out4, size_t cap, unsigned long long *stats,
-                  cg_canon_sel_t cs = cg_canon_all());
+                  cg_canon_sel_t cs = cg_canon_all(), KeyPrune kp = KeyPrune());
 
 
 /* Sorts keys[0..n) ascending and writes the unique keys to dst (capacity n),

@@ -91,7 +91,7 @@ __global__ __launch_bounds__(MV_THREADS, 1) void k_move(
         uint32_t k = 0;
         if (i < n) {
             const uint64_t b = in[i];
-            if (!(r.prune && cg_prune(b, r.stsl, r.ltc, r.sl))) {
+            if (!(r.prune && r.prune_inputs && cg_prune(b, r.stsl, r.ltc, r.sl))) {
                 uint64_t y = cg_rot_cw(b);
 #pragma unroll
                 for (int d = 0; d < 4; d++) {
@@ -192,7 +192,7 @@ __global__ __launch_bounds__(MV_THREADS, 8) void k_move_gen(
             const size_t i = b0 + (size_t)k * MV_THREADS + threadIdx.x;
             if (i >= n) continue;
             const uint64_t b = in[i];
-            if (r.prune && cg_prune(b, r.stsl, r.ltc, r.sl)) continue;
+            if (r.prune && r.prune_inputs && cg_prune(b, r.stsl, r.ltc, r.sl)) continue;
             const uint64_t y = cg_rot_cw(b);
 #pragma unroll
             for (int d = 0; d < 4; d++) {
@@ -263,6 +263,7 @@ __device__ __forceinline__ int next_cell(uint64_t &z)
 struct SpawnPairs {
     uint64_t sb[SP_THREADS];
     uint8_t pb[16 * SP_THREADS], pc[16 * SP_THREADS];
+    uint8_t pf[SP_THREADS];   /* per board: its 2-spawns fail the prune test */
     uint32_t wtmp[17];
 };
 
@@ -276,7 +277,8 @@ __global__ __launch_bounds__(SP_THREADS) void k_spawn(const uint64_t *__restrict
                                                       const unsigned long long *dn,
                                                       uint64_t *__restrict__ out2,
                                                       uint64_t *__restrict__ out4, size_t cap,
-                                                      unsigned long long *stats, cg_canon_sel_t cs)
+                                                      unsigned long long *stats, cg_canon_sel_t cs,
+                                                      KeyPrune kp)
 {
     __shared__ SpawnPairs sp;
     __shared__ unsigned long long base_sh;
@@ -300,6 +302,12 @@ __global__ __launch_bounds__(SP_THREADS) void k_spawn(const uint64_t *__restrict
         nb = i + stride < n ? in[i + stride] : 0;
         uint64_t z = i < n ? cg_zmask(b) : 0;
         uint32_t e = (uint32_t)cg_popc64(z);
+        if (kp.on) {
+            /* a 2-spawn fills one empty cell with a 2: the same multiset
+             * whichever cell, so one test per board (the lowest empty cell) */
+            const uint64_t lb = z & (0ULL - z);
+            sp.pf[threadIdx.x] = (uint8_t)(z && cg_prune(b | lb, kp.stsl, kp.ltc, kp.sl) ? 1u : 0u);
+        }
         uint32_t tot;
         uint32_t off = block_excl_scan<uint32_t>(e, &tot, sp.wtmp);
         sp.sb[threadIdx.x] = b;
@@ -314,6 +322,7 @@ __global__ __launch_bounds__(SP_THREADS) void k_spawn(const uint64_t *__restrict
             int x = sp.pc[p];
             uint64_t c2 = cg_canon_fast(cg_set_tile(bb, x, 1), cs, 0),
                      c4 = cg_canon_fast(cg_set_tile(bb, x, 2), cs, 0);
+            if (kp.on && sp.pf[sp.pb[p]]) c2 = 0;
             size_t o = base + p;
             if (o < cap) {   /* past the capacity: only counted (the host retries) */
                 out2[o] = c2;
@@ -329,7 +338,8 @@ __global__ __launch_bounds__(SP_THREADS) void k_spawn(const uint64_t *__restrict
 }
 
 void launch_spawn(hipStream_t st, const uint64_t *in, size_t n, const unsigned long long *dn,
-                  uint64_t *out2, uint64_t *out4, size_t cap, unsigned long long *stats, cg_canon_sel_t cs)
+                  uint64_t *out2, uint64_t *out4, size_t cap, unsigned long long *stats, cg_canon_sel_t cs,
+                  KeyPrune kp)
 {
     if (!n) return;
     size_t blocks = (n + SP_THREADS - 1) / SP_THREADS;
@@ -339,7 +349,7 @@ void launch_spawn(hipStream_t st, const uint64_t *in, size_t n, const unsigned l
      * the one pair now, 19.1 with 512 and 18.8 with 1024) */
     if (blocks > 512) blocks = 512;
     hipLaunchKernelGGL(k_spawn, dim3((unsigned)blocks), dim3(SP_THREADS), 0, st, in, n, dn, out2,
-                       out4, cap, stats, cs);
+                       out4, cap, stats, cs, kp);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1221,7 +1231,7 @@ __global__ __launch_bounds__(MV_THREADS, 8) void k_bmove(const uint64_t *__restr
             const BatchTable &T = tabs[tid];
             if (step >= T.nsteps) continue;
             const uint64_t b = bdec(ck, T);
-            if (r.prune && cg_prune(b, r.stsl, r.ltc, r.sl)) continue;
+            if (r.prune && r.prune_inputs && cg_prune(b, r.stsl, r.ltc, r.sl)) continue;
             const uint64_t y = cg_rot_cw(b);
 #pragma unroll
             for (int d = 0; d < 4; d++) {

@@ -515,8 +515,9 @@ PRUNE_PATHS = {
 @pytest.mark.parametrize("name", ["dpdgap_prune", "c1_hard_prune", "dpdgap_hard_prune"])
 def test_tablebase_prune_paths(cg, name, path):
     """With prune, the moved boards are tested by the candidate sort once per
-    distinct board (the test depends only on the board's cells); every path
-    that sort can take drops the same boards as the test in the move kernel
+    distinct board and the spawned boards by the spawn kernel once per board
+    and value (the test depends only on the board's cells); every path the
+    sort can take drops the same boards as the tests in the move kernel
     (prune_in_sort 0): the hash-set buckets, buckets sorted whole by the
     bitonic network, the LSD compaction, the LSD repeat of an oversize
     bucket, the sort sized on the host, LSD passes over the distinct values."""
@@ -683,7 +684,7 @@ def test_adjoint_vs_oracle_and_fd(cg):
     tb.close()
 
 
-@pytest.mark.parametrize("name", ["c1_ignore_f", "dpdgap_prune"])
+@pytest.mark.parametrize("name", ["c1_ignore_f", "dpdgap_prune", "c1_hard_prune", "dpdgap_hard_prune"])
 def test_tablebase_premove_vs_oracle(cg, name):
     """premove = 1 (the initial boards are positions after the move, taken in
     any orientation): layers and values against the oracle's premove
