@@ -530,13 +530,19 @@ def batch_bench(args):
 
     tb = cg.Tablebase(r)   # one device-resident table reused: its arena is kept
 
+    per_table = []
+
     def one_at_a_time():
         tot = 0
+        per_table.clear()
         t0 = time.perf_counter()
         for c in cfgs:
+            ta = time.perf_counter()
             tb.generate(c["initial"], c["start"], c["end"])
             tb.solve(c["winstates"])
-            tot += tb.total_boards()
+            n = tb.total_boards()
+            per_table.append((n, time.perf_counter() - ta))
+            tot += n
         return tot, time.perf_counter() - t0
 
     B = cg.Batch(r)
@@ -556,6 +562,7 @@ def batch_bench(args):
     c0 = cg.tune_get("cell_code_sorts")
     r0 = cg.tune_get("sample_repeats")
     n1, t_one = one_at_a_time()
+    alone = sorted(per_table, reverse=True)
     c1 = cg.tune_get("cell_code_sorts")
     r1 = cg.tune_get("sample_repeats")
     # the one-at-a-time leg's kernels (an untimed pass with every kernel timed)
@@ -589,6 +596,23 @@ def batch_bench(args):
     finally:
         cg.tune_set("batch_graph", 1)
     sizes = sorted((B.total_boards_table(t) for t in range(len(cfgs))), reverse=True)
+    # The largest tables, holding half of the boards, run alone at a rate
+    # that batching cannot raise (each already fills the GPU): at that rate
+    # for every board the batch would take nb / rate, which bounds its
+    # speed-up over the one-at-a-time leg.
+    half, acc, t_big = 0, 0, 0.0
+    for n, t in alone:
+        if acc >= n1 / 2:
+            break
+        acc += n
+        t_big += t
+        half += 1
+    big_rate = acc / t_big if t_big > 0 else 0.0
+    alone_out = {"tables_holding_half_the_boards": half, "their_boards": acc, "their_seconds": t_big,
+                 "their_boards_per_s": big_rate,
+                 "rest_seconds": t_one - t_big,
+                 "speedup_if_every_board_ran_at_their_rate": (t_one / (n1 / big_rate)) if big_rate else None,
+                 "per_table_boards_seconds": [[n, round(t, 6)] for n, t in alone]}
     out = {"metric": "batched tablebases: boards/s, batched vs one at a time",
            "tables": len(cfgs), "boards": nb, "one_at_a_time_s": t_one,
            "batched_generate_s": tg, "batched_solve_s": ts,
@@ -598,6 +622,7 @@ def batch_bench(args):
            "batched_eager_generate_s": tg_eager, "batched_eager_solve_s": ts_eager,
            "speedup_eager_steps": t_one / (tg_eager + ts_eager),
            "largest_tables": sizes[:5], "median_table": sizes[len(sizes) // 2],
+           "one_at_a_time_tables": alone_out,
            "batched_kernels_ms": {k: v[1] for k, v in kern.items()},
            "one_at_a_time_kernels": {k: [v[0], v[1]] for k, v in kern_one.items()},
            "sample_sort_repeats": {"one_at_a_time": r1 - r0, "batched": r2 - r1},
