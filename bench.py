@@ -444,6 +444,7 @@ def run_autotune(cg, args):
         cg.tune_set("builtin", 0)
         cg.tune_set("cell_code", args.cell_code)
         cg.tune_set("carry_stream", args.carry_stream)
+        cg.tune_set("carry_sort", args.carry_sort)
         cg.tune_set("sample_sort", args.sample_sort)
         cg.tune_set("canon_stab", args.canon_stab)
         cg.tune_set("one_sync", args.one_sync)
@@ -451,6 +452,7 @@ def run_autotune(cg, args):
             cg.tune_set("sample_bucket", args.sample_bucket)
         return {"source": "built-in defaults: 8-bit digits, 8192-key tiles, window 16, dir delta 0",
                 "cell_code": args.cell_code, "carry_stream": args.carry_stream,
+                "carry_sort": args.carry_sort,
                 "sample_sort": args.sample_sort, "sample_bucket": cg.tune_get("sample_bucket"),
                 "canon_stab": args.canon_stab, "one_sync": args.one_sync}
     if mode == "stored" and os.path.exists(cg.TUNED_TABLE):
@@ -464,6 +466,7 @@ def run_autotune(cg, args):
             json.dump(rep, f)
     cg.tune_set("cell_code", args.cell_code)
     cg.tune_set("carry_stream", args.carry_stream)
+    cg.tune_set("carry_sort", args.carry_sort)
     cg.tune_set("sample_sort", args.sample_sort)
     cg.tune_set("canon_stab", args.canon_stab)
     cg.tune_set("one_sync", args.one_sync)
@@ -478,7 +481,8 @@ def run_autotune(cg, args):
         sel[name] = {nb: rep["sort"][cls]["pick"][nb] for nb in ("24", "32", "36", "40", "44", "48", "54")}
     return {"source": src, "seconds": time.perf_counter() - t0, "dir_delta": dir_delta,
             "sort_picks_bits_x_kpt/window": sel, "cell_code": args.cell_code,
-            "carry_stream": args.carry_stream, "sample_sort": args.sample_sort,
+            "carry_stream": args.carry_stream, "carry_sort": args.carry_sort,
+            "sample_sort": args.sample_sort,
             "sample_bucket": cg.tune_get("sample_bucket"), "canon_stab": args.canon_stab,
             "one_sync": args.one_sync}
 
@@ -807,6 +811,9 @@ def main():
     ap.add_argument("--tuning", choices=("stored", "auto", "off"), default="stored",
                     help="sort/directory tuning: stored autotuner table, run the autotuner, or defaults")
     ap.add_argument("--save-tuning", default=None, help="write the autotuner report (JSON) here")
+    ap.add_argument("--carry-sort", type=int, choices=(0, 1), default=0,
+                    help="1: each step's spawn-4 carry is sorted and deduplicated before it is "
+                         "moved (on the stream --carry-stream names); 0: moved as written")
     ap.add_argument("--carry-stream", type=int, choices=(0, 1), default=1,
                     help="0: the carry sort on the main stream instead of a second one")
     ap.add_argument("--dir-delta", type=int, choices=range(-4, 4), default=None,
@@ -876,7 +883,12 @@ def main():
     stage_ms = {k: kern[k]["ms"] for k in ("move", "spawn", "solve")}
     stage_ms["sort"] = sum(kern[k]["ms"] for k in SORT_KINDS)
     dom = max(STAGES, key=lambda k: stage_ms[k])
-    cg.prof_select(dom)
+    # the second stream's units from this step (every launch timed); the
+    # timed steps time the main stream's launches only (fewer events in the
+    # timed region)
+    second = stage_roofline(cg, dom, 1)["second_stream"]
+    second["source"] = "the untimed step with every kernel timed"
+    cg.prof_select(dom + "@main")
     cg.prof_reset()
     cg.lib().cg_dev_sync()
     gen_t, sol_t = [], []
@@ -890,11 +902,13 @@ def main():
     value = total / T
 
     roof = stage_roofline(cg, dom, args.steps)
-    if dom == "sort" and args.carry_stream == 1 and not args.no_isolated:
+    roof["second_stream"] = second
+    if dom == "sort" and args.carry_sort == 1 and args.carry_stream == 1 and not args.no_isolated:
         # the same stage with nothing beside it: one untimed step with the
         # carry sort on the main stream, so no kernel of another stream
         # shares the GPU with a sort's kernels while they are timed
         cg.prof_reset()
+        cg.prof_select(dom)
         cg.tune_set("carry_stream", 0)
         step()
         cg.lib().cg_dev_sync()

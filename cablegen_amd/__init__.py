@@ -402,8 +402,10 @@ def prof_query(kernel):
 
 
 def prof_select(kernel=None):
-    """time only one kernel's launches (None: every kernel)"""
-    lib().cg_prof_select(kernel.encode() if kernel else None)
+    """time only one kernel's launches (None: every kernel; "<kernel>@main":
+    its launches on the main stream only)"""
+    if lib().cg_prof_select(kernel.encode() if kernel else None) != 0:
+        raise RuntimeError("cg_prof_select(%r) failed" % (kernel,))
 
 
 def prof_query_stream(kernel, stream):

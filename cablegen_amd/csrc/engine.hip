@@ -239,6 +239,7 @@ struct ProfRec {
 };
 static bool g_prof = false;
 static int g_prof_only = -1;   /* >= 0: only this kernel kind is timed */
+static bool g_prof_main = false;   /* only launches on the main stream are timed */
 /* cg_prof_select("sort"): every kernel of the sort stage is timed */
 static constexpr int PK_GROUP_SORT = 1000;
 static bool is_sort_kind(int kind)
@@ -279,6 +280,7 @@ static bool prof_wants(int kind)
 static int prof_begin(int kind, hipStream_t st = nullptr)
 {
     if (!prof_wants(kind)) return -1;
+    if (g_prof_main && st && st != E.st) return -1;
     ProfRec r;
     r.kind = kind;
     r.st = st ? st : E.st;
@@ -355,20 +357,26 @@ extern "C" void cg_prof_enable(int on)
 
 extern "C" int cg_prof_select(const char *kernel)
 {
+    g_prof_main = false;
     if (!kernel || !*kernel) {
         g_prof_only = -1;
         return 0;
     }
-    for (int i = 0; i < PK_N; i++)
-        if (!strcmp(kernel, PK_NAMES[i])) {
-            g_prof_only = i;
-            return 0;
-        }
-    if (!strcmp(kernel, "sort")) {
-        g_prof_only = PK_GROUP_SORT;
-        return 0;
+    /* "<kernel>@main": that kernel's launches on the main stream only */
+    std::string k(kernel);
+    const size_t at = k.find('@');
+    if (at != std::string::npos) {
+        if (k.substr(at) != "@main") return fail_msg("cg_prof_select: unknown stream");
+        k = k.substr(0, at);
     }
-    return fail_msg("cg_prof_select: unknown kernel");
+    int sel = -2;
+    for (int i = 0; i < PK_N; i++)
+        if (k == PK_NAMES[i]) sel = i;
+    if (k == "sort") sel = PK_GROUP_SORT;
+    if (sel == -2) return fail_msg("cg_prof_select: unknown kernel");
+    g_prof_only = sel;
+    g_prof_main = at != std::string::npos;
+    return 0;
 }
 
 extern "C" void cg_prof_reset(void)
@@ -699,6 +707,9 @@ struct StepCounts {
     int steps = 0;
     bool spawns_pruned = true;
     KeyPrune kp4 = KeyPrune();   /* the prune test the carry sort applies */
+    /* the carry C' = S4 kept as written (a multiset, like P), or sorted
+     * and deduplicated on the second stream; fixed at the first step */
+    bool carry_sorted = true;
 };
 
 /* spawn buffers are sized before |M_s| is known: for about twice the previous
@@ -828,6 +839,14 @@ static int g_one_sync = 1;
  * them in the move kernel (knob prune_in_sort) */
 static int g_prune_in_sort = 1;
 
+/* the spawn-4 carry of a layer step moved as written, duplicates included
+ * (knob carry_sort 0), or sorted and deduplicated on the second stream (1):
+ * P_{s+4} is a multiset either way, and M_{s+4} = uniq(moves(P_{s+4})) is
+ * the same set.  A C2 spawn-4 set holds each board 1.3 to 1.45 times; the
+ * moves and the candidate sort of those duplicates cost less than the carry
+ * sort did: C2 generated in 181-182 ms against 212-214 ms. */
+static int g_carry_sort = 0;
+
 /* One layer step (generate.c:409-550): the pre-move boards P and the carried
  * spawn-4 set C give M_s = uniq(moves(P)) (from `la`), the next pre-move
  * boards P_{s+2} = S2 u C (written to nxtP) and the new carry C' = uniq(S4)
@@ -925,7 +944,9 @@ static int gen_step(const cg_rules &r, const uint64_t *P, size_t nP, uint64_t *C
     /* spawns of M_s (the kernel reads |M_s| on the device; the grid covers
      * the bound): S2 to the front of nxtP, S4 to the spawn-4 buffer the
      * pending carry sort does not read */
-    Buf &S4 = E.S4[pc.s4];
+    if (pc.steps == 0) pc.carry_sorted = g_carry_sort != 0;
+    /* unsorted, the 4-spawns go straight to the carry buffer */
+    Buf &S4 = pc.carry_sorted ? E.S4[pc.s4] : nxtC;
     const size_t mrows = mcap;
     size_t emax = dev ? pc.emax : (size_t)(16 - nonzero_nibbles(ncand ? mand : ~0ULL));
     size_t cap = emax * spawn_rows_estimate(pc, mrows);
@@ -937,7 +958,8 @@ static int gen_step(const cg_rules &r, const uint64_t *P, size_t nP, uint64_t *C
         if (ensure(nxtP, cap + cbound + 1) || ensure(S4, cap + 1)) return -1;
         launch_init_stats(E.st, E.d_cnt + 8);
         hs = prof_begin(PK_SPAWN);
-        launch_spawn(E.st, *M, mrows, E.d_cnt + 7, nxtP.p, S4.p, cap, E.d_cnt + 8, cs, kps);
+        launch_spawn(E.st, *M, mrows, E.d_cnt + 7, nxtP.p, S4.p, cap, E.d_cnt + 8, cs, kps,
+                     pc.carry_sorted ? 0 : 1);
         prof_end(hs, 0);
         /* the previous step's carry sort (second stream) overlapped this
          * step so far; its count is read with the others */
@@ -983,6 +1005,15 @@ static int gen_step(const cg_rules &r, const uint64_t *P, size_t nP, uint64_t *C
     la.shrink(*M, mcap, *nM);
     /* P_{s+2} = S2 u C: the carry after the 2-spawns */
     if (nC) HC(hipMemcpyAsync(nxtP.p + sumE, C, nC * sizeof(uint64_t), hipMemcpyDeviceToDevice, E.st));
+    if (!pc.carry_sorted) {
+        /* C' = S4 as the spawn kernel wrote it into nxtC */
+        pc.steps++;
+        pc.spawns_pruned = pc.spawns_pruned && kps.on;
+        if (vb) vb->s = s + 2;
+        *nP2 = sumE + nC;
+        *nC2 = sumE;
+        return 0;
+    }
     const uint64_t or4 = E.h_cnt[11], and4 = E.h_cnt[12];
     /* C' = uniq(S4) on the second stream (the spawns finished at the
      * synchronisation above); it overlaps the next step's moves, and its
@@ -1115,6 +1146,12 @@ static int key_or_and(const uint64_t *keys, size_t n, uint64_t *o, uint64_t *a)
     return 0;
 }
 
+/* the layer directories of a generation built after its last step
+ * (knob dir_deferred): built between the steps, their kernels share the GPU
+ * with the carry sort of the second stream and take about 2.5 times longer
+ * (C2: 21.8 ms per generation against 8.7 ms alone) */
+static int g_dir_deferred = 1;
+
 struct cg_tablebase {
     cg_rules r;
     int start = 0;
@@ -1219,7 +1256,7 @@ extern "C" int cg_tb_generate_bounded(cg_tablebase *tb, const uint64_t *initial,
         l.plan = plan_index(l.n, mor, mand);
         l.dir = (uint32_t *)tb->arena.alloc(l.plan.dir_entries * 4);
         if (!l.dir) return fail_msg("index allocation failed");
-        if (build_index(l.keys, l.n, l.plan, l.dir)) return -1;
+        if (!g_dir_deferred && build_index(l.keys, l.n, l.plan, l.dir)) return -1;
         tb->L.push_back(l);
         total += l.n;
         if (last_sum) *last_sum = s;
@@ -1230,6 +1267,9 @@ extern "C" int cg_tb_generate_bounded(cg_tablebase *tb, const uint64_t *initial,
         if (max_boards && total > max_boards) break;
     }
     if (counts_finish(pc, tb->C.p)) return -1;
+    if (g_dir_deferred)
+        for (const TbLayer &l : tb->L)
+            if (build_index(l.keys, l.n, l.plan, l.dir)) return -1;
     return check_sync(E.st, "cg_tb_generate");
 }
 
@@ -1701,6 +1741,16 @@ extern "C" int cg_tune_set(const char *knob, int value)
         sort_set_bucket_bitonic(value);
         return 0;
     }
+    if (!strcmp(knob, "carry_sort")) {
+        if (value != 0 && value != 1) return fail_msg("carry_sort: 0 or 1");
+        g_carry_sort = value;
+        return 0;
+    }
+    if (!strcmp(knob, "dir_deferred")) {
+        if (value != 0 && value != 1) return fail_msg("dir_deferred: 0 or 1");
+        g_dir_deferred = value;
+        return 0;
+    }
     if (!strcmp(knob, "prune_in_sort")) {
         if (value != 0 && value != 1) return fail_msg("prune_in_sort: 0 or 1");
         g_prune_in_sort = value;
@@ -1751,6 +1801,8 @@ extern "C" int cg_tune_set(const char *knob, int value)
         g_canon_stab = 1;
         g_one_sync = 1;
         g_prune_in_sort = 1;
+        g_dir_deferred = 1;
+        g_carry_sort = 0;
         g_batch_graph = 1;
         sort_set_sample(1, 5000, 0);
         sort_set_sample_limits(8192, 0);
@@ -1775,6 +1827,8 @@ extern "C" int cg_tune_get(const char *knob)
     if (!strcmp(knob, "canon_stab")) return g_canon_stab;
     if (!strcmp(knob, "one_sync")) return g_one_sync;
     if (!strcmp(knob, "prune_in_sort")) return g_prune_in_sort;
+    if (!strcmp(knob, "dir_deferred")) return g_dir_deferred;
+    if (!strcmp(knob, "carry_sort")) return g_carry_sort;
     if (!strcmp(knob, "batch_graph")) return g_batch_graph;
     if (!strcmp(knob, "sample_bmax")) return sort_sample_bmax();
     if (!strcmp(knob, "bucket_bitonic")) return sort_bucket_bitonic();
@@ -2530,6 +2584,10 @@ extern "C" int cg_batch_generate(cg_batch *b)
      * the pre-move boards is in the other */
     Buf *cbuf[2] = { &b->C, &b->C2 };
     int cw = 0;
+    /* carry_sort 0: a step's spawn-4 boards are moved two steps later as
+     * written; prev_s4 is the count of the previous step's */
+    const bool carry_sorted = g_carry_sort != 0;
+    size_t prev_s4 = 0;
     auto enqueue_deferred = [&](void) -> int {
         if (df.cp_n) launch_copy_and_u64(E.st, df.cp_dst, df.cp_src, df.cp_n, ~0ULL);
         if (df.dir_step >= 0) {
@@ -2554,13 +2612,14 @@ extern "C" int cg_batch_generate(cg_batch *b)
         const bool graph = dev && g_batch_graph;
         /* buffers and allocations first: nothing may allocate inside a
          * capture */
-        const size_t cbound = df.carry ? df.c_n : 0;   /* the carry this step's spawns sit before */
+        /* the carry this step's spawns sit before */
+        const size_t cbound = carry_sorted ? (df.carry ? df.c_n : 0) : prev_s4;
         size_t cap = (size_t)maxfree * spawn_rows_estimate(pc, cmax);
         Buf &S4 = E.S4[pc.s4];
         st.keys = (uint64_t *)b->arena.alloc(std::max<size_t>(cmax, 1) * 8);
         if (!st.keys) return fail_msg("batch layer allocation failed");
         if (ensure(E.A, cmax + 1) || ensure(b->M, cmax + 1) || ensure(b->P2, cap + cbound + 1) ||
-            ensure(S4, cap + 1) || ensure(*cbuf[cw], cbound + 1))
+            ensure(S4, cap + 1) || (carry_sorted && ensure(*cbuf[cw], cbound + 1)))
             return -1;
         {
             size_t scr = sort_scratch_bytes(cmax);
@@ -2711,6 +2770,27 @@ extern "C" int cg_batch_generate(cg_batch *b)
         HC(hipMemcpyAsync(d_seg, st.off.data(), (ntab + 1) * sizeof(size_t), hipMemcpyHostToDevice, E.st));
         HC(hipMemcpyAsync(d_plan, st.bplan.data(), ntab * sizeof(BatchIndexPlan), hipMemcpyHostToDevice, E.st));
         b->steps.push_back(std::move(st));
+        if (!carry_sorted) {
+            /* P_{k+1} = S2 u the previous step's spawn-4 boards as written
+             * (the spawn-4 buffers alternate between steps); the next
+             * launch carries this layer's directories */
+            if (prev_s4)
+                HC(hipMemcpyAsync(b->P2.p + sumE, E.S4[pc.s4 ^ 1].p, prev_s4 * sizeof(uint64_t),
+                                  hipMemcpyDeviceToDevice, E.st));
+            df.carry = false;
+            df.cp_n = 0;
+            df.dir_step = k;
+            df.dir_entries = entries;
+            df.d_seg = d_seg;
+            df.d_plan = d_plan;
+            pc.s4 ^= 1;
+            const size_t n2 = sumE + prev_s4;
+            prev_s4 = sumE;
+            std::swap(b->P, b->P2);
+            nP = n2;
+            nC = 0;
+            continue;
+        }
         /* what the next launch carries: P_{k+1} = S2 u C (the carry before
          * this step's, now settled), the carry sort of this step's spawn-4
          * boards, this layer's directories */

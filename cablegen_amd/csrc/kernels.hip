@@ -263,7 +263,7 @@ __device__ __forceinline__ int next_cell(uint64_t &z)
 struct SpawnPairs {
     uint64_t sb[SP_THREADS];
     uint8_t pb[16 * SP_THREADS], pc[16 * SP_THREADS];
-    uint8_t pf[SP_THREADS];   /* per board: its 2-spawns fail the prune test */
+    uint8_t pf[SP_THREADS];   /* per board: bit 0 its 2-spawns fail the prune test, bit 1 its 4-spawns */
     uint32_t wtmp[17];
 };
 
@@ -278,7 +278,7 @@ __global__ __launch_bounds__(SP_THREADS) void k_spawn(const uint64_t *__restrict
                                                       uint64_t *__restrict__ out2,
                                                       uint64_t *__restrict__ out4, size_t cap,
                                                       unsigned long long *stats, cg_canon_sel_t cs,
-                                                      KeyPrune kp)
+                                                      KeyPrune kp, int prune4)
 {
     __shared__ SpawnPairs sp;
     __shared__ unsigned long long base_sh;
@@ -303,10 +303,14 @@ __global__ __launch_bounds__(SP_THREADS) void k_spawn(const uint64_t *__restrict
         uint64_t z = i < n ? cg_zmask(b) : 0;
         uint32_t e = (uint32_t)cg_popc64(z);
         if (kp.on) {
-            /* a 2-spawn fills one empty cell with a 2: the same multiset
-             * whichever cell, so one test per board (the lowest empty cell) */
+            /* a spawn fills one empty cell with a 2 (or a 4): the same
+             * multiset whichever cell, so one test per board and value (at
+             * the lowest empty cell) */
             const uint64_t lb = z & (0ULL - z);
-            sp.pf[threadIdx.x] = (uint8_t)(z && cg_prune(b | lb, kp.stsl, kp.ltc, kp.sl) ? 1u : 0u);
+            uint32_t f = 0;
+            if (z && cg_prune(b | lb, kp.stsl, kp.ltc, kp.sl)) f |= 1u;
+            if (z && prune4 && cg_prune(b | (lb << 1), kp.stsl, kp.ltc, kp.sl)) f |= 2u;
+            sp.pf[threadIdx.x] = (uint8_t)f;
         }
         uint32_t tot;
         uint32_t off = block_excl_scan<uint32_t>(e, &tot, sp.wtmp);
@@ -322,7 +326,11 @@ __global__ __launch_bounds__(SP_THREADS) void k_spawn(const uint64_t *__restrict
             int x = sp.pc[p];
             uint64_t c2 = cg_canon_fast(cg_set_tile(bb, x, 1), cs, 0),
                      c4 = cg_canon_fast(cg_set_tile(bb, x, 2), cs, 0);
-            if (kp.on && sp.pf[sp.pb[p]]) c2 = 0;
+            if (kp.on) {
+                const uint32_t f = sp.pf[sp.pb[p]];
+                c2 = (f & 1u) ? 0ULL : c2;
+                c4 = (f & 2u) ? 0ULL : c4;
+            }
             size_t o = base + p;
             if (o < cap) {   /* past the capacity: only counted (the host retries) */
                 out2[o] = c2;
@@ -339,7 +347,7 @@ __global__ __launch_bounds__(SP_THREADS) void k_spawn(const uint64_t *__restrict
 
 void launch_spawn(hipStream_t st, const uint64_t *in, size_t n, const unsigned long long *dn,
                   uint64_t *out2, uint64_t *out4, size_t cap, unsigned long long *stats, cg_canon_sel_t cs,
-                  KeyPrune kp)
+                  KeyPrune kp, int prune4)
 {
     if (!n) return;
     size_t blocks = (n + SP_THREADS - 1) / SP_THREADS;
@@ -349,7 +357,7 @@ void launch_spawn(hipStream_t st, const uint64_t *in, size_t n, const unsigned l
      * the one pair now, 19.1 with 512 and 18.8 with 1024) */
     if (blocks > 512) blocks = 512;
     hipLaunchKernelGGL(k_spawn, dim3((unsigned)blocks), dim3(SP_THREADS), 0, st, in, n, dn, out2,
-                       out4, cap, stats, cs, kp);
+                       out4, cap, stats, cs, kp, prune4);
 }
 
 /* ------------------------------------------------------------------------ */

@@ -202,8 +202,12 @@ int cg_dev_sync(void);
  *                layer, a step that overflows them is repeated with exact
  *                buffers; 1: sized for half the previous layer, so that
  *                growing layers take the repeat (for tests);
- *   "carry_stream"  1: the carry sort runs on a second stream beside the
- *                next step's moves; 0: on the main stream after the spawns;
+ *   "carry_sort" 0: a layer step's spawn-4 boards are moved as written
+ *                (duplicates included) two steps later; 1: they are sorted
+ *                and deduplicated first (the carry sort);
+ *   "carry_stream"  with carry_sort 1 (and in batches): 1: the carry sort
+ *                runs on a second stream beside the next step's moves; 0: on
+ *                the main stream after the spawns;
  *   "cell_code"  1: a generation's sorts code pairs of varying cells as
  *                a * base + b when the layer's tile sum bounds the cell
  *                values and that saves a radix pass; 0: bit packing only;
@@ -234,6 +238,8 @@ int cg_dev_sync(void);
  *   "prune_in_sort"  with prune, 1: the moved boards of a layer step are
  *                tested by the sort of its candidates, once per distinct
  *                board; 0: each moved board is tested in the move kernel;
+ *   "dir_deferred"  1: the layer directories of a generation are built
+ *                after its last step; 0: after each step;
  *   "one_sync"   1: a layer step reads its counters from the device once
  *                (the candidate sort sized on the device); 0: twice;
  *   "batch_graph"  1: each batched layer step is captured and launched as
@@ -243,7 +249,8 @@ int cg_dev_sync(void);
  *                cell_code 1, carry_stream 1, canon_stab 1, sample_sort 1
  *                with 5000 keys per bucket, capacity 131072, 8192 buckets,
  *                probes 64, bucket_bitonic 0, distinct_bitonic 2048 with 9
- *                bits, prune_in_sort 1, one_sync 1, batch_graph 1;
+ *                bits, prune_in_sort 1, dir_deferred 1, carry_sort 0, one_sync 1,
+ *                batch_graph 1;
  *   "reset"      the state after initialisation: the built-in defaults, then
  *                the table loaded at initialisation if there was one.
  * Returns 0, or -1 for an unknown knob or value.  cg_tune_get also reads
@@ -283,11 +290,13 @@ int cg_tune_load(const char *path);
  * algorithmic bytes of each launch. */
 void cg_prof_enable(int on);
 void cg_prof_reset(void);
-/* kernel: "move", "spawn", "sort_hist", "sort_pass", "unique", "solve",
- * "adjoint", "lookup".  Returns the number of launches. */
+/* kernel: "move", "spawn", "sort_hist", "sort_pass", "unique", "sort_split",
+ * "sort_part", "sort_local", "solve", "adjoint", "lookup".  Returns the
+ * number of launches. */
 long cg_prof_query(const char *kernel, double *total_ms, double *total_bytes);
 /* times only the launches of one kernel (names as for cg_prof_query; NULL or
- * "" times every kernel; "sort" times every kernel of the sort stage): each
+ * "" times every kernel; "sort" times every kernel of the sort stage; a name
+ * followed by "@main" times those launches on the main stream only): each
  * timed launch records two events */
 int cg_prof_select(const char *kernel);
 /* the same for the launches of one stream: 0 the library's main stream, 1

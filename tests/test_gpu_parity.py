@@ -508,6 +508,8 @@ PRUNE_PATHS = {
     "oversize_repeat": [("sample_cap", 64)],
     "two_syncs": [("one_sync", 0)],
     "lsd_distinct": [("distinct_bitonic", 0)],
+    "carry_sorted": [("carry_sort", 1)],
+    "carry_sorted_lsd": [("carry_sort", 1), ("sample_sort", 0)],
 }
 
 
@@ -530,11 +532,42 @@ def test_tablebase_prune_paths(cg, name, path):
     assert cg.tune_get("prune_in_sort") == 1
 
 
+@pytest.mark.parametrize("name", ["c1", "3x4", "dpdgap_noprune", "two_initial", "free_formation"])
+@pytest.mark.parametrize("repeat", [False, True])
+def test_tablebase_carry_sorted(cg, name, repeat):
+    """The spawn-4 carry sorted and deduplicated before it is moved
+    (carry_sort 1) instead of moved as the spawn kernel wrote it (the
+    default): the pre-move sets are multisets either way and the layers are
+    the same; also through the spawn-buffer repeat."""
+    try:
+        cg.tune_set("carry_sort", 1)
+        if repeat:
+            cg.tune_set("spawn_estimate", 1)
+        test_tablebase_vs_oracle(cg, name)
+    finally:
+        cg.tune_set("reset", 0)
+    assert cg.tune_get("carry_sort") == 0
+
+
+@pytest.mark.parametrize("name", ["c1", "dpdgap_prune"])
+def test_tablebase_directories_per_step(cg, name):
+    """The layer directories built after each step (dir_deferred 0) instead
+    of after the last one: the same tables and values."""
+    try:
+        cg.tune_set("dir_deferred", 0)
+        assert cg.tune_get("dir_deferred") == 0
+        test_tablebase_vs_oracle(cg, name)
+    finally:
+        cg.tune_set("reset", 0)
+    assert cg.tune_get("dir_deferred") == 1
+
+
 @pytest.mark.parametrize("name", ["c1", "dpdgap_prune"])
 def test_tablebase_carry_sort_on_main_stream(cg, name):
     """The carry sort scheduled on the main stream (carry_stream 0) instead of
     the second one gives the same tables."""
     try:
+        cg.tune_set("carry_sort", 1)
         cg.tune_set("carry_stream", 0)
         assert cg.tune_get("carry_stream") == 0
         test_tablebase_vs_oracle(cg, name)
@@ -565,6 +598,7 @@ def test_tablebase_oversize_carry_on_main_stream(cg, name):
     their error words and both are redone on the LSD path."""
     try:
         cg.tune_set("sample_cap", 64)
+        cg.tune_set("carry_sort", 1)
         cg.tune_set("carry_stream", 0)
         test_tablebase_vs_oracle(cg, name)
     finally:
@@ -578,12 +612,14 @@ def test_lsd_generation_after_oversize(cg):
     overwritten) still matches the oracle."""
     try:
         cg.tune_set("sample_cap", 64)
+        cg.tune_set("carry_sort", 1)
         test_tablebase_vs_oracle(cg, "3x4")   # every sample sort overflows and is repeated
     finally:
         cg.tune_set("reset", 0)
     assert cg.tune_get("sample_repeats") > 0
     try:
         cg.tune_set("sample_sort", 0)
+        cg.tune_set("carry_sort", 1)
         test_tablebase_vs_oracle(cg, "c1")
         test_tablebase_vs_oracle(cg, "two_initial")
     finally:
@@ -914,28 +950,39 @@ def test_batch_spawn_buffer_repeat(cg):
     B.close()
 
 
-@pytest.mark.parametrize("mode", ["eager", "two_syncs", "carry_main"])
+BATCH_MODES = {
+    "eager": [("batch_graph", 0)],
+    "two_syncs": [("one_sync", 0)],
+    "carry_sorted": [("carry_sort", 1)],
+    "carry_sorted_main": [("carry_sort", 1), ("carry_stream", 0)],
+    "carry_sorted_eager": [("carry_sort", 1), ("batch_graph", 0)],
+    "carry_sorted_two_syncs": [("carry_sort", 1), ("one_sync", 0)],
+}
+
+
+@pytest.mark.parametrize("mode", list(BATCH_MODES))
 def test_batch_step_modes(cg, mode):
     """The batched step's launches enqueued one by one (batch_graph 0), with
-    the candidate count read before the M sort (one_sync 0), or with the
-    carry sort on the main stream inside the step's graph: the same layers
-    and values as each table alone."""
+    the candidate count read before the M sort (one_sync 0), with the
+    spawn-4 carry sorted before it is moved (carry_sort 1; on the second
+    stream, a branch of the step's graph, or on the main stream), and their
+    combinations: the same layers and values as each table alone."""
     from cablegen_amd.batch_configs import batch
     cfgs = batch(10, lo=3, hi=14)
     r = cg.rules(ignore_f=True)
     B = cg.Batch(r)
     for c in cfgs:
         B.add(c["initial"], c["start"], c["end"], c["winstates"])
-    knob = {"eager": ("batch_graph", 0), "two_syncs": ("one_sync", 0), "carry_main": ("carry_stream", 0)}[mode]
     try:
-        cg.tune_set(*knob)
+        for k, v in BATCH_MODES[mode]:
+            cg.tune_set(k, v)
         g0 = cg.tune_get("graph_steps")
         B.generate()
         launched = cg.tune_get("graph_steps") - g0
         B.solve(use_graph=False)
     finally:
         cg.tune_set("reset", 0)
-    assert (launched == 0) == (mode != "carry_main")
+    assert (launched == 0) == any(k in ("batch_graph", "one_sync") for k, _ in BATCH_MODES[mode])
     for t, c in enumerate(cfgs):
         tb = cg.Tablebase(r)
         tb.generate(c["initial"], c["start"], c["end"])
@@ -1103,13 +1150,15 @@ def test_sample_sort_oversize_repeats_on_lsd(cg):
 @pytest.mark.parametrize("name", ["c1", "dpdgap_prune"])
 def test_tablebase_sample_sort_repeat_path(cg, name):
     """Generation with every sample sort overflowing (capacity 64): each
-    step's M sort and carry sort are repeated on the LSD path and the table
-    still matches the oracle."""
-    try:
-        cg.tune_set("sample_cap", 64)
-        test_tablebase_vs_oracle(cg, name)
-    finally:
-        cg.tune_set("reset", 0)
+    step's M sort (and, with carry_sort 1, its carry sort) is repeated on the
+    LSD path and the table still matches the oracle."""
+    for carry in (0, 1):
+        try:
+            cg.tune_set("sample_cap", 64)
+            cg.tune_set("carry_sort", carry)
+            test_tablebase_vs_oracle(cg, name)
+        finally:
+            cg.tune_set("reset", 0)
 
 
 def test_batch_sample_sort_repeat_path(cg):
@@ -1119,18 +1168,23 @@ def test_batch_sample_sort_repeat_path(cg):
     B = cg.Batch(r)
     for c in cfgs:
         B.add(c["initial"], c["start"], c["end"], c["winstates"])
-    try:
-        cg.tune_set("sample_cap", 64)
-        B.generate()
-        B.solve(use_graph=False)
-    finally:
-        cg.tune_set("reset", 0)
-    for t, c in enumerate(cfgs):
+    singles = []
+    for c in cfgs:
         tb = cg.Tablebase(r)
         tb.generate(c["initial"], c["start"], c["end"])
         tb.solve(c["winstates"])
-        for i in range(tb.num_layers):
-            assert np.array_equal(B.keys(t, i), tb.keys(i)), (t, i)
-            assert np.array_equal(B.values(t, i).view(np.uint64), tb.values(i).view(np.uint64))
+        singles.append([(tb.keys(i), tb.values(i).view(np.uint64)) for i in range(tb.num_layers)])
         tb.close()
+    for carry in (0, 1):
+        try:
+            cg.tune_set("sample_cap", 64)
+            cg.tune_set("carry_sort", carry)
+            B.generate()
+            B.solve(use_graph=False)
+        finally:
+            cg.tune_set("reset", 0)
+        for t, layers in enumerate(singles):
+            for i, (k, v) in enumerate(layers):
+                assert np.array_equal(B.keys(t, i), k), (carry, t, i)
+                assert np.array_equal(B.values(t, i).view(np.uint64), v), (carry, t, i)
     B.close()
