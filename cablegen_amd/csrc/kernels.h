@@ -154,14 +154,19 @@ void launch_count_empties(hipStream_t st, const uint64_t *in, size_t n, unsigned
  * count above cap means the buffers were too small and the launch must be
  * repeated.  With kp.on, a 2-spawn that fails the prune test is written to
  * out2 as 0, a board with no tile, which has no legal move: the 2-spawns
- * are only moved, so they need no prune test of their own; with prune4 too,
- * likewise a 4-spawn to out4 (for 4-spawns that are moved unsorted; a sort
- * of them drops such boards itself, KeyPrune, where zeros would pile up in
- * one bucket).  The test depends only on the multiset of cells, so it is
+ * are only moved, so they need no prune test of their own.  unsorted4: the
+ * 4-spawns are moved unsorted too, so with kp.on a failing one is written
+ * to out4 as 0 as well (a sort of them drops such boards itself, KeyPrune,
+ * where zeros would pile up in one bucket), and their OR / AND is not
+ * collected.  The test depends only on the multiset of cells, so it is
  * taken once per board and value, not once per empty cell. */
 void launch_spawn(hipStream_t st, const uint64_t *in, size_t n, const unsigned long long *dn,
                   uint64_t *out2, uint64_t *out4, size_t cap, unsigned long long *stats,
-                  cg_canon_sel_t cs = cg_canon_all(), KeyPrune kp = KeyPrune(), int prune4 = 0);
+                  cg_canon_sel_t cs = cg_canon_all(), KeyPrune kp = KeyPrune(), int unsorted4 = 0);
+/* blocks of a spawn launch whose 4-spawns are moved unsorted (no OR / AND
+ * atomics at the end of each block; default 1024; 512 otherwise) */
+void spawn_set_blocks(int b);
+int spawn_blocks();
 
 
 /* Sorts keys[0..n) ascending and writes the unique keys to dst (capacity n),

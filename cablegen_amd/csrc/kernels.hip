@@ -278,7 +278,7 @@ __global__ __launch_bounds__(SP_THREADS) void k_spawn(const uint64_t *__restrict
                                                       uint64_t *__restrict__ out2,
                                                       uint64_t *__restrict__ out4, size_t cap,
                                                       unsigned long long *stats, cg_canon_sel_t cs,
-                                                      KeyPrune kp, int prune4)
+                                                      KeyPrune kp, int unsorted4)
 {
     __shared__ SpawnPairs sp;
     __shared__ unsigned long long base_sh;
@@ -309,7 +309,7 @@ __global__ __launch_bounds__(SP_THREADS) void k_spawn(const uint64_t *__restrict
             const uint64_t lb = z & (0ULL - z);
             uint32_t f = 0;
             if (z && cg_prune(b | lb, kp.stsl, kp.ltc, kp.sl)) f |= 1u;
-            if (z && prune4 && cg_prune(b | (lb << 1), kp.stsl, kp.ltc, kp.sl)) f |= 2u;
+            if (z && unsorted4 && cg_prune(b | (lb << 1), kp.stsl, kp.ltc, kp.sl)) f |= 2u;
             sp.pf[threadIdx.x] = (uint8_t)f;
         }
         uint32_t tot;
@@ -342,12 +342,19 @@ __global__ __launch_bounds__(SP_THREADS) void k_spawn(const uint64_t *__restrict
         base += tot;
         __syncthreads();
     }
-    block_or_and(or4, and4, &stats[3], &stats[4], ro, ra);
+    /* the OR/AND of the 4-spawns: the key bits of their sort */
+    if (!unsorted4) block_or_and(or4, and4, &stats[3], &stats[4], ro, ra);
 }
+
+/* with no OR/AND atomics, more blocks: C2 spawn 25.2 ms per table with 512
+ * blocks, 19.8 with 1024, 20.3 with 2048, 21.8 with 4096 */
+static int g_spawn_blocks = 1024;
+void spawn_set_blocks(int b) { g_spawn_blocks = b > 0 ? b : 1024; }
+int spawn_blocks() { return g_spawn_blocks; }
 
 void launch_spawn(hipStream_t st, const uint64_t *in, size_t n, const unsigned long long *dn,
                   uint64_t *out2, uint64_t *out4, size_t cap, unsigned long long *stats, cg_canon_sel_t cs,
-                  KeyPrune kp, int prune4)
+                  KeyPrune kp, int unsorted4)
 {
     if (!n) return;
     size_t blocks = (n + SP_THREADS - 1) / SP_THREADS;
@@ -355,9 +362,10 @@ void launch_spawn(hipStream_t st, const uint64_t *in, size_t n, const unsigned l
      * kernel (with two OR/AND pairs per block, C2 spawn took 22.4 ms per step
      * with 512 blocks, 25.1 with 1024, 38.7 with 2048, 29.7 with 256; with
      * the one pair now, 19.1 with 512 and 18.8 with 1024) */
-    if (blocks > 512) blocks = 512;
+    const size_t cap_blocks = unsorted4 ? (size_t)g_spawn_blocks : 512;
+    if (blocks > cap_blocks) blocks = cap_blocks;
     hipLaunchKernelGGL(k_spawn, dim3((unsigned)blocks), dim3(SP_THREADS), 0, st, in, n, dn, out2,
-                       out4, cap, stats, cs, kp, prune4);
+                       out4, cap, stats, cs, kp, unsorted4);
 }
 
 /* ------------------------------------------------------------------------ */

@@ -205,6 +205,8 @@ int cg_dev_sync(void);
  *   "carry_sort" 0: a layer step's spawn-4 boards are moved as written
  *                (duplicates included) two steps later; 1: they are sorted
  *                and deduplicated first (the carry sort);
+ *   "spawn_blocks"  blocks of a spawn launch whose 4-spawns are moved
+ *                unsorted (64..65536, default 1024);
  *   "carry_stream"  with carry_sort 1 (and in batches): 1: the carry sort
  *                runs on a second stream beside the next step's moves; 0: on
  *                the main stream after the spawns;
@@ -249,7 +251,8 @@ int cg_dev_sync(void);
  *                cell_code 1, carry_stream 1, canon_stab 1, sample_sort 1
  *                with 5000 keys per bucket, capacity 131072, 8192 buckets,
  *                probes 64, bucket_bitonic 0, distinct_bitonic 2048 with 9
- *                bits, prune_in_sort 1, dir_deferred 1, carry_sort 0, one_sync 1,
+ *                bits, prune_in_sort 1, dir_deferred 1, carry_sort 0, spawn_blocks
+ *                1024, one_sync 1,
  *                batch_graph 1;
  *   "reset"      the state after initialisation: the built-in defaults, then
  *                the table loaded at initialisation if there was one.
