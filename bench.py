@@ -902,7 +902,11 @@ def main():
     value = total / T
 
     roof = stage_roofline(cg, dom, args.steps)
-    roof["second_stream"] = second
+    roof.pop("second_stream", None)
+    if second["units"]:
+        roof["second_stream"] = second
+    else:
+        roof["stream"] = "main (no sort runs on the second stream: the carry is moved unsorted)"
     if dom == "sort" and args.carry_sort == 1 and args.carry_stream == 1 and not args.no_isolated:
         # the same stage with nothing beside it: one untimed step with the
         # carry sort on the main stream, so no kernel of another stream
