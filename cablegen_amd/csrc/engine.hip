@@ -1741,6 +1741,11 @@ extern "C" int cg_tune_set(const char *knob, int value)
         sort_set_bucket_bitonic(value);
         return 0;
     }
+    if (!strcmp(knob, "solve_waves")) {
+        if (value != 0 && value != 5 && value != 6) return fail_msg("solve_waves: 0, 5 or 6");
+        solve_set_waves(value);
+        return 0;
+    }
     if (!strcmp(knob, "spawn_blocks")) {
         if (value < 64 || value > 65536) return fail_msg("spawn_blocks: 64..65536");
         spawn_set_blocks(value);
@@ -1809,6 +1814,7 @@ extern "C" int cg_tune_set(const char *knob, int value)
         g_dir_deferred = 1;
         g_carry_sort = 0;
         spawn_set_blocks(1024);
+        solve_set_waves(5);
         g_batch_graph = 1;
         sort_set_sample(1, 5000, 0);
         sort_set_sample_limits(8192, 0);
@@ -1836,6 +1842,7 @@ extern "C" int cg_tune_get(const char *knob)
     if (!strcmp(knob, "dir_deferred")) return g_dir_deferred;
     if (!strcmp(knob, "carry_sort")) return g_carry_sort;
     if (!strcmp(knob, "spawn_blocks")) return spawn_blocks();
+    if (!strcmp(knob, "solve_waves")) return solve_waves();
     if (!strcmp(knob, "batch_graph")) return g_batch_graph;
     if (!strcmp(knob, "sample_bmax")) return sort_sample_bmax();
     if (!strcmp(knob, "bucket_bitonic")) return sort_bucket_bitonic();

@@ -302,6 +302,10 @@ void launch_or_and(hipStream_t st, const uint64_t *k, size_t n, unsigned long lo
 
 
 /* solve one layer: V (and optionally the 2-bit argmax word per board) */
+/* the solve kernel's registers fitted to 5 (the default) or 6 waves per
+ * SIMD (0: the compiler's choice, 4 waves) */
+void solve_set_waves(int w);
+int solve_waves();
 void launch_solve(hipStream_t st, const uint64_t *M, size_t n, double *V, uint64_t *argmax,
                   LayerView t2, LayerView t4, WinRules w, double p4, const uint16_t *lut, int ff,
                   unsigned long long *stats);

@@ -1028,7 +1028,10 @@ __device__ __forceinline__ void add_stats(unsigned long long *stats, uint32_t nl
     }
 }
 
-__global__ __launch_bounds__(SV_THREADS) void k_solve(const uint64_t *__restrict__ M, size_t n,
+/* MW: the waves per SIMD the registers are fitted to (0: the compiler's
+ * choice, 4 waves at 111 VGPRs) */
+template <int MW>
+__global__ __launch_bounds__(SV_THREADS, MW > 0 ? MW : 1) void k_solve(const uint64_t *__restrict__ M, size_t n,
                                                       double *__restrict__ V,
                                                       uint64_t *__restrict__ argmax, LayerView t2,
                                                       LayerView t4, WinRules win, double p4,
@@ -1045,6 +1048,10 @@ __global__ __launch_bounds__(SV_THREADS) void k_solve(const uint64_t *__restrict
     add_stats(stats, nlook, nhit, red);
 }
 
+/* C2 solve 75.7 ms at the compiler's 111 VGPRs (4 waves per SIMD), 73.6 ms
+ * fitted to 5 waves (96 VGPRs), 83.5 ms at 6 (80 VGPRs, 40 spilled) */
+static int g_solve_waves = 5;
+
 void launch_solve(hipStream_t st, const uint64_t *M, size_t n, double *V, uint64_t *argmax,
                   LayerView t2, LayerView t4, WinRules w, double p4, const uint16_t *lut, int ff,
                   unsigned long long *stats)
@@ -1054,9 +1061,19 @@ void launch_solve(hipStream_t st, const uint64_t *M, size_t n, double *V, uint64
      * with 2048, 79.3 with 4096, 75.9 with 8192 and 77.3 with 16384 or more */
     size_t tiles = (n + SV_THREADS - 1) / SV_THREADS;
     size_t blocks = tiles < 8192 ? tiles : 8192;
-    hipLaunchKernelGGL(k_solve, dim3((unsigned)blocks), dim3(SV_THREADS), 0, st, M, n, V, argmax, t2,
-                       t4, w, p4, lut, ff, stats);
+    if (g_solve_waves == 5)
+        hipLaunchKernelGGL(k_solve<5>, dim3((unsigned)blocks), dim3(SV_THREADS), 0, st, M, n, V, argmax, t2,
+                           t4, w, p4, lut, ff, stats);
+    else if (g_solve_waves == 6)
+        hipLaunchKernelGGL(k_solve<6>, dim3((unsigned)blocks), dim3(SV_THREADS), 0, st, M, n, V, argmax, t2,
+                           t4, w, p4, lut, ff, stats);
+    else
+        hipLaunchKernelGGL(k_solve<0>, dim3((unsigned)blocks), dim3(SV_THREADS), 0, st, M, n, V, argmax, t2,
+                           t4, w, p4, lut, ff, stats);
 }
+
+void solve_set_waves(int w) { g_solve_waves = (w == 5 || w == 6) ? w : 0; }
+int solve_waves() { return g_solve_waves; }
 
 /* ------------------------------------------------------------------------ */
 /* adjoint                                                                    */

@@ -549,6 +549,20 @@ def test_tablebase_carry_sorted(cg, name, repeat):
     assert cg.tune_get("carry_sort") == 0
 
 
+@pytest.mark.parametrize("waves", [0, 6])
+def test_tablebase_solve_waves(cg, waves):
+    """The solve kernel with its registers fitted to 6 waves per SIMD (some
+    spilled) or left to the compiler (4 waves), against the oracle; every
+    other test runs the default, 5 waves."""
+    try:
+        cg.tune_set("solve_waves", waves)
+        test_tablebase_vs_oracle(cg, "dpdgap_prune")
+        test_tablebase_vs_oracle(cg, "score")
+    finally:
+        cg.tune_set("reset", 0)
+    assert cg.tune_get("solve_waves") == 5
+
+
 @pytest.mark.parametrize("name", ["c1", "dpdgap_prune"])
 def test_tablebase_directories_per_step(cg, name):
     """The layer directories built after each step (dir_deferred 0) instead
