@@ -15,8 +15,9 @@ Also reported:
                 untimed step with every kernel timed, then timed in the timed
                 steps with HIP events on the stream each launch runs on.
                 achieved = algorithmic bytes / time over its main-stream units
-                (the carry sorts, on the second stream, share the GPU with the
-                next step's moves and are reported beside it).  Algorithmic
+                (with --carry-sort 1 the carry sorts, on the second stream,
+                share the GPU with the next step's moves and are reported
+                beside it).  Algorithmic
                 bytes: the payload read once and written once -- a sort
                 8 B per key + 8 B per unique key (extra radix passes are
                 implementation overhead), the move kernel 8 B per board + 8 B
@@ -272,7 +273,7 @@ PMC_KERNELS = {"move": "::k_move", "spawn": "::k_spawn(", "sort_hist": "::k_os_h
                "sort_split:gather": "::k_ss_gather(", "sort_split:runs": "::k_ss_runs(",
                "sort_split:select": "::k_ss_select(", "sort_split:rank": "::k_ss_rank(",
                "sort_split:pick": "::k_ss_pick(",
-               "sort_part": "::k_ss_part", "sort_local": "::k_ss_local(", "solve": "::k_solve(",
+               "sort_part": "::k_ss_part", "sort_local": "::k_ss_local(", "solve": "::k_solve<",
                "lookup": "::k_lookup(", "adjoint": "::k_adjoint("}
 # SQ counter sets (one rocprofv3 run each) and the ratios printed from them:
 # per-wave cycles issuing VALU / LDS / VMEM, waiting on LDS or on anything,
@@ -327,7 +328,7 @@ def prof_summary(args):
                               ("sort_split", "::k_ss_select"), ("sort_part", "::k_ss_part"),
                               ("sort_local", "::k_ss_local"), ("sort_hist", "::k_os_hist"),
                               ("sort_pass", "::k_os_pass"), ("unique", "::k_os_unique"),
-                              ("move", "::k_move_gen"), ("spawn", "::k_spawn("), ("solve", "::k_solve(")):
+                              ("move", "::k_move_gen"), ("spawn", "::k_spawn("), ("solve", "::k_solve<")):
                 name = "%s (%s)" % (name, pat.strip(":(")) if name.startswith("sort_split") else name
                 acc = {True: [], False: []}
                 for r in rows:

@@ -171,6 +171,10 @@ __device__ __forceinline__ void hash_set_flush(const unsigned long long *ht, uin
  * appended with one output reservation per round (hash_set_flush). */
 constexpr int MG_BPT = 2;   /* boards per thread per round */
 
+/* PRUNE: some prune test runs here (a generation's first step, or with the
+ * tests left in the move kernel); without it the kernel holds no prune code
+ * and its parameters take no registers */
+template <bool PRUNE>
 __global__ __launch_bounds__(MV_THREADS, 8) void k_move_gen(
     const uint64_t *__restrict__ in, size_t n, const unsigned long long *dn,
     uint64_t *__restrict__ out, unsigned long long *stats, const uint16_t *__restrict__ lut,
@@ -192,13 +196,13 @@ __global__ __launch_bounds__(MV_THREADS, 8) void k_move_gen(
             const size_t i = b0 + (size_t)k * MV_THREADS + threadIdx.x;
             if (i >= n) continue;
             const uint64_t b = in[i];
-            if (r.prune && r.prune_inputs && cg_prune(b, r.stsl, r.ltc, r.sl)) continue;
+            if (PRUNE && r.prune_inputs && cg_prune(b, r.stsl, r.ltc, r.sl)) continue;
             const uint64_t y = cg_rot_cw(b);
 #pragma unroll
             for (int d = 0; d < 4; d++) {
                 uint64_t t = d < 2 ? b : y;
                 if (cg_move_hori_tab(lut, &t, d == 1 || d == 2, r.ff) &&
-                    !(r.prune && r.prune_moves && cg_prune(t, r.stsl, r.ltc, r.sl)))
+                    !(PRUNE && r.prune_moves && cg_prune(t, r.stsl, r.ltc, r.sl)))
                     mv_first(ht, cg_canon_fast(t, r.cs, d >= 2));
             }
         }
@@ -215,8 +219,11 @@ void launch_move(hipStream_t st, const uint64_t *in, size_t n, const unsigned lo
     if (!n) return;
     size_t blocks = (n + MV_THREADS - 1) / MV_THREADS;
     if (blocks > 512) blocks = 512;
-    if (dedupe)
-        hipLaunchKernelGGL(k_move_gen, dim3((unsigned)blocks), dim3(MV_THREADS), 0, st, in, n, dn,
+    if (dedupe && r.prune && (r.prune_inputs || r.prune_moves))
+        hipLaunchKernelGGL(k_move_gen<true>, dim3((unsigned)blocks), dim3(MV_THREADS), 0, st, in, n, dn,
+                           out, stats, lut, r);
+    else if (dedupe)
+        hipLaunchKernelGGL(k_move_gen<false>, dim3((unsigned)blocks), dim3(MV_THREADS), 0, st, in, n, dn,
                            out, stats, lut, r);
     else
         hipLaunchKernelGGL(k_move, dim3((unsigned)blocks), dim3(MV_THREADS), 0, st, in, n, dn,
