@@ -1092,6 +1092,10 @@ struct TbLayer {
     uint64_t *arg = nullptr;
     uint32_t *dir = nullptr;
     LayerIndexPlan plan;
+    /* the arena blocks the values and argmax words use, kept for the next
+     * solve of the same generation (a re-solve allocates nothing) */
+    double *vals_buf = nullptr;
+    uint64_t *arg_buf = nullptr;
 };
 
 static LayerView make_view(const uint64_t *keys, const double *vals, size_t n, const uint32_t *dir,
@@ -1320,9 +1324,11 @@ extern "C" int cg_tb_solve(cg_tablebase *tb, const uint64_t *winstates, size_t n
         int s = tb->start + 2 * i;
         if (s < end_solve || s <= 0) break;
         TbLayer &l = tb->L[i];
-        l.vals = (double *)tb->arena.alloc(std::max<size_t>(l.n, 1) * 8);
-        if (keep_argmax) l.arg = (uint64_t *)tb->arena.alloc(std::max<size_t>(l.n, 1) * 8);
-        if (!l.vals || (keep_argmax && !l.arg)) return fail_msg("value allocation failed");
+        if (!l.vals_buf) l.vals_buf = (double *)tb->arena.alloc(std::max<size_t>(l.n, 1) * 8);
+        if (keep_argmax && !l.arg_buf) l.arg_buf = (uint64_t *)tb->arena.alloc(std::max<size_t>(l.n, 1) * 8);
+        if (!l.vals_buf || (keep_argmax && !l.arg_buf)) return fail_msg("value allocation failed");
+        l.vals = l.vals_buf;
+        if (keep_argmax) l.arg = l.arg_buf;
         if (solve_layer_dev(tb->r, l.keys, l.n, l.vals, l.arg, view_of(tb, i + 1), view_of(tb, i + 2),
                             w, p4))
             return -1;
