@@ -489,11 +489,23 @@ TB_CASES = {
 }
 
 
+# the oracle's tables by case name: many tests run the same case with
+# different knobs, and the oracle's answer does not depend on them
+_ORACLE_TABLES = {}
+
+
+def oracle_case(name):
+    if name not in _ORACLE_TABLES:
+        rules, initial, win, start, end = TB_CASES[name]
+        cfg = O.make_cfg(cores=8, **rules)
+        _ORACLE_TABLES[name] = oracle_table(cfg, initial, win, start, end)
+    return _ORACLE_TABLES[name]
+
+
 @pytest.mark.parametrize("name", list(TB_CASES))
 def test_tablebase_vs_oracle(cg, name):
     rules, initial, win, start, end = TB_CASES[name]
-    cfg = O.make_cfg(cores=8, **rules)
-    layers, vals = oracle_table(cfg, initial, win, start, end)
+    layers, vals = oracle_case(name)
     tb = cg.Tablebase(cg.rules(**rules))
     tb.generate(initial, start, end)
     tb.solve(win)
