@@ -889,7 +889,9 @@ static int gen_step(const cg_rules &r, const uint64_t *P, size_t nP, uint64_t *C
         kps.sl = r.smallest_large;
         if (!premoved) kp = kps;
     }
-    launch_init_stats(E.st, E.d_cnt);
+    /* the move's and the spawns' statistics, both cleared now (the spawns'
+     * again only when they are repeated) */
+    launch_init_stats(E.st, E.d_cnt, E.d_cnt + 8);
     int h = prof_begin(PK_MOVE);
     if (premoved) {
         /* premove: the boards of P are the layer's positions -- their
@@ -956,7 +958,7 @@ static int gen_step(const cg_rules &r, const uint64_t *P, size_t nP, uint64_t *C
     bool settled = false;
     for (int attempt = 0; attempt < 4 && !settled; attempt++) {
         if (ensure(nxtP, cap + cbound + 1) || ensure(S4, cap + 1)) return -1;
-        launch_init_stats(E.st, E.d_cnt + 8);
+        if (attempt > 0) launch_init_stats(E.st, E.d_cnt + 8);
         hs = prof_begin(PK_SPAWN);
         launch_spawn(E.st, *M, mrows, E.d_cnt + 7, nxtP.p, S4.p, cap, E.d_cnt + 8, cs, kps,
                      pc.carry_sorted ? 0 : 1);

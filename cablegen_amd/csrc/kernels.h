@@ -295,8 +295,9 @@ void launch_publish(hipStream_t st, const unsigned long long *cnt, int lo, int n
                     unsigned long long *hpub, int err_at);
 /* dst[i] = src[i] & mask */
 void launch_copy_and_u64(hipStream_t st, uint64_t *dst, const uint64_t *src, size_t n, uint64_t mask);
-/* stats[0] = 0 (count), stats[1,3] = 0 (OR), stats[2,4] = ~0 (AND) */
-void launch_init_stats(hipStream_t st, unsigned long long *stats);
+/* stats[0] = 0 (count), stats[1,3] = 0 (OR), stats[2,4] = ~0 (AND); the
+ * same for stats2 when given (two producers' statistics, one launch) */
+void launch_init_stats(hipStream_t st, unsigned long long *stats, unsigned long long *stats2 = nullptr);
 /* OR / AND of keys into stats[1] / stats[2] */
 void launch_or_and(hipStream_t st, const uint64_t *k, size_t n, unsigned long long *stats);
 

@@ -440,15 +440,19 @@ void launch_publish(hipStream_t st, const unsigned long long *cnt, int lo, int n
 }
 
 /* producer statistics: [0] count, then (OR, AND) pairs; AND starts all-ones */
-__global__ void k_init_stats(unsigned long long *s)
+__global__ void k_init_stats(unsigned long long *s, unsigned long long *s2)
 {
     /* slots 5..7 hold sort counts that later launches read on the device */
     if (threadIdx.x < 5) s[threadIdx.x] = (threadIdx.x == 2 || threadIdx.x == 4) ? ~0ULL : 0ULL;
+    if (s2 && threadIdx.x >= 32 && threadIdx.x < 37) {
+        const int i = threadIdx.x - 32;
+        s2[i] = (i == 2 || i == 4) ? ~0ULL : 0ULL;
+    }
 }
 
-void launch_init_stats(hipStream_t st, unsigned long long *stats)
+void launch_init_stats(hipStream_t st, unsigned long long *stats, unsigned long long *stats2)
 {
-    hipLaunchKernelGGL(k_init_stats, dim3(1), dim3(64), 0, st, stats);
+    hipLaunchKernelGGL(k_init_stats, dim3(1), dim3(64), 0, st, stats, stats2);
 }
 
 /* OR / AND of a key array into stats[1], stats[2] */
