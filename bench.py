@@ -451,10 +451,15 @@ def run_autotune(cg, args):
         cg.tune_set("one_sync", args.one_sync)
         if args.sample_bucket:
             cg.tune_set("sample_bucket", args.sample_bucket)
+        if args.sample_bmax:
+            cg.tune_set("sample_bmax", args.sample_bmax)
+        knobs = set_knobs(cg, args)
         return {"source": "built-in defaults: 8-bit digits, 8192-key tiles, window 16, dir delta 0",
+                "knobs": knobs,
                 "cell_code": args.cell_code, "carry_stream": args.carry_stream,
                 "carry_sort": args.carry_sort,
                 "sample_sort": args.sample_sort, "sample_bucket": cg.tune_get("sample_bucket"),
+                "sample_bmax": cg.tune_get("sample_bmax"),
                 "canon_stab": args.canon_stab, "one_sync": args.one_sync}
     if mode == "stored" and os.path.exists(cg.TUNED_TABLE):
         rep = cg.load_tuning()
@@ -473,6 +478,9 @@ def run_autotune(cg, args):
     cg.tune_set("one_sync", args.one_sync)
     if args.sample_bucket:
         cg.tune_set("sample_bucket", args.sample_bucket)
+    if args.sample_bmax:
+        cg.tune_set("sample_bmax", args.sample_bmax)
+    knobs = set_knobs(cg, args)
     dir_delta = rep["dir"]["pick"]
     if args.dir_delta is not None:
         dir_delta = args.dir_delta
@@ -484,8 +492,18 @@ def run_autotune(cg, args):
             "sort_picks_bits_x_kpt/window": sel, "cell_code": args.cell_code,
             "carry_stream": args.carry_stream, "carry_sort": args.carry_sort,
             "sample_sort": args.sample_sort,
-            "sample_bucket": cg.tune_get("sample_bucket"), "canon_stab": args.canon_stab,
-            "one_sync": args.one_sync}
+            "sample_bucket": cg.tune_get("sample_bucket"), "sample_bmax": cg.tune_get("sample_bmax"),
+            "canon_stab": args.canon_stab, "one_sync": args.one_sync, "knobs": knobs}
+
+
+def set_knobs(cg, args):
+    """Applies each --set KNOB=VALUE (cg_tune_set) and returns them."""
+    out = {}
+    for kv in args.set or []:
+        k, v = kv.split("=", 1)
+        cg.tune_set(k, int(v))
+        out[k] = cg.tune_get(k)
+    return out
 
 
 def census(args):
@@ -823,6 +841,10 @@ def main():
                     help="1: mid-size sorts take the sample sort (default); 0: LSD radix only")
     ap.add_argument("--sample-bucket", type=int, default=0,
                     help="mean keys per bucket of the sample sort (0: the library's default)")
+    ap.add_argument("--set", action="append", metavar="KNOB=VALUE",
+                    help="set a library knob (cg_tune_set) after the tuning table; repeatable")
+    ap.add_argument("--sample-bmax", type=int, choices=(2048, 8192, 16384), default=0,
+                    help="most buckets of a sample sort (0: the library's default)")
     ap.add_argument("--cell-code", type=int, choices=(0, 1), default=1,
                     help="1: layer sorts code pairs of cells when the tile sum bounds them (default)")
     ap.add_argument("--one-sync", type=int, choices=(0, 1), default=1,

@@ -1740,8 +1740,13 @@ extern "C" int cg_tune_set(const char *knob, int value)
         return 0;
     }
     if (!strcmp(knob, "sample_bmax")) {
-        if (value != 2048 && value != 8192) return fail_msg("sample_bmax: 2048 or 8192");
+        if (value != 2048 && value != 8192 && value != 16384) return fail_msg("sample_bmax: 2048, 8192 or 16384");
         sort_set_sample_limits(value, sort_sample_probes());
+        return 0;
+    }
+    if (!strcmp(knob, "part_blocks")) {
+        if (value < 0 || value > (1 << 20)) return fail_msg("part_blocks: 0..2^20");
+        sort_set_part_blocks(value);
         return 0;
     }
     if (!strcmp(knob, "bucket_bitonic")) {
@@ -1825,8 +1830,9 @@ extern "C" int cg_tune_set(const char *knob, int value)
         solve_set_waves(5);
         g_batch_graph = 1;
         sort_set_sample(1, 5000, 0);
-        sort_set_sample_limits(8192, 0);
+        sort_set_sample_limits(16384, 0);
         sort_set_bucket_bitonic(0);
+        sort_set_part_blocks(512);
         sort_set_distinct_bitonic(2048, 9);
         if (!strcmp(knob, "reset") && !g_tune_path.empty()) return cg_tune_load(g_tune_path.c_str());
         return 0;
@@ -1854,6 +1860,7 @@ extern "C" int cg_tune_get(const char *knob)
     if (!strcmp(knob, "batch_graph")) return g_batch_graph;
     if (!strcmp(knob, "sample_bmax")) return sort_sample_bmax();
     if (!strcmp(knob, "bucket_bitonic")) return sort_bucket_bitonic();
+    if (!strcmp(knob, "part_blocks")) return sort_part_blocks();
     if (!strcmp(knob, "distinct_bitonic")) return sort_distinct_bitonic_max();
     if (!strcmp(knob, "distinct_bitonic_bits")) return sort_distinct_bitonic_bits();
     if (!strcmp(knob, "sample_probes")) return sort_sample_probes();
@@ -2650,9 +2657,9 @@ extern "C" int cg_batch_generate(cg_batch *b)
         }
         if (dev) {
             const size_t pk = sort_part_keys(cmax, SORT_SAMPLE);
-            /* look-back words for up to 8192 buckets (the sample sort's
-             * largest), epochs for its launch */
-            if ((pk && ensure(E.Pt, pk)) || sort_reserve(E.st, 0, 8192, 4)) return -1;
+            /* look-back words for the sample sort's most buckets, epochs
+             * for its launch */
+            if ((pk && ensure(E.Pt, pk)) || sort_reserve(E.st, 0, (size_t)sort_sample_bmax(), 4)) return -1;
         }
         if (df.carry) {
             /* the carry sort's buffers and look-back words (second stream) */

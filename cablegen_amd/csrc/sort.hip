@@ -535,7 +535,8 @@ constexpr int SS_KS32 = 16;                 /* rows per thread of a bucket's sor
 constexpr int SS_KS64 = 8;                  /* 64-bit values: 8192 */
 constexpr int SS_PK = 8;                    /* keys per thread of a partition tile */
 constexpr int SS_TILE = SS_T * SS_PK;       /* 8192 */
-constexpr int SS_BMAX = 8192;               /* buckets */
+constexpr int SS_BMAX = 16384;              /* buckets */
+constexpr int SS_BMID = 8192;               /* buckets of the partition with 64-bit splitters in LDS */
 constexpr int SS_BSMALL = 2048;             /* buckets of the partition with two blocks per CU */
 /* A bucket's keys land in SS_NX regions, one per group of partition blocks
  * (blockIdx % SS_NX, which on MI355X is the XCD the block runs on): SS_NX
@@ -551,7 +552,7 @@ static_assert((uint64_t)SS_BMAX * SS_BSTRIDE < 0xffffffffULL, "places in part ar
  * p = 0..SS_GUIDE, which narrows a key's splitter search to a few entries */
 constexpr int SS_GUIDE = 4096;
 constexpr int SS_S = SS_LCAP;               /* samples up to SS_BSMALL buckets (8 per bucket) */
-constexpr int SS_SMAX = 4 * SS_S;            /* samples of the largest sorts */
+constexpr int SS_SMAX = 8 * SS_S;            /* samples of the largest sorts */
 /* keys a bucket's regions hold; the default cap on a bucket's keys (only its
  * distinct values must fit the LDS) */
 constexpr int SS_KEYCAP = SS_NX * SS_SUBCAP;
@@ -909,9 +910,16 @@ __global__ __launch_bounds__(SS_T) void k_ss_pick(const uint32_t *__restrict__ r
     }
 }
 
+/* Splitters as the partition keeps them in LDS: 64-bit keys up to SS_BMID
+ * buckets; above that their sample values e = (splitter - min) >> z, 32 bits
+ * and exact (every splitter is min + (e << z)), so 16384 splitters, their
+ * counters and the guide fit 136 KiB. */
+template <int BM> struct SsSplT { using type = uint64_t; };
+template <> struct SsSplT<SS_BMAX> { using type = uint32_t; };
+
 template <int BM>
 struct SsPartLds {
-    uint64_t spl[BM];
+    typename SsSplT<BM>::type spl[BM];
     uint32_t cnt[BM];        /* counts, then each bucket's place in part (~0u: region full) */
     uint16_t guide[SS_GUIDE + 1];
 };
@@ -920,9 +928,14 @@ struct SsPartLds {
  * guide's bounds) and its rank among the tile's keys of that bucket (an LDS
  * atomic); one reservation per (tile, bucket) on the bucket's counter of
  * this block's region group; every key written from registers to part[b *
- * SS_BSTRIDE + group * SS_SUBCAP + reservation + rank].  Without a staging
- * tile the LDS holds only the splitters, guide and counters, so two blocks
- * share a CU; a tile's run of one bucket is a few keys either way.  A tile
+ * SS_BSTRIDE + group * SS_SUBCAP + reservation + rank].  A block loads the
+ * splitters and builds the guide once, then strides over the tiles (a grid
+ * of fewer blocks than tiles); its group stays blockIdx % SS_NX, so with a
+ * grid that is a multiple of SS_NX tile t writes group t % SS_NX.  Without
+ * a staging tile the LDS holds only the splitters, guide and counters, so
+ * two blocks share a CU up to SS_BSMALL buckets (one block above it: 104
+ * KiB at SS_BMID, 136 KiB at SS_BMAX with 32-bit splitters); a tile's run
+ * of one bucket is a few keys either way.  A tile
  * whose reservation would pass the region's end writes none of that
  * bucket's keys (the bucket is then oversize). */
 template <int BM>
@@ -936,16 +949,21 @@ __global__ __launch_bounds__(SS_T, BM <= SS_BSMALL ? 2 : 1) void k_ss_part(const
     __shared__ SsPartLds<BM> P;
     const size_t n = ss_count(dn, nmax);
     const int B = (int)ss_nbuckets(n, sh);
-    /* the grid covers nmax keys: blocks past n (and every block of a
-     * one-block sort) have nothing to do */
-    if (B <= 1 || (size_t)blockIdx.x * SS_TILE >= n) return;
+    const size_t ntiles = (n + SS_TILE - 1) / SS_TILE;
+    /* the grid covers nmax keys or fewer blocks that stride over the tiles:
+     * blocks past n's tiles (and every block of a one-block sort) have
+     * nothing to do */
+    if (B <= 1 || (size_t)blockIdx.x >= ntiles) return;
     const int nspl = B - 1;
     const int grp = blockIdx.x % SS_NX;
-    for (int i = threadIdx.x; i < nspl; i += SS_T) P.spl[i] = spl_g[i];
-    for (int i = threadIdx.x; i < B; i += SS_T) P.cnt[i] = 0;
     uint64_t smin;
     int nbs, zs;
     ss_sample_range(mnx, &smin, &nbs, &zs);
+    constexpr bool S32 = sizeof(P.spl[0]) == 4;
+    for (int i = threadIdx.x; i < nspl; i += SS_T) {
+        if constexpr (S32) P.spl[i] = (uint32_t)((spl_g[i] - smin) >> zs);
+        else P.spl[i] = spl_g[i];
+    }
     /* guide[p]: splitters below smin + (p << gs); the sample's range fits
      * SS_GUIDE steps of 2^gs */
     const int gs = nbs > 12 ? nbs - 12 : 0;
@@ -958,61 +976,82 @@ __global__ __launch_bounds__(SS_T, BM <= SS_BSMALL ? 2 : 1) void k_ss_part(const
             const uint64_t bound = smin + ((uint64_t)p << gs);
             while (lo < hi) {
                 const int mid = (lo + hi) >> 1;
-                if (P.spl[mid] < bound) lo = mid + 1;
+                uint64_t sv;
+                if constexpr (S32) sv = smin + ((uint64_t)P.spl[mid] << zs);
+                else sv = P.spl[mid];
+                if (sv < bound) lo = mid + 1;
                 else hi = mid;
             }
         }
         P.guide[p] = (uint16_t)lo;
     }
-    __syncthreads();
-    const size_t tile0 = (size_t)blockIdx.x * SS_TILE;
-    uint64_t k[SS_PK];
-    uint32_t br[SS_PK];   /* bucket << 16 | rank in the tile's bucket */
+    for (size_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        for (int i = threadIdx.x; i < B; i += SS_T) P.cnt[i] = 0;
+        __syncthreads();   /* the guide (first tile), the cleared counters */
+        const size_t tile0 = tile * SS_TILE;
+        uint64_t k[SS_PK];
+        uint32_t br[SS_PK];   /* bucket << 16 | rank in the tile's bucket */
 #pragma unroll
-    for (int j = 0; j < SS_PK; j++) {
-        const size_t i = tile0 + (size_t)j * SS_T + threadIdx.x;
-        k[j] = i < n ? keys[i] : 0;
-    }
+        for (int j = 0; j < SS_PK; j++) {
+            const size_t i = tile0 + (size_t)j * SS_T + threadIdx.x;
+            k[j] = i < n ? keys[i] : 0;
+        }
 #pragma unroll
-    for (int j = 0; j < SS_PK; j++) {
-        const size_t i = tile0 + (size_t)j * SS_T + threadIdx.x;
-        if (i < n) {
-            uint64_t pp = k[j] < smin ? 0 : (k[j] - smin) >> gs;
-            const int p = pp < (uint64_t)SS_GUIDE ? (int)pp : SS_GUIDE - 1;
-            int lo = P.guide[p], hi = P.guide[p + 1];
-            while (lo < hi) {
-                const int mid = (lo + hi) >> 1;
-                if (P.spl[mid] <= k[j]) lo = mid + 1;
-                else hi = mid;
+        for (int j = 0; j < SS_PK; j++) {
+            const size_t i = tile0 + (size_t)j * SS_T + threadIdx.x;
+            if (i < n) {
+                uint64_t pp = k[j] < smin ? 0 : (k[j] - smin) >> gs;
+                const int p = pp < (uint64_t)SS_GUIDE ? (int)pp : SS_GUIDE - 1;
+                int lo = P.guide[p], hi = P.guide[p + 1];
+                if constexpr (S32) {
+                    /* for a key >= smin, min + (e << z) <= key exactly when e <=
+                     * (key - min) >> z (capped at 2^32 - 1, which every e is
+                     * within); a key below smin is below every splitter */
+                    const uint64_t q = (k[j] - smin) >> zs;
+                    const uint32_t kv = q > 0xffffffffULL ? 0xffffffffu : (uint32_t)q;
+                    if (k[j] < smin) hi = lo;
+                    while (lo < hi) {
+                        const int mid = (lo + hi) >> 1;
+                        if (P.spl[mid] <= kv) lo = mid + 1;
+                        else hi = mid;
+                    }
+                } else {
+                    while (lo < hi) {
+                        const int mid = (lo + hi) >> 1;
+                        if (P.spl[mid] <= k[j]) lo = mid + 1;
+                        else hi = mid;
+                    }
+                }
+                br[j] = ((uint32_t)lo << 16) | atomicAdd(&P.cnt[lo], 1u);
             }
-            br[j] = ((uint32_t)lo << 16) | atomicAdd(&P.cnt[lo], 1u);
         }
-    }
-    __syncthreads();
-    /* BM / SS_T buckets per thread: the reservations */
-    constexpr int PER = BM / SS_T;
-    static_assert(BM % SS_T == 0 && BM <= 65536, "buckets per thread; bucket << 16 in br");
-    bool over = false;
+        __syncthreads();
+        /* BM / SS_T buckets per thread: the reservations */
+        constexpr int PER = BM / SS_T;
+        static_assert(BM % SS_T == 0 && BM <= 65536, "buckets per thread; bucket << 16 in br");
+        bool over = false;
 #pragma unroll
-    for (int q = 0; q < PER; q++) {
-        const int b = threadIdx.x + q * SS_T;
-        const uint32_t c = b < B ? P.cnt[b] : 0;
-        if (c) {
-            const unsigned long long g = atomicAdd(&gcur[b * SS_NX + grp], (unsigned long long)c);
-            const bool o = g + c > (unsigned long long)SS_SUBCAP;
-            over |= o;
-            P.cnt[b] = o ? ~0u : (uint32_t)((uint64_t)b * SS_BSTRIDE + (uint64_t)grp * SS_SUBCAP + g);
+        for (int q = 0; q < PER; q++) {
+            const int b = threadIdx.x + q * SS_T;
+            const uint32_t c = b < B ? P.cnt[b] : 0;
+            if (c) {
+                const unsigned long long g = atomicAdd(&gcur[b * SS_NX + grp], (unsigned long long)c);
+                const bool o = g + c > (unsigned long long)SS_SUBCAP;
+                over |= o;
+                P.cnt[b] = o ? ~0u : (uint32_t)((uint64_t)b * SS_BSTRIDE + (uint64_t)grp * SS_SUBCAP + g);
+            }
         }
-    }
-    if (over) atomicOr(err, SORT_ERR_OVERSIZE);
-    __syncthreads();
+        if (over) atomicOr(err, SORT_ERR_OVERSIZE);
+        __syncthreads();
 #pragma unroll
-    for (int j = 0; j < SS_PK; j++) {
-        const size_t i = tile0 + (size_t)j * SS_T + threadIdx.x;
-        if (i < n) {
-            const uint32_t base = P.cnt[br[j] >> 16];
-            if (base != ~0u) part[base + (br[j] & 0xffffu)] = k[j];
+        for (int j = 0; j < SS_PK; j++) {
+            const size_t i = tile0 + (size_t)j * SS_T + threadIdx.x;
+            if (i < n) {
+                const uint32_t base = P.cnt[br[j] >> 16];
+                if (base != ~0u) part[base + (br[j] & 0xffffu)] = k[j];
+            }
         }
+        __syncthreads();   /* every place read before the next tile's counters */
     }
 }
 
@@ -1770,7 +1809,7 @@ static OsScratch carve(void *scratch)
 static int g_ss_on = 1;              /* sample sort for sizes it handles */
 static int g_ss_bavg = 5000;         /* mean keys per bucket (move candidates) */
 static int g_ss_cap = SS_KEYCAP;     /* keys of a bucket (lowered only to test the repeat) */
-static int g_ss_bmax = SS_BMAX;      /* most buckets of a sort (SS_BSMALL or SS_BMAX) */
+static int g_ss_bmax = SS_BMAX;      /* most buckets of a sort (SS_BSMALL, SS_BMID or SS_BMAX) */
 static int g_ss_probes = SS_PROBES;  /* longest hash-set probe of a bucket the direct sort can take */
 /* buckets that fit one block: bitonic sort of every key first (1), or the
  * hash set first with the bitonic sort when the set fills (0, the default:
@@ -1783,6 +1822,12 @@ static int g_ss_bitonic = 0;
  * with 2048 and 9, 142.6-143.3 ms with LSD passes only, 142.6-145.0 ms
  * with 4096 and 17, 147.2-147.6 ms with 16384 (every bucket). */
 static int g_ss_dmax = 2048, g_ss_dnb = 9;
+/* partition blocks: 0, one per 8192-key tile; else at most this many,
+ * striding over the tiles (the splitters and guide loaded once a block).
+ * C2 partitions per table: 23.4 ms at 512 blocks, 27.4 at 256, 25.7 at 768,
+ * 25.8 one per tile; 64 batched small tables (16384 buckets): 7.4 ms at
+ * 256 to 768, 7.6 at 1024, 11.2 one per tile. */
+static int g_ss_part_blocks = 512;
 static unsigned long long g_ss_sorts = 0;
 
 void sort_set_sample(int on, int bucket_avg, int cap)
@@ -1793,10 +1838,12 @@ void sort_set_sample(int on, int bucket_avg, int cap)
 }
 void sort_set_sample_limits(int bmax, int probes)
 {
-    g_ss_bmax = bmax == SS_BSMALL ? SS_BSMALL : SS_BMAX;
+    g_ss_bmax = (bmax == SS_BSMALL || bmax == SS_BMID) ? bmax : SS_BMAX;
     g_ss_probes = probes > 0 ? probes : SS_PROBES;
 }
 void sort_set_bucket_bitonic(int on) { g_ss_bitonic = on ? 1 : 0; }
+void sort_set_part_blocks(int nb) { g_ss_part_blocks = nb > 0 ? nb : 0; }
+int sort_part_blocks() { return g_ss_part_blocks; }
 void sort_set_distinct_bitonic(int dmax, int dnb)
 {
     g_ss_dmax = dmax < 0 ? 0 : (dmax > SS_LCAP ? SS_LCAP : dmax);
@@ -1823,11 +1870,13 @@ static SsShape ss_shape(int mode)
     return SsShape{g_ss_bavg, g_ss_bavg > 7000 ? g_ss_bavg : 7000, g_ss_bmax};
 }
 
-/* The largest sort a shape takes: sh.bmax buckets of sh.bhi keys.  Larger
- * buckets were measured: batched candidate sets of 57M to 115M keys in
- * buckets of up to 14000 keys (their moves are already deduplicated per
- * round, so the buckets hold mostly distinct keys spanning 2^32 or more)
- * took the batched generation from 88 to 290 ms against the LSD path. */
+/* The largest sort a shape takes: sh.bmax buckets of sh.bhi keys (1.15e8
+ * keys at 16384 buckets of 7000).  More buckets, not larger ones: batched
+ * candidate sets of 57M to 115M keys (their moves already deduplicated per
+ * round, so the buckets hold mostly distinct keys spanning 2^32 or more) in
+ * 8192 buckets of up to 14000 keys took the batched generation from 88 to
+ * 290 ms against the LSD path; in 16384 buckets of 7000, from 75 to 59 ms
+ * (64 small tables) and 667 to 524 ms (the specified draw). */
 size_t sort_sample_limit(int mode)
 {
     if (!g_ss_on || mode <= 0) return 0;
@@ -1956,7 +2005,9 @@ static void ss_sort_unique(hipStream_t st, const uint64_t *keys, const unsigned 
          * the largest sorts.  The sample, its sorted runs and (for a larger
          * sample) the ranks go to the start of part, which the partition
          * overwrites only after the splitters are made. */
-        const int S = Bmax <= (size_t)SS_BSMALL ? SS_S : (Bmax <= 2 * (size_t)SS_BSMALL ? 2 * SS_S : SS_SMAX);
+        const int S = Bmax <= (size_t)SS_BSMALL ? SS_S
+                      : Bmax <= 2 * (size_t)SS_BSMALL ? 2 * SS_S
+                      : Bmax <= (size_t)SS_BMID ? 4 * SS_S : SS_SMAX;
         const int nrun = S / SS_RUN;
         uint32_t *runs = reinterpret_cast<uint32_t *>(part + S);
         hipLaunchKernelGGL(k_ss_gather, dim3(S / SS_T), dim3(SS_T), 0, st, keys, dn, nmax, sh, S, part,
@@ -1978,9 +2029,14 @@ static void ss_sort_unique(hipStream_t st, const uint64_t *keys, const unsigned 
         snote(hipGetLastError());
         prof_hook_end(h, 8.0 * S + 8.0 * (Bmax - 1), st);
         h = prof_hook_begin(9, st);
-        const size_t tiles = (nmax + SS_TILE - 1) / SS_TILE;
+        size_t tiles = (nmax + SS_TILE - 1) / SS_TILE;
+        if (g_ss_part_blocks > 0 && tiles > (size_t)g_ss_part_blocks) tiles = (size_t)g_ss_part_blocks;
         if (Bmax <= (size_t)SS_BSMALL)
             hipLaunchKernelGGL(k_ss_part<SS_BSMALL>, dim3((unsigned)tiles), dim3(SS_T), 0, st, keys, dn, nmax,
+                               sh, (const uint64_t *)s.ss_spl, (const unsigned long long *)s.ss_mnx, part,
+                               s.ss_cnt, d_err);
+        else if (Bmax <= (size_t)SS_BMID)
+            hipLaunchKernelGGL(k_ss_part<SS_BMID>, dim3((unsigned)tiles), dim3(SS_T), 0, st, keys, dn, nmax,
                                sh, (const uint64_t *)s.ss_spl, (const unsigned long long *)s.ss_mnx, part,
                                s.ss_cnt, d_err);
         else

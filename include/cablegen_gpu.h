@@ -219,7 +219,7 @@ int cg_dev_sync(void);
  *                canonical form through the formation's stabiliser (the
  *                images that put the formation at its canonical place; the
  *                full max of 8 for any other board); 0: always the max of 8;
- *   "sample_sort"  1: sorts of up to 8192 buckets of 7000 keys (5.7e7)
+ *   "sample_sort"  1: sorts of up to 16384 buckets of 7000 keys (1.15e8)
  *                partition into LDS-sized buckets by splitters from a
  *                regular sample and deduplicate and sort each bucket in LDS
  *                (a sort of at most 8192 keys is one block); 0: every sort
@@ -228,11 +228,15 @@ int cg_dev_sync(void);
  *   "sample_cap" keys a bucket may hold (0: 131072, its 8 regions); a bucket
  *                above it makes the library repeat that sort on the LSD path
  *                (lowered only to exercise the repeat);
- *   "sample_bmax"  most buckets of a sort: 8192 (sorts up to 5.7e7 keys),
- *                or 2048 (up to 1.4e7; larger sorts take the LSD path);
+ *   "sample_bmax"  most buckets of a sort: 16384 (sorts up to 1.15e8 keys),
+ *                8192 (up to 5.7e7) or 2048 (up to 1.4e7); larger sorts
+ *                take the LSD path;
  *   "sample_probes"  longest hash-set probe of a bucket that one block can
  *                also sort whole (0: 64); past it the bucket is sorted by a
  *                bitonic network, equal runs dropped;
+ *   "part_blocks"  sample sort partition blocks: at most this many, each
+ *                striding over the 8192-key tiles (default 512); 0: one
+ *                block per tile;
  *   "bucket_bitonic"  1: buckets that fit one block are sorted whole by the
  *                bitonic network first; 0: through the LDS hash set first;
  *   "distinct_bitonic"  a bucket's distinct values are sorted by a bitonic
@@ -251,8 +255,8 @@ int cg_dev_sync(void);
  *   "builtin"    built-in defaults: 8-bit digits, 8192-key tiles, window 16
  *                for every entry, no forced field, delta 0, spawn_estimate 0,
  *                cell_code 1, carry_stream 1, canon_stab 1, sample_sort 1
- *                with 5000 keys per bucket, capacity 131072, 8192 buckets,
- *                probes 64, bucket_bitonic 0, distinct_bitonic 2048 with 9
+ *                with 5000 keys per bucket, capacity 131072, 16384 buckets,
+ *                probes 64, part_blocks 512, bucket_bitonic 0, distinct_bitonic 2048 with 9
  *                bits, prune_in_sort 1, dir_deferred 1, carry_sort 0, spawn_blocks
  *                1024, solve_waves 5, one_sync 1,
  *                batch_graph 1;

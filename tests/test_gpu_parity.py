@@ -1056,9 +1056,9 @@ def test_sample_sort_sizes(cg, n):
 @pytest.mark.parametrize("n", [12000007, 20000003, 30000001])
 def test_sample_sort_large(cg, n):
     """Sorts above 2048 buckets of 5000 keys: 2048 buckets up to 14.3M keys,
-    then buckets of 7000 keys (up to 8192): splitters from 16384, 32768 or
-    65536 samples (the larger samples ranked in groups of runs), the
-    partition with up to 8192 splitters in LDS; against numpy's unique on
+    then buckets of 7000 keys (up to 8192 here): splitters from 16384, 32768
+    or 65536 samples (the larger samples ranked in groups of runs), the
+    partition with up to 8192 64-bit splitters in LDS; against numpy's unique on
     DPDGap boards with repeats.  With at most 2048 buckets (knob
     sample_bmax) the largest take the LSD path."""
     rng = np.random.default_rng(n + 1)
@@ -1070,7 +1070,59 @@ def test_sample_sort_large(cg, n):
         _sample_sort_check(cg, a, expect_sample=n <= 14336000)
     finally:
         cg.tune_set("reset", 0)
-    assert cg.tune_get("sample_bmax") == 8192
+    assert cg.tune_get("sample_bmax") == 16384
+
+
+@pytest.mark.parametrize("n", [60000011, 100000007])
+def test_sample_sort_16384_buckets(cg, n):
+    """Sorts above 8192 buckets of 7000 keys (the default sample_bmax
+    16384): 131072 samples ranked in groups of runs, the partition holding
+    its 16383 splitters in LDS as 32-bit sample values (exact: a splitter is
+    min + (value << z)); against numpy's unique on DPDGap boards with repeats
+    and, for the smaller size, on distinct full 64-bit keys (buckets spanning
+    2^32 or more).  With sample_bmax 8192 they take the LSD path."""
+    rng = np.random.default_rng(n + 2)
+    b = rand_boards(rng, 0x00000000FFF0FFF0, n // 3)
+    a = b[rng.integers(0, len(b), size=n)]
+    del b
+    assert cg.tune_get("sample_bmax") == 16384
+    rep = cg.tune_get("sample_repeats")
+    _sample_sort_check(cg, a)
+    print("n=%d boards: LSD repeats %d" % (n, cg.tune_get("sample_repeats") - rep))
+    if n < 80000000:
+        a = rng.integers(0, 2**64 - 1, size=n, dtype=np.uint64, endpoint=True)
+        a[:3] = (0, 2**64 - 1, 1)
+        rep = cg.tune_get("sample_repeats")
+        _sample_sort_check(cg, a)
+        print("n=%d 64-bit keys: LSD repeats %d" % (n, cg.tune_get("sample_repeats") - rep))
+    try:
+        cg.tune_set("sample_bmax", 8192)
+        before = cg.tune_get("sample_sorts")
+        _sort_check(cg, a[: 60000000])
+        assert cg.tune_get("sample_sorts") == before
+    finally:
+        cg.tune_set("reset", 0)
+    assert cg.tune_get("sample_bmax") == 16384
+
+
+@pytest.mark.parametrize("blocks", [0, 1, 7, 8])
+def test_sample_sort_part_blocks(cg, blocks):
+    """The partition's grid (knob part_blocks): one block per 8192-key tile
+    (0), or fewer blocks striding over the tiles (one block for every tile,
+    and counts that are and are not a multiple of the 8 region groups), at
+    2048, 8192 and 16384 buckets, against numpy's unique."""
+    rng = np.random.default_rng(blocks + 77)
+    try:
+        cg.tune_set("part_blocks", blocks)
+        assert cg.tune_get("part_blocks") == blocks
+        for n in (3000017, 20000003, 60000011):
+            if blocks == 1 and n > 20000003:
+                continue
+            b = rand_boards(rng, 0x00000000FFF0FFF0, n // 3)
+            _sample_sort_check(cg, b[rng.integers(0, len(b), size=n)])
+    finally:
+        cg.tune_set("reset", 0)
+    assert cg.tune_get("part_blocks") == 512
 
 
 def test_sample_sort_key_shapes(cg):
