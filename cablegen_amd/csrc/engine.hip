@@ -1744,6 +1744,11 @@ extern "C" int cg_tune_set(const char *knob, int value)
         sort_set_sample_limits(value, sort_sample_probes());
         return 0;
     }
+    if (!strcmp(knob, "part_prefetch")) {
+        if (value != 0 && value != 1) return fail_msg("part_prefetch: 0 or 1");
+        sort_set_part_prefetch(value);
+        return 0;
+    }
     if (!strcmp(knob, "part_blocks")) {
         if (value < 0 || value > (1 << 20)) return fail_msg("part_blocks: 0..2^20");
         sort_set_part_blocks(value);
@@ -1833,6 +1838,7 @@ extern "C" int cg_tune_set(const char *knob, int value)
         sort_set_sample_limits(16384, 0);
         sort_set_bucket_bitonic(0);
         sort_set_part_blocks(512);
+        sort_set_part_prefetch(1);
         sort_set_distinct_bitonic(2048, 9);
         if (!strcmp(knob, "reset") && !g_tune_path.empty()) return cg_tune_load(g_tune_path.c_str());
         return 0;
@@ -1861,6 +1867,7 @@ extern "C" int cg_tune_get(const char *knob)
     if (!strcmp(knob, "sample_bmax")) return sort_sample_bmax();
     if (!strcmp(knob, "bucket_bitonic")) return sort_bucket_bitonic();
     if (!strcmp(knob, "part_blocks")) return sort_part_blocks();
+    if (!strcmp(knob, "part_prefetch")) return sort_part_prefetch();
     if (!strcmp(knob, "distinct_bitonic")) return sort_distinct_bitonic_max();
     if (!strcmp(knob, "distinct_bitonic_bits")) return sort_distinct_bitonic_bits();
     if (!strcmp(knob, "sample_probes")) return sort_sample_probes();

@@ -244,6 +244,8 @@ void sort_set_bucket_bitonic(int on);
 int sort_bucket_bitonic();
 void sort_set_part_blocks(int nb);
 int sort_part_blocks();
+void sort_set_part_prefetch(int on);
+int sort_part_prefetch();
 /* a bucket's distinct values sorted by a bitonic network instead of 8-bit
  * LSD passes when there are at most dmax of them (0: never) and they span
  * at least dnb bits */

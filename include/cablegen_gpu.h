@@ -237,6 +237,9 @@ int cg_dev_sync(void);
  *   "part_blocks"  sample sort partition blocks: at most this many, each
  *                striding over the 8192-key tiles (default 512); 0: one
  *                block per tile;
+ *   "part_prefetch"  1: a partition block loads its next tile's keys while
+ *                the current tile reserves its places (default); 0: at the
+ *                start of each tile;
  *   "bucket_bitonic"  1: buckets that fit one block are sorted whole by the
  *                bitonic network first; 0: through the LDS hash set first;
  *   "distinct_bitonic"  a bucket's distinct values are sorted by a bitonic
@@ -256,7 +259,7 @@ int cg_dev_sync(void);
  *                for every entry, no forced field, delta 0, spawn_estimate 0,
  *                cell_code 1, carry_stream 1, canon_stab 1, sample_sort 1
  *                with 5000 keys per bucket, capacity 131072, 16384 buckets,
- *                probes 64, part_blocks 512, bucket_bitonic 0, distinct_bitonic 2048 with 9
+ *                probes 64, part_blocks 512, part_prefetch 1, bucket_bitonic 0, distinct_bitonic 2048 with 9
  *                bits, prune_in_sort 1, dir_deferred 1, carry_sort 0, spawn_blocks
  *                1024, solve_waves 5, one_sync 1,
  *                batch_graph 1;

@@ -1105,16 +1105,19 @@ def test_sample_sort_16384_buckets(cg, n):
     assert cg.tune_get("sample_bmax") == 16384
 
 
-@pytest.mark.parametrize("blocks", [0, 1, 7, 8])
-def test_sample_sort_part_blocks(cg, blocks):
+@pytest.mark.parametrize("blocks,prefetch", [(0, 1), (1, 1), (7, 1), (8, 1), (512, 0), (7, 0)])
+def test_sample_sort_part_blocks(cg, blocks, prefetch):
     """The partition's grid (knob part_blocks): one block per 8192-key tile
     (0), or fewer blocks striding over the tiles (one block for every tile,
-    and counts that are and are not a multiple of the 8 region groups), at
-    2048, 8192 and 16384 buckets, against numpy's unique."""
-    rng = np.random.default_rng(blocks + 77)
+    and counts that are and are not a multiple of the 8 region groups), with
+    the next tile's keys loaded ahead (knob part_prefetch, the default) or
+    not, at 2048, 8192 and 16384 buckets, against numpy's unique."""
+    rng = np.random.default_rng(blocks + 77 + prefetch)
     try:
         cg.tune_set("part_blocks", blocks)
+        cg.tune_set("part_prefetch", prefetch)
         assert cg.tune_get("part_blocks") == blocks
+        assert cg.tune_get("part_prefetch") == prefetch
         for n in (3000017, 20000003, 60000011):
             if blocks == 1 and n > 20000003:
                 continue
@@ -1123,6 +1126,7 @@ def test_sample_sort_part_blocks(cg, blocks):
     finally:
         cg.tune_set("reset", 0)
     assert cg.tune_get("part_blocks") == 512
+    assert cg.tune_get("part_prefetch") == 1
 
 
 def test_sample_sort_key_shapes(cg):
