@@ -309,6 +309,7 @@ static void prof_set_bytes(int h, double bytes)
 
 /* per-launch lookup counters of the solve kernel (pairs of u64) */
 static unsigned long long *g_stats = nullptr;
+static int g_prof_count = 1;   /* timed solves count their lookups */
 static int g_stats_cap = 0, g_stats_used = 0;
 
 static void prof_collect()
@@ -1286,9 +1287,10 @@ static int solve_layer_dev(const cg_rules &r, const uint64_t *M, size_t n, doubl
     if (!lut) return fail_msg("row table upload failed");
     unsigned long long *stats = nullptr;
     int slot = -1;
-    /* lookup counters (the algorithmic bytes) only when the solve is timed:
-     * they cost atomics at the end of every block */
-    if (prof_wants(PK_SOLVE)) {
+    /* lookup counters (the algorithmic bytes) only when the solve is timed
+     * and counting is on: they cost atomics at the end of every block and the
+     * registers of the kernel instance that keeps them */
+    if (prof_wants(PK_SOLVE) && g_prof_count) {
         if (g_stats_used == g_stats_cap) {
             /* grow: collect pending records first so the old array can go */
             prof_collect();
@@ -1744,6 +1746,11 @@ extern "C" int cg_tune_set(const char *knob, int value)
         sort_set_sample_limits(value, sort_sample_probes());
         return 0;
     }
+    if (!strcmp(knob, "prof_count")) {
+        if (value != 0 && value != 1) return fail_msg("prof_count: 0 or 1");
+        g_prof_count = value;
+        return 0;
+    }
     if (!strcmp(knob, "part_prefetch")) {
         if (value != 0 && value != 1) return fail_msg("part_prefetch: 0 or 1");
         sort_set_part_prefetch(value);
@@ -1832,13 +1839,14 @@ extern "C" int cg_tune_set(const char *knob, int value)
         g_dir_deferred = 1;
         g_carry_sort = 0;
         spawn_set_blocks(1024);
-        solve_set_waves(5);
+        solve_set_waves(6);
         g_batch_graph = 1;
         sort_set_sample(1, 5000, 0);
         sort_set_sample_limits(16384, 0);
         sort_set_bucket_bitonic(0);
         sort_set_part_blocks(512);
         sort_set_part_prefetch(1);
+        g_prof_count = 1;
         sort_set_distinct_bitonic(2048, 9);
         if (!strcmp(knob, "reset") && !g_tune_path.empty()) return cg_tune_load(g_tune_path.c_str());
         return 0;
@@ -1868,6 +1876,7 @@ extern "C" int cg_tune_get(const char *knob)
     if (!strcmp(knob, "bucket_bitonic")) return sort_bucket_bitonic();
     if (!strcmp(knob, "part_blocks")) return sort_part_blocks();
     if (!strcmp(knob, "part_prefetch")) return sort_part_prefetch();
+    if (!strcmp(knob, "prof_count")) return g_prof_count;
     if (!strcmp(knob, "distinct_bitonic")) return sort_distinct_bitonic_max();
     if (!strcmp(knob, "distinct_bitonic_bits")) return sort_distinct_bitonic_bits();
     if (!strcmp(knob, "sample_probes")) return sort_sample_probes();

@@ -867,7 +867,9 @@ __device__ __forceinline__ void best_successors(uint64_t c2, uint64_t c4, const 
     const uint64_t y2 = cg_rot_cw(c2), y4 = cg_rot_cw(c4);
     uint64_t q[8];
     uint32_t lo[8], hi[8];
-    bool live[8], f[8];
+    /* per-slot flags as bits of one word per lane (bit s: slot s) rather
+     * than eight lane masks each */
+    uint32_t live = 0, f = 0;
 #pragma unroll
     for (int s = 0; s < 8; s++) {
         const int d = s & 3;
@@ -876,9 +878,8 @@ __device__ __forceinline__ void best_successors(uint64_t c2, uint64_t c4, const 
         const bool ok = cg_move_hori_tab(lut, &m, d == 1 || d == 2, ff);
         nlook += ok;
         q[s] = cg_canon(m);
-        live[s] = ok && t.n != 0 && (q[s] & t.cmask) == t.cval;
+        live |= (uint32_t)(ok && t.n != 0 && (q[s] & t.cmask) == t.cval) << s;
         lo[s] = hi[s] = 0;
-        f[s] = false;
     }
     /* the loads are unconditional, from a zero word where a slot has nothing
      * to read, so all eight are issued before the first wait */
@@ -886,7 +887,7 @@ __device__ __forceinline__ void best_successors(uint64_t c2, uint64_t c4, const 
     for (int s = 0; s < 8; s++) {
         const LayerView &t = s < 4 ? t2 : t4;
         const uint32_t *p = reinterpret_cast<const uint32_t *>(g_zero2);
-        if (live[s]) p = t.dir + (xform(q[s], t.x) >> t.shift);
+        if ((live >> s) & 1u) p = t.dir + (xform(q[s], t.x) >> t.shift);
         lo[s] = p[0];
         hi[s] = p[1];
     }
@@ -906,7 +907,7 @@ __device__ __forceinline__ void best_successors(uint64_t c2, uint64_t c4, const 
                     lo[s] = mid + 1;
                 } else {
                     hi[s] = mid;
-                    f[s] = k[s] == q[s];
+                    f = (f & ~(1u << s)) | ((uint32_t)(k[s] == q[s]) << s);
                 }
                 any |= lo[s] < hi[s];
             }
@@ -917,8 +918,9 @@ __device__ __forceinline__ void best_successors(uint64_t c2, uint64_t c4, const 
 #pragma unroll
     for (int s = 0; s < 8; s++) {
         const LayerView &t = s < 4 ? t2 : t4;
-        v[s] = *(f[s] ? t.vals + lo[s] : reinterpret_cast<const double *>(g_zero2));
-        nhit += f[s];
+        const uint32_t fs = (f >> s) & 1u;
+        v[s] = *(fs ? t.vals + lo[s] : reinterpret_cast<const double *>(g_zero2));
+        nhit += fs;
     }
     double b2 = 0.0, b4 = 0.0;
     int d2 = 0, d4 = 0;
@@ -1040,8 +1042,9 @@ __device__ __forceinline__ void add_stats(unsigned long long *stats, uint32_t nl
 }
 
 /* MW: the waves per SIMD the registers are fitted to (0: the compiler's
- * choice, 4 waves at 111 VGPRs) */
-template <int MW>
+ * choice, 4 waves); ST: the lookup counts are summed into stats (the
+ * counters are dead code, and free their registers, without it) */
+template <int MW, bool ST>
 __global__ __launch_bounds__(SV_THREADS, MW > 0 ? MW : 1) void k_solve(const uint64_t *__restrict__ M, size_t n,
                                                       double *__restrict__ V,
                                                       uint64_t *__restrict__ argmax, LayerView t2,
@@ -1056,12 +1059,14 @@ __global__ __launch_bounds__(SV_THREADS, MW > 0 ? MW : 1) void k_solve(const uin
      * flight at any time come from one narrow window of boards */
     for (size_t tile = blockIdx.x; tile * SV_THREADS < n; tile += gridDim.x)
         solve_tile(M, n, tile, V, argmax, t2, t4, win, p4, lut, ff, nlook, nhit, sm);
-    add_stats(stats, nlook, nhit, red);
+    if constexpr (ST) add_stats(stats, nlook, nhit, red);
 }
 
-/* C2 solve 75.7 ms at the compiler's 111 VGPRs (4 waves per SIMD), 73.6 ms
- * fitted to 5 waves (96 VGPRs), 83.5 ms at 6 (80 VGPRs, 40 spilled) */
-static int g_solve_waves = 5;
+/* C2 solve with the per-slot flags as bit masks and without the lookup
+ * counters: 71.4 ms fitted to 6 waves per SIMD (80 VGPRs, 10 spilled), 73.6
+ * at 5 (96 VGPRs, none spilled), 72.7 at the compiler's 4 waves; with the
+ * flags as lane masks and the counters, 75.7 / 73.6 / 83.5 ms at 4 / 5 / 6 */
+static int g_solve_waves = 6;
 
 void launch_solve(hipStream_t st, const uint64_t *M, size_t n, double *V, uint64_t *argmax,
                   LayerView t2, LayerView t4, WinRules w, double p4, const uint16_t *lut, int ff,
@@ -1072,15 +1077,19 @@ void launch_solve(hipStream_t st, const uint64_t *M, size_t n, double *V, uint64
      * with 2048, 79.3 with 4096, 75.9 with 8192 and 77.3 with 16384 or more */
     size_t tiles = (n + SV_THREADS - 1) / SV_THREADS;
     size_t blocks = tiles < 8192 ? tiles : 8192;
-    if (g_solve_waves == 5)
-        hipLaunchKernelGGL(k_solve<5>, dim3((unsigned)blocks), dim3(SV_THREADS), 0, st, M, n, V, argmax, t2,
-                           t4, w, p4, lut, ff, stats);
-    else if (g_solve_waves == 6)
-        hipLaunchKernelGGL(k_solve<6>, dim3((unsigned)blocks), dim3(SV_THREADS), 0, st, M, n, V, argmax, t2,
-                           t4, w, p4, lut, ff, stats);
-    else
-        hipLaunchKernelGGL(k_solve<0>, dim3((unsigned)blocks), dim3(SV_THREADS), 0, st, M, n, V, argmax, t2,
-                           t4, w, p4, lut, ff, stats);
+#define CG_SOLVE(MW)                                                                                      \
+    do {                                                                                                  \
+        if (stats)                                                                                        \
+            hipLaunchKernelGGL((k_solve<MW, true>), dim3((unsigned)blocks), dim3(SV_THREADS), 0, st, M, n, V, \
+                               argmax, t2, t4, w, p4, lut, ff, stats);                                    \
+        else                                                                                              \
+            hipLaunchKernelGGL((k_solve<MW, false>), dim3((unsigned)blocks), dim3(SV_THREADS), 0, st, M, n,   \
+                               V, argmax, t2, t4, w, p4, lut, ff, stats);                                 \
+    } while (0)
+    if (g_solve_waves == 5) CG_SOLVE(5);
+    else if (g_solve_waves == 6) CG_SOLVE(6);
+    else CG_SOLVE(0);
+#undef CG_SOLVE
 }
 
 void solve_set_waves(int w) { g_solve_waves = (w == 5 || w == 6) ? w : 0; }

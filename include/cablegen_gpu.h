@@ -205,8 +205,12 @@ int cg_dev_sync(void);
  *   "carry_sort" 0: a layer step's spawn-4 boards are moved as written
  *                (duplicates included) two steps later; 1: they are sorted
  *                and deduplicated first (the carry sort);
- *   "solve_waves"  the solve kernel's registers fitted to 5 (default) or
- *                6 waves per SIMD; 0: the compiler's choice (4 waves);
+ *   "solve_waves"  the solve kernel's registers fitted to 6 (default) or
+ *                5 waves per SIMD; 0: the compiler's choice (4 waves);
+ *   "prof_count"  1 (default): a solve timed by the profiler also counts its
+ *                lookups (the algorithmic bytes) in a kernel instance that
+ *                keeps the counters; 0: timed without counting (the
+ *                production kernel);
  *   "spawn_blocks"  blocks of a spawn launch whose 4-spawns are moved
  *                unsorted (64..65536, default 1024);
  *   "carry_stream"  with carry_sort 1 (and in batches): 1: the carry sort
@@ -261,7 +265,7 @@ int cg_dev_sync(void);
  *                with 5000 keys per bucket, capacity 131072, 16384 buckets,
  *                probes 64, part_blocks 512, part_prefetch 1, bucket_bitonic 0, distinct_bitonic 2048 with 9
  *                bits, prune_in_sort 1, dir_deferred 1, carry_sort 0, spawn_blocks
- *                1024, solve_waves 5, one_sync 1,
+ *                1024, solve_waves 6, prof_count 1, one_sync 1,
  *                batch_graph 1;
  *   "reset"      the state after initialisation: the built-in defaults, then
  *                the table loaded at initialisation if there was one.

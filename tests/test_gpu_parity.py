@@ -561,18 +561,40 @@ def test_tablebase_carry_sorted(cg, name, repeat):
     assert cg.tune_get("carry_sort") == 0
 
 
-@pytest.mark.parametrize("waves", [0, 6])
+@pytest.mark.parametrize("waves", [0, 5])
 def test_tablebase_solve_waves(cg, waves):
-    """The solve kernel with its registers fitted to 6 waves per SIMD (some
-    spilled) or left to the compiler (4 waves), against the oracle; every
-    other test runs the default, 5 waves."""
+    """The solve kernel with its registers fitted to 5 waves per SIMD or left
+    to the compiler (4 waves), against the oracle; every other test runs the
+    default, 6 waves."""
     try:
         cg.tune_set("solve_waves", waves)
         test_tablebase_vs_oracle(cg, "dpdgap_prune")
         test_tablebase_vs_oracle(cg, "score")
     finally:
         cg.tune_set("reset", 0)
-    assert cg.tune_get("solve_waves") == 5
+    assert cg.tune_get("solve_waves") == 6
+
+
+@pytest.mark.parametrize("count", [1, 0])
+def test_tablebase_solve_profiled(cg, count):
+    """A solve timed by the profiler: with prof_count 1 the kernel instance
+    that counts its lookups (the only one the default runs leave out), with 0
+    the production instance under the timers; values against the oracle."""
+    try:
+        cg.tune_set("prof_count", count)
+        cg.prof_enable(True)
+        cg.prof_select("solve")
+        cg.prof_reset()
+        test_tablebase_vs_oracle(cg, "dpdgap_prune")
+        n, ms, by = cg.prof_query("solve")
+        assert n > 0 and ms > 0
+        if count:
+            assert by > 0
+    finally:
+        cg.prof_enable(False)
+        cg.prof_select(None)
+        cg.tune_set("reset", 0)
+    assert cg.tune_get("prof_count") == 1
 
 
 @pytest.mark.parametrize("name", ["c1", "dpdgap_prune"])
